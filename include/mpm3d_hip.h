@@ -1,0 +1,103 @@
+/*
+ * mpm3d_hip.h -- C-ABI of libmpm3d_hip.so, the MI355X (gfx950) engine for
+ * the MPM3D explicit updated-Lagrangian time step.
+ *
+ * The reference calls its step as argument-less external subroutines over
+ * module globals (update_step.f90, called from MPM3D.f90:69-113).  Each of
+ * those routines has one entry point here taking an explicit context; the
+ * context owns device-resident structure-of-arrays particle and grid state
+ * for the whole run.  A Fortran host binds these through ISO_C_BINDING
+ * (mpm3d-f90_amd/fortran/mpm_hip.f90).
+ *
+ * Conventions: every function returns 0 (MPM_OK) or a negative MPM_ERR_*
+ * code; mpm_last_error() returns a message for the calling thread's last
+ * failure.  No C++ exceptions cross this boundary.  All calls on one context
+ * are ordered on that context's HIP stream; functions that return host data
+ * synchronise that stream.  A context may be driven by one host thread at a
+ * time; distinct contexts are independent.
+ */
+#ifndef MPM3D_HIP_H
+#define MPM3D_HIP_H
+
+#include <stdint.h>
+#include "mpm3d_problem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_OK 0
+#define MPM_ERR_ARG (-1)          /* bad argument */
+#define MPM_ERR_IO (-2)           /* file could not be opened */
+#define MPM_ERR_DECK (-3)         /* deck syntax / semantics */
+#define MPM_ERR_UNSUPPORTED (-4)  /* feature outside the explicit UL step */
+#define MPM_ERR_HIP (-5)          /* HIP runtime error */
+#define MPM_ERR_NOMEM (-6)        /* device memory / HBM budget exhausted */
+#define MPM_ERR_STATE (-7)        /* call out of order / context unusable */
+#define MPM_ERR_NUMERIC (-8)      /* device flagged NaN / negative volume */
+
+/* arithmetic of the bulk particle/grid state */
+#define MPM_PREC_FP64 0           /* plain FP64 everywhere */
+#define MPM_PREC_FP32 1           /* FP32 bulk, FP64 positions and small-mass division */
+
+typedef struct mpm_ctx mpm_ctx;
+
+const char* mpm_last_error(void);
+const char* mpm_version(void);
+
+/* ---- input deck (replaces InputPara, DataIn.f90:34-103) ---------------- */
+/* Reads a .mpm deck; *out is allocated by the library (mpm_problem_free). */
+int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, int errlen);
+void mpm_problem_free(mpm_problem* p);
+
+/* ---- context -------------------------------------------------------------
+ * Uploads the problem (particles as SoA, grid description, materials,
+ * flags) to the current HIP device.  The device keeps DT, time and step
+ * count; the host copy of the problem may be freed afterwards. */
+int mpm_ctx_create(const mpm_problem* prob, int precision, mpm_ctx** out);
+int mpm_ctx_destroy(mpm_ctx* ctx);
+
+/* ---- one entry point per reference phase ------------------------------- */
+/* istep += 1; CurrentTime += DT (MPM3D.f90:72-74) */
+int mpm_begin_step(mpm_ctx* ctx);
+/* GridMomentumInitial + ParticleInitial (update_step.f90:288-453) */
+int mpm_grid_momentum_initial(mpm_ctx* ctx);
+/* ApplyBoundaryConditions (update_step.f90:1795-1830) */
+int mpm_apply_boundary_conditions(mpm_ctx* ctx);
+/* ParticleStressUpdate (update_step.f90:456-647, Constitution.f90:79-368) */
+int mpm_particle_stress_update(mpm_ctx* ctx);
+/* GridMomentumUpdate (update_step.f90:831-1013) */
+int mpm_grid_momentum_update(mpm_ctx* ctx);
+/* IntegrateMomentum (update_step.f90:1194-1239) */
+int mpm_integrate_momentum(mpm_ctx* ctx);
+/* ParticlePositionUpdate (update_step.f90:1378-1526) */
+int mpm_particle_position_update(mpm_ctx* ctx);
+/* GridMomentumMUSL (update_step.f90:1668-1793) */
+int mpm_grid_momentum_musl(mpm_ctx* ctx);
+/* calcEnergy (Particle.f90:204-230); any output pointer may be NULL */
+int mpm_calc_energy(mpm_ctx* ctx, double* eng_kinetic, double* eng_internal, double* momentum3);
+/* UpdateDT (update_step.f90:1833-1874); *dt may be NULL */
+int mpm_update_dt(mpm_ctx* ctx, double* dt);
+
+/* nsteps whole steps in the reference order (MPM3D.f90:69-113), fused
+ * kernels, no host synchronisation inside */
+int mpm_step(mpm_ctx* ctx, int nsteps);
+
+/* ---- state exchange (synchronising) ---------------------------------- */
+/* pstate: PF_COUNT * nb_particle doubles, original particle order */
+int mpm_download_particles(mpm_ctx* ctx, double* pstate);
+/* gstate: GF_COUNT * nb_node doubles, node order of the reference */
+int mpm_download_grid(mpm_ctx* ctx, double* gstate);
+/* icell: nb_particle, 1-based cell index of the last GridMomentumInitial, -1 outside */
+int mpm_download_icell(mpm_ctx* ctx, int32_t* icell);
+/* overwrite particle state (e.g. a host-side restart); same layout as download */
+int mpm_upload_particles(mpm_ctx* ctx, const double* pstate);
+/* out[0..7] = istep, CurrentTime, DT, EngKinetic, EngInternal, Momentum(3) */
+int mpm_get_scalars(mpm_ctx* ctx, double* out8);
+int mpm_nb_node(const mpm_ctx* ctx);
+int mpm_sync(mpm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPM3D_HIP_H */

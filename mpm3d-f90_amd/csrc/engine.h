@@ -1,0 +1,102 @@
+// engine.h -- device data layout of the MI355X engine (shared by the kernels
+// and the host-side engine).
+//
+// Particles of all instances live in one structure-of-arrays pool.  Each
+// instance owns a contiguous particle range padded to PGRAN slots and a
+// contiguous node range padded to NGRAN nodes, so a thread block never
+// straddles two instances: kernels read their instance from a per-block
+// table with one scalar load.
+#pragma once
+#include <cstdint>
+
+#define PGRAN 256
+#define NGRAN 256
+
+// per-instance constants (device memory, read through the scalar cache)
+struct InstC {
+  int ncx, ncy, ncz, ncxy, ncell;
+  int ngx, ngy, ngz, ngxy, nnode;
+  int node_off;          // first node of the instance in the node pool
+  int p_begin, p_end;    // particle range [p_begin, p_end) in the pool
+  int shape, scheme;
+  int flags;             // IF_* below
+  int mat_off, nmat;     // material range in the material table
+  int cell_off;          // sort-key offset of this instance's cells
+  double span0[3];       // SpanX(1), SpanY(1), SpanZ(1)
+  double span1[3];       // adjusted upper ends
+  double dcell, idcell, ijac, ijac4, cutoff;
+  double qlt, damp, bq1, bq2, dtscale, endtime;
+};
+
+enum InstFlags {
+  IF_QUASI_LOAD = 1,
+  IF_QUASI_DAMP = 2,
+  IF_GRAVITY = 4,
+  IF_UVFP = 8,
+  IF_LOADS = 16,         // some particle carries a prescribed load FXp
+  IF_ENDTIME = 32        // instance stops itself when time > endtime
+};
+
+// per-instance time state, advanced on the device (graph-capturable)
+struct InstT {
+  double time, dt;
+  double ek, ei, mom[3], maxv;
+  double ek_prev;
+  int istep;
+  int done;
+  int nan_count;         // particles whose stress update produced NaN
+  int negvol_count;      // particles with negative volume (warning in the reference)
+};
+
+// per-material constants, all derived exactly as the reference derives them
+// per call (Constitution.f90:156-175) so the values are bit-identical
+struct MatC {
+  int type, eos;
+  double density, young_eff, poisson, G2, K3, lamda, PlaMod, miu2, yield0;
+  double Gmod, Kmod, q_fai, k_fai, q_psi, tenf;   // Drucker-Prager (tenf = min(ten_f, k/q))
+  double c0;             // EOS 4 sound speed
+  double csound;         // SetDT/UpdateDT sound speed max(c, D, Wavespd)
+};
+
+struct BodyC {
+  int mat;               // global material index
+  int pad;
+  double grav[3];
+};
+
+// particle fields kept in the bulk type T
+enum TField {
+  T_VX = 0, T_VY, T_VZ,
+  T_BX, T_BY, T_BZ,      // G2P velocity (XX = Xp + b*DT applied by the stress pass)
+  T_FX, T_FY, T_FZ,      // prescribed load FXp
+  T_MASS, T_VOL, T_SM, T_SEQV,
+  T_SD0, T_SD1, T_SD2, T_SD3, T_SD4, T_SD5,
+  T_SIGY, T_EPEFF, T_IE, T_Q, T_CP,
+  T_COUNT
+};
+
+// particle fields always kept in FP64
+enum DField { D_X = 0, D_Y, D_Z, D_F0, D_F1, D_F2, D_F3, D_F4, D_F5, D_F6, D_F7, D_F8, D_COUNT };
+
+// particle integer fields
+enum IField { I_BODY = 0, I_ORIG, I_CELL, I_COUNT };
+
+template <typename T>
+struct PoolPtrs {
+  T* t;          // T_COUNT * np
+  double* d;     // D_COUNT * np
+  int* i;        // I_COUNT * np
+  long np;       // padded particle capacity (stride of every field)
+  // grid (node pool, stride nn)
+  double* mg;
+  T* pg;         // 3 * nn  momentum PXg
+  T* fg;         // 3 * nn  force FXg
+  T* p2;         // 3 * nn  MUSL re-mapped momentum
+  uint8_t* fix;  // nn, bit0 x, bit1 y, bit2 z
+  long nn;
+  const InstC* inst;
+  InstT* ts;
+  const MatC* mats;
+  const BodyC* bodies;
+  double* part;  // energy partials: 6 per particle block
+};

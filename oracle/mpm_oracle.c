@@ -1,0 +1,988 @@
+/*
+ * mpm_oracle.c -- CPU FP64 restatement of the MPM3D explicit step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).  Build:
+ *   gcc -O2 -ffp-contract=off -fPIC -shared mpm_oracle.c -o _build/libmpm_oracle.so -lm
+ * (oracle/Makefile).  -ffp-contract=off matches the reference build flags: no
+ * FMA contraction, so every + - * / sqrt below rounds exactly as the
+ * Fortran statements it restates.
+ *
+ * Each routine cites the reference statement block it follows.  Array
+ * indices of grid nodes are kept 1-based (as the reference's InflNode) so the
+ * reference's in-range tests and index arithmetic (including its wrap of
+ * linear node indices at low domain faces) carry over unchanged.
+ */
+#include "mpm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+struct oracle {
+  mpm_problem prob;     /* scalars; mats/bodies point into own copies */
+  mpm_material* mats;
+  mpm_body* bodies;
+  int n;                /* particles */
+  double* ps;           /* PF_COUNT * n */
+  int32_t* pbody;
+  int32_t* icell;       /* per particle, 1-based cell or -1 */
+  double* xn;           /* 3 * n natural coords (linear shape) */
+  /* grid */
+  int ncx, ncy, ncz, ncxy, ncell;
+  int ngx, ngy, ngz, ngxy, nnode;
+  double* g;            /* GF_COUNT * nnode */
+  int8_t* fix;          /* 3 * nnode */
+  double ijac, ijac4, idcell;
+  /* time */
+  int istep;
+  double time, dt;
+  double ek, ei, mom[3], ek_prev;
+};
+
+#define P(o, f, i) ((o)->ps[(size_t)(f) * (o)->n + (i)])
+#define G(o, f, k) ((o)->g[(size_t)(f) * (o)->nnode + (k)])
+
+/* node coordinate of 1-based linear node index k (Grid.f90:296-302) */
+static void node_xg(const oracle* o, int k, double xg[3]) {
+  int k0 = k - 1;
+  int iz = k0 / o->ngxy;
+  int r = k0 - iz * o->ngxy;
+  int iy = r / o->ngx;
+  int ix = r - iy * o->ngx;
+  xg[0] = (double)ix * o->prob.dcell + o->prob.span[0];
+  xg[1] = (double)iy * o->prob.dcell + o->prob.span[2];
+  xg[2] = (double)iz * o->prob.dcell + o->prob.span[4];
+}
+
+/* first node of 1-based cell c (Grid.f90:366-383: CellsNode(c,1)) */
+static int cell_node1(const oracle* o, int c) {
+  int c0 = c - 1;
+  int iz = c0 / o->ncxy;
+  int r = c0 - iz * o->ncxy;
+  int iy = r / o->ncx;
+  int ix = r - iy * o->ncx;
+  return iz * o->ngxy + iy * o->ngx + ix + 1;
+}
+
+oracle* oracle_create(const mpm_problem* p) {
+  oracle* o = (oracle*)calloc(1, sizeof(oracle));
+  o->prob = *p;
+  o->mats = (mpm_material*)malloc(sizeof(mpm_material) * p->nb_mat);
+  memcpy(o->mats, p->mats, sizeof(mpm_material) * p->nb_mat);
+  o->bodies = (mpm_body*)malloc(sizeof(mpm_body) * p->nb_body);
+  memcpy(o->bodies, p->bodies, sizeof(mpm_body) * p->nb_body);
+  o->n = p->nb_particle;
+  o->ps = (double*)malloc(sizeof(double) * PF_COUNT * o->n);
+  memcpy(o->ps, p->pstate, sizeof(double) * PF_COUNT * o->n);
+  o->pbody = (int32_t*)malloc(sizeof(int32_t) * o->n);
+  memcpy(o->pbody, p->pbody, sizeof(int32_t) * o->n);
+  o->icell = (int32_t*)calloc(o->n, sizeof(int32_t));
+  o->xn = (double*)calloc((size_t)3 * o->n, sizeof(double));
+  o->ncx = p->ncx; o->ncy = p->ncy; o->ncz = p->ncz;
+  o->ncxy = o->ncx * o->ncy;
+  o->ncell = o->ncxy * o->ncz;
+  o->ngx = o->ncx + 1; o->ngy = o->ncy + 1; o->ngz = o->ncz + 1;
+  o->ngxy = o->ngx * o->ngy;
+  o->nnode = o->ngxy * o->ngz;
+  o->g = (double*)calloc((size_t)GF_COUNT * o->nnode, sizeof(double));
+  o->fix = (int8_t*)calloc((size_t)3 * o->nnode, 1);
+  /* fixities (Grid.f90:291-359) */
+  const int* FixS = p->fix;
+  int wide = (p->shape == MPM_SHAPE_GIMP || p->shape == MPM_SHAPE_BSPLINE || p->shape == MPM_SHAPE_SGMP);
+  for (int iz = 1; iz <= o->ngz; iz++)
+    for (int iy = 1; iy <= o->ngy; iy++)
+      for (int ix = 1; ix <= o->ngx; ix++) {
+        int k = (iz - 1) * o->ngxy + (iy - 1) * o->ngx + ix - 1;
+        int fx = 0, fy = 0, fz = 0;
+        int NGx = o->ngx, NGy = o->ngy, NGz = o->ngz;
+        if (wide) {
+          if (((ix == 1 || ix == 2) && FixS[0] == 1) || ((ix == NGx || ix == NGx - 1) && FixS[1] == 1) ||
+              ((iy == 1 || iy == 2) && FixS[2] == 1) || ((iy == NGy || iy == NGy - 1) && FixS[3] == 1) ||
+              ((iz == 1 || iz == 2) && FixS[4] == 1) || ((iz == NGz || iz == NGz - 1) && FixS[5] == 1))
+            fx = fy = fz = 1;
+          if (((ix == 1 || ix == 2) && FixS[0] == 2) || ((ix == NGx || ix == NGx - 1) && FixS[1] == 2)) fx = 1;
+          if (((iy == 1 || iy == 2) && FixS[2] == 2) || ((iy == NGy || iy == NGy - 1) && FixS[3] == 2)) fy = 1;
+          if (((iz == 1 || iz == 2) && FixS[4] == 2) || ((iz == NGz || iz == NGz - 1) && FixS[5] == 2)) fz = 1;
+        }
+        if ((ix == 1 && FixS[0] == 1) || (ix == NGx && FixS[1] == 1) || (iy == 1 && FixS[2] == 1) ||
+            (iy == NGy && FixS[3] == 1) || (iz == 1 && FixS[4] == 1) || (iz == NGz && FixS[5] == 1))
+          fx = fy = fz = 1;
+        if ((ix == 1 && FixS[0] == 2) || (ix == NGx && FixS[1] == 2)) fx = 1;
+        if ((iy == 1 && FixS[2] == 2) || (iy == NGy && FixS[3] == 2)) fy = 1;
+        if ((iz == 1 && FixS[4] == 2) || (iz == NGz && FixS[5] == 2)) fz = 1;
+        o->fix[k] = (int8_t)fx;
+        o->fix[o->nnode + k] = (int8_t)fy;
+        o->fix[2 * o->nnode + k] = (int8_t)fz;
+      }
+  o->ijac = 2.0 / p->dcell;          /* Grid.f90:414 */
+  o->ijac4 = o->ijac * 0.125;        /* :415 */
+  o->idcell = 1.0 / p->dcell;        /* :416 */
+  o->istep = 0;
+  o->time = 0.0;
+  o->dt = p->dt;
+  oracle_calc_energy(o);             /* MPM3D.f90:42 */
+  return o;
+}
+
+void oracle_destroy(oracle* o) {
+  if (!o) return;
+  free(o->mats); free(o->bodies); free(o->ps); free(o->pbody); free(o->icell);
+  free(o->xn); free(o->g); free(o->fix);
+  free(o);
+}
+
+int oracle_nb_node(const oracle* o) { return o->nnode; }
+int oracle_nb_particle(const oracle* o) { return o->n; }
+
+void oracle_begin_step(oracle* o) {
+  o->istep += 1;
+  o->time = o->time + o->dt;
+  o->ei = 0.0;
+}
+
+/* ---------------------------------------------------------------- cells */
+
+/* InWhichCell (Grid.f90:123-155) */
+static int in_which_cell(const oracle* o, const double x[3]) {
+  const double* s = o->prob.span;
+  if (x[0] < s[0] || x[0] > s[1] || x[1] < s[2] || x[1] > s[3] || x[2] < s[4] || x[2] > s[5]) return -1;
+  int ix = (int)((x[0] - s[0]) / o->prob.dcell) + 1;
+  int iy = (int)((x[1] - s[2]) / o->prob.dcell) + 1;
+  int iz = (int)((x[2] - s[4]) / o->prob.dcell) + 1;
+  int c = (iz - 1) * o->ncxy + (iy - 1) * o->ncx + ix;
+  if (c > o->ncell || c <= 0) c = -1;
+  return c;
+}
+
+/* ParticleInitial (update_step.f90:408-453), non-SGMP part */
+static void particle_initial(oracle* o) {
+  for (int p = 0; p < o->n; p++) {
+    double x[3] = {P(o, PF_X, p), P(o, PF_Y, p), P(o, PF_Z, p)};
+    int c = in_which_cell(o, x);
+    o->icell[p] = c;
+    if (c > 0) {
+      double xg[3];
+      node_xg(o, cell_node1(o, c), xg);
+      for (int d = 0; d < 3; d++) {
+        double cxg = xg[d] + o->prob.dcell / 2.0;   /* Grid.f90:380 */
+        o->xn[3 * p + d] = (x[d] - cxg) * o->ijac;
+      }
+    } else {
+      /* the reference reads cell_list(-1) here; the value is never used */
+      o->xn[3 * p] = o->xn[3 * p + 1] = o->xn[3 * p + 2] = 0.0;
+    }
+  }
+}
+
+/* ------------------------------------------------------- shape functions */
+
+typedef struct {
+  int nb;
+  int node[27];       /* 1-based linear node index (may be out of range) */
+  double shp[27], dx[27], dy[27], dz[27];
+} shape_t;
+
+static const int SNX[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+static const int SNY[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+static const int SNZ[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+
+/* NShape (Grid.f90:461-501) */
+static void nshape(const oracle* o, const double xn[3], shape_t* s) {
+  for (int i = 0; i < 8; i++) {
+    double sx = (double)SNX[i] * xn[0] + 1.0;
+    double sy = (double)SNY[i] * xn[1] + 1.0;
+    double sz = (double)SNZ[i] * xn[2] + 1.0;
+    s->shp[i] = sx * sy * sz * 0.125;
+    s->dx[i] = (double)SNX[i] * sy * sz * o->ijac4;
+    s->dy[i] = (double)SNY[i] * sx * sz * o->ijac4;
+    s->dz[i] = (double)SNZ[i] * sx * sy * o->ijac4;
+  }
+  s->nb = 8;
+}
+
+/* node offsets (dx,dy,dz) of FindInflNode's list, Grid.f90:621-847, for
+ * the z/y/x cases lo(-1) / hi(+2) / mid(0); zc is -1 (lo) or 2 (hi). */
+static int gimp_offsets(int zc, int yc, int xc, int off[27][3]) {
+  static const int base[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  int n = 0;
+  for (int i = 0; i < 8; i++) {
+    off[n][0] = base[i][0]; off[n][1] = base[i][1]; off[n][2] = base[i][2]; n++;
+  }
+#define ADD(a, b, c) do { off[n][0] = (a); off[n][1] = (b); off[n][2] = (c); n++; } while (0)
+  if (zc != 0) {
+    ADD(0, 0, zc); ADD(1, 0, zc); ADD(1, 1, zc); ADD(0, 1, zc);            /* 9-12 */
+    if (yc == -1) {
+      ADD(0, -1, zc); ADD(1, -1, zc); ADD(0, -1, 0); ADD(1, -1, 0); ADD(0, -1, 1); ADD(1, -1, 1);
+      if (xc == -1) {
+        ADD(-1, 0, 0); ADD(-1, 1, 0); ADD(-1, 0, 1); ADD(-1, 1, 1); ADD(-1, 0, zc); ADD(-1, 1, zc);
+        ADD(-1, -1, zc); ADD(-1, -1, 0); ADD(-1, -1, 1);
+      } else if (xc == 2) {
+        ADD(2, 0, 0); ADD(2, 1, 0); ADD(2, 0, 1); ADD(2, 1, 1); ADD(2, 0, zc); ADD(2, 1, zc);
+        ADD(2, -1, zc); ADD(2, -1, 0); ADD(2, -1, 1);
+      }
+    } else if (yc == 2) {
+      ADD(1, 2, zc); ADD(0, 2, zc); ADD(1, 2, 0); ADD(0, 2, 0); ADD(1, 2, 1); ADD(0, 2, 1);
+      if (xc == -1) {
+        ADD(-1, 0, 0); ADD(-1, 1, 0); ADD(-1, 0, 1); ADD(-1, 1, 1); ADD(-1, 0, zc); ADD(-1, 1, zc);
+        ADD(-1, 2, zc); ADD(-1, 2, 0); ADD(-1, 2, 1);
+      } else if (xc == 2) {
+        ADD(2, 0, 0); ADD(2, 1, 0); ADD(2, 0, 1); ADD(2, 1, 1); ADD(2, 0, zc); ADD(2, 1, zc);
+        ADD(2, 2, zc); ADD(2, 2, 0); ADD(2, 2, 1);
+      }
+    } else {
+      if (xc == -1) {
+        ADD(-1, 0, zc); ADD(-1, 1, zc); ADD(-1, 0, 0); ADD(-1, 1, 0); ADD(-1, 0, 1); ADD(-1, 1, 1);
+      } else if (xc == 2) {
+        ADD(2, 0, zc); ADD(2, 1, zc); ADD(2, 0, 0); ADD(2, 1, 0); ADD(2, 0, 1); ADD(2, 1, 1);
+      }
+    }
+  } else {
+    if (yc == -1) {
+      ADD(0, -1, 0); ADD(1, -1, 0); ADD(0, -1, 1); ADD(1, -1, 1);
+      if (xc == -1) {
+        ADD(-1, 0, 0); ADD(-1, 1, 0); ADD(-1, 0, 1); ADD(-1, 1, 1); ADD(-1, -1, 0); ADD(-1, -1, 1);
+      } else if (xc == 2) {
+        ADD(2, 0, 0); ADD(2, 1, 0); ADD(2, 0, 1); ADD(2, 1, 1); ADD(2, -1, 0); ADD(2, -1, 1);
+      }
+    } else if (yc == 2) {
+      ADD(1, 2, 0); ADD(0, 2, 0); ADD(1, 2, 1); ADD(0, 2, 1);
+      if (xc == -1) {
+        ADD(-1, 0, 0); ADD(-1, 1, 0); ADD(-1, 0, 1); ADD(-1, 1, 1); ADD(-1, 2, 0); ADD(-1, 2, 1);
+      } else if (xc == 2) {
+        ADD(2, 0, 0); ADD(2, 1, 0); ADD(2, 0, 1); ADD(2, 1, 1); ADD(2, 2, 0); ADD(2, 2, 1);
+      }
+    } else {
+      if (xc == -1) {
+        ADD(-1, 0, 0); ADD(-1, 1, 0); ADD(-1, 0, 1); ADD(-1, 1, 1);
+      } else if (xc == 2) {
+        ADD(2, 0, 0); ADD(2, 1, 0); ADD(2, 0, 1); ADD(2, 1, 1);
+      }
+    }
+  }
+#undef ADD
+  return n;
+}
+
+/* FindInflNode + NShape_GIMP (Grid.f90:546-599, 602-849) */
+static void gimp_shape(const oracle* o, int p, int c, shape_t* s) {
+  double x[3] = {P(o, PF_X, p), P(o, PF_Y, p), P(o, PF_Z, p)};
+  int n1 = cell_node1(o, c);
+  double xg[3], rpg1[3];
+  node_xg(o, n1, xg);
+  for (int d = 0; d < 3; d++) rpg1[d] = (x[d] - xg[d]) * o->idcell;
+  int zc = rpg1[2] < 0.25 ? -1 : (rpg1[2] > 0.75 ? 2 : 0);
+  int yc = rpg1[1] < 0.25 ? -1 : (rpg1[1] > 0.75 ? 2 : 0);
+  int xc = rpg1[0] < 0.25 ? -1 : (rpg1[0] > 0.75 ? 2 : 0);
+  int off[27][3];
+  int nb = gimp_offsets(zc, yc, xc, off);
+  s->nb = nb;
+  for (int i = 0; i < nb; i++) {
+    int k = n1 + off[i][0] + off[i][1] * o->ngx + off[i][2] * o->ngxy;
+    s->node[i] = k;
+    double rpg[3];
+    if (k <= o->nnode && k > 0) {
+      node_xg(o, k, xg);
+      for (int d = 0; d < 3; d++) rpg[d] = (x[d] - xg[d]) * o->idcell;
+    } else {
+      for (int d = 0; d < 3; d++) rpg[d] = 100.0 * o->idcell;
+      s->shp[i] = s->dx[i] = s->dy[i] = s->dz[i] = 0.0;
+      continue;   /* NShape_GIMP skips out-of-grid nodes */
+    }
+    double sh[3], dn[3];
+    for (int j = 0; j < 3; j++) {
+      double r = fabs(rpg[j]);
+      if (r < 0.25) {
+        sh[j] = (7 - 16 * r * r) * 0.125;
+        dn[j] = -4 * rpg[j];
+      } else if (r < 0.75) {
+        sh[j] = 1 - r;
+        dn[j] = -copysign(1.0, rpg[j]);
+      } else if (r < 1.25) {
+        double t = (5 - 4 * r) * 0.25;
+        sh[j] = t * t;
+        dn[j] = 2 * rpg[j] - copysign(1.0, rpg[j]) * 2.5;
+      } else {
+        sh[j] = 0.0;
+        dn[j] = 0.0;
+      }
+    }
+    s->shp[i] = sh[0] * sh[1] * sh[2];
+    s->dx[i] = dn[0] * sh[1] * sh[2] * o->idcell;
+    s->dy[i] = sh[0] * dn[1] * sh[2] * o->idcell;
+    s->dz[i] = sh[0] * sh[1] * dn[2] * o->idcell;
+  }
+}
+
+/* BFindInflNode + NShape_Bspline (Grid.f90:852-1049).  bsp_idx[k] gives the
+ * per-axis weight index (0: node i-1, 1: node i, 2: node i+1) of node k. */
+static const int BSP_IDX[27][3] = {
+    {1, 1, 1}, {2, 1, 1}, {2, 2, 1}, {1, 2, 1}, {1, 1, 2}, {2, 1, 2}, {2, 2, 2}, {1, 2, 2}, {1, 1, 0},
+    {2, 1, 0}, {2, 2, 0}, {1, 2, 0}, {1, 0, 0}, {2, 0, 0}, {1, 0, 1}, {2, 0, 1}, {1, 0, 2}, {2, 0, 2},
+    {0, 1, 1}, {0, 2, 1}, {0, 1, 2}, {0, 2, 2}, {0, 1, 0}, {0, 2, 0}, {0, 0, 0}, {0, 0, 1}, {0, 0, 2}};
+
+static void bspline_shape(const oracle* o, int p, int c, shape_t* s) {
+  double x[3] = {P(o, PF_X, p), P(o, PF_Y, p), P(o, PF_Z, p)};
+  int n1 = cell_node1(o, c);
+  double xg[3];
+  node_xg(o, n1, xg);
+  const double c02 = (double)0.2f, c04 = (double)0.4f;   /* FP32 literals, Grid.f90:927,960-962 */
+  double sh[3][3], dn[3][3];
+  for (int d = 0; d < 3; d++) {
+    double r = (x[d] - xg[d]) * o->idcell * c02 + c04;
+    double a = 3 - 5 * r, b = 5 * r - 2;
+    sh[0][d] = 0.5 * (a * a);
+    sh[1][d] = -(25 * (r * r)) + 25 * r - 5.5;
+    sh[2][d] = 0.5 * (b * b);
+    dn[0][d] = (5 * r - 3) * 5 * c02 * o->idcell;
+    dn[1][d] = (-(50 * r) + 25) * c02 * o->idcell;
+    dn[2][d] = (5 * r - 2) * 5 * c02 * o->idcell;
+  }
+  s->nb = 27;
+  for (int k = 0; k < 27; k++) {
+    int a = BSP_IDX[k][0], b = BSP_IDX[k][1], cc = BSP_IDX[k][2];
+    s->node[k] = n1 + (a - 1) + (b - 1) * o->ngx + (cc - 1) * o->ngxy;
+    s->shp[k] = sh[a][0] * sh[b][1] * sh[cc][2];
+    s->dx[k] = dn[a][0] * sh[b][1] * sh[cc][2];
+    s->dy[k] = sh[a][0] * dn[b][1] * sh[cc][2];
+    s->dz[k] = sh[a][0] * sh[b][1] * dn[cc][2];
+  }
+}
+
+static void shape_of(const oracle* o, int p, shape_t* s) {
+  int c = o->icell[p];
+  switch (o->prob.shape) {
+    case MPM_SHAPE_GIMP: gimp_shape(o, p, c, s); break;
+    case MPM_SHAPE_BSPLINE: bspline_shape(o, p, c, s); break;
+    default: {
+      int n1 = cell_node1(o, c);
+      nshape(o, &o->xn[3 * p], s);
+      s->node[0] = n1;
+      s->node[1] = n1 + 1;
+      s->node[2] = n1 + 1 + o->ngx;
+      s->node[3] = n1 + o->ngx;
+      s->node[4] = n1 + o->ngxy;
+      s->node[5] = n1 + o->ngxy + 1;
+      s->node[6] = n1 + o->ngxy + 1 + o->ngx;
+      s->node[7] = n1 + o->ngxy + o->ngx;
+    }
+  }
+}
+
+static inline int in_grid(const oracle* o, int k) { return k <= o->nnode && k > 0; }
+
+/* ----------------------------------------------------------- grid phases */
+
+void oracle_grid_momentum_initial(oracle* o) {
+  for (int k = 0; k < o->nnode; k++) G(o, GF_MG, k) = G(o, GF_PX, k) = G(o, GF_PY, k) = G(o, GF_PZ, k) = 0.0;
+  particle_initial(o);
+  shape_t s;
+  for (int p = 0; p < o->n; p++) {
+    if (o->icell[p] < 0) continue;
+    double mp = P(o, PF_MASS, p);
+    shape_of(o, p, &s);
+    for (int i = 0; i < s.nb; i++) {
+      int k = s.node[i];
+      if (!in_grid(o, k)) continue;
+      k -= 1;
+      double shm = s.shp[i] * mp;
+      G(o, GF_MG, k) = G(o, GF_MG, k) + shm;
+      G(o, GF_PX, k) = G(o, GF_PX, k) + P(o, PF_VX, p) * shm;
+      G(o, GF_PY, k) = G(o, GF_PY, k) + P(o, PF_VY, p) * shm;
+      G(o, GF_PZ, k) = G(o, GF_PZ, k) + P(o, PF_VZ, p) * shm;
+    }
+  }
+}
+
+void oracle_apply_bc(oracle* o) {
+  for (int k = 0; k < o->nnode; k++)
+    for (int d = 0; d < 3; d++)
+      if (o->fix[d * o->nnode + k]) {
+        G(o, GF_PX + d, k) = 0.0;
+        G(o, GF_FX + d, k) = 0.0;
+      }
+}
+
+void oracle_grid_momentum_update(oracle* o) {
+  for (int k = 0; k < o->nnode; k++) G(o, GF_FX, k) = G(o, GF_FY, k) = G(o, GF_FZ, k) = 0.0;
+  const mpm_problem* pr = &o->prob;
+  shape_t s;
+  for (int p = 0; p < o->n; p++) {
+    double fx[3];
+    for (int d = 0; d < 3; d++) {
+      double f = P(o, PF_FX + d, p);
+      if (pr->quasi_load) {
+        if (o->time >= pr->quasi_load_time) fx[d] = f;
+        else fx[d] = f / pr->quasi_load_time * o->time;
+      } else {
+        fx[d] = f;
+      }
+    }
+    if (pr->quasi_damp)
+      for (int d = 0; d < 3; d++) fx[d] = fx[d] - pr->damp_factor * P(o, PF_MASS, p) * P(o, PF_VX + d, p);
+    if (pr->gravity) {
+      const mpm_body* b = &o->bodies[o->pbody[p]];
+      for (int d = 0; d < 3; d++) fx[d] = fx[d] + P(o, PF_MASS, p) * b->grav[d];
+    }
+    double vol = P(o, PF_VOL, p);
+    if (o->icell[p] < 0) continue;
+    double smq = P(o, PF_SM, p) - P(o, PF_Q, p);
+    double sxx = smq + P(o, PF_SDXX, p);
+    double syy = smq + P(o, PF_SDYY, p);
+    double szz = smq + P(o, PF_SDZZ, p);
+    double sxy = P(o, PF_SDXY, p), syz = P(o, PF_SDYZ, p), sxz = P(o, PF_SDXZ, p);
+    shape_of(o, p, &s);
+    for (int i = 0; i < s.nb; i++) {
+      int k = s.node[i];
+      if (!in_grid(o, k)) continue;
+      k -= 1;
+      double dX = s.dx[i], dY = s.dy[i], dZ = s.dz[i], N = s.shp[i];
+      double fi0 = -(sxx * dX + sxy * dY + sxz * dZ) * vol;
+      double fi1 = -(sxy * dX + syy * dY + syz * dZ) * vol;
+      double fi2 = -(sxz * dX + syz * dY + szz * dZ) * vol;
+      G(o, GF_FX, k) = G(o, GF_FX, k) + fi0 + fx[0] * N;
+      G(o, GF_FY, k) = G(o, GF_FY, k) + fi1 + fx[1] * N;
+      G(o, GF_FZ, k) = G(o, GF_FZ, k) + fi2 + fx[2] * N;
+    }
+  }
+}
+
+void oracle_integrate_momentum(oracle* o) {
+  for (int k = 0; k < o->nnode; k++) {
+    for (int d = 0; d < 3; d++) {
+      if (o->istep == 1) G(o, GF_PX + d, k) = G(o, GF_PX + d, k) + G(o, GF_FX + d, k) * o->dt * 0.5;
+      else G(o, GF_PX + d, k) = G(o, GF_PX + d, k) + G(o, GF_FX + d, k) * o->dt;
+    }
+    for (int d = 0; d < 3; d++)
+      if (o->fix[d * o->nnode + k]) {
+        G(o, GF_PX + d, k) = 0.0;
+        G(o, GF_FX + d, k) = 0.0;
+      }
+  }
+}
+
+void oracle_particle_position_update(oracle* o) {
+  shape_t s;
+  const double cutoff = o->prob.cutoff;
+  for (int p = 0; p < o->n; p++) {
+    if (o->icell[p] < 0) continue;
+    double vx[3] = {0, 0, 0}, ax[3] = {0, 0, 0};
+    shape_of(o, p, &s);
+    for (int i = 0; i < s.nb; i++) {
+      int k = s.node[i];
+      if (!in_grid(o, k)) continue;
+      k -= 1;
+      double mg = G(o, GF_MG, k);
+      if (mg > cutoff) {
+        double N = s.shp[i];
+        for (int d = 0; d < 3; d++) {
+          vx[d] = vx[d] + N * (G(o, GF_PX + d, k) / mg);
+          ax[d] = ax[d] + N * G(o, GF_FX + d, k) / mg;
+        }
+      }
+    }
+    for (int d = 0; d < 3; d++) {
+      P(o, PF_XX + d, p) = P(o, PF_X + d, p) + vx[d] * o->dt;
+      if (o->istep == 1) P(o, PF_VX + d, p) = P(o, PF_VX + d, p) + ax[d] * o->dt * 0.5;
+      else P(o, PF_VX + d, p) = P(o, PF_VX + d, p) + ax[d] * o->dt;
+    }
+    if (o->prob.scheme == MPM_USF)
+      for (int d = 0; d < 3; d++) P(o, PF_X + d, p) = P(o, PF_XX + d, p);
+  }
+}
+
+void oracle_grid_momentum_musl(oracle* o) {
+  for (int k = 0; k < o->nnode; k++) G(o, GF_PX, k) = G(o, GF_PY, k) = G(o, GF_PZ, k) = 0.0;
+  shape_t s;
+  for (int p = 0; p < o->n; p++) {
+    if (o->icell[p] < 0) continue;
+    double mp = P(o, PF_MASS, p);
+    shape_of(o, p, &s);
+    for (int i = 0; i < s.nb; i++) {
+      int k = s.node[i];
+      if (!in_grid(o, k)) continue;
+      k -= 1;
+      double shm = s.shp[i] * mp;
+      for (int d = 0; d < 3; d++) G(o, GF_PX + d, k) = G(o, GF_PX + d, k) + P(o, PF_VX + d, p) * shm;
+    }
+  }
+  oracle_apply_bc(o);
+}
+
+/* ------------------------------------------------------- constitution */
+
+typedef struct {
+  /* module variables of MaterialModel (Constitution.f90:24-75) used here */
+  double dinc[6], vold, vol_, dvol, sm, smold, sd[6], sig[6], sold[6], bqf;
+  double seqv, epeff_, sig_y_, depeff, ratio, G2, K3, PlaMod, lamda, miu2;
+  double iener, specen, ieinc, den0_, den_, vol0_, cp, yield0_;
+  double F[3][3];
+} cst_t;
+
+/* EquivalentStress returns a default REAL (Constitution.f90:679-699) */
+static double equivalent_stress(const cst_t* m) {
+  const double* sd = m->sd;
+  double J2 = 0.5 * (sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2]) + sd[3] * sd[3] + sd[4] * sd[4] + sd[5] * sd[5];
+  J2 = J2 * 3.0;
+  return (double)(float)sqrt(J2);
+}
+
+/* sigrot (Constitution.f90:527-575) */
+static void sigrot(cst_t* m, const double vort[3]) {
+  double* sig = m->sig;
+  double q0 = 2.0 * sig[5] * vort[2];
+  double q1 = 2.0 * sig[4] * vort[1];
+  double q2 = 2.0 * sig[3] * vort[0];
+  double rot[6];
+  rot[0] = -q0 + q1;
+  rot[1] = +q0 - q2;
+  rot[2] = -q1 + q2;
+  rot[3] = vort[0] * (sig[1] - sig[2]) + vort[2] * sig[4] - vort[1] * sig[5];
+  rot[4] = vort[1] * (sig[2] - sig[0]) + vort[0] * sig[5] - vort[2] * sig[3];
+  rot[5] = vort[2] * (sig[0] - sig[1]) + vort[1] * sig[3] - vort[0] * sig[4];
+  for (int i = 0; i < 6; i++) sig[i] = sig[i] + rot[i];
+  m->sm = (sig[0] + sig[1] + sig[2]) / 3.0;
+  m->sd[0] = sig[0] - m->sm;
+  m->sd[1] = sig[1] - m->sm;
+  m->sd[2] = sig[2] - m->sm;
+  m->sd[3] = sig[3];
+  m->sd[4] = sig[4];
+  m->sd[5] = sig[5];
+}
+
+static void elastic_devi(cst_t* m) {   /* :578-604 */
+  double dem = (m->dinc[0] + m->dinc[1] + m->dinc[2]) / 3.0;
+  double G = 0.5 * m->G2;
+  m->sd[0] = m->sd[0] + m->G2 * (m->dinc[0] - dem);
+  m->sd[1] = m->sd[1] + m->G2 * (m->dinc[1] - dem);
+  m->sd[2] = m->sd[2] + m->G2 * (m->dinc[2] - dem);
+  m->sd[3] = m->sd[3] + G * m->dinc[3];
+  m->sd[4] = m->sd[4] + G * m->dinc[4];
+  m->sd[5] = m->sd[5] + G * m->dinc[5];
+}
+
+static void elastic_p(cst_t* m) {      /* :607-624 */
+  double dem = (m->dinc[0] + m->dinc[1] + m->dinc[2]) / 3.0;
+  double dsm = m->K3 * dem;
+  m->sm = m->sm + dsm;
+}
+
+static double sig_dot_dinc(const cst_t* m) {
+  return m->sig[0] * m->dinc[0] + m->sig[1] * m->dinc[1] + m->sig[2] * m->dinc[2] + m->sig[3] * m->dinc[3] +
+         m->sig[4] * m->dinc[4] + m->sig[5] * m->dinc[5];
+}
+
+static double new_stress_dot_dinc(const cst_t* m, double ieinc) {
+  return ieinc + (m->sd[0] + m->sm) * m->dinc[0] + (m->sd[1] + m->sm) * m->dinc[1] +
+         (m->sd[2] + m->sm) * m->dinc[2] + m->sd[3] * m->dinc[3] + m->sd[4] * m->dinc[4] + m->sd[5] * m->dinc[5];
+}
+
+/* bulkq (Constitution.f90:1543-1563) */
+static void bulkq(cst_t* m, const oracle* o) {
+  double dd = (m->dinc[0] + m->dinc[1] + m->dinc[2]) / o->dt;
+  dd = dd < 0.0 ? dd : 0.0;
+  double h = o->prob.dcell;
+  m->bqf = m->den_ * h * h * o->prob.bq1 * dd * dd - o->prob.bq2 * m->den_ * h * m->cp * dd;
+}
+
+/* lieupd (Constitution.f90:627-646); vavg is a default REAL */
+static void lieupd(cst_t* m, const oracle* o) {
+  if (m->sm < 0) bulkq(m, o);
+  m->ieinc = m->ieinc - 2.0 * m->bqf * (m->dinc[0] + m->dinc[1] + m->dinc[2]);
+  float vavg = (float)(m->vol_ + m->vold);
+  m->iener = m->iener + 0.25 * m->ieinc * (double)vavg;
+}
+
+/* hieupd, non-failed branch (Constitution.f90:649-676) */
+static void hieupd(cst_t* m) {
+  float vavg = (float)(m->vol_ + m->vold);
+  double e = m->dinc[0] * (m->sold[0] + m->sd[0]) + m->dinc[1] * (m->sold[1] + m->sd[1]) +
+             m->dinc[2] * (m->sold[2] + m->sd[2]) + m->dinc[3] * (m->sold[3] + m->sd[3]) +
+             m->dinc[4] * (m->sold[4] + m->sd[4]) + m->dinc[5] * (m->sold[5] + m->sd[5]);
+  m->ieinc = 0.25 * e * (double)vavg + m->dvol * (m->smold - 2.0 * m->bqf);
+  m->iener = m->iener + m->ieinc;
+  m->specen = m->iener / m->vol0_;
+}
+
+static void m3dm1(cst_t* m) {          /* :702-724 elastic */
+  m->ieinc = sig_dot_dinc(m);
+  elastic_devi(m);
+  elastic_p(m);
+  m->seqv = equivalent_stress(m);
+  m->ieinc = new_stress_dot_dinc(m, m->ieinc);
+}
+
+static void m3dm2(cst_t* m) {          /* :885-923 elastic-perfectly plastic */
+  m->ieinc = sig_dot_dinc(m);
+  elastic_devi(m);
+  m->seqv = equivalent_stress(m);
+  if (m->seqv > m->yield0_) {
+    m->depeff = (m->seqv - m->sig_y_) / (1.5 * m->G2);
+    m->epeff_ = m->epeff_ + m->depeff;
+    m->ratio = m->yield0_ / m->seqv;
+    for (int i = 0; i < 6; i++) m->sd[i] = m->sd[i] * m->ratio;
+    m->seqv = m->seqv * m->ratio;
+  }
+  elastic_p(m);
+  m->ieinc = new_stress_dot_dinc(m, m->ieinc);
+}
+
+static void m3dm3(cst_t* m) {          /* :926-966 J2 isotropic linear hardening */
+  m->ieinc = sig_dot_dinc(m);
+  elastic_devi(m);
+  m->seqv = equivalent_stress(m);
+  if (m->seqv > m->sig_y_) {
+    m->depeff = (m->seqv - m->sig_y_) / (1.5 * m->G2 + m->PlaMod);
+    m->epeff_ = m->epeff_ + m->depeff;
+    m->sig_y_ = m->sig_y_ + m->PlaMod * m->depeff;
+    m->ratio = m->sig_y_ / m->seqv;
+    for (int i = 0; i < 6; i++) m->sd[i] = m->sd[i] * m->ratio;
+    m->seqv = m->seqv * m->ratio;
+  }
+  elastic_p(m);
+  m->ieinc = new_stress_dot_dinc(m, m->ieinc);
+}
+
+static void m3dm7(cst_t* m, double DT) {   /* :1124-1142 null material */
+  for (int i = 0; i < 6; i++) m->sd[i] = 0.0;
+  /* tri = 1/3*(...): integer division 1/3 = 0 */
+  double tri = 0.0 * (m->dinc[0] / DT + m->dinc[1] / DT + m->dinc[2] / DT);
+  m->sd[0] = m->miu2 * (m->dinc[0] / DT - tri);
+  m->sd[1] = m->miu2 * (m->dinc[1] / DT - tri);
+  m->sd[2] = m->miu2 * (m->dinc[2] / DT - tri);
+  m->sd[3] = 0.5 * m->miu2 * m->dinc[3] / DT;
+  m->sd[4] = 0.5 * m->miu2 * m->dinc[4] / DT;
+  m->sd[5] = 0.5 * m->miu2 * m->dinc[5] / DT;
+}
+
+static void m3dm9(cst_t* m, const mpm_material* mat, double young, double poisson) {  /* :1214-1330 */
+  double Gmod = young / (2 * (1 + poisson));
+  double Kmod = young / (3 * (1 - 2 * poisson));
+  double qfai = mat->q_fai, kfai = mat->k_fai, qpsi = mat->q_psi;
+  m->ieinc = sig_dot_dinc(m);
+  double tenf;
+  if (qfai == 0.0) {
+    tenf = 0.0;
+  } else {
+    double tenf_max = kfai / qfai;
+    tenf = mat->ten_f < tenf_max ? mat->ten_f : tenf_max;
+  }
+  elastic_devi(m);
+  elastic_p(m);
+  const double* sd = m->sd;
+  double J2 = 0.5 * (sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2]) + sd[3] * sd[3] + sd[4] * sd[4] + sd[5] * sd[5];
+  double Tau = sqrt(J2);
+  m->seqv = Tau * (double)sqrtf(3.0f);
+  double dpFi = Tau + qfai * m->sm - kfai;
+  double dpsig = m->sm - tenf;
+  const double third32 = (double)(1.0f / 3.0f), twoninth32 = (double)(2.0f / 9.0f);
+  int shear = 0;
+  if (dpsig < 0.0) {
+    if (dpFi > 0.0) shear = 1;
+  } else {
+    double alphap = sqrt(1 + qfai * qfai) - qfai;
+    double Taup = kfai - qfai * tenf;
+    double dp_hfai = Tau - Taup - alphap * dpsig;
+    if (dp_hfai > 0.0) {
+      shear = 1;
+    } else {
+      double dlamd = (m->sm - tenf) / Kmod;
+      m->sm = tenf;
+      m->depeff = dlamd * third32 * (double)sqrtf(2.0f);
+      m->epeff_ = m->epeff_ + m->depeff;
+    }
+  }
+  if (shear) {
+    double dlamd = dpFi / (Gmod + Kmod * qfai * qpsi);
+    m->sm = m->sm - Kmod * qpsi * dlamd;
+    double newTau = kfai - qfai * m->sm;
+    m->ratio = newTau / Tau;
+    for (int i = 0; i < 6; i++) m->sd[i] = m->sd[i] * m->ratio;
+    m->seqv = m->seqv * m->ratio;
+    m->depeff = dlamd * sqrt(third32 + twoninth32 * (qpsi * qpsi));
+    m->epeff_ = m->epeff_ + m->depeff;
+  }
+  m->ieinc = new_stress_dot_dinc(m, m->ieinc);
+}
+
+static double det3(const double F[3][3]) {
+  return F[0][0] * F[1][1] * F[2][2] - F[0][0] * F[1][2] * F[2][1] - F[1][0] * F[0][1] * F[2][2] +
+         F[1][0] * F[0][2] * F[2][1] + F[2][0] * F[0][1] * F[1][2] - F[2][0] * F[0][2] * F[1][1];
+}
+
+static void m3dm10(cst_t* m) {         /* :725-780 neo-Hookean */
+  double G = 0.5 * m->G2;
+  double (*F)[3] = m->F;
+  double J = det3(m->F);
+  double B[6];
+  B[0] = F[0][0] * F[0][0] + F[0][1] * F[0][1] + F[0][2] * F[0][2] - 1.0;
+  B[1] = F[1][0] * F[1][0] + F[1][1] * F[1][1] + F[1][2] * F[1][2] - 1.0;
+  B[2] = F[2][0] * F[2][0] + F[2][1] * F[2][1] + F[2][2] * F[2][2] - 1.0;
+  B[3] = F[0][0] * F[1][0] + F[0][1] * F[1][1] + F[0][2] * F[1][2];
+  B[4] = F[1][0] * F[2][0] + F[1][1] * F[2][1] + F[1][2] * F[2][2];
+  B[5] = F[0][0] * F[2][0] + F[0][1] * F[2][1] + F[0][2] * F[2][2];
+  m->ieinc = sig_dot_dinc(m);
+  double iJ = 1.0 / J;
+  double lj = m->lamda * log(J) * 1.0;
+  m->sig[0] = iJ * (G * B[0] + lj);
+  m->sig[1] = iJ * (G * B[1] + lj);
+  m->sig[2] = iJ * (G * B[2] + lj);
+  m->sig[3] = iJ * G * B[4];
+  m->sig[4] = iJ * G * B[5];
+  m->sig[5] = iJ * G * B[3];
+  m->sm = (m->sig[0] + m->sig[1] + m->sig[2]) / 3.0;
+  m->sd[0] = m->sig[0] - m->sm;
+  m->sd[1] = m->sig[1] - m->sm;
+  m->sd[2] = m->sig[2] - m->sm;
+  m->sd[3] = m->sig[3];
+  m->sd[4] = m->sig[4];
+  m->sd[5] = m->sig[5];
+  m->seqv = equivalent_stress(m);
+  m->ieinc = new_stress_dot_dinc(m, m->ieinc);
+}
+
+/* eos4 (Constitution.f90:1513-1541) via seleos, non-failed particle */
+static void eos4(cst_t* m, const oracle* o, const mpm_material* mat) {
+  double c0 = mat->ceos[0];
+  if (m->sm < 0) bulkq(m, o);
+  hieupd(m);
+  double pnew = c0 * c0 * (m->den_ - m->den0_);
+  m->iener = m->iener - m->dvol * pnew;
+  m->sm = -pnew;
+}
+
+/* Constitution (Constitution.f90:79-368) */
+static void constitution(oracle* o, int p, const double de[6], const double vort[3], const double deFp[3][3]) {
+  cst_t m;
+  memset(&m, 0, sizeof m);
+  for (int i = 0; i < 6; i++) m.dinc[i] = de[i];
+  m.vold = P(o, PF_VOL, p);
+  if (o->prob.update_v_by_fp) {
+    double deF[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        deF[i][j] = deFp[i][j];
+        m.F[i][j] = P(o, PF_FP + 3 * i + j, p);
+      }
+    for (int i = 0; i < 3; i++) {
+      deF[i][i] = deF[i][i] + 1.0;
+      m.F[i][i] = m.F[i][i] + 1.0;
+    }
+    double detF = det3(deF);
+    P(o, PF_VOL, p) = m.vold * detF;
+  } else {
+    P(o, PF_VOL, p) = m.vold * (1 + de[0] + de[1] + de[2]);
+  }
+  m.vol_ = P(o, PF_VOL, p);
+  m.dvol = 0.5 * (m.vol_ - m.vold);
+  const mpm_material* mat = &o->mats[o->bodies[o->pbody[p]].mat - 1];
+  m.epeff_ = P(o, PF_EPEFF, p);
+  m.sig_y_ = P(o, PF_SIGY, p);
+  m.seqv = P(o, PF_SEQV, p);
+  m.iener = P(o, PF_IE, p);
+  double mp = P(o, PF_MASS, p);
+  m.miu2 = 2 * mat->miu;
+  double young = mat->young, poisson = mat->poisson;
+  m.yield0_ = mat->yield0;
+  double tangmod = mat->tangmod;
+  m.den0_ = mat->density;
+  m.vol0_ = mp / m.den0_;
+  m.den_ = mp / m.vol_;
+  m.specen = m.iener / m.vol0_;
+  const double DMG = 0.0;
+  young = young * (1.0 - DMG + 0x1p-52);                 /* epsilon(1.0d0) */
+  m.G2 = young / (1 + poisson);
+  m.K3 = young / (1 - 2 * poisson);
+  m.lamda = (young * poisson) / ((1 + poisson) * (1 - 2 * poisson));
+  m.PlaMod = young * tangmod / (young - tangmod);
+  m.cp = P(o, PF_CP, p);
+  m.sm = P(o, PF_SM, p);
+  m.smold = m.sm;
+  m.sd[0] = P(o, PF_SDXX, p);
+  m.sd[1] = P(o, PF_SDYY, p);
+  m.sd[2] = P(o, PF_SDZZ, p);
+  m.sd[3] = P(o, PF_SDYZ, p);
+  m.sd[4] = P(o, PF_SDXZ, p);
+  m.sd[5] = P(o, PF_SDXY, p);
+  m.sig[0] = m.sd[0] + m.sm;
+  m.sig[1] = m.sd[1] + m.sm;
+  m.sig[2] = m.sd[2] + m.sm;
+  m.sig[3] = m.sd[3];
+  m.sig[4] = m.sd[4];
+  m.sig[5] = m.sd[5];
+  for (int i = 0; i < 6; i++) m.sold[i] = m.sd[i];
+  m.bqf = 0.0;
+  switch (mat->type) {
+    case MPM_MAT_ELASTIC: sigrot(&m, vort); m3dm1(&m); lieupd(&m, o); break;
+    case MPM_MAT_PLA1: sigrot(&m, vort); m3dm2(&m); lieupd(&m, o); break;
+    case MPM_MAT_PLA2: sigrot(&m, vort); m3dm3(&m); lieupd(&m, o); break;
+    case MPM_MAT_NULL: m3dm7(&m, o->dt); eos4(&m, o, mat); break;
+    case MPM_MAT_DP: sigrot(&m, vort); m3dm9(&m, mat, young, poisson); lieupd(&m, o); break;
+    case MPM_MAT_NEOH: m3dm10(&m); lieupd(&m, o); break;
+    default: break;
+  }
+  P(o, PF_SM, p) = m.sm;
+  P(o, PF_SEQV, p) = m.seqv;
+  P(o, PF_SDXX, p) = m.sd[0];
+  P(o, PF_SDYY, p) = m.sd[1];
+  P(o, PF_SDZZ, p) = m.sd[2];
+  P(o, PF_SDYZ, p) = m.sd[3];
+  P(o, PF_SDXZ, p) = m.sd[4];
+  P(o, PF_SDXY, p) = m.sd[5];
+  P(o, PF_SIGY, p) = m.sig_y_;
+  P(o, PF_EPEFF, p) = m.epeff_;
+  P(o, PF_IE, p) = m.iener;
+  P(o, PF_CP, p) = m.cp;
+  P(o, PF_Q, p) = m.bqf;
+}
+
+/* ParticleStressUpdate (update_step.f90:456-647), non-SGMP part */
+void oracle_particle_stress_update(oracle* o) {
+  shape_t s;
+  const double cutoff = o->prob.cutoff;
+  for (int p = 0; p < o->n; p++) {
+    double de[6] = {0, 0, 0, 0, 0, 0}, vort[3] = {0, 0, 0}, deFp[3][3];
+    memset(deFp, 0, sizeof deFp);
+    if (o->icell[p] < 0) continue;
+    shape_of(o, p, &s);
+    for (int i = 0; i < s.nb; i++) {
+      int k = s.node[i];
+      if (!in_grid(o, k)) continue;
+      k -= 1;
+      double mg = G(o, GF_MG, k);
+      if (mg > cutoff) {
+        double v[3] = {G(o, GF_PX, k) / mg, G(o, GF_PY, k) / mg, G(o, GF_PZ, k) / mg};
+        double dX = s.dx[i], dY = s.dy[i], dZ = s.dz[i];
+        de[0] = de[0] + dX * v[0];
+        de[1] = de[1] + dY * v[1];
+        de[2] = de[2] + dZ * v[2];
+        de[3] = de[3] + (dY * v[2] + dZ * v[1]);
+        de[4] = de[4] + (dZ * v[0] + dX * v[2]);
+        de[5] = de[5] + (dX * v[1] + dY * v[0]);
+        vort[0] = vort[0] + (dY * v[2] - dZ * v[1]);
+        vort[1] = vort[1] + (dZ * v[0] - dX * v[2]);
+        vort[2] = vort[2] + (dX * v[1] - dY * v[0]);
+        for (int d = 0; d < 3; d++) {
+          deFp[d][0] = deFp[d][0] + dX * v[d];
+          deFp[d][1] = deFp[d][1] + dY * v[d];
+          deFp[d][2] = deFp[d][2] + dZ * v[d];
+        }
+      }
+    }
+    for (int i = 0; i < 6; i++) de[i] = de[i] * o->dt;
+    for (int i = 0; i < 3; i++) vort[i] = vort[i] * o->dt / 2.0;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) deFp[i][j] = deFp[i][j] * o->dt;
+    if (o->prob.update_v_by_fp) {
+      double F0[3][3];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) F0[i][j] = P(o, PF_FP + 3 * i + j, p);
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          P(o, PF_FP + 3 * i + j, p) = F0[i][j] + deFp[i][j] + deFp[i][0] * F0[0][j] + deFp[i][1] * F0[1][j] +
+                                       deFp[i][2] * F0[2][j];
+    }
+    constitution(o, p, de, vort, (const double(*)[3])deFp);
+    if (o->prob.scheme != MPM_USF)
+      for (int d = 0; d < 3; d++) P(o, PF_X + d, p) = P(o, PF_XX + d, p);
+  }
+}
+
+/* --------------------------------------------------------- reductions */
+
+void oracle_calc_energy(oracle* o) {
+  o->ek_prev = o->ek;
+  o->ek = 0.0;
+  o->ei = 0.0;
+  o->mom[0] = o->mom[1] = o->mom[2] = 0.0;
+  for (int p = 0; p < o->n; p++) {
+    double mp = P(o, PF_MASS, p);
+    double vx = P(o, PF_VX, p), vy = P(o, PF_VY, p), vz = P(o, PF_VZ, p);
+    double dot = 0.0;
+    dot = dot + vx * vx;
+    dot = dot + vy * vy;
+    dot = dot + vz * vz;
+    o->ek = o->ek + dot * mp * 0.5;
+    o->ei = o->ei + P(o, PF_IE, p);
+    o->mom[0] = o->mom[0] + mp * vx;
+    o->mom[1] = o->mom[1] + mp * vy;
+    o->mom[2] = o->mom[2] + mp * vz;
+  }
+}
+
+void oracle_update_dt(oracle* o) {
+  double maxv = 0.0;
+  for (int p = 0; p < o->n; p++) {
+    double a = fabs(P(o, PF_VX, p));
+    double b = fabs(P(o, PF_VY, p));
+    double c = fabs(P(o, PF_VZ, p));
+    double mv = a;
+    if (b > mv) mv = b;
+    if (c > mv) mv = c;
+    if (maxv < mv) maxv = mv;
+  }
+  double dt = (double)1e6f;
+  for (int m = 0; m < o->prob.nb_mat; m++) {
+    const mpm_material* mt = &o->mats[m];
+    double E = mt->young, nu = mt->poisson, ro = mt->density;
+    double vv = sqrt(E * (1 - nu) / (1 + nu) / (1 - 2 * nu) / ro);
+    vv = fmax(vv, mt->det_speed);
+    vv = fmax(vv, mt->wavespd);
+    vv = vv + maxv;
+    dt = fmin(dt, o->prob.dcell / vv);
+  }
+  o->dt = dt * o->prob.dtscale;
+}
+
+void oracle_step(oracle* o, int nsteps) {
+  for (int k = 0; k < nsteps; k++) {
+    oracle_begin_step(o);
+    oracle_grid_momentum_initial(o);
+    if (o->prob.scheme == MPM_USF) {
+      oracle_apply_bc(o);
+      oracle_particle_stress_update(o);
+    }
+    oracle_grid_momentum_update(o);
+    oracle_integrate_momentum(o);
+    oracle_particle_position_update(o);
+    if (o->prob.scheme == MPM_MUSL) {
+      oracle_grid_momentum_musl(o);
+      oracle_apply_bc(o);
+    }
+    if (o->prob.scheme != MPM_USF) oracle_particle_stress_update(o);
+    oracle_calc_energy(o);
+    oracle_update_dt(o);
+  }
+}
+
+/* ------------------------------------------------------------ getters */
+
+void oracle_get_particles(const oracle* o, double* out) {
+  memcpy(out, o->ps, sizeof(double) * PF_COUNT * o->n);
+}
+
+void oracle_set_particles(oracle* o, const double* in) {
+  memcpy(o->ps, in, sizeof(double) * PF_COUNT * o->n);
+}
+
+void oracle_get_icell(const oracle* o, int32_t* out) {
+  memcpy(out, o->icell, sizeof(int32_t) * o->n);
+}
+
+void oracle_get_grid(const oracle* o, double* out) {
+  memcpy(out, o->g, sizeof(double) * GF_COUNT * o->nnode);
+}
+
+void oracle_get_fix(const oracle* o, int8_t* out) {
+  memcpy(out, o->fix, (size_t)3 * o->nnode);
+}
+
+void oracle_get_scalars(const oracle* o, double* out) {
+  out[0] = o->istep;
+  out[1] = o->time;
+  out[2] = o->dt;
+  out[3] = o->ek;
+  out[4] = o->ei;
+  out[5] = o->mom[0];
+  out[6] = o->mom[1];
+  out[7] = o->mom[2];
+}

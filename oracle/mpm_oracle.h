@@ -1,0 +1,56 @@
+/*
+ * mpm_oracle.h -- CPU FP64 restatement of the MPM3D explicit step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the device path is compared
+ * against; it is never linked into the product library.  Only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py load it.
+ *
+ * It follows the reference statement by statement, in the reference's
+ * summation and evaluation order (ascending particle index, reference node
+ * order per shape function), so that it is bit-exact against the Fortran
+ * build in oracle/_ref on the same deck.
+ */
+#ifndef MPM_ORACLE_H
+#define MPM_ORACLE_H
+
+#include <stdint.h>
+#include "../include/mpm3d_problem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle oracle;
+
+oracle* oracle_create(const mpm_problem* p);
+void oracle_destroy(oracle* o);
+int oracle_nb_node(const oracle* o);
+int oracle_nb_particle(const oracle* o);
+
+/* MPM3D.f90:72-74 */
+void oracle_begin_step(oracle* o);
+/* update_step.f90 phases */
+void oracle_grid_momentum_initial(oracle* o);     /* :288-406 (+ ParticleInitial :408-453) */
+void oracle_apply_bc(oracle* o);                  /* :1795-1830 */
+void oracle_particle_stress_update(oracle* o);    /* :456-647 */
+void oracle_grid_momentum_update(oracle* o);      /* :831-1013 */
+void oracle_integrate_momentum(oracle* o);        /* :1194-1239 */
+void oracle_particle_position_update(oracle* o);  /* :1378-1526 */
+void oracle_grid_momentum_musl(oracle* o);        /* :1668-1793 */
+void oracle_calc_energy(oracle* o);               /* Particle.f90:204-230 */
+void oracle_update_dt(oracle* o);                 /* update_step.f90:1833-1874 */
+/* whole steps in the order of MPM3D.f90:69-113 */
+void oracle_step(oracle* o, int nsteps);
+
+void oracle_get_particles(const oracle* o, double* out);   /* PF_COUNT * nb_particle */
+void oracle_get_icell(const oracle* o, int32_t* out);      /* nb_particle, 1-based, -1 outside */
+void oracle_get_grid(const oracle* o, double* out);        /* GF_COUNT * nb_node */
+void oracle_get_fix(const oracle* o, int8_t* out);         /* 3 * nb_node */
+/* out[0..7] = istep, CurrentTime, DT, EngKinetic, EngInternal, Momentum(3) */
+void oracle_get_scalars(const oracle* o, double* out);
+void oracle_set_particles(oracle* o, const double* in);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
