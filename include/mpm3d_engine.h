@@ -1,0 +1,80 @@
+/*
+ * mpm3d_engine.h -- batched / served form of the MI355X step engine.
+ *
+ * The reference runs one problem per process (MPM3D.f90).  An engine holds
+ * many independent instances in one device pool and advances all live
+ * instances with one launch per phase per (shape, scheme) group; a step
+ * sequence can be captured once as a HIP graph and replayed.  DT, time and
+ * the end-of-run test live on the device, so instances of different sizes
+ * and time steps stop on their own.  Instances can be added and removed
+ * while others are resident (continuous batching).  None of these entry
+ * points has a counterpart in the reference; they serve the ensemble and
+ * serving drivers.
+ */
+#ifndef MPM3D_ENGINE_H
+#define MPM3D_ENGINE_H
+
+#include <stdint.h>
+#include "mpm3d_problem.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpm_engine mpm_engine;
+typedef struct mpm_ctx mpm_ctx;
+
+/* explicit capacities (particles, nodes, instances, material and body slots) */
+int mpm_engine_create(long max_particles, long max_nodes, int max_instances, int max_materials, int max_bodies,
+                      int precision, mpm_engine** out);
+/* capacities derived from an HBM byte budget */
+int mpm_engine_create_budget(double hbm_bytes, int max_instances, int precision, mpm_engine** out);
+int mpm_engine_destroy(mpm_engine* e);
+/* device bytes held by the pool */
+long mpm_engine_device_bytes(mpm_engine* e);
+
+/* admit an instance; MPM_ERR_NOMEM when it does not fit now.  With
+ * stop_at_endtime the instance marks itself done once CurrentTime > endtime
+ * (MPM3D.f90:152-158). */
+int mpm_engine_add(mpm_engine* e, const mpm_problem* p, int stop_at_endtime, int* id);
+int mpm_engine_remove(mpm_engine* e, int id);
+/* largest free particle / node ranges and free instance slots */
+int mpm_engine_capacity(mpm_engine* e, long* free_particles, long* free_nodes, int* free_instances);
+
+/* advance every live, unfinished instance by nsteps; with use_graph the
+ * step sequence of graph_steps steps is captured once and replayed */
+int mpm_engine_step(mpm_engine* e, int nsteps, int use_graph, int graph_steps);
+int mpm_engine_sync(mpm_engine* e);
+/* stable re-sort of every instance's particles by cell */
+int mpm_engine_sort(mpm_engine* e);
+/* block sizes for particle and node kernels (64/128/256), re-sort period */
+int mpm_engine_set_tuning(mpm_engine* e, int block_particles, int block_nodes, int sort_every);
+/* interleaved (1) or contiguous (0) lane -> particle mapping in particle kernels */
+int mpm_engine_set_interleave(mpm_engine* e, int enable);
+
+/* per-instance state (synchronising); layouts as in mpm3d_hip.h */
+int mpm_engine_download_particles(mpm_engine* e, int id, double* pstate);
+int mpm_engine_download_grid(mpm_engine* e, int id, double* gstate);
+int mpm_engine_download_icell(mpm_engine* e, int id, int32_t* icell);
+int mpm_engine_get_scalars(mpm_engine* e, int id, double* out8);
+/* original particle index held by each device slot (the sort permutation) */
+int mpm_engine_download_order(mpm_engine* e, int id, int32_t* orig);
+/* done flag (1 finished, 0 running, -1 no such instance) and istep */
+int mpm_engine_status(mpm_engine* e, int n, const int* ids, int* done, int* istep);
+
+/* Profiling: with enable != 0 every launch of the following steps is
+ * bracketed by HIP events on the engine's stream; kernel_times returns the
+ * accumulated milliseconds and launch counts per kernel class
+ * (0 zero, 1 p2g, 2 bc, 3 stress, 4 integrate, 5 g2p, 6 finalize, 7 begin;
+ * names from mpm_engine_kernel_name). */
+int mpm_engine_profile(mpm_engine* e, int enable);
+int mpm_engine_kernel_times(mpm_engine* e, double* total_ms8, long* launches8);
+const char* mpm_engine_kernel_name(int kid);
+
+/* the engine behind a single-problem context */
+int mpm_ctx_engine(mpm_ctx* ctx, mpm_engine** out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

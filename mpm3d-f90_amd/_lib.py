@@ -106,6 +106,29 @@ def lib():
         "mpm_get_scalars": (C.c_int, [vp, P(C.c_double)]),
         "mpm_nb_node": (C.c_int, [vp]),
         "mpm_sync": (C.c_int, [vp]),
+        # batched engine (include/mpm3d_engine.h)
+        "mpm_engine_create": (C.c_int, [C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, P(vp)]),
+        "mpm_engine_create_budget": (C.c_int, [C.c_double, C.c_int, C.c_int, P(vp)]),
+        "mpm_engine_destroy": (C.c_int, [vp]),
+        "mpm_engine_device_bytes": (C.c_long, [vp]),
+        "mpm_engine_add": (C.c_int, [vp, P(ProblemStruct), C.c_int, P(C.c_int)]),
+        "mpm_engine_remove": (C.c_int, [vp, C.c_int]),
+        "mpm_engine_capacity": (C.c_int, [vp, P(C.c_long), P(C.c_long), P(C.c_int)]),
+        "mpm_engine_step": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "mpm_engine_sync": (C.c_int, [vp]),
+        "mpm_engine_sort": (C.c_int, [vp]),
+        "mpm_engine_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "mpm_engine_set_interleave": (C.c_int, [vp, C.c_int]),
+        "mpm_engine_download_particles": (C.c_int, [vp, C.c_int, P(C.c_double)]),
+        "mpm_engine_download_grid": (C.c_int, [vp, C.c_int, P(C.c_double)]),
+        "mpm_engine_download_icell": (C.c_int, [vp, C.c_int, P(C.c_int32)]),
+        "mpm_engine_get_scalars": (C.c_int, [vp, C.c_int, P(C.c_double)]),
+        "mpm_engine_download_order": (C.c_int, [vp, C.c_int, P(C.c_int32)]),
+        "mpm_engine_status": (C.c_int, [vp, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "mpm_ctx_engine": (C.c_int, [vp, P(vp)]),
+        "mpm_engine_profile": (C.c_int, [vp, C.c_int]),
+        "mpm_engine_kernel_times": (C.c_int, [vp, P(C.c_double), P(C.c_long)]),
+        "mpm_engine_kernel_name": (C.c_char_p, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

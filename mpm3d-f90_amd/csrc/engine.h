@@ -69,14 +69,16 @@ enum TField {
   T_VX = 0, T_VY, T_VZ,
   T_BX, T_BY, T_BZ,      // G2P velocity (XX = Xp + b*DT applied by the stress pass)
   T_FX, T_FY, T_FZ,      // prescribed load FXp
-  T_MASS, T_VOL, T_SM, T_SEQV,
+  T_MASS, T_SM, T_SEQV,
   T_SD0, T_SD1, T_SD2, T_SD3, T_SD4, T_SD5,
-  T_SIGY, T_EPEFF, T_IE, T_Q, T_CP,
+  T_SIGY, T_EPEFF, T_Q, T_CP,
   T_COUNT
 };
 
-// particle fields always kept in FP64
-enum DField { D_X = 0, D_Y, D_Z, D_F0, D_F1, D_F2, D_F3, D_F4, D_F5, D_F6, D_F7, D_F8, D_COUNT };
+// particle fields always kept in FP64: position, volume and internal energy
+// (accumulated every step; the weakly compressible EOS takes a small
+// difference of densities) and F (det F, ln J of the neo-Hookean model)
+enum DField { D_X = 0, D_Y, D_Z, D_VOL, D_IE, D_F0, D_F1, D_F2, D_F3, D_F4, D_F5, D_F6, D_F7, D_F8, D_COUNT };
 
 // particle integer fields
 enum IField { I_BODY = 0, I_ORIG, I_CELL, I_COUNT };
@@ -92,6 +94,8 @@ struct PoolPtrs {
   T* pg;         // 3 * nn  momentum PXg
   T* fg;         // 3 * nn  force FXg
   T* p2;         // 3 * nn  MUSL re-mapped momentum
+  T* vg;         // 3 * nn  nodal velocity PXg/Mg (0 where Mg <= CutOff)
+  T* ag;         // 3 * nn  nodal acceleration FXg/Mg (0 where Mg <= CutOff)
   uint8_t* fix;  // nn, bit0 x, bit1 y, bit2 z
   long nn;
   const InstC* inst;
@@ -99,4 +103,5 @@ struct PoolPtrs {
   const MatC* mats;
   const BodyC* bodies;
   double* part;  // energy partials: 6 per particle block
+  int ilv;       // interleaved lane -> particle mapping in particle kernels
 };

@@ -1,0 +1,1385 @@
+// engine.hip -- host side of the MI355X engine: device pool, instance
+// tables, step sequences, graph capture, and the extern "C" entry points of
+// include/mpm3d_hip.h.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "kernels.h"
+#include "mpm3d_hip.h"
+#include "mpm3d_engine.h"
+
+// ----------------------------------------------------------- error state
+
+static thread_local std::string g_err;
+
+struct MpmFail {
+  int code;
+};
+
+[[noreturn]] static void fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  throw MpmFail{code};
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) fail(MPM_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));        \
+  } while (0)
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return MPM_OK;
+  } catch (const MpmFail& e) {
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host out of memory";
+    return MPM_ERR_NOMEM;
+  }
+}
+
+extern "C" const char* mpm_last_error(void) { return g_err.c_str(); }
+extern "C" const char* mpm_version(void) { return "mpm3d-mi355x 0.1 (gfx950)"; }
+
+// ------------------------------------------------------ range allocator
+
+struct RangeAlloc {  // first-fit over [0, cap) in units
+  long cap = 0;
+  std::map<long, long> free_;  // start -> length
+  void init(long c) {
+    cap = c;
+    free_.clear();
+    if (c > 0) free_[0] = c;
+  }
+  long alloc(long n) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        long s = it->first, len = it->second;
+        free_.erase(it);
+        if (len > n) free_[s + n] = len - n;
+        return s;
+      }
+    }
+    return -1;
+  }
+  void release(long s, long n) {
+    auto it = free_.emplace(s, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  long largest() const {
+    long m = 0;
+    for (auto& kv : free_) m = std::max(m, kv.second);
+    return m;
+  }
+};
+
+template <typename T>
+static T* dalloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) fail(MPM_ERR_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+  return (T*)p;
+}
+
+static long round_up(long a, long b) { return (a + b - 1) / b * b; }
+
+// ------------------------------------------------------------- engine
+
+struct HostInst {
+  bool live = false;
+  InstC c{};
+  long pg0 = 0, npg = 0;   // particle granules
+  long ng0 = 0, nng = 0;   // node granules
+  int mat0 = 0, nmat = 0, body0 = 0, nbody = 0;
+  int n = 0;
+  int group = -1;
+  bool p_in_p2 = false;    // latest PXg lives in the re-map buffer
+  bool xx_pending = false; // G2P done, Xp not yet moved (per-phase mode)
+};
+
+struct Group {
+  int shape = 0, scheme = 0;
+  std::vector<int> insts;
+  std::vector<int> pgran, ngran;
+  int* d_pgran = nullptr;
+  int* d_ngran = nullptr;
+  int* d_ilist = nullptr;
+  size_t cap_p = 0, cap_n = 0, cap_i = 0;
+};
+
+struct mpm_engine {
+  int prec = MPM_PREC_FP64;
+  size_t tsz = 8;
+  hipStream_t s = nullptr;
+  int device = 0;
+  long np = 0, nn = 0;            // capacities (slots)
+  int max_inst = 0, max_mat = 0, max_body = 0;
+  RangeAlloc pa, na, ma, ba;      // particle granules, node granules, materials, bodies
+  std::vector<HostInst> inst;
+  std::vector<int> free_ids;
+  // device buffers
+  void* t = nullptr;
+  double* d = nullptr;
+  int* ii = nullptr;
+  double* mg = nullptr;
+  void* pg = nullptr;
+  void* fg = nullptr;
+  void* p2 = nullptr;
+  void* vg = nullptr;
+  void* ag = nullptr;
+  uint8_t* fix = nullptr;
+  InstC* d_inst = nullptr;
+  InstT* d_ts = nullptr;
+  MatC* d_mats = nullptr;
+  BodyC* d_bodies = nullptr;
+  double* part = nullptr;
+  int* d_ginst = nullptr;         // instance of every particle granule
+  int* d_nginst = nullptr;        // instance of every node granule
+  std::vector<int> h_ginst, h_nginst;
+  // launch lists
+  std::vector<Group> groups;
+  bool lists_dirty = true;
+  int* d_ilist_all = nullptr;
+  size_t cap_ilist = 0;
+  int n_active = 0;
+  int bs_p = 256, bs_n = 256;     // block sizes (autotuned)
+  int ilv = 1;                    // interleaved lane mapping in particle kernels
+  int sort_every = 20;            // steps between re-sorts in mpm_step (0: never)
+  long steps_done = 0;
+  // sort scratch (lazily allocated)
+  unsigned* k_in = nullptr;
+  unsigned* k_out = nullptr;
+  int* v_in = nullptr;
+  int* v_out = nullptr;
+  void* cub_tmp = nullptr;
+  size_t cub_bytes = 0;
+  void* t_alt = nullptr;
+  double* d_alt = nullptr;
+  int* i_alt = nullptr;
+  // graph
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_steps = 0;
+  bool graph_dirty = true;
+  // profiling mode: HIP events around every launch of mpm_step / engine steps
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<int> ev_kid;
+  double kt_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  long kt_n[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+  template <typename T>
+  PoolPtrs<T> ptrs() const {
+    PoolPtrs<T> P;
+    P.t = (T*)t;
+    P.d = d;
+    P.i = ii;
+    P.np = np;
+    P.mg = mg;
+    P.pg = (T*)pg;
+    P.fg = (T*)fg;
+    P.p2 = (T*)p2;
+    P.vg = (T*)vg;
+    P.ag = (T*)ag;
+    P.fix = fix;
+    P.nn = nn;
+    P.inst = d_inst;
+    P.ts = d_ts;
+    P.mats = d_mats;
+    P.bodies = d_bodies;
+    P.part = part;
+    P.ilv = ilv;
+    return P;
+  }
+};
+
+static void free_engine(mpm_engine* e) {
+  if (!e) return;
+  for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  void* bufs[] = {e->t, e->d, e->ii, e->mg, e->pg, e->fg, e->p2, e->vg, e->ag, e->fix, e->d_inst, e->d_ts, e->d_mats,
+                  e->d_bodies, e->part, e->d_ginst, e->d_nginst, e->d_ilist_all, e->k_in, e->k_out, e->v_in,
+                  e->v_out, e->cub_tmp, e->t_alt, e->d_alt, e->i_alt};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto& g : e->groups) {
+    if (g.d_pgran) (void)hipFree(g.d_pgran);
+    if (g.d_ngran) (void)hipFree(g.d_ngran);
+    if (g.d_ilist) (void)hipFree(g.d_ilist);
+  }
+  if (e->s) (void)hipStreamDestroy(e->s);
+  delete e;
+}
+
+// bytes of device memory per particle slot / node for a precision
+static size_t bytes_per_particle(int prec) {
+  size_t tsz = prec == MPM_PREC_FP32 ? 4 : 8;
+  return T_COUNT * tsz + D_COUNT * 8 + I_COUNT * 4;
+}
+static size_t bytes_per_node(int prec) {
+  size_t tsz = prec == MPM_PREC_FP32 ? 4 : 8;
+  return 8 + 15 * tsz + 1;
+}
+
+static mpm_engine* new_engine(long np_slots, long nn_slots, int max_inst, int max_mat, int max_body, int prec) {
+  std::unique_ptr<mpm_engine, void (*)(mpm_engine*)> e(new mpm_engine, free_engine);
+  e->prec = prec;
+  e->tsz = prec == MPM_PREC_FP32 ? 4 : 8;
+  HIPCHK(hipGetDevice(&e->device));
+  HIPCHK(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking));
+  e->np = round_up(std::max(np_slots, (long)PGRAN), PGRAN);
+  e->nn = round_up(std::max(nn_slots, (long)NGRAN), NGRAN);
+  e->max_inst = max_inst;
+  e->max_mat = max_mat;
+  e->max_body = max_body;
+  e->pa.init(e->np / PGRAN);
+  e->na.init(e->nn / NGRAN);
+  e->ma.init(max_mat);
+  e->ba.init(max_body);
+  e->t = dalloc<char>((size_t)T_COUNT * e->np * e->tsz);
+  e->d = dalloc<double>((size_t)D_COUNT * e->np);
+  e->ii = dalloc<int>((size_t)I_COUNT * e->np);
+  e->mg = dalloc<double>(e->nn);
+  e->pg = dalloc<char>(3 * e->nn * e->tsz);
+  e->fg = dalloc<char>(3 * e->nn * e->tsz);
+  e->p2 = dalloc<char>(3 * e->nn * e->tsz);
+  e->vg = dalloc<char>(3 * e->nn * e->tsz);
+  e->ag = dalloc<char>(3 * e->nn * e->tsz);
+  e->fix = dalloc<uint8_t>(e->nn);
+  e->d_inst = dalloc<InstC>(max_inst);
+  e->d_ts = dalloc<InstT>(max_inst);
+  e->d_mats = dalloc<MatC>(max_mat);
+  e->d_bodies = dalloc<BodyC>(max_body);
+  e->part = dalloc<double>((size_t)(e->np / PGRAN) * (PGRAN / 64) * 6);
+  e->d_ginst = dalloc<int>(e->np / PGRAN);
+  e->d_nginst = dalloc<int>(e->nn / NGRAN);
+  e->h_ginst.assign(e->np / PGRAN, 0);
+  e->h_nginst.assign(e->nn / NGRAN, 0);
+  e->inst.resize(max_inst);
+  for (int k = max_inst - 1; k >= 0; k--) e->free_ids.push_back(k);
+  HIPCHK(hipMemsetAsync(e->t, 0, (size_t)T_COUNT * e->np * e->tsz, e->s));
+  HIPCHK(hipMemsetAsync(e->d, 0, (size_t)D_COUNT * e->np * 8, e->s));
+  HIPCHK(hipMemsetAsync(e->mg, 0, e->nn * 8, e->s));
+  HIPCHK(hipMemsetAsync(e->pg, 0, 3 * e->nn * e->tsz, e->s));
+  HIPCHK(hipMemsetAsync(e->fg, 0, 3 * e->nn * e->tsz, e->s));
+  HIPCHK(hipMemsetAsync(e->p2, 0, 3 * e->nn * e->tsz, e->s));
+  HIPCHK(hipMemsetAsync(e->vg, 0, 3 * e->nn * e->tsz, e->s));
+  HIPCHK(hipMemsetAsync(e->ag, 0, 3 * e->nn * e->tsz, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  return e.release();
+}
+
+// material constants exactly as the reference derives them per call
+static MatC make_mat(const mpm_material& m) {
+  MatC c{};
+  c.type = m.type;
+  c.eos = m.eos_type;
+  c.density = m.density;
+  const double DMG = 0.0;
+  double young = m.young * (1.0 - DMG + 0x1p-52);   // Constitution.f90:171
+  double poisson = m.poisson;
+  c.young_eff = young;
+  c.poisson = poisson;
+  c.G2 = young / (1 + poisson);
+  c.K3 = young / (1 - 2 * poisson);
+  c.lamda = (young * poisson) / ((1 + poisson) * (1 - 2 * poisson));
+  c.PlaMod = young * m.tangmod / (young - m.tangmod);
+  c.miu2 = 2 * m.miu;
+  c.yield0 = m.yield0;
+  c.Gmod = young / (2 * (1 + poisson));            // Constitution.f90:1232-1233
+  c.Kmod = young / (3 * (1 - 2 * poisson));
+  c.q_fai = m.q_fai;
+  c.k_fai = m.k_fai;
+  c.q_psi = m.q_psi;
+  if (m.q_fai == 0.0) {
+    c.tenf = 0.0;
+  } else {
+    double tmax = m.k_fai / m.q_fai;
+    c.tenf = m.ten_f < tmax ? m.ten_f : tmax;
+  }
+  c.c0 = m.ceos[0];
+  // SetDT / UpdateDT sound speed (DataIn.f90:1285-1290, update_step.f90:1861-1866)
+  double E = m.young, nu = m.poisson, ro = m.density;
+  double cp = std::sqrt(E * (1 - nu) / (1 + nu) / (1 - 2 * nu) / ro);
+  cp = std::fmax(cp, m.det_speed);
+  cp = std::fmax(cp, m.wavespd);
+  c.csound = cp;
+  return c;
+}
+
+static void check_problem(const mpm_problem* p) {
+  if (!p || p->nb_particle <= 0 || p->nb_mat <= 0 || p->nb_body <= 0 || !p->pstate || !p->pbody || !p->mats ||
+      !p->bodies)
+    fail(MPM_ERR_ARG, "incomplete problem description");
+  if (p->shape == MPM_SHAPE_SGMP)
+    fail(MPM_ERR_UNSUPPORTED, "SGMP (auxiliary-grid) shape functions are not available on the device path");
+  if (p->shape < 0 || p->shape > 2 || p->scheme < 0 || p->scheme > 2) fail(MPM_ERR_ARG, "bad shape/scheme");
+  for (int m = 0; m < p->nb_mat; m++) {
+    int t = p->mats[m].type;
+    if (t != MPM_MAT_ELASTIC && t != MPM_MAT_PLA1 && t != MPM_MAT_PLA2 && t != MPM_MAT_NULL && t != MPM_MAT_DP &&
+        t != MPM_MAT_NEOH)
+      fail(MPM_ERR_UNSUPPORTED, "material type %d is not on the device path", t);
+    if (t == MPM_MAT_NULL && p->mats[m].eos_type != 4) fail(MPM_ERR_UNSUPPORTED, "null material needs EOS 4");
+  }
+  for (int b = 0; b < p->nb_body; b++)
+    if (p->bodies[b].mat < 1 || p->bodies[b].mat > p->nb_mat) fail(MPM_ERR_ARG, "body %d has no material", b);
+  if (p->ncx <= 0 || p->ncy <= 0 || p->ncz <= 0 || !(p->dcell > 0)) fail(MPM_ERR_ARG, "bad grid");
+  long ncell = (long)p->ncx * p->ncy * p->ncz;
+  long nnode = (long)(p->ncx + 1) * (p->ncy + 1) * (p->ncz + 1);
+  if (nnode > 0x7fffffffL || ncell > 0x7fffffffL) fail(MPM_ERR_ARG, "grid too large");
+}
+
+template <typename T>
+static void upload_particles_T(mpm_engine* e, const HostInst& h, const double* ps, int n, bool first) {
+  const long np = e->np, pad = h.npg * PGRAN, b0 = h.pg0 * PGRAN;
+  std::vector<T> tb((size_t)T_COUNT * pad, (T)0);
+  std::vector<double> db((size_t)D_COUNT * pad, 0.0);
+  auto P = [&](int f, int i) { return ps[(size_t)f * n + i]; };
+  for (int i = 0; i < n; i++) {
+    tb[(size_t)T_VX * pad + i] = (T)P(PF_VX, i);
+    tb[(size_t)T_VY * pad + i] = (T)P(PF_VY, i);
+    tb[(size_t)T_VZ * pad + i] = (T)P(PF_VZ, i);
+    tb[(size_t)T_FX * pad + i] = (T)P(PF_FX, i);
+    tb[(size_t)T_FY * pad + i] = (T)P(PF_FY, i);
+    tb[(size_t)T_FZ * pad + i] = (T)P(PF_FZ, i);
+    tb[(size_t)T_MASS * pad + i] = (T)P(PF_MASS, i);
+    db[(size_t)D_VOL * pad + i] = P(PF_VOL, i);
+    tb[(size_t)T_SM * pad + i] = (T)P(PF_SM, i);
+    tb[(size_t)T_SEQV * pad + i] = (T)P(PF_SEQV, i);
+    tb[(size_t)T_SD0 * pad + i] = (T)P(PF_SDXX, i);
+    tb[(size_t)T_SD1 * pad + i] = (T)P(PF_SDYY, i);
+    tb[(size_t)T_SD2 * pad + i] = (T)P(PF_SDZZ, i);
+    tb[(size_t)T_SD3 * pad + i] = (T)P(PF_SDYZ, i);
+    tb[(size_t)T_SD4 * pad + i] = (T)P(PF_SDXZ, i);
+    tb[(size_t)T_SD5 * pad + i] = (T)P(PF_SDXY, i);
+    tb[(size_t)T_SIGY * pad + i] = (T)P(PF_SIGY, i);
+    tb[(size_t)T_EPEFF * pad + i] = (T)P(PF_EPEFF, i);
+    db[(size_t)D_IE * pad + i] = P(PF_IE, i);
+    tb[(size_t)T_Q * pad + i] = (T)P(PF_Q, i);
+    tb[(size_t)T_CP * pad + i] = (T)P(PF_CP, i);
+    db[(size_t)D_X * pad + i] = P(PF_X, i);
+    db[(size_t)D_Y * pad + i] = P(PF_Y, i);
+    db[(size_t)D_Z * pad + i] = P(PF_Z, i);
+    for (int k = 0; k < 9; k++) db[(size_t)(D_F0 + k) * pad + i] = P(PF_FP + k, i);
+  }
+  if (!first) {
+    // upload keeps the device permutation: scatter host rows to their slots
+    std::vector<int> orig(pad);
+    HIPCHK(hipMemcpy(orig.data(), e->ii + (size_t)I_ORIG * np + b0, pad * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<T> tb2(tb.size());
+    std::vector<double> db2(db.size());
+    for (long j = 0; j < pad; j++) {
+      int o = (j < n) ? orig[j] : (int)j;
+      for (int f = 0; f < T_COUNT; f++) tb2[(size_t)f * pad + j] = tb[(size_t)f * pad + o];
+      for (int f = 0; f < D_COUNT; f++) db2[(size_t)f * pad + j] = db[(size_t)f * pad + o];
+    }
+    tb.swap(tb2);
+    db.swap(db2);
+  }
+  for (int f = 0; f < T_COUNT; f++)
+    HIPCHK(hipMemcpy((char*)e->t + ((size_t)f * np + b0) * sizeof(T), &tb[(size_t)f * pad], pad * sizeof(T),
+                     hipMemcpyHostToDevice));
+  for (int f = 0; f < D_COUNT; f++)
+    HIPCHK(hipMemcpy(e->d + (size_t)f * np + b0, &db[(size_t)f * pad], pad * sizeof(double), hipMemcpyHostToDevice));
+}
+
+static void upload_particles(mpm_engine* e, const HostInst& h, const double* ps, int n, bool first) {
+  if (e->prec == MPM_PREC_FP32) upload_particles_T<float>(e, h, ps, n, first);
+  else upload_particles_T<double>(e, h, ps, n, first);
+}
+
+// add one instance to the pool; returns instance id
+static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
+  check_problem(p);
+  if (e->free_ids.empty()) fail(MPM_ERR_NOMEM, "no free instance slot");
+  const int n = p->nb_particle;
+  const long npg = (n + PGRAN - 1) / PGRAN;
+  const long nnode = (long)(p->ncx + 1) * (p->ncy + 1) * (p->ncz + 1);
+  const long nng = (nnode + NGRAN - 1) / NGRAN;
+  long pg0 = e->pa.alloc(npg);
+  if (pg0 < 0) fail(MPM_ERR_NOMEM, "particle pool full (%d particles requested)", n);
+  long ng0 = e->na.alloc(nng);
+  if (ng0 < 0) {
+    e->pa.release(pg0, npg);
+    fail(MPM_ERR_NOMEM, "node pool full (%ld nodes requested)", nnode);
+  }
+  long m0 = e->ma.alloc(p->nb_mat);
+  long b0 = m0 >= 0 ? e->ba.alloc(p->nb_body) : -1;
+  if (m0 < 0 || b0 < 0) {
+    if (m0 >= 0) e->ma.release(m0, p->nb_mat);
+    e->pa.release(pg0, npg);
+    e->na.release(ng0, nng);
+    fail(MPM_ERR_NOMEM, "material/body table full");
+  }
+  const int id = e->free_ids.back();
+  e->free_ids.pop_back();
+  HostInst& h = e->inst[id];
+  h = HostInst{};
+  h.live = true;
+  h.pg0 = pg0;
+  h.npg = npg;
+  h.ng0 = ng0;
+  h.nng = nng;
+  h.mat0 = (int)m0;
+  h.nmat = p->nb_mat;
+  h.body0 = (int)b0;
+  h.nbody = p->nb_body;
+  h.n = n;
+  InstC& c = h.c;
+  c.ncx = p->ncx; c.ncy = p->ncy; c.ncz = p->ncz;
+  c.ncxy = c.ncx * c.ncy;
+  c.ncell = c.ncxy * c.ncz;
+  c.ngx = c.ncx + 1; c.ngy = c.ncy + 1; c.ngz = c.ncz + 1;
+  c.ngxy = c.ngx * c.ngy;
+  c.nnode = (int)nnode;
+  c.node_off = (int)(ng0 * NGRAN);
+  c.p_begin = (int)(pg0 * PGRAN);
+  c.p_end = c.p_begin + n;
+  c.shape = p->shape;
+  c.scheme = p->scheme;
+  c.flags = (p->quasi_load ? IF_QUASI_LOAD : 0) | (p->quasi_damp ? IF_QUASI_DAMP : 0) |
+            (p->gravity ? IF_GRAVITY : 0) | (p->update_v_by_fp ? IF_UVFP : 0) | flags_extra;
+  bool loads = false;
+  for (int i = 0; i < n && !loads; i++)
+    for (int d = 0; d < 3; d++)
+      if (p->pstate[(size_t)(PF_FX + d) * n + i] != 0.0) loads = true;
+  if (loads) c.flags |= IF_LOADS;
+  c.mat_off = (int)m0;
+  c.nmat = p->nb_mat;
+  c.cell_off = 0;
+  for (int k = 0; k < 3; k++) {
+    c.span0[k] = p->span[2 * k];
+    c.span1[k] = p->span[2 * k + 1];
+  }
+  c.dcell = p->dcell;
+  c.idcell = 1.0 / p->dcell;      // Grid.f90:416
+  c.ijac = 2.0 / p->dcell;        // :414
+  c.ijac4 = c.ijac * 0.125;       // :415
+  c.cutoff = p->cutoff;
+  c.qlt = p->quasi_load_time;
+  c.damp = p->damp_factor;
+  c.bq1 = p->bq1;
+  c.bq2 = p->bq2;
+  c.dtscale = p->dtscale;
+  c.endtime = p->endtime;
+  HIPCHK(hipMemcpy(e->d_inst + id, &c, sizeof(InstC), hipMemcpyHostToDevice));
+  // materials and bodies
+  std::vector<MatC> mats(p->nb_mat);
+  for (int m = 0; m < p->nb_mat; m++) mats[m] = make_mat(p->mats[m]);
+  HIPCHK(hipMemcpy(e->d_mats + m0, mats.data(), sizeof(MatC) * p->nb_mat, hipMemcpyHostToDevice));
+  std::vector<BodyC> bodies(p->nb_body);
+  for (int b = 0; b < p->nb_body; b++) {
+    bodies[b].mat = (int)m0 + p->bodies[b].mat - 1;
+    for (int k = 0; k < 3; k++) bodies[b].grav[k] = p->bodies[b].grav[k];
+  }
+  HIPCHK(hipMemcpy(e->d_bodies + b0, bodies.data(), sizeof(BodyC) * p->nb_body, hipMemcpyHostToDevice));
+  // time state: calcEnergy of the initial state is done on the device below
+  InstT ts{};
+  ts.dt = p->dt;
+  HIPCHK(hipMemcpy(e->d_ts + id, &ts, sizeof(InstT), hipMemcpyHostToDevice));
+  // particle arrays
+  upload_particles(e, h, p->pstate, n, true);
+  {
+    const long pad = npg * PGRAN, b0p = pg0 * PGRAN;
+    std::vector<int> ib((size_t)I_COUNT * pad, 0);
+    for (long j = 0; j < pad; j++) {
+      ib[(size_t)I_BODY * pad + j] = (j < n) ? (int)b0 + p->pbody[j] : (int)b0;
+      ib[(size_t)I_ORIG * pad + j] = (int)j;
+      ib[(size_t)I_CELL * pad + j] = -1;
+    }
+    for (int f = 0; f < I_COUNT; f++)
+      HIPCHK(hipMemcpy(e->ii + (size_t)f * e->np + b0p, &ib[(size_t)f * pad], pad * sizeof(int),
+                       hipMemcpyHostToDevice));
+  }
+  // node fixities (Grid.f90:291-359)
+  {
+    std::vector<uint8_t> fx(nng * NGRAN, 0);
+    const int* FixS = p->fix;
+    const bool wide = p->shape == MPM_SHAPE_GIMP || p->shape == MPM_SHAPE_BSPLINE || p->shape == MPM_SHAPE_SGMP;
+    const int NGx = c.ngx, NGy = c.ngy, NGz = c.ngz;
+    for (int iz = 1; iz <= NGz; iz++)
+      for (int iy = 1; iy <= NGy; iy++)
+        for (int ix = 1; ix <= NGx; ix++) {
+          int k = (iz - 1) * c.ngxy + (iy - 1) * NGx + ix - 1;
+          int bx = 0, by = 0, bz = 0;
+          if (wide) {
+            if (((ix == 1 || ix == 2) && FixS[0] == 1) || ((ix == NGx || ix == NGx - 1) && FixS[1] == 1) ||
+                ((iy == 1 || iy == 2) && FixS[2] == 1) || ((iy == NGy || iy == NGy - 1) && FixS[3] == 1) ||
+                ((iz == 1 || iz == 2) && FixS[4] == 1) || ((iz == NGz || iz == NGz - 1) && FixS[5] == 1))
+              bx = by = bz = 1;
+            if (((ix == 1 || ix == 2) && FixS[0] == 2) || ((ix == NGx || ix == NGx - 1) && FixS[1] == 2)) bx = 1;
+            if (((iy == 1 || iy == 2) && FixS[2] == 2) || ((iy == NGy || iy == NGy - 1) && FixS[3] == 2)) by = 1;
+            if (((iz == 1 || iz == 2) && FixS[4] == 2) || ((iz == NGz || iz == NGz - 1) && FixS[5] == 2)) bz = 1;
+          }
+          if ((ix == 1 && FixS[0] == 1) || (ix == NGx && FixS[1] == 1) || (iy == 1 && FixS[2] == 1) ||
+              (iy == NGy && FixS[3] == 1) || (iz == 1 && FixS[4] == 1) || (iz == NGz && FixS[5] == 1))
+            bx = by = bz = 1;
+          if ((ix == 1 && FixS[0] == 2) || (ix == NGx && FixS[1] == 2)) bx = 1;
+          if ((iy == 1 && FixS[2] == 2) || (iy == NGy && FixS[3] == 2)) by = 1;
+          if ((iz == 1 && FixS[4] == 2) || (iz == NGz && FixS[5] == 2)) bz = 1;
+          fx[k] = (uint8_t)(bx | (by << 1) | (bz << 2));
+        }
+    HIPCHK(hipMemcpy(e->fix + ng0 * NGRAN, fx.data(), fx.size(), hipMemcpyHostToDevice));
+  }
+  for (long g = 0; g < npg; g++) e->h_ginst[pg0 + g] = id;
+  for (long g = 0; g < nng; g++) e->h_nginst[ng0 + g] = id;
+  HIPCHK(hipMemcpy(e->d_ginst + pg0, &e->h_ginst[pg0], npg * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->d_nginst + ng0, &e->h_nginst[ng0], nng * sizeof(int), hipMemcpyHostToDevice));
+  e->lists_dirty = true;
+  e->graph_dirty = true;
+  return id;
+}
+
+static void remove_instance(mpm_engine* e, int id) {
+  if (id < 0 || id >= e->max_inst || !e->inst[id].live) fail(MPM_ERR_ARG, "no live instance %d", id);
+  HIPCHK(hipStreamSynchronize(e->s));
+  HostInst& h = e->inst[id];
+  e->pa.release(h.pg0, h.npg);
+  e->na.release(h.ng0, h.nng);
+  e->ma.release(h.mat0, h.nmat);
+  e->ba.release(h.body0, h.nbody);
+  h.live = false;
+  e->free_ids.push_back(id);
+  e->lists_dirty = true;
+  e->graph_dirty = true;
+}
+
+template <typename V>
+static void upload_list(int*& dptr, size_t& cap, const V& v) {
+  if (v.size() > cap || !dptr) {
+    if (dptr) HIPCHK(hipFree(dptr));
+    cap = std::max<size_t>(v.size(), 16);
+    dptr = dalloc<int>(cap);
+  }
+  if (!v.empty()) HIPCHK(hipMemcpy(dptr, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+static void rebuild_lists(mpm_engine* e) {
+  if (!e->lists_dirty) return;
+  HIPCHK(hipStreamSynchronize(e->s));
+  std::vector<Group> old;
+  old.swap(e->groups);
+  std::map<std::pair<int, int>, int> gid;
+  std::vector<int> all;
+  for (int id = 0; id < e->max_inst; id++) {
+    const HostInst& h = e->inst[id];
+    if (!h.live) continue;
+    all.push_back(id);
+    auto key = std::make_pair(h.c.shape, h.c.scheme);
+    auto it = gid.find(key);
+    int g;
+    if (it == gid.end()) {
+      g = (int)e->groups.size();
+      gid[key] = g;
+      e->groups.emplace_back();
+      e->groups[g].shape = h.c.shape;
+      e->groups[g].scheme = h.c.scheme;
+      if (g < (int)old.size()) {   // reuse device lists
+        e->groups[g].d_pgran = old[g].d_pgran; e->groups[g].cap_p = old[g].cap_p; old[g].d_pgran = nullptr;
+        e->groups[g].d_ngran = old[g].d_ngran; e->groups[g].cap_n = old[g].cap_n; old[g].d_ngran = nullptr;
+        e->groups[g].d_ilist = old[g].d_ilist; e->groups[g].cap_i = old[g].cap_i; old[g].d_ilist = nullptr;
+      }
+    } else {
+      g = it->second;
+    }
+    Group& G = e->groups[g];
+    G.insts.push_back(id);
+    for (long k = 0; k < h.npg; k++) G.pgran.push_back((int)(h.pg0 + k));
+    for (long k = 0; k < h.nng; k++) G.ngran.push_back((int)(h.ng0 + k));
+  }
+  for (auto& g : old) {
+    if (g.d_pgran) (void)hipFree(g.d_pgran);
+    if (g.d_ngran) (void)hipFree(g.d_ngran);
+    if (g.d_ilist) (void)hipFree(g.d_ilist);
+  }
+  for (auto& G : e->groups) {
+    upload_list(G.d_pgran, G.cap_p, G.pgran);
+    upload_list(G.d_ngran, G.cap_n, G.ngran);
+    upload_list(G.d_ilist, G.cap_i, G.insts);
+  }
+  upload_list(e->d_ilist_all, e->cap_ilist, all);
+  e->n_active = (int)all.size();
+  e->lists_dirty = false;
+}
+
+static LaunchP plist(const mpm_engine* e, const Group& G) {
+  LaunchP L;
+  L.glist = G.d_pgran;
+  L.ginst = e->d_ginst;
+  L.bs = e->bs_p;
+  L.bpg = PGRAN / e->bs_p;
+  L.nblk = (int)G.pgran.size() * L.bpg;
+  return L;
+}
+
+static LaunchP nlist(const mpm_engine* e, const Group& G) {
+  LaunchP L;
+  L.glist = G.d_ngran;
+  L.ginst = e->d_nginst;
+  L.bs = e->bs_n;
+  L.bpg = NGRAN / e->bs_n;
+  L.nblk = (int)G.ngran.size() * L.bpg;
+  return L;
+}
+
+// --------------------------------------------------------- phases
+
+enum Phase {
+  PH_BEGIN, PH_GMI, PH_BC, PH_STRESS, PH_GMU, PH_INTEGRATE, PH_G2P, PH_MUSL, PH_ENERGY, PH_DT
+};
+
+template <typename T>
+static void run_phase(mpm_engine* e, int ph) {
+  rebuild_lists(e);
+  const PoolPtrs<T> P = e->ptrs<T>();
+  hipStream_t s = e->s;
+  if (ph == PH_BEGIN) {
+    mpmk::launch_begin_step(e->d_ts, e->d_ilist_all, e->n_active, s);
+    return;
+  }
+  for (Group& G : e->groups) {
+    const LaunchP L = plist(e, G), N = nlist(e, G);
+    switch (ph) {
+      case PH_GMI:
+        mpmk::launch_zero_nodes<double>(e->mg, 1, e->nn, N, s);
+        mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
+        mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, L, s);
+        for (int id : G.insts) e->inst[id].p_in_p2 = false;
+        break;
+      case PH_BC:
+        mpmk::launch_bc<T>(P, N, 0, 1, 0, s);
+        break;
+      case PH_STRESS:
+        mpmk::launch_bc<T>(P, N, 0, 0, 1, s);   // nodal velocity of the current PXg
+        mpmk::launch_stress<T>(G.shape, G.scheme == MPM_USF ? 0 : ST_MOVE, P, L, s);
+        for (int id : G.insts) e->inst[id].xx_pending = false;
+        break;
+      case PH_GMU:
+        mpmk::launch_zero_nodes<T>((T*)e->fg, 3, e->nn, N, s);
+        mpmk::launch_p2g<T>(G.shape, P2G_FORCE, P, L, s);
+        break;
+      case PH_INTEGRATE:
+        mpmk::launch_integrate<T>(P, N, 0, s);
+        break;
+      case PH_G2P:
+        mpmk::launch_node_va<T>(P, N, s);
+        mpmk::launch_g2p<T>(G.shape, G.scheme == MPM_USF ? G2P_MOVE : 0, P, L, s);
+        for (int id : G.insts) e->inst[id].xx_pending = G.scheme != MPM_USF;
+        break;
+      case PH_MUSL:
+        mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
+        mpmk::launch_p2g<T>(G.shape, P2G_MOM, P, L, s);
+        mpmk::launch_bc<T>(P, N, 0, 1, 0, s);
+        break;
+      case PH_ENERGY:
+        mpmk::launch_energy<T>(P, L, s);
+        mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), L.bpg, FIN_ENERGY, s);
+        break;
+      case PH_DT:
+        mpmk::launch_energy<T>(P, L, s);
+        mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), L.bpg, FIN_DT, s);
+        break;
+    }
+  }
+  HIPCHK(hipGetLastError());
+}
+
+static void phase(mpm_engine* e, int ph) {
+  if (e->prec == MPM_PREC_FP32) run_phase<float>(e, ph);
+  else run_phase<double>(e, ph);
+}
+
+// kernel classes timed by the profiling mode
+enum KernelId { KI_ZERO = 0, KI_P2G, KI_BC, KI_STRESS, KI_INTEGRATE, KI_G2P, KI_FINALIZE, KI_BEGIN, KI_COUNT };
+static const char* KI_NAMES[KI_COUNT] = {"zero", "p2g", "bc", "stress", "integrate", "g2p", "finalize", "begin"};
+
+// HIP-event bracket around one launch when the engine is profiling
+template <typename F>
+static void timed(mpm_engine* e, int kid, F&& launch) {
+  if (!e->prof) {
+    launch();
+    return;
+  }
+  if (e->ev_used + 2 > e->ev_pool.size()) {
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    e->ev_pool.push_back(a);
+    e->ev_pool.push_back(b);
+  }
+  hipEvent_t a = e->ev_pool[e->ev_used], b = e->ev_pool[e->ev_used + 1];
+  e->ev_used += 2;
+  HIPCHK(hipEventRecord(a, e->s));
+  launch();
+  HIPCHK(hipEventRecord(b, e->s));
+  e->ev_kid.push_back(kid);
+}
+
+// one fused step of every active instance (MPM3D.f90:69-113)
+template <typename T>
+static void enqueue_step(mpm_engine* e, int fin_flags) {
+  const PoolPtrs<T> P = e->ptrs<T>();
+  hipStream_t s = e->s;
+  timed(e, KI_BEGIN, [&] { mpmk::launch_begin_step(e->d_ts, e->d_ilist_all, e->n_active, s); });
+  for (Group& G : e->groups) {
+    const LaunchP L = plist(e, G), N = nlist(e, G);
+    timed(e, KI_ZERO, [&] {
+      mpmk::launch_zero_nodes<double>(e->mg, 1, e->nn, N, s);
+      mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
+      mpmk::launch_zero_nodes<T>((T*)e->fg, 3, e->nn, N, s);
+      if (G.scheme == MPM_MUSL) mpmk::launch_zero_nodes<T>((T*)e->p2, 3, e->nn, N, s);
+    });
+    if (G.scheme == MPM_USF) {
+      timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, L, s); });
+      timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 0, 1, 1, s); });
+      timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_FORCE, P, L, s); });
+      timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
+      timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, G2P_MOVE | G2P_ENERGY, P, L, s); });
+    } else if (G.scheme == MPM_USL) {
+      timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE, P, L, s); });
+      timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
+      timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, 0, P, L, s); });
+      timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_MOVE | ST_ENERGY, P, L, s); });
+    } else {
+      timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE, P, L, s); });
+      timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
+      timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, G2P_REMAP, P, L, s); });
+      timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 1, 1, 1, s); });
+      timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_MOVE | ST_ENERGY, P, L, s); });
+    }
+    timed(e, KI_FINALIZE, [&] {
+      mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), L.bpg, FIN_ENERGY | FIN_DT | fin_flags, s);
+    });
+  }
+}
+
+// fold the recorded event pairs into per-kernel totals (synchronising)
+static void collect_times(mpm_engine* e) {
+  if (e->ev_kid.empty()) return;
+  HIPCHK(hipStreamSynchronize(e->s));
+  for (size_t k = 0; k < e->ev_kid.size(); k++) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev_pool[2 * k], e->ev_pool[2 * k + 1]));
+    e->kt_ms[e->ev_kid[k]] += ms;
+    e->kt_n[e->ev_kid[k]] += 1;
+  }
+  e->ev_kid.clear();
+  e->ev_used = 0;
+}
+
+static void mark_after_step(mpm_engine* e) {
+  for (auto& G : e->groups)
+    for (int id : G.insts) {
+      e->inst[id].p_in_p2 = G.scheme == MPM_MUSL;
+      e->inst[id].xx_pending = false;
+    }
+}
+
+template <typename T>
+static void permute_pool(mpm_engine* e);
+
+static void sort_particles(mpm_engine* e);
+
+static void step_plain(mpm_engine* e, int nsteps, int fin_flags) {
+  rebuild_lists(e);
+  for (int k = 0; k < nsteps; k++) {
+    if (e->sort_every > 0 && e->steps_done % e->sort_every == 0) sort_particles(e);
+    if (e->prec == MPM_PREC_FP32) enqueue_step<float>(e, fin_flags);
+    else enqueue_step<double>(e, fin_flags);
+    e->steps_done++;
+  }
+  HIPCHK(hipGetLastError());
+  mark_after_step(e);
+}
+
+// graph of `per` fused steps, replayed; instances finish on their own
+static void step_graph(mpm_engine* e, int nsteps, int per, int fin_flags) {
+  rebuild_lists(e);
+  if (per < 1) per = 1;
+  if (e->graph_dirty || e->graph_steps != per) {
+    if (e->gexec) HIPCHK(hipGraphExecDestroy(e->gexec));
+    if (e->graph) HIPCHK(hipGraphDestroy(e->graph));
+    e->gexec = nullptr;
+    e->graph = nullptr;
+    const bool was_prof = e->prof;
+    e->prof = false;
+    HIPCHK(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < per; k++) {
+      if (e->prec == MPM_PREC_FP32) enqueue_step<float>(e, fin_flags);
+      else enqueue_step<double>(e, fin_flags);
+    }
+    HIPCHK(hipStreamEndCapture(e->s, &e->graph));
+    e->prof = was_prof;
+    HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+    e->graph_steps = per;
+    e->graph_dirty = false;
+  }
+  int done = 0;
+  while (done + per <= nsteps) {
+    const long se = e->sort_every;
+    if (se > 0 && (e->steps_done % se == 0 || e->steps_done / se != (e->steps_done + per - 1) / se))
+      sort_particles(e);
+    HIPCHK(hipGraphLaunch(e->gexec, e->s));
+    done += per;
+    e->steps_done += per;
+  }
+  if (done < nsteps) step_plain(e, nsteps - done, fin_flags);
+  mark_after_step(e);
+}
+
+// ---------------------------------------------------------- sorting
+
+// Stable sort of every instance's particles by cell (segmented LSD radix
+// sort: equal keys keep their order, so the permutation is the one a
+// stable CPU sort of the same icell array gives), then gather all fields.
+static void sort_particles(mpm_engine* e) {
+  rebuild_lists(e);
+  const long np = e->np;
+  if (!e->k_in) {
+    e->k_in = dalloc<unsigned>(np);
+    e->k_out = dalloc<unsigned>(np);
+    e->v_in = dalloc<int>(np);
+    e->v_out = dalloc<int>(np);
+    e->t_alt = dalloc<char>((size_t)T_COUNT * np * e->tsz);
+    e->d_alt = dalloc<double>((size_t)D_COUNT * np);
+    e->i_alt = dalloc<int>((size_t)I_COUNT * np);
+  }
+  std::vector<int> seg_b, seg_e;
+  unsigned maxkey = 1;
+  for (auto& G : e->groups) {
+    const LaunchP L = plist(e, G);
+    mpmk::launch_sort_keys(e->d, np, e->d_inst, L, e->k_in, e->v_in, e->s);
+    for (int id : G.insts) {
+      const HostInst& h = e->inst[id];
+      seg_b.push_back((int)(h.pg0 * PGRAN));
+      seg_e.push_back((int)((h.pg0 + h.npg) * PGRAN));
+      const unsigned B = SORT_BRICK, nb = ((h.c.ncx + B - 1) / B) * ((h.c.ncy + B - 1) / B) * ((h.c.ncz + B - 1) / B);
+      maxkey = std::max<unsigned>(maxkey, nb * B * B * B);
+    }
+  }
+  if (seg_b.empty()) return;
+  int end_bit = 1;
+  while (end_bit < 32 && (1u << end_bit) <= maxkey) end_bit++;
+  int *d_b = dalloc<int>(seg_b.size()), *d_e = dalloc<int>(seg_e.size());
+  HIPCHK(hipMemcpyAsync(d_b, seg_b.data(), seg_b.size() * sizeof(int), hipMemcpyHostToDevice, e->s));
+  HIPCHK(hipMemcpyAsync(d_e, seg_e.data(), seg_e.size() * sizeof(int), hipMemcpyHostToDevice, e->s));
+  size_t need = 0;
+  const bool one = seg_b.size() == 1;
+  // one instance: plain radix sort of its slot range; several: segmented
+  auto run_sort = [&](void* tmp, size_t& bytes) {
+    if (one)
+      return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, e->k_in + seg_b[0], e->k_out + seg_b[0],
+                                                e->v_in + seg_b[0], e->v_out + seg_b[0], seg_e[0] - seg_b[0], 0,
+                                                end_bit, e->s);
+    return hipcub::DeviceSegmentedRadixSort::SortPairs(tmp, bytes, e->k_in, e->k_out, e->v_in, e->v_out, (int)np,
+                                                       (int)seg_b.size(), d_b, d_e, 0, end_bit, e->s);
+  };
+  HIPCHK(run_sort(nullptr, need));
+  if (need > e->cub_bytes) {
+    if (e->cub_tmp) HIPCHK(hipFree(e->cub_tmp));
+    e->cub_tmp = dalloc<char>(need);
+    e->cub_bytes = need;
+  }
+  HIPCHK(run_sort(e->cub_tmp, need));
+  for (auto& G : e->groups) {
+    const LaunchP L = plist(e, G);
+    if (e->prec == MPM_PREC_FP32)
+      mpmk::launch_permute<float>((float*)e->t_alt, (const float*)e->t, e->v_out, L, np, T_COUNT, e->s);
+    else
+      mpmk::launch_permute<double>((double*)e->t_alt, (const double*)e->t, e->v_out, L, np, T_COUNT, e->s);
+    mpmk::launch_permute<double>(e->d_alt, e->d, e->v_out, L, np, D_COUNT, e->s);
+    mpmk::launch_permute<int>(e->i_alt, e->ii, e->v_out, L, np, I_COUNT, e->s);
+  }
+  // copy back so the particle arrays keep their addresses (captured graphs stay valid)
+  HIPCHK(hipMemcpyAsync(e->t, e->t_alt, (size_t)T_COUNT * np * e->tsz, hipMemcpyDeviceToDevice, e->s));
+  HIPCHK(hipMemcpyAsync(e->d, e->d_alt, (size_t)D_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
+  HIPCHK(hipMemcpyAsync(e->ii, e->i_alt, (size_t)I_COUNT * np * sizeof(int), hipMemcpyDeviceToDevice, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  HIPCHK(hipFree(d_b));
+  HIPCHK(hipFree(d_e));
+}
+
+// --------------------------------------------------------- downloads
+
+template <typename T>
+static void download_particles_T(mpm_engine* e, int id, double* out) {
+  HostInst& h = e->inst[id];
+  const long np = e->np, pad = h.npg * PGRAN, b0 = h.pg0 * PGRAN;
+  const int n = h.n;
+  std::vector<T> tb((size_t)T_COUNT * pad);
+  std::vector<double> db((size_t)D_COUNT * pad);
+  std::vector<int> ib((size_t)I_COUNT * pad);
+  HIPCHK(hipStreamSynchronize(e->s));
+  for (int f = 0; f < T_COUNT; f++)
+    HIPCHK(hipMemcpy(&tb[(size_t)f * pad], (char*)e->t + ((size_t)f * np + b0) * sizeof(T), pad * sizeof(T),
+                     hipMemcpyDeviceToHost));
+  for (int f = 0; f < D_COUNT; f++)
+    HIPCHK(hipMemcpy(&db[(size_t)f * pad], e->d + (size_t)f * np + b0, pad * sizeof(double), hipMemcpyDeviceToHost));
+  for (int f = 0; f < I_COUNT; f++)
+    HIPCHK(hipMemcpy(&ib[(size_t)f * pad], e->ii + (size_t)f * np + b0, pad * sizeof(int), hipMemcpyDeviceToHost));
+  InstT ts;
+  HIPCHK(hipMemcpy(&ts, e->d_ts + id, sizeof ts, hipMemcpyDeviceToHost));
+  for (long j = 0; j < n; j++) {
+    const int o = ib[(size_t)I_ORIG * pad + j];
+    auto O = [&](int f) -> double& { return out[(size_t)f * n + o]; };
+    auto Tv = [&](int f) { return (double)tb[(size_t)f * pad + j]; };
+    O(PF_X) = db[(size_t)D_X * pad + j];
+    O(PF_Y) = db[(size_t)D_Y * pad + j];
+    O(PF_Z) = db[(size_t)D_Z * pad + j];
+    O(PF_VX) = Tv(T_VX); O(PF_VY) = Tv(T_VY); O(PF_VZ) = Tv(T_VZ);
+    O(PF_FX) = Tv(T_FX); O(PF_FY) = Tv(T_FY); O(PF_FZ) = Tv(T_FZ);
+    O(PF_MASS) = Tv(T_MASS); O(PF_SM) = Tv(T_SM); O(PF_SEQV) = Tv(T_SEQV);
+    O(PF_SDXX) = Tv(T_SD0); O(PF_SDYY) = Tv(T_SD1); O(PF_SDZZ) = Tv(T_SD2);
+    O(PF_SDYZ) = Tv(T_SD3); O(PF_SDXZ) = Tv(T_SD4); O(PF_SDXY) = Tv(T_SD5);
+    O(PF_SIGY) = Tv(T_SIGY); O(PF_EPEFF) = Tv(T_EPEFF); O(PF_Q) = Tv(T_Q); O(PF_CP) = Tv(T_CP);
+    for (int k = 0; k < 9; k++) O(PF_FP + k) = db[(size_t)(D_F0 + k) * pad + j];
+    O(PF_VOL) = db[(size_t)D_VOL * pad + j];
+    O(PF_IE) = db[(size_t)D_IE * pad + j];
+    for (int k = 0; k < 3; k++) {
+      const double x = db[(size_t)(D_X + k) * pad + j];
+      O(PF_XX + k) = h.xx_pending ? x + Tv(T_BX + k) * ts.dt : x;
+    }
+  }
+}
+
+static void download_particles(mpm_engine* e, int id, double* out) {
+  if (e->prec == MPM_PREC_FP32) download_particles_T<float>(e, id, out);
+  else download_particles_T<double>(e, id, out);
+}
+
+template <typename T>
+static void download_grid_T(mpm_engine* e, int id, double* out) {
+  HostInst& h = e->inst[id];
+  const long nn = h.c.nnode, n0 = h.ng0 * NGRAN;
+  HIPCHK(hipStreamSynchronize(e->s));
+  std::vector<double> m(nn);
+  std::vector<T> v(nn);
+  HIPCHK(hipMemcpy(m.data(), e->mg + n0, nn * sizeof(double), hipMemcpyDeviceToHost));
+  for (long k = 0; k < nn; k++) out[k] = m[k];
+  const T* psrc = (const T*)(h.p_in_p2 ? e->p2 : e->pg);
+  for (int c = 0; c < 3; c++) {
+    HIPCHK(hipMemcpy(v.data(), psrc + (size_t)c * e->nn + n0, nn * sizeof(T), hipMemcpyDeviceToHost));
+    for (long k = 0; k < nn; k++) out[(size_t)(GF_PX + c) * nn + k] = (double)v[k];
+    HIPCHK(hipMemcpy(v.data(), (const T*)e->fg + (size_t)c * e->nn + n0, nn * sizeof(T), hipMemcpyDeviceToHost));
+    for (long k = 0; k < nn; k++) out[(size_t)(GF_FX + c) * nn + k] = (double)v[k];
+  }
+}
+
+static void download_grid(mpm_engine* e, int id, double* out) {
+  if (e->prec == MPM_PREC_FP32) download_grid_T<float>(e, id, out);
+  else download_grid_T<double>(e, id, out);
+}
+
+static void download_icell(mpm_engine* e, int id, int32_t* out) {
+  HostInst& h = e->inst[id];
+  const long pad = h.npg * PGRAN, b0 = h.pg0 * PGRAN;
+  std::vector<int> c(pad), o(pad);
+  HIPCHK(hipStreamSynchronize(e->s));
+  HIPCHK(hipMemcpy(c.data(), e->ii + (size_t)I_CELL * e->np + b0, pad * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(o.data(), e->ii + (size_t)I_ORIG * e->np + b0, pad * sizeof(int), hipMemcpyDeviceToHost));
+  for (int j = 0; j < h.n; j++) out[o[j]] = c[j];
+}
+
+static void get_scalars(mpm_engine* e, int id, double* out) {
+  InstT ts;
+  HIPCHK(hipStreamSynchronize(e->s));
+  HIPCHK(hipMemcpy(&ts, e->d_ts + id, sizeof ts, hipMemcpyDeviceToHost));
+  out[0] = ts.istep;
+  out[1] = ts.time;
+  out[2] = ts.dt;
+  out[3] = ts.ek;
+  out[4] = ts.ei;
+  out[5] = ts.mom[0];
+  out[6] = ts.mom[1];
+  out[7] = ts.mom[2];
+}
+
+static mpm_engine* check_ctx(mpm_engine* e) {
+  if (!e) fail(MPM_ERR_ARG, "null context");
+  return e;
+}
+
+static int live_id(mpm_engine* e, int id) {
+  if (id < 0 || id >= e->max_inst || !e->inst[id].live) fail(MPM_ERR_ARG, "no live instance %d", id);
+  return id;
+}
+
+// calcEnergy of one freshly added instance (MPM3D.f90:42)
+static void initial_energy(mpm_engine* e, int id) {
+  const HostInst& h = e->inst[id];
+  std::vector<int> gl;
+  for (long g = 0; g < h.npg; g++) gl.push_back((int)(h.pg0 + g));
+  int* d_gl = dalloc<int>(gl.size());
+  int* d_il = dalloc<int>(1);
+  HIPCHK(hipMemcpyAsync(d_gl, gl.data(), gl.size() * sizeof(int), hipMemcpyHostToDevice, e->s));
+  HIPCHK(hipMemcpyAsync(d_il, &id, sizeof(int), hipMemcpyHostToDevice, e->s));
+  LaunchP L;
+  L.glist = d_gl;
+  L.ginst = e->d_ginst;
+  L.bs = e->bs_p;
+  L.bpg = PGRAN / e->bs_p;
+  L.nblk = (int)gl.size() * L.bpg;
+  if (e->prec == MPM_PREC_FP32) {
+    mpmk::launch_energy<float>(e->ptrs<float>(), L, e->s);
+    mpmk::launch_finalize<float>(e->ptrs<float>(), d_il, 1, L.bpg, FIN_ENERGY, e->s);
+  } else {
+    mpmk::launch_energy<double>(e->ptrs<double>(), L, e->s);
+    mpmk::launch_finalize<double>(e->ptrs<double>(), d_il, 1, L.bpg, FIN_ENERGY, e->s);
+  }
+  HIPCHK(hipStreamSynchronize(e->s));
+  HIPCHK(hipFree(d_gl));
+  HIPCHK(hipFree(d_il));
+}
+
+// ================================================================ C-ABI
+
+struct mpm_ctx {
+  mpm_engine* e;
+};
+
+extern "C" int mpm_ctx_create(const mpm_problem* prob, int precision, mpm_ctx** out) {
+  return guarded([&] {
+    if (!out) fail(MPM_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (precision != MPM_PREC_FP64 && precision != MPM_PREC_FP32) fail(MPM_ERR_ARG, "bad precision");
+    check_problem(prob);
+    const long nnode = (long)(prob->ncx + 1) * (prob->ncy + 1) * (prob->ncz + 1);
+    mpm_engine* e = new_engine(prob->nb_particle, nnode, 1, prob->nb_mat, prob->nb_body, precision);
+    try {
+      add_instance(e, prob, 0);
+      rebuild_lists(e);
+      initial_energy(e, 0);   // MPM3D.f90:42 calcEnergy after the input phase
+    } catch (...) {
+      free_engine(e);
+      throw;
+    }
+    *out = new mpm_ctx{e};
+  });
+}
+
+extern "C" int mpm_ctx_destroy(mpm_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    if (ctx->e) (void)hipStreamSynchronize(ctx->e->s);
+    free_engine(ctx->e);
+    delete ctx;
+  });
+}
+
+#define CTX_PHASE(fname, ph)                      \
+  extern "C" int fname(mpm_ctx* ctx) {            \
+    return guarded([&] {                          \
+      if (!ctx) fail(MPM_ERR_ARG, "null context"); \
+      phase(ctx->e, ph);                          \
+    });                                           \
+  }
+CTX_PHASE(mpm_begin_step, PH_BEGIN)
+CTX_PHASE(mpm_grid_momentum_initial, PH_GMI)
+CTX_PHASE(mpm_apply_boundary_conditions, PH_BC)
+CTX_PHASE(mpm_particle_stress_update, PH_STRESS)
+CTX_PHASE(mpm_grid_momentum_update, PH_GMU)
+CTX_PHASE(mpm_integrate_momentum, PH_INTEGRATE)
+CTX_PHASE(mpm_particle_position_update, PH_G2P)
+CTX_PHASE(mpm_grid_momentum_musl, PH_MUSL)
+#undef CTX_PHASE
+
+extern "C" int mpm_calc_energy(mpm_ctx* ctx, double* ek, double* ei, double* mom3) {
+  return guarded([&] {
+    if (!ctx) fail(MPM_ERR_ARG, "null context");
+    phase(ctx->e, PH_ENERGY);
+    double s[8];
+    get_scalars(ctx->e, 0, s);
+    if (ek) *ek = s[3];
+    if (ei) *ei = s[4];
+    if (mom3) { mom3[0] = s[5]; mom3[1] = s[6]; mom3[2] = s[7]; }
+  });
+}
+
+extern "C" int mpm_update_dt(mpm_ctx* ctx, double* dt) {
+  return guarded([&] {
+    if (!ctx) fail(MPM_ERR_ARG, "null context");
+    phase(ctx->e, PH_DT);
+    if (dt) {
+      double s[8];
+      get_scalars(ctx->e, 0, s);
+      *dt = s[2];
+    }
+  });
+}
+
+extern "C" int mpm_step(mpm_ctx* ctx, int nsteps) {
+  return guarded([&] {
+    if (!ctx || nsteps < 0) fail(MPM_ERR_ARG, "bad arguments");
+    step_plain(ctx->e, nsteps, 0);
+  });
+}
+
+extern "C" int mpm_download_particles(mpm_ctx* ctx, double* ps) {
+  return guarded([&] {
+    if (!ctx || !ps) fail(MPM_ERR_ARG, "bad arguments");
+    download_particles(ctx->e, 0, ps);
+  });
+}
+
+extern "C" int mpm_upload_particles(mpm_ctx* ctx, const double* ps) {
+  return guarded([&] {
+    if (!ctx || !ps) fail(MPM_ERR_ARG, "bad arguments");
+    HIPCHK(hipStreamSynchronize(ctx->e->s));
+    upload_particles(ctx->e, ctx->e->inst[0], ps, ctx->e->inst[0].n, false);
+  });
+}
+
+extern "C" int mpm_download_grid(mpm_ctx* ctx, double* g) {
+  return guarded([&] {
+    if (!ctx || !g) fail(MPM_ERR_ARG, "bad arguments");
+    download_grid(ctx->e, 0, g);
+  });
+}
+
+extern "C" int mpm_download_icell(mpm_ctx* ctx, int32_t* c) {
+  return guarded([&] {
+    if (!ctx || !c) fail(MPM_ERR_ARG, "bad arguments");
+    download_icell(ctx->e, 0, c);
+  });
+}
+
+extern "C" int mpm_get_scalars(mpm_ctx* ctx, double* out8) {
+  return guarded([&] {
+    if (!ctx || !out8) fail(MPM_ERR_ARG, "bad arguments");
+    get_scalars(ctx->e, 0, out8);
+  });
+}
+
+extern "C" int mpm_nb_node(const mpm_ctx* ctx) { return ctx ? ctx->e->inst[0].c.nnode : -1; }
+
+extern "C" int mpm_sync(mpm_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) fail(MPM_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(ctx->e->s));
+  });
+}
+
+// ---------------------------------------------------- engine (batched) API
+
+extern "C" int mpm_engine_create(long max_particles, long max_nodes, int max_instances, int max_materials,
+                                 int max_bodies, int precision, mpm_engine** out) {
+  return guarded([&] {
+    if (!out || max_particles <= 0 || max_nodes <= 0 || max_instances <= 0 || max_materials <= 0 || max_bodies <= 0)
+      fail(MPM_ERR_ARG, "bad arguments");
+    if (precision != MPM_PREC_FP64 && precision != MPM_PREC_FP32) fail(MPM_ERR_ARG, "bad precision");
+    *out = new_engine(max_particles, max_nodes, max_instances, max_materials, max_bodies, precision);
+  });
+}
+
+extern "C" int mpm_engine_create_budget(double hbm_bytes, int max_instances, int precision, mpm_engine** out) {
+  return guarded([&] {
+    if (!out || !(hbm_bytes > 0) || max_instances <= 0) fail(MPM_ERR_ARG, "bad arguments");
+    // split the budget: particles with sort double-buffers, nodes ~ particles/2
+    const double bp = 2.0 * bytes_per_particle(precision) + 16, bn = bytes_per_node(precision);
+    const long np = (long)(hbm_bytes / (bp + 0.5 * bn));
+    const long nn = np / 2;
+    *out = new_engine(np, nn, max_instances, 16 * max_instances, 16 * max_instances, precision);
+  });
+}
+
+extern "C" int mpm_engine_destroy(mpm_engine* e) {
+  return guarded([&] {
+    if (e) (void)hipStreamSynchronize(e->s);
+    free_engine(e);
+  });
+}
+
+extern "C" int mpm_engine_add(mpm_engine* e, const mpm_problem* p, int stop_at_endtime, int* id) {
+  return guarded([&] {
+    check_ctx(e);
+    if (!id) fail(MPM_ERR_ARG, "id is NULL");
+    *id = add_instance(e, p, stop_at_endtime ? IF_ENDTIME : 0);
+    e->lists_dirty = true;
+    rebuild_lists(e);
+    initial_energy(e, *id);   // MPM3D.f90:42
+  });
+}
+
+extern "C" int mpm_engine_remove(mpm_engine* e, int id) {
+  return guarded([&] {
+    check_ctx(e);
+    remove_instance(e, id);
+  });
+}
+
+extern "C" int mpm_engine_step(mpm_engine* e, int nsteps, int use_graph, int graph_steps) {
+  return guarded([&] {
+    check_ctx(e);
+    if (nsteps < 0) fail(MPM_ERR_ARG, "nsteps < 0");
+    if (use_graph) step_graph(e, nsteps, graph_steps, FIN_DONE);
+    else step_plain(e, nsteps, FIN_DONE);
+  });
+}
+
+extern "C" int mpm_engine_sync(mpm_engine* e) {
+  return guarded([&] {
+    check_ctx(e);
+    HIPCHK(hipStreamSynchronize(e->s));
+  });
+}
+
+extern "C" int mpm_engine_sort(mpm_engine* e) {
+  return guarded([&] {
+    check_ctx(e);
+    sort_particles(e);
+  });
+}
+
+extern "C" int mpm_engine_set_interleave(mpm_engine* e, int enable) {
+  return guarded([&] {
+    check_ctx(e);
+    e->ilv = enable != 0;
+    e->graph_dirty = true;
+  });
+}
+
+extern "C" int mpm_engine_set_tuning(mpm_engine* e, int block_particles, int block_nodes, int sort_every) {
+  return guarded([&] {
+    check_ctx(e);
+    auto ok = [](int b) { return b == 64 || b == 128 || b == 256; };
+    if (!ok(block_particles) || !ok(block_nodes) || sort_every < 0) fail(MPM_ERR_ARG, "bad tuning values");
+    e->bs_p = block_particles;
+    e->bs_n = block_nodes;
+    e->sort_every = sort_every;
+    e->graph_dirty = true;
+  });
+}
+
+extern "C" int mpm_engine_download_particles(mpm_engine* e, int id, double* ps) {
+  return guarded([&] {
+    check_ctx(e);
+    download_particles(e, live_id(e, id), ps);
+  });
+}
+
+extern "C" int mpm_engine_download_grid(mpm_engine* e, int id, double* g) {
+  return guarded([&] {
+    check_ctx(e);
+    download_grid(e, live_id(e, id), g);
+  });
+}
+
+extern "C" int mpm_engine_download_icell(mpm_engine* e, int id, int32_t* c) {
+  return guarded([&] {
+    check_ctx(e);
+    download_icell(e, live_id(e, id), c);
+  });
+}
+
+extern "C" int mpm_engine_profile(mpm_engine* e, int enable) {
+  return guarded([&] {
+    check_ctx(e);
+    collect_times(e);
+    e->prof = enable != 0;
+    for (int k = 0; k < KI_COUNT; k++) {
+      e->kt_ms[k] = 0;
+      e->kt_n[k] = 0;
+    }
+  });
+}
+
+extern "C" int mpm_engine_kernel_times(mpm_engine* e, double* total_ms8, long* launches8) {
+  return guarded([&] {
+    check_ctx(e);
+    collect_times(e);
+    for (int k = 0; k < KI_COUNT; k++) {
+      if (total_ms8) total_ms8[k] = e->kt_ms[k];
+      if (launches8) launches8[k] = e->kt_n[k];
+    }
+  });
+}
+
+extern "C" const char* mpm_engine_kernel_name(int kid) { return (kid >= 0 && kid < KI_COUNT) ? KI_NAMES[kid] : ""; }
+
+extern "C" int mpm_engine_download_order(mpm_engine* e, int id, int32_t* orig) {
+  return guarded([&] {
+    check_ctx(e);
+    const HostInst& h = e->inst[live_id(e, id)];
+    HIPCHK(hipStreamSynchronize(e->s));
+    HIPCHK(hipMemcpy(orig, e->ii + (size_t)I_ORIG * e->np + h.pg0 * PGRAN, h.n * sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" int mpm_engine_get_scalars(mpm_engine* e, int id, double* out8) {
+  return guarded([&] {
+    check_ctx(e);
+    get_scalars(e, live_id(e, id), out8);
+  });
+}
+
+extern "C" int mpm_engine_status(mpm_engine* e, int n, const int* ids, int* done, int* istep) {
+  return guarded([&] {
+    check_ctx(e);
+    HIPCHK(hipStreamSynchronize(e->s));
+    std::vector<InstT> ts(e->max_inst);
+    HIPCHK(hipMemcpy(ts.data(), e->d_ts, sizeof(InstT) * e->max_inst, hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; k++) {
+      int id = ids[k];
+      if (id < 0 || id >= e->max_inst || !e->inst[id].live) {
+        done[k] = -1;
+        if (istep) istep[k] = -1;
+        continue;
+      }
+      done[k] = ts[id].done;
+      if (istep) istep[k] = ts[id].istep;
+    }
+  });
+}
+
+extern "C" int mpm_engine_capacity(mpm_engine* e, long* free_particles, long* free_nodes, int* free_instances) {
+  return guarded([&] {
+    check_ctx(e);
+    if (free_particles) *free_particles = e->pa.largest() * PGRAN;
+    if (free_nodes) *free_nodes = e->na.largest() * NGRAN;
+    if (free_instances) *free_instances = (int)e->free_ids.size();
+  });
+}
+
+extern "C" long mpm_engine_device_bytes(mpm_engine* e) {
+  if (!e) return -1;
+  return (long)(e->np * (2 * bytes_per_particle(e->prec) + 16) + e->nn * bytes_per_node(e->prec));
+}
+
+extern "C" int mpm_ctx_engine(mpm_ctx* ctx, mpm_engine** out) {
+  return guarded([&] {
+    if (!ctx || !out) fail(MPM_ERR_ARG, "bad arguments");
+    *out = ctx->e;
+  });
+}

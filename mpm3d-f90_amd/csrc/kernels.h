@@ -1,0 +1,59 @@
+// kernels.h -- launch interface of kernels.hip (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+#include "mpm3d_problem.h"
+
+// k_p2g modes
+#define P2G_ICELL 1   // locate particles (ParticleInitial) and store icell
+#define P2G_MASS 2    // Mg += N m
+#define P2G_MOM 4     // PXg += v N m
+#define P2G_FORCE 8   // FXg += f_int + f_ext
+#define P2G_TO_P2 16  // momentum into the re-map buffer
+
+// k_g2p modes
+#define G2P_MOVE 1    // USF: Xp = XX
+#define G2P_REMAP 2   // MUSL: re-map momentum into p2
+#define G2P_ENERGY 4  // energy partials
+
+// k_stress modes
+#define ST_MOVE 1     // USL/MUSL: Xp = XX
+#define ST_FORCE 2    // USF: force scatter with the updated stress
+#define ST_ENERGY 4   // energy partials
+
+// edge of the cubic cell bricks of the particle sort key
+#define SORT_BRICK 4
+
+// k_finalize parts
+#define FIN_ENERGY 1
+#define FIN_DT 2
+#define FIN_DONE 4
+
+// a launch over a list of granules (particle or node)
+struct LaunchP {
+  const int* glist;   // device: granule ids
+  const int* ginst;   // device: instance of every pool granule
+  int nblk;           // blocks = granules * bpg
+  int bpg;            // blocks per granule = GRAN / bs
+  int bs;             // block size
+};
+
+namespace mpmk {
+template <typename T> void launch_p2g(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
+template <typename T> void launch_g2p(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
+template <typename T> void launch_stress(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
+template <typename T> void launch_integrate(const PoolPtrs<T>& P, const LaunchP& L, int va, hipStream_t s);
+template <typename T>
+void launch_bc(const PoolPtrs<T>& P, const LaunchP& L, int use_p2, int bc, int vel, hipStream_t s);
+template <typename T> void launch_node_va(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
+template <typename T> void launch_zero_nodes(T* a, int ncomp, long stride, const LaunchP& L, hipStream_t s);
+template <typename T> void launch_energy(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
+template <typename T>
+void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int bpg, int what, hipStream_t s);
+void launch_begin_step(InstT* ts, const int* ilist, int n, hipStream_t s);
+template <typename S>
+void launch_permute(S* dst, const S* src, const int* perm, const LaunchP& L, long np, int nfield, hipStream_t s);
+void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP& L, unsigned* keys, int* vals,
+                      hipStream_t s);
+}  // namespace mpmk

@@ -1,0 +1,1419 @@
+// kernels.hip -- gfx950 kernels of the explicit MPM step.
+//
+// Every particle kernel runs one particle per lane over the granules of the
+// active instances; every node kernel one node per lane.  Shape-function
+// weights are built per axis in registers and multiplied out per node in
+// the reference's association order, so FP64 results match the reference
+// up to the summation order of the scatter (P2G) sums.
+//
+// Reference correspondences are cited per routine.  T is the bulk type
+// (double: plain FP64 path; float: FP32 bulk with FP64 positions, FP64
+// grid mass and FP64 momentum/mass division).
+#include <hip/hip_runtime.h>
+
+#include "engine.h"
+#include "kernels.h"
+
+namespace mpmk {
+
+// ------------------------------------------------------------ utilities
+
+// Particle slot of this thread.  With ilv the block's slots are dealt out
+// round-robin over the 32-lane halves of the waves (slot = (t % 32) * bs/32
+// + t / 32): neighbouring slots, which share a cell after the sort, land in
+// different lanes of different instructions, so scatter atomics into the
+// LDS tile rarely hit one address from many lanes at once.
+__device__ __forceinline__ long part_index(int ilv, const int* glist, int bpg, int& gran) {
+  const int b = blockIdx.x;
+  gran = glist[b / bpg];
+  const int t = threadIdx.x;
+  const int j = ilv ? (t & 31) * (int)(blockDim.x >> 5) + (t >> 5) : t;
+  return (long)gran * PGRAN + (long)(b % bpg) * blockDim.x + j;
+}
+
+__device__ __forceinline__ long node_index(const int* glist, int bpg, int& gran) {
+  const int b = blockIdx.x;
+  gran = glist[b / bpg];
+  return (long)gran * NGRAN + (long)(b % bpg) * blockDim.x + threadIdx.x;
+}
+
+template <typename T>
+__device__ __forceinline__ void atomic_add(T* p, T v) {
+  unsafeAtomicAdd(p, v);
+}
+
+// momentum / mass at a node: in the FP32 path the division is done in FP64
+template <typename T>
+__device__ __forceinline__ T div_mass(T p, double mg) {
+  return (T)((double)p / mg);
+}
+template <>
+__device__ __forceinline__ double div_mass<double>(double p, double mg) {
+  return p / mg;
+}
+
+// ------------------------------------------------- cell location (a2)
+
+// InWhichCell (Grid.f90:123-155).  Returns the 1-based cell or -1; the
+// cell's coordinates are decomposed from the linear index so that the
+// reference's wrap at x == SpanX(2) is reproduced.
+__device__ __forceinline__ int in_which_cell(const InstC& I, double x, double y, double z) {
+  if (x < I.span0[0] || x > I.span1[0] || y < I.span0[1] || y > I.span1[1] || z < I.span0[2] || z > I.span1[2])
+    return -1;
+  const int ix = (int)((x - I.span0[0]) / I.dcell) + 1;
+  const int iy = (int)((y - I.span0[1]) / I.dcell) + 1;
+  const int iz = (int)((z - I.span0[2]) / I.dcell) + 1;
+  int c = (iz - 1) * I.ncxy + (iy - 1) * I.ncx + ix;
+  if (c > I.ncell || c <= 0) c = -1;
+  return c;
+}
+
+__device__ __forceinline__ void cell_coords(const InstC& I, int c, int& cx, int& cy, int& cz) {
+  const int c0 = c - 1;
+  cz = c0 / I.ncxy;
+  const int r = c0 - cz * I.ncxy;
+  cy = r / I.ncx;
+  cx = r - cy * I.ncx;
+}
+
+// ------------------------------------------------ shape functions (a3-a5)
+
+template <typename T>
+struct Stencil {
+  T w[3][3];   // [axis][slot] weights
+  T d[3][3];   // [axis][slot] derivative factors
+  int base[3]; // node coordinate of slot 0 per axis
+  int act[3];  // active slots per axis (GIMP: 2 or 3)
+  int node0;   // local linear index of the cell's first node
+};
+
+// per-axis weights.  c: 0-based cell coordinate; s0: span start.
+template <int SHAPE, typename T>
+__device__ __forceinline__ void axis_weights(const InstC& I, int ax, double x, int c, Stencil<T>& S) {
+  const double xg1 = (double)c * I.dcell + I.span0[ax];   // Grid.f90:300-302
+  if (SHAPE == MPM_SHAPE_LINEAR) {
+    // NShape (Grid.f90:461-501): XN = (Xp - Cxg)*iJacobi, Cxg = Xg1 + DCell/2
+    const double cxg = xg1 + I.dcell / 2.0;
+    const double xn = (x - cxg) * I.ijac;
+    S.w[ax][0] = (T)(-xn + 1.0);
+    S.w[ax][1] = (T)(xn + 1.0);
+    S.d[ax][0] = (T)-1;
+    S.d[ax][1] = (T)1;
+    S.base[ax] = c;
+    S.act[ax] = 2;
+  } else if (SHAPE == MPM_SHAPE_GIMP) {
+    // FindInflNode case choice + NShape_GIMP (Grid.f90:546-599, 621-847)
+    const double r1 = (x - xg1) * I.idcell;
+    int base = c, act = 2;
+    if (r1 < 0.25) { base = c - 1; act = 3; }
+    else if (r1 > 0.75) { act = 3; }
+    S.base[ax] = base;
+    S.act[ax] = act;
+    const int nmax = (ax == 0 ? I.ncx : (ax == 1 ? I.ncy : I.ncz));
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      T sh = (T)0, dn = (T)0;
+      const int n = base + s;
+      if (s < act && n >= 0 && n <= nmax) {
+        const double xg = (double)n * I.dcell + I.span0[ax];
+        const T rpg = (T)((x - xg) * I.idcell);
+        const T r = fabs(rpg);
+        if (r < (T)0.25) {
+          sh = ((T)7 - (T)16 * r * r) * (T)0.125;
+          dn = (T)-4 * rpg;
+        } else if (r < (T)0.75) {
+          sh = (T)1 - r;
+          dn = -copysign((T)1, rpg);
+        } else if (r < (T)1.25) {
+          const T t = ((T)5 - (T)4 * r) * (T)0.25;
+          sh = t * t;
+          dn = (T)2 * rpg - copysign((T)1, rpg) * (T)2.5;
+        }
+      }
+      S.w[ax][s] = sh;
+      S.d[ax][s] = dn;
+    }
+  } else {  // B-spline: BFindInflNode + NShape_Bspline (Grid.f90:852-1049)
+    const double c02 = (double)0.2f, c04 = (double)0.4f;   // FP32 literals
+    const T r = (T)((x - xg1) * I.idcell * c02 + c04);
+    const T idc = (T)I.idcell, f02 = (T)c02;
+    const T a = (T)3 - (T)5 * r, b = (T)5 * r - (T)2;
+    S.w[ax][0] = (T)0.5 * (a * a);
+    S.w[ax][1] = -((T)25 * (r * r)) + (T)25 * r - (T)5.5;
+    S.w[ax][2] = (T)0.5 * (b * b);
+    S.d[ax][0] = ((T)5 * r - (T)3) * (T)5 * f02 * idc;
+    S.d[ax][1] = (-((T)50 * r) + (T)25) * f02 * idc;
+    S.d[ax][2] = ((T)5 * r - (T)2) * (T)5 * f02 * idc;
+    S.base[ax] = c - 1;
+    S.act[ax] = 3;
+  }
+}
+
+template <int SHAPE, typename T>
+__device__ __forceinline__ void build_stencil(const InstC& I, double x, double y, double z, int c, Stencil<T>& S) {
+  int cx, cy, cz;
+  cell_coords(I, c, cx, cy, cz);
+  axis_weights<SHAPE, T>(I, 0, x, cx, S);
+  axis_weights<SHAPE, T>(I, 1, y, cy, S);
+  axis_weights<SHAPE, T>(I, 2, z, cz, S);
+  S.node0 = cz * I.ngxy + cy * I.ngx + cx;
+}
+
+// node order of NShape / CellsNode (Grid.f90:372-379)
+__device__ constexpr int LIN_ORD[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                          {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+// node order of BFindInflNode with per-axis slot (0: i-1, 1: i, 2: i+1)
+__device__ constexpr int BSP_ORD[27][3] = {
+    {1, 1, 1}, {2, 1, 1}, {2, 2, 1}, {1, 2, 1}, {1, 1, 2}, {2, 1, 2}, {2, 2, 2}, {1, 2, 2}, {1, 1, 0},
+    {2, 1, 0}, {2, 2, 0}, {1, 2, 0}, {1, 0, 0}, {2, 0, 0}, {1, 0, 1}, {2, 0, 1}, {1, 0, 2}, {2, 0, 2},
+    {0, 1, 1}, {0, 2, 1}, {0, 1, 2}, {0, 2, 2}, {0, 1, 0}, {0, 2, 0}, {0, 0, 0}, {0, 0, 1}, {0, 0, 2}};
+
+// Visit the influence nodes of one particle:
+//   f(local linear node, node x, node y, node z, N, dN/dx, dN/dy, dN/dz).
+// The linear index follows the reference's index arithmetic (B-spline wraps
+// at low faces); the 3D coordinates are the unwrapped ones.
+// Linear and B-spline visit in the reference's node order; GIMP in z-y-x order.
+template <int SHAPE, typename T, typename F>
+__device__ __forceinline__ void for_nodes(const InstC& I, const Stencil<T>& S, F&& f) {
+  if (SHAPE == MPM_SHAPE_LINEAR) {
+    const T ij4 = (T)I.ijac4;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int a = LIN_ORD[k][0], b = LIN_ORD[k][1], c = LIN_ORD[k][2];
+      const int node = S.node0 + a + b * I.ngx + c * I.ngxy;
+      const T wxy = S.w[0][a] * S.w[1][b];
+      const T N = wxy * S.w[2][c] * (T)0.125;
+      const T dx = S.d[0][a] * S.w[1][b] * S.w[2][c] * ij4;
+      const T dy = S.w[0][a] * S.d[1][b] * S.w[2][c] * ij4;
+      const T dz = wxy * S.d[2][c] * ij4;
+      f(node, S.base[0] + a, S.base[1] + b, S.base[2] + c, N, dx, dy, dz);
+    }
+  } else if (SHAPE == MPM_SHAPE_BSPLINE) {
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+      const int a = BSP_ORD[k][0], b = BSP_ORD[k][1], c = BSP_ORD[k][2];
+      // linear index arithmetic of BFindInflNode, including its wrap at low faces
+      const int node = S.node0 + (a - 1) + (b - 1) * I.ngx + (c - 1) * I.ngxy;
+      if (node < 0 || node >= I.nnode) continue;
+      const T wxy = S.w[0][a] * S.w[1][b];
+      const T N = wxy * S.w[2][c];
+      const T dx = S.d[0][a] * S.w[1][b] * S.w[2][c];
+      const T dy = S.w[0][a] * S.d[1][b] * S.w[2][c];
+      const T dz = wxy * S.d[2][c];
+      f(node, S.base[0] + a, S.base[1] + b, S.base[2] + c, N, dx, dy, dz);
+    }
+  } else {  // GIMP
+    const T idc = (T)I.idcell;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if (c >= S.act[2]) continue;
+      const int nz = S.base[2] + c;
+      if (nz < 0 || nz > I.ncz) continue;
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        if (b >= S.act[1]) continue;
+        const int ny = S.base[1] + b;
+        if (ny < 0 || ny > I.ncy) continue;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          if (a >= S.act[0]) continue;
+          const int nx = S.base[0] + a;
+          if (nx < 0 || nx > I.ncx) continue;
+          const int node = nz * I.ngxy + ny * I.ngx + nx;
+          const T wxy = S.w[0][a] * S.w[1][b];
+          const T N = wxy * S.w[2][c];
+          const T dx = S.d[0][a] * S.w[1][b] * S.w[2][c] * idc;
+          const T dy = S.w[0][a] * S.d[1][b] * S.w[2][c] * idc;
+          const T dz = wxy * S.d[2][c] * idc;
+          f(node, nx, ny, nz, N, dx, dy, dz);
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------ particle field access
+
+template <typename T>
+struct PV {
+  const PoolPtrs<T>& P;
+  long i;
+  __device__ T& t(int f) const { return P.t[(long)f * P.np + i]; }
+  __device__ double& d(int f) const { return P.d[(long)f * P.np + i]; }
+  __device__ int& n(int f) const { return P.i[(long)f * P.np + i]; }
+};
+
+// external particle force (update_step.f90:888-903)
+template <typename T>
+__device__ __forceinline__ void external_force(const PoolPtrs<T>& P, const PV<T>& p, const InstC& I, double time,
+                                               T fx[3]) {
+  const int fl = I.flags;
+  if (fl & IF_LOADS) {
+    fx[0] = p.t(T_FX); fx[1] = p.t(T_FY); fx[2] = p.t(T_FZ);
+    if (fl & IF_QUASI_LOAD) {
+      if (!(time >= I.qlt)) {
+        const T q = (T)I.qlt, tt = (T)time;
+#pragma unroll
+        for (int d = 0; d < 3; d++) fx[d] = fx[d] / q * tt;
+      }
+    }
+  } else {
+    fx[0] = fx[1] = fx[2] = (T)0;
+  }
+  if (fl & IF_QUASI_DAMP) {
+    const T dm = (T)I.damp * p.t(T_MASS);
+    fx[0] = fx[0] - dm * p.t(T_VX);
+    fx[1] = fx[1] - dm * p.t(T_VY);
+    fx[2] = fx[2] - dm * p.t(T_VZ);
+  }
+  if (fl & IF_GRAVITY) {
+    const BodyC& b = P.bodies[p.n(I_BODY)];
+    const T m = p.t(T_MASS);
+    fx[0] = fx[0] + m * (T)b.grav[0];
+    fx[1] = fx[1] + m * (T)b.grav[1];
+    fx[2] = fx[2] + m * (T)b.grav[2];
+  }
+}
+
+// ------------------------------------------------- energy block partials
+
+// block-level deterministic reduction of (ke, ie, mx, my, mz, maxv)
+__device__ __forceinline__ void block_energy(double* part, int gran, int sub, double v[6]) {
+  __shared__ double red[6][16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 5; q++) v[q] += __shfl_xor(v[q], off);
+    v[5] = fmax(v[5], __shfl_xor(v[5], off));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0)
+    for (int q = 0; q < 6; q++) red[q][wave] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int w = 0; w < nw; w++) {
+      for (int q = 0; q < 5; q++) s[q] += red[q][w];
+      s[5] = fmax(s[5], red[5][w]);
+    }
+    double* o = part + ((long)gran * (PGRAN / 64) + sub) * 6;
+    for (int q = 0; q < 6; q++) o[q] = s[q];
+  }
+}
+
+// calcEnergy terms of one particle (Particle.f90:528-535) and max |v_i| (update_step.f90:1852)
+template <typename T>
+__device__ __forceinline__ void particle_energy(const PV<T>& p, double v[6]) {
+  const double m = (double)p.t(T_MASS);
+  const double vx = (double)p.t(T_VX), vy = (double)p.t(T_VY), vz = (double)p.t(T_VZ);
+  double dot = 0.0;
+  dot = dot + vx * vx;
+  dot = dot + vy * vy;
+  dot = dot + vz * vz;
+  v[0] = dot * m * 0.5;
+  v[1] = p.d(D_IE);
+  v[2] = m * vx;
+  v[3] = m * vy;
+  v[4] = m * vz;
+  v[5] = fmax(fmax(fabs(vx), fabs(vy)), fabs(vz));
+}
+
+// ============================================================ LDS tiles
+//
+// A block's particles are spatially compact (generator order or cell sort),
+// so the nodes they touch fit a small box.  The box is computed
+// collectively; when it fits TILE_BYTES of LDS, gathers read the box from
+// LDS (loaded once, coalesced) and scatters accumulate with LDS atomics and
+// flush each node once with one global atomic per field.  Otherwise the
+// block falls back to direct global access.  Nodes outside the box (the
+// B-spline low-face wrap) always take the direct path.
+
+#define TILE_BYTES 32768
+
+struct Box {
+  int x0, y0, z0, bx, by, bz, n, ok;
+  __device__ __forceinline__ int idx(int nx, int ny, int nz) const {
+    const int ex = nx - x0, ey = ny - y0, ez = nz - z0;
+    if ((unsigned)ex >= (unsigned)bx || (unsigned)ey >= (unsigned)by || (unsigned)ez >= (unsigned)bz) return -1;
+    return ex + bx * (ey + by * ez);
+  }
+  __device__ __forceinline__ int node(const InstC& I, int e) const {
+    const int ex = e % bx, r = e / bx, ey = r % by, ez = r / by;
+    return (z0 + ez) * I.ngxy + (y0 + ey) * I.ngx + (x0 + ex);
+  }
+};
+
+template <int SHAPE>
+__device__ __forceinline__ Box block_box(const InstC& I, bool act, int cx, int cy, int cz, int cap) {
+  __shared__ int sb[6][16];
+  __shared__ Box res;
+  int v[6];
+  v[0] = act ? cx : 0x7fffffff; v[1] = act ? cy : 0x7fffffff; v[2] = act ? cz : 0x7fffffff;
+  v[3] = act ? cx : -1; v[4] = act ? cy : -1; v[5] = act ? cz : -1;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 3; q++) v[q] = min(v[q], __shfl_xor(v[q], off));
+#pragma unroll
+    for (int q = 3; q < 6; q++) v[q] = max(v[q], __shfl_xor(v[q], off));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int q = 0; q < 6; q++) sb[q][wave] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int w = 1; w < nw; w++) {
+      for (int q = 0; q < 3; q++) sb[q][0] = min(sb[q][0], sb[q][w]);
+      for (int q = 3; q < 6; q++) sb[q][0] = max(sb[q][0], sb[q][w]);
+    }
+    const int lo = SHAPE == MPM_SHAPE_LINEAR ? 0 : -1;
+    const int hi = SHAPE == MPM_SHAPE_GIMP ? 2 : 1;
+    Box b;
+    if (sb[3][0] < 0) {   // no particle in the grid
+      b.x0 = b.y0 = b.z0 = 0;
+      b.bx = b.by = b.bz = 0;
+      b.n = 0;
+      b.ok = 1;
+    } else {
+      b.x0 = max(sb[0][0] + lo, 0);
+      b.y0 = max(sb[1][0] + lo, 0);
+      b.z0 = max(sb[2][0] + lo, 0);
+      b.bx = min(sb[3][0] + hi, I.ncx) - b.x0 + 1;
+      b.by = min(sb[4][0] + hi, I.ncy) - b.y0 + 1;
+      b.bz = min(sb[5][0] + hi, I.ncz) - b.z0 + 1;
+      const long n = (long)b.bx * b.by * b.bz;
+      b.ok = n <= cap;
+      b.n = b.ok ? (int)n : 0;
+    }
+    res = b;
+  }
+  __syncthreads();
+  return res;
+}
+
+// ================================================================ kernels
+
+// GridMomentumInitial / GridMomentumUpdate / GridMomentumMUSL scatters
+// (update_step.f90:335-392, 881-988, 1705-1758) with optional fusion.
+template <int SHAPE, typename T, int MODE>
+__global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
+  int gran;
+  const long i = part_index(P.ilv, glist, bpg, gran);
+  const int ii = ginst[gran];
+  const InstC& I = P.inst[ii];
+  if (P.ts[ii].done) return;
+  constexpr bool MASS = (MODE & P2G_MASS) != 0, MOM = (MODE & P2G_MOM) != 0, FRC = (MODE & P2G_FORCE) != 0;
+  constexpr int NT = (MOM ? 3 : 0) + (FRC ? 3 : 0);
+  constexpr int CAP = TILE_BYTES / ((MASS ? 8 : 0) + NT * 8);
+  __shared__ __align__(16) char smem[TILE_BYTES];
+  double* tm = (double*)smem;                            // CAP (mass)
+  double* tf = (double*)(smem + (MASS ? 8 * CAP : 0));   // NT * CAP, accumulated in FP64
+  PV<T> p{P, i};
+  const bool live = i < I.p_end;
+  int c = -1;
+  double x = 0, y = 0, z = 0;
+  if (live) {
+    x = p.d(D_X); y = p.d(D_Y); z = p.d(D_Z);
+    if (MODE & P2G_ICELL) {
+      c = in_which_cell(I, x, y, z);
+      p.n(I_CELL) = c;
+    } else {
+      c = p.n(I_CELL);
+    }
+  } else if (MODE & P2G_ICELL) {
+    p.n(I_CELL) = -1;
+  }
+  const bool act = c >= 0;
+  int cx = 0, cy = 0, cz = 0;
+  if (act) cell_coords(I, c, cx, cy, cz);
+  const Box B = block_box<SHAPE>(I, act, cx, cy, cz, CAP);
+  const long nn = P.nn;
+  if (B.ok) {
+    for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
+      if (MASS) tm[e] = 0.0;
+#pragma unroll
+      for (int f = 0; f < NT; f++) tf[f * CAP + e] = 0.0;
+    }
+    __syncthreads();
+  }
+  double* Mg = P.mg + I.node_off;
+  T* Pg = ((MODE & P2G_TO_P2) ? P.p2 : P.pg) + I.node_off;
+  T* Fg = P.fg + I.node_off;
+  if (act) {
+    Stencil<T> S;
+    build_stencil<SHAPE, T>(I, x, y, z, c, S);
+    T m = (T)0, vx = (T)0, vy = (T)0, vz = (T)0;
+    if (MASS || MOM) {
+      m = p.t(T_MASS);
+      vx = p.t(T_VX); vy = p.t(T_VY); vz = p.t(T_VZ);
+    }
+    T fx[3] = {(T)0, (T)0, (T)0};
+    T sxx = 0, syy = 0, szz = 0, sxy = 0, syz = 0, sxz = 0, vol = 0;
+    if (FRC) {
+      external_force(P, p, I, P.ts[ii].time, fx);
+      const T smq = p.t(T_SM) - p.t(T_Q);
+      sxx = smq + p.t(T_SD0); syy = smq + p.t(T_SD1); szz = smq + p.t(T_SD2);
+      sxy = p.t(T_SD5); syz = p.t(T_SD3); sxz = p.t(T_SD4);
+      vol = (T)p.d(D_VOL);
+    }
+    for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T dX, T dY, T dZ) {
+      const int e = B.ok ? B.idx(nx, ny, nz) : -1;
+      T val[6];
+      int q = 0;
+      const T shm = N * m;
+      if (MOM) { val[q++] = vx * shm; val[q++] = vy * shm; val[q++] = vz * shm; }
+      if (FRC) {
+        val[q++] = -(sxx * dX + sxy * dY + sxz * dZ) * vol + fx[0] * N;
+        val[q++] = -(sxy * dX + syy * dY + syz * dZ) * vol + fx[1] * N;
+        val[q++] = -(sxz * dX + syz * dY + szz * dZ) * vol + fx[2] * N;
+      }
+      if (e >= 0) {
+        if (MASS) atomicAdd(&tm[e], (double)shm);
+#pragma unroll
+        for (int f = 0; f < NT; f++) atomicAdd(&tf[f * CAP + e], (double)val[f]);
+      } else {
+        if (MASS) atomic_add(&Mg[k], (double)shm);
+        q = 0;
+        if (MOM) { atomic_add(&Pg[k], val[0]); atomic_add(&Pg[nn + k], val[1]); atomic_add(&Pg[2 * nn + k], val[2]); q = 3; }
+        if (FRC) { atomic_add(&Fg[k], val[q]); atomic_add(&Fg[nn + k], val[q + 1]); atomic_add(&Fg[2 * nn + k], val[q + 2]); }
+      }
+    });
+  }
+  if (B.ok && B.n) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
+      const int k = B.node(I, e);
+      if (MASS) { const double v = tm[e]; if (v != 0.0) atomic_add(&Mg[k], v); }
+      int f = 0;
+      if (MOM)
+        for (int d = 0; d < 3; d++, f++) { const double v = tf[f * CAP + e]; if (v != 0.0) atomic_add(&Pg[d * nn + k], (T)v); }
+      if (FRC)
+        for (int d = 0; d < 3; d++, f++) { const double v = tf[f * CAP + e]; if (v != 0.0) atomic_add(&Fg[d * nn + k], (T)v); }
+    }
+  }
+}
+
+// IntegrateMomentum (update_step.f90:1194-1239).  With VA the same pass
+// also forms the nodal velocity PXg/Mg and acceleration FXg/Mg that
+// ParticlePositionUpdate interpolates (update_step.f90:1506-1509), once per
+// node instead of once per particle-node pair; both are 0 where
+// Mg <= CutOff, which the particle sums then skip.
+template <typename T>
+__global__ void __launch_bounds__(256) k_integrate(PoolPtrs<T> P, const int* nglist, const int* nginst, int bpg,
+                                                   int va) {
+  int gran;
+  const long j = node_index(nglist, bpg, gran);
+  const int ii = nginst[gran];
+  const InstC& I = P.inst[ii];
+  const InstT& ts = P.ts[ii];
+  if (ts.done) return;
+  if (j >= (long)I.node_off + I.nnode) return;
+  const long nn = P.nn;
+  const T dt = (T)ts.dt;
+  const bool half = ts.istep == 1;
+  const uint8_t fx = P.fix[j];
+  const double mg = P.mg[j];
+  const bool heavy = mg > I.cutoff;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    T pg = P.pg[d * nn + j];
+    T f = P.fg[d * nn + j];
+    if (half) pg = pg + f * dt * (T)0.5;
+    else pg = pg + f * dt;
+    if (fx & (1 << d)) {
+      pg = (T)0;
+      f = (T)0;
+      P.fg[d * nn + j] = (T)0;
+    }
+    P.pg[d * nn + j] = pg;
+    if (va) {
+      P.vg[d * nn + j] = heavy ? div_mass(pg, mg) : (T)0;
+      P.ag[d * nn + j] = heavy ? div_mass(f, mg) : (T)0;
+    }
+  }
+}
+
+// ApplyBoundaryConditions (update_step.f90:1795-1830) on PXg (or the
+// re-mapped momentum) and FXg; with VEL also the nodal velocity the
+// strain-rate gather of ParticleStressUpdate reads (update_step.f90:586-587).
+template <typename T>
+__global__ void __launch_bounds__(256) k_bc(PoolPtrs<T> P, const int* nglist, const int* nginst, int bpg, int use_p2,
+                                            int bc, int vel) {
+  int gran;
+  const long j = node_index(nglist, bpg, gran);
+  const int ii = nginst[gran];
+  const InstC& I = P.inst[ii];
+  if (P.ts[ii].done) return;
+  if (j >= (long)I.node_off + I.nnode) return;
+  const uint8_t fx = bc ? P.fix[j] : 0;
+  T* pg = use_p2 ? P.p2 : P.pg;
+  const long nn = P.nn;
+  const double mg = P.mg[j];
+  const bool heavy = mg > I.cutoff;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    T p = pg[d * nn + j];
+    if (fx & (1 << d)) {
+      p = (T)0;
+      pg[d * nn + j] = (T)0;
+      P.fg[d * nn + j] = (T)0;
+    }
+    if (vel) P.vg[d * nn + j] = heavy ? div_mass(p, mg) : (T)0;
+  }
+}
+
+// nodal acceleration only (per-phase ParticlePositionUpdate after a host
+// driven IntegrateMomentum)
+template <typename T>
+__global__ void __launch_bounds__(256) k_node_va(PoolPtrs<T> P, const int* nglist, const int* nginst, int bpg) {
+  int gran;
+  const long j = node_index(nglist, bpg, gran);
+  const int ii = nginst[gran];
+  const InstC& I = P.inst[ii];
+  if (P.ts[ii].done) return;
+  if (j >= (long)I.node_off + I.nnode) return;
+  const long nn = P.nn;
+  const double mg = P.mg[j];
+  const bool heavy = mg > I.cutoff;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    P.vg[d * nn + j] = heavy ? div_mass(P.pg[d * nn + j], mg) : (T)0;
+    P.ag[d * nn + j] = heavy ? div_mass(P.fg[d * nn + j], mg) : (T)0;
+  }
+}
+
+// fill kernel (used instead of hipMemsetAsync so steps can be graph-captured)
+template <typename T>
+__global__ void __launch_bounds__(256) k_zero_nodes(T* a, int ncomp, long stride, const int* nglist, int bpg) {
+  int gran;
+  const long j = node_index(nglist, bpg, gran);
+  for (int d = 0; d < ncomp; d++) a[d * stride + j] = (T)0;
+}
+
+// Gather the block's node box (NT fields of T) into LDS.
+template <typename T, int NT>
+__device__ __forceinline__ void load_tile(const InstC& I, const Box& B, T* tf, int cap, const T* const* src,
+                                          long nn) {
+  for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
+    const int k = B.node(I, e);
+#pragma unroll
+    for (int f = 0; f < NT; f++) tf[f * cap + e] = src[f / 3][(f % 3) * nn + k];
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void zero_tile(const Box& B, double* ta, int cap) {
+  for (int e = threadIdx.x; e < B.n; e += blockDim.x)
+#pragma unroll
+    for (int f = 0; f < NT; f++) ta[f * cap + e] = 0.0;
+}
+
+// flush a 3-component FP64 accumulation tile into a grid field of type T
+template <typename T>
+__device__ __forceinline__ void flush_tile3(const InstC& I, const Box& B, const double* ta, int cap, T* dst, long nn) {
+  for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
+    const int k = B.node(I, e);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const double v = ta[d * cap + e];
+      if (v != 0.0) atomic_add(&dst[d * nn + k], (T)v);
+    }
+  }
+}
+
+// ParticlePositionUpdate (update_step.f90:1434-1522), optionally fused with
+// the MUSL re-map of momentum (GridMomentumMUSL, :1705-1758) into p2 and
+// with the energy partials (when it is the last particle pass).
+template <int SHAPE, typename T, int MODE>
+__global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
+  int gran;
+  const long i = part_index(P.ilv, glist, bpg, gran);
+  const int ii = ginst[gran];
+  const InstC& I = P.inst[ii];
+  const InstT& ts = P.ts[ii];
+  if (ts.done) return;
+  constexpr int GB = 6 * (int)sizeof(T);   // V, A per node
+  constexpr int CAP = TILE_BYTES / (GB > 24 ? GB : 24);
+  __shared__ __align__(16) char smem[TILE_BYTES];
+  T* tf = (T*)smem;
+  double* ta = (double*)smem;   // re-map accumulation, reuses the gather tile
+  PV<T> p{P, i};
+  const bool live = i < I.p_end;
+  const int c = live ? p.n(I_CELL) : -1;
+  const bool act = c >= 0;
+  int cx = 0, cy = 0, cz = 0;
+  if (act) cell_coords(I, c, cx, cy, cz);
+  const Box B = block_box<SHAPE>(I, act, cx, cy, cz, CAP);
+  const long nn = P.nn;
+  const T* Vg = P.vg + I.node_off;
+  const T* Ag = P.ag + I.node_off;
+  if (B.ok && B.n) {
+    const T* srcs[2] = {Vg, Ag};
+    load_tile<T, 6>(I, B, tf, CAP, srcs, nn);
+    __syncthreads();
+  }
+  double ev[6] = {0, 0, 0, 0, 0, 0};
+  Stencil<T> S;
+  T vn[3] = {(T)0, (T)0, (T)0};
+  const double x = act ? p.d(D_X) : 0.0, y = act ? p.d(D_Y) : 0.0, z = act ? p.d(D_Z) : 0.0;
+  if (act) {
+    build_stencil<SHAPE, T>(I, x, y, z, c, S);
+    T vx[3] = {(T)0, (T)0, (T)0}, ax[3] = {(T)0, (T)0, (T)0};
+    for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) {
+      const int e = B.ok ? B.idx(nx, ny, nz) : -1;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        const T v = e >= 0 ? tf[d * CAP + e] : Vg[d * nn + k];
+        const T a = e >= 0 ? tf[(3 + d) * CAP + e] : Ag[d * nn + k];
+        vx[d] = vx[d] + N * v;
+        ax[d] = ax[d] + N * a;
+      }
+    });
+    const double dt = ts.dt;
+    const T tdt = (T)dt;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const T v = p.t(T_VX + d);
+      vn[d] = (ts.istep == 1) ? v + ax[d] * tdt * (T)0.5 : v + ax[d] * tdt;
+      p.t(T_VX + d) = vn[d];
+      p.t(T_BX + d) = vx[d];
+    }
+    if (MODE & G2P_MOVE) {   // USF: Xp = XX = Xp + vx*DT
+      p.d(D_X) = x + (double)vx[0] * dt;
+      p.d(D_Y) = y + (double)vx[1] * dt;
+      p.d(D_Z) = z + (double)vx[2] * dt;
+    }
+  }
+  if (MODE & G2P_REMAP) {
+    T* P2 = P.p2 + I.node_off;
+    if (B.ok && B.n) {
+      __syncthreads();
+      zero_tile<3>(B, ta, CAP);
+      __syncthreads();
+    }
+    if (act) {
+      const T m = p.t(T_MASS);
+      for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) {
+        const T shm = N * m;
+        const int e = B.ok ? B.idx(nx, ny, nz) : -1;
+        if (e >= 0) {
+#pragma unroll
+          for (int d = 0; d < 3; d++) atomicAdd(&ta[d * CAP + e], (double)(vn[d] * shm));
+        } else {
+#pragma unroll
+          for (int d = 0; d < 3; d++) atomic_add(&P2[d * nn + k], vn[d] * shm);
+        }
+      });
+    }
+    if (B.ok && B.n) {
+      __syncthreads();
+      flush_tile3<T>(I, B, ta, CAP, P2, nn);
+    }
+  }
+  if (MODE & G2P_ENERGY) {
+    if (live) particle_energy(p, ev);
+    block_energy(P.part, gran, blockIdx.x % bpg, ev);
+  }
+}
+
+// ------------------------------------------------------- constitution
+
+template <typename T>
+struct Cst {
+  T dinc[6];
+  double vold, vol, dvol, iener;   // FP64 in both paths
+  T sm, smold, sd[6], sig[6], sold[6], bqf;
+  T seqv, epeff, sigy, ieinc, den0, den, vol0, cp;
+};
+
+template <typename T>
+__device__ __forceinline__ T eq_stress(const Cst<T>& m) {   // EquivalentStress, FP32 result
+  const T* sd = m.sd;
+  T J2 = (T)0.5 * (sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2]) + sd[3] * sd[3] + sd[4] * sd[4] + sd[5] * sd[5];
+  J2 = J2 * (T)3.0;
+  return (T)(float)sqrt(J2);
+}
+
+template <typename T>
+__device__ __forceinline__ void sigrot(Cst<T>& m, const T vort[3]) {   // Constitution.f90:527-575
+  T* sig = m.sig;
+  const T q0 = (T)2 * sig[5] * vort[2];
+  const T q1 = (T)2 * sig[4] * vort[1];
+  const T q2 = (T)2 * sig[3] * vort[0];
+  T rot[6];
+  rot[0] = -q0 + q1;
+  rot[1] = q0 - q2;
+  rot[2] = -q1 + q2;
+  rot[3] = vort[0] * (sig[1] - sig[2]) + vort[2] * sig[4] - vort[1] * sig[5];
+  rot[4] = vort[1] * (sig[2] - sig[0]) + vort[0] * sig[5] - vort[2] * sig[3];
+  rot[5] = vort[2] * (sig[0] - sig[1]) + vort[1] * sig[3] - vort[0] * sig[4];
+#pragma unroll
+  for (int k = 0; k < 6; k++) sig[k] = sig[k] + rot[k];
+  m.sm = (sig[0] + sig[1] + sig[2]) / (T)3;
+  m.sd[0] = sig[0] - m.sm;
+  m.sd[1] = sig[1] - m.sm;
+  m.sd[2] = sig[2] - m.sm;
+  m.sd[3] = sig[3];
+  m.sd[4] = sig[4];
+  m.sd[5] = sig[5];
+}
+
+template <typename T>
+__device__ __forceinline__ void elastic_devi(Cst<T>& m, T G2) {
+  const T dem = (m.dinc[0] + m.dinc[1] + m.dinc[2]) / (T)3;
+  const T G = (T)0.5 * G2;
+  m.sd[0] = m.sd[0] + G2 * (m.dinc[0] - dem);
+  m.sd[1] = m.sd[1] + G2 * (m.dinc[1] - dem);
+  m.sd[2] = m.sd[2] + G2 * (m.dinc[2] - dem);
+  m.sd[3] = m.sd[3] + G * m.dinc[3];
+  m.sd[4] = m.sd[4] + G * m.dinc[4];
+  m.sd[5] = m.sd[5] + G * m.dinc[5];
+}
+
+template <typename T>
+__device__ __forceinline__ void elastic_p(Cst<T>& m, T K3) {
+  const T dem = (m.dinc[0] + m.dinc[1] + m.dinc[2]) / (T)3;
+  const T dsm = K3 * dem;
+  m.sm = m.sm + dsm;
+}
+
+template <typename T>
+__device__ __forceinline__ T sig_dot(const Cst<T>& m) {
+  return m.sig[0] * m.dinc[0] + m.sig[1] * m.dinc[1] + m.sig[2] * m.dinc[2] + m.sig[3] * m.dinc[3] +
+         m.sig[4] * m.dinc[4] + m.sig[5] * m.dinc[5];
+}
+
+template <typename T>
+__device__ __forceinline__ T new_dot(const Cst<T>& m, T e) {
+  return e + (m.sd[0] + m.sm) * m.dinc[0] + (m.sd[1] + m.sm) * m.dinc[1] + (m.sd[2] + m.sm) * m.dinc[2] +
+         m.sd[3] * m.dinc[3] + m.sd[4] * m.dinc[4] + m.sd[5] * m.dinc[5];
+}
+
+template <typename T>
+__device__ __forceinline__ void bulkq(Cst<T>& m, const InstC& I, T dt) {   // Constitution.f90:1543-1563
+  T dd = (m.dinc[0] + m.dinc[1] + m.dinc[2]) / dt;
+  dd = dd < (T)0 ? dd : (T)0;
+  const T h = (T)I.dcell;
+  m.bqf = m.den * h * h * (T)I.bq1 * dd * dd - (T)I.bq2 * m.den * h * m.cp * dd;
+}
+
+template <typename T>
+__device__ __forceinline__ void lieupd(Cst<T>& m, const InstC& I, T dt) {   // :627-646
+  if (m.sm < (T)0) bulkq(m, I, dt);
+  m.ieinc = m.ieinc - (T)2 * m.bqf * (m.dinc[0] + m.dinc[1] + m.dinc[2]);
+  const float vavg = (float)(m.vol + m.vold);
+  m.iener = m.iener + (double)((T)0.25 * m.ieinc) * (double)vavg;
+}
+
+template <typename T>
+__device__ __forceinline__ void hieupd(Cst<T>& m) {   // :649-676, non-failed branch
+  const float vavg = (float)(m.vol + m.vold);
+  const T e = m.dinc[0] * (m.sold[0] + m.sd[0]) + m.dinc[1] * (m.sold[1] + m.sd[1]) +
+              m.dinc[2] * (m.sold[2] + m.sd[2]) + m.dinc[3] * (m.sold[3] + m.sd[3]) +
+              m.dinc[4] * (m.sold[4] + m.sd[4]) + m.dinc[5] * (m.sold[5] + m.sd[5]);
+  m.ieinc = (T)0.25 * e * (T)vavg + (T)m.dvol * (m.smold - (T)2 * m.bqf);
+  m.iener = m.iener + m.ieinc;
+}
+
+__device__ __forceinline__ double det3(const double F[3][3]) {
+  return F[0][0] * F[1][1] * F[2][2] - F[0][0] * F[1][2] * F[2][1] - F[1][0] * F[0][1] * F[2][2] +
+         F[1][0] * F[0][2] * F[2][1] + F[2][0] * F[0][1] * F[1][2] - F[2][0] * F[0][2] * F[1][1];
+}
+
+// Constitution (Constitution.f90:79-368) for the models on the path:
+// elas (M3DM1), pla1 (M3DM2), pla2 (M3DM3), null+EOS4 (M3DM7/eos4),
+// Drucker-Prager (M3DM9), neo-Hookean (M3DM10).
+template <typename T>
+__device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, const MatC& mc, T dt,
+                                             const T de[6], const T vort[3], const double F[3][3],
+                                             const double deF[3][3]) {
+  Cst<T> m;
+#pragma unroll
+  for (int k = 0; k < 6; k++) m.dinc[k] = de[k];
+  m.vold = p.d(D_VOL);
+  if (I.flags & IF_UVFP) {
+    const double detF = det3(deF);
+    m.vol = m.vold * detF;
+  } else {
+    m.vol = m.vold * (1.0 + (double)de[0] + (double)de[1] + (double)de[2]);
+  }
+  p.d(D_VOL) = m.vol;
+  m.dvol = 0.5 * (m.vol - m.vold);
+  m.epeff = p.t(T_EPEFF);
+  m.sigy = p.t(T_SIGY);
+  m.seqv = p.t(T_SEQV);
+  m.iener = p.d(D_IE);
+  const T mp = p.t(T_MASS);
+  m.den0 = (T)mc.density;
+  m.vol0 = mp / m.den0;
+  m.den = (T)((double)mp / m.vol);
+  m.cp = p.t(T_CP);
+  m.sm = p.t(T_SM);
+  m.smold = m.sm;
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    m.sd[k] = p.t(T_SD0 + k);
+    m.sold[k] = m.sd[k];
+  }
+  m.sig[0] = m.sd[0] + m.sm;
+  m.sig[1] = m.sd[1] + m.sm;
+  m.sig[2] = m.sd[2] + m.sm;
+  m.sig[3] = m.sd[3];
+  m.sig[4] = m.sd[4];
+  m.sig[5] = m.sd[5];
+  m.bqf = (T)0;
+  m.ieinc = (T)0;
+  const T G2 = (T)mc.G2, K3 = (T)mc.K3;
+  switch (mc.type) {
+    case MPM_MAT_ELASTIC:
+      sigrot(m, vort);
+      m.ieinc = sig_dot(m);
+      elastic_devi(m, G2);
+      elastic_p(m, K3);
+      m.seqv = eq_stress(m);
+      m.ieinc = new_dot(m, m.ieinc);
+      lieupd(m, I, dt);
+      break;
+    case MPM_MAT_PLA1: {
+      sigrot(m, vort);
+      m.ieinc = sig_dot(m);
+      elastic_devi(m, G2);
+      m.seqv = eq_stress(m);
+      const T y0 = (T)mc.yield0;
+      if (m.seqv > y0) {
+        const T depeff = (m.seqv - m.sigy) / ((T)1.5 * G2);
+        m.epeff = m.epeff + depeff;
+        const T ratio = y0 / m.seqv;
+#pragma unroll
+        for (int k = 0; k < 6; k++) m.sd[k] = m.sd[k] * ratio;
+        m.seqv = m.seqv * ratio;
+      }
+      elastic_p(m, K3);
+      m.ieinc = new_dot(m, m.ieinc);
+      lieupd(m, I, dt);
+      break;
+    }
+    case MPM_MAT_PLA2: {
+      sigrot(m, vort);
+      m.ieinc = sig_dot(m);
+      elastic_devi(m, G2);
+      m.seqv = eq_stress(m);
+      if (m.seqv > m.sigy) {
+        const T H = (T)mc.PlaMod;
+        const T depeff = (m.seqv - m.sigy) / ((T)1.5 * G2 + H);
+        m.epeff = m.epeff + depeff;
+        m.sigy = m.sigy + H * depeff;
+        const T ratio = m.sigy / m.seqv;
+#pragma unroll
+        for (int k = 0; k < 6; k++) m.sd[k] = m.sd[k] * ratio;
+        m.seqv = m.seqv * ratio;
+      }
+      elastic_p(m, K3);
+      m.ieinc = new_dot(m, m.ieinc);
+      lieupd(m, I, dt);
+      break;
+    }
+    case MPM_MAT_NULL: {
+      const T miu2 = (T)mc.miu2;
+      const T tri = (T)0 * (m.dinc[0] / dt + m.dinc[1] / dt + m.dinc[2] / dt);   // 1/3 is integer 0
+      m.sd[0] = miu2 * (m.dinc[0] / dt - tri);
+      m.sd[1] = miu2 * (m.dinc[1] / dt - tri);
+      m.sd[2] = miu2 * (m.dinc[2] / dt - tri);
+      m.sd[3] = (T)0.5 * miu2 * m.dinc[3] / dt;
+      m.sd[4] = (T)0.5 * miu2 * m.dinc[4] / dt;
+      m.sd[5] = (T)0.5 * miu2 * m.dinc[5] / dt;
+      // eos4 (:1513-1541)
+      if (m.sm < (T)0) bulkq(m, I, dt);
+      hieupd(m);
+      const double pnew = mc.c0 * mc.c0 * ((double)mp / m.vol - mc.density);
+      m.iener = m.iener - m.dvol * pnew;
+      m.sm = (T)(-pnew);
+      break;
+    }
+    case MPM_MAT_DP: {   // M3DM9 (:1214-1330)
+      sigrot(m, vort);
+      const T Gmod = (T)mc.Gmod, Kmod = (T)mc.Kmod;
+      const T qfai = (T)mc.q_fai, kfai = (T)mc.k_fai, qpsi = (T)mc.q_psi, tenf = (T)mc.tenf;
+      m.ieinc = sig_dot(m);
+      elastic_devi(m, G2);
+      elastic_p(m, K3);
+      const T* sd = m.sd;
+      const T J2 = (T)0.5 * (sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2]) + sd[3] * sd[3] + sd[4] * sd[4] +
+                   sd[5] * sd[5];
+      const T Tau = sqrt(J2);
+      m.seqv = Tau * (T)sqrtf(3.0f);
+      const T dpFi = Tau + qfai * m.sm - kfai;
+      const T dpsig = m.sm - tenf;
+      const T third = (T)(1.0f / 3.0f), twoninth = (T)(2.0f / 9.0f);
+      bool shear = false;
+      if (dpsig < (T)0) {
+        shear = dpFi > (T)0;
+      } else {
+        const T alphap = sqrt((T)1 + qfai * qfai) - qfai;
+        const T Taup = kfai - qfai * tenf;
+        const T dp_hfai = Tau - Taup - alphap * dpsig;
+        if (dp_hfai > (T)0) {
+          shear = true;
+        } else {
+          const T dlamd = (m.sm - tenf) / Kmod;
+          m.sm = tenf;
+          m.epeff = m.epeff + dlamd * third * (T)sqrtf(2.0f);
+        }
+      }
+      if (shear) {
+        const T dlamd = dpFi / (Gmod + Kmod * qfai * qpsi);
+        m.sm = m.sm - Kmod * qpsi * dlamd;
+        const T newTau = kfai - qfai * m.sm;
+        const T ratio = newTau / Tau;
+#pragma unroll
+        for (int k = 0; k < 6; k++) m.sd[k] = m.sd[k] * ratio;
+        m.seqv = m.seqv * ratio;
+        m.epeff = m.epeff + dlamd * sqrt(third + twoninth * (qpsi * qpsi));
+      }
+      m.ieinc = new_dot(m, m.ieinc);
+      lieupd(m, I, dt);
+      break;
+    }
+    case MPM_MAT_NEOH: {   // M3DM10 (:725-780), evaluated in FP64
+      const double G = 0.5 * mc.G2;
+      const double J = det3(F);
+      double B[6];
+      B[0] = F[0][0] * F[0][0] + F[0][1] * F[0][1] + F[0][2] * F[0][2] - 1.0;
+      B[1] = F[1][0] * F[1][0] + F[1][1] * F[1][1] + F[1][2] * F[1][2] - 1.0;
+      B[2] = F[2][0] * F[2][0] + F[2][1] * F[2][1] + F[2][2] * F[2][2] - 1.0;
+      B[3] = F[0][0] * F[1][0] + F[0][1] * F[1][1] + F[0][2] * F[1][2];
+      B[4] = F[1][0] * F[2][0] + F[1][1] * F[2][1] + F[1][2] * F[2][2];
+      B[5] = F[0][0] * F[2][0] + F[0][1] * F[2][1] + F[0][2] * F[2][2];
+      m.ieinc = sig_dot(m);
+      const double iJ = 1.0 / J;
+      const double lj = mc.lamda * log(J) * 1.0;
+      double s[6];
+      s[0] = iJ * (G * B[0] + lj);
+      s[1] = iJ * (G * B[1] + lj);
+      s[2] = iJ * (G * B[2] + lj);
+      s[3] = iJ * G * B[4];
+      s[4] = iJ * G * B[5];
+      s[5] = iJ * G * B[3];
+      const double smd = (s[0] + s[1] + s[2]) / 3.0;
+      m.sm = (T)smd;
+      m.sd[0] = (T)(s[0] - smd);
+      m.sd[1] = (T)(s[1] - smd);
+      m.sd[2] = (T)(s[2] - smd);
+      m.sd[3] = (T)s[3];
+      m.sd[4] = (T)s[4];
+      m.sd[5] = (T)s[5];
+      m.seqv = eq_stress(m);
+      m.ieinc = new_dot(m, m.ieinc);
+      lieupd(m, I, dt);
+      break;
+    }
+    default:
+      break;
+  }
+  p.t(T_SM) = m.sm;
+  p.t(T_SEQV) = m.seqv;
+#pragma unroll
+  for (int k = 0; k < 6; k++) p.t(T_SD0 + k) = m.sd[k];
+  p.t(T_SIGY) = m.sigy;
+  p.t(T_EPEFF) = m.epeff;
+  p.d(D_IE) = m.iener;
+  p.t(T_CP) = m.cp;
+  p.t(T_Q) = m.bqf;
+}
+
+// ParticleStressUpdate (update_step.f90:456-647): strain-rate gather, F
+// update, constitution; optionally Xp = XX (USL/MUSL), the USF force
+// scatter, and the energy partials.
+template <int SHAPE, typename T, int MODE>
+__global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
+  int gran;
+  const long i = part_index(P.ilv, glist, bpg, gran);
+  const int ii = ginst[gran];
+  const InstC& I = P.inst[ii];
+  const InstT& ts = P.ts[ii];
+  if (ts.done) return;
+  constexpr int GB = 3 * (int)sizeof(T);   // nodal velocity per node
+  constexpr int CAP = TILE_BYTES / (GB > 24 ? GB : 24);
+  __shared__ __align__(16) char smem[TILE_BYTES];
+  T* tf = (T*)smem;
+  double* ta = (double*)smem;   // force accumulation after the gather
+  PV<T> p{P, i};
+  const bool live = i < I.p_end;
+  const int c = live ? p.n(I_CELL) : -1;
+  const bool act = c >= 0;
+  int cx = 0, cy = 0, cz = 0;
+  if (act) cell_coords(I, c, cx, cy, cz);
+  const Box B = block_box<SHAPE>(I, act, cx, cy, cz, CAP);
+  const long nn = P.nn;
+  const T* Vg = P.vg + I.node_off;
+  if (B.ok && B.n) {
+    const T* srcs[1] = {Vg};
+    load_tile<T, 3>(I, B, tf, CAP, srcs, nn);
+    __syncthreads();
+  }
+  double ev[6] = {0, 0, 0, 0, 0, 0};
+  Stencil<T> S;
+  const double x = act ? p.d(D_X) : 0.0, y = act ? p.d(D_Y) : 0.0, z = act ? p.d(D_Z) : 0.0;
+  if (act) {
+    build_stencil<SHAPE, T>(I, x, y, z, c, S);
+    T de[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, vort[3] = {(T)0, (T)0, (T)0};
+    T L[3][3] = {{(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}};
+    const bool uvfp = (I.flags & IF_UVFP) != 0;
+    for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T, T dX, T dY, T dZ) {
+      const int e = B.ok ? B.idx(nx, ny, nz) : -1;
+      const T v0 = e >= 0 ? tf[e] : Vg[k];
+      const T v1 = e >= 0 ? tf[CAP + e] : Vg[nn + k];
+      const T v2 = e >= 0 ? tf[2 * CAP + e] : Vg[2 * nn + k];
+      de[0] = de[0] + dX * v0;
+      de[1] = de[1] + dY * v1;
+      de[2] = de[2] + dZ * v2;
+      de[3] = de[3] + (dY * v2 + dZ * v1);
+      de[4] = de[4] + (dZ * v0 + dX * v2);
+      de[5] = de[5] + (dX * v1 + dY * v0);
+      vort[0] = vort[0] + (dY * v2 - dZ * v1);
+      vort[1] = vort[1] + (dZ * v0 - dX * v2);
+      vort[2] = vort[2] + (dX * v1 - dY * v0);
+      if (uvfp) {
+        L[0][0] = L[0][0] + dX * v0; L[0][1] = L[0][1] + dY * v0; L[0][2] = L[0][2] + dZ * v0;
+        L[1][0] = L[1][0] + dX * v1; L[1][1] = L[1][1] + dY * v1; L[1][2] = L[1][2] + dZ * v1;
+        L[2][0] = L[2][0] + dX * v2; L[2][1] = L[2][1] + dY * v2; L[2][2] = L[2][2] + dZ * v2;
+      }
+    });
+    const double dtd = ts.dt;
+    const T dt = (T)dtd;
+#pragma unroll
+    for (int k = 0; k < 6; k++) de[k] = de[k] * dt;
+#pragma unroll
+    for (int k = 0; k < 3; k++) vort[k] = vort[k] * dt / (T)2;
+    double F[3][3], deF[3][3];
+    if (uvfp) {
+      double F0[3][3], dF[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          dF[a][b] = (double)(L[a][b] * dt);
+          F0[a][b] = p.d(D_F0 + 3 * a + b);
+        }
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          const double f = F0[a][b] + dF[a][b] + dF[a][0] * F0[0][b] + dF[a][1] * F0[1][b] + dF[a][2] * F0[2][b];
+          p.d(D_F0 + 3 * a + b) = f;
+          F[a][b] = f;
+          deF[a][b] = dF[a][b];
+        }
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        deF[a][a] = deF[a][a] + 1.0;
+        F[a][a] = F[a][a] + 1.0;
+      }
+    }
+    const MatC& mc = P.mats[P.bodies[p.n(I_BODY)].mat];
+    constitution<T>(p, I, mc, dt, de, vort, F, deF);
+    if (MODE & ST_MOVE) {   // Xp = XX (update_step.f90:642)
+      p.d(D_X) = x + (double)p.t(T_BX) * dtd;
+      p.d(D_Y) = y + (double)p.t(T_BY) * dtd;
+      p.d(D_Z) = z + (double)p.t(T_BZ) * dtd;
+    }
+  }
+  if (MODE & ST_FORCE) {   // USF: GridMomentumUpdate scatter right after the update
+    T* Fg = P.fg + I.node_off;
+    if (B.ok && B.n) {
+      __syncthreads();
+      zero_tile<3>(B, ta, CAP);
+      __syncthreads();
+    }
+    if (act) {
+      T fx[3];
+      external_force(P, p, I, ts.time, fx);
+      const T smq = p.t(T_SM) - p.t(T_Q);
+      const T sxx = smq + p.t(T_SD0), syy = smq + p.t(T_SD1), szz = smq + p.t(T_SD2);
+      const T sxy = p.t(T_SD5), syz = p.t(T_SD3), sxz = p.t(T_SD4);
+      const T vol = (T)p.d(D_VOL);
+      for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T dX, T dY, T dZ) {
+        T f[3];
+        f[0] = -(sxx * dX + sxy * dY + sxz * dZ) * vol + fx[0] * N;
+        f[1] = -(sxy * dX + syy * dY + syz * dZ) * vol + fx[1] * N;
+        f[2] = -(sxz * dX + syz * dY + szz * dZ) * vol + fx[2] * N;
+        const int e = B.ok ? B.idx(nx, ny, nz) : -1;
+        if (e >= 0) {
+#pragma unroll
+          for (int d = 0; d < 3; d++) atomicAdd(&ta[d * CAP + e], (double)f[d]);
+        } else {
+#pragma unroll
+          for (int d = 0; d < 3; d++) atomic_add(&Fg[d * nn + k], f[d]);
+        }
+      });
+    }
+    if (B.ok && B.n) {
+      __syncthreads();
+      flush_tile3<T>(I, B, ta, CAP, Fg, nn);
+    }
+  }
+  if (MODE & ST_ENERGY) {
+    if (live) particle_energy(p, ev);
+    block_energy(P.part, gran, blockIdx.x % bpg, ev);
+  }
+}
+
+// calcEnergy partials alone (per-phase entry point)
+template <typename T>
+__global__ void __launch_bounds__(256) k_energy(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
+  int gran;
+  const long i = part_index(P.ilv, glist, bpg, gran);
+  const int ii = ginst[gran];
+  const InstC& I = P.inst[ii];
+  if (P.ts[ii].done) return;
+  double ev[6] = {0, 0, 0, 0, 0, 0};
+  if (i < I.p_end) particle_energy(PV<T>{P, i}, ev);
+  block_energy(P.part, gran, blockIdx.x % bpg, ev);
+}
+
+// calcEnergy / UpdateDT finalisation: one block per active instance sums
+// its block partials in a fixed order; DT from max |v| (update_step.f90:1845-1872);
+// optional end-of-run test (MPM3D.f90:152-158).
+template <typename T>
+__global__ void __launch_bounds__(256) k_finalize(PoolPtrs<T> P, const int* ilist, int bpg, int what) {
+  const int ii = ilist[blockIdx.x];
+  const InstC& I = P.inst[ii];
+  InstT& ts = P.ts[ii];
+  if (ts.done) return;
+  const long g0 = I.p_begin / PGRAN;
+  const long g1 = (I.p_end + PGRAN - 1) / PGRAN;
+  const long nblk = (g1 - g0) * bpg;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (long b = threadIdx.x; b < nblk; b += blockDim.x) {
+    const long g = g0 + b / bpg, sub = b % bpg;
+    const double* o = P.part + (g * (PGRAN / 64) + sub) * 6;
+    for (int q = 0; q < 5; q++) s[q] += o[q];
+    s[5] = fmax(s[5], o[5]);
+  }
+  __shared__ double red[6][256];
+  for (int q = 0; q < 6; q++) red[q][threadIdx.x] = s[q];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      for (int q = 0; q < 5; q++) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+      red[5][threadIdx.x] = fmax(red[5][threadIdx.x], red[5][threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (what & FIN_ENERGY) {
+    ts.ek_prev = ts.ek;
+    ts.ek = red[0][0];
+    ts.ei = red[1][0];
+    ts.mom[0] = red[2][0];
+    ts.mom[1] = red[3][0];
+    ts.mom[2] = red[4][0];
+  }
+  ts.maxv = red[5][0];
+  if (what & FIN_DT) {
+    double dt = (double)1e6f;
+    for (int m = 0; m < I.nmat; m++) {
+      const double vv = P.mats[I.mat_off + m].csound + ts.maxv;
+      dt = fmin(dt, I.dcell / vv);
+    }
+    ts.dt = dt * I.dtscale;
+  }
+  if ((what & FIN_DONE) && (I.flags & IF_ENDTIME)) {
+    bool stop;
+    if (I.flags & IF_QUASI_LOAD)
+      stop = (ts.time > I.endtime && ts.ek / (ts.ek + ts.ei) < 0.005) && ts.time > 100.0;
+    else
+      stop = ts.time > I.endtime;
+    if (stop) ts.done = 1;
+  }
+}
+
+// istep += 1; CurrentTime += DT (MPM3D.f90:72-74)
+__global__ void k_begin_step(InstT* ts, const int* ilist, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  InstT& t = ts[ilist[k]];
+  if (t.done) return;
+  t.istep += 1;
+  t.time = t.time + t.dt;
+  t.ei = 0.0;
+}
+
+// gather one T field through a permutation (sort support)
+template <typename S>
+__global__ void __launch_bounds__(256) k_permute(S* dst, const S* src, const int* perm, const int* glist, int bpg,
+                                                 long np, int nfield) {
+  int gran;
+  const long j = part_index(0, glist, bpg, gran);
+  const long from = perm[j];
+  for (int f = 0; f < nfield; f++) dst[(long)f * np + j] = src[(long)f * np + from];
+}
+
+// Sort key of a particle: its cell in brick-major order (bricks of
+// SORT_BRICK^3 cells, cells x-fastest inside a brick), so that a block of
+// consecutive particles covers a compact box of nodes.  The cell is located
+// from the current position (InWhichCell); particles outside the grid and
+// padding slots get the largest key of their instance.
+__device__ __forceinline__ unsigned brick_key(const InstC& I, int c) {
+  int cx, cy, cz;
+  cell_coords(I, c, cx, cy, cz);
+  const int nbx = (I.ncx + SORT_BRICK - 1) / SORT_BRICK, nby = (I.ncy + SORT_BRICK - 1) / SORT_BRICK;
+  const unsigned brick = (unsigned)(((cz / SORT_BRICK) * nby + cy / SORT_BRICK) * nbx + cx / SORT_BRICK);
+  const unsigned local = (unsigned)(((cz % SORT_BRICK) * SORT_BRICK + cy % SORT_BRICK) * SORT_BRICK + cx % SORT_BRICK);
+  return brick * (SORT_BRICK * SORT_BRICK * SORT_BRICK) + local;
+}
+
+__global__ void __launch_bounds__(256) k_sort_keys(const double* X, long np, const InstC* inst, const int* glist,
+                                                   const int* ginst, int bpg, unsigned* keys, int* vals) {
+  int gran;
+  const long j = part_index(0, glist, bpg, gran);
+  const InstC& I = inst[ginst[gran]];
+  int c = -1;
+  if (j < I.p_end) c = in_which_cell(I, X[j], X[np + j], X[2 * np + j]);
+  const int nbx = (I.ncx + SORT_BRICK - 1) / SORT_BRICK, nby = (I.ncy + SORT_BRICK - 1) / SORT_BRICK;
+  const int nbz = (I.ncz + SORT_BRICK - 1) / SORT_BRICK;
+  keys[j] = c > 0 ? brick_key(I, c) : (unsigned)nbx * nby * nbz * (SORT_BRICK * SORT_BRICK * SORT_BRICK);
+  vals[j] = (int)j;
+}
+
+// ---------------------------------------------------------- dispatch
+
+template <int SHAPE, typename T>
+void launch_p2g_s(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  dim3 g(L.nblk), b(L.bs);
+#define P2G_CASE(M) \
+  case M: hipLaunchKernelGGL((k_p2g<SHAPE, T, M>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+  switch (mode) {
+    P2G_CASE(P2G_ICELL | P2G_MASS | P2G_MOM)
+    P2G_CASE(P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE)
+    P2G_CASE(P2G_FORCE)
+    P2G_CASE(P2G_MOM)
+    default: break;
+  }
+#undef P2G_CASE
+}
+
+template <int SHAPE, typename T>
+void launch_g2p_s(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  dim3 g(L.nblk), b(L.bs);
+#define G2P_CASE(M) \
+  case M: hipLaunchKernelGGL((k_g2p<SHAPE, T, M>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+  switch (mode) {
+    G2P_CASE(0)
+    G2P_CASE(G2P_MOVE)
+    G2P_CASE(G2P_MOVE | G2P_ENERGY)
+    G2P_CASE(G2P_REMAP)
+    default: break;
+  }
+#undef G2P_CASE
+}
+
+template <int SHAPE, typename T>
+void launch_stress_s(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  dim3 g(L.nblk), b(L.bs);
+#define ST_CASE(M) \
+  case M: hipLaunchKernelGGL((k_stress<SHAPE, T, M>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+  switch (mode) {
+    ST_CASE(0)
+    ST_CASE(ST_MOVE)
+    ST_CASE(ST_FORCE)
+    ST_CASE(ST_MOVE | ST_ENERGY)
+    default: break;
+  }
+#undef ST_CASE
+}
+
+template <typename T>
+void launch_p2g(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  if (!L.nblk) return;
+  switch (shape) {
+    case MPM_SHAPE_LINEAR: launch_p2g_s<MPM_SHAPE_LINEAR, T>(mode, P, L, s); break;
+    case MPM_SHAPE_GIMP: launch_p2g_s<MPM_SHAPE_GIMP, T>(mode, P, L, s); break;
+    default: launch_p2g_s<MPM_SHAPE_BSPLINE, T>(mode, P, L, s); break;
+  }
+}
+
+template <typename T>
+void launch_g2p(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  if (!L.nblk) return;
+  switch (shape) {
+    case MPM_SHAPE_LINEAR: launch_g2p_s<MPM_SHAPE_LINEAR, T>(mode, P, L, s); break;
+    case MPM_SHAPE_GIMP: launch_g2p_s<MPM_SHAPE_GIMP, T>(mode, P, L, s); break;
+    default: launch_g2p_s<MPM_SHAPE_BSPLINE, T>(mode, P, L, s); break;
+  }
+}
+
+template <typename T>
+void launch_stress(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  if (!L.nblk) return;
+  switch (shape) {
+    case MPM_SHAPE_LINEAR: launch_stress_s<MPM_SHAPE_LINEAR, T>(mode, P, L, s); break;
+    case MPM_SHAPE_GIMP: launch_stress_s<MPM_SHAPE_GIMP, T>(mode, P, L, s); break;
+    default: launch_stress_s<MPM_SHAPE_BSPLINE, T>(mode, P, L, s); break;
+  }
+}
+
+template <typename T>
+void launch_integrate(const PoolPtrs<T>& P, const LaunchP& L, int va, hipStream_t s) {
+  if (L.nblk) hipLaunchKernelGGL(k_integrate<T>, dim3(L.nblk), dim3(L.bs), 0, s, P, L.glist, L.ginst, L.bpg, va);
+}
+
+template <typename T>
+void launch_bc(const PoolPtrs<T>& P, const LaunchP& L, int use_p2, int bc, int vel, hipStream_t s) {
+  if (L.nblk)
+    hipLaunchKernelGGL(k_bc<T>, dim3(L.nblk), dim3(L.bs), 0, s, P, L.glist, L.ginst, L.bpg, use_p2, bc, vel);
+}
+
+template <typename T>
+void launch_node_va(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  if (L.nblk) hipLaunchKernelGGL(k_node_va<T>, dim3(L.nblk), dim3(L.bs), 0, s, P, L.glist, L.ginst, L.bpg);
+}
+
+template <typename T>
+void launch_zero_nodes(T* a, int ncomp, long stride, const LaunchP& L, hipStream_t s) {
+  if (L.nblk) hipLaunchKernelGGL(k_zero_nodes<T>, dim3(L.nblk), dim3(L.bs), 0, s, a, ncomp, stride, L.glist, L.bpg);
+}
+
+template <typename T>
+void launch_energy(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  if (L.nblk) hipLaunchKernelGGL(k_energy<T>, dim3(L.nblk), dim3(L.bs), 0, s, P, L.glist, L.ginst, L.bpg);
+}
+
+template <typename T>
+void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int bpg, int what, hipStream_t s) {
+  if (ninst) hipLaunchKernelGGL(k_finalize<T>, dim3(ninst), dim3(256), 0, s, P, ilist, bpg, what);
+}
+
+void launch_begin_step(InstT* ts, const int* ilist, int n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_begin_step, dim3((n + 255) / 256), dim3(256), 0, s, ts, ilist, n);
+}
+
+template <typename S>
+void launch_permute(S* dst, const S* src, const int* perm, const LaunchP& L, long np, int nfield, hipStream_t s) {
+  if (L.nblk) hipLaunchKernelGGL(k_permute<S>, dim3(L.nblk), dim3(L.bs), 0, s, dst, src, perm, L.glist, L.bpg, np, nfield);
+}
+
+void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP& L, unsigned* keys, int* vals,
+                      hipStream_t s) {
+  if (L.nblk)
+    hipLaunchKernelGGL(k_sort_keys, dim3(L.nblk), dim3(L.bs), 0, s, X, np, inst, L.glist, L.ginst, L.bpg, keys, vals);
+}
+
+#define INST(T)                                                                                     \
+  template void launch_p2g<T>(int, int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);          \
+  template void launch_g2p<T>(int, int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);          \
+  template void launch_stress<T>(int, int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);       \
+  template void launch_integrate<T>(const PoolPtrs<T>&, const LaunchP&, int, hipStream_t);         \
+  template void launch_bc<T>(const PoolPtrs<T>&, const LaunchP&, int, int, int, hipStream_t);      \
+  template void launch_node_va<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                \
+  template void launch_zero_nodes<T>(T*, int, long, const LaunchP&, hipStream_t);                  \
+  template void launch_energy<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                 \
+  template void launch_finalize<T>(const PoolPtrs<T>&, const int*, int, int, int, hipStream_t);    \
+  template void launch_permute<T>(T*, const T*, const int*, const LaunchP&, long, int, hipStream_t);
+INST(double)
+INST(float)
+template void launch_permute<int>(int*, const int*, const int*, const LaunchP&, long, int, hipStream_t);
+template void launch_zero_nodes<uint8_t>(uint8_t*, int, long, const LaunchP&, hipStream_t);
+
+}  // namespace mpmk

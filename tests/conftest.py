@@ -1,0 +1,113 @@
+"""Shared test setup.
+
+* registers the ``gpu`` marker (tests that need an MI355X);
+* loads the package directory ``mpm3d-f90_amd/`` as ``mpm3d_f90_amd``;
+* puts ``oracle/`` on sys.path for the CPU oracle (the checker).
+"""
+import importlib.util
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG_DIR = os.path.join(ROOT, "mpm3d-f90_amd")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+DECK_DIR = os.path.join(ROOT, "tests", "golden", "decks")
+GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
+
+
+def load_package():
+    if "mpm3d_f90_amd" in sys.modules:
+        return sys.modules["mpm3d_f90_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "mpm3d_f90_amd", os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["mpm3d_f90_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+if ORACLE_DIR not in sys.path:
+    sys.path.insert(0, ORACLE_DIR)
+
+DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
+         "plate_neoh_musl_qulo", "dam_null_eos4_musl"]
+
+
+def deck_path(name):
+    return os.path.join(DECK_DIR, name + ".mpm")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def mpm():
+    return load_package()
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    import pyoracle
+    pyoracle.build()
+    return pyoracle
+
+
+def normwise_rel(a, b):
+    """max |a - b| / max |b| per field (rows); 0 where both are all zero."""
+    a = np.atleast_2d(a)
+    b = np.atleast_2d(b)
+    out = []
+    for x, y in zip(a, b):
+        ok = np.isfinite(y)
+        den = np.max(np.abs(y[ok])) if ok.any() else 0.0
+        num = np.max(np.abs(x[ok] - y[ok])) if ok.any() else 0.0
+        out.append(0.0 if num == 0 else num / (den if den > 0 else 1e-300))
+    return np.array(out)
+
+
+# Physical field groups for the parity metric: the error of every field is
+# measured against the largest magnitude of its group (a stress component that
+# is pure round-off in a symmetric problem is judged on the stress scale).
+PGROUPS = {
+    "x": ["x", "y", "z"],
+    "v": ["vx", "vy", "vz"],
+    "stress": ["sm", "seqv", "sdxx", "sdyy", "sdzz", "sdyz", "sdxz", "sdxy", "q"],
+    "vol": ["vol"],
+    "sig_y": ["sig_y"],
+    "epeff": ["epeff"],
+    "ie": ["ie"],
+    "fp": ["fp%d%d" % (i, j) for i in (1, 2, 3) for j in (1, 2, 3)],
+}
+GGROUPS = {"mg": ["mg"], "p": ["px", "py", "pz"], "f": ["fx", "fy", "fz"]}
+
+
+def group_errors(a, b, names, groups, unit_scale=("fp",)):
+    """dict group -> max |a-b| / max(|b|) over the group's fields (F - I is
+    measured against 1, i.e. against F)."""
+    idx = {n: i for i, n in enumerate(names)}
+    out = {}
+    for g, fields in groups.items():
+        rows = [idx[f] for f in fields]
+        A = a[rows]
+        B = b[rows]
+        ok = np.isfinite(B) & np.isfinite(A)
+        bad_nan = int(np.sum(np.isfinite(B) != np.isfinite(A)))
+        den = np.max(np.abs(B[ok])) if ok.any() else 0.0
+        if g in unit_scale:
+            den = max(den, 1.0)
+        num = np.max(np.abs(A[ok] - B[ok])) if ok.any() else 0.0
+        e = 0.0 if num == 0 else num / (den if den > 0 else 1e-300)
+        out[g] = float("inf") if bad_nan else e
+    return out

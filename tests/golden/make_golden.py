@@ -1,0 +1,65 @@
+"""Writes the golden fixtures tests/golden/<deck>.npz from the reference.
+
+Runs oracle/_ref/ref_driver (the reference's own step routines compiled
+from /root/reference/src by oracle/build_ref.sh, plus the dump driver) on
+every deck in tests/golden/decks/ and stores, per deck:
+
+* ``s0_<field>``  particle state after the input phase (InputPara +
+  calcEnergy), the fields the deck reader must reproduce;
+* ``s10_<field>`` particle state after 10 steps, every field;
+* ``g10``         grid state (Mg, PXg, FXg) after 10 steps;
+* ``icell10``     cell index of every particle in step 10;
+* ``fix``         node fixities;
+* ``scal0`` / ``scal10``: DT, time, EngKinetic, EngInternal, Momentum(3),
+  CutOff, DCell, Span(6), DTScale.
+
+Only run in the container that has the reference:  python tests/golden/make_golden.py
+"""
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import pyoracle  # noqa: E402
+
+DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
+         "plate_neoh_musl_qulo", "dam_null_eos4_musl"]
+S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp"]
+
+
+def scal(d):
+    return np.concatenate([[d["dt"], d["time"], d["ek"], d["ei"]], d["mom"], [d["cutoff"], d["dcell"]],
+                           d["span"], [d["dtscale"]]])
+
+
+def main():
+    if not os.path.exists(pyoracle.REF_DRIVER):
+        subprocess.run([os.path.join(ROOT, "oracle", "build_ref.sh")], check=True)
+    for name in DECKS:
+        tmp = tempfile.mkdtemp(prefix="golden_")
+        try:
+            shutil.copy(os.path.join(HERE, "decks", name + ".mpm"), tmp)
+            pyoracle.run_ref(os.path.join(tmp, name + ".mpm"), 10, os.path.join(tmp, "d"), [0, 10], cwd=tmp)
+            d0 = pyoracle.read_ref_dump(os.path.join(tmp, "d_s00000.bin"))
+            d10 = pyoracle.read_ref_dump(os.path.join(tmp, "d_s00010.bin"))
+        finally:
+            shutil.rmtree(tmp)
+        out = {"scal0": scal(d0), "scal10": scal(d10), "g10": d10["grid"], "icell10": d10["icell"],
+               "fix": d10["fix"], "ncell": np.array(d10["ncell"])}
+        for f in S0_FIELDS:
+            out["s0_" + f] = d0["particles"][f]
+        for f, v in d10["particles"].items():
+            out["s10_" + f] = v
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **out)
+        print(path, os.path.getsize(path))
+
+
+if __name__ == "__main__":
+    main()

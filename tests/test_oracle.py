@@ -1,0 +1,163 @@
+"""The CPU oracle (oracle/mpm_oracle.c) is pinned to the reference.
+
+* against the committed golden fixtures (the Fortran build's own state after
+  10 steps): bit-exact, every particle field, grid field, icell and scalar;
+* against the live reference build (oracle/_ref/ref_driver) when it exists
+  in this container: bit-exact after 200 steps, at steps 1, 50, 200;
+* against closed forms and invariants that hold independently of the
+  reference: mass and momentum conservation of P2G, exact transport of a
+  uniform velocity field, free fall of an unstressed body, DT formula.
+"""
+import os
+import shutil
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import DECKS, GOLDEN_DIR, deck_path, load_package
+
+PF_SKIP_NAN = ()
+
+
+def arr_eq(a, b):
+    return np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("name", DECKS)
+def test_oracle_matches_reference_fixture_bitexact(name, oracle_mod):
+    mpm = load_package()
+    g = np.load(os.path.join(GOLDEN_DIR, name + ".npz"))
+    prob = mpm.read_deck(deck_path(name))
+    o = oracle_mod.Oracle(prob)
+    o.step(10)
+    ps = o.particles()
+    for key in g.files:
+        if key.startswith("s10_"):
+            f = key[4:]
+            assert arr_eq(ps[mpm.PF[f]], g[key]), f
+    assert arr_eq(o.grid(), g["g10"])
+    assert np.array_equal(o.icell(), g["icell10"])
+    assert np.array_equal(o.fix(), g["fix"])
+    sc = o.scalars()
+    assert sc[2] == g["scal10"][0] and sc[1] == g["scal10"][1]
+    assert sc[3] == g["scal10"][2] and sc[4] == g["scal10"][3]
+    assert np.array_equal(sc[5:8], g["scal10"][4:7])
+
+
+@pytest.mark.parametrize("name", DECKS)
+def test_oracle_matches_live_reference_200_steps(name, oracle_mod):
+    if not os.path.exists(oracle_mod.REF_DRIVER):
+        pytest.skip("reference build oracle/_ref not present on this machine")
+    mpm = load_package()
+    tmp = tempfile.mkdtemp(prefix="refcmp_")
+    try:
+        shutil.copy(deck_path(name), tmp)
+        oracle_mod.run_ref(os.path.join(tmp, name + ".mpm"), 200, os.path.join(tmp, "d"), [1, 50, 200], cwd=tmp)
+        prob = mpm.read_deck(deck_path(name))
+        o = oracle_mod.Oracle(prob)
+        done = 0
+        for s in (1, 50, 200):
+            o.step(s - done)
+            done = s
+            d = oracle_mod.read_ref_dump(os.path.join(tmp, "d_s%05d.bin" % s))
+            ps = o.particles()
+            for f, v in d["particles"].items():
+                assert arr_eq(ps[mpm.PF[f]], v), (s, f)
+            assert arr_eq(o.grid(), d["grid"]), s
+            assert np.array_equal(o.icell(), d["icell"]), s
+            sc = o.scalars()
+            assert sc[2] == d["dt"] and sc[3] == d["ek"] and sc[4] == d["ei"], s
+    finally:
+        shutil.rmtree(tmp)
+
+
+def _block_deck(tmp_path, shape_kw, scheme_kw, vel="0.3 -0.2 0.1", load="", extra=""):
+    text = """mpm3d *** invariant
+nbco 1
+nbbo 1
+nbmp 512
+nmat 1
+spx 0 2.0
+spy 0 2.0
+spz 0 2.0
+dcell 0.2
+{shape}
+{scheme}
+{extra}
+material
+ 1 elas 1000 1e6 0.25
+particle bloc 1 1 1000 0.1 0.6 0.6 0.6 8 8 8
+velo
+body 1 {vel}
+endv
+{load}
+endi
+""".format(shape=shape_kw, scheme=scheme_kw, vel=vel, load=load, extra=extra)
+    p = tmp_path / "inv.mpm"
+    p.write_text(text)
+    return str(p)
+
+
+@pytest.mark.parametrize("shape", ["", "gimp", "bimp"])
+def test_p2g_conserves_mass_and_momentum(tmp_path, shape, oracle_mod):
+    mpm = load_package()
+    prob = mpm.read_deck(_block_deck(tmp_path, shape, "musl on"))
+    o = oracle_mod.Oracle(prob)
+    o.phase("begin_step")
+    o.phase("grid_momentum_initial")
+    g = o.grid()
+    m = prob.field("mass")
+    assert abs(g[0].sum() - m.sum()) <= 1e-13 * m.sum()
+    for d, f in enumerate(("vx", "vy", "vz")):
+        ref = (m * prob.field(f)).sum()
+        assert abs(g[1 + d].sum() - ref) <= 1e-13 * abs(ref)
+
+
+@pytest.mark.parametrize("shape", ["", "gimp", "bimp"])
+def test_uniform_velocity_is_transported_exactly(tmp_path, shape, oracle_mod):
+    """Unstressed body, no loads: in the first step every particle keeps its
+    velocity, and particles whose whole stencil lies inside the body (all
+    nodes above the mass cut-off) move by exactly v*DT (partition of unity
+    of every shape family)."""
+    mpm = load_package()
+    prob = mpm.read_deck(_block_deck(tmp_path, shape, "musl on", vel="0.5 -0.25 0.125"))
+    o = oracle_mod.Oracle(prob)
+    x0 = prob.field("x").copy()
+    y0 = prob.field("y").copy()
+    z0 = prob.field("z").copy()
+    dt = o.scalars()[2]
+    o.step(1)
+    ps = o.particles()
+    assert np.allclose(ps[mpm.PF["vx"]], 0.5, rtol=1e-14, atol=0)
+    assert np.allclose(ps[mpm.PF["vz"]], 0.125, rtol=1e-14, atol=0)
+    inner = (np.abs(x0 - 1.0) < 0.2) & (np.abs(y0 - 1.0) < 0.2) & (np.abs(z0 - 1.0) < 0.2)
+    assert inner.sum() >= 8
+    assert np.allclose(ps[mpm.PF["x"]][inner], x0[inner] + 0.5 * dt, rtol=0, atol=1e-14)
+    assert np.max(np.abs(ps[mpm.PF["sdxx"]][inner])) < 1e-6   # round-off level strain only
+
+
+def test_free_fall_half_step_leapfrog(tmp_path, oracle_mod):
+    """An unstressed body under gravity g: v(t_n) = g*(t_n - DT_1/2) with the
+    half first step of IntegrateMomentum / ParticlePositionUpdate."""
+    mpm = load_package()
+    path = _block_deck(tmp_path, "", "usl on", vel="0 0 0", load="load\ngrav 1 0 0 -9.8\nendl")
+    prob = mpm.read_deck(path)
+    o = oracle_mod.Oracle(prob)
+    dt1 = o.scalars()[2]
+    o.step(4)
+    t = o.scalars()[1]
+    g = float(np.float32(-9.8))
+    vz = o.particles()[mpm.PF["vz"]]
+    assert np.allclose(vz, g * (t - 0.5 * dt1), rtol=1e-10, atol=0)
+
+
+def test_update_dt_closed_form(tmp_path, oracle_mod):
+    mpm = load_package()
+    prob = mpm.read_deck(_block_deck(tmp_path, "", "musl on", vel="0.5 -3.0 0.125"))
+    o = oracle_mod.Oracle(prob)
+    o.step(1)
+    E, nu, ro = prob.mats[0].young, prob.mats[0].poisson, prob.mats[0].density
+    c = np.sqrt(E * (1 - nu) / (1 + nu) / (1 - 2 * nu) / ro)
+    vmax = np.max(np.abs(o.particles()[mpm.PF["vx"]:mpm.PF["vz"] + 1]))
+    assert o.scalars()[2] == prob.dcell / (c + vmax) * prob.dtscale
