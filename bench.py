@@ -49,19 +49,35 @@ def load_package():
 
 
 # Algorithmic bytes per particle / per touched node of each kernel class of
-# the USF step (B-spline, Drucker-Prager), by bulk type size t (4 or 8).
-# Particle fields: x,y,z 8 B each; volume and internal energy 8 B; cell and
-# body ids 4 B; all other particle fields t B.  Grid: Mg 8 B, PXg/FXg 3t B.
+# the USF step (B-spline, Drucker-Prager), by bulk type size t (4 or 8):
+# every field the kernel must read plus every field it must write, once.
+# Particle: x,y,z 8 B each; volume and internal energy 8 B; cell and body
+# ids 4 B; every other particle field t B.  Grid: velocity V, acceleration
+# A, momentum and force 3t B each, mass 8 B.
 def kernel_bytes(kid, t, loads=True):
     if kid == "p2g":        # read x, m, v; write icell | write Mg, PXg
         return 24 + t + 3 * t + 4, 8 + 3 * t
-    if kid == "stress":     # read x, cell, body, m, vol, ie, 11 stress/state fields, loads;
-        rd = 24 + 4 + 4 + t + 8 + 8 + 11 * t + (3 * t if loads else 0)
-        wr = 8 + 8 + 12 * t  # write vol, ie, sm, seqv, sd6, sig_y, epeff, q, cp
-        return rd + wr, 8 + 3 * t + 3 * t   # read Mg, PXg | write FXg
-    if kid == "g2p":        # read x, cell, v, m, ie; write v, vbar, x | read Mg, PXg, FXg
-        return 24 + 4 + 3 * t + t + 8 + 3 * t + 3 * t + 24, 8 + 6 * t
+    if kid == "stress":     # read x, cell, body, vol, ie, sm, seqv, sd(6), sig_y, epeff, cp, loads
+        rd = 24 + 4 + 4 + 8 + 8 + 11 * t + (3 * t if loads else 0)
+        wr = 8 + 8 + 12 * t  # write vol, ie, sm, seqv, sd(6), sig_y, epeff, q, cp
+        return rd + wr, 6 * t            # read V | write FXg
+    if kid == "g2p":        # read x, cell, v, m, ie; write v, vbar, x | read V, A
+        return 24 + 4 + 3 * t + t + 8 + 3 * t + 3 * t + 24, 6 * t
     return 0, 0
+
+
+def measured_traffic(kernel, prec):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    PMC summary (FETCH_SIZE and WRITE_SIZE, calibrated against device copies
+    of known size); None when absent."""
+    path = os.path.join(ROOT, "profiles", "slope_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\|\s*%s\s*\|\s*%s\s*\|.*\|\s*([0-9.e+]+)\s*\|\s*$" % (kernel, prec), line)
+        if m:
+            return float(m.group(1))
+    return None
 
 
 def time_steps(ctx, steps, warmup, profile):
@@ -207,7 +223,7 @@ def main():
         "speedup_fp32_vs_fp64": value / v64,
         "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, "fp32"),
                      "algorithmic_bytes": alg, "avg_launch_ms": ms_avg},
         "parity": parity or None,
         "cpu_baseline": cpu,

@@ -10,6 +10,7 @@ Packed-FP32 instruction selection is switched off for all device code.
 import argparse
 import concurrent.futures as cf
 import os
+import shutil
 import subprocess
 import sys
 
@@ -57,6 +58,13 @@ def build(jobs=4, verbose=False):
     tmp = OUT + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, OUT)
+    # Fortran host (ISO_C_BINDING) when a Fortran compiler is present
+    if shutil.which("amdflang") or os.path.exists("/opt/rocm/bin/amdflang"):
+        env = dict(os.environ, PATH=os.environ.get("PATH", "") + ":/opt/rocm/bin")
+        r = subprocess.run(["make", "-s", "-B", "-C", os.path.join(HERE, "fortran")], capture_output=True,
+                           text=True, env=env)
+        if r.returncode != 0:
+            raise RuntimeError("Fortran host build failed:\n" + r.stdout[-3000:] + r.stderr[-3000:])
     return OUT
 
 

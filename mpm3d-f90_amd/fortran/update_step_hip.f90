@@ -1,0 +1,300 @@
+! ---------------------------------------------------------------------------
+! update_step_hip -- drop-in replacement of the reference's update_step.f90.
+!
+! Defines the same argument-less external subroutines the reference driver
+! calls (MPM3D.f90:77-113): GridMomentumInitial, ApplyBoundaryConditions,
+! ParticleStressUpdate, GridMomentumUpdate, IntegrateMomentum,
+! ParticlePositionUpdate, GridMomentumMUSL, UpdateDT.  Linked in place of
+! update_step.o, they run the step on the MI355X through libmpm3d_hip.so.
+!
+! State: on the first call the module globals built by InputPara
+! (particle_list, body_list, mat_list, grid description, flags, DT) are
+! packed into an mpm_problem and uploaded; from then on the device owns the
+! state.  After the last particle phase of each step the particle state is
+! copied back into particle_list so that the reference's own calcEnergy,
+! OutCurve and writers see current values; UpdateDT returns the device DT
+! into the module variable DT.
+!
+! Environment: MPM3D_PRECISION=fp32 selects the FP32-bulk path;
+! MPM3D_DUMP=<file> writes the particle state after every step (stream,
+! n, istep, 8 scalars, PF_COUNT x n doubles).
+!
+! Compiles against the reference's module files (ParticleData, GridData,
+! MaterialData); total-Lagrangian, contact and stress-smoothing routines are
+! present only to satisfy the link and stop when called.
+! ---------------------------------------------------------------------------
+module update_step_hip_state
+  use, intrinsic :: iso_c_binding
+  use mpm_hip
+  implicit none
+  type(c_ptr), save :: ctx = c_null_ptr
+  type(mpm_problem_t), target, save :: prob
+  type(mpm_material_t), allocatable, target, save :: mats(:)
+  type(mpm_body_t), allocatable, target, save :: bodies(:)
+  real(c_double), allocatable, target, save :: ps(:)
+  integer(c_int32_t), allocatable, target, save :: pbody(:)
+  integer, save :: nsteps_done = 0
+
+contains
+
+  subroutine ensure_context()
+    use ParticleData
+    use GridData
+    use MaterialData
+    integer :: i, m, b, n, j
+    integer(c_int) :: prec
+    character(len=16) :: env
+    real(8) :: dt0, cp0
+    if (c_associated(ctx)) return
+    n = nb_particle
+    prob%nb_particle = n
+    prob%nb_body = nb_body
+    prob%nb_mat = nb_mat
+    if (SGMP) then
+       prob%shape = MPM_SHAPE_SGMP
+    else if (GIMP) then
+       prob%shape = MPM_SHAPE_GIMP
+    else if (Bspline) then
+       prob%shape = MPM_SHAPE_BSPLINE
+    else
+       prob%shape = MPM_SHAPE_LINEAR
+    end if
+    if (USF .and. MUSL) stop 'update_step_hip: USF together with MUSL is not supported'
+    if (USF) then
+       prob%scheme = MPM_USF
+    else if (MUSL) then
+       prob%scheme = MPM_MUSL
+    else
+       prob%scheme = MPM_USL
+    end if
+    prob%quasi_load = merge(1, 0, QuasiLoad)
+    prob%quasi_damp = merge(1, 0, QuasiDamp)
+    prob%gravity = merge(1, 0, Gravity)
+    prob%update_v_by_fp = merge(1, 0, UpdateVbyFp)
+    prob%fix = FixS
+    prob%ncx = NumCellx; prob%ncy = NumCelly; prob%ncz = NumCellz; prob%pad0 = 0
+    prob%span = [SpanX(1), SpanX(2), SpanY(1), SpanY(2), SpanZ(1), SpanZ(2)]
+    prob%dcell = DCell
+    prob%cutoff = CutOff
+    prob%dt = DT
+    prob%dtscale = DTScale
+    prob%endtime = EndTime
+    prob%quasi_load_time = QuasiLoadTime
+    prob%damp_factor = DampFactor
+    prob%bq1 = bq1
+    prob%bq2 = bq2
+    allocate(mats(nb_mat), bodies(nb_body), ps(PF_COUNT * n), pbody(n))
+    do m = 1, nb_mat
+       mats(m)%mtype = mat_list(m)%MatType
+       mats(m)%eos_type = mat_list(m)%EosType
+       mats(m)%density = mat_list(m)%Density
+       mats(m)%young = mat_list(m)%Young
+       mats(m)%poisson = mat_list(m)%Poisson
+       mats(m)%yield0 = mat_list(m)%Yield0
+       mats(m)%tangmod = mat_list(m)%TangMod
+       mats(m)%miu = mat_list(m)%miu
+       mats(m)%bulk_modulus = mat_list(m)%BulkModulus
+       mats(m)%wavespd = mat_list(m)%Wavespd
+       mats(m)%q_fai = mat_list(m)%q_fai
+       mats(m)%k_fai = mat_list(m)%k_fai
+       mats(m)%q_psi = mat_list(m)%q_psi
+       mats(m)%ten_f = mat_list(m)%ten_f
+       mats(m)%ceos = mat_list(m)%cEos
+       ! SetDT/UpdateDT take max(c, D, Wavespd) (DataIn.f90:1289-1290), but the
+       ! input phase sets D only for HE burn and Wavespd only for the null
+       ! material; otherwise the reference reads whatever the allocation held.
+       ! Those undefined values are replaced by 0 here.
+       mats(m)%det_speed = 0d0
+       if (mat_list(m)%MatType /= 9) mats(m)%wavespd = 0d0
+    end do
+    ! initial DT of SetDT (DataIn.f90:1278-1295) from the defined values
+    dt0 = 1.0e6
+    do m = 1, nb_mat
+       cp0 = sqrt(mats(m)%young * (1 - mats(m)%poisson) / (1 + mats(m)%poisson) / &
+                  (1 - 2 * mats(m)%poisson) / mats(m)%density)
+       cp0 = max(cp0, mats(m)%det_speed)
+       cp0 = max(cp0, mats(m)%wavespd)
+       dt0 = min(dt0, DCell / cp0)
+    end do
+    DT = dt0 * DTScale
+    prob%dt = DT
+    do b = 1, nb_body
+       bodies(b)%mat = body_list(b)%mat
+       bodies(b)%com = body_list(b)%comID
+       bodies(b)%par_begin = body_list(b)%par_begin - 1
+       bodies(b)%par_end = body_list(b)%par_end
+       bodies(b)%grav = body_list(b)%Gravp
+       do i = body_list(b)%par_begin, body_list(b)%par_end
+          pbody(i) = b - 1
+       end do
+    end do
+    call pack_particles()
+    prob%mats = c_loc(mats)
+    prob%bodies = c_loc(bodies)
+    prob%pstate = c_loc(ps)
+    prob%pbody = c_loc(pbody)
+    prec = MPM_PREC_FP64
+    call get_environment_variable('MPM3D_PRECISION', env)
+    if (trim(env) == 'fp32') prec = MPM_PREC_FP32
+    call mpm_check(mpm_ctx_create(c_loc(prob), prec, ctx), 'mpm_ctx_create')
+    j = 0
+  end subroutine ensure_context
+
+  subroutine pack_particles()
+    use ParticleData
+    integer :: i, n, a, c
+    n = nb_particle
+    do i = 1, n
+       associate (p => particle_list(i))
+         ps(PF_X * n + i) = p%Xp(1); ps(PF_Y * n + i) = p%Xp(2); ps(PF_Z * n + i) = p%Xp(3)
+         ps(PF_VX * n + i) = p%VXp(1); ps(PF_VY * n + i) = p%VXp(2); ps(PF_VZ * n + i) = p%VXp(3)
+         ps(PF_FX * n + i) = p%FXp(1); ps(PF_FY * n + i) = p%FXp(2); ps(PF_FZ * n + i) = p%FXp(3)
+         ps(PF_MASS * n + i) = p%mass
+         ps(PF_VOL * n + i) = p%VOL
+         ps(PF_SM * n + i) = p%SM
+         ps(PF_SEQV * n + i) = p%Seqv
+         ps(PF_SDXX * n + i) = p%SDxx; ps(PF_SDYY * n + i) = p%SDyy; ps(PF_SDZZ * n + i) = p%SDzz
+         ps(PF_SDYZ * n + i) = p%SDyz; ps(PF_SDXZ * n + i) = p%SDxz; ps(PF_SDXY * n + i) = p%SDxy
+         ps(PF_SIGY * n + i) = p%sig_y
+         ps(PF_EPEFF * n + i) = p%epeff
+         ps(PF_IE * n + i) = p%ie
+         ps(PF_Q * n + i) = p%q
+         ps(PF_CP * n + i) = p%cp
+         do a = 1, 3
+            do c = 1, 3
+               ps((PF_FP + 3 * (a - 1) + (c - 1)) * n + i) = p%Fp(a, c)
+            end do
+         end do
+         ps((PF_XX + 0) * n + i) = p%XX(1); ps((PF_XX + 1) * n + i) = p%XX(2); ps((PF_XX + 2) * n + i) = p%XX(3)
+       end associate
+    end do
+  end subroutine pack_particles
+
+  ! device -> particle_list (after the last particle phase of a step)
+  subroutine unpack_particles()
+    use ParticleData
+    integer :: i, n, a, c
+    n = nb_particle
+    call mpm_check(mpm_download_particles(ctx, ps), 'mpm_download_particles')
+    do i = 1, n
+       associate (p => particle_list(i))
+         p%Xp = [ps(PF_X * n + i), ps(PF_Y * n + i), ps(PF_Z * n + i)]
+         p%XX = [ps(PF_XX * n + i), ps((PF_XX + 1) * n + i), ps((PF_XX + 2) * n + i)]
+         p%VXp = [ps(PF_VX * n + i), ps(PF_VY * n + i), ps(PF_VZ * n + i)]
+         p%VOL = ps(PF_VOL * n + i)
+         p%SM = ps(PF_SM * n + i)
+         p%Seqv = ps(PF_SEQV * n + i)
+         p%SDxx = ps(PF_SDXX * n + i); p%SDyy = ps(PF_SDYY * n + i); p%SDzz = ps(PF_SDZZ * n + i)
+         p%SDyz = ps(PF_SDYZ * n + i); p%SDxz = ps(PF_SDXZ * n + i); p%SDxy = ps(PF_SDXY * n + i)
+         p%sig_y = ps(PF_SIGY * n + i)
+         p%epeff = ps(PF_EPEFF * n + i)
+         p%ie = ps(PF_IE * n + i)
+         p%q = ps(PF_Q * n + i)
+         p%cp = ps(PF_CP * n + i)
+         do a = 1, 3
+            do c = 1, 3
+               p%Fp(a, c) = ps((PF_FP + 3 * (a - 1) + (c - 1)) * n + i)
+            end do
+         end do
+       end associate
+    end do
+  end subroutine unpack_particles
+
+  subroutine maybe_dump()
+    use ParticleData
+    character(len=1024) :: f
+    real(c_double) :: sc(8)
+    integer :: u
+    call get_environment_variable('MPM3D_DUMP', f)
+    if (len_trim(f) == 0) return
+    call mpm_check(mpm_get_scalars(ctx, sc), 'mpm_get_scalars')
+    open(newunit=u, file=trim(f), access='stream', form='unformatted', status='replace')
+    write(u) int(nb_particle, 4), int(istep, 4), sc
+    write(u) ps
+    close(u)
+  end subroutine maybe_dump
+
+end module update_step_hip_state
+
+
+subroutine GridMomentumInitial()
+  use update_step_hip_state
+  implicit none
+  call ensure_context()
+  call mpm_check(mpm_begin_step(ctx), 'begin_step')   ! the driver already advanced istep/CurrentTime
+  call mpm_check(mpm_grid_momentum_initial(ctx), 'GridMomentumInitial')
+end subroutine GridMomentumInitial
+
+subroutine ApplyBoundaryConditions()
+  use update_step_hip_state
+  implicit none
+  call mpm_check(mpm_apply_boundary_conditions(ctx), 'ApplyBoundaryConditions')
+end subroutine ApplyBoundaryConditions
+
+subroutine ParticleStressUpdate()
+  use update_step_hip_state
+  use ParticleData, only: USF
+  implicit none
+  call mpm_check(mpm_particle_stress_update(ctx), 'ParticleStressUpdate')
+  if (.not. USF) call unpack_particles()
+end subroutine ParticleStressUpdate
+
+subroutine GridMomentumUpdate()
+  use update_step_hip_state
+  implicit none
+  call mpm_check(mpm_grid_momentum_update(ctx), 'GridMomentumUpdate')
+end subroutine GridMomentumUpdate
+
+subroutine IntegrateMomentum()
+  use update_step_hip_state
+  implicit none
+  call mpm_check(mpm_integrate_momentum(ctx), 'IntegrateMomentum')
+end subroutine IntegrateMomentum
+
+subroutine ParticlePositionUpdate()
+  use update_step_hip_state
+  use ParticleData, only: USF
+  implicit none
+  call mpm_check(mpm_particle_position_update(ctx), 'ParticlePositionUpdate')
+  if (USF) call unpack_particles()
+end subroutine ParticlePositionUpdate
+
+subroutine GridMomentumMUSL()
+  use update_step_hip_state
+  implicit none
+  call mpm_check(mpm_grid_momentum_musl(ctx), 'GridMomentumMUSL')
+end subroutine GridMomentumMUSL
+
+subroutine UpdateDT()
+  use update_step_hip_state
+  use ParticleData, only: DT
+  implicit none
+  real(c_double) :: dt_dev
+  call mpm_check(mpm_update_dt(ctx, dt_dev), 'UpdateDT')
+  DT = dt_dev
+  nsteps_done = nsteps_done + 1
+  call maybe_dump()
+end subroutine UpdateDT
+
+! --- routines of the reference outside the device path (link only) --------
+subroutine SmoothStressbyGrid()
+  stop 'update_step_hip: stress smoothing (smoo) is not available in the device build'
+end subroutine SmoothStressbyGrid
+subroutine TLSmoothStressbyGrid()
+  stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
+end subroutine TLSmoothStressbyGrid
+subroutine Lagr_NodContact()
+  stop 'update_step_hip: contact is not available in the device build'
+end subroutine Lagr_NodContact
+subroutine GridMassInitial()
+  stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
+end subroutine GridMassInitial
+subroutine TLGridMomentumInitial()
+  stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
+end subroutine TLGridMomentumInitial
+subroutine TLParticleStressUpdate()
+  stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
+end subroutine TLParticleStressUpdate
+subroutine TLGridMomentumUpdate()
+  stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
+end subroutine TLGridMomentumUpdate

@@ -45,3 +45,16 @@ $FC $FFLAGS -c "$HERE/ref_driver.f90" -o "$OUT/ref_driver.o"
 $FC $FFLAGS -o "$OUT/ref_driver" "$OUT/ref_driver.o" FFI.o Material.o Particle.o \
     Grid.o DataOut.o DataIn.o Constitution.o update_step.o
 echo "built $OUT/ref_driver"
+
+# Drop-in check: the reference's own main program (MPM3D.f90) and input
+# phase linked with mpm3d-f90_amd/fortran/update_step_hip.f90 in place of
+# update_step.o, driving libmpm3d_hip.so.  Built only when the library exists.
+LIBDIR="$HERE/../mpm3d-f90_amd"
+if [ -f "$LIBDIR/libmpm3d_hip.so" ]; then
+  $FC $FFLAGS -c "$LIBDIR/fortran/mpm_hip.f90" -o "$OUT/mpm_hip.o"
+  $FC $FFLAGS -w -c "$LIBDIR/fortran/update_step_hip.f90" -o "$OUT/update_step_hip.o"
+  $FC $FFLAGS -w -c "$REF/src/MPM3D.f90" -o "$OUT/MPM3D.o"
+  $FC $FFLAGS -o "$OUT/mpm3d_dropin" MPM3D.o update_step_hip.o mpm_hip.o FFI.o Material.o Particle.o \
+      Grid.o DataOut.o DataIn.o Constitution.o -L"$LIBDIR" -lmpm3d_hip -Wl,-rpath,"$LIBDIR"
+  echo "built $OUT/mpm3d_dropin"
+fi
