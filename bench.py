@@ -1,23 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark of the MPM3D explicit step on one MI355X.
 
-Workload: the single 3D soil slope (Drucker-Prager, quadratic B-spline, USF,
-ramped gravity load), 40 m profile with a 45-degree face, 0.3 m thick, cell
-0.1 m, 2x2x2 particles per cell: 1,289,970 particles, 500,526 nodes.
-
 One step = one explicit UL step of every particle (MPM3D.f90:69-113).
+
+Workloads:
+* batched ensemble (value): 256 seeded Drucker-Prager slope instances
+  (particles log-uniform in 2e4-4e5, 3.27e7 in total; slope angle, friction
+  and cohesion varied), quadratic B-spline, USF, ramped gravity load, all in
+  one device pool: one launch per phase for all instances, steps replayed
+  from a captured HIP graph, launch shape picked by the engine's autotuner;
+* single slope (FP64-plain comparison): 40 m profile with a 45-degree face,
+  0.3 m thick, cell 0.1 m, 2x2x2 particles per cell: 1,289,970 particles,
+  500,526 nodes, one instance on the plain fused-step path.
+
 Reported:
-* value: particle-steps/s of the FP32-bulk path (FP32 particle/grid
-  arithmetic; FP64 positions, volume, internal energy and grid mass), inputs
-  resident in HBM, K timed steps after W warm-up steps;
-* fp64_plain: the same workload on the plain FP64 path;
-* parity: both paths against the CPU FP64 oracle after a few steps at full
-  size (normwise per physical group);
+* value: particle-steps/s of the FP32-bulk batched ensemble (FP32
+  particle/grid arithmetic; FP64 positions, volume, internal energy, F and
+  grid mass), inputs resident in HBM, K timed steps after W warm-up steps;
+* fp64_plain / fp32_plain / fp64_batched and the speedup over fp64_plain;
+* parity: the single slope after a few steps and three ensemble instances
+  after every step the engine ran, against the CPU FP64 oracle (normwise per
+  physical group);
 * roofline: the dominant kernel's algorithmic bytes per launch over its
-  average launch time (HIP events on the engine stream, timed region);
+  average launch time (HIP events around every launch of a profiled run of
+  the same engine right after the timed region);
 * cpu_baseline: the reference's own Fortran build (oracle/_ref/ref_driver)
-  timed on the host for a bounded number of steps of the same deck, 1 core;
-  the CPU oracle is timed instead when that binary cannot run.
+  timed on the host on the first three ensemble instances for a bounded
+  number of steps, 1 core; the CPU oracle is timed instead when that binary
+  cannot run.
 """
 import argparse
 import json
@@ -66,11 +76,11 @@ def kernel_bytes(kid, t, loads=True):
     return 0, 0
 
 
-def measured_traffic(kernel, prec):
-    """HBM bytes per launch of a kernel class from the committed rocprofv3
-    PMC summary (FETCH_SIZE and WRITE_SIZE, calibrated against device copies
-    of known size); None when absent."""
-    path = os.path.join(ROOT, "profiles", "slope_pmc_traffic.md")
+def measured_traffic(kernel, prec, summary):
+    """HBM bytes per launch of a kernel class from a committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, calibrated against device copies of
+    known size); None when absent."""
+    path = os.path.join(ROOT, "profiles", summary)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -113,36 +123,156 @@ GROUPS = {"x": ["x", "y", "z"], "v": ["vx", "vy", "vz"],
           "vol": ["vol"], "ie": ["ie"], "epeff": ["epeff"]}
 
 
-def cpu_baseline(deck_text, prob, mpm, nsteps, log):
-    """Time the reference Fortran build on the same deck (1 core); fall back
-    to the CPU oracle (a C restatement of the same step)."""
+def cpu_baseline(sample, mpm, decks, nsteps, log):
+    """Time the reference Fortran build on a few instances of the workload
+    (1 core, nsteps each, step loop only); fall back to the CPU oracle (a C
+    restatement of the same step)."""
     ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    sizes = [n for _, n, _ in sample]
     if os.path.exists(ref):
         d = tempfile.mkdtemp(prefix="mpmref_")
         try:
-            with open(os.path.join(d, "slope.mpm"), "w") as f:
-                f.write(deck_text)
-            r = subprocess.run(["taskset", "-c", "0", ref, "slope.mpm", str(nsteps), os.path.join(d, "o"), "-1"],
-                               cwd=d, capture_output=True, text=True, timeout=600)
-            m = re.search(r"STEPLOOP\s+(\S+)\s+(\d+)\s+(\d+)", r.stdout)
-            if r.returncode == 0 and m:
-                sec, n, k = float(m.group(1)), int(m.group(2)), int(m.group(3))
-                return {"value": n * k / sec, "unit": "particle-steps/s", "cores": 1, "kind": "reference",
-                        "sample": "%d steps of the full %d-particle slope deck, Fortran reference "
-                                  "(flang -O2 -ffp-contract=off), step loop only" % (k, n)}
-            log("reference binary did not run (rc=%s); timing the oracle instead" % r.returncode)
+            tot_sec, tot_ps = 0.0, 0
+            for k, (text, n, _) in enumerate(sample):
+                with open(os.path.join(d, "s%d.mpm" % k), "w") as f:
+                    f.write(text)
+                r = subprocess.run(["taskset", "-c", "0", ref, "s%d.mpm" % k, str(nsteps), os.path.join(d, "o"),
+                                    "-1"], cwd=d, capture_output=True, text=True, timeout=600)
+                m = re.search(r"STEPLOOP\s+(\S+)\s+(\d+)\s+(\d+)", r.stdout)
+                if r.returncode != 0 or not m:
+                    raise OSError("reference binary rc=%s" % r.returncode)
+                tot_sec += float(m.group(1))
+                tot_ps += int(m.group(2)) * int(m.group(3))
+            return {"value": tot_ps / tot_sec, "unit": "particle-steps/s", "cores": 1, "kind": "reference",
+                    "sample": "%d steps of each of the first %d ensemble instances (%s particles), Fortran "
+                              "reference (flang -O2 -ffp-contract=off), step loop only, %.1f s"
+                              % (nsteps, len(sample), "+".join(map(str, sizes)), tot_sec)}
         except (OSError, subprocess.SubprocessError) as e:
             log("reference binary unavailable (%s); timing the oracle instead" % e)
         finally:
             shutil.rmtree(d, ignore_errors=True)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    o = pyoracle.Oracle(prob)
+    tot_sec, tot_ps = 0.0, 0
+    for k, (text, n, _) in enumerate(sample):
+        o = pyoracle.Oracle(decks.load(text, mpm, "s%d" % k))
+        t0 = time.perf_counter()
+        o.step(nsteps)
+        tot_sec += time.perf_counter() - t0
+        tot_ps += n * nsteps
+    return {"value": tot_ps / tot_sec, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "%d steps of each of the first %d ensemble instances (%s particles), C oracle"
+                      % (nsteps, len(sample), "+".join(map(str, sizes)))}
+
+
+def largest_divisor_upto(p, cap):
+    return max(d for d in range(1, min(p, cap) + 1) if p % d == 0)
+
+
+def touched_nodes(eng, ids):
+    return int(sum(np.count_nonzero(eng.grid(i)[0]) for i in ids))
+
+
+def roofline(kt, n_particles, n_touched, tsz, traffic_of):
+    dom = max(("p2g", "stress", "g2p"), key=lambda k: kt[k][0])
+    ms_avg = kt[dom][0] / max(kt[dom][1], 1)
+    bp, bn = kernel_bytes(dom, tsz)
+    alg = bp * n_particles + bn * n_touched
+    achieved = alg / (ms_avg * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(dom), "algorithmic_bytes": alg,
+            "avg_launch_ms": ms_avg}
+
+
+def run_plain(mpm, prob, a, log):
+    """The single slope on the plain (one-instance, fused-step) path."""
+    res = {}
+    for prec in ("fp64", "fp32"):
+        ctx = mpm.Context(prob, prec)
+        sec, kt = time_steps(ctx, a.steps, a.warmup, profile=True)
+        touched = int(np.count_nonzero(ctx.grid()[0]))
+        res[prec] = {"sec": sec, "kt": kt, "touched": touched, "value": prob.n * a.steps / sec}
+        log("plain %s: %.3f ms/step, %.4g particle-steps/s" % (prec, sec / a.steps * 1e3, res[prec]["value"]))
+        ctx.close()
+    return res
+
+
+def run_ensemble(mpm, decks, a, log):
+    """The seeded slope ensemble in one engine per precision: warm-up,
+    autotune, then K graph-replayed steps timed with the host clock around
+    synchronisations; a separate plain run of a few steps with HIP events
+    around every launch gives the per-kernel averages."""
+    ens = decks.slope_ensemble(a.ensemble, seed=a.seed)
+    total_p = sum(n for _, n, _ in ens)
+    total_n = sum(m["nodes"] for _, _, m in ens)
+    slack = (a.ensemble + 4) * 256
+    total_b = sum(m["bodies"] for _, _, m in ens)
+    engs = {prec: mpm.Engine(max_particles=total_p + slack, max_nodes=total_n + slack,
+                             max_instances=a.ensemble, max_materials=a.ensemble, max_bodies=total_b,
+                             precision=prec) for prec in ("fp32", "fp64")}
+    ids = {prec: [] for prec in engs}
     t0 = time.perf_counter()
-    o.step(nsteps)
-    sec = time.perf_counter() - t0
-    return {"value": prob.n * nsteps / sec, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "%d steps of the full %d-particle slope deck, C oracle" % (nsteps, prob.n)}
+    for k, (text, n, meta) in enumerate(ens):
+        prob = decks.load(text, mpm, "slope%d" % k)
+        for prec, eng in engs.items():
+            ids[prec].append(eng.add(prob))
+        del prob
+        if (k + 1) % 64 == 0:
+            log("ensemble: %d/%d instances resident (%.1f s)" % (k + 1, a.ensemble, time.perf_counter() - t0))
+    log("ensemble: %d instances, %d particles, %d nodes" % (a.ensemble, total_p, total_n))
+    out = {}
+    for prec, eng in engs.items():
+        eng.step(a.warmup)
+        eng.sync()
+        tune = eng.autotune(trial_steps=4)
+        per = largest_divisor_upto(tune["sort_every"], 10) if tune["sort_every"] > 0 else 10
+        log("ensemble %s tuning: %s, graph of %d steps" % (prec, json.dumps(tune), per))
+        eng.step(per, graph=True, graph_steps=per)   # capture outside the timed region
+        eng.sync()
+        t0 = time.perf_counter()
+        eng.step(a.steps, graph=True, graph_steps=per)
+        eng.sync()
+        sec = time.perf_counter() - t0
+        eng.profile(True)
+        eng.step(a.profile_steps)
+        kt = eng.kernel_times()
+        eng.profile(False)
+        out[prec] = {"sec": sec, "kt": kt, "tuning": tune, "graph_steps": per,
+                     "value": total_p * a.steps / sec, "ms_per_step": sec / a.steps * 1e3}
+        log("ensemble %s: %.3f ms/step, %.4g particle-steps/s" % (prec, out[prec]["ms_per_step"],
+                                                                 out[prec]["value"]))
+    out["touched"] = touched_nodes(engs["fp32"], ids["fp32"])
+    out["particles"] = total_p
+    out["nodes"] = total_n
+    out["ens"] = ens
+    out["engs"] = engs
+    out["ids"] = ids
+    return out
+
+
+def ensemble_parity(mpm, decks, E, a, log):
+    """FP32 and FP64 batched results of a few instances (smallest, second
+    smallest, median size) against the CPU FP64 oracle after the same steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    sizes = [n for _, n, _ in E["ens"]]
+    order = np.argsort(sizes)
+    pick = [int(order[0]), int(order[1]), int(order[len(order) // 2])]
+    res = []
+    for k in pick:
+        text, n, meta = E["ens"][k]
+        prob = decks.load(text, mpm, "slope%d" % k)
+        steps = int(E["engs"]["fp32"].scalars(E["ids"]["fp32"][k])[0])
+        o = pyoracle.Oracle(prob)
+        o.step(steps)
+        ref = o.particles()
+        r = {"instance": k, "particles": n, "steps": steps}
+        for prec in ("fp32", "fp64"):
+            assert int(E["engs"][prec].scalars(E["ids"][prec][k])[0]) == steps
+            r[prec + "_vs_fp64_oracle"] = group_err(mpm, E["engs"][prec].particles(E["ids"][prec][k]), ref, GROUPS)
+        res.append(r)
+        log("ensemble parity:", json.dumps(r))
+    return res
 
 
 def main():
@@ -150,8 +280,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--check-steps", type=int, default=2, help="steps compared with the CPU oracle")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="steps of the CPU baseline sample")
+    ap.add_argument("--ensemble", type=int, default=256, help="instances of the batched slope ensemble")
+    ap.add_argument("--seed", type=int, default=0, help="seed of the ensemble generator")
+    ap.add_argument("--profile-steps", type=int, default=5, help="steps with per-launch HIP events")
+    ap.add_argument("--check-steps", type=int, default=2, help="steps of the single slope compared with the oracle")
+    ap.add_argument("--cpu-steps", type=int, default=30, help="steps of the CPU baseline sample per instance")
     ap.add_argument("--no-cpu", action="store_true", help="skip oracle parity and CPU baseline")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -163,17 +296,10 @@ def main():
     from mpm3d_f90_amd import decks
     text, n = decks.slope()
     prob = decks.load(text, mpm, "slope")
-    log("workload: slope, %d particles, %d nodes" % (prob.n, prob.nb_node))
+    log("single slope: %d particles, %d nodes" % (prob.n, prob.nb_node))
 
-    res = {}
-    for prec in ("fp64", "fp32"):
-        ctx = mpm.Context(prob, prec)
-        sec, kt = time_steps(ctx, a.steps, a.warmup, profile=True)
-        g = ctx.grid()
-        touched = int(np.count_nonzero(g[0]))
-        res[prec] = {"sec": sec, "kt": kt, "touched": touched}
-        log("%s: %.3f ms/step, %.4g particle-steps/s" % (prec, sec / a.steps * 1e3, prob.n * a.steps / sec))
-        ctx.close()
+    plain = run_plain(mpm, prob, a, log)
+    E = run_ensemble(mpm, decks, a, log)
 
     parity = {}
     cpu = None
@@ -183,48 +309,56 @@ def main():
         o = pyoracle.Oracle(prob)
         o.step(a.check_steps)
         ref = o.particles()
+        single = {}
         for prec in ("fp64", "fp32"):
             ctx = mpm.Context(prob, prec)
             ctx.step(a.check_steps)
-            parity[prec + "_vs_fp64_oracle"] = group_err(mpm, ctx.particles(), ref, GROUPS)
+            single[prec + "_vs_fp64_oracle"] = group_err(mpm, ctx.particles(), ref, GROUPS)
             ctx.close()
-        parity["steps"] = a.check_steps
-        log("parity:", json.dumps(parity))
-        cpu = cpu_baseline(text, prob, mpm, a.cpu_steps, log)
+        single["steps"] = a.check_steps
+        parity["single_slope"] = single
+        log("single slope parity:", json.dumps(single))
+        parity["ensemble"] = ensemble_parity(mpm, decks, E, a, log)
+        parity["max_fp32_err"] = max(max(r["fp32_vs_fp64_oracle"].values()) for r in parity["ensemble"])
+        sample = [E["ens"][k] for k in range(3)]
+        cpu = cpu_baseline(sample, mpm, decks, a.cpu_steps, log)
         log("cpu baseline:", json.dumps(cpu))
 
-    r32 = res["fp32"]
-    kt = r32["kt"]
-    dom = max(("p2g", "stress", "g2p"), key=lambda k: kt[k][0])
-    ms_avg = kt[dom][0] / max(kt[dom][1], 1)
-    bp, bn = kernel_bytes(dom, 4)
-    alg = bp * prob.n + bn * r32["touched"]
-    achieved = alg / (ms_avg * 1e-3) / 1e9
-    value = prob.n * a.steps / r32["sec"]
-    v64 = prob.n * a.steps / res["fp64"]["sec"]
+    e32 = E["fp32"]
+    kt = e32["kt"]
+    traffic = lambda k: measured_traffic(k, "fp32", "ensemble_pmc_traffic.md")
+    v64p = plain["fp64"]["value"]
     out = {
         "metric": "particle-steps/s FP32-batched vs FP64-plain @<=1e-5 rel err; p50/p99 latency",
-        "value": value,
+        "value": e32["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": r32["sec"] / a.steps * 1e3,
+        "ms_per_step": e32["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "slope: Drucker-Prager soil, quadratic B-spline, USF, 40 m x 0.3 m x 20 m "
-                               "profile with 45-degree face, h=0.1 m, 2x2x2 particles/cell, gravity ramp",
-                   "particles": prob.n, "nodes": prob.nb_node, "nodes_touched": r32["touched"],
-                   "precision": "FP32 bulk; FP64 positions, volume, internal energy, grid mass"},
-        "fp64_plain": {"value": v64, "ms_per_step": res["fp64"]["sec"] / a.steps * 1e3},
-        "speedup_fp32_vs_fp64": value / v64,
+        "config": {"workload": "batched slope ensemble: %d Drucker-Prager slope instances (seed %d; particles "
+                               "log-uniform 2e4-4e5, angle U(30,60), friction U(20,40), cohesion lognormal "
+                               "median 10 kPa), quadratic B-spline, USF, h=0.1 m, 2x2x2 particles/cell, "
+                               "one launch per phase, graph-replayed steps, autotuned launch shape"
+                               % (a.ensemble, a.seed),
+                   "instances": a.ensemble, "particles": E["particles"], "nodes": E["nodes"],
+                   "nodes_touched": E["touched"],
+                   "precision": "FP32 bulk; FP64 positions, volume, internal energy, F, grid mass"},
+        "fp64_plain": {"workload": "single slope, %d particles, %d nodes" % (prob.n, prob.nb_node),
+                       "value": v64p, "ms_per_step": plain["fp64"]["sec"] / a.steps * 1e3},
+        "speedup_vs_fp64_plain": e32["value"] / v64p,
+        "fp32_plain": {"value": plain["fp32"]["value"], "ms_per_step": plain["fp32"]["sec"] / a.steps * 1e3},
+        "fp64_batched": {"value": E["fp64"]["value"], "ms_per_step": E["fp64"]["ms_per_step"]},
+        "tuning": {"fp32": e32["tuning"], "fp64": E["fp64"]["tuning"], "graph_steps": e32["graph_steps"]},
         "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, "fp32"),
-                     "algorithmic_bytes": alg, "avg_launch_ms": ms_avg},
+        "roofline": roofline(kt, E["particles"], E["touched"], 4, traffic),
+        "roofline_plain_fp32": roofline(plain["fp32"]["kt"], prob.n, plain["fp32"]["touched"], 4,
+                                        lambda k: measured_traffic(k, "fp32", "slope_pmc_traffic.md")),
         "parity": parity or None,
         "cpu_baseline": cpu,
     }

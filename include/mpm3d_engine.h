@@ -52,6 +52,27 @@ int mpm_engine_set_tuning(mpm_engine* e, int block_particles, int block_nodes, i
 /* interleaved (1) or contiguous (0) lane -> particle mapping in particle kernels */
 int mpm_engine_set_interleave(mpm_engine* e, int enable);
 
+/* Launch-shape settings.  sort_brick is the edge (cells) of the cubic cell
+ * bricks of the particle sort key (1, 2, 4, 8): the "cell bin" that keeps a
+ * block's particles inside a compact box of nodes.  step_ms / sort_ms /
+ * trials are filled by mpm_engine_autotune and ignored on input. */
+typedef struct mpm_tuning {
+  int32_t block_particles;   /* 64, 128, 256 */
+  int32_t block_nodes;       /* 64, 128, 256 */
+  int32_t sort_every;        /* steps between re-sorts, 0 = never */
+  int32_t sort_brick;        /* 1, 2, 4, 8 */
+  int32_t interleave;        /* 0 / 1 */
+  int32_t trials;            /* configurations timed by the autotuner */
+  double step_ms;            /* mean step time of the chosen shape, sort excluded */
+  double sort_ms;            /* time of one re-sort */
+} mpm_tuning;
+int mpm_engine_get_tuning(mpm_engine* e, mpm_tuning* out);
+int mpm_engine_apply_tuning(mpm_engine* e, const mpm_tuning* t);
+/* Time launch shapes on the resident instances (trial_steps graph-replayed
+ * steps each, from a device snapshot of the state that is restored
+ * afterwards, so the simulation is unchanged) and apply the fastest. */
+int mpm_engine_autotune(mpm_engine* e, int trial_steps, mpm_tuning* best);
+
 /* per-instance state (synchronising); layouts as in mpm3d_hip.h */
 int mpm_engine_download_particles(mpm_engine* e, int id, double* pstate);
 int mpm_engine_download_grid(mpm_engine* e, int id, double* gstate);

@@ -182,6 +182,12 @@ def _kernel_times(h):
     return {_lib.lib().mpm_engine_kernel_name(k).decode(): (ms[k], n[k]) for k in range(8)}
 
 
+class _Sizes:
+    def __init__(self, n, nb_node):
+        self.n = n
+        self.nb_node = nb_node
+
+
 class Engine:
     """Many independent instances in one device pool (batched ensembles and
     continuous-batching serving)."""
@@ -219,7 +225,8 @@ class Engine:
         i = C.c_int()
         check(_lib.lib().mpm_engine_add(self._h, C.byref(s), int(bool(stop_at_endtime)), C.byref(i)),
               "mpm_engine_add")
-        self.inst[i.value] = problem
+        # sizes only: the host copy of the state is not needed once uploaded
+        self.inst[i.value] = _Sizes(problem.n, problem.nb_node)
         return i.value
 
     def try_add(self, problem, stop_at_endtime=False):
@@ -259,6 +266,25 @@ class Engine:
     def set_tuning(self, block_particles=256, block_nodes=256, sort_every=0):
         check(_lib.lib().mpm_engine_set_tuning(self._h, int(block_particles), int(block_nodes),
                                                int(sort_every)), "mpm_engine_set_tuning")
+
+    def tuning(self):
+        t = _lib.Tuning()
+        check(_lib.lib().mpm_engine_get_tuning(self._h, C.byref(t)), "mpm_engine_get_tuning")
+        return t.as_dict()
+
+    def apply_tuning(self, **kw):
+        t = _lib.Tuning()
+        check(_lib.lib().mpm_engine_get_tuning(self._h, C.byref(t)), "mpm_engine_get_tuning")
+        for k, v in kw.items():
+            setattr(t, k, v)
+        check(_lib.lib().mpm_engine_apply_tuning(self._h, C.byref(t)), "mpm_engine_apply_tuning")
+
+    def autotune(self, trial_steps=6):
+        """Time launch shapes on the resident instances (state is restored
+        afterwards) and apply the fastest; returns the chosen settings."""
+        t = _lib.Tuning()
+        check(_lib.lib().mpm_engine_autotune(self._h, int(trial_steps), C.byref(t)), "mpm_engine_autotune")
+        return t.as_dict()
 
     def particles(self, iid):
         n = self.inst[iid].n

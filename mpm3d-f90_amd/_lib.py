@@ -59,6 +59,16 @@ class ProblemStruct(C.Structure):
                 ("pstate", C.POINTER(C.c_double)), ("pbody", C.POINTER(C.c_int32))]
 
 
+class Tuning(C.Structure):
+    """mpm_tuning (include/mpm3d_engine.h)"""
+    _fields_ = [("block_particles", C.c_int32), ("block_nodes", C.c_int32), ("sort_every", C.c_int32),
+                ("sort_brick", C.c_int32), ("interleave", C.c_int32), ("trials", C.c_int32),
+                ("step_ms", C.c_double), ("sort_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class MpmError(RuntimeError):
     """A non-zero status from the C-ABI, with the library's message."""
 
@@ -119,6 +129,9 @@ def lib():
         "mpm_engine_sort": (C.c_int, [vp]),
         "mpm_engine_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mpm_engine_set_interleave": (C.c_int, [vp, C.c_int]),
+        "mpm_engine_get_tuning": (C.c_int, [vp, P(Tuning)]),
+        "mpm_engine_apply_tuning": (C.c_int, [vp, P(Tuning)]),
+        "mpm_engine_autotune": (C.c_int, [vp, C.c_int, P(Tuning)]),
         "mpm_engine_download_particles": (C.c_int, [vp, C.c_int, P(C.c_double)]),
         "mpm_engine_download_grid": (C.c_int, [vp, C.c_int, P(C.c_double)]),
         "mpm_engine_download_icell": (C.c_int, [vp, C.c_int, P(C.c_int32)]),

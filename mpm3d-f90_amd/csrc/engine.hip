@@ -175,6 +175,7 @@ struct mpm_engine {
   int bs_p = 256, bs_n = 256;     // block sizes (autotuned)
   int ilv = 1;                    // interleaved lane mapping in particle kernels
   int sort_every = 20;            // steps between re-sorts in mpm_step (0: never)
+  int brick = 4;                  // edge (cells) of the bricks of the sort key
   long steps_done = 0;
   // sort scratch (lazily allocated)
   unsigned* k_in = nullptr;
@@ -186,6 +187,8 @@ struct mpm_engine {
   void* t_alt = nullptr;
   double* d_alt = nullptr;
   int* i_alt = nullptr;
+  int* d_seg = nullptr;           // segment begin/end slots, 2 x max_inst
+  std::vector<int> h_seg;         // host copy of d_seg (kept alive for the async upload)
   // graph
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -231,7 +234,7 @@ static void free_engine(mpm_engine* e) {
   if (e->graph) (void)hipGraphDestroy(e->graph);
   void* bufs[] = {e->t, e->d, e->ii, e->mg, e->pg, e->fg, e->p2, e->vg, e->ag, e->fix, e->d_inst, e->d_ts, e->d_mats,
                   e->d_bodies, e->part, e->d_ginst, e->d_nginst, e->d_ilist_all, e->k_in, e->k_out, e->v_in,
-                  e->v_out, e->cub_tmp, e->t_alt, e->d_alt, e->i_alt};
+                  e->v_out, e->cub_tmp, e->t_alt, e->d_alt, e->i_alt, e->d_seg};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& g : e->groups) {
@@ -861,9 +864,10 @@ static void step_graph(mpm_engine* e, int nsteps, int per, int fin_flags) {
 
 // ---------------------------------------------------------- sorting
 
-// Stable sort of every instance's particles by cell (segmented LSD radix
-// sort: equal keys keep their order, so the permutation is the one a
-// stable CPU sort of the same icell array gives), then gather all fields.
+// Stable sort of every instance's particles by brick-major cell key
+// (segmented LSD radix sort: equal keys keep their order, so the
+// permutation is the one a stable CPU sort of the same keys gives), then
+// gather all fields.
 static void sort_particles(mpm_engine* e) {
   rebuild_lists(e);
   const long np = e->np;
@@ -876,38 +880,48 @@ static void sort_particles(mpm_engine* e) {
     e->d_alt = dalloc<double>((size_t)D_COUNT * np);
     e->i_alt = dalloc<int>((size_t)I_COUNT * np);
   }
-  std::vector<int> seg_b, seg_e;
+  if (!e->d_seg) e->d_seg = dalloc<int>(2 * (size_t)e->max_inst);
+  std::vector<int> seg;
   unsigned maxkey = 1;
+  const unsigned B = (unsigned)e->brick;
   for (auto& G : e->groups) {
     const LaunchP L = plist(e, G);
-    mpmk::launch_sort_keys(e->d, np, e->d_inst, L, e->k_in, e->v_in, e->s);
+    mpmk::launch_sort_keys(e->d, np, e->d_inst, L, e->brick, e->k_in, e->v_in, e->s);
     for (int id : G.insts) {
       const HostInst& h = e->inst[id];
-      seg_b.push_back((int)(h.pg0 * PGRAN));
-      seg_e.push_back((int)((h.pg0 + h.npg) * PGRAN));
-      const unsigned B = SORT_BRICK, nb = ((h.c.ncx + B - 1) / B) * ((h.c.ncy + B - 1) / B) * ((h.c.ncz + B - 1) / B);
+      seg.push_back((int)(h.pg0 * PGRAN));
+      const unsigned nb = ((h.c.ncx + B - 1) / B) * ((h.c.ncy + B - 1) / B) * ((h.c.ncz + B - 1) / B);
       maxkey = std::max<unsigned>(maxkey, nb * B * B * B);
     }
   }
-  if (seg_b.empty()) return;
+  const int nseg = (int)seg.size();
+  if (!nseg) return;
+  for (auto& G : e->groups)
+    for (int id : G.insts) {
+      const HostInst& h = e->inst[id];
+      seg.push_back((int)((h.pg0 + h.npg) * PGRAN));
+    }
   int end_bit = 1;
   while (end_bit < 32 && (1u << end_bit) <= maxkey) end_bit++;
-  int *d_b = dalloc<int>(seg_b.size()), *d_e = dalloc<int>(seg_e.size());
-  HIPCHK(hipMemcpyAsync(d_b, seg_b.data(), seg_b.size() * sizeof(int), hipMemcpyHostToDevice, e->s));
-  HIPCHK(hipMemcpyAsync(d_e, seg_e.data(), seg_e.size() * sizeof(int), hipMemcpyHostToDevice, e->s));
+  if (seg != e->h_seg) {
+    // the previous upload may still read h_seg: wait before replacing it
+    HIPCHK(hipStreamSynchronize(e->s));
+    e->h_seg = seg;
+    HIPCHK(hipMemcpyAsync(e->d_seg, e->h_seg.data(), seg.size() * sizeof(int), hipMemcpyHostToDevice, e->s));
+  }
   size_t need = 0;
-  const bool one = seg_b.size() == 1;
+  const int b0 = seg[0], n0 = seg[nseg] - seg[0];
   // one instance: plain radix sort of its slot range; several: segmented
   auto run_sort = [&](void* tmp, size_t& bytes) {
-    if (one)
-      return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, e->k_in + seg_b[0], e->k_out + seg_b[0],
-                                                e->v_in + seg_b[0], e->v_out + seg_b[0], seg_e[0] - seg_b[0], 0,
-                                                end_bit, e->s);
+    if (nseg == 1)
+      return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, e->k_in + b0, e->k_out + b0, e->v_in + b0,
+                                                e->v_out + b0, n0, 0, end_bit, e->s);
     return hipcub::DeviceSegmentedRadixSort::SortPairs(tmp, bytes, e->k_in, e->k_out, e->v_in, e->v_out, (int)np,
-                                                       (int)seg_b.size(), d_b, d_e, 0, end_bit, e->s);
+                                                       nseg, e->d_seg, e->d_seg + nseg, 0, end_bit, e->s);
   };
   HIPCHK(run_sort(nullptr, need));
   if (need > e->cub_bytes) {
+    HIPCHK(hipStreamSynchronize(e->s));
     if (e->cub_tmp) HIPCHK(hipFree(e->cub_tmp));
     e->cub_tmp = dalloc<char>(need);
     e->cub_bytes = need;
@@ -926,9 +940,199 @@ static void sort_particles(mpm_engine* e) {
   HIPCHK(hipMemcpyAsync(e->t, e->t_alt, (size_t)T_COUNT * np * e->tsz, hipMemcpyDeviceToDevice, e->s));
   HIPCHK(hipMemcpyAsync(e->d, e->d_alt, (size_t)D_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
   HIPCHK(hipMemcpyAsync(e->ii, e->i_alt, (size_t)I_COUNT * np * sizeof(int), hipMemcpyDeviceToDevice, e->s));
+}
+
+// --------------------------------------------------------- autotuner
+
+// Device copy of the resident particle state and per-instance step state,
+// so that tuning trials leave the simulation where it was.
+struct Snapshot {
+  void* t = nullptr;
+  double* d = nullptr;
+  int* i = nullptr;
+  InstT* ts = nullptr;
+  long steps = 0;
+  std::vector<std::pair<bool, bool>> flags;
+};
+
+static void snap_free(Snapshot& S) {
+  void* b[] = {S.t, S.d, S.i, S.ts};
+  for (void* p : b)
+    if (p) (void)hipFree(p);
+  S = Snapshot();
+}
+
+static void snap_take(mpm_engine* e, Snapshot& S) {
+  const long np = e->np;
+  S.t = dalloc<char>((size_t)T_COUNT * np * e->tsz);
+  S.d = dalloc<double>((size_t)D_COUNT * np);
+  S.i = dalloc<int>((size_t)I_COUNT * np);
+  S.ts = dalloc<InstT>(e->max_inst);
+  HIPCHK(hipMemcpyAsync(S.t, e->t, (size_t)T_COUNT * np * e->tsz, hipMemcpyDeviceToDevice, e->s));
+  HIPCHK(hipMemcpyAsync(S.d, e->d, (size_t)D_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
+  HIPCHK(hipMemcpyAsync(S.i, e->ii, (size_t)I_COUNT * np * sizeof(int), hipMemcpyDeviceToDevice, e->s));
+  HIPCHK(hipMemcpyAsync(S.ts, e->d_ts, sizeof(InstT) * e->max_inst, hipMemcpyDeviceToDevice, e->s));
+  S.steps = e->steps_done;
+  S.flags.clear();
+  for (const HostInst& h : e->inst) S.flags.emplace_back(h.p_in_p2, h.xx_pending);
+}
+
+static void snap_restore(mpm_engine* e, const Snapshot& S) {
+  const long np = e->np;
+  HIPCHK(hipMemcpyAsync(e->t, S.t, (size_t)T_COUNT * np * e->tsz, hipMemcpyDeviceToDevice, e->s));
+  HIPCHK(hipMemcpyAsync(e->d, S.d, (size_t)D_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
+  HIPCHK(hipMemcpyAsync(e->ii, S.i, (size_t)I_COUNT * np * sizeof(int), hipMemcpyDeviceToDevice, e->s));
+  HIPCHK(hipMemcpyAsync(e->d_ts, S.ts, sizeof(InstT) * e->max_inst, hipMemcpyDeviceToDevice, e->s));
+  e->steps_done = S.steps;
+  for (size_t k = 0; k < e->inst.size() && k < S.flags.size(); k++) {
+    e->inst[k].p_in_p2 = S.flags[k].first;
+    e->inst[k].xx_pending = S.flags[k].second;
+  }
+}
+
+static void apply_tuning(mpm_engine* e, const mpm_tuning& t) {
+  auto ok = [](int b) { return b == 64 || b == 128 || b == 256; };
+  if (!ok(t.block_particles) || !ok(t.block_nodes) || t.sort_every < 0 ||
+      !(t.sort_brick == 1 || t.sort_brick == 2 || t.sort_brick == 4 || t.sort_brick == 8))
+    fail(MPM_ERR_ARG, "bad tuning values");
+  e->bs_p = t.block_particles;
+  e->bs_n = t.block_nodes;
+  e->sort_every = t.sort_every;
+  e->brick = t.sort_brick;
+  e->ilv = t.interleave != 0;
+  e->graph_dirty = true;
+}
+
+static mpm_tuning current_tuning(const mpm_engine* e) {
+  mpm_tuning t{};
+  t.block_particles = e->bs_p;
+  t.block_nodes = e->bs_n;
+  t.sort_every = e->sort_every;
+  t.sort_brick = e->brick;
+  t.interleave = e->ilv;
+  return t;
+}
+
+// mean milliseconds of one graph-replayed step over n steps, no re-sort
+static double time_trial(mpm_engine* e, int n, std::vector<float>* each = nullptr) {
+  const int se = e->sort_every;
+  e->sort_every = 0;
+  step_graph(e, 1, 1, FIN_DONE);   // capture (if needed) + warm-up
+  std::vector<hipEvent_t> ev(each ? n + 1 : 2);
+  for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+  HIPCHK(hipEventRecord(ev[0], e->s));
+  for (int k = 0; k < n; k++) {
+    step_graph(e, 1, 1, FIN_DONE);
+    if (each) HIPCHK(hipEventRecord(ev[k + 1], e->s));
+  }
+  if (!each) HIPCHK(hipEventRecord(ev[1], e->s));
   HIPCHK(hipStreamSynchronize(e->s));
-  HIPCHK(hipFree(d_b));
-  HIPCHK(hipFree(d_e));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, ev[0], ev.back()));
+  if (each) {
+    each->resize(n);
+    for (int k = 0; k < n; k++) HIPCHK(hipEventElapsedTime(&(*each)[k], ev[k], ev[k + 1]));
+  }
+  for (auto& x : ev) HIPCHK(hipEventDestroy(x));
+  e->sort_every = se;
+  return ms / n;
+}
+
+static double time_sort(mpm_engine* e) {
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipEventRecord(a, e->s));
+  sort_particles(e);
+  HIPCHK(hipEventRecord(b, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, a, b));
+  HIPCHK(hipEventDestroy(a));
+  HIPCHK(hipEventDestroy(b));
+  return ms;
+}
+
+// Coordinate search over the launch shape on the resident workload:
+// (1) sort-brick edge x particle block size x lane mapping, timed right
+// after a re-sort; (2) node block size; (3) re-sort period, from the
+// measured cost of one sort and the measured growth of the step time as
+// particles drift from their sorted order: the period P minimising
+// (sort + sum of the first P step times) / P.  Every trial starts from a
+// snapshot of the state, which is restored at the end.
+static mpm_tuning autotune(mpm_engine* e, int trial_steps) {
+  rebuild_lists(e);
+  if (e->n_active == 0) fail(MPM_ERR_ARG, "autotune: no instances");
+  if (trial_steps < 1) trial_steps = 1;
+  Snapshot S;
+  mpm_tuning best = current_tuning(e);
+  const bool was_prof = e->prof;
+  e->prof = false;
+  int trials = 0;
+  try {
+    snap_take(e, S);
+    double best_ms = 1e300;
+    auto trial = [&](const mpm_tuning& t) {
+      snap_restore(e, S);
+      apply_tuning(e, t);
+      sort_particles(e);
+      const double ms = time_trial(e, trial_steps);
+      trials++;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    };
+    mpm_tuning t = best;
+    t.block_nodes = 256;
+    for (int B : {2, 4, 8})
+      for (int bs : {64, 128, 256})
+        for (int il : {0, 1}) {
+          t.sort_brick = B;
+          t.block_particles = bs;
+          t.interleave = il;
+          trial(t);
+        }
+    t = best;
+    for (int bn : {64, 128}) {
+      t.block_nodes = bn;
+      trial(t);
+    }
+    // re-sort period
+    snap_restore(e, S);
+    apply_tuning(e, best);
+    const double sort_ms = time_sort(e);
+    const int K = 64;
+    std::vector<float> each;
+    time_trial(e, K, &each);
+    double acc = sort_ms, best_cost = 1e300;
+    int best_p = K;
+    for (int P = 1; P <= K; P++) {
+      acc += each[P - 1];
+      if ((P & (P - 1)) == 0 || P == 10 || P == 20 || P == 40) {   // 1,2,4,8,10,16,20,32,40,64
+        const double c = acc / P;
+        if (c < best_cost) {
+          best_cost = c;
+          best_p = P;
+        }
+      }
+    }
+    best.sort_every = best_p;
+    best.step_ms = best_ms;
+    best.sort_ms = sort_ms;
+    best.trials = trials + 1;
+    snap_restore(e, S);
+    apply_tuning(e, best);
+    HIPCHK(hipStreamSynchronize(e->s));
+  } catch (...) {
+    e->prof = was_prof;
+    (void)hipStreamSynchronize(e->s);
+    snap_free(S);
+    throw;
+  }
+  e->prof = was_prof;
+  snap_free(S);
+  return best;
 }
 
 // --------------------------------------------------------- downloads
@@ -1279,6 +1483,30 @@ extern "C" int mpm_engine_set_tuning(mpm_engine* e, int block_particles, int blo
     e->bs_n = block_nodes;
     e->sort_every = sort_every;
     e->graph_dirty = true;
+  });
+}
+
+extern "C" int mpm_engine_get_tuning(mpm_engine* e, mpm_tuning* out) {
+  return guarded([&] {
+    check_ctx(e);
+    if (!out) fail(MPM_ERR_ARG, "out is NULL");
+    *out = current_tuning(e);
+  });
+}
+
+extern "C" int mpm_engine_apply_tuning(mpm_engine* e, const mpm_tuning* t) {
+  return guarded([&] {
+    check_ctx(e);
+    if (!t) fail(MPM_ERR_ARG, "tuning is NULL");
+    apply_tuning(e, *t);
+  });
+}
+
+extern "C" int mpm_engine_autotune(mpm_engine* e, int trial_steps, mpm_tuning* best) {
+  return guarded([&] {
+    check_ctx(e);
+    const mpm_tuning t = autotune(e, trial_steps);
+    if (best) *best = t;
   });
 }
 

@@ -22,9 +22,6 @@
 #define ST_FORCE 2    // USF: force scatter with the updated stress
 #define ST_ENERGY 4   // energy partials
 
-// edge of the cubic cell bricks of the particle sort key
-#define SORT_BRICK 4
-
 // k_finalize parts
 #define FIN_ENERGY 1
 #define FIN_DT 2
@@ -54,6 +51,6 @@ void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int bpg,
 void launch_begin_step(InstT* ts, const int* ilist, int n, hipStream_t s);
 template <typename S>
 void launch_permute(S* dst, const S* src, const int* perm, const LaunchP& L, long np, int nfield, hipStream_t s);
-void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP& L, unsigned* keys, int* vals,
-                      hipStream_t s);
+void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP& L, int brick, unsigned* keys,
+                      int* vals, hipStream_t s);
 }  // namespace mpmk

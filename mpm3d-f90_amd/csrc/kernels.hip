@@ -1250,30 +1250,29 @@ __global__ void __launch_bounds__(256) k_permute(S* dst, const S* src, const int
   for (int f = 0; f < nfield; f++) dst[(long)f * np + j] = src[(long)f * np + from];
 }
 
-// Sort key of a particle: its cell in brick-major order (bricks of
-// SORT_BRICK^3 cells, cells x-fastest inside a brick), so that a block of
-// consecutive particles covers a compact box of nodes.  The cell is located
-// from the current position (InWhichCell); particles outside the grid and
-// padding slots get the largest key of their instance.
-__device__ __forceinline__ unsigned brick_key(const InstC& I, int c) {
+// Sort key of a particle: its cell in brick-major order (bricks of B^3
+// cells, cells x-fastest inside a brick), so that a block of consecutive
+// particles covers a compact box of nodes.  The cell is located from the
+// current position (InWhichCell); particles outside the grid and padding
+// slots get the largest key of their instance.  B = 1 is the plain cell order.
+__device__ __forceinline__ unsigned brick_key(const InstC& I, int c, int B) {
   int cx, cy, cz;
   cell_coords(I, c, cx, cy, cz);
-  const int nbx = (I.ncx + SORT_BRICK - 1) / SORT_BRICK, nby = (I.ncy + SORT_BRICK - 1) / SORT_BRICK;
-  const unsigned brick = (unsigned)(((cz / SORT_BRICK) * nby + cy / SORT_BRICK) * nbx + cx / SORT_BRICK);
-  const unsigned local = (unsigned)(((cz % SORT_BRICK) * SORT_BRICK + cy % SORT_BRICK) * SORT_BRICK + cx % SORT_BRICK);
-  return brick * (SORT_BRICK * SORT_BRICK * SORT_BRICK) + local;
+  const int nbx = (I.ncx + B - 1) / B, nby = (I.ncy + B - 1) / B;
+  const unsigned brick = (unsigned)(((cz / B) * nby + cy / B) * nbx + cx / B);
+  const unsigned local = (unsigned)(((cz % B) * B + cy % B) * B + cx % B);
+  return brick * (unsigned)(B * B * B) + local;
 }
 
 __global__ void __launch_bounds__(256) k_sort_keys(const double* X, long np, const InstC* inst, const int* glist,
-                                                   const int* ginst, int bpg, unsigned* keys, int* vals) {
+                                                   const int* ginst, int bpg, int B, unsigned* keys, int* vals) {
   int gran;
   const long j = part_index(0, glist, bpg, gran);
   const InstC& I = inst[ginst[gran]];
   int c = -1;
   if (j < I.p_end) c = in_which_cell(I, X[j], X[np + j], X[2 * np + j]);
-  const int nbx = (I.ncx + SORT_BRICK - 1) / SORT_BRICK, nby = (I.ncy + SORT_BRICK - 1) / SORT_BRICK;
-  const int nbz = (I.ncz + SORT_BRICK - 1) / SORT_BRICK;
-  keys[j] = c > 0 ? brick_key(I, c) : (unsigned)nbx * nby * nbz * (SORT_BRICK * SORT_BRICK * SORT_BRICK);
+  const int nbx = (I.ncx + B - 1) / B, nby = (I.ncy + B - 1) / B, nbz = (I.ncz + B - 1) / B;
+  keys[j] = c > 0 ? brick_key(I, c, B) : (unsigned)nbx * nby * nbz * (unsigned)(B * B * B);
   vals[j] = (int)j;
 }
 
@@ -1394,10 +1393,11 @@ void launch_permute(S* dst, const S* src, const int* perm, const LaunchP& L, lon
   if (L.nblk) hipLaunchKernelGGL(k_permute<S>, dim3(L.nblk), dim3(L.bs), 0, s, dst, src, perm, L.glist, L.bpg, np, nfield);
 }
 
-void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP& L, unsigned* keys, int* vals,
-                      hipStream_t s) {
+void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP& L, int brick, unsigned* keys,
+                      int* vals, hipStream_t s) {
   if (L.nblk)
-    hipLaunchKernelGGL(k_sort_keys, dim3(L.nblk), dim3(L.bs), 0, s, X, np, inst, L.glist, L.ginst, L.bpg, keys, vals);
+    hipLaunchKernelGGL(k_sort_keys, dim3(L.nblk), dim3(L.bs), 0, s, X, np, inst, L.glist, L.ginst, L.bpg, brick, keys,
+                       vals);
 }
 
 #define INST(T)                                                                                     \

@@ -119,9 +119,15 @@ def slope_ensemble(count, seed=0, n_min=2e4, n_max=4e5, **kw):
         t = max(0.1, min(0.5, target / (base_area * 8000)))
         scale = math.sqrt(max(target / (t * 8000 * base_area), 0.05))
         L, H, B = 40.0 * scale, 20.0 * scale, 5.0 * scale
-        text, n = slope(length=round(L, 1), height=round(H, 1), base=round(B, 1), angle_deg=ang,
-                        thickness=round(t, 1), c=coh, phi=phi, toe=round(10.0 * scale, 1), **kw)
-        out.append((text, n, {"angle": ang, "phi": phi, "c": coh}))
+        L, H, B, t = round(L, 1), round(H, 1), round(B, 1), round(t, 1)
+        text, n = slope(length=L, height=H, base=B, angle_deg=ang, thickness=t, c=coh, phi=phi,
+                        toe=round(10.0 * scale, 1), **kw)
+        h = kw.get("h", 0.1)
+        # grid nodes as SetGridData counts them (Grid.f90:250-274), spans as in slope()
+        cells = [int((L + 2 * h) / h + 0.5), int((t + 2 * h) / h + 0.5), int((H + 6 * h) / h + 0.5)]
+        nodes = (cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1)
+        out.append((text, n, {"angle": ang, "phi": phi, "c": coh, "nodes": nodes,
+                              "bodies": text.count("\nparticle ")}))
     return out
 
 

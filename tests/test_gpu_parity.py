@@ -198,11 +198,14 @@ def test_graph_replay_matches_plain_steps(mpm):
     b.close()
 
 
-@pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "slope_dp_bspline_usf"])
-def test_sort_is_the_stable_cell_order(mpm, oracle_mod, name):
+@pytest.mark.parametrize("name,brick", [("bar_j2_gimp_musl", 4), ("slope_dp_bspline_usf", 4),
+                                        ("slope_dp_bspline_usf", 1), ("bar_j2_gimp_musl", 2),
+                                        ("plate_neoh_musl_qulo", 8)])
+def test_sort_is_the_stable_cell_order(mpm, oracle_mod, name, brick):
     p = mpm.read_deck(deck_path(name))
     eng = mpm.Engine(max_particles=p.n + 256, max_nodes=p.nb_node + 256, max_instances=2)
     i = eng.add(p)
+    eng.apply_tuning(sort_brick=brick)
     ref = mpm.Engine(max_particles=p.n + 256, max_nodes=p.nb_node + 256, max_instances=2)
     j = ref.add(p)
     eng.step(3)
@@ -210,8 +213,8 @@ def test_sort_is_the_stable_cell_order(mpm, oracle_mod, name):
     order0 = eng.order(i)
     ps = eng.particles(i)
     eng.sort()
-    # brick-major cell key (bricks of 4^3 cells), stable in the current slot order
-    key = brick_key(p, oracle_cells(p, ps))
+    # brick-major cell key (bricks of B^3 cells), stable in the current slot order
+    key = brick_key(p, oracle_cells(p, ps), brick)
     expect = order0[np.argsort(key[order0], kind="stable")]
     assert np.array_equal(eng.order(i), expect)
     eng.step(5)
@@ -241,6 +244,39 @@ def brick_key(p, c, B=4):
     nbx, nby, nbz = -(-p.ncx // B), -(-p.ncy // B), -(-p.ncz // B)
     key = (((cz // B) * nby + cy // B) * nbx + cx // B) * B ** 3 + ((cz % B) * B + cy % B) * B + cx % B
     return np.where(c > 0, key, nbx * nby * nbz * B ** 3)
+
+
+def test_autotune_leaves_the_state_and_picks_valid_shapes(mpm):
+    """The autotuner times trial steps from a snapshot and restores it: the
+    run continues exactly as an untuned engine's (to FP64 round-off, since
+    the chosen launch shape changes the scatter order)."""
+    names = ["slope_dp_bspline_usf", "bar_j2_gimp_musl"]
+    probs = [mpm.read_deck(deck_path(n)) for n in names]
+    np_ = sum(p.n for p in probs) + 8 * 256
+    nn_ = sum(p.nb_node for p in probs) + 8 * 256
+    a = mpm.Engine(max_particles=np_, max_nodes=nn_, max_instances=4)
+    b = mpm.Engine(max_particles=np_, max_nodes=nn_, max_instances=4)
+    ia = [a.add(p) for p in probs]
+    ib = [b.add(p) for p in probs]
+    a.step(3)
+    b.step(3)
+    before = [a.particles(x) for x in ia]
+    t = a.autotune(trial_steps=2)
+    assert t["block_particles"] in (64, 128, 256) and t["block_nodes"] in (64, 128, 256)
+    assert t["sort_brick"] in (2, 4, 8) and t["interleave"] in (0, 1)
+    assert 1 <= t["sort_every"] <= 64 and t["trials"] == 18 + 2 + 1
+    assert t["step_ms"] > 0 and t["sort_ms"] > 0
+    assert a.tuning()["sort_brick"] == t["sort_brick"]
+    for x, p0 in zip(ia, before):
+        assert a.scalars(x)[0] == 3
+        assert np.array_equal(a.particles(x), p0)
+    a.step(6, graph=True, graph_steps=2)
+    b.step(6)
+    for x, y, n in zip(ia, ib, names):
+        assert a.scalars(x)[0] == 9
+        assert_close(perr(mpm, a.particles(x), b.particles(y)), 1e-12, None, n + " tuned")
+    a.close()
+    b.close()
 
 
 def test_instances_stop_at_their_own_endtime(mpm):
