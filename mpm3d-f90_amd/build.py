@@ -4,7 +4,9 @@
 
 HIP sources are compiled with hipcc --offload-arch=gfx950; the deck reader
 (host only) with g++.  -ffp-contract=off keeps the FP64 path free of FMA
-contraction so its per-particle arithmetic rounds like the reference's.
+contraction so its per-particle arithmetic rounds like the reference's; the
+FP32-bulk kernels are a second object of the same source with contraction
+allowed.
 Packed-FP32 instruction selection is switched off for all device code.
 """
 import argparse
@@ -27,7 +29,10 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contra
              "-Wno-unused-command-line-argument", "-I" + INC, "-I" + CSRC]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-I" + INC, "-I" + CSRC]
 
-HIP_SOURCES = ["kernels.hip", "engine.hip"]
+# (source, object, extra flags): kernels.hip is compiled once per precision
+HIP_SOURCES = [("kernels.hip", "kernels_f64.o", ["-DMPMK_PART=1"]),
+               ("kernels.hip", "kernels_f32.o", ["-DMPMK_PART=2", "-ffp-contract=fast"]),
+               ("engine.hip", "engine.o", [])]
 CXX_SOURCES = ["deck.cpp"]
 
 
@@ -43,9 +48,9 @@ def build(jobs=4, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     jobs_list = []
     objs = []
-    for s in HIP_SOURCES:
-        o = os.path.join(BUILD, s.replace(".hip", ".o"))
-        jobs_list.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
+    for s, obj, extra in HIP_SOURCES:
+        o = os.path.join(BUILD, obj)
+        jobs_list.append([HIPCC] + HIP_FLAGS + extra + ["-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     for s in CXX_SOURCES:
         o = os.path.join(BUILD, s.replace(".cpp", ".o"))

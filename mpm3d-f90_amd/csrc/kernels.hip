@@ -14,6 +14,14 @@
 #include "engine.h"
 #include "kernels.h"
 
+// This file is compiled twice: MPMK_PART 1 instantiates the FP64 path (and
+// the precision-independent kernels) without FMA contraction, so its
+// per-particle arithmetic rounds like the reference's; MPMK_PART 2 the
+// FP32-bulk path with contraction allowed.
+#ifndef MPMK_PART
+#define MPMK_PART 0
+#endif
+
 namespace mpmk {
 
 // ------------------------------------------------------------ utilities
@@ -332,6 +340,7 @@ __device__ __forceinline__ void particle_energy(const PV<T>& p, double v[6]) {
 
 struct Box {
   int x0, y0, z0, bx, by, bz, n, ok;
+  int inner;   // box not clamped at a grid face: every stencil node is a box node
   __device__ __forceinline__ int idx(int nx, int ny, int nz) const {
     const int ex = nx - x0, ey = ny - y0, ez = nz - z0;
     if ((unsigned)ex >= (unsigned)bx || (unsigned)ey >= (unsigned)by || (unsigned)ez >= (unsigned)bz) return -1;
@@ -375,7 +384,10 @@ __device__ __forceinline__ Box block_box(const InstC& I, bool act, int cx, int c
       b.bx = b.by = b.bz = 0;
       b.n = 0;
       b.ok = 1;
+      b.inner = 0;
     } else {
+      b.inner = sb[0][0] + lo >= 0 && sb[1][0] + lo >= 0 && sb[2][0] + lo >= 0 && sb[3][0] + hi <= I.ncx &&
+                sb[4][0] + hi <= I.ncy && sb[5][0] + hi <= I.ncz;
       b.x0 = max(sb[0][0] + lo, 0);
       b.y0 = max(sb[1][0] + lo, 0);
       b.z0 = max(sb[2][0] + lo, 0);
@@ -390,6 +402,208 @@ __device__ __forceinline__ Box block_box(const InstC& I, bool act, int cx, int c
   }
   __syncthreads();
   return res;
+}
+
+// ===================================================== tile-frame slots
+//
+// In a block whose box is inner, every particle's influence nodes are the
+// NS^3 slots e0 + a + b*bx + c*bx*by of the box (NS = 2 linear, 3 GIMP and
+// B-spline; GIMP slots outside the particle's 2- or 3-node case carry zero
+// weight, so they add exact zeros).  Weights factor per axis:
+//   N = w[0][a] w[1][b] w[2][c],  dN/dx = d[0][a] w[1][b] w[2][c], ...
+// so gathers are evaluated by sum factorisation (axis by axis) and the
+// block-uniform strides keep the per-node index work to one add.  The sums
+// associate differently from the reference's node-by-node loop (round-off
+// level differences, like the scatter order).
+
+template <int SHAPE>
+struct NSlot {
+  static constexpr int n = SHAPE == MPM_SHAPE_LINEAR ? 2 : 3;
+};
+
+template <typename T>
+struct Slots {
+  T w[3][3];
+  T d[3][3];
+  int e0;
+};
+
+template <int SHAPE, typename T>
+__device__ __forceinline__ void make_slots(const InstC& I, const Box& B, double x, double y, double z, int c,
+                                           Slots<T>& S) {
+  Stencil<T> st;
+  build_stencil<SHAPE, T>(I, x, y, z, c, st);
+  // linear: N = s_x s_y s_z / 8 with s = 1 -+ xi, dN/dx = +-s_y s_z ijac/8
+  const T ws = SHAPE == MPM_SHAPE_LINEAR ? (T)0.5 : (T)1;
+  const T ds = SHAPE == MPM_SHAPE_LINEAR ? (T)(4.0 * I.ijac4) : SHAPE == MPM_SHAPE_GIMP ? (T)I.idcell : (T)1;
+#pragma unroll
+  for (int ax = 0; ax < 3; ax++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      S.w[ax][k] = st.w[ax][k] * ws;
+      S.d[ax][k] = st.d[ax][k] * ds;
+    }
+  S.e0 = (st.base[0] - B.x0) + B.bx * ((st.base[1] - B.y0) + B.by * (st.base[2] - B.z0));
+}
+
+// gather tiles in the fast path: NF fields per node, node-major with a
+// 4-element stride per group of up to 4 fields (one 16 B LDS read for FP32)
+template <typename T, int NF>
+struct AosTile {
+  static constexpr int G = (NF + 3) / 4;   // 4-field groups per node
+  static constexpr int S = 4 * G;          // elements per node
+};
+
+template <typename T, int NF>
+__device__ __forceinline__ void load_tile_aos(const InstC& I, const Box& B, T* tf, const T* const* src, long nn) {
+  constexpr int S = AosTile<T, NF>::S;
+  for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
+    const int k = B.node(I, e);
+#pragma unroll
+    for (int f = 0; f < NF; f++) tf[e * S + f] = src[f / 3][(f % 3) * nn + k];
+  }
+}
+
+template <typename T, int NF>
+__device__ __forceinline__ void read_node(const T* tf, int e, T v[NF]) {
+  constexpr int S = AosTile<T, NF>::S;
+  const T* q = tf + e * S;
+  if (sizeof(T) == 4) {
+#pragma unroll
+    for (int g = 0; g < (NF + 3) / 4; g++) {
+      const float4 u = *(const float4*)(q + 4 * g);
+      const float uu[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+      for (int f = 0; f < 4; f++)
+        if (4 * g + f < NF) v[4 * g + f] = (T)uu[f];
+    }
+  } else {
+#pragma unroll
+    for (int g = 0; g < (NF + 1) / 2; g++) {
+      const double2 u = *(const double2*)(q + 2 * g);
+      if (2 * g < NF) v[2 * g] = (T)u.x;
+      if (2 * g + 1 < NF) v[2 * g + 1] = (T)u.y;
+    }
+  }
+}
+
+// sum_n N_n v_n for NF fields
+template <int NS, typename T, int NF>
+__device__ __forceinline__ void gather_w(const Slots<T>& S, const T* tf, int sb, int sc, T out[NF]) {
+#pragma unroll
+  for (int f = 0; f < NF; f++) out[f] = (T)0;
+#pragma unroll
+  for (int c = 0; c < NS; c++) {
+    T t[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) t[f] = (T)0;
+#pragma unroll
+    for (int b = 0; b < NS; b++) {
+      const int eb = S.e0 + b * sb + c * sc;
+      T u[NF];
+#pragma unroll
+      for (int f = 0; f < NF; f++) u[f] = (T)0;
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        T v[NF];
+        read_node<T, NF>(tf, eb + a, v);
+#pragma unroll
+        for (int f = 0; f < NF; f++) u[f] += S.w[0][a] * v[f];
+      }
+#pragma unroll
+      for (int f = 0; f < NF; f++) t[f] += S.w[1][b] * u[f];
+    }
+#pragma unroll
+    for (int f = 0; f < NF; f++) out[f] += S.w[2][c] * t[f];
+  }
+}
+
+// velocity gradient L[i][j] = sum_n v_n,i dN_n/dx_j
+template <int NS, typename T>
+__device__ __forceinline__ void gather_grad(const Slots<T>& S, const T* tf, int sb, int sc, T L[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) L[i][j] = (T)0;
+#pragma unroll
+  for (int c = 0; c < NS; c++) {
+    T tx[3] = {(T)0, (T)0, (T)0}, ty[3] = {(T)0, (T)0, (T)0}, tw[3] = {(T)0, (T)0, (T)0};
+#pragma unroll
+    for (int b = 0; b < NS; b++) {
+      const int eb = S.e0 + b * sb + c * sc;
+      T sx[3] = {(T)0, (T)0, (T)0}, sw[3] = {(T)0, (T)0, (T)0};
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        T v[3];
+        read_node<T, 3>(tf, eb + a, v);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          sx[i] += S.d[0][a] * v[i];
+          sw[i] += S.w[0][a] * v[i];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        tx[i] += S.w[1][b] * sx[i];
+        ty[i] += S.d[1][b] * sw[i];
+        tw[i] += S.w[1][b] * sw[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      L[i][0] += S.w[2][c] * tx[i];
+      L[i][1] += S.w[2][c] * ty[i];
+      L[i][2] += S.d[2][c] * tw[i];
+    }
+  }
+}
+
+// scatter of mass (FP64) and NV vector fields q_f * N into FP64 tile
+// accumulators (field-major, stride cap)
+template <int NS, typename T, bool MASS, int NV>
+__device__ __forceinline__ void scatter_w(const Slots<T>& S, int sb, int sc, T m, const T q[3], double* tm,
+                                          double* tv, int cap) {
+#pragma unroll
+  for (int c = 0; c < NS; c++)
+#pragma unroll
+    for (int b = 0; b < NS; b++) {
+      const T mw = m * (S.w[1][b] * S.w[2][c]);
+      const int eb = S.e0 + b * sb + c * sc;
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        const T shm = S.w[0][a] * mw;
+        const int e = eb + a;
+        if (MASS) atomicAdd(&tm[e], (double)shm);
+#pragma unroll
+        for (int f = 0; f < NV; f++) atomicAdd(&tv[f * cap + e], (double)(q[f] * shm));
+      }
+    }
+}
+
+// nodal force f_n = -(sigma . grad N_n) vol + fext N_n into FP64 tile
+// accumulators; sv[i][j] = -vol sigma_ij
+template <int NS, typename T>
+__device__ __forceinline__ void scatter_force(const Slots<T>& S, int sb, int sc, const T sv[3][3], const T fx[3],
+                                              double* tv, int cap) {
+#pragma unroll
+  for (int c = 0; c < NS; c++)
+#pragma unroll
+    for (int b = 0; b < NS; b++) {
+      const T wyz = S.w[1][b] * S.w[2][c], dyz = S.d[1][b] * S.w[2][c], wdz = S.w[1][b] * S.d[2][c];
+      T A[3], Q[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        A[i] = sv[i][0] * wyz;
+        Q[i] = sv[i][1] * dyz + sv[i][2] * wdz + fx[i] * wyz;
+      }
+      const int eb = S.e0 + b * sb + c * sc;
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        const int e = eb + a;
+#pragma unroll
+        for (int i = 0; i < 3; i++) atomicAdd(&tv[i * cap + e], (double)(S.d[0][a] * A[i] + S.w[0][a] * Q[i]));
+      }
+    }
 }
 
 // ================================================================ kernels
@@ -441,8 +655,6 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
   T* Pg = ((MODE & P2G_TO_P2) ? P.p2 : P.pg) + I.node_off;
   T* Fg = P.fg + I.node_off;
   if (act) {
-    Stencil<T> S;
-    build_stencil<SHAPE, T>(I, x, y, z, c, S);
     T m = (T)0, vx = (T)0, vy = (T)0, vz = (T)0;
     if (MASS || MOM) {
       m = p.t(T_MASS);
@@ -457,6 +669,24 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
       sxy = p.t(T_SD5); syz = p.t(T_SD3); sxz = p.t(T_SD4);
       vol = (T)p.d(D_VOL);
     }
+    if (B.ok && B.inner) {
+      constexpr int NS = NSlot<SHAPE>::n;
+      Slots<T> S;
+      make_slots<SHAPE, T>(I, B, x, y, z, c, S);
+      const int sb = B.bx, sc = B.bx * B.by;
+      if (MASS || MOM) {
+        const T q[3] = {vx, vy, vz};
+        scatter_w<NS, T, MASS, MOM ? 3 : 0>(S, sb, sc, m, q, tm, tf, CAP);
+      }
+      if (FRC) {
+        const T sv[3][3] = {{-(sxx * vol), -(sxy * vol), -(sxz * vol)},
+                            {-(sxy * vol), -(syy * vol), -(syz * vol)},
+                            {-(sxz * vol), -(syz * vol), -(szz * vol)}};
+        scatter_force<NS, T>(S, sb, sc, sv, fx, tf + (MOM ? 3 : 0) * CAP, CAP);
+      }
+    } else {
+    Stencil<T> S;
+    build_stencil<SHAPE, T>(I, x, y, z, c, S);
     for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T dX, T dY, T dZ) {
       const int e = B.ok ? B.idx(nx, ny, nz) : -1;
       T val[6];
@@ -479,6 +709,7 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
         if (FRC) { atomic_add(&Fg[k], val[q]); atomic_add(&Fg[nn + k], val[q + 1]); atomic_add(&Fg[2 * nn + k], val[q + 2]); }
       }
     });
+    }
   }
   if (B.ok && B.n) {
     __syncthreads();
@@ -633,8 +864,9 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
   const InstC& I = P.inst[ii];
   const InstT& ts = P.ts[ii];
   if (ts.done) return;
-  constexpr int GB = 6 * (int)sizeof(T);   // V, A per node
-  constexpr int CAP = TILE_BYTES / (GB > 24 ? GB : 24);
+  constexpr int NS = NSlot<SHAPE>::n;
+  constexpr int NB = AosTile<T, 6>::S * (int)sizeof(T);   // V, A per node (fast-path layout)
+  constexpr int CAP = TILE_BYTES / (NB > 24 ? NB : 24);
   __shared__ __align__(16) char smem[TILE_BYTES];
   T* tf = (T*)smem;
   double* ta = (double*)smem;   // re-map accumulation, reuses the gather tile
@@ -645,31 +877,46 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
   int cx = 0, cy = 0, cz = 0;
   if (act) cell_coords(I, c, cx, cy, cz);
   const Box B = block_box<SHAPE>(I, act, cx, cy, cz, CAP);
+  const bool fast = B.ok && B.inner;
+  const int sb = B.bx, sc = B.bx * B.by;
   const long nn = P.nn;
   const T* Vg = P.vg + I.node_off;
   const T* Ag = P.ag + I.node_off;
   if (B.ok && B.n) {
     const T* srcs[2] = {Vg, Ag};
-    load_tile<T, 6>(I, B, tf, CAP, srcs, nn);
+    if (fast) load_tile_aos<T, 6>(I, B, tf, srcs, nn);
+    else load_tile<T, 6>(I, B, tf, CAP, srcs, nn);
     __syncthreads();
   }
   double ev[6] = {0, 0, 0, 0, 0, 0};
   Stencil<T> S;
+  Slots<T> F;
   T vn[3] = {(T)0, (T)0, (T)0};
   const double x = act ? p.d(D_X) : 0.0, y = act ? p.d(D_Y) : 0.0, z = act ? p.d(D_Z) : 0.0;
   if (act) {
-    build_stencil<SHAPE, T>(I, x, y, z, c, S);
     T vx[3] = {(T)0, (T)0, (T)0}, ax[3] = {(T)0, (T)0, (T)0};
-    for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) {
-      const int e = B.ok ? B.idx(nx, ny, nz) : -1;
+    if (fast) {
+      make_slots<SHAPE, T>(I, B, x, y, z, c, F);
+      T va[6];
+      gather_w<NS, T, 6>(F, tf, sb, sc, va);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        const T v = e >= 0 ? tf[d * CAP + e] : Vg[d * nn + k];
-        const T a = e >= 0 ? tf[(3 + d) * CAP + e] : Ag[d * nn + k];
-        vx[d] = vx[d] + N * v;
-        ax[d] = ax[d] + N * a;
+        vx[d] = va[d];
+        ax[d] = va[3 + d];
       }
-    });
+    } else {
+      build_stencil<SHAPE, T>(I, x, y, z, c, S);
+      for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) {
+        const int e = B.ok ? B.idx(nx, ny, nz) : -1;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const T v = e >= 0 ? tf[d * CAP + e] : Vg[d * nn + k];
+          const T a = e >= 0 ? tf[(3 + d) * CAP + e] : Ag[d * nn + k];
+          vx[d] = vx[d] + N * v;
+          ax[d] = ax[d] + N * a;
+        }
+      });
+    }
     const double dt = ts.dt;
     const T tdt = (T)dt;
 #pragma unroll
@@ -694,17 +941,21 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
     }
     if (act) {
       const T m = p.t(T_MASS);
-      for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) {
-        const T shm = N * m;
-        const int e = B.ok ? B.idx(nx, ny, nz) : -1;
-        if (e >= 0) {
+      if (fast) {
+        scatter_w<NS, T, false, 3>(F, sb, sc, m, vn, nullptr, ta, CAP);
+      } else {
+        for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) {
+          const T shm = N * m;
+          const int e = B.ok ? B.idx(nx, ny, nz) : -1;
+          if (e >= 0) {
 #pragma unroll
-          for (int d = 0; d < 3; d++) atomicAdd(&ta[d * CAP + e], (double)(vn[d] * shm));
-        } else {
+            for (int d = 0; d < 3; d++) atomicAdd(&ta[d * CAP + e], (double)(vn[d] * shm));
+          } else {
 #pragma unroll
-          for (int d = 0; d < 3; d++) atomic_add(&P2[d * nn + k], vn[d] * shm);
-        }
-      });
+            for (int d = 0; d < 3; d++) atomic_add(&P2[d * nn + k], vn[d] * shm);
+          }
+        });
+      }
     }
     if (B.ok && B.n) {
       __syncthreads();
@@ -1033,8 +1284,9 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
   const InstC& I = P.inst[ii];
   const InstT& ts = P.ts[ii];
   if (ts.done) return;
-  constexpr int GB = 3 * (int)sizeof(T);   // nodal velocity per node
-  constexpr int CAP = TILE_BYTES / (GB > 24 ? GB : 24);
+  constexpr int NS = NSlot<SHAPE>::n;
+  constexpr int NB = AosTile<T, 3>::S * (int)sizeof(T);   // nodal velocity per node (fast-path layout)
+  constexpr int CAP = TILE_BYTES / (NB > 24 ? NB : 24);
   __shared__ __align__(16) char smem[TILE_BYTES];
   T* tf = (T*)smem;
   double* ta = (double*)smem;   // force accumulation after the gather
@@ -1045,21 +1297,38 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
   int cx = 0, cy = 0, cz = 0;
   if (act) cell_coords(I, c, cx, cy, cz);
   const Box B = block_box<SHAPE>(I, act, cx, cy, cz, CAP);
+  const bool fast = B.ok && B.inner;
+  const int sb = B.bx, sc = B.bx * B.by;
   const long nn = P.nn;
   const T* Vg = P.vg + I.node_off;
   if (B.ok && B.n) {
     const T* srcs[1] = {Vg};
-    load_tile<T, 3>(I, B, tf, CAP, srcs, nn);
+    if (fast) load_tile_aos<T, 3>(I, B, tf, srcs, nn);
+    else load_tile<T, 3>(I, B, tf, CAP, srcs, nn);
     __syncthreads();
   }
   double ev[6] = {0, 0, 0, 0, 0, 0};
   Stencil<T> S;
+  Slots<T> FS;
   const double x = act ? p.d(D_X) : 0.0, y = act ? p.d(D_Y) : 0.0, z = act ? p.d(D_Z) : 0.0;
   if (act) {
-    build_stencil<SHAPE, T>(I, x, y, z, c, S);
     T de[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, vort[3] = {(T)0, (T)0, (T)0};
     T L[3][3] = {{(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}};
     const bool uvfp = (I.flags & IF_UVFP) != 0;
+    if (fast) {
+      make_slots<SHAPE, T>(I, B, x, y, z, c, FS);
+      gather_grad<NS, T>(FS, tf, sb, sc, L);
+      de[0] = L[0][0];
+      de[1] = L[1][1];
+      de[2] = L[2][2];
+      de[3] = L[2][1] + L[1][2];
+      de[4] = L[0][2] + L[2][0];
+      de[5] = L[1][0] + L[0][1];
+      vort[0] = L[2][1] - L[1][2];
+      vort[1] = L[0][2] - L[2][0];
+      vort[2] = L[1][0] - L[0][1];
+    } else {
+    build_stencil<SHAPE, T>(I, x, y, z, c, S);
     for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T, T dX, T dY, T dZ) {
       const int e = B.ok ? B.idx(nx, ny, nz) : -1;
       const T v0 = e >= 0 ? tf[e] : Vg[k];
@@ -1080,6 +1349,7 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
         L[2][0] = L[2][0] + dX * v2; L[2][1] = L[2][1] + dY * v2; L[2][2] = L[2][2] + dZ * v2;
       }
     });
+    }
     const double dtd = ts.dt;
     const T dt = (T)dtd;
 #pragma unroll
@@ -1133,6 +1403,12 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
       const T sxx = smq + p.t(T_SD0), syy = smq + p.t(T_SD1), szz = smq + p.t(T_SD2);
       const T sxy = p.t(T_SD5), syz = p.t(T_SD3), sxz = p.t(T_SD4);
       const T vol = (T)p.d(D_VOL);
+      if (fast) {
+        const T sv[3][3] = {{-(sxx * vol), -(sxy * vol), -(sxz * vol)},
+                            {-(sxy * vol), -(syy * vol), -(syz * vol)},
+                            {-(sxz * vol), -(syz * vol), -(szz * vol)}};
+        scatter_force<NS, T>(FS, sb, sc, sv, fx, ta, CAP);
+      } else
       for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T dX, T dY, T dZ) {
         T f[3];
         f[0] = -(sxx * dX + sxy * dY + sxz * dZ) * vol + fx[0] * N;
@@ -1229,6 +1505,7 @@ __global__ void __launch_bounds__(256) k_finalize(PoolPtrs<T> P, const int* ilis
   }
 }
 
+#if MPMK_PART != 2   // defined once, in the FP64 object
 // istep += 1; CurrentTime += DT (MPM3D.f90:72-74)
 __global__ void k_begin_step(InstT* ts, const int* ilist, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1239,6 +1516,7 @@ __global__ void k_begin_step(InstT* ts, const int* ilist, int n) {
   t.time = t.time + t.dt;
   t.ei = 0.0;
 }
+#endif
 
 // gather one T field through a permutation (sort support)
 template <typename S>
@@ -1250,6 +1528,7 @@ __global__ void __launch_bounds__(256) k_permute(S* dst, const S* src, const int
   for (int f = 0; f < nfield; f++) dst[(long)f * np + j] = src[(long)f * np + from];
 }
 
+#if MPMK_PART != 2
 // Sort key of a particle: its cell in brick-major order (bricks of B^3
 // cells, cells x-fastest inside a brick), so that a block of consecutive
 // particles covers a compact box of nodes.  The cell is located from the
@@ -1275,6 +1554,7 @@ __global__ void __launch_bounds__(256) k_sort_keys(const double* X, long np, con
   keys[j] = c > 0 ? brick_key(I, c, B) : (unsigned)nbx * nby * nbz * (unsigned)(B * B * B);
   vals[j] = (int)j;
 }
+#endif
 
 // ---------------------------------------------------------- dispatch
 
@@ -1384,13 +1664,14 @@ void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int bpg,
   if (ninst) hipLaunchKernelGGL(k_finalize<T>, dim3(ninst), dim3(256), 0, s, P, ilist, bpg, what);
 }
 
-void launch_begin_step(InstT* ts, const int* ilist, int n, hipStream_t s) {
-  if (n) hipLaunchKernelGGL(k_begin_step, dim3((n + 255) / 256), dim3(256), 0, s, ts, ilist, n);
-}
-
 template <typename S>
 void launch_permute(S* dst, const S* src, const int* perm, const LaunchP& L, long np, int nfield, hipStream_t s) {
   if (L.nblk) hipLaunchKernelGGL(k_permute<S>, dim3(L.nblk), dim3(L.bs), 0, s, dst, src, perm, L.glist, L.bpg, np, nfield);
+}
+
+#if MPMK_PART != 2
+void launch_begin_step(InstT* ts, const int* ilist, int n, hipStream_t s) {
+  if (n) hipLaunchKernelGGL(k_begin_step, dim3((n + 255) / 256), dim3(256), 0, s, ts, ilist, n);
 }
 
 void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP& L, int brick, unsigned* keys,
@@ -1399,6 +1680,7 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
     hipLaunchKernelGGL(k_sort_keys, dim3(L.nblk), dim3(L.bs), 0, s, X, np, inst, L.glist, L.ginst, L.bpg, brick, keys,
                        vals);
 }
+#endif
 
 #define INST(T)                                                                                     \
   template void launch_p2g<T>(int, int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);          \
@@ -1411,9 +1693,13 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
   template void launch_energy<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                 \
   template void launch_finalize<T>(const PoolPtrs<T>&, const int*, int, int, int, hipStream_t);    \
   template void launch_permute<T>(T*, const T*, const int*, const LaunchP&, long, int, hipStream_t);
+#if MPMK_PART != 2
 INST(double)
-INST(float)
 template void launch_permute<int>(int*, const int*, const int*, const LaunchP&, long, int, hipStream_t);
 template void launch_zero_nodes<uint8_t>(uint8_t*, int, long, const LaunchP&, hipStream_t);
+#endif
+#if MPMK_PART != 1
+INST(float)
+#endif
 
 }  // namespace mpmk
