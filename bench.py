@@ -275,6 +275,39 @@ def ensemble_parity(mpm, decks, E, a, log):
     return res
 
 
+def run_serving(mpm, decks, a, log):
+    """Continuous-batching serving of a 70/30 mix of slope and Taylor-bar
+    runs (100-300 steps each, FP32 bulk) under Poisson arrivals, one pool
+    sized by the HBM budget; p50/p99 time to solution per arrival rate."""
+    from mpm3d_f90_amd import serve
+    t0 = time.perf_counter()
+    pool = serve.request_pool(mpm, decks, count=48, seed=a.seed)
+    log("serving: pool of %d requests (%d slopes, %d bars, %d-%d particles) built in %.1f s"
+        % (len(pool), sum(k == "slope" for _, _, k in pool), sum(k == "bar" for _, _, k in pool),
+           min(p.n for p, _, _ in pool), max(p.n for p, _, _ in pool), time.perf_counter() - t0))
+    eng = mpm.Engine(hbm_bytes=a.serve_budget, max_instances=512, precision="fp32")
+    srv = serve.Server(eng, chunk=8, graph=True)
+    # warm-up: one request through the whole path (capture, admission, retire)
+    srv.submit(serve.Request(-1, pool[0][0], 16, arrival=0.0, kind="warm"))
+    srv.admit(0.0)
+    while not srv.idle():
+        srv.advance()
+        srv.retire(lambda: 0.0)
+    runs = []
+    for k, rate in enumerate(float(r) for r in a.serve_rates.split(",")):
+        srv.done.clear()
+        srv.max_running = 0
+        st = serve.run_poisson(srv, pool, rate, a.serve_window, seed=a.seed + k, max_wall=a.serve_window + 60)
+        log("serving %.0f req/s: %s" % (rate, json.dumps(st)))
+        runs.append(st)
+    out = {"workload": "70%% slopes (ensemble generator), 30%% Taylor bars (v U(100,250) m/s, R U(2,5) mm), "
+                       "100-300 steps per request, FP32 bulk, HBM budget %.0f GB, chunks of 8 graph-replayed steps"
+                       % (a.serve_budget / 1e9),
+           "pool_device_bytes": eng.device_bytes(), "runs": runs}
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +319,9 @@ def main():
     ap.add_argument("--check-steps", type=int, default=2, help="steps of the single slope compared with the oracle")
     ap.add_argument("--cpu-steps", type=int, default=30, help="steps of the CPU baseline sample per instance")
     ap.add_argument("--no-cpu", action="store_true", help="skip oracle parity and CPU baseline")
+    ap.add_argument("--serve-rates", default="10,50,200", help="Poisson arrival rates (req/s) of the serving runs")
+    ap.add_argument("--serve-window", type=float, default=8.0, help="seconds of arrivals per serving run (0: skip)")
+    ap.add_argument("--serve-budget", type=float, default=200e9, help="HBM budget (bytes) of the serving pool")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (instances shard "
@@ -320,6 +356,10 @@ def main():
         log("single slope parity:", json.dumps(single))
         parity["ensemble"] = ensemble_parity(mpm, decks, E, a, log)
         parity["max_fp32_err"] = max(max(r["fp32_vs_fp64_oracle"].values()) for r in parity["ensemble"])
+    for eng in E["engs"].values():
+        eng.close()
+    serving = run_serving(mpm, decks, a, log) if a.serve_window > 0 else None
+    if not a.no_cpu:
         sample = [E["ens"][k] for k in range(3)]
         cpu = cpu_baseline(sample, mpm, decks, a.cpu_steps, log)
         log("cpu baseline:", json.dumps(cpu))
@@ -360,6 +400,7 @@ def main():
         "roofline_plain_fp32": roofline(plain["fp32"]["kt"], prob.n, plain["fp32"]["touched"], 4,
                                         lambda k: measured_traffic(k, "fp32", "slope_pmc_traffic.md")),
         "parity": parity or None,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

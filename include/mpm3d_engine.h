@@ -38,6 +38,9 @@ long mpm_engine_device_bytes(mpm_engine* e);
  * (MPM3D.f90:152-158). */
 int mpm_engine_add(mpm_engine* e, const mpm_problem* p, int stop_at_endtime, int* id);
 int mpm_engine_remove(mpm_engine* e, int id);
+/* stop an instance after max_steps steps (0: no limit), in addition to the
+ * endtime test; a served request that asks for a fixed number of steps */
+int mpm_engine_set_step_limit(mpm_engine* e, int id, int max_steps);
 /* largest free particle / node ranges and free instance slots */
 int mpm_engine_capacity(mpm_engine* e, long* free_particles, long* free_nodes, int* free_instances);
 
@@ -52,15 +55,16 @@ int mpm_engine_set_tuning(mpm_engine* e, int block_particles, int block_nodes, i
 /* interleaved (1) or contiguous (0) lane -> particle mapping in particle kernels */
 int mpm_engine_set_interleave(mpm_engine* e, int enable);
 
-/* Launch-shape settings.  sort_brick is the edge (cells) of the cubic cell
- * bricks of the particle sort key (1, 2, 4, 8): the "cell bin" that keeps a
- * block's particles inside a compact box of nodes.  step_ms / sort_ms /
- * trials are filled by mpm_engine_autotune and ignored on input. */
+/* Launch-shape settings.  sort_brick selects the cell order of the particle
+ * sort ("cell bins" that keep a block's particles inside a compact box of
+ * nodes): 0 the 3D Hilbert curve (default), B = 1, 2, 4, 8 brick-major order
+ * with B^3-cell bricks.  step_ms / sort_ms / trials are filled by
+ * mpm_engine_autotune and ignored on input. */
 typedef struct mpm_tuning {
   int32_t block_particles;   /* 64, 128, 256 */
   int32_t block_nodes;       /* 64, 128, 256 */
   int32_t sort_every;        /* steps between re-sorts, 0 = never */
-  int32_t sort_brick;        /* 1, 2, 4, 8 */
+  int32_t sort_brick;        /* 0 (Hilbert), 1, 2, 4, 8 */
   int32_t interleave;        /* 0 / 1 */
   int32_t trials;            /* configurations timed by the autotuner */
   double step_ms;            /* mean step time of the chosen shape, sort excluded */

@@ -18,6 +18,7 @@ from . import _lib
 from ._lib import (PF, PF_NAMES, PF_COUNT, GF, GF_NAMES, GF_COUNT, SHAPES, SCHEMES, PREC_FP64,
                    PREC_FP32, MpmError, check, dptr, iptr)
 from .problem import Problem
+from . import serve
 
 __all__ = ["Problem", "Context", "Engine", "MpmError", "PF", "PF_NAMES", "PF_COUNT", "GF",
            "GF_NAMES", "GF_COUNT", "PREC_FP64", "PREC_FP32", "read_deck"]
@@ -236,6 +237,11 @@ class Engine:
             if e.code == -6:
                 return None
             raise
+
+    def set_step_limit(self, iid, max_steps):
+        """Stop the instance after max_steps steps (0: no limit)."""
+        check(_lib.lib().mpm_engine_set_step_limit(self._h, int(iid), int(max_steps)),
+              "mpm_engine_set_step_limit")
 
     def remove(self, iid):
         check(_lib.lib().mpm_engine_remove(self._h, int(iid)), "mpm_engine_remove")

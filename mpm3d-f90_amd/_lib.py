@@ -123,6 +123,7 @@ def lib():
         "mpm_engine_device_bytes": (C.c_long, [vp]),
         "mpm_engine_add": (C.c_int, [vp, P(ProblemStruct), C.c_int, P(C.c_int)]),
         "mpm_engine_remove": (C.c_int, [vp, C.c_int]),
+        "mpm_engine_set_step_limit": (C.c_int, [vp, C.c_int, C.c_int]),
         "mpm_engine_capacity": (C.c_int, [vp, P(C.c_long), P(C.c_long), P(C.c_int)]),
         "mpm_engine_step": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mpm_engine_sync": (C.c_int, [vp]),

@@ -22,11 +22,22 @@ struct InstC {
   int flags;             // IF_* below
   int mat_off, nmat;     // material range in the material table
   int cell_off;          // sort-key offset of this instance's cells
+  int max_steps;         // > 0: the instance stops itself after this many steps
   double span0[3];       // SpanX(1), SpanY(1), SpanZ(1)
   double span1[3];       // adjusted upper ends
   double dcell, idcell, ijac, ijac4, cutoff;
   double qlt, damp, bq1, bq2, dtscale, endtime;
 };
+
+// bits per axis of the Hilbert sort order of an instance (cells per axis
+// rounded up to a power of two), 0 when its grid is too large for a 30-bit
+// key (then the brick order with 4^3-cell bricks is used)
+__host__ __device__ inline int hilbert_bits(const InstC& I) {
+  const int m = I.ncx > I.ncy ? (I.ncx > I.ncz ? I.ncx : I.ncz) : (I.ncy > I.ncz ? I.ncy : I.ncz);
+  int b = 1;
+  while ((1 << b) < m) b++;
+  return b <= 10 ? b : 0;
+}
 
 enum InstFlags {
   IF_QUASI_LOAD = 1,

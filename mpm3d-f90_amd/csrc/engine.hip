@@ -172,10 +172,10 @@ struct mpm_engine {
   int* d_ilist_all = nullptr;
   size_t cap_ilist = 0;
   int n_active = 0;
-  int bs_p = 256, bs_n = 256;     // block sizes (autotuned)
+  int bs_p = 128, bs_n = 256;     // block sizes (autotuned)
   int ilv = 1;                    // interleaved lane mapping in particle kernels
   int sort_every = 20;            // steps between re-sorts in mpm_step (0: never)
-  int brick = 4;                  // edge (cells) of the bricks of the sort key
+  int brick = 0;                  // sort order: 0 Hilbert, B > 0 bricks of B^3 cells
   long steps_done = 0;
   // sort scratch (lazily allocated)
   unsigned* k_in = nullptr;
@@ -482,6 +482,7 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   c.mat_off = (int)m0;
   c.nmat = p->nb_mat;
   c.cell_off = 0;
+  c.max_steps = 0;
   for (int k = 0; k < 3; k++) {
     c.span0[k] = p->span[2 * k];
     c.span1[k] = p->span[2 * k + 1];
@@ -864,12 +865,19 @@ static void step_graph(mpm_engine* e, int nsteps, int per, int fin_flags) {
 
 // ---------------------------------------------------------- sorting
 
-// Stable sort of every instance's particles by brick-major cell key
+// largest sort key of an instance (the key of particles outside the grid)
+static unsigned max_sort_key(const InstC& c, int brick) {
+  const int hb = brick == 0 ? hilbert_bits(c) : 0;
+  if (hb > 0) return 1u << (3 * hb);
+  const unsigned B = brick == 0 ? 4u : (unsigned)brick;
+  return ((c.ncx + B - 1) / B) * ((c.ncy + B - 1) / B) * ((c.ncz + B - 1) / B) * B * B * B;
+}
+
+// Stable sort of every instance's particles by cell key (Hilbert or brick order)
 // (segmented LSD radix sort: equal keys keep their order, so the
 // permutation is the one a stable CPU sort of the same keys gives), then
 // gather all fields.
-static void sort_particles(mpm_engine* e) {
-  rebuild_lists(e);
+static void ensure_sort_buffers(mpm_engine* e) {
   const long np = e->np;
   if (!e->k_in) {
     e->k_in = dalloc<unsigned>(np);
@@ -880,18 +888,72 @@ static void sort_particles(mpm_engine* e) {
     e->d_alt = dalloc<double>((size_t)D_COUNT * np);
     e->i_alt = dalloc<int>((size_t)I_COUNT * np);
   }
+}
+
+// Sort one newly admitted instance (its slot range only), so that it joins
+// the resident ones in cell order instead of in generator order.
+static void sort_instance(mpm_engine* e, int id) {
+  ensure_sort_buffers(e);
+  const HostInst& h = e->inst[id];
+  const long np = e->np, b0 = h.pg0 * PGRAN, pad = h.npg * PGRAN;
+  std::vector<int> gl(h.npg);
+  for (long g = 0; g < h.npg; g++) gl[g] = (int)(h.pg0 + g);
+  int* d_gl = dalloc<int>(gl.size());
+  HIPCHK(hipMemcpy(d_gl, gl.data(), gl.size() * sizeof(int), hipMemcpyHostToDevice));
+  LaunchP L;
+  L.glist = d_gl;
+  L.ginst = e->d_ginst;
+  L.bs = 256;
+  L.bpg = PGRAN / 256;
+  L.nblk = (int)(h.npg * L.bpg);
+  mpmk::launch_sort_keys(e->d, np, e->d_inst, L, e->brick, e->k_in, e->v_in, e->s);
+  int end_bit = 1;
+  const unsigned maxkey = max_sort_key(h.c, e->brick);
+  while (end_bit < 32 && (1u << end_bit) <= maxkey) end_bit++;
+  size_t need = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, e->k_in + b0, e->k_out + b0, e->v_in + b0, e->v_out + b0,
+                                            (int)pad, 0, end_bit, e->s));
+  if (need > e->cub_bytes) {
+    HIPCHK(hipStreamSynchronize(e->s));
+    if (e->cub_tmp) HIPCHK(hipFree(e->cub_tmp));
+    e->cub_tmp = dalloc<char>(need);
+    e->cub_bytes = need;
+  }
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(e->cub_tmp, need, e->k_in + b0, e->k_out + b0, e->v_in + b0,
+                                            e->v_out + b0, (int)pad, 0, end_bit, e->s));
+  if (e->prec == MPM_PREC_FP32)
+    mpmk::launch_permute<float>((float*)e->t_alt, (const float*)e->t, e->v_out, L, np, T_COUNT, e->s);
+  else
+    mpmk::launch_permute<double>((double*)e->t_alt, (const double*)e->t, e->v_out, L, np, T_COUNT, e->s);
+  mpmk::launch_permute<double>(e->d_alt, e->d, e->v_out, L, np, D_COUNT, e->s);
+  mpmk::launch_permute<int>(e->i_alt, e->ii, e->v_out, L, np, I_COUNT, e->s);
+  for (int f = 0; f < T_COUNT; f++)
+    HIPCHK(hipMemcpyAsync((char*)e->t + ((size_t)f * np + b0) * e->tsz, (char*)e->t_alt + ((size_t)f * np + b0) * e->tsz,
+                          pad * e->tsz, hipMemcpyDeviceToDevice, e->s));
+  for (int f = 0; f < D_COUNT; f++)
+    HIPCHK(hipMemcpyAsync(e->d + (size_t)f * np + b0, e->d_alt + (size_t)f * np + b0, pad * sizeof(double),
+                          hipMemcpyDeviceToDevice, e->s));
+  for (int f = 0; f < I_COUNT; f++)
+    HIPCHK(hipMemcpyAsync(e->ii + (size_t)f * np + b0, e->i_alt + (size_t)f * np + b0, pad * sizeof(int),
+                          hipMemcpyDeviceToDevice, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  HIPCHK(hipFree(d_gl));
+}
+
+static void sort_particles(mpm_engine* e) {
+  rebuild_lists(e);
+  const long np = e->np;
+  ensure_sort_buffers(e);
   if (!e->d_seg) e->d_seg = dalloc<int>(2 * (size_t)e->max_inst);
   std::vector<int> seg;
   unsigned maxkey = 1;
-  const unsigned B = (unsigned)e->brick;
   for (auto& G : e->groups) {
     const LaunchP L = plist(e, G);
     mpmk::launch_sort_keys(e->d, np, e->d_inst, L, e->brick, e->k_in, e->v_in, e->s);
     for (int id : G.insts) {
       const HostInst& h = e->inst[id];
       seg.push_back((int)(h.pg0 * PGRAN));
-      const unsigned nb = ((h.c.ncx + B - 1) / B) * ((h.c.ncy + B - 1) / B) * ((h.c.ncz + B - 1) / B);
-      maxkey = std::max<unsigned>(maxkey, nb * B * B * B);
+      maxkey = std::max<unsigned>(maxkey, max_sort_key(h.c, e->brick));
     }
   }
   const int nseg = (int)seg.size();
@@ -993,7 +1055,7 @@ static void snap_restore(mpm_engine* e, const Snapshot& S) {
 static void apply_tuning(mpm_engine* e, const mpm_tuning& t) {
   auto ok = [](int b) { return b == 64 || b == 128 || b == 256; };
   if (!ok(t.block_particles) || !ok(t.block_nodes) || t.sort_every < 0 ||
-      !(t.sort_brick == 1 || t.sort_brick == 2 || t.sort_brick == 4 || t.sort_brick == 8))
+      !(t.sort_brick == 0 || t.sort_brick == 1 || t.sort_brick == 2 || t.sort_brick == 4 || t.sort_brick == 8))
     fail(MPM_ERR_ARG, "bad tuning values");
   e->bs_p = t.block_particles;
   e->bs_n = t.block_nodes;
@@ -1054,7 +1116,8 @@ static double time_sort(mpm_engine* e) {
 }
 
 // Coordinate search over the launch shape on the resident workload:
-// (1) sort-brick edge x particle block size x lane mapping, timed right
+// (1) sort order (Hilbert, 2^3 or 4^3 bricks) x particle block size x lane
+// mapping, timed right
 // after a re-sort; (2) node block size; (3) re-sort period, from the
 // measured cost of one sort and the measured growth of the step time as
 // particles drift from their sorted order: the period P minimising
@@ -1085,7 +1148,7 @@ static mpm_tuning autotune(mpm_engine* e, int trial_steps) {
     };
     mpm_tuning t = best;
     t.block_nodes = 256;
-    for (int B : {2, 4, 8})
+    for (int B : {0, 2, 4})
       for (int bs : {64, 128, 256})
         for (int il : {0, 1}) {
           t.sort_brick = B;
@@ -1414,7 +1477,9 @@ extern "C" int mpm_engine_create_budget(double hbm_bytes, int max_instances, int
     const double bp = 2.0 * bytes_per_particle(precision) + 16, bn = bytes_per_node(precision);
     const long np = (long)(hbm_bytes / (bp + 0.5 * bn));
     const long nn = np / 2;
-    *out = new_engine(np, nn, max_instances, 16 * max_instances, 16 * max_instances, precision);
+    // decks built from many particle generators have one body per generator
+    // line (DataIn.f90:568-846): 32 B per body slot, so be generous
+    *out = new_engine(np, nn, max_instances, 16 * max_instances, 2048 * max_instances, precision);
   });
 }
 
@@ -1432,7 +1497,20 @@ extern "C" int mpm_engine_add(mpm_engine* e, const mpm_problem* p, int stop_at_e
     *id = add_instance(e, p, stop_at_endtime ? IF_ENDTIME : 0);
     e->lists_dirty = true;
     rebuild_lists(e);
+    if (e->sort_every > 0) sort_instance(e, *id);
     initial_energy(e, *id);   // MPM3D.f90:42
+  });
+}
+
+extern "C" int mpm_engine_set_step_limit(mpm_engine* e, int id, int max_steps) {
+  return guarded([&] {
+    check_ctx(e);
+    HostInst& h = e->inst[live_id(e, id)];
+    if (max_steps < 0) fail(MPM_ERR_ARG, "max_steps < 0");
+    h.c.max_steps = max_steps;
+    HIPCHK(hipStreamSynchronize(e->s));
+    HIPCHK(hipMemcpy(e->d_inst + id, &h.c, sizeof(InstC), hipMemcpyHostToDevice));
+    e->graph_dirty = true;
   });
 }
 

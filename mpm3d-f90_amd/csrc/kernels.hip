@@ -330,13 +330,17 @@ __device__ __forceinline__ void particle_energy(const PV<T>& p, double v[6]) {
 //
 // A block's particles are spatially compact (generator order or cell sort),
 // so the nodes they touch fit a small box.  The box is computed
-// collectively; when it fits TILE_BYTES of LDS, gathers read the box from
+// collectively; when it fits TILE_NODES nodes, gathers read the box from
 // LDS (loaded once, coalesced) and scatters accumulate with LDS atomics and
 // flush each node once with one global atomic per field.  Otherwise the
 // block falls back to direct global access.  Nodes outside the box (the
 // B-spline low-face wrap) always take the direct path.
 
-#define TILE_BYTES 32768
+// Nodes of a block's LDS tile.  A cell-sorted block of 256 particles
+// touches 150-300 nodes (two bricks when it straddles a brick boundary);
+// blocks whose box does not fit take the direct path.  Each kernel sizes its
+// LDS array as TILE_NODES times its bytes per node.
+#define TILE_NODES 512
 
 struct Box {
   int x0, y0, z0, bx, by, bz, n, ok;
@@ -560,9 +564,9 @@ __device__ __forceinline__ void gather_grad(const Slots<T>& S, const T* tf, int 
 
 // scatter of mass (FP64) and NV vector fields q_f * N into FP64 tile
 // accumulators (field-major, stride cap)
-template <int NS, typename T, bool MASS, int NV>
-__device__ __forceinline__ void scatter_w(const Slots<T>& S, int sb, int sc, T m, const T q[3], double* tm,
-                                          double* tv, int cap) {
+template <int NS, typename T, bool MASS, int NV, typename A>
+__device__ __forceinline__ void scatter_w(const Slots<T>& S, int sb, int sc, T m, const T q[3], double* tm, A* tv,
+                                          int cap) {
 #pragma unroll
   for (int c = 0; c < NS; c++)
 #pragma unroll
@@ -575,25 +579,25 @@ __device__ __forceinline__ void scatter_w(const Slots<T>& S, int sb, int sc, T m
         const int e = eb + a;
         if (MASS) atomicAdd(&tm[e], (double)shm);
 #pragma unroll
-        for (int f = 0; f < NV; f++) atomicAdd(&tv[f * cap + e], (double)(q[f] * shm));
+        for (int f = 0; f < NV; f++) atomicAdd(&tv[f * cap + e], (A)(q[f] * shm));
       }
     }
 }
 
 // nodal force f_n = -(sigma . grad N_n) vol + fext N_n into FP64 tile
 // accumulators; sv[i][j] = -vol sigma_ij
-template <int NS, typename T>
+template <int NS, typename T, typename A>
 __device__ __forceinline__ void scatter_force(const Slots<T>& S, int sb, int sc, const T sv[3][3], const T fx[3],
-                                              double* tv, int cap) {
+                                              A* tv, int cap) {
 #pragma unroll
   for (int c = 0; c < NS; c++)
 #pragma unroll
     for (int b = 0; b < NS; b++) {
       const T wyz = S.w[1][b] * S.w[2][c], dyz = S.d[1][b] * S.w[2][c], wdz = S.w[1][b] * S.d[2][c];
-      T A[3], Q[3];
+      T Af[3], Q[3];
 #pragma unroll
       for (int i = 0; i < 3; i++) {
-        A[i] = sv[i][0] * wyz;
+        Af[i] = sv[i][0] * wyz;
         Q[i] = sv[i][1] * dyz + sv[i][2] * wdz + fx[i] * wyz;
       }
       const int eb = S.e0 + b * sb + c * sc;
@@ -601,7 +605,7 @@ __device__ __forceinline__ void scatter_force(const Slots<T>& S, int sb, int sc,
       for (int a = 0; a < NS; a++) {
         const int e = eb + a;
 #pragma unroll
-        for (int i = 0; i < 3; i++) atomicAdd(&tv[i * cap + e], (double)(S.d[0][a] * A[i] + S.w[0][a] * Q[i]));
+        for (int i = 0; i < 3; i++) atomicAdd(&tv[i * cap + e], (A)(S.d[0][a] * Af[i] + S.w[0][a] * Q[i]));
       }
     }
 }
@@ -619,10 +623,13 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
   if (P.ts[ii].done) return;
   constexpr bool MASS = (MODE & P2G_MASS) != 0, MOM = (MODE & P2G_MOM) != 0, FRC = (MODE & P2G_FORCE) != 0;
   constexpr int NT = (MOM ? 3 : 0) + (FRC ? 3 : 0);
-  constexpr int CAP = TILE_BYTES / ((MASS ? 8 : 0) + NT * 8);
-  __shared__ __align__(16) char smem[TILE_BYTES];
-  double* tm = (double*)smem;                            // CAP (mass)
-  double* tf = (double*)(smem + (MASS ? 8 * CAP : 0));   // NT * CAP, accumulated in FP64
+  // tile accumulation in FP64 for both paths: ds_add_f64 measured several
+  // times faster than ds_add_f32 for these scatter patterns on gfx950
+  using A = double;
+  constexpr int CAP = TILE_NODES;
+  __shared__ __align__(16) char smem[CAP * ((MASS ? 8 : 0) + NT * (int)sizeof(A))];
+  double* tm = (double*)smem;                       // CAP (mass)
+  A* tf = (A*)(smem + (MASS ? 8 * CAP : 0));        // NT * CAP
   PV<T> p{P, i};
   const bool live = i < I.p_end;
   int c = -1;
@@ -647,7 +654,7 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
     for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
       if (MASS) tm[e] = 0.0;
 #pragma unroll
-      for (int f = 0; f < NT; f++) tf[f * CAP + e] = 0.0;
+      for (int f = 0; f < NT; f++) tf[f * CAP + e] = (A)0;
     }
     __syncthreads();
   }
@@ -701,7 +708,7 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
       if (e >= 0) {
         if (MASS) atomicAdd(&tm[e], (double)shm);
 #pragma unroll
-        for (int f = 0; f < NT; f++) atomicAdd(&tf[f * CAP + e], (double)val[f]);
+        for (int f = 0; f < NT; f++) atomicAdd(&tf[f * CAP + e], (A)val[f]);
       } else {
         if (MASS) atomic_add(&Mg[k], (double)shm);
         q = 0;
@@ -718,9 +725,9 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
       if (MASS) { const double v = tm[e]; if (v != 0.0) atomic_add(&Mg[k], v); }
       int f = 0;
       if (MOM)
-        for (int d = 0; d < 3; d++, f++) { const double v = tf[f * CAP + e]; if (v != 0.0) atomic_add(&Pg[d * nn + k], (T)v); }
+        for (int d = 0; d < 3; d++, f++) { const A v = tf[f * CAP + e]; if (v != (A)0) atomic_add(&Pg[d * nn + k], (T)v); }
       if (FRC)
-        for (int d = 0; d < 3; d++, f++) { const double v = tf[f * CAP + e]; if (v != 0.0) atomic_add(&Fg[d * nn + k], (T)v); }
+        for (int d = 0; d < 3; d++, f++) { const A v = tf[f * CAP + e]; if (v != (A)0) atomic_add(&Fg[d * nn + k], (T)v); }
     }
   }
 }
@@ -833,22 +840,22 @@ __device__ __forceinline__ void load_tile(const InstC& I, const Box& B, T* tf, i
   }
 }
 
-template <int NT>
-__device__ __forceinline__ void zero_tile(const Box& B, double* ta, int cap) {
+template <int NT, typename A>
+__device__ __forceinline__ void zero_tile(const Box& B, A* ta, int cap) {
   for (int e = threadIdx.x; e < B.n; e += blockDim.x)
 #pragma unroll
-    for (int f = 0; f < NT; f++) ta[f * cap + e] = 0.0;
+    for (int f = 0; f < NT; f++) ta[f * cap + e] = (A)0;
 }
 
-// flush a 3-component FP64 accumulation tile into a grid field of type T
-template <typename T>
-__device__ __forceinline__ void flush_tile3(const InstC& I, const Box& B, const double* ta, int cap, T* dst, long nn) {
+// flush a 3-component accumulation tile into a grid field of type T
+template <typename T, typename A>
+__device__ __forceinline__ void flush_tile3(const InstC& I, const Box& B, const A* ta, int cap, T* dst, long nn) {
   for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
     const int k = B.node(I, e);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      const double v = ta[d * cap + e];
-      if (v != 0.0) atomic_add(&dst[d * nn + k], (T)v);
+      const A v = ta[d * cap + e];
+      if (v != (A)0) atomic_add(&dst[d * nn + k], (T)v);
     }
   }
 }
@@ -865,11 +872,12 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
   const InstT& ts = P.ts[ii];
   if (ts.done) return;
   constexpr int NS = NSlot<SHAPE>::n;
+  using A = double;   // re-map accumulation type (see k_p2g)
   constexpr int NB = AosTile<T, 6>::S * (int)sizeof(T);   // V, A per node (fast-path layout)
-  constexpr int CAP = TILE_BYTES / (NB > 24 ? NB : 24);
-  __shared__ __align__(16) char smem[TILE_BYTES];
+  constexpr int CAP = TILE_NODES;
+  __shared__ __align__(16) char smem[CAP * (NB > 3 * (int)sizeof(A) ? NB : 3 * (int)sizeof(A))];
   T* tf = (T*)smem;
-  double* ta = (double*)smem;   // re-map accumulation, reuses the gather tile
+  A* ta = (A*)smem;   // re-map accumulation, reuses the gather tile
   PV<T> p{P, i};
   const bool live = i < I.p_end;
   const int c = live ? p.n(I_CELL) : -1;
@@ -949,7 +957,7 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
           const int e = B.ok ? B.idx(nx, ny, nz) : -1;
           if (e >= 0) {
 #pragma unroll
-            for (int d = 0; d < 3; d++) atomicAdd(&ta[d * CAP + e], (double)(vn[d] * shm));
+            for (int d = 0; d < 3; d++) atomicAdd(&ta[d * CAP + e], (A)(vn[d] * shm));
           } else {
 #pragma unroll
             for (int d = 0; d < 3; d++) atomic_add(&P2[d * nn + k], vn[d] * shm);
@@ -1285,11 +1293,12 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
   const InstT& ts = P.ts[ii];
   if (ts.done) return;
   constexpr int NS = NSlot<SHAPE>::n;
+  using A = double;   // force accumulation type (see k_p2g)
   constexpr int NB = AosTile<T, 3>::S * (int)sizeof(T);   // nodal velocity per node (fast-path layout)
-  constexpr int CAP = TILE_BYTES / (NB > 24 ? NB : 24);
-  __shared__ __align__(16) char smem[TILE_BYTES];
+  constexpr int CAP = TILE_NODES;
+  __shared__ __align__(16) char smem[CAP * (NB > 3 * (int)sizeof(A) ? NB : 3 * (int)sizeof(A))];
   T* tf = (T*)smem;
-  double* ta = (double*)smem;   // force accumulation after the gather
+  A* ta = (A*)smem;   // force accumulation after the gather
   PV<T> p{P, i};
   const bool live = i < I.p_end;
   const int c = live ? p.n(I_CELL) : -1;
@@ -1417,7 +1426,7 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
         const int e = B.ok ? B.idx(nx, ny, nz) : -1;
         if (e >= 0) {
 #pragma unroll
-          for (int d = 0; d < 3; d++) atomicAdd(&ta[d * CAP + e], (double)f[d]);
+          for (int d = 0; d < 3; d++) atomicAdd(&ta[d * CAP + e], (A)f[d]);
         } else {
 #pragma unroll
           for (int d = 0; d < 3; d++) atomic_add(&Fg[d * nn + k], f[d]);
@@ -1495,7 +1504,7 @@ __global__ void __launch_bounds__(256) k_finalize(PoolPtrs<T> P, const int* ilis
     }
     ts.dt = dt * I.dtscale;
   }
-  if ((what & FIN_DONE) && (I.flags & IF_ENDTIME)) {
+  if ((what & FIN_DONE) && (I.flags & IF_ENDTIME)) {   // MPM3D.f90:152-158
     bool stop;
     if (I.flags & IF_QUASI_LOAD)
       stop = (ts.time > I.endtime && ts.ek / (ts.ek + ts.ei) < 0.005) && ts.time > 100.0;
@@ -1503,6 +1512,7 @@ __global__ void __launch_bounds__(256) k_finalize(PoolPtrs<T> P, const int* ilis
       stop = ts.time > I.endtime;
     if (stop) ts.done = 1;
   }
+  if ((what & FIN_DONE) && I.max_steps > 0 && ts.istep >= I.max_steps) ts.done = 1;
 }
 
 #if MPMK_PART != 2   // defined once, in the FP64 object
@@ -1543,6 +1553,41 @@ __device__ __forceinline__ unsigned brick_key(const InstC& I, int c, int B) {
   return brick * (unsigned)(B * B * B) + local;
 }
 
+// Index of cell (x, y, z) along the 3D Hilbert curve of 2^bits cells per
+// axis: J. Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707
+// (2004) -- axes to transposed Gray code, then the bits interleaved (x most
+// significant).  Consecutive cells along the curve are face neighbours, so a
+// block of consecutive particles covers a compact box even at the ragged
+// edges of a body, where brick orders jump between rows of bricks.
+__device__ __forceinline__ unsigned hilbert_key(unsigned x, unsigned y, unsigned z, int bits) {
+  unsigned X[3] = {x, y, z};
+  const unsigned M = 1u << (bits - 1);
+  for (unsigned Q = M; Q > 1; Q >>= 1) {
+    const unsigned P = Q - 1;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        const unsigned t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  X[1] ^= X[0];
+  X[2] ^= X[1];
+  unsigned t = 0;
+  for (unsigned Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  unsigned h = 0;
+  for (int b = bits - 1; b >= 0; b--)
+#pragma unroll
+    for (int i = 0; i < 3; i++) h = (h << 1) | (((X[i] ^ t) >> b) & 1u);
+  return h;
+}
+
+// B = 0: Hilbert order; B > 0: brick-major order with B^3-cell bricks
 __global__ void __launch_bounds__(256) k_sort_keys(const double* X, long np, const InstC* inst, const int* glist,
                                                    const int* ginst, int bpg, int B, unsigned* keys, int* vals) {
   int gran;
@@ -1550,8 +1595,20 @@ __global__ void __launch_bounds__(256) k_sort_keys(const double* X, long np, con
   const InstC& I = inst[ginst[gran]];
   int c = -1;
   if (j < I.p_end) c = in_which_cell(I, X[j], X[np + j], X[2 * np + j]);
-  const int nbx = (I.ncx + B - 1) / B, nby = (I.ncy + B - 1) / B, nbz = (I.ncz + B - 1) / B;
-  keys[j] = c > 0 ? brick_key(I, c, B) : (unsigned)nbx * nby * nbz * (unsigned)(B * B * B);
+  const int hb = B == 0 ? hilbert_bits(I) : 0;
+  if (hb > 0) {
+    unsigned key = 1u << (3 * hb);
+    if (c > 0) {
+      int cx, cy, cz;
+      cell_coords(I, c, cx, cy, cz);
+      key = hilbert_key((unsigned)cx, (unsigned)cy, (unsigned)cz, hb);
+    }
+    keys[j] = key;
+  } else {
+    if (B == 0) B = 4;
+    const int nbx = (I.ncx + B - 1) / B, nby = (I.ncy + B - 1) / B, nbz = (I.ncz + B - 1) / B;
+    keys[j] = c > 0 ? brick_key(I, c, B) : (unsigned)nbx * nby * nbz * (unsigned)(B * B * B);
+  }
   vals[j] = (int)j;
 }
 #endif
