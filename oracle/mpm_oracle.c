@@ -33,6 +33,14 @@ struct oracle {
   double* g;            /* GF_COUNT * nnode */
   int8_t* fix;          /* 3 * nnode */
   double ijac, ijac4, idcell;
+  /* SGMP auxiliary grid: one node per cell centre (Grid.f90:50-65, 252-268) */
+  int sgmp;
+  int cncx, cncy, cncz, cncxy, cncell;   /* CenterNumCell* */
+  int8_t* bdt;          /* 3 * nnode: node_list%bd_type */
+  int32_t* cicell;      /* per particle: centericell, -1 outside */
+  int32_t* inode;       /* per particle: nearest node (1-based) */
+  double* xxn;          /* 3 * n: natural coords in the auxiliary cell */
+  double *cmg, *cp, *cfint, *cfext, *co, *cw, *cdefp, *cv, *ca;   /* per cell: 1,3,6,3,6,3,9,3,3 */
   /* time */
   int istep;
   double time, dt;
@@ -62,6 +70,382 @@ static int cell_node1(const oracle* o, int c) {
   int iy = r / o->ncx;
   int ix = r - iy * o->ncx;
   return iz * o->ngxy + iy * o->ngx + ix + 1;
+}
+
+/* ================================================================ SGMP */
+
+/* Auxiliary-grid sizes and the nodes' boundary types (Grid.f90:250-268,
+ * 296-316).  bd_type is 1 on low and 2 on high faces whose FixS is set; with
+ * SGMP also on the second node layer.  The assignments are made in the
+ * reference's statement order (a later one overrides). */
+static void sgmp_setup(oracle* o) {
+  const int* FixS = o->prob.fix;
+  o->cncx = o->ncx - 1;
+  o->cncy = o->ncy - 1;
+  o->cncz = o->ncz - 1;
+  if (o->cncx == 0) o->cncx = 1;
+  if (o->cncy == 0) o->cncy = 1;
+  if (o->cncz == 0) o->cncz = 1;
+  o->cncxy = o->cncx * o->cncy;
+  o->cncell = o->cncxy * o->cncz;
+  o->bdt = (int8_t*)calloc((size_t)3 * o->nnode, 1);
+  const int NGx = o->ngx, NGy = o->ngy, NGz = o->ngz;
+  for (int iz = 1; iz <= NGz; iz++)
+    for (int iy = 1; iy <= NGy; iy++)
+      for (int ix = 1; ix <= NGx; ix++) {
+        int k = (iz - 1) * o->ngxy + (iy - 1) * NGx + ix - 1;
+        int8_t* b0 = &o->bdt[k];
+        int8_t* b1 = &o->bdt[o->nnode + k];
+        int8_t* b2 = &o->bdt[2 * o->nnode + k];
+        if (ix == 1 && FixS[0] != 0) *b0 = 1;
+        if (ix == NGx && FixS[1] != 0) *b0 = 2;
+        if (iy == 1 && FixS[2] != 0) *b1 = 1;
+        if (iy == NGy && FixS[3] != 0) *b1 = 2;
+        if (iz == 1 && FixS[4] != 0) *b2 = 1;
+        if (iz == NGz && FixS[5] != 0) *b2 = 2;
+        if (ix == 2 && FixS[0] != 0) *b0 = 1;
+        if (ix == NGx - 1 && FixS[1] != 0) *b0 = 2;
+        if (iy == 2 && FixS[2] != 0) *b1 = 1;
+        if (iy == NGy - 1 && FixS[3] != 0) *b1 = 2;
+        if (iz == 2 && FixS[4] != 0) *b2 = 1;
+        if (iz == NGz - 1 && FixS[5] != 0) *b2 = 2;
+      }
+  o->cicell = (int32_t*)calloc(o->n, sizeof(int32_t));
+  o->inode = (int32_t*)calloc(o->n, sizeof(int32_t));
+  o->xxn = (double*)calloc((size_t)3 * o->n, sizeof(double));
+  const size_t nc = (size_t)o->ncell;   /* nb_centernode = NumCell */
+  o->cmg = (double*)calloc(nc, sizeof(double));
+  o->cp = (double*)calloc(3 * nc, sizeof(double));
+  o->cfint = (double*)calloc(6 * nc, sizeof(double));
+  o->cfext = (double*)calloc(3 * nc, sizeof(double));
+  o->co = (double*)calloc(6 * nc, sizeof(double));
+  o->cw = (double*)calloc(3 * nc, sizeof(double));
+  o->cdefp = (double*)calloc(9 * nc, sizeof(double));
+  o->cv = (double*)calloc(3 * nc, sizeof(double));
+  o->ca = (double*)calloc(3 * nc, sizeof(double));
+}
+
+/* CenterInWhichCell (Grid.f90:157-190): int() truncates toward zero, and
+ * the x index is not fenced per axis (it wraps into the next row) */
+static int center_in_which_cell(const oracle* o, const double x[3]) {
+  const double* s = o->prob.span;
+  const double h = o->prob.dcell;
+  int ix = (int)((x[0] - s[0] - h / 2.0) / h) + 1;
+  int iy = (int)((x[1] - s[2] - h / 2.0) / h) + 1;
+  int iz = (int)((x[2] - s[4] - h / 2.0) / h) + 1;
+  int c = (iz - 1) * o->cncxy + (iy - 1) * o->cncx + ix;
+  if (c > o->cncell || c <= 0) c = -1;
+  return c;
+}
+
+/* CenterCellNode(ac, 1..8) (Grid.f90:386-403): the 8 cell centres (1-based
+ * cell indices) around auxiliary cell ac, in SNX/SNY/SNZ order */
+static void center_cell_nodes(const oracle* o, int ac, int cn[8]) {
+  int a0 = ac - 1;
+  int iz = a0 / o->cncxy;
+  int r = a0 - iz * o->cncxy;
+  int iy = r / o->cncx;
+  int ix = r - iy * o->cncx + 1;
+  int n1 = iz * o->ncxy + iy * o->ncx + ix;
+  int n5 = n1 + o->ncxy;
+  cn[0] = n1; cn[1] = n1 + 1; cn[2] = n1 + 1 + o->ncx; cn[3] = n1 + o->ncx;
+  cn[4] = n5; cn[5] = n5 + 1; cn[6] = n5 + 1 + o->ncx; cn[7] = n5 + o->ncx;
+}
+
+/* CellsNode(c, 1..8) (Grid.f90:366-383), 1-based node indices */
+static void cell_nodes(const oracle* o, int c, int cn[8]);
+
+/* SGNShape (Grid.f90:434-458) */
+static void sgn_shape(const double xi[3], double shp[8]) {
+  static const int SNX[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, SNY[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                   SNZ[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+  for (int i = 0; i < 8; i++) {
+    double sx = SNX[i] * xi[0] + 1.0, sy = SNY[i] * xi[1] + 1.0, sz = SNZ[i] * xi[2] + 1.0;
+    shp[i] = sx * sy * sz * 0.125;
+  }
+}
+
+/* CNShape (Grid.f90:503-544): derivatives at the cell centre, SN * iJacobi4 */
+static void cn_shape(const oracle* o, double dx[8], double dy[8], double dz[8]) {
+  static const int SNX[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, SNY[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                   SNZ[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+  for (int i = 0; i < 8; i++) {
+    double sx = SNX[i] * 0.0 + 1.0, sy = SNY[i] * 0.0 + 1.0, sz = SNZ[i] * 0.0 + 1.0;
+    dx[i] = SNX[i] * sy * sz * o->ijac4;
+    dy[i] = SNY[i] * sx * sz * o->ijac4;
+    dz[i] = SNZ[i] * sx * sy * o->ijac4;
+  }
+}
+
+static inline int in_aux(const oracle* o, int c) { return c <= o->ncell && c > 0; }
+
+static void cell_nodes(const oracle* o, int c, int cn[8]) {
+  int n1 = cell_node1(o, c);
+  cn[0] = n1; cn[1] = n1 + 1; cn[2] = n1 + 1 + o->ngx; cn[3] = n1 + o->ngx;
+  cn[4] = n1 + o->ngxy; cn[5] = n1 + o->ngxy + 1; cn[6] = n1 + o->ngxy + 1 + o->ngx; cn[7] = n1 + o->ngxy + o->ngx;
+}
+
+/* GridMomentumInitial, SGMP branch (update_step.f90:318-404): particles to
+ * the auxiliary nodes, then each cell centre with Cmg > CutOff spreads 1/8
+ * of its mass and momentum to the cell's 8 nodes, cells in ascending order */
+static void sgmp_momentum_initial(oracle* o) {
+  const size_t nc = (size_t)o->ncell;
+  memset(o->cmg, 0, nc * sizeof(double));
+  memset(o->cp, 0, 3 * nc * sizeof(double));
+  double shp[8];
+  int cn[8];
+  for (int p = 0; p < o->n; p++) {
+    if (o->cicell[p] < 0) continue;
+    double mp = P(o, PF_MASS, p);
+    center_cell_nodes(o, o->cicell[p], cn);
+    sgn_shape(&o->xxn[3 * p], shp);
+    for (int i = 0; i < 8; i++) {
+      if (!in_aux(o, cn[i])) continue;
+      int k = cn[i] - 1;
+      double shm = shp[i] * mp;
+      o->cmg[k] = o->cmg[k] + shm;
+      for (int d = 0; d < 3; d++) o->cp[3 * k + d] = o->cp[3 * k + d] + P(o, PF_VX + d, p) * shm;
+    }
+  }
+  for (int c = 1; c <= o->ncell; c++) {
+    if (!(o->cmg[c - 1] > o->prob.cutoff)) continue;
+    cell_nodes(o, c, cn);
+    for (int j = 0; j < 8; j++) {
+      int k = cn[j] - 1;
+      G(o, GF_MG, k) = G(o, GF_MG, k) + 0.125 * o->cmg[c - 1];
+      for (int d = 0; d < 3; d++) G(o, GF_PX + d, k) = G(o, GF_PX + d, k) + 0.125 * o->cp[3 * (c - 1) + d];
+    }
+  }
+}
+
+/* ParticleStressUpdate, SGMP: strain rate, spin and velocity gradient at
+ * every cell centre from its 8 nodes (update_step.f90:497-525) */
+static void sgmp_cell_gradients(oracle* o) {
+  const size_t nc = (size_t)o->ncell;
+  memset(o->co, 0, 6 * nc * sizeof(double));
+  memset(o->cw, 0, 3 * nc * sizeof(double));
+  memset(o->cdefp, 0, 9 * nc * sizeof(double));
+  double DX[8], DY[8], DZ[8];
+  cn_shape(o, DX, DY, DZ);
+  int cn[8];
+  for (int c = 1; c <= o->ncell; c++) {
+    double* co = &o->co[6 * (c - 1)];
+    double* cw = &o->cw[3 * (c - 1)];
+    double* cf = &o->cdefp[9 * (c - 1)];
+    cell_nodes(o, c, cn);
+    for (int j = 0; j < 8; j++) {
+      int k = cn[j] - 1;
+      double mg = G(o, GF_MG, k);
+      if (!(mg > o->prob.cutoff)) continue;
+      double v[3] = {G(o, GF_PX, k) / mg, G(o, GF_PY, k) / mg, G(o, GF_PZ, k) / mg};
+      co[0] = co[0] + DX[j] * v[0];
+      co[1] = co[1] + DY[j] * v[1];
+      co[2] = co[2] + DZ[j] * v[2];
+      co[3] = co[3] + DZ[j] * v[1] + DY[j] * v[2];
+      co[4] = co[4] + DZ[j] * v[0] + DX[j] * v[2];
+      co[5] = co[5] + DY[j] * v[0] + DX[j] * v[1];
+      cw[0] = cw[0] + DY[j] * v[2] - DZ[j] * v[1];
+      cw[1] = cw[1] + DZ[j] * v[0] - DX[j] * v[2];
+      cw[2] = cw[2] + DX[j] * v[1] - DY[j] * v[0];
+      for (int n = 0; n < 3; n++) {
+        cf[3 * n + 0] = cf[3 * n + 0] + DX[j] * v[n];
+        cf[3 * n + 1] = cf[3 * n + 1] + DY[j] * v[n];
+        cf[3 * n + 2] = cf[3 * n + 2] + DZ[j] * v[n];
+      }
+    }
+  }
+}
+
+/* the particle's de, vort, deFp interpolated from the auxiliary nodes
+ * (update_step.f90:542-562); 0 if the particle has no auxiliary cell */
+static int sgmp_particle_strain(const oracle* o, int p, double de[6], double vort[3], double deFp[3][3]) {
+  if (o->cicell[p] < 0) return 0;
+  double shp[8];
+  int cn[8];
+  center_cell_nodes(o, o->cicell[p], cn);
+  sgn_shape(&o->xxn[3 * p], shp);
+  for (int i = 0; i < 8; i++) {
+    if (!in_aux(o, cn[i])) continue;
+    int k = cn[i] - 1;
+    for (int c = 0; c < 6; c++) de[c] = de[c] + o->co[6 * k + c] * shp[i];
+    for (int c = 0; c < 3; c++) vort[c] = vort[c] + o->cw[3 * k + c] * shp[i];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) deFp[a][b] = deFp[a][b] + shp[i] * o->cdefp[9 * k + 3 * a + b];
+  }
+  return 1;
+}
+
+/* GridMomentumUpdate, SGMP: particle forces to the auxiliary nodes
+ * (update_step.f90:906-930), then from every cell centre to its 8 nodes
+ * (:990-1011) */
+static void sgmp_force_particle(oracle* o, int p, const double fx[3], double vol, double sxx, double syy, double szz,
+                                double sxy, double syz, double sxz) {
+  double shp[8];
+  int cn[8];
+  center_cell_nodes(o, o->cicell[p], cn);
+  sgn_shape(&o->xxn[3 * p], shp);
+  for (int i = 0; i < 8; i++) {
+    if (!in_aux(o, cn[i])) continue;
+    int k = cn[i] - 1;
+    double N = shp[i];
+    for (int d = 0; d < 3; d++) o->cfext[3 * k + d] = o->cfext[3 * k + d] + fx[d] * N;
+    double* fi = &o->cfint[6 * k];
+    fi[0] = fi[0] + N * sxx * vol;
+    fi[1] = fi[1] + N * syy * vol;
+    fi[2] = fi[2] + N * szz * vol;
+    fi[3] = fi[3] + N * sxy * vol;
+    fi[4] = fi[4] + N * syz * vol;
+    fi[5] = fi[5] + N * sxz * vol;
+  }
+}
+
+static void sgmp_force_cells(oracle* o) {
+  double DX[8], DY[8], DZ[8];
+  cn_shape(o, DX, DY, DZ);
+  int cn[8];
+  for (int c = 1; c <= o->ncell; c++) {
+    const double* fi = &o->cfint[6 * (c - 1)];
+    const double* fe = &o->cfext[3 * (c - 1)];
+    cell_nodes(o, c, cn);
+    for (int j = 0; j < 8; j++) {
+      int k = cn[j] - 1;
+      double f[3];
+      f[0] = -(fi[0] * DX[j] + fi[3] * DY[j] + fi[5] * DZ[j]);
+      f[1] = -(fi[3] * DX[j] + fi[1] * DY[j] + fi[4] * DZ[j]);
+      f[2] = -(fi[5] * DX[j] + fi[4] * DY[j] + fi[2] * DZ[j]);
+      for (int d = 0; d < 3; d++) G(o, GF_FX + d, k) = G(o, GF_FX + d, k) + 0.125 * fe[d] + f[d];
+    }
+  }
+}
+
+/* ParticlePositionUpdate, SGMP: nodal velocity and acceleration averaged
+ * to every cell centre (update_step.f90:1405-1417), overwritten on cells of
+ * fixed or symmetric faces by BoundaryAuxiliaryNodeUpdate (:1528-1666) */
+static void sgmp_cell_kinematics(oracle* o) {
+  const size_t nc = (size_t)o->ncell;
+  memset(o->cv, 0, 3 * nc * sizeof(double));
+  memset(o->ca, 0, 3 * nc * sizeof(double));
+  int cn[8];
+  for (int c = 1; c <= o->ncell; c++) {
+    cell_nodes(o, c, cn);
+    for (int j = 0; j < 8; j++) {
+      int k = cn[j] - 1;
+      double mg = G(o, GF_MG, k);
+      if (!(mg > o->prob.cutoff)) continue;
+      for (int d = 0; d < 3; d++) {
+        o->cv[3 * (c - 1) + d] = o->cv[3 * (c - 1) + d] + 0.125 * G(o, GF_PX + d, k) / mg;
+        o->ca[3 * (c - 1) + d] = o->ca[3 * (c - 1) + d] + 0.125 * G(o, GF_FX + d, k) / mg;
+      }
+    }
+  }
+  /* BoundaryAuxiliaryNodeUpdate: for each axis with more than one cell and
+   * each of its two faces with FixS set, the face's cells take the nodal
+   * values interpolated at their face point (natural coordinate +-1 towards
+   * the inside along every axis where the cell is a first or last one) */
+  const int* FixS = o->prob.fix;
+  const int nc3[3] = {o->ncx, o->ncy, o->ncz};
+  for (int ax = 0; ax < 3; ax++) {
+    if (nc3[ax] <= 1) continue;
+    for (int side = 0; side < 2; side++) {
+      if (FixS[2 * ax + side] == 0) continue;
+      const int fixed = side == 0 ? 1 : nc3[ax];
+      for (int iz = 1; iz <= o->ncz; iz++)
+        for (int iy = 1; iy <= o->ncy; iy++)
+          for (int ix = 1; ix <= o->ncx; ix++) {
+            const int id3[3] = {ix, iy, iz};
+            if (id3[ax] != fixed) continue;
+            double cxx[3] = {0.0, 0.0, 0.0};
+            for (int a = 0; a < 3; a++) {
+              if (a != ax && nc3[a] <= 1) continue;
+              if (id3[a] == 1) cxx[a] = 1.0;
+              if (id3[a] == nc3[a]) cxx[a] = -1.0;
+            }
+            int c = ix + (iy - 1) * o->ncx + (iz - 1) * o->ncxy;
+            double shp[8], v[3] = {0, 0, 0}, a3[3] = {0, 0, 0};
+            sgn_shape(cxx, shp);
+            cell_nodes(o, c, cn);
+            for (int j = 0; j < 8; j++) {
+              int k = cn[j] - 1;
+              double mg = G(o, GF_MG, k);
+              if (!(mg > o->prob.cutoff)) continue;
+              for (int d = 0; d < 3; d++) {
+                v[d] = v[d] + shp[j] * G(o, GF_PX + d, k) / mg;
+                a3[d] = a3[d] + shp[j] * G(o, GF_FX + d, k) / mg;
+              }
+            }
+            for (int d = 0; d < 3; d++) {
+              o->cv[3 * (c - 1) + d] = v[d];
+              o->ca[3 * (c - 1) + d] = a3[d];
+            }
+          }
+    }
+  }
+}
+
+/* the particle's velocity and acceleration from the auxiliary nodes
+ * (update_step.f90:1434-1473); particles whose nearest node lies on a fixed
+ * or symmetric face use AuxiliaryGridSHP (Grid.f90:193-220, its integer
+ * p_type read as /= 0) */
+static int sgmp_particle_kinematics(const oracle* o, int p, double vx[3], double ax[3]) {
+  if (o->cicell[p] < 0) return 0;
+  int cn[8];
+  center_cell_nodes(o, o->cicell[p], cn);
+  const int nd = o->inode[p] - 1;
+  int pt[3] = {0, 0, 0};
+  if (nd >= 0 && nd < o->nnode) {   /* (outside the grid the reference reads out of bounds) */
+    pt[0] = o->bdt[nd];
+    pt[1] = o->bdt[o->nnode + nd];
+    pt[2] = o->bdt[2 * o->nnode + nd];
+  }
+  double xi[3];
+  for (int d = 0; d < 3; d++) {
+    if (pt[d] != 0) {
+      double xn = o->xn[3 * p + d];
+      xi[d] = xn < 0 ? 1 + 2 * xn : 2 * xn - 1;
+    } else {
+      xi[d] = o->xxn[3 * p + d];
+    }
+  }
+  double shp[8];
+  sgn_shape(xi, shp);
+  for (int i = 0; i < 8; i++) {
+    if (!in_aux(o, cn[i])) continue;
+    int k = cn[i] - 1;
+    for (int d = 0; d < 3; d++) {
+      vx[d] = vx[d] + shp[i] * o->cv[3 * k + d];
+      ax[d] = ax[d] + shp[i] * o->ca[3 * k + d];
+    }
+  }
+  return 1;
+}
+
+/* GridMomentumMUSL, SGMP (update_step.f90:1689-1707, 1760-1768): momentum
+ * re-map through the auxiliary nodes, spread to the nodes without the
+ * CutOff test */
+static void sgmp_momentum_musl(oracle* o) {
+  const size_t nc = (size_t)o->ncell;
+  memset(o->cp, 0, 3 * nc * sizeof(double));
+  double shp[8];
+  int cn[8];
+  for (int p = 0; p < o->n; p++) {
+    if (o->cicell[p] < 0) continue;
+    double mp = P(o, PF_MASS, p);
+    center_cell_nodes(o, o->cicell[p], cn);
+    sgn_shape(&o->xxn[3 * p], shp);
+    for (int i = 0; i < 8; i++) {
+      if (!in_aux(o, cn[i])) continue;
+      int k = cn[i] - 1;
+      double shm = shp[i] * mp;
+      for (int d = 0; d < 3; d++) o->cp[3 * k + d] = o->cp[3 * k + d] + P(o, PF_VX + d, p) * shm;
+    }
+  }
+  for (int c = 1; c <= o->ncell; c++) {
+    cell_nodes(o, c, cn);
+    for (int j = 0; j < 8; j++) {
+      int k = cn[j] - 1;
+      for (int d = 0; d < 3; d++) G(o, GF_PX + d, k) = G(o, GF_PX + d, k) + 0.125 * o->cp[3 * (c - 1) + d];
+    }
+  }
 }
 
 oracle* oracle_create(const mpm_problem* p) {
@@ -114,6 +498,8 @@ oracle* oracle_create(const mpm_problem* p) {
         o->fix[o->nnode + k] = (int8_t)fy;
         o->fix[2 * o->nnode + k] = (int8_t)fz;
       }
+  o->sgmp = p->shape == MPM_SHAPE_SGMP;
+  if (o->sgmp) sgmp_setup(o);
   o->ijac = 2.0 / p->dcell;          /* Grid.f90:414 */
   o->ijac4 = o->ijac * 0.125;        /* :415 */
   o->idcell = 1.0 / p->dcell;        /* :416 */
@@ -128,6 +514,9 @@ void oracle_destroy(oracle* o) {
   if (!o) return;
   free(o->mats); free(o->bodies); free(o->ps); free(o->pbody); free(o->icell);
   free(o->xn); free(o->g); free(o->fix);
+  free(o->bdt); free(o->cicell); free(o->inode); free(o->xxn);
+  free(o->cmg); free(o->cp); free(o->cfint); free(o->cfext); free(o->co); free(o->cw); free(o->cdefp);
+  free(o->cv); free(o->ca);
   free(o);
 }
 
@@ -170,6 +559,30 @@ static void particle_initial(oracle* o) {
     } else {
       /* the reference reads cell_list(-1) here; the value is never used */
       o->xn[3 * p] = o->xn[3 * p + 1] = o->xn[3 * p + 2] = 0.0;
+    }
+    if (o->sgmp) {   /* update_step.f90:432-449 */
+      const double* sp = o->prob.span;
+      const double h = o->prob.dcell;
+      o->cicell[p] = center_in_which_cell(o, x);
+      int ix = (int)((x[0] - sp[0]) / h + 0.5) + 1;
+      int iy = (int)((x[1] - sp[2]) / h + 0.5) + 1;
+      int iz = (int)((x[2] - sp[4]) / h + 0.5) + 1;
+      int nd = (iz - 1) * o->ngxy + (iy - 1) * o->ngx + ix;
+      o->inode[p] = nd;
+      if (nd < 1 || nd > o->nnode) {   /* outside the grid: undefined in the reference */
+        o->xxn[3 * p] = o->xxn[3 * p + 1] = o->xxn[3 * p + 2] = 0.0;
+        continue;
+      }
+      double xg[3];
+      node_xg(o, nd, xg);
+      for (int d = 0; d < 3; d++) {
+        double v = (x[d] - xg[d]) * o->ijac;
+        if (o->bdt[(size_t)d * o->nnode + nd - 1] != 0) {
+          if (v > 0) v = 1;
+          else if (v < 0) v = -1;
+        }
+        o->xxn[3 * p + d] = v;
+      }
     }
   }
 }
@@ -375,6 +788,10 @@ static inline int in_grid(const oracle* o, int k) { return k <= o->nnode && k > 
 void oracle_grid_momentum_initial(oracle* o) {
   for (int k = 0; k < o->nnode; k++) G(o, GF_MG, k) = G(o, GF_PX, k) = G(o, GF_PY, k) = G(o, GF_PZ, k) = 0.0;
   particle_initial(o);
+  if (o->sgmp) {
+    sgmp_momentum_initial(o);
+    return;
+  }
   shape_t s;
   for (int p = 0; p < o->n; p++) {
     if (o->icell[p] < 0) continue;
@@ -403,6 +820,10 @@ void oracle_apply_bc(oracle* o) {
 }
 
 void oracle_grid_momentum_update(oracle* o) {
+  if (o->sgmp) {
+    memset(o->cfint, 0, 6 * (size_t)o->ncell * sizeof(double));
+    memset(o->cfext, 0, 3 * (size_t)o->ncell * sizeof(double));
+  }
   for (int k = 0; k < o->nnode; k++) G(o, GF_FX, k) = G(o, GF_FY, k) = G(o, GF_FZ, k) = 0.0;
   const mpm_problem* pr = &o->prob;
   shape_t s;
@@ -424,6 +845,13 @@ void oracle_grid_momentum_update(oracle* o) {
       for (int d = 0; d < 3; d++) fx[d] = fx[d] + P(o, PF_MASS, p) * b->grav[d];
     }
     double vol = P(o, PF_VOL, p);
+    if (o->sgmp) {
+      if (o->cicell[p] < 0) continue;
+      double smq = P(o, PF_SM, p) - P(o, PF_Q, p);
+      sgmp_force_particle(o, p, fx, vol, smq + P(o, PF_SDXX, p), smq + P(o, PF_SDYY, p), smq + P(o, PF_SDZZ, p),
+                          P(o, PF_SDXY, p), P(o, PF_SDYZ, p), P(o, PF_SDXZ, p));
+      continue;
+    }
     if (o->icell[p] < 0) continue;
     double smq = P(o, PF_SM, p) - P(o, PF_Q, p);
     double sxx = smq + P(o, PF_SDXX, p);
@@ -444,6 +872,7 @@ void oracle_grid_momentum_update(oracle* o) {
       G(o, GF_FZ, k) = G(o, GF_FZ, k) + fi2 + fx[2] * N;
     }
   }
+  if (o->sgmp) sgmp_force_cells(o);
 }
 
 void oracle_integrate_momentum(oracle* o) {
@@ -463,10 +892,16 @@ void oracle_integrate_momentum(oracle* o) {
 void oracle_particle_position_update(oracle* o) {
   shape_t s;
   const double cutoff = o->prob.cutoff;
+  if (o->sgmp) sgmp_cell_kinematics(o);
   for (int p = 0; p < o->n; p++) {
-    if (o->icell[p] < 0) continue;
     double vx[3] = {0, 0, 0}, ax[3] = {0, 0, 0};
-    shape_of(o, p, &s);
+    if (o->sgmp) {
+      if (!sgmp_particle_kinematics(o, p, vx, ax)) continue;
+      s.nb = 0;
+    } else {
+      if (o->icell[p] < 0) continue;
+      shape_of(o, p, &s);
+    }
     for (int i = 0; i < s.nb; i++) {
       int k = s.node[i];
       if (!in_grid(o, k)) continue;
@@ -492,6 +927,11 @@ void oracle_particle_position_update(oracle* o) {
 
 void oracle_grid_momentum_musl(oracle* o) {
   for (int k = 0; k < o->nnode; k++) G(o, GF_PX, k) = G(o, GF_PY, k) = G(o, GF_PZ, k) = 0.0;
+  if (o->sgmp) {
+    sgmp_momentum_musl(o);
+    oracle_apply_bc(o);
+    return;
+  }
   shape_t s;
   for (int p = 0; p < o->n; p++) {
     if (o->icell[p] < 0) continue;
@@ -839,11 +1279,17 @@ static void constitution(oracle* o, int p, const double de[6], const double vort
 void oracle_particle_stress_update(oracle* o) {
   shape_t s;
   const double cutoff = o->prob.cutoff;
+  if (o->sgmp) sgmp_cell_gradients(o);
   for (int p = 0; p < o->n; p++) {
     double de[6] = {0, 0, 0, 0, 0, 0}, vort[3] = {0, 0, 0}, deFp[3][3];
     memset(deFp, 0, sizeof deFp);
-    if (o->icell[p] < 0) continue;
-    shape_of(o, p, &s);
+    if (o->sgmp) {
+      if (!sgmp_particle_strain(o, p, de, vort, deFp)) continue;
+      s.nb = 0;
+    } else {
+      if (o->icell[p] < 0) continue;
+      shape_of(o, p, &s);
+    }
     for (int i = 0; i < s.nb; i++) {
       int k = s.node[i];
       if (!in_grid(o, k)) continue;

@@ -34,6 +34,10 @@ if ORACLE_DIR not in sys.path:
 
 DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
          "plate_neoh_musl_qulo", "dam_null_eos4_musl"]
+# auxiliary-grid (SGMP) decks: clamped plate (MUSL), water column with
+# symmetric walls (MUSL), J2 block on a clamped floor (USF)
+SGMP_DECKS = ["plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf"]
+ALL_DECKS = DECKS + SGMP_DECKS
 
 
 def deck_path(name):

@@ -13,7 +13,8 @@ every deck in tests/golden/decks/ and stores, per deck:
 * ``scal0`` / ``scal10``: DT, time, EngKinetic, EngInternal, Momentum(3),
   CutOff, DCell, Span(6), DTScale.
 
-Only run in the container that has the reference:  python tests/golden/make_golden.py
+Only run in the container that has the reference:
+    python tests/golden/make_golden.py [deck ...]
 """
 import os
 import shutil
@@ -29,7 +30,8 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import pyoracle  # noqa: E402
 
 DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
-         "plate_neoh_musl_qulo", "dam_null_eos4_musl"]
+         "plate_neoh_musl_qulo", "dam_null_eos4_musl",
+         "plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf"]
 S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp"]
 
 
@@ -38,10 +40,10 @@ def scal(d):
                            d["span"], [d["dtscale"]]])
 
 
-def main():
+def main(names):
     if not os.path.exists(pyoracle.REF_DRIVER):
         subprocess.run([os.path.join(ROOT, "oracle", "build_ref.sh")], check=True)
-    for name in DECKS:
+    for name in names:
         tmp = tempfile.mkdtemp(prefix="golden_")
         try:
             shutil.copy(os.path.join(HERE, "decks", name + ".mpm"), tmp)
@@ -62,4 +64,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or DECKS)   # deck names, default all

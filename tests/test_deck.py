@@ -9,7 +9,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import DECKS, GOLDEN_DIR, deck_path, load_package
+from conftest import ALL_DECKS as DECKS, GOLDEN_DIR, deck_path, load_package
 
 S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp"]
 

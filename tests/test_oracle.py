@@ -15,7 +15,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import DECKS, GOLDEN_DIR, deck_path, load_package
+from conftest import ALL_DECKS as DECKS, GOLDEN_DIR, deck_path, load_package
 
 PF_SKIP_NAN = ()
 
@@ -99,7 +99,7 @@ endi
     return str(p)
 
 
-@pytest.mark.parametrize("shape", ["", "gimp", "bimp"])
+@pytest.mark.parametrize("shape", ["", "gimp", "bimp", "sgmp"])
 def test_p2g_conserves_mass_and_momentum(tmp_path, shape, oracle_mod):
     mpm = load_package()
     prob = mpm.read_deck(_block_deck(tmp_path, shape, "musl on"))
@@ -114,7 +114,7 @@ def test_p2g_conserves_mass_and_momentum(tmp_path, shape, oracle_mod):
         assert abs(g[1 + d].sum() - ref) <= 1e-13 * abs(ref)
 
 
-@pytest.mark.parametrize("shape", ["", "gimp", "bimp"])
+@pytest.mark.parametrize("shape", ["", "gimp", "bimp", "sgmp"])
 def test_uniform_velocity_is_transported_exactly(tmp_path, shape, oracle_mod):
     """Unstressed body, no loads: in the first step every particle keeps its
     velocity, and particles whose whole stencil lies inside the body (all
