@@ -23,6 +23,8 @@ struct InstC {
   int mat_off, nmat;     // material range in the material table
   int cell_off;          // sort-key offset of this instance's cells
   int max_steps;         // > 0: the instance stops itself after this many steps
+  int cncx, cncxy, cncell;   // SGMP auxiliary cells: CenterNumCellx, ..xy, CenterNumCell
+  int fixs[6];           // FixS of the deck (x0 xn y0 yn z0 zn)
   double span0[3];       // SpanX(1), SpanY(1), SpanZ(1)
   double span1[3];       // adjusted upper ends
   double dcell, idcell, ijac, ijac4, cutoff;
@@ -107,7 +109,9 @@ struct PoolPtrs {
   T* p2;         // 3 * nn  MUSL re-mapped momentum
   T* vg;         // 3 * nn  nodal velocity PXg/Mg (0 where Mg <= CutOff)
   T* ag;         // 3 * nn  nodal acceleration FXg/Mg (0 where Mg <= CutOff)
-  uint8_t* fix;  // nn, bit0 x, bit1 y, bit2 z
+  uint8_t* fix;  // nn, bit0-2 fixity x/y/z; bit3-5 SGMP boundary node (bd_type /= 0) x/y/z
+  T* aux;        // AUX_COUNT * nn  SGMP cell-centre (auxiliary node) fields, indexed by cell
+  double* cmg;   // nn  SGMP auxiliary node mass
   long nn;
   const InstC* inst;
   InstT* ts;

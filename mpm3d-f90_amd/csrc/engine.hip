@@ -158,6 +158,8 @@ struct mpm_engine {
   void* vg = nullptr;
   void* ag = nullptr;
   uint8_t* fix = nullptr;
+  void* aux = nullptr;            // SGMP auxiliary-node fields (allocated with the first SGMP instance)
+  double* cmg = nullptr;
   InstC* d_inst = nullptr;
   InstT* d_ts = nullptr;
   MatC* d_mats = nullptr;
@@ -216,6 +218,8 @@ struct mpm_engine {
     P.vg = (T*)vg;
     P.ag = (T*)ag;
     P.fix = fix;
+    P.aux = (T*)aux;
+    P.cmg = cmg;
     P.nn = nn;
     P.inst = d_inst;
     P.ts = d_ts;
@@ -234,7 +238,7 @@ static void free_engine(mpm_engine* e) {
   if (e->graph) (void)hipGraphDestroy(e->graph);
   void* bufs[] = {e->t, e->d, e->ii, e->mg, e->pg, e->fg, e->p2, e->vg, e->ag, e->fix, e->d_inst, e->d_ts, e->d_mats,
                   e->d_bodies, e->part, e->d_ginst, e->d_nginst, e->d_ilist_all, e->k_in, e->k_out, e->v_in,
-                  e->v_out, e->cub_tmp, e->t_alt, e->d_alt, e->i_alt, e->d_seg};
+                  e->v_out, e->cub_tmp, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& g : e->groups) {
@@ -346,9 +350,7 @@ static void check_problem(const mpm_problem* p) {
   if (!p || p->nb_particle <= 0 || p->nb_mat <= 0 || p->nb_body <= 0 || !p->pstate || !p->pbody || !p->mats ||
       !p->bodies)
     fail(MPM_ERR_ARG, "incomplete problem description");
-  if (p->shape == MPM_SHAPE_SGMP)
-    fail(MPM_ERR_UNSUPPORTED, "SGMP (auxiliary-grid) shape functions are not available on the device path");
-  if (p->shape < 0 || p->shape > 2 || p->scheme < 0 || p->scheme > 2) fail(MPM_ERR_ARG, "bad shape/scheme");
+  if (p->shape < 0 || p->shape > 3 || p->scheme < 0 || p->scheme > 2) fail(MPM_ERR_ARG, "bad shape/scheme");
   for (int m = 0; m < p->nb_mat; m++) {
     int t = p->mats[m].type;
     if (t != MPM_MAT_ELASTIC && t != MPM_MAT_PLA1 && t != MPM_MAT_PLA2 && t != MPM_MAT_NULL && t != MPM_MAT_DP &&
@@ -424,6 +426,13 @@ static void upload_particles(mpm_engine* e, const HostInst& h, const double* ps,
 }
 
 // add one instance to the pool; returns instance id
+static void ensure_aux(mpm_engine* e) {
+  if (e->aux) return;
+  e->aux = dalloc<char>((size_t)AUX_COUNT * e->nn * e->tsz);
+  e->cmg = dalloc<double>(e->nn);
+  e->graph_dirty = true;
+}
+
 static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   check_problem(p);
   if (e->free_ids.empty()) fail(MPM_ERR_NOMEM, "no free instance slot");
@@ -483,6 +492,12 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   c.nmat = p->nb_mat;
   c.cell_off = 0;
   c.max_steps = 0;
+  // SGMP auxiliary cells (Grid.f90:254-268)
+  c.cncx = std::max(c.ncx - 1, 1);
+  c.cncxy = c.cncx * std::max(c.ncy - 1, 1);
+  c.cncell = c.cncxy * std::max(c.ncz - 1, 1);
+  for (int k = 0; k < 6; k++) c.fixs[k] = p->fix[k];
+  if (p->shape == MPM_SHAPE_SGMP) ensure_aux(e);
   for (int k = 0; k < 3; k++) {
     c.span0[k] = p->span[2 * k];
     c.span1[k] = p->span[2 * k + 1];
@@ -553,7 +568,14 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
           if ((ix == 1 && FixS[0] == 2) || (ix == NGx && FixS[1] == 2)) bx = 1;
           if ((iy == 1 && FixS[2] == 2) || (iy == NGy && FixS[3] == 2)) by = 1;
           if ((iz == 1 && FixS[4] == 2) || (iz == NGz && FixS[5] == 2)) bz = 1;
-          fx[k] = (uint8_t)(bx | (by << 1) | (bz << 2));
+          // SGMP boundary nodes: bd_type /= 0 (Grid.f90:303-316), first two layers of a face with FixS set
+          int qx = 0, qy = 0, qz = 0;
+          if (p->shape == MPM_SHAPE_SGMP) {
+            qx = ((ix == 1 || ix == 2) && FixS[0] != 0) || ((ix == NGx || ix == NGx - 1) && FixS[1] != 0);
+            qy = ((iy == 1 || iy == 2) && FixS[2] != 0) || ((iy == NGy || iy == NGy - 1) && FixS[3] != 0);
+            qz = ((iz == 1 || iz == 2) && FixS[4] != 0) || ((iz == NGz || iz == NGz - 1) && FixS[5] != 0);
+          }
+          fx[k] = (uint8_t)(bx | (by << 1) | (bz << 2) | (qx << 3) | (qy << 4) | (qz << 5));
         }
     HIPCHK(hipMemcpy(e->fix + ng0 * NGRAN, fx.data(), fx.size(), hipMemcpyHostToDevice));
   }
@@ -675,11 +697,19 @@ static void run_phase(mpm_engine* e, int ph) {
   }
   for (Group& G : e->groups) {
     const LaunchP L = plist(e, G), N = nlist(e, G);
+    const bool sg = G.shape == MPM_SHAPE_SGMP;
     switch (ph) {
       case PH_GMI:
-        mpmk::launch_zero_nodes<double>(e->mg, 1, e->nn, N, s);
-        mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
-        mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, L, s);
+        if (sg) {   // particles -> auxiliary nodes -> grid nodes
+          mpmk::launch_zero_nodes<double>(e->cmg, 1, e->nn, N, s);
+          mpmk::launch_zero_nodes<T>((T*)e->aux, 3, e->nn, N, s);
+          mpmk::launch_sgmp_p2g<T>(P2G_ICELL | P2G_MASS | P2G_MOM, P, L, s);
+          mpmk::launch_sgmp_nodes<T>(SGN_MASSMOM, P, N, s);
+        } else {
+          mpmk::launch_zero_nodes<double>(e->mg, 1, e->nn, N, s);
+          mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
+          mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, L, s);
+        }
         for (int id : G.insts) e->inst[id].p_in_p2 = false;
         break;
       case PH_BC:
@@ -687,24 +717,38 @@ static void run_phase(mpm_engine* e, int ph) {
         break;
       case PH_STRESS:
         mpmk::launch_bc<T>(P, N, 0, 0, 1, s);   // nodal velocity of the current PXg
+        if (sg) mpmk::launch_sgmp_cells<T>(SGC_GRAD, P, N, s);
         mpmk::launch_stress<T>(G.shape, G.scheme == MPM_USF ? 0 : ST_MOVE, P, L, s);
         for (int id : G.insts) e->inst[id].xx_pending = false;
         break;
       case PH_GMU:
-        mpmk::launch_zero_nodes<T>((T*)e->fg, 3, e->nn, N, s);
-        mpmk::launch_p2g<T>(G.shape, P2G_FORCE, P, L, s);
+        if (sg) {
+          mpmk::launch_zero_nodes<T>((T*)e->aux + AUX_FORCE_OFF * e->nn, 9, e->nn, N, s);
+          mpmk::launch_sgmp_p2g<T>(P2G_FORCE, P, L, s);
+          mpmk::launch_sgmp_nodes<T>(SGN_FORCE, P, N, s);
+        } else {
+          mpmk::launch_zero_nodes<T>((T*)e->fg, 3, e->nn, N, s);
+          mpmk::launch_p2g<T>(G.shape, P2G_FORCE, P, L, s);
+        }
         break;
       case PH_INTEGRATE:
         mpmk::launch_integrate<T>(P, N, 0, s);
         break;
       case PH_G2P:
         mpmk::launch_node_va<T>(P, N, s);
+        if (sg) mpmk::launch_sgmp_cells<T>(SGC_KIN, P, N, s);
         mpmk::launch_g2p<T>(G.shape, G.scheme == MPM_USF ? G2P_MOVE : 0, P, L, s);
         for (int id : G.insts) e->inst[id].xx_pending = G.scheme != MPM_USF;
         break;
       case PH_MUSL:
-        mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
-        mpmk::launch_p2g<T>(G.shape, P2G_MOM, P, L, s);
+        if (sg) {
+          mpmk::launch_zero_nodes<T>((T*)e->aux, 3, e->nn, N, s);
+          mpmk::launch_sgmp_p2g<T>(P2G_MOM, P, L, s);
+          mpmk::launch_sgmp_nodes<T>(SGN_MOM, P, N, s);
+        } else {
+          mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
+          mpmk::launch_p2g<T>(G.shape, P2G_MOM, P, L, s);
+        }
         mpmk::launch_bc<T>(P, N, 0, 1, 0, s);
         break;
       case PH_ENERGY:
@@ -751,6 +795,60 @@ static void timed(mpm_engine* e, int kid, F&& launch) {
   e->ev_kid.push_back(kid);
 }
 
+// one step of an SGMP group: particle passes through the auxiliary nodes,
+// with the centre <-> node passes between them (timed with their phase)
+template <typename T>
+static void enqueue_sgmp(mpm_engine* e, Group& G, const PoolPtrs<T>& P, const LaunchP& L, const LaunchP& N) {
+  hipStream_t s = e->s;
+  const long nn = e->nn;
+  T* aux = (T*)e->aux;
+  auto p2g = [&](int mode) {
+    timed(e, KI_P2G, [&] {
+      mpmk::launch_zero_nodes<double>(e->cmg, 1, nn, N, s);
+      mpmk::launch_zero_nodes<T>(aux, (mode & P2G_FORCE) ? 12 : 3, nn, N, s);
+      mpmk::launch_sgmp_p2g<T>(mode, P, L, s);
+      mpmk::launch_sgmp_nodes<T>((mode & P2G_FORCE) ? SGN_MASSMOM | SGN_FORCE : SGN_MASSMOM, P, N, s);
+    });
+  };
+  if (G.scheme == MPM_USF) {
+    p2g(P2G_ICELL | P2G_MASS | P2G_MOM);
+    timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 0, 1, 1, s); });
+    timed(e, KI_STRESS, [&] {
+      mpmk::launch_sgmp_cells<T>(SGC_GRAD, P, N, s);
+      mpmk::launch_stress<T>(MPM_SHAPE_SGMP, 0, P, L, s);
+    });
+    timed(e, KI_P2G, [&] {
+      mpmk::launch_zero_nodes<T>(aux + AUX_FORCE_OFF * nn, 9, nn, N, s);
+      mpmk::launch_sgmp_p2g<T>(P2G_FORCE, P, L, s);
+      mpmk::launch_sgmp_nodes<T>(SGN_FORCE, P, N, s);
+    });
+    timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
+    timed(e, KI_G2P, [&] {
+      mpmk::launch_sgmp_cells<T>(SGC_KIN, P, N, s);
+      mpmk::launch_g2p<T>(MPM_SHAPE_SGMP, G2P_MOVE | G2P_ENERGY, P, L, s);
+    });
+  } else {
+    p2g(P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE);
+    timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
+    timed(e, KI_G2P, [&] {
+      mpmk::launch_sgmp_cells<T>(SGC_KIN, P, N, s);
+      mpmk::launch_g2p<T>(MPM_SHAPE_SGMP, 0, P, L, s);
+    });
+    if (G.scheme == MPM_MUSL) {
+      timed(e, KI_P2G, [&] {
+        mpmk::launch_zero_nodes<T>(aux, 3, nn, N, s);
+        mpmk::launch_sgmp_p2g<T>(P2G_MOM, P, L, s);
+        mpmk::launch_sgmp_nodes<T>(SGN_MOM | SGN_TO_P2, P, N, s);
+      });
+      timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 1, 1, 1, s); });
+    }
+    timed(e, KI_STRESS, [&] {
+      mpmk::launch_sgmp_cells<T>(SGC_GRAD, P, N, s);
+      mpmk::launch_stress<T>(MPM_SHAPE_SGMP, ST_MOVE | ST_ENERGY, P, L, s);
+    });
+  }
+}
+
 // one fused step of every active instance (MPM3D.f90:69-113)
 template <typename T>
 static void enqueue_step(mpm_engine* e, int fin_flags) {
@@ -765,7 +863,9 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
       mpmk::launch_zero_nodes<T>((T*)e->fg, 3, e->nn, N, s);
       if (G.scheme == MPM_MUSL) mpmk::launch_zero_nodes<T>((T*)e->p2, 3, e->nn, N, s);
     });
-    if (G.scheme == MPM_USF) {
+    if (G.shape == MPM_SHAPE_SGMP) {
+      enqueue_sgmp<T>(e, G, P, L, N);
+    } else if (G.scheme == MPM_USF) {
       timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, L, s); });
       timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 0, 1, 1, s); });
       timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_FORCE, P, L, s); });
@@ -1680,7 +1780,8 @@ extern "C" int mpm_engine_capacity(mpm_engine* e, long* free_particles, long* fr
 
 extern "C" long mpm_engine_device_bytes(mpm_engine* e) {
   if (!e) return -1;
-  return (long)(e->np * (2 * bytes_per_particle(e->prec) + 16) + e->nn * bytes_per_node(e->prec));
+  return (long)(e->np * (2 * bytes_per_particle(e->prec) + 16) + e->nn * bytes_per_node(e->prec) +
+                (e->aux ? e->nn * (AUX_COUNT * e->tsz + 8) : 0));
 }
 
 extern "C" int mpm_ctx_engine(mpm_ctx* ctx, mpm_engine** out) {

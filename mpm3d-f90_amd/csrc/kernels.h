@@ -22,6 +22,19 @@
 #define ST_FORCE 2    // USF: force scatter with the updated stress
 #define ST_ENERGY 4   // energy partials
 
+// SGMP: spreads from the auxiliary nodes to the grid nodes
+#define SGN_MASSMOM 1   // Mg, PXg from Cmg, Cpxg (CutOff test)
+#define SGN_FORCE 2     // FXg from Cfext, Cfint
+#define SGN_MOM 4       // PXg from Cpxg (MUSL, no CutOff test)
+#define SGN_TO_P2 8     // SGN_MOM into the re-map buffer
+// SGMP: cell-centre values from the grid nodes
+#define SGC_GRAD 1      // strain rate, spin, velocity gradient
+#define SGC_KIN 2       // velocity, acceleration
+// SGMP auxiliary-node fields per cell (the passes alias the storage); the
+// force pass keeps Cfext, Cfint in slots 3..11, behind the momentum slots
+#define AUX_COUNT 18
+#define AUX_FORCE_OFF 3
+
 // k_finalize parts
 #define FIN_ENERGY 1
 #define FIN_DT 2
@@ -51,6 +64,9 @@ void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int bpg,
 void launch_begin_step(InstT* ts, const int* ilist, int n, hipStream_t s);
 template <typename S>
 void launch_permute(S* dst, const S* src, const int* perm, const LaunchP& L, long np, int nfield, hipStream_t s);
+template <typename T> void launch_sgmp_p2g(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
+template <typename T> void launch_sgmp_nodes(int what, const PoolPtrs<T>& P, const LaunchP& N, hipStream_t s);
+template <typename T> void launch_sgmp_cells(int what, const PoolPtrs<T>& P, const LaunchP& N, hipStream_t s);
 void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP& L, int brick, unsigned* keys,
                       int* vals, hipStream_t s);
 }  // namespace mpmk

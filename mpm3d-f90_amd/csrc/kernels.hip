@@ -1281,6 +1281,53 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
   p.t(T_Q) = m.bqf;
 }
 
+// ParticleStressUpdate after the gather (update_step.f90:566-642): strain
+// and spin increments, F update (UVFp), Constitution, and Xp = XX for
+// USL/MUSL.  L is the velocity gradient sum (deFp before the DT scaling).
+template <typename T>
+__device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p, const InstC& I, const InstT& ts,
+                                            T de[6], T vort[3], const T L[3][3], bool uvfp, double x, double y,
+                                            double z, bool move) {
+    const double dtd = ts.dt;
+    const T dt = (T)dtd;
+#pragma unroll
+    for (int k = 0; k < 6; k++) de[k] = de[k] * dt;
+#pragma unroll
+    for (int k = 0; k < 3; k++) vort[k] = vort[k] * dt / (T)2;
+    double F[3][3], deF[3][3];
+    if (uvfp) {
+      double F0[3][3], dF[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          dF[a][b] = (double)(L[a][b] * dt);
+          F0[a][b] = p.d(D_F0 + 3 * a + b);
+        }
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          const double f = F0[a][b] + dF[a][b] + dF[a][0] * F0[0][b] + dF[a][1] * F0[1][b] + dF[a][2] * F0[2][b];
+          p.d(D_F0 + 3 * a + b) = f;
+          F[a][b] = f;
+          deF[a][b] = dF[a][b];
+        }
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        deF[a][a] = deF[a][a] + 1.0;
+        F[a][a] = F[a][a] + 1.0;
+      }
+    }
+    const MatC& mc = P.mats[P.bodies[p.n(I_BODY)].mat];
+    constitution<T>(p, I, mc, dt, de, vort, F, deF);
+    if (move) {   // Xp = XX (update_step.f90:642)
+      p.d(D_X) = x + (double)p.t(T_BX) * dtd;
+      p.d(D_Y) = y + (double)p.t(T_BY) * dtd;
+      p.d(D_Z) = z + (double)p.t(T_BZ) * dtd;
+    }
+}
+
 // ParticleStressUpdate (update_step.f90:456-647): strain-rate gather, F
 // update, constitution; optionally Xp = XX (USL/MUSL), the USF force
 // scatter, and the energy partials.
@@ -1359,44 +1406,7 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
       }
     });
     }
-    const double dtd = ts.dt;
-    const T dt = (T)dtd;
-#pragma unroll
-    for (int k = 0; k < 6; k++) de[k] = de[k] * dt;
-#pragma unroll
-    for (int k = 0; k < 3; k++) vort[k] = vort[k] * dt / (T)2;
-    double F[3][3], deF[3][3];
-    if (uvfp) {
-      double F0[3][3], dF[3][3];
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-          dF[a][b] = (double)(L[a][b] * dt);
-          F0[a][b] = p.d(D_F0 + 3 * a + b);
-        }
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) {
-          const double f = F0[a][b] + dF[a][b] + dF[a][0] * F0[0][b] + dF[a][1] * F0[1][b] + dF[a][2] * F0[2][b];
-          p.d(D_F0 + 3 * a + b) = f;
-          F[a][b] = f;
-          deF[a][b] = dF[a][b];
-        }
-#pragma unroll
-      for (int a = 0; a < 3; a++) {
-        deF[a][a] = deF[a][a] + 1.0;
-        F[a][a] = F[a][a] + 1.0;
-      }
-    }
-    const MatC& mc = P.mats[P.bodies[p.n(I_BODY)].mat];
-    constitution<T>(p, I, mc, dt, de, vort, F, deF);
-    if (MODE & ST_MOVE) {   // Xp = XX (update_step.f90:642)
-      p.d(D_X) = x + (double)p.t(T_BX) * dtd;
-      p.d(D_Y) = y + (double)p.t(T_BY) * dtd;
-      p.d(D_Z) = z + (double)p.t(T_BZ) * dtd;
-    }
+    stress_tail<T>(P, p, I, ts, de, vort, L, uvfp, x, y, z, (MODE & ST_MOVE) != 0);
   }
   if (MODE & ST_FORCE) {   // USF: GridMomentumUpdate scatter right after the update
     T* Fg = P.fg + I.node_off;
@@ -1439,6 +1449,422 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
     }
   }
   if (MODE & ST_ENERGY) {
+    if (live) particle_energy(p, ev);
+    block_energy(P.part, gran, blockIdx.x % bpg, ev);
+  }
+}
+
+// ======================================================== SGMP (a6)
+//
+// The auxiliary grid has one node per cell centre (nb_centernode = NumCell,
+// Grid.f90:264-268), stored per instance at the cell's index in the node
+// range.  Particles map to the 8 centres around their nearest grid node with
+// trilinear weights SGNShape(XXN) (Grid.f90:434-458).  Particle passes
+// scatter to / gather from the centres; the spreads from centres to grid
+// nodes (every node sums its adjacent cells in ascending cell order, which is
+// the reference's summation order) and the node-to-centre averages are
+// separate node- and cell-parallel kernels.
+
+// auxiliary-node field slots (the three passes alias the same storage)
+enum { AX_P = 0, AX_FE = 3, AX_FI = 6 };   // momentum, external and internal force
+enum { AX_CO = 0, AX_CW = 6, AX_DF = 9 };  // strain rate, spin, velocity gradient
+enum { AX_V = 0, AX_A = 3 };               // velocity, acceleration
+
+__device__ constexpr int SGN[3][8] = {{-1, 1, 1, -1, -1, 1, 1, -1}, {-1, -1, 1, 1, -1, -1, 1, 1},
+                                      {-1, -1, -1, -1, 1, 1, 1, 1}};
+
+struct SgLoc {
+  int cn1;        // first auxiliary node (1-based cell index) of the particle's auxiliary cell
+  int bd;         // bits: nearest node is a boundary node (bd_type /= 0) along x, y, z
+  double xxn[3];  // natural coordinates in the auxiliary cell (snapped on boundary axes)
+};
+
+// ParticleInitial, SGMP part (update_step.f90:432-449), with
+// CenterInWhichCell (Grid.f90:157-190) and CenterCellNode (Grid.f90:386-403)
+__device__ __forceinline__ bool sgmp_locate(const InstC& I, const uint8_t* fix, double x, double y, double z,
+                                            SgLoc& L) {
+  const double h = I.dcell;
+  const double X[3] = {x, y, z};
+  const int ix = (int)((x - I.span0[0] - h / 2.0) / h) + 1;
+  const int iy = (int)((y - I.span0[1] - h / 2.0) / h) + 1;
+  const int iz = (int)((z - I.span0[2] - h / 2.0) / h) + 1;
+  const int ac = (iz - 1) * I.cncxy + (iy - 1) * I.cncx + ix;
+  if (ac > I.cncell || ac <= 0) return false;
+  const int a0 = ac - 1, az = a0 / I.cncxy, r = a0 - az * I.cncxy, ay = r / I.cncx, ax = r - ay * I.cncx + 1;
+  L.cn1 = az * I.ncxy + ay * I.ncx + ax;
+  const int jx = (int)((x - I.span0[0]) / h + 0.5) + 1;
+  const int jy = (int)((y - I.span0[1]) / h + 0.5) + 1;
+  const int jz = (int)((z - I.span0[2]) / h + 0.5) + 1;
+  const int nd = (jz - 1) * I.ngxy + (jy - 1) * I.ngx + jx;
+  L.bd = 0;
+  if (nd < 1 || nd > I.nnode) {   // outside the grid (undefined in the reference)
+    L.xxn[0] = L.xxn[1] = L.xxn[2] = 0.0;
+    return true;
+  }
+  const int n0 = nd - 1, nz = n0 / I.ngxy, rr = n0 - nz * I.ngxy, ny = rr / I.ngx, nx = rr - ny * I.ngx;
+  const double xg[3] = {(double)nx * h + I.span0[0], (double)ny * h + I.span0[1], (double)nz * h + I.span0[2]};
+  const int fb = fix[I.node_off + n0];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double v = (X[d] - xg[d]) * I.ijac;
+    if ((fb >> (3 + d)) & 1) {
+      if (v > 0) v = 1;
+      else if (v < 0) v = -1;
+      L.bd |= 1 << d;
+    }
+    L.xxn[d] = v;
+  }
+  return true;
+}
+
+// 1-based cell index of auxiliary node n of the cell (0 when outside)
+__device__ __forceinline__ int sgmp_node(const InstC& I, const SgLoc& L, int n) {
+  const int off = (n & 3) == 1 ? 1 : (n & 3) == 2 ? 1 + I.ncx : (n & 3) == 3 ? I.ncx : 0;
+  const int k = L.cn1 + off + (n >= 4 ? I.ncxy : 0);
+  return (k <= I.ncell && k > 0) ? k : 0;
+}
+
+// SGNShape (Grid.f90:434-458)
+template <typename T>
+__device__ __forceinline__ void sgn_shape(const double xi[3], T shp[8]) {
+  const T a = (T)xi[0], b = (T)xi[1], c = (T)xi[2];
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    const T sx = (T)SGN[0][n] * a + (T)1, sy = (T)SGN[1][n] * b + (T)1, sz = (T)SGN[2][n] * c + (T)1;
+    shp[n] = sx * sy * sz * (T)0.125;
+  }
+}
+
+// Particle scatters to the auxiliary nodes: mass and momentum
+// (GridMomentumInitial, update_step.f90:318-345), forces (GridMomentumUpdate,
+// :906-930), momentum re-map (GridMomentumMUSL, :1689-1707).
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
+  int gran;
+  const long i = part_index(0, glist, bpg, gran);
+  const int ii = ginst[gran];
+  const InstC& I = P.inst[ii];
+  if (P.ts[ii].done) return;
+  PV<T> p{P, i};
+  const bool live = i < I.p_end;
+  if (MODE & P2G_ICELL) p.n(I_CELL) = live ? in_which_cell(I, p.d(D_X), p.d(D_Y), p.d(D_Z)) : -1;
+  if (!live) return;
+  SgLoc L;
+  if (!sgmp_locate(I, P.fix, p.d(D_X), p.d(D_Y), p.d(D_Z), L)) return;
+  T shp[8];
+  sgn_shape<T>(L.xxn, shp);
+  const long nn = P.nn;
+  T* aux = P.aux + I.node_off;
+  double* cmg = P.cmg + I.node_off;
+  T m = (T)0, v[3] = {(T)0, (T)0, (T)0};
+  if (MODE & (P2G_MASS | P2G_MOM)) {
+    m = p.t(T_MASS);
+    v[0] = p.t(T_VX); v[1] = p.t(T_VY); v[2] = p.t(T_VZ);
+  }
+  T fx[3] = {(T)0, (T)0, (T)0}, sg[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, vol = (T)0;
+  if (MODE & P2G_FORCE) {
+    external_force(P, p, I, P.ts[ii].time, fx);
+    const T smq = p.t(T_SM) - p.t(T_Q);
+    sg[0] = smq + p.t(T_SD0); sg[1] = smq + p.t(T_SD1); sg[2] = smq + p.t(T_SD2);
+    sg[3] = p.t(T_SD5); sg[4] = p.t(T_SD3); sg[5] = p.t(T_SD4);   // Cfint(4..6): sxy, syz, sxz
+    vol = (T)p.d(D_VOL);
+  }
+#pragma unroll
+  for (int n = 0; n < 8; n++) {
+    const int k = sgmp_node(I, L, n);
+    if (!k) continue;
+    const long c = k - 1;
+    const T N = shp[n];
+    if (MODE & (P2G_MASS | P2G_MOM)) {
+      const T shm = N * m;
+      if (MODE & P2G_MASS) atomic_add(&cmg[c], (double)shm);
+#pragma unroll
+      for (int d = 0; d < 3; d++) atomic_add(&aux[(AX_P + d) * nn + c], v[d] * shm);
+    }
+    if (MODE & P2G_FORCE) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) atomic_add(&aux[(AX_FE + d) * nn + c], fx[d] * N);
+#pragma unroll
+      for (int q = 0; q < 6; q++) atomic_add(&aux[(AX_FI + q) * nn + c], N * sg[q] * vol);
+    }
+  }
+}
+
+// spreads from the auxiliary nodes to the grid nodes: SGN_MASSMOM
+// (update_step.f90:394-405), SGN_FORCE (:990-1011), SGN_MOM (:1760-1768)
+
+// index of node (a,b,c) of a cell in the CellsNode order (Grid.f90:372-379)
+__device__ __forceinline__ int cell_local(int a, int b, int c) { return 4 * c + (b ? (a ? 2 : 3) : (a ? 1 : 0)); }
+
+template <typename T, int WHAT>
+__global__ void __launch_bounds__(256) k_sgmp_nodes(PoolPtrs<T> P, const int* nglist, const int* nginst, int bpg) {
+  int gran;
+  const long j = node_index(nglist, bpg, gran);
+  const int ii = nginst[gran];
+  const InstC& I = P.inst[ii];
+  if (P.ts[ii].done) return;
+  if (j >= (long)I.node_off + I.nnode) return;
+  const int k = (int)(j - I.node_off);
+  const int nz = k / I.ngxy, r = k - nz * I.ngxy, ny = r / I.ngx, nx = r - ny * I.ngx;
+  const long nn = P.nn;
+  const T* aux = P.aux + I.node_off;
+  const double* cmg = P.cmg + I.node_off;
+  const T ij4 = (T)I.ijac4;
+  double mg = 0.0;
+  T pv[3] = {(T)0, (T)0, (T)0}, fv[3] = {(T)0, (T)0, (T)0};
+  for (int cz = nz - 1; cz <= nz; cz++) {
+    if (cz < 0 || cz >= I.ncz) continue;
+    for (int cy = ny - 1; cy <= ny; cy++) {
+      if (cy < 0 || cy >= I.ncy) continue;
+      for (int cx = nx - 1; cx <= nx; cx++) {
+        if (cx < 0 || cx >= I.ncx) continue;
+        const long c = (long)cz * I.ncxy + cy * I.ncx + cx;
+        if (WHAT & SGN_MASSMOM) {
+          if (cmg[c] > I.cutoff) {
+            mg = mg + 0.125 * cmg[c];
+#pragma unroll
+            for (int d = 0; d < 3; d++) pv[d] = pv[d] + (T)0.125 * aux[(AX_P + d) * nn + c];
+          }
+        }
+        if (WHAT & SGN_MOM) {
+#pragma unroll
+          for (int d = 0; d < 3; d++) pv[d] = pv[d] + (T)0.125 * aux[(AX_P + d) * nn + c];
+        }
+        if (WHAT & SGN_FORCE) {
+          const int jl = cell_local(nx - cx, ny - cy, nz - cz);
+          const T DX = (T)SGN[0][jl] * ij4, DY = (T)SGN[1][jl] * ij4, DZ = (T)SGN[2][jl] * ij4;
+          T fi[6];
+#pragma unroll
+          for (int q = 0; q < 6; q++) fi[q] = aux[(AX_FI + q) * nn + c];
+          const T f0 = -(fi[0] * DX + fi[3] * DY + fi[5] * DZ);
+          const T f1 = -(fi[3] * DX + fi[1] * DY + fi[4] * DZ);
+          const T f2 = -(fi[5] * DX + fi[4] * DY + fi[2] * DZ);
+          fv[0] = fv[0] + (T)0.125 * aux[(AX_FE + 0) * nn + c] + f0;
+          fv[1] = fv[1] + (T)0.125 * aux[(AX_FE + 1) * nn + c] + f1;
+          fv[2] = fv[2] + (T)0.125 * aux[(AX_FE + 2) * nn + c] + f2;
+        }
+      }
+    }
+  }
+  if (WHAT & SGN_MASSMOM) P.mg[j] = mg;
+  if (WHAT & (SGN_MASSMOM | SGN_MOM)) {
+    T* dst = (WHAT & SGN_TO_P2) ? P.p2 : P.pg;
+#pragma unroll
+    for (int d = 0; d < 3; d++) dst[d * nn + j] = pv[d];
+  }
+  if (WHAT & SGN_FORCE)
+#pragma unroll
+    for (int d = 0; d < 3; d++) P.fg[d * nn + j] = fv[d];
+}
+
+// cell-centre values from the grid nodes: SGC_GRAD (update_step.f90:497-525),
+// SGC_KIN (:1405-1417 and BoundaryAuxiliaryNodeUpdate :1528-1666)
+
+template <typename T, int WHAT>
+__global__ void __launch_bounds__(256) k_sgmp_cells(PoolPtrs<T> P, const int* nglist, const int* nginst, int bpg) {
+  int gran;
+  const long j = node_index(nglist, bpg, gran);
+  const int ii = nginst[gran];
+  const InstC& I = P.inst[ii];
+  if (P.ts[ii].done) return;
+  const int c = (int)(j - I.node_off);
+  if (c >= I.ncell) return;
+  const int cz = c / I.ncxy, r = c - cz * I.ncxy, cy = r / I.ncx, cx = r - cy * I.ncx;
+  const long nn = P.nn;
+  const long n1 = I.node_off + (long)cz * I.ngxy + (long)cy * I.ngx + cx;
+  const long nodes[8] = {n1, n1 + 1, n1 + 1 + I.ngx, n1 + I.ngx,
+                         n1 + I.ngxy, n1 + I.ngxy + 1, n1 + I.ngxy + 1 + I.ngx, n1 + I.ngxy + I.ngx};
+  T* aux = P.aux + I.node_off;
+  if (WHAT & SGC_GRAD) {
+    const T ij4 = (T)I.ijac4;
+    T co[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, cw[3] = {(T)0, (T)0, (T)0};
+    T df[9] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
+#pragma unroll
+    for (int jl = 0; jl < 8; jl++) {
+      const long k = nodes[jl];
+      if (!(P.mg[k] > I.cutoff)) continue;
+      const T v[3] = {P.vg[k], P.vg[nn + k], P.vg[2 * nn + k]};
+      const T DX = (T)SGN[0][jl] * ij4, DY = (T)SGN[1][jl] * ij4, DZ = (T)SGN[2][jl] * ij4;
+      co[0] = co[0] + DX * v[0];
+      co[1] = co[1] + DY * v[1];
+      co[2] = co[2] + DZ * v[2];
+      co[3] = co[3] + DZ * v[1] + DY * v[2];
+      co[4] = co[4] + DZ * v[0] + DX * v[2];
+      co[5] = co[5] + DY * v[0] + DX * v[1];
+      cw[0] = cw[0] + DY * v[2] - DZ * v[1];
+      cw[1] = cw[1] + DZ * v[0] - DX * v[2];
+      cw[2] = cw[2] + DX * v[1] - DY * v[0];
+#pragma unroll
+      for (int n = 0; n < 3; n++) {
+        df[3 * n + 0] = df[3 * n + 0] + DX * v[n];
+        df[3 * n + 1] = df[3 * n + 1] + DY * v[n];
+        df[3 * n + 2] = df[3 * n + 2] + DZ * v[n];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) aux[(AX_CO + q) * nn + c] = co[q];
+#pragma unroll
+    for (int q = 0; q < 3; q++) aux[(AX_CW + q) * nn + c] = cw[q];
+#pragma unroll
+    for (int q = 0; q < 9; q++) aux[(AX_DF + q) * nn + c] = df[q];
+  }
+  if (WHAT & SGC_KIN) {
+    // a centre on a fixed or symmetric face takes the nodal values at its
+    // face point; elsewhere the mean of its 8 nodes.  0.125 and the face
+    // weights are powers of two (or 0), so (w PXg)/Mg = w (PXg/Mg) exactly.
+    const int id[3] = {cx, cy, cz}, nc[3] = {I.ncx, I.ncy, I.ncz};
+    bool face = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+      if (nc[a] > 1 && ((id[a] == 0 && I.fixs[2 * a] != 0) || (id[a] == nc[a] - 1 && I.fixs[2 * a + 1] != 0)))
+        face = true;
+    T w[8];
+    if (face) {
+      double cxx[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        if (nc[a] <= 1) continue;
+        if (id[a] == 0) cxx[a] = 1.0;
+        if (id[a] == nc[a] - 1) cxx[a] = -1.0;
+      }
+      sgn_shape<T>(cxx, w);
+    } else {
+#pragma unroll
+      for (int jl = 0; jl < 8; jl++) w[jl] = (T)0.125;
+    }
+    T cv[3] = {(T)0, (T)0, (T)0}, ca[3] = {(T)0, (T)0, (T)0};
+#pragma unroll
+    for (int jl = 0; jl < 8; jl++) {
+      const long k = nodes[jl];
+      if (!(P.mg[k] > I.cutoff)) continue;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        cv[d] = cv[d] + w[jl] * P.vg[d * nn + k];
+        ca[d] = ca[d] + w[jl] * P.ag[d * nn + k];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      aux[(AX_V + d) * nn + c] = cv[d];
+      aux[(AX_A + d) * nn + c] = ca[d];
+    }
+  }
+}
+
+// ParticleStressUpdate, SGMP: de, vort, deFp interpolated from the centres
+// (update_step.f90:542-562), then the common update
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_sgmp_stress(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
+  int gran;
+  const long i = part_index(0, glist, bpg, gran);
+  const int ii = ginst[gran];
+  const InstC& I = P.inst[ii];
+  const InstT& ts = P.ts[ii];
+  if (ts.done) return;
+  PV<T> p{P, i};
+  const bool live = i < I.p_end;
+  SgLoc L;
+  const double x = live ? p.d(D_X) : 0.0, y = live ? p.d(D_Y) : 0.0, z = live ? p.d(D_Z) : 0.0;
+  const bool act = live && sgmp_locate(I, P.fix, x, y, z, L);
+  if (act) {
+    T shp[8];
+    sgn_shape<T>(L.xxn, shp);
+    const long nn = P.nn;
+    const T* aux = P.aux + I.node_off;
+    T de[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, vort[3] = {(T)0, (T)0, (T)0};
+    T Lg[3][3] = {{(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}};
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      const int k = sgmp_node(I, L, n);
+      if (!k) continue;
+      const long c = k - 1;
+#pragma unroll
+      for (int q = 0; q < 6; q++) de[q] = de[q] + aux[(AX_CO + q) * nn + c] * shp[n];
+#pragma unroll
+      for (int q = 0; q < 3; q++) vort[q] = vort[q] + aux[(AX_CW + q) * nn + c] * shp[n];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) Lg[a][b] = Lg[a][b] + shp[n] * aux[(AX_DF + 3 * a + b) * nn + c];
+    }
+    stress_tail<T>(P, p, I, ts, de, vort, Lg, (I.flags & IF_UVFP) != 0, x, y, z, (MODE & ST_MOVE) != 0);
+  }
+  if (MODE & ST_ENERGY) {
+    double ev[6] = {0, 0, 0, 0, 0, 0};
+    if (live) particle_energy(p, ev);
+    block_energy(P.part, gran, blockIdx.x % bpg, ev);
+  }
+}
+
+// ParticlePositionUpdate, SGMP (update_step.f90:1434-1525): particles whose
+// nearest node is a boundary node interpolate at AuxiliaryGridSHP's
+// coordinates (Grid.f90:193-220, its integer p_type read as /= 0)
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_sgmp_g2p(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
+  int gran;
+  const long i = part_index(0, glist, bpg, gran);
+  const int ii = ginst[gran];
+  const InstC& I = P.inst[ii];
+  const InstT& ts = P.ts[ii];
+  if (ts.done) return;
+  PV<T> p{P, i};
+  const bool live = i < I.p_end;
+  SgLoc L;
+  const double x = live ? p.d(D_X) : 0.0, y = live ? p.d(D_Y) : 0.0, z = live ? p.d(D_Z) : 0.0;
+  const bool act = live && sgmp_locate(I, P.fix, x, y, z, L);
+  if (act) {
+    double xi[3] = {L.xxn[0], L.xxn[1], L.xxn[2]};
+    if (L.bd) {
+      // XN = (Xp - Cxg(icell)) * iJacobi (update_step.f90:431); 0 outside the grid
+      const int c = p.n(I_CELL);
+      double xn[3] = {0.0, 0.0, 0.0};
+      if (c > 0) {
+        int cx, cy, cz;
+        cell_coords(I, c, cx, cy, cz);
+        const int cc[3] = {cx, cy, cz};
+        const double X[3] = {x, y, z};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          const double cxg = ((double)cc[d] * I.dcell + I.span0[d]) + I.dcell / 2.0;
+          xn[d] = (X[d] - cxg) * I.ijac;
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        if ((L.bd >> d) & 1) xi[d] = xn[d] < 0 ? 1 + 2 * xn[d] : 2 * xn[d] - 1;
+    }
+    T shp[8];
+    sgn_shape<T>(xi, shp);
+    const long nn = P.nn;
+    const T* aux = P.aux + I.node_off;
+    T vx[3] = {(T)0, (T)0, (T)0}, ax[3] = {(T)0, (T)0, (T)0};
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+      const int k = sgmp_node(I, L, n);
+      if (!k) continue;
+      const long c = k - 1;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        vx[d] = vx[d] + shp[n] * aux[(AX_V + d) * nn + c];
+        ax[d] = ax[d] + shp[n] * aux[(AX_A + d) * nn + c];
+      }
+    }
+    const double dt = ts.dt;
+    const T tdt = (T)dt;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const T v = p.t(T_VX + d);
+      p.t(T_VX + d) = (ts.istep == 1) ? v + ax[d] * tdt * (T)0.5 : v + ax[d] * tdt;
+      p.t(T_BX + d) = vx[d];
+    }
+    if (MODE & G2P_MOVE) {
+      p.d(D_X) = x + (double)vx[0] * dt;
+      p.d(D_Y) = y + (double)vx[1] * dt;
+      p.d(D_Z) = z + (double)vx[2] * dt;
+    }
+  }
+  if (MODE & G2P_ENERGY) {
+    double ev[6] = {0, 0, 0, 0, 0, 0};
     if (live) particle_energy(p, ev);
     block_energy(P.part, gran, blockIdx.x % bpg, ev);
   }
@@ -1673,6 +2099,14 @@ void launch_p2g(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hip
 template <typename T>
 void launch_g2p(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
   if (!L.nblk) return;
+  if (shape == MPM_SHAPE_SGMP) {
+    dim3 g(L.nblk), b(L.bs);
+    if (mode == G2P_MOVE) hipLaunchKernelGGL((k_sgmp_g2p<T, G2P_MOVE>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
+    else if (mode == (G2P_MOVE | G2P_ENERGY))
+      hipLaunchKernelGGL((k_sgmp_g2p<T, G2P_MOVE | G2P_ENERGY>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
+    else hipLaunchKernelGGL((k_sgmp_g2p<T, 0>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
+    return;
+  }
   switch (shape) {
     case MPM_SHAPE_LINEAR: launch_g2p_s<MPM_SHAPE_LINEAR, T>(mode, P, L, s); break;
     case MPM_SHAPE_GIMP: launch_g2p_s<MPM_SHAPE_GIMP, T>(mode, P, L, s); break;
@@ -1683,11 +2117,60 @@ void launch_g2p(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hip
 template <typename T>
 void launch_stress(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
   if (!L.nblk) return;
+  if (shape == MPM_SHAPE_SGMP) {
+    dim3 g(L.nblk), b(L.bs);
+    if (mode == ST_MOVE) hipLaunchKernelGGL((k_sgmp_stress<T, ST_MOVE>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
+    else if (mode == (ST_MOVE | ST_ENERGY))
+      hipLaunchKernelGGL((k_sgmp_stress<T, ST_MOVE | ST_ENERGY>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
+    else hipLaunchKernelGGL((k_sgmp_stress<T, 0>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
+    return;
+  }
   switch (shape) {
     case MPM_SHAPE_LINEAR: launch_stress_s<MPM_SHAPE_LINEAR, T>(mode, P, L, s); break;
     case MPM_SHAPE_GIMP: launch_stress_s<MPM_SHAPE_GIMP, T>(mode, P, L, s); break;
     default: launch_stress_s<MPM_SHAPE_BSPLINE, T>(mode, P, L, s); break;
   }
+}
+
+template <typename T>
+void launch_sgmp_p2g(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  if (!L.nblk) return;
+  dim3 g(L.nblk), b(L.bs);
+#define SG_CASE(M) \
+  case M: hipLaunchKernelGGL((k_sgmp_p2g<T, M>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+  switch (mode) {
+    SG_CASE(P2G_ICELL | P2G_MASS | P2G_MOM)
+    SG_CASE(P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE)
+    SG_CASE(P2G_FORCE)
+    SG_CASE(P2G_MOM)
+    default: break;
+  }
+#undef SG_CASE
+}
+
+template <typename T>
+void launch_sgmp_nodes(int what, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  if (!L.nblk) return;
+  dim3 g(L.nblk), b(L.bs);
+#define SN_CASE(M) \
+  case M: hipLaunchKernelGGL((k_sgmp_nodes<T, M>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+  switch (what) {
+    SN_CASE(SGN_MASSMOM)
+    SN_CASE(SGN_MASSMOM | SGN_FORCE)
+    SN_CASE(SGN_FORCE)
+    SN_CASE(SGN_MOM)
+    SN_CASE(SGN_MOM | SGN_TO_P2)
+    default: break;
+  }
+#undef SN_CASE
+}
+
+template <typename T>
+void launch_sgmp_cells(int what, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  if (!L.nblk) return;
+  dim3 g(L.nblk), b(L.bs);
+  if (what == SGC_GRAD) hipLaunchKernelGGL((k_sgmp_cells<T, SGC_GRAD>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
+  else hipLaunchKernelGGL((k_sgmp_cells<T, SGC_KIN>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
 }
 
 template <typename T>
@@ -1749,7 +2232,10 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
   template void launch_zero_nodes<T>(T*, int, long, const LaunchP&, hipStream_t);                  \
   template void launch_energy<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                 \
   template void launch_finalize<T>(const PoolPtrs<T>&, const int*, int, int, int, hipStream_t);    \
-  template void launch_permute<T>(T*, const T*, const int*, const LaunchP&, long, int, hipStream_t);
+  template void launch_permute<T>(T*, const T*, const int*, const LaunchP&, long, int, hipStream_t);  \
+  template void launch_sgmp_p2g<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);         \
+  template void launch_sgmp_nodes<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);       \
+  template void launch_sgmp_cells<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);
 #if MPMK_PART != 2
 INST(double)
 template void launch_permute<int>(int*, const int*, const int*, const LaunchP&, long, int, hipStream_t);

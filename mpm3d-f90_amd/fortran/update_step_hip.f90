@@ -106,6 +106,13 @@ contains
        ! Those undefined values are replaced by 0 here.
        mats(m)%det_speed = 0d0
        if (mat_list(m)%MatType /= 9) mats(m)%wavespd = 0d0
+       ! the null material reads no Young's modulus or Poisson ratio
+       ! (DataIn.f90:486-491), yet SetDT/UpdateDT evaluate the elastic wave
+       ! speed from them: 0 here, so its wave speed is Wavespd
+       if (mat_list(m)%MatType == 9) then
+          mats(m)%young = 0d0
+          mats(m)%poisson = 0d0
+       end if
     end do
     ! initial DT of SetDT (DataIn.f90:1278-1295) from the defined values
     dt0 = 1.0e6

@@ -33,7 +33,8 @@ def read_dump(path, n_expected):
     return int(istep), sc, ps
 
 
-@pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "plate_neoh_musl_qulo"])
+@pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "plate_neoh_musl_qulo",
+                                  "plate_neoh_sgmp_musl"])
 def test_fortran_host_matches_oracle(oracle_mod, name):
     if not os.path.exists(HOST):
         pytest.skip("Fortran host not built")
@@ -57,7 +58,8 @@ def test_fortran_host_matches_oracle(oracle_mod, name):
         shutil.rmtree(tmp)
 
 
-@pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "slope_dp_bspline_usf"])
+@pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "dam_null_sgmp_musl",
+                                  "block_j2_sgmp_usf"])
 def test_reference_driver_runs_the_device_step(oracle_mod, name):
     """The reference's MPM3D.f90 loop, unchanged, over update_step_hip."""
     if not os.path.exists(DROPIN):

@@ -17,12 +17,14 @@ import os
 import numpy as np
 import pytest
 
-from conftest import DECKS, GOLDEN_DIR, PGROUPS, GGROUPS, deck_path, group_errors, load_package
+from conftest import ALL_DECKS as DECKS, GOLDEN_DIR, PGROUPS, GGROUPS, deck_path, group_errors, load_package
 
 pytestmark = pytest.mark.gpu
 
 TOL64 = {name: 1e-12 for name in DECKS}
-TOL64_LOOSE = {"dam_null_eos4_musl": {"stress": 1e-9, "ie": 1e-9}}
+TOL64_LOOSE = {"dam_null_eos4_musl": {"stress": 1e-9, "ie": 1e-9},
+               "dam_null_sgmp_musl": {"stress": 1e-9, "ie": 1e-9}}
+WATER = ("dam_null_eos4_musl", "dam_null_sgmp_musl")
 
 
 def perr(mpm, a, b):
@@ -118,7 +120,7 @@ def test_each_phase_entry_point_matches_oracle(mpm, oracle_mod, name):
         seq += ["grid_momentum_musl", "apply_bc"]
     if not usf:
         seq += ["particle_stress_update"]
-    tol_loose = {"stress": 1e-6, "ie": 1e-6} if name == "dam_null_eos4_musl" else None
+    tol_loose = {"stress": 1e-6, "ie": 1e-6} if name in WATER else None
     for step in (1, 2):
         ctx.begin_step()
         o.phase("begin_step")
@@ -128,7 +130,7 @@ def test_each_phase_entry_point_matches_oracle(mpm, oracle_mod, name):
             where = "%s step %d after %s" % (name, step, ph)
             assert_close(gerr(mpm, ctx.grid(), o.grid()), 1e-12, None, where)
             pe = perr(mpm, ctx.particles(), o.particles())
-            if step == 1 or name == "dam_null_eos4_musl":
+            if step == 1 or name in WATER:
                 # stresses of the first steps are round-off of a zero state (the
                 # water column's pressure builds up over several steps); the
                 # fused-step test compares them from step 2 on
@@ -147,15 +149,18 @@ def test_each_phase_entry_point_matches_oracle(mpm, oracle_mod, name):
 
 @pytest.mark.parametrize("name", DECKS)
 def test_fp32_bulk_within_1e5_of_fp64_oracle(mpm, oracle_mod, name):
-    """FP32 bulk after 10 steps.  The water column is compared after 200
-    steps: its pressure c^2 (rho - rho0) starts from zero and during the first
-    tens of steps is the small difference of FP32 strain increments of an
-    almost rigidly falling column (relative error 1.5e-2 at step 10, 9e-5 at
-    step 50, 1.7e-6 at step 200 on MI355X)."""
+    """FP32 bulk after 10 steps.  The water columns are compared later: their
+    pressure c^2 (rho - rho0) starts from zero and during the first tens of
+    steps is the small difference of FP32 strain increments of an almost
+    rigidly falling column, so its relative error is large while the pressure
+    is small (GIMP column: 1.5e-2 at step 10, 9e-5 at step 50, 1.1e-6 at step
+    200; SGMP column, whose cell-centre averaging smooths the pressure build-up
+    further: internal energy 1.4e-4, 4e-4, 5.7e-5, 1.1e-5, 6.9e-6 at steps 10,
+    50, 200, 400, 800, on MI355X)."""
     prob = mpm.read_deck(deck_path(name))
     ctx = mpm.Context(prob, "fp32")
     o = oracle_mod.Oracle(prob)
-    nst = 200 if name == "dam_null_eos4_musl" else 10
+    nst = {"dam_null_eos4_musl": 200, "dam_null_sgmp_musl": 800}.get(name, 10)
     ctx.step(nst)
     o.step(nst)
     pe = perr(mpm, ctx.particles(), o.particles())
