@@ -275,6 +275,92 @@ def ensemble_parity(mpm, decks, E, a, log):
     return res
 
 
+def run_bar(mpm, decks, a, log):
+    """The Taylor bar (GIMP, J2, MUSL; the reference's CPU-runnable case) on
+    the plain path, FP64 and FP32, and the reference's CPU step on it."""
+    text, n = decks.taylor_bar()
+    prob = decks.load(text, mpm, "taylor")
+    out = {"workload": "Taylor bar R=3.2 mm L=32.4 mm, 190 m/s, copper J2 (400 MPa, H=100 MPa), GIMP, MUSL, "
+                       "h=0.4 mm, 2x2x2 particles/cell", "particles": prob.n}
+    for prec in ("fp64", "fp32"):
+        ctx = mpm.Context(prob, prec)
+        sec, kt = time_steps(ctx, a.steps, a.warmup, profile=False)
+        out[prec] = {"value": prob.n * a.steps / sec, "ms_per_step": sec / a.steps * 1e3}
+        ctx.close()
+    log("taylor bar: %s" % json.dumps(out))
+    if not a.no_cpu:
+        ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+        if os.path.exists(ref):
+            d = tempfile.mkdtemp(prefix="mpmbar_")
+            try:
+                with open(os.path.join(d, "bar.mpm"), "w") as f:
+                    f.write(text)
+                r = subprocess.run(["taskset", "-c", "0", ref, "bar.mpm", "20", os.path.join(d, "o"), "-1"], cwd=d,
+                                   capture_output=True, text=True, timeout=600)
+                m = re.search(r"STEPLOOP\s+(\S+)\s+(\d+)\s+(\d+)", r.stdout)
+                if r.returncode == 0 and m:
+                    out["cpu_reference"] = {"value": int(m.group(2)) * int(m.group(3)) / float(m.group(1)),
+                                            "cores": 1, "sample": "20 steps, step loop only"}
+            finally:
+                shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
+def run_sgmp(mpm, decks, a, log):
+    """SGMP workloads: 64 Cook-membrane plates (nearly incompressible
+    neo-Hookean) and one 3D dam break in one engine, FP32 bulk (FP64 F, det F,
+    volume), graph-replayed steps after autotuning; the Cook plate's FP32
+    result against the CPU FP64 oracle after a few steps."""
+    cook_text, n_cook = decks.cook_membrane()
+    dam_text, n_dam = decks.dam_break()
+    cook = decks.load(cook_text, mpm, "cook")
+    dam = decks.load(dam_text, mpm, "dam")
+    nc = a.sgmp_cooks
+    tot_p = nc * cook.n + dam.n
+    eng = mpm.Engine(max_particles=tot_p + (nc + 2) * 256, max_nodes=nc * cook.nb_node + dam.nb_node + (nc + 2) * 256,
+                     max_instances=nc + 1, max_materials=nc + 1, max_bodies=nc * len(cook.bodies) + 4,
+                     precision="fp32")
+    ids = [eng.add(cook) for _ in range(nc)] + [eng.add(dam)]
+    log("sgmp: %d Cook plates (%d particles each) + dam break (%d particles): %d particles"
+        % (nc, cook.n, dam.n, tot_p))
+    eng.step(2)
+    eng.sync()
+    tune = eng.autotune(trial_steps=3)
+    per = largest_divisor_upto(tune["sort_every"], 10) if tune["sort_every"] > 0 else 10
+    eng.step(per, graph=True, graph_steps=per)
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(a.sgmp_steps, graph=True, graph_steps=per)
+    eng.sync()
+    sec = time.perf_counter() - t0
+    eng.profile(True)
+    eng.step(3)
+    kt = eng.kernel_times()
+    eng.profile(False)
+    eng.close()
+    out = {"workload": "%d Cook-membrane plates 48x44(->16)x10 mm, h=0.5 mm, 2x2x2 particles/cell, neo-Hookean "
+                       "E=240 MPa nu=0.4999 rho=1e-3 g/mm^3, shear traction ramp, + 3D dam break 0.6x0.3x0.3 m, "
+                       "h=5 mm, c=50 m/s; SGMP, MUSL, FP32 bulk" % nc,
+           "particles": tot_p, "steps": a.sgmp_steps, "ms_per_step": sec / a.sgmp_steps * 1e3,
+           "value": tot_p * a.sgmp_steps / sec, "unit": "particle-steps/s", "tuning": tune,
+           "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()}}
+    log("sgmp: %.3f ms/step, %.4g particle-steps/s" % (out["ms_per_step"], out["value"]))
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        # 10 steps: in the first steps the ramped load and the stress are
+        # still near zero, and the relative error of a near-zero state is large
+        ctx = mpm.Context(cook, "fp32")
+        ctx.step(10)
+        o = pyoracle.Oracle(cook)
+        o.step(10)
+        out["parity_cook_fp32_vs_fp64_oracle"] = group_err(mpm, ctx.particles(), o.particles(), GROUPS)
+        out["parity_steps"] = 10
+        ctx.close()
+        log("sgmp parity:", json.dumps(out["parity_cook_fp32_vs_fp64_oracle"]))
+    return out
+
+
 def run_serving(mpm, decks, a, log):
     """Continuous-batching serving of a 70/30 mix of slope and Taylor-bar
     runs (100-300 steps each, FP32 bulk) under Poisson arrivals, one pool
@@ -319,6 +405,8 @@ def main():
     ap.add_argument("--check-steps", type=int, default=2, help="steps of the single slope compared with the oracle")
     ap.add_argument("--cpu-steps", type=int, default=30, help="steps of the CPU baseline sample per instance")
     ap.add_argument("--no-cpu", action="store_true", help="skip oracle parity and CPU baseline")
+    ap.add_argument("--sgmp-cooks", type=int, default=64, help="Cook plates of the SGMP batch (0: skip)")
+    ap.add_argument("--sgmp-steps", type=int, default=10, help="timed steps of the SGMP batch")
     ap.add_argument("--serve-rates", default="10,50,200", help="Poisson arrival rates (req/s) of the serving runs")
     ap.add_argument("--serve-window", type=float, default=8.0, help="seconds of arrivals per serving run (0: skip)")
     ap.add_argument("--serve-budget", type=float, default=200e9, help="HBM budget (bytes) of the serving pool")
@@ -358,6 +446,8 @@ def main():
         parity["max_fp32_err"] = max(max(r["fp32_vs_fp64_oracle"].values()) for r in parity["ensemble"])
     for eng in E["engs"].values():
         eng.close()
+    bar = run_bar(mpm, decks, a, log)
+    sgmp = run_sgmp(mpm, decks, a, log) if a.sgmp_cooks > 0 else None
     serving = run_serving(mpm, decks, a, log) if a.serve_window > 0 else None
     if not a.no_cpu:
         sample = [E["ens"][k] for k in range(3)]
@@ -400,6 +490,8 @@ def main():
         "roofline_plain_fp32": roofline(plain["fp32"]["kt"], prob.n, plain["fp32"]["touched"], 4,
                                         lambda k: measured_traffic(k, "fp32", "slope_pmc_traffic.md")),
         "parity": parity or None,
+        "taylor_bar": bar,
+        "sgmp": sgmp,
         "serving": serving,
         "cpu_baseline": cpu,
     }

@@ -89,9 +89,10 @@ enum TField {
 };
 
 // particle fields always kept in FP64: position, volume and internal energy
-// (accumulated every step; the weakly compressible EOS takes a small
-// difference of densities) and F (det F, ln J of the neo-Hookean model)
-enum DField { D_X = 0, D_Y, D_Z, D_VOL, D_IE, D_F0, D_F1, D_F2, D_F3, D_F4, D_F5, D_F6, D_F7, D_F8, D_COUNT };
+// (accumulated every step), F (det F, ln J of the neo-Hookean model) and the
+// mass as read (the weakly compressible EOS takes the small difference
+// m/V - rho0: an FP32 mass alone would offset it by 1e-5 of itself)
+enum DField { D_X = 0, D_Y, D_Z, D_VOL, D_IE, D_F0, D_F1, D_F2, D_F3, D_F4, D_F5, D_F6, D_F7, D_F8, D_MASS, D_COUNT };
 
 // particle integer fields
 enum IField { I_BODY = 0, I_ORIG, I_CELL, I_COUNT };

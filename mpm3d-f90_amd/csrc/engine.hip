@@ -398,6 +398,7 @@ static void upload_particles_T(mpm_engine* e, const HostInst& h, const double* p
     db[(size_t)D_Y * pad + i] = P(PF_Y, i);
     db[(size_t)D_Z * pad + i] = P(PF_Z, i);
     for (int k = 0; k < 9; k++) db[(size_t)(D_F0 + k) * pad + i] = P(PF_FP + k, i);
+    db[(size_t)D_MASS * pad + i] = P(PF_MASS, i);
   }
   if (!first) {
     // upload keeps the device permutation: scatter host rows to their slots
@@ -1327,7 +1328,7 @@ static void download_particles_T(mpm_engine* e, int id, double* out) {
     O(PF_Z) = db[(size_t)D_Z * pad + j];
     O(PF_VX) = Tv(T_VX); O(PF_VY) = Tv(T_VY); O(PF_VZ) = Tv(T_VZ);
     O(PF_FX) = Tv(T_FX); O(PF_FY) = Tv(T_FY); O(PF_FZ) = Tv(T_FZ);
-    O(PF_MASS) = Tv(T_MASS); O(PF_SM) = Tv(T_SM); O(PF_SEQV) = Tv(T_SEQV);
+    O(PF_MASS) = db[(size_t)D_MASS * pad + j]; O(PF_SM) = Tv(T_SM); O(PF_SEQV) = Tv(T_SEQV);
     O(PF_SDXX) = Tv(T_SD0); O(PF_SDYY) = Tv(T_SD1); O(PF_SDZZ) = Tv(T_SD2);
     O(PF_SDYZ) = Tv(T_SD3); O(PF_SDXZ) = Tv(T_SD4); O(PF_SDXY) = Tv(T_SD5);
     O(PF_SIGY) = Tv(T_SIGY); O(PF_EPEFF) = Tv(T_EPEFF); O(PF_Q) = Tv(T_Q); O(PF_CP) = Tv(T_CP);

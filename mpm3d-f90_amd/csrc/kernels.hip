@@ -356,8 +356,10 @@ struct Box {
   }
 };
 
-template <int SHAPE>
-__device__ __forceinline__ Box block_box(const InstC& I, bool act, int cx, int cy, int cz, int cap) {
+// Box of the points (cx..cx+hi, ...) from (cx+lo, ...) of the block's
+// active threads, clamped to [0, mx] x [0, my] x [0, mz].
+__device__ __forceinline__ Box block_box_n(bool act, int cx, int cy, int cz, int lo, int hi, int mx, int my, int mz,
+                                           int cap) {
   __shared__ int sb[6][16];
   __shared__ Box res;
   int v[6];
@@ -380,8 +382,6 @@ __device__ __forceinline__ Box block_box(const InstC& I, bool act, int cx, int c
       for (int q = 0; q < 3; q++) sb[q][0] = min(sb[q][0], sb[q][w]);
       for (int q = 3; q < 6; q++) sb[q][0] = max(sb[q][0], sb[q][w]);
     }
-    const int lo = SHAPE == MPM_SHAPE_LINEAR ? 0 : -1;
-    const int hi = SHAPE == MPM_SHAPE_GIMP ? 2 : 1;
     Box b;
     if (sb[3][0] < 0) {   // no particle in the grid
       b.x0 = b.y0 = b.z0 = 0;
@@ -390,14 +390,14 @@ __device__ __forceinline__ Box block_box(const InstC& I, bool act, int cx, int c
       b.ok = 1;
       b.inner = 0;
     } else {
-      b.inner = sb[0][0] + lo >= 0 && sb[1][0] + lo >= 0 && sb[2][0] + lo >= 0 && sb[3][0] + hi <= I.ncx &&
-                sb[4][0] + hi <= I.ncy && sb[5][0] + hi <= I.ncz;
+      b.inner = sb[0][0] + lo >= 0 && sb[1][0] + lo >= 0 && sb[2][0] + lo >= 0 && sb[3][0] + hi <= mx &&
+                sb[4][0] + hi <= my && sb[5][0] + hi <= mz;
       b.x0 = max(sb[0][0] + lo, 0);
       b.y0 = max(sb[1][0] + lo, 0);
       b.z0 = max(sb[2][0] + lo, 0);
-      b.bx = min(sb[3][0] + hi, I.ncx) - b.x0 + 1;
-      b.by = min(sb[4][0] + hi, I.ncy) - b.y0 + 1;
-      b.bz = min(sb[5][0] + hi, I.ncz) - b.z0 + 1;
+      b.bx = min(sb[3][0] + hi, mx) - b.x0 + 1;
+      b.by = min(sb[4][0] + hi, my) - b.y0 + 1;
+      b.bz = min(sb[5][0] + hi, mz) - b.z0 + 1;
       const long n = (long)b.bx * b.by * b.bz;
       b.ok = n <= cap;
       b.n = b.ok ? (int)n : 0;
@@ -406,6 +406,15 @@ __device__ __forceinline__ Box block_box(const InstC& I, bool act, int cx, int c
   }
   __syncthreads();
   return res;
+}
+
+// node box of a block (stencil reach per shape: linear 0..+1, B-spline
+// -1..+1, GIMP -1..+2 cells around the particles' cells)
+template <int SHAPE>
+__device__ __forceinline__ Box block_box(const InstC& I, bool act, int cx, int cy, int cz, int cap) {
+  const int lo = SHAPE == MPM_SHAPE_LINEAR ? 0 : -1;
+  const int hi = SHAPE == MPM_SHAPE_GIMP ? 2 : 1;
+  return block_box_n(act, cx, cy, cz, lo, hi, I.ncx, I.ncy, I.ncz, cap);
 }
 
 // ===================================================== tile-frame slots
@@ -1185,7 +1194,7 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
       // eos4 (:1513-1541)
       if (m.sm < (T)0) bulkq(m, I, dt);
       hieupd(m);
-      const double pnew = mc.c0 * mc.c0 * ((double)mp / m.vol - mc.density);
+      const double pnew = mc.c0 * mc.c0 * (p.d(D_MASS) / m.vol - mc.density);
       m.iener = m.iener - m.dvol * pnew;
       m.sm = (T)(-pnew);
       break;
@@ -1541,51 +1550,119 @@ __device__ __forceinline__ void sgn_shape(const double xi[3], T shp[8]) {
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
   int gran;
-  const long i = part_index(0, glist, bpg, gran);
+  const long i = part_index(P.ilv, glist, bpg, gran);
   const int ii = ginst[gran];
   const InstC& I = P.inst[ii];
   if (P.ts[ii].done) return;
+  constexpr bool MASS = (MODE & P2G_MASS) != 0, MOM = (MODE & P2G_MOM) != 0, FRC = (MODE & P2G_FORCE) != 0;
+  // LDS accumulation tile over the block's box of auxiliary nodes (FP64):
+  // mass, 3 momentum, 3 external and 6 internal force components
+  constexpr int NF = (MASS ? 1 : 0) + (MOM ? 3 : 0) + (FRC ? 9 : 0);
+  constexpr int CAP = NF > 4 ? TILE_NODES / 2 : TILE_NODES;
+  __shared__ double tile[CAP * NF];
   PV<T> p{P, i};
   const bool live = i < I.p_end;
   if (MODE & P2G_ICELL) p.n(I_CELL) = live ? in_which_cell(I, p.d(D_X), p.d(D_Y), p.d(D_Z)) : -1;
-  if (!live) return;
   SgLoc L;
-  if (!sgmp_locate(I, P.fix, p.d(D_X), p.d(D_Y), p.d(D_Z), L)) return;
-  T shp[8];
-  sgn_shape<T>(L.xxn, shp);
+  const bool act = live && sgmp_locate(I, P.fix, p.d(D_X), p.d(D_Y), p.d(D_Z), L);
+  // auxiliary coordinates of node 0 of the particle's auxiliary cell
+  int ax = 0, ay = 0, az = 0;
+  if (act) {
+    const int c0 = L.cn1 - 1;
+    az = c0 / I.ncxy;
+    const int r = c0 - az * I.ncxy;
+    ay = r / I.ncx;
+    ax = r - ay * I.ncx;
+  }
+  const Box B = block_box_n(act, ax, ay, az, 0, 1, I.ncx - 1, I.ncy - 1, I.ncz - 1, CAP);
+  // with fewer than 2 cells along an axis the reference's linear index of
+  // the +1 neighbour leaves the axis: direct path
+  const bool fast = B.ok && B.inner && I.ncx > 1 && I.ncy > 1 && I.ncz > 1;
+  if (fast) {
+    for (int e = threadIdx.x; e < B.n; e += blockDim.x)
+#pragma unroll
+      for (int f = 0; f < NF; f++) tile[f * CAP + e] = 0.0;
+    __syncthreads();
+  }
   const long nn = P.nn;
   T* aux = P.aux + I.node_off;
   double* cmg = P.cmg + I.node_off;
-  T m = (T)0, v[3] = {(T)0, (T)0, (T)0};
-  if (MODE & (P2G_MASS | P2G_MOM)) {
-    m = p.t(T_MASS);
-    v[0] = p.t(T_VX); v[1] = p.t(T_VY); v[2] = p.t(T_VZ);
-  }
-  T fx[3] = {(T)0, (T)0, (T)0}, sg[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, vol = (T)0;
-  if (MODE & P2G_FORCE) {
-    external_force(P, p, I, P.ts[ii].time, fx);
-    const T smq = p.t(T_SM) - p.t(T_Q);
-    sg[0] = smq + p.t(T_SD0); sg[1] = smq + p.t(T_SD1); sg[2] = smq + p.t(T_SD2);
-    sg[3] = p.t(T_SD5); sg[4] = p.t(T_SD3); sg[5] = p.t(T_SD4);   // Cfint(4..6): sxy, syz, sxz
-    vol = (T)p.d(D_VOL);
-  }
-#pragma unroll
-  for (int n = 0; n < 8; n++) {
-    const int k = sgmp_node(I, L, n);
-    if (!k) continue;
-    const long c = k - 1;
-    const T N = shp[n];
-    if (MODE & (P2G_MASS | P2G_MOM)) {
-      const T shm = N * m;
-      if (MODE & P2G_MASS) atomic_add(&cmg[c], (double)shm);
-#pragma unroll
-      for (int d = 0; d < 3; d++) atomic_add(&aux[(AX_P + d) * nn + c], v[d] * shm);
+  if (act) {
+    T shp[8];
+    sgn_shape<T>(L.xxn, shp);
+    T m = (T)0, v[3] = {(T)0, (T)0, (T)0};
+    if (MASS || MOM) {
+      m = p.t(T_MASS);
+      v[0] = p.t(T_VX); v[1] = p.t(T_VY); v[2] = p.t(T_VZ);
     }
-    if (MODE & P2G_FORCE) {
+    T fx[3] = {(T)0, (T)0, (T)0}, sg[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, vol = (T)0;
+    if (FRC) {
+      external_force(P, p, I, P.ts[ii].time, fx);
+      const T smq = p.t(T_SM) - p.t(T_Q);
+      sg[0] = smq + p.t(T_SD0); sg[1] = smq + p.t(T_SD1); sg[2] = smq + p.t(T_SD2);
+      sg[3] = p.t(T_SD5); sg[4] = p.t(T_SD3); sg[5] = p.t(T_SD4);   // Cfint(4..6): sxy, syz, sxz
+      vol = (T)p.d(D_VOL);
+    }
+    const int eb = fast ? (ax - B.x0) + B.bx * ((ay - B.y0) + B.by * (az - B.z0)) : 0;
 #pragma unroll
-      for (int d = 0; d < 3; d++) atomic_add(&aux[(AX_FE + d) * nn + c], fx[d] * N);
+    for (int n = 0; n < 8; n++) {
+      const T N = shp[n];
+      T val[NF];
+      int q = 0;
+      if (MASS || MOM) {
+        const T shm = N * m;
+        if (MASS) val[q++] = shm;
+        if (MOM)
 #pragma unroll
-      for (int q = 0; q < 6; q++) atomic_add(&aux[(AX_FI + q) * nn + c], N * sg[q] * vol);
+          for (int d = 0; d < 3; d++) val[q++] = v[d] * shm;
+      }
+      if (FRC) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) val[q++] = fx[d] * N;
+#pragma unroll
+        for (int k = 0; k < 6; k++) val[q++] = N * sg[k] * vol;
+      }
+      if (fast) {
+        // node n of the SNX/SNY/SNZ order: x + 1 for n & 3 in {1, 2}, y + 1 for {2, 3}, z + 1 for n >= 4
+        const int e = eb + ((n & 3) == 1 || (n & 3) == 2 ? 1 : 0) + ((n & 3) >= 2 ? B.bx : 0) +
+                      (n >= 4 ? B.bx * B.by : 0);
+#pragma unroll
+        for (int f = 0; f < NF; f++) atomicAdd(&tile[f * CAP + e], (double)val[f]);
+      } else {
+        const int k = sgmp_node(I, L, n);
+        if (!k) continue;
+        const long c = k - 1;
+        q = 0;
+        if (MASS) atomic_add(&cmg[c], (double)val[q++]);
+        if (MOM)
+#pragma unroll
+          for (int d = 0; d < 3; d++) atomic_add(&aux[(AX_P + d) * nn + c], val[q++]);
+        if (FRC)
+#pragma unroll
+          for (int f = 0; f < 9; f++) atomic_add(&aux[(AX_FE + f) * nn + c], val[q++]);
+      }
+    }
+  }
+  if (fast) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
+      const int ex = e % B.bx, r = e / B.bx, ey = r % B.by, ez = r / B.by;
+      const long c = (long)(B.z0 + ez) * I.ncxy + (long)(B.y0 + ey) * I.ncx + (B.x0 + ex);
+      int q = 0;
+      if (MASS) {
+        const double v = tile[q++ * CAP + e];
+        if (v != 0.0) atomic_add(&cmg[c], v);
+      }
+      if (MOM)
+        for (int d = 0; d < 3; d++) {
+          const double v = tile[q++ * CAP + e];
+          if (v != 0.0) atomic_add(&aux[(AX_P + d) * nn + c], (T)v);
+        }
+      if (FRC)
+        for (int f = 0; f < 9; f++) {
+          const double v = tile[q++ * CAP + e];
+          if (v != 0.0) atomic_add(&aux[(AX_FE + f) * nn + c], (T)v);
+        }
     }
   }
 }

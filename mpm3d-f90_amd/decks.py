@@ -14,6 +14,8 @@ hand-written deck).  Units and geometry:
   USF, gravity applied as a body load ramped over 0.5 s (QuLo).
 * ``slope_ensemble``: seeded variants of ``slope`` (size, angle, friction,
   cohesion) for the batched ensemble.
+* ``cook_membrane``: tapered neo-Hookean plate (nu = 0.4999) with a shear
+  traction ramp, SGMP; ``dam_break``: weakly compressible water column, SGMP.
 """
 import math
 import os
@@ -129,6 +131,62 @@ def slope_ensemble(count, seed=0, n_min=2e4, n_max=4e5, **kw):
         out.append((text, n, {"angle": ang, "phi": phi, "c": coh, "nodes": nodes,
                               "bodies": text.count("\nparticle ")}))
     return out
+
+
+def cook_membrane(dp=0.25, h=0.5, thickness=10.0, rho=1.0e-3, E=240.0, nu=0.4999, shear=100.0, ramp=1.0,
+                  damp=0.3, dtscale=0.9, endtime=5.0):
+    """Cook-membrane-like tapered plate (mm-g-N-ms-MPa): vertices (0,0),
+    (48,44), (48,60), (0,44) in x-y, `thickness` in z; nearly incompressible
+    neo-Hookean; left face clamped with one extra grid layer; shear traction
+    `shear` (N, total) on the right-edge particles in +y, ramped over `ramp`
+    (QuLo); damping; SGMP, MUSL.  One 'bloc' column per x position."""
+    nx = int(round(48.0 / dp))
+    nz = int(round(thickness / dp))
+    cols = []
+    for i in range(nx):
+        x = (i + 0.5) * dp
+        ylo, yhi = 44.0 * x / 48.0, 44.0 + 16.0 * x / 48.0
+        jlo = int(math.ceil(ylo / dp - 0.5))
+        jhi = int(math.floor(yhi / dp - 0.5))
+        cols.append((i, jlo, jhi - jlo + 1))
+    n = sum(ny * nz for _, _, ny in cols)
+    L = ["mpm3d *** Cook membrane plate 48x44(->16)x%g mm, h=%g, SGMP" % (thickness, h),
+         "nbco 1", "nbbo %d" % len(cols), "nbmp %d" % n, "nmat 1",
+         "spx %.6f %.6f" % (-h, 48.0 + 2 * h), "spy %.6f %.6f" % (-2 * h, 60.0 + 2 * h),
+         "spz %.6f %.6f" % (-2 * h, thickness + 2 * h), "dcell %g" % h,
+         "musl on", "sgmp", "dtscale %g" % dtscale, "endtime %g" % endtime, "QuLo %g" % ramp, "Damp %g" % damp,
+         "fixed 1 0 0 0 0 0", "material", " 1 neoh %g %g %g" % (rho, E, nu)]
+    for i, jlo, ny in cols:
+        L.append("particle bloc 1 1 %g %g %.6f %.6f 0.0 1 %d %d" % (rho, dp, i * dp, jlo * dp, ny, nz))
+    # right-edge particles: the last column's block, x fastest then y then z
+    last = cols[-1][2] * nz
+    first = n - last + 1
+    f = shear / last
+    L.append("load")
+    for pid in range(first, n + 1):
+        L.append("part %d 0.0 %.9e 0.0" % (pid, f))
+    L += ["endl", "endi"]
+    return "\n".join(L) + "\n", n
+
+
+def dam_break(h=0.005, width=0.6, depth=0.3, height=0.3, tank=1.6, c=50.0, rho=1000.0, dtscale=0.5, endtime=1.0,
+              sgmp=True):
+    """3D dam break (SI): water column width x depth x height at the x = 0
+    wall of a tank `tank` long, weakly compressible (null material, EOS 4,
+    sound speed c, K = rho c^2), gravity; walls and floor symmetric (fixed 2)
+    with one extra grid layer; 2x2x2 particles per cell; SGMP, MUSL."""
+    dp = h / 2
+    nx, ny, nz = int(round(width / dp)), int(round(depth / dp)), int(round(height / dp))
+    n = nx * ny * nz
+    L = ["mpm3d *** dam break %gx%gx%g m, h=%g" % (width, depth, height, h),
+         "nbco 1", "nbbo 1", "nbmp %d" % n, "nmat 1",
+         "spx %.6f %.6f" % (-h, tank + h), "spy %.6f %.6f" % (-h, depth + h), "spz %.6f %.6f" % (-h, 2 * height),
+         "dcell %g" % h, "musl on"] + (["sgmp"] if sgmp else []) + [
+         "dtscale %g" % dtscale, "endtime %g" % endtime, "fixed 2 2 2 2 2 0", "material",
+         " 1 null %g %g 1.0e-3" % (rho, rho * c * c), "seos 1 4 %g" % c,
+         "particle bloc 1 1 %g %g 0.0 0.0 0.0 %d %d %d" % (rho, dp, nx, ny, nz),
+         "load", "grav 1 0.0 0.0 -9.81", "endl", "endi"]
+    return "\n".join(L) + "\n", n
 
 
 def write_deck(text, directory=None, name="synthetic"):
