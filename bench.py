@@ -76,15 +76,17 @@ def kernel_bytes(kid, t, loads=True):
     return 0, 0
 
 
-def measured_traffic(kernel, prec, summary):
-    """HBM bytes per launch of a kernel class from a committed rocprofv3 PMC
-    summary (FETCH_SIZE and WRITE_SIZE, calibrated against device copies of
-    known size); None when absent."""
-    path = os.path.join(ROOT, "profiles", summary)
+def measured_traffic(kernel, prec, workload):
+    """HBM bytes per launch of a kernel class on a workload ("ensemble" or
+    "slope") from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE, calibrated against device copies of known size); None when
+    absent."""
+    path = os.path.join(ROOT, "profiles", "slope_ensemble_pmc_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        m = re.match(r"\|\s*%s\s*\|\s*%s\s*\|.*\|\s*([0-9.e+]+)\s*\|\s*$" % (kernel, prec), line)
+        m = re.match(r"\|\s*%s\s*\|\s*%s\s*\|\s*%s\s*\|.*\|\s*([0-9.e+]+)\s*\|\s*$" % (kernel, prec, workload),
+                     line)
         if m:
             return float(m.group(1))
     return None
@@ -224,13 +226,13 @@ def run_ensemble(mpm, decks, a, log):
     for prec, eng in engs.items():
         eng.step(a.warmup)
         eng.sync()
-        tune = eng.autotune(trial_steps=4)
+        tune = eng.autotune(trial_steps=4) if a.autotune else eng.tuning()
         per = largest_divisor_upto(tune["sort_every"], 10) if tune["sort_every"] > 0 else 10
         log("ensemble %s tuning: %s, graph of %d steps" % (prec, json.dumps(tune), per))
-        eng.step(per, graph=True, graph_steps=per)   # capture outside the timed region
+        eng.step(per, graph=a.graph, graph_steps=per)   # capture outside the timed region
         eng.sync()
         t0 = time.perf_counter()
-        eng.step(a.steps, graph=True, graph_steps=per)
+        eng.step(a.steps, graph=a.graph, graph_steps=per)
         eng.sync()
         sec = time.perf_counter() - t0
         eng.profile(True)
@@ -327,10 +329,10 @@ def run_sgmp(mpm, decks, a, log):
     eng.sync()
     tune = eng.autotune(trial_steps=3)
     per = largest_divisor_upto(tune["sort_every"], 10) if tune["sort_every"] > 0 else 10
-    eng.step(per, graph=True, graph_steps=per)
+    eng.step(per, graph=a.graph, graph_steps=per)
     eng.sync()
     t0 = time.perf_counter()
-    eng.step(a.sgmp_steps, graph=True, graph_steps=per)
+    eng.step(a.sgmp_steps, graph=a.graph, graph_steps=per)
     eng.sync()
     sec = time.perf_counter() - t0
     eng.profile(True)
@@ -405,6 +407,9 @@ def main():
     ap.add_argument("--check-steps", type=int, default=2, help="steps of the single slope compared with the oracle")
     ap.add_argument("--cpu-steps", type=int, default=30, help="steps of the CPU baseline sample per instance")
     ap.add_argument("--no-cpu", action="store_true", help="skip oracle parity and CPU baseline")
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="timed steps as plain launches instead of graph replays (for tracing tools)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false", help="keep the default launch shape")
     ap.add_argument("--sgmp-cooks", type=int, default=64, help="Cook plates of the SGMP batch (0: skip)")
     ap.add_argument("--sgmp-steps", type=int, default=10, help="timed steps of the SGMP batch")
     ap.add_argument("--serve-rates", default="10,50,200", help="Poisson arrival rates (req/s) of the serving runs")
@@ -456,7 +461,7 @@ def main():
 
     e32 = E["fp32"]
     kt = e32["kt"]
-    traffic = lambda k: measured_traffic(k, "fp32", "ensemble_pmc_traffic.md")
+    traffic = lambda k: measured_traffic(k, "fp32", "ensemble")
     v64p = plain["fp64"]["value"]
     out = {
         "metric": "particle-steps/s FP32-batched vs FP64-plain @<=1e-5 rel err; p50/p99 latency",
@@ -488,7 +493,7 @@ def main():
         "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()},
         "roofline": roofline(kt, E["particles"], E["touched"], 4, traffic),
         "roofline_plain_fp32": roofline(plain["fp32"]["kt"], prob.n, plain["fp32"]["touched"], 4,
-                                        lambda k: measured_traffic(k, "fp32", "slope_pmc_traffic.md")),
+                                        lambda k: measured_traffic(k, "fp32", "slope")),
         "parity": parity or None,
         "taylor_bar": bar,
         "sgmp": sgmp,

@@ -72,8 +72,8 @@ typedef struct mpm_tuning {
 } mpm_tuning;
 int mpm_engine_get_tuning(mpm_engine* e, mpm_tuning* out);
 int mpm_engine_apply_tuning(mpm_engine* e, const mpm_tuning* t);
-/* Time launch shapes on the resident instances (trial_steps graph-replayed
- * steps each, from a device snapshot of the state that is restored
+/* Time launch shapes on the resident instances (trial_steps steps each,
+ * from a device snapshot of the state that is restored
  * afterwards, so the simulation is unchanged) and apply the fastest. */
 int mpm_engine_autotune(mpm_engine* e, int trial_steps, mpm_tuning* best);
 

@@ -1176,16 +1176,18 @@ static mpm_tuning current_tuning(const mpm_engine* e) {
   return t;
 }
 
-// mean milliseconds of one graph-replayed step over n steps, no re-sort
+// mean milliseconds of one step over n steps, no re-sort (plain launches:
+// the trials would otherwise capture and instantiate a graph per shape, and
+// the launch overhead is not what is being compared)
 static double time_trial(mpm_engine* e, int n, std::vector<float>* each = nullptr) {
   const int se = e->sort_every;
   e->sort_every = 0;
-  step_graph(e, 1, 1, FIN_DONE);   // capture (if needed) + warm-up
+  step_plain(e, 1, FIN_DONE);   // warm-up
   std::vector<hipEvent_t> ev(each ? n + 1 : 2);
   for (auto& x : ev) HIPCHK(hipEventCreate(&x));
   HIPCHK(hipEventRecord(ev[0], e->s));
   for (int k = 0; k < n; k++) {
-    step_graph(e, 1, 1, FIN_DONE);
+    step_plain(e, 1, FIN_DONE);
     if (each) HIPCHK(hipEventRecord(ev[k + 1], e->s));
   }
   if (!each) HIPCHK(hipEventRecord(ev[1], e->s));
