@@ -124,11 +124,13 @@ struct HostInst {
   int n = 0;
   int group = -1;
   bool p_in_p2 = false;    // latest PXg lives in the re-map buffer
+  int matmask = 0;         // material models used (bits 1 << type)
   bool xx_pending = false; // G2P done, Xp not yet moved (per-phase mode)
 };
 
 struct Group {
   int shape = 0, scheme = 0;
+  int matmask = 0;   // material models of the group's instances (bits 1 << type)
   std::vector<int> insts;
   std::vector<int> pgran, ngran;
   int* d_pgran = nullptr;
@@ -470,6 +472,7 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   h.body0 = (int)b0;
   h.nbody = p->nb_body;
   h.n = n;
+  for (int m = 0; m < p->nb_mat; m++) h.matmask |= 1 << p->mats[m].type;
   InstC& c = h.c;
   c.ncx = p->ncx; c.ncy = p->ncy; c.ncz = p->ncz;
   c.ncxy = c.ncx * c.ncy;
@@ -643,6 +646,7 @@ static void rebuild_lists(mpm_engine* e) {
     }
     Group& G = e->groups[g];
     G.insts.push_back(id);
+    G.matmask |= h.matmask;
     for (long k = 0; k < h.npg; k++) G.pgran.push_back((int)(h.pg0 + k));
     for (long k = 0; k < h.nng; k++) G.ngran.push_back((int)(h.ng0 + k));
   }
@@ -719,7 +723,7 @@ static void run_phase(mpm_engine* e, int ph) {
       case PH_STRESS:
         mpmk::launch_bc<T>(P, N, 0, 0, 1, s);   // nodal velocity of the current PXg
         if (sg) mpmk::launch_sgmp_cells<T>(SGC_GRAD, P, N, s);
-        mpmk::launch_stress<T>(G.shape, G.scheme == MPM_USF ? 0 : ST_MOVE, P, L, s);
+        mpmk::launch_stress<T>(G.shape, G.scheme == MPM_USF ? 0 : ST_MOVE, MM_ALL, P, L, s);
         for (int id : G.insts) e->inst[id].xx_pending = false;
         break;
       case PH_GMU:
@@ -816,7 +820,7 @@ static void enqueue_sgmp(mpm_engine* e, Group& G, const PoolPtrs<T>& P, const La
     timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 0, 1, 1, s); });
     timed(e, KI_STRESS, [&] {
       mpmk::launch_sgmp_cells<T>(SGC_GRAD, P, N, s);
-      mpmk::launch_stress<T>(MPM_SHAPE_SGMP, 0, P, L, s);
+      mpmk::launch_stress<T>(MPM_SHAPE_SGMP, 0, MM_ALL, P, L, s);
     });
     timed(e, KI_P2G, [&] {
       mpmk::launch_zero_nodes<T>(aux + AUX_FORCE_OFF * nn, 9, nn, N, s);
@@ -845,7 +849,7 @@ static void enqueue_sgmp(mpm_engine* e, Group& G, const PoolPtrs<T>& P, const La
     }
     timed(e, KI_STRESS, [&] {
       mpmk::launch_sgmp_cells<T>(SGC_GRAD, P, N, s);
-      mpmk::launch_stress<T>(MPM_SHAPE_SGMP, ST_MOVE | ST_ENERGY, P, L, s);
+      mpmk::launch_stress<T>(MPM_SHAPE_SGMP, ST_MOVE | ST_ENERGY, MM_ALL, P, L, s);
     });
   }
 }
@@ -869,20 +873,20 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
     } else if (G.scheme == MPM_USF) {
       timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, L, s); });
       timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 0, 1, 1, s); });
-      timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_FORCE, P, L, s); });
+      timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_FORCE, G.matmask, P, L, s); });
       timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
       timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, G2P_MOVE | G2P_ENERGY, P, L, s); });
     } else if (G.scheme == MPM_USL) {
       timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE, P, L, s); });
       timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
       timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, 0, P, L, s); });
-      timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_MOVE | ST_ENERGY, P, L, s); });
+      timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_MOVE | ST_ENERGY, G.matmask, P, L, s); });
     } else {
       timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE, P, L, s); });
       timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
       timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, G2P_REMAP, P, L, s); });
       timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 1, 1, 1, s); });
-      timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_MOVE | ST_ENERGY, P, L, s); });
+      timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_MOVE | ST_ENERGY, G.matmask, P, L, s); });
     }
     timed(e, KI_FINALIZE, [&] {
       mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), L.bpg, FIN_ENERGY | FIN_DT | fin_flags, s);

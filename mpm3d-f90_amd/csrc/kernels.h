@@ -35,6 +35,9 @@
 #define AUX_COUNT 18
 #define AUX_FORCE_OFF 3
 
+// all material models (general stress kernel)
+#define MM_ALL 0xffff
+
 // k_finalize parts
 #define FIN_ENERGY 1
 #define FIN_DT 2
@@ -52,7 +55,9 @@ struct LaunchP {
 namespace mpmk {
 template <typename T> void launch_p2g(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_g2p(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
-template <typename T> void launch_stress(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
+// mm: material models present in the launch's instances (bits 1 << type)
+template <typename T>
+void launch_stress(int shape, int mode, int mm, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_integrate(const PoolPtrs<T>& P, const LaunchP& L, int va, hipStream_t s);
 template <typename T>
 void launch_bc(const PoolPtrs<T>& P, const LaunchP& L, int use_p2, int bc, int vel, hipStream_t s);

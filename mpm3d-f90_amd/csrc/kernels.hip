@@ -1092,7 +1092,10 @@ __device__ __forceinline__ double det3(const double F[3][3]) {
 // Constitution (Constitution.f90:79-368) for the models on the path:
 // elas (M3DM1), pla1 (M3DM2), pla2 (M3DM3), null+EOS4 (M3DM7/eos4),
 // Drucker-Prager (M3DM9), neo-Hookean (M3DM10).
-template <typename T>
+// material models compiled into a stress kernel: bit (1 << type); a launch
+// over a group whose instances use one model gets the kernel with only that
+// model (fewer registers, higher occupancy), mixed groups the general one
+template <typename T, int MM = MM_ALL>
 __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, const MatC& mc, T dt,
                                              const T de[6], const T vort[3], const double F[3][3],
                                              const double deF[3][3]) {
@@ -1133,7 +1136,11 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
   m.bqf = (T)0;
   m.ieinc = (T)0;
   const T G2 = (T)mc.G2, K3 = (T)mc.K3;
-  switch (mc.type) {
+  // a model outside MM cannot occur here (the engine picks MM from the group)
+  const int mtype = MM == (1 << MPM_MAT_ELASTIC) ? MPM_MAT_ELASTIC : MM == (1 << MPM_MAT_PLA1) ? MPM_MAT_PLA1
+                  : MM == (1 << MPM_MAT_PLA2) ? MPM_MAT_PLA2 : MM == (1 << MPM_MAT_NULL) ? MPM_MAT_NULL
+                  : MM == (1 << MPM_MAT_DP) ? MPM_MAT_DP : MM == (1 << MPM_MAT_NEOH) ? MPM_MAT_NEOH : mc.type;
+  switch (mtype) {
     case MPM_MAT_ELASTIC:
       sigrot(m, vort);
       m.ieinc = sig_dot(m);
@@ -1293,7 +1300,7 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
 // ParticleStressUpdate after the gather (update_step.f90:566-642): strain
 // and spin increments, F update (UVFp), Constitution, and Xp = XX for
 // USL/MUSL.  L is the velocity gradient sum (deFp before the DT scaling).
-template <typename T>
+template <typename T, int MM>
 __device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p, const InstC& I, const InstT& ts,
                                             T de[6], T vort[3], const T L[3][3], bool uvfp, double x, double y,
                                             double z, bool move) {
@@ -1329,7 +1336,7 @@ __device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p
       }
     }
     const MatC& mc = P.mats[P.bodies[p.n(I_BODY)].mat];
-    constitution<T>(p, I, mc, dt, de, vort, F, deF);
+    constitution<T, MM>(p, I, mc, dt, de, vort, F, deF);
     if (move) {   // Xp = XX (update_step.f90:642)
       p.d(D_X) = x + (double)p.t(T_BX) * dtd;
       p.d(D_Y) = y + (double)p.t(T_BY) * dtd;
@@ -1340,7 +1347,7 @@ __device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p
 // ParticleStressUpdate (update_step.f90:456-647): strain-rate gather, F
 // update, constitution; optionally Xp = XX (USL/MUSL), the USF force
 // scatter, and the energy partials.
-template <int SHAPE, typename T, int MODE>
+template <int SHAPE, typename T, int MODE, int MM>
 __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
   int gran;
   const long i = part_index(P.ilv, glist, bpg, gran);
@@ -1415,7 +1422,7 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
       }
     });
     }
-    stress_tail<T>(P, p, I, ts, de, vort, L, uvfp, x, y, z, (MODE & ST_MOVE) != 0);
+    stress_tail<T, MM>(P, p, I, ts, de, vort, L, uvfp, x, y, z, (MODE & ST_MOVE) != 0);
   }
   if (MODE & ST_FORCE) {   // USF: GridMomentumUpdate scatter right after the update
     T* Fg = P.fg + I.node_off;
@@ -1830,7 +1837,7 @@ __global__ void __launch_bounds__(256) k_sgmp_cells(PoolPtrs<T> P, const int* ng
 
 // ParticleStressUpdate, SGMP: de, vort, deFp interpolated from the centres
 // (update_step.f90:542-562), then the common update
-template <typename T, int MODE>
+template <typename T, int MODE, int MM>
 __global__ void __launch_bounds__(256) k_sgmp_stress(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
   int gran;
   const long i = part_index(0, glist, bpg, gran);
@@ -1864,7 +1871,7 @@ __global__ void __launch_bounds__(256) k_sgmp_stress(PoolPtrs<T> P, const int* g
 #pragma unroll
         for (int b = 0; b < 3; b++) Lg[a][b] = Lg[a][b] + shp[n] * aux[(AX_DF + 3 * a + b) * nn + c];
     }
-    stress_tail<T>(P, p, I, ts, de, vort, Lg, (I.flags & IF_UVFP) != 0, x, y, z, (MODE & ST_MOVE) != 0);
+    stress_tail<T, MM>(P, p, I, ts, de, vort, Lg, (I.flags & IF_UVFP) != 0, x, y, z, (MODE & ST_MOVE) != 0);
   }
   if (MODE & ST_ENERGY) {
     double ev[6] = {0, 0, 0, 0, 0, 0};
@@ -2148,19 +2155,35 @@ void launch_g2p_s(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t 
 #undef G2P_CASE
 }
 
-template <int SHAPE, typename T>
-void launch_stress_s(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+// the fused-step modes get one kernel per single material model, the
+// per-phase modes only the general one
+template <int SHAPE, typename T, int MODE>
+void launch_stress_m(int mm, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
   dim3 g(L.nblk), b(L.bs);
-#define ST_CASE(M) \
-  case M: hipLaunchKernelGGL((k_stress<SHAPE, T, M>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+#define ST_M(M) \
+  case M: hipLaunchKernelGGL((k_stress<SHAPE, T, MODE, M>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+  switch (mm) {
+    ST_M(1 << MPM_MAT_ELASTIC)
+    ST_M(1 << MPM_MAT_PLA1)
+    ST_M(1 << MPM_MAT_PLA2)
+    ST_M(1 << MPM_MAT_NULL)
+    ST_M(1 << MPM_MAT_DP)
+    ST_M(1 << MPM_MAT_NEOH)
+    default: hipLaunchKernelGGL((k_stress<SHAPE, T, MODE, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+  }
+#undef ST_M
+}
+
+template <int SHAPE, typename T>
+void launch_stress_s(int mode, int mm, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  dim3 g(L.nblk), b(L.bs);
   switch (mode) {
-    ST_CASE(0)
-    ST_CASE(ST_MOVE)
-    ST_CASE(ST_FORCE)
-    ST_CASE(ST_MOVE | ST_ENERGY)
+    case ST_FORCE: launch_stress_m<SHAPE, T, ST_FORCE>(mm, P, L, s); break;
+    case ST_MOVE | ST_ENERGY: launch_stress_m<SHAPE, T, ST_MOVE | ST_ENERGY>(mm, P, L, s); break;
+    case ST_MOVE: hipLaunchKernelGGL((k_stress<SHAPE, T, ST_MOVE, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+    case 0: hipLaunchKernelGGL((k_stress<SHAPE, T, 0, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
     default: break;
   }
-#undef ST_CASE
 }
 
 template <typename T>
@@ -2192,20 +2215,20 @@ void launch_g2p(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hip
 }
 
 template <typename T>
-void launch_stress(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+void launch_stress(int shape, int mode, int mm, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
   if (!L.nblk) return;
   if (shape == MPM_SHAPE_SGMP) {
     dim3 g(L.nblk), b(L.bs);
-    if (mode == ST_MOVE) hipLaunchKernelGGL((k_sgmp_stress<T, ST_MOVE>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
+    if (mode == ST_MOVE) hipLaunchKernelGGL((k_sgmp_stress<T, ST_MOVE, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
     else if (mode == (ST_MOVE | ST_ENERGY))
-      hipLaunchKernelGGL((k_sgmp_stress<T, ST_MOVE | ST_ENERGY>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
-    else hipLaunchKernelGGL((k_sgmp_stress<T, 0>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
+      hipLaunchKernelGGL((k_sgmp_stress<T, ST_MOVE | ST_ENERGY, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
+    else hipLaunchKernelGGL((k_sgmp_stress<T, 0, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
     return;
   }
   switch (shape) {
-    case MPM_SHAPE_LINEAR: launch_stress_s<MPM_SHAPE_LINEAR, T>(mode, P, L, s); break;
-    case MPM_SHAPE_GIMP: launch_stress_s<MPM_SHAPE_GIMP, T>(mode, P, L, s); break;
-    default: launch_stress_s<MPM_SHAPE_BSPLINE, T>(mode, P, L, s); break;
+    case MPM_SHAPE_LINEAR: launch_stress_s<MPM_SHAPE_LINEAR, T>(mode, mm, P, L, s); break;
+    case MPM_SHAPE_GIMP: launch_stress_s<MPM_SHAPE_GIMP, T>(mode, mm, P, L, s); break;
+    default: launch_stress_s<MPM_SHAPE_BSPLINE, T>(mode, mm, P, L, s); break;
   }
 }
 
@@ -2302,7 +2325,7 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
 #define INST(T)                                                                                     \
   template void launch_p2g<T>(int, int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);          \
   template void launch_g2p<T>(int, int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);          \
-  template void launch_stress<T>(int, int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);       \
+  template void launch_stress<T>(int, int, int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);  \
   template void launch_integrate<T>(const PoolPtrs<T>&, const LaunchP&, int, hipStream_t);         \
   template void launch_bc<T>(const PoolPtrs<T>&, const LaunchP&, int, int, int, hipStream_t);      \
   template void launch_node_va<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                \
