@@ -67,6 +67,8 @@ typedef struct mpm_tuning {
   int32_t sort_brick;        /* 0 (Hilbert), 1, 2, 4, 8 */
   int32_t interleave;        /* 0 / 1 */
   int32_t trials;            /* configurations timed by the autotuner */
+  int32_t block_scatter;     /* particles per block of the particle-to-grid scatters (64, 128, 256; 0 keeps) */
+  int32_t pad;
   double step_ms;            /* mean step time of the chosen shape, sort excluded */
   double sort_ms;            /* time of one re-sort */
 } mpm_tuning;

@@ -177,6 +177,7 @@ struct mpm_engine {
   size_t cap_ilist = 0;
   int n_active = 0;
   int bs_p = 128, bs_n = 256;     // block sizes (autotuned)
+  int bs_s = 256;                 // block size of the particle-to-grid scatter launches
   int ilv = 1;                    // interleaved lane mapping in particle kernels
   int sort_every = 20;            // steps between re-sorts in mpm_step (0: never)
   int brick = 0;                  // sort order: 0 Hilbert, B > 0 bricks of B^3 cells
@@ -665,12 +666,12 @@ static void rebuild_lists(mpm_engine* e) {
   e->lists_dirty = false;
 }
 
-static LaunchP plist(const mpm_engine* e, const Group& G) {
+static LaunchP plist(const mpm_engine* e, const Group& G, int bs = 0) {
   LaunchP L;
   L.glist = G.d_pgran;
   L.ginst = e->d_ginst;
-  L.bs = e->bs_p;
-  L.bpg = PGRAN / e->bs_p;
+  L.bs = bs > 0 ? bs : e->bs_p;
+  L.bpg = PGRAN / L.bs;
   L.nblk = (int)G.pgran.size() * L.bpg;
   return L;
 }
@@ -701,19 +702,19 @@ static void run_phase(mpm_engine* e, int ph) {
     return;
   }
   for (Group& G : e->groups) {
-    const LaunchP L = plist(e, G), N = nlist(e, G);
+    const LaunchP L = plist(e, G), N = nlist(e, G), LS = plist(e, G, e->bs_s);
     const bool sg = G.shape == MPM_SHAPE_SGMP;
     switch (ph) {
       case PH_GMI:
         if (sg) {   // particles -> auxiliary nodes -> grid nodes
           mpmk::launch_zero_nodes<double>(e->cmg, 1, e->nn, N, s);
           mpmk::launch_zero_nodes<T>((T*)e->aux, 3, e->nn, N, s);
-          mpmk::launch_sgmp_p2g<T>(P2G_ICELL | P2G_MASS | P2G_MOM, P, L, s);
+          mpmk::launch_sgmp_p2g<T>(P2G_ICELL | P2G_MASS | P2G_MOM, P, LS, s);
           mpmk::launch_sgmp_nodes<T>(SGN_MASSMOM, P, N, s);
         } else {
           mpmk::launch_zero_nodes<double>(e->mg, 1, e->nn, N, s);
           mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
-          mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, L, s);
+          mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, LS, s);
         }
         for (int id : G.insts) e->inst[id].p_in_p2 = false;
         break;
@@ -729,11 +730,11 @@ static void run_phase(mpm_engine* e, int ph) {
       case PH_GMU:
         if (sg) {
           mpmk::launch_zero_nodes<T>((T*)e->aux + AUX_FORCE_OFF * e->nn, 9, e->nn, N, s);
-          mpmk::launch_sgmp_p2g<T>(P2G_FORCE, P, L, s);
+          mpmk::launch_sgmp_p2g<T>(P2G_FORCE, P, LS, s);
           mpmk::launch_sgmp_nodes<T>(SGN_FORCE, P, N, s);
         } else {
           mpmk::launch_zero_nodes<T>((T*)e->fg, 3, e->nn, N, s);
-          mpmk::launch_p2g<T>(G.shape, P2G_FORCE, P, L, s);
+          mpmk::launch_p2g<T>(G.shape, P2G_FORCE, P, LS, s);
         }
         break;
       case PH_INTEGRATE:
@@ -748,11 +749,11 @@ static void run_phase(mpm_engine* e, int ph) {
       case PH_MUSL:
         if (sg) {
           mpmk::launch_zero_nodes<T>((T*)e->aux, 3, e->nn, N, s);
-          mpmk::launch_sgmp_p2g<T>(P2G_MOM, P, L, s);
+          mpmk::launch_sgmp_p2g<T>(P2G_MOM, P, LS, s);
           mpmk::launch_sgmp_nodes<T>(SGN_MOM, P, N, s);
         } else {
           mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
-          mpmk::launch_p2g<T>(G.shape, P2G_MOM, P, L, s);
+          mpmk::launch_p2g<T>(G.shape, P2G_MOM, P, LS, s);
         }
         mpmk::launch_bc<T>(P, N, 0, 1, 0, s);
         break;
@@ -803,7 +804,8 @@ static void timed(mpm_engine* e, int kid, F&& launch) {
 // one step of an SGMP group: particle passes through the auxiliary nodes,
 // with the centre <-> node passes between them (timed with their phase)
 template <typename T>
-static void enqueue_sgmp(mpm_engine* e, Group& G, const PoolPtrs<T>& P, const LaunchP& L, const LaunchP& N) {
+static void enqueue_sgmp(mpm_engine* e, Group& G, const PoolPtrs<T>& P, const LaunchP& L, const LaunchP& LS,
+                         const LaunchP& N) {
   hipStream_t s = e->s;
   const long nn = e->nn;
   T* aux = (T*)e->aux;
@@ -811,7 +813,7 @@ static void enqueue_sgmp(mpm_engine* e, Group& G, const PoolPtrs<T>& P, const La
     timed(e, KI_P2G, [&] {
       mpmk::launch_zero_nodes<double>(e->cmg, 1, nn, N, s);
       mpmk::launch_zero_nodes<T>(aux, (mode & P2G_FORCE) ? 12 : 3, nn, N, s);
-      mpmk::launch_sgmp_p2g<T>(mode, P, L, s);
+      mpmk::launch_sgmp_p2g<T>(mode, P, LS, s);
       mpmk::launch_sgmp_nodes<T>((mode & P2G_FORCE) ? SGN_MASSMOM | SGN_FORCE : SGN_MASSMOM, P, N, s);
     });
   };
@@ -824,7 +826,7 @@ static void enqueue_sgmp(mpm_engine* e, Group& G, const PoolPtrs<T>& P, const La
     });
     timed(e, KI_P2G, [&] {
       mpmk::launch_zero_nodes<T>(aux + AUX_FORCE_OFF * nn, 9, nn, N, s);
-      mpmk::launch_sgmp_p2g<T>(P2G_FORCE, P, L, s);
+      mpmk::launch_sgmp_p2g<T>(P2G_FORCE, P, LS, s);
       mpmk::launch_sgmp_nodes<T>(SGN_FORCE, P, N, s);
     });
     timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
@@ -842,7 +844,7 @@ static void enqueue_sgmp(mpm_engine* e, Group& G, const PoolPtrs<T>& P, const La
     if (G.scheme == MPM_MUSL) {
       timed(e, KI_P2G, [&] {
         mpmk::launch_zero_nodes<T>(aux, 3, nn, N, s);
-        mpmk::launch_sgmp_p2g<T>(P2G_MOM, P, L, s);
+        mpmk::launch_sgmp_p2g<T>(P2G_MOM, P, LS, s);
         mpmk::launch_sgmp_nodes<T>(SGN_MOM | SGN_TO_P2, P, N, s);
       });
       timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 1, 1, 1, s); });
@@ -861,7 +863,7 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
   hipStream_t s = e->s;
   timed(e, KI_BEGIN, [&] { mpmk::launch_begin_step(e->d_ts, e->d_ilist_all, e->n_active, s); });
   for (Group& G : e->groups) {
-    const LaunchP L = plist(e, G), N = nlist(e, G);
+    const LaunchP L = plist(e, G), N = nlist(e, G), LS = plist(e, G, e->bs_s);
     timed(e, KI_ZERO, [&] {
       mpmk::launch_zero_nodes<double>(e->mg, 1, e->nn, N, s);
       mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
@@ -869,20 +871,20 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
       if (G.scheme == MPM_MUSL) mpmk::launch_zero_nodes<T>((T*)e->p2, 3, e->nn, N, s);
     });
     if (G.shape == MPM_SHAPE_SGMP) {
-      enqueue_sgmp<T>(e, G, P, L, N);
+      enqueue_sgmp<T>(e, G, P, L, LS, N);
     } else if (G.scheme == MPM_USF) {
-      timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, L, s); });
+      timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, LS, s); });
       timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 0, 1, 1, s); });
       timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_FORCE, G.matmask, P, L, s); });
       timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
       timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, G2P_MOVE | G2P_ENERGY, P, L, s); });
     } else if (G.scheme == MPM_USL) {
-      timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE, P, L, s); });
+      timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE, P, LS, s); });
       timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
       timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, 0, P, L, s); });
       timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_MOVE | ST_ENERGY, G.matmask, P, L, s); });
     } else {
-      timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE, P, L, s); });
+      timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE, P, LS, s); });
       timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
       timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, G2P_REMAP, P, L, s); });
       timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 1, 1, 1, s); });
@@ -1167,6 +1169,10 @@ static void apply_tuning(mpm_engine* e, const mpm_tuning& t) {
   e->sort_every = t.sort_every;
   e->brick = t.sort_brick;
   e->ilv = t.interleave != 0;
+  if (t.block_scatter) {
+    if (!ok(t.block_scatter)) fail(MPM_ERR_ARG, "bad tuning values");
+    e->bs_s = t.block_scatter;
+  }
   e->graph_dirty = true;
 }
 
@@ -1177,6 +1183,7 @@ static mpm_tuning current_tuning(const mpm_engine* e) {
   t.sort_every = e->sort_every;
   t.sort_brick = e->brick;
   t.interleave = e->ilv;
+  t.block_scatter = e->bs_s;
   return t;
 }
 
@@ -1225,7 +1232,8 @@ static double time_sort(mpm_engine* e) {
 // Coordinate search over the launch shape on the resident workload:
 // (1) sort order (Hilbert, 2^3 or 4^3 bricks) x particle block size x lane
 // mapping, timed right
-// after a re-sort; (2) node block size; (3) re-sort period, from the
+// after a re-sort; (2) node block size; (3) block size of the
+// particle-to-grid scatters; (4) re-sort period, from the
 // measured cost of one sort and the measured growth of the step time as
 // particles drift from their sorted order: the period P minimising
 // (sort + sum of the first P step times) / P.  Every trial starts from a
@@ -1266,6 +1274,13 @@ static mpm_tuning autotune(mpm_engine* e, int trial_steps) {
     t = best;
     for (int bn : {64, 128}) {
       t.block_nodes = bn;
+      trial(t);
+    }
+    t = best;
+    const int bsc0 = best.block_scatter;   // already timed
+    for (int bsc : {64, 128, 256}) {   // particle-to-grid scatters: bigger blocks flush fewer tile nodes
+      if (bsc == bsc0) continue;
+      t.block_scatter = bsc;
       trial(t);
     }
     // re-sort period

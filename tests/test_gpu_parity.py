@@ -320,7 +320,8 @@ def test_autotune_leaves_the_state_and_picks_valid_shapes(mpm):
     t = a.autotune(trial_steps=2)
     assert t["block_particles"] in (64, 128, 256) and t["block_nodes"] in (64, 128, 256)
     assert t["sort_brick"] in (0, 2, 4) and t["interleave"] in (0, 1)
-    assert 1 <= t["sort_every"] <= 64 and t["trials"] == 18 + 2 + 1
+    assert t["block_scatter"] in (64, 128, 256)
+    assert 1 <= t["sort_every"] <= 64 and t["trials"] == 18 + 2 + 2 + 1
     assert t["step_ms"] > 0 and t["sort_ms"] > 0
     assert a.tuning()["sort_brick"] == t["sort_brick"]
     for x, p0 in zip(ia, before):
