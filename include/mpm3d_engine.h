@@ -68,7 +68,7 @@ typedef struct mpm_tuning {
   int32_t interleave;        /* 0 / 1 */
   int32_t trials;            /* configurations timed by the autotuner */
   int32_t block_scatter;     /* particles per block of the particle-to-grid scatters (64, 128, 256; 0 keeps) */
-  int32_t pad;
+  int32_t tile_nodes;        /* node capacity of the particle kernels' LDS tiles (8..512; 0 keeps) */
   double step_ms;            /* mean step time of the chosen shape, sort excluded */
   double sort_ms;            /* time of one re-sort */
 } mpm_tuning;

@@ -63,7 +63,7 @@ class Tuning(C.Structure):
     """mpm_tuning (include/mpm3d_engine.h)"""
     _fields_ = [("block_particles", C.c_int32), ("block_nodes", C.c_int32), ("sort_every", C.c_int32),
                 ("sort_brick", C.c_int32), ("interleave", C.c_int32), ("trials", C.c_int32),
-                ("block_scatter", C.c_int32), ("pad", C.c_int32), ("step_ms", C.c_double), ("sort_ms", C.c_double)]
+                ("block_scatter", C.c_int32), ("tile_nodes", C.c_int32), ("step_ms", C.c_double), ("sort_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

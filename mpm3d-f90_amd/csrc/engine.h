@@ -120,4 +120,5 @@ struct PoolPtrs {
   const BodyC* bodies;
   double* part;  // energy partials: 6 per particle block
   int ilv;       // interleaved lane -> particle mapping in particle kernels
+  int tile_cap;  // nodes a block's LDS tile may hold (<= TILE_NODES; smaller values exercise the fallbacks)
 };

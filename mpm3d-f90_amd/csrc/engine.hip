@@ -178,6 +178,7 @@ struct mpm_engine {
   int n_active = 0;
   int bs_p = 128, bs_n = 256;     // block sizes (autotuned)
   int bs_s = 256;                 // block size of the particle-to-grid scatter launches
+  int tile_cap = 512;             // nodes per LDS tile of the particle kernels (TILE_NODES)
   int ilv = 1;                    // interleaved lane mapping in particle kernels
   int sort_every = 20;            // steps between re-sorts in mpm_step (0: never)
   int brick = 0;                  // sort order: 0 Hilbert, B > 0 bricks of B^3 cells
@@ -230,6 +231,7 @@ struct mpm_engine {
     P.bodies = d_bodies;
     P.part = part;
     P.ilv = ilv;
+    P.tile_cap = tile_cap;
     return P;
   }
 };
@@ -1173,6 +1175,10 @@ static void apply_tuning(mpm_engine* e, const mpm_tuning& t) {
     if (!ok(t.block_scatter)) fail(MPM_ERR_ARG, "bad tuning values");
     e->bs_s = t.block_scatter;
   }
+  if (t.tile_nodes) {
+    if (t.tile_nodes < 8 || t.tile_nodes > 512) fail(MPM_ERR_ARG, "bad tuning values");
+    e->tile_cap = t.tile_nodes;
+  }
   e->graph_dirty = true;
 }
 
@@ -1184,6 +1190,7 @@ static mpm_tuning current_tuning(const mpm_engine* e) {
   t.sort_brick = e->brick;
   t.interleave = e->ilv;
   t.block_scatter = e->bs_s;
+  t.tile_nodes = e->tile_cap;
   return t;
 }
 

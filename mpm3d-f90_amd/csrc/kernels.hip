@@ -417,6 +417,74 @@ __device__ __forceinline__ Box block_box(const InstC& I, bool act, int cx, int c
   return block_box_n(act, cx, cy, cz, lo, hi, I.ncx, I.ncy, I.ncz, cap);
 }
 
+// Two node boxes of one block.  A block of consecutive particles in the
+// cell order can straddle a jump of the space-filling curve (where the curve
+// leaves the occupied cells and re-enters elsewhere); its single box then
+// exceeds the tile and the block would fall back to global atomics.  Such a
+// block is split at the largest jump between consecutive particles and each
+// side gets a box of its own; the two share the tile (box 1 from slot
+// b[0].n).  A single-box block has split = blockDim.x.
+struct Box2 {
+  Box b[2];
+  int split, n, ok, inner;
+  __device__ __forceinline__ int side(int j) const { return j >= split; }
+  __device__ __forceinline__ int off(int k) const { return k ? b[0].n : 0; }
+  __device__ __forceinline__ int node(const InstC& I, int e) const {
+    return e < b[0].n ? b[0].node(I, e) : b[1].node(I, e - b[0].n);
+  }
+  // tile slot of a node for the per-node (non-inner) paths, which only
+  // single-box blocks take
+  __device__ __forceinline__ int idx(int nx, int ny, int nz) const { return b[0].idx(nx, ny, nz); }
+};
+
+// jl: the particle's index in the block (particle order); scratch: 4 ints
+// per thread of LDS that is free until the function returns (the tile)
+template <int SHAPE>
+__device__ Box2 block_box2(const InstC& I, bool act, int cx, int cy, int cz, int jl, int cap, int* scratch) {
+  __shared__ int wbest[16];
+  Box2 r;
+  r.b[0] = block_box<SHAPE>(I, act, cx, cy, cz, cap);
+  r.b[1] = r.b[0];
+  r.b[1].n = 0;
+  r.split = blockDim.x;
+  r.n = r.b[0].n;
+  r.ok = r.b[0].ok;
+  r.inner = r.b[0].inner;
+  if (r.ok) return r;   // block-uniform
+  const int t = threadIdx.x;
+  scratch[4 * jl] = cx;
+  scratch[4 * jl + 1] = cy;
+  scratch[4 * jl + 2] = cz;
+  scratch[4 * jl + 3] = act ? 1 : 0;
+  __syncthreads();
+  int key = -1;   // (jump << 16) | (0xffff - j) at particle j = t: the largest jump, first on ties
+  if (t >= 1 && scratch[4 * t + 3] && scratch[4 * t - 1]) {
+    const int d = max(max(abs(scratch[4 * t] - scratch[4 * t - 4]), abs(scratch[4 * t + 1] - scratch[4 * t - 3])),
+                      abs(scratch[4 * t + 2] - scratch[4 * t - 2]));
+    key = (min(d, 32767) << 16) | (0xffff - t);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) key = max(key, __shfl_xor(key, o));
+  if ((t & 63) == 0) wbest[t >> 6] = key;
+  __syncthreads();
+  key = wbest[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); w++) key = max(key, wbest[w]);
+  if (key < 0) return r;
+  const int j = 0xffff - (key & 0xffff);
+  const bool s1 = jl >= j;
+  const Box b0 = block_box<SHAPE>(I, act && !s1, cx, cy, cz, cap);
+  const Box b1 = block_box<SHAPE>(I, act && s1, cx, cy, cz, cap);
+  if (b0.ok && b1.ok && b0.inner && b1.inner && b0.n + b1.n <= cap) {
+    r.b[0] = b0;
+    r.b[1] = b1;
+    r.split = j;
+    r.n = b0.n + b1.n;
+    r.ok = 1;
+    r.inner = 1;
+  }
+  return r;
+}
+
 // ===================================================== tile-frame slots
 //
 // In a block whose box is inner, every particle's influence nodes are the
@@ -467,8 +535,8 @@ struct AosTile {
   static constexpr int S = 4 * G;          // elements per node
 };
 
-template <typename T, int NF>
-__device__ __forceinline__ void load_tile_aos(const InstC& I, const Box& B, T* tf, const T* const* src, long nn) {
+template <typename T, int NF, typename BX>
+__device__ __forceinline__ void load_tile_aos(const InstC& I, const BX& B, T* tf, const T* const* src, long nn) {
   constexpr int S = AosTile<T, NF>::S;
   for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
     const int k = B.node(I, e);
@@ -619,6 +687,222 @@ __device__ __forceinline__ void scatter_force(const Slots<T>& S, int sb, int sc,
     }
 }
 
+// ========================================================= run-summed scatters
+//
+// After the cell sort a block's particles come in runs that share a cell,
+// so the same NS^3 tile slots.  Adding every particle's node values to the
+// LDS tile one by one keeps the LDS busy (27 atomics per field and particle
+// for B-spline, lanes of one instruction spread over the tile's banks).
+// Instead the particles of an inner block stage their weights and payload
+// in LDS, one thread per run sums the run's node values in registers, slab
+// by slab, and only the run sums reach the tile.  The per-particle terms are
+// the ones of scatter_w / scatter_force; only the summation order changes
+// (as with any other order of the atomics).
+//
+// Stage layout (dynamic LDS): NG groups of 16 bytes per particle slot, the
+// slot of block-local particle j at j + j/8 (run heads are about 8 apart
+// after the sort; the skew puts their 16-byte reads in different banks),
+// then the run key of every particle (its stencil's first tile slot, or -1)
+// and the compacted list of run heads.
+
+template <typename T>
+struct V16;
+template <>
+struct V16<float> {
+  using type = float4;
+  static constexpr int n = 4;
+  __device__ static type make(const float* v) { return make_float4(v[0], v[1], v[2], v[3]); }
+  __device__ static void split(const type& u, float* v) { v[0] = u.x; v[1] = u.y; v[2] = u.z; v[3] = u.w; }
+};
+template <>
+struct V16<double> {
+  using type = double2;
+  static constexpr int n = 2;
+  __device__ static type make(const double* v) { return make_double2(v[0], v[1]); }
+  __device__ static void split(const type& u, double* v) { v[0] = u.x; v[1] = u.y; }
+};
+
+// record of one staged particle: the per-axis weights w[3][NS], their
+// derivative factors d[3][NS] (force scatters), mass and velocity (mass /
+// momentum scatters), -vol*sigma (xx yy zz xy yz xz) and the external force
+template <int NS, bool MQ, bool FRC>
+struct RunRec {
+  static constexpr int W = 0;
+  static constexpr int D = 3 * NS;                       // valid if FRC
+  static constexpr int M = D + (FRC ? 3 * NS : 0);       // m, q0, q1, q2 if MQ
+  static constexpr int S = M + (MQ ? 4 : 0);             // sv[6], fx[3] if FRC
+  static constexpr int NV = S + (FRC ? 9 : 0);
+};
+
+__host__ __device__ constexpr int run_slots(int bs) { return bs + bs / 8; }
+
+template <typename T, int NV>
+__host__ __device__ constexpr int run_groups() { return (NV + 16 / (int)sizeof(T) - 1) / (16 / (int)sizeof(T)); }
+
+template <typename T, int NV>
+__host__ __device__ constexpr size_t run_stage_bytes(int bs) {
+  return (size_t)run_groups<T, NV>() * run_slots(bs) * 16 + (size_t)(2 * bs + 4) * sizeof(int);
+}
+
+template <typename T, int NV>
+struct RunStage {
+  static constexpr int NG = run_groups<T, NV>();
+  static constexpr int GV = V16<T>::n;
+  typename V16<T>::type* g;
+  int* key;
+  int* head;
+  int* nh;
+  int slots, n;
+  __device__ RunStage(char* mem, int bs) {
+    slots = run_slots(bs);
+    n = bs;
+    g = (typename V16<T>::type*)mem;
+    key = (int*)(mem + (size_t)NG * slots * 16);
+    head = key + bs;
+    nh = head + bs;
+  }
+  __device__ void put(int j, const T (&v)[NG * GV]) const {
+    const int s = j + (j >> 3);
+#pragma unroll
+    for (int q = 0; q < NG; q++) g[q * slots + s] = V16<T>::make(v + q * GV);
+  }
+  __device__ void get(int j, T (&v)[NG * GV]) const {
+    const int s = j + (j >> 3);
+#pragma unroll
+    for (int q = 0; q < NG; q++) V16<T>::split(g[q * slots + s], v + q * GV);
+  }
+};
+
+// fill a record's weight part from the particle's slots
+template <int NS, typename T, int NVV, bool FRC>
+__device__ __forceinline__ void run_rec_slots(const Slots<T>& S, T (&v)[NVV]) {
+#pragma unroll
+  for (int ax = 0; ax < 3; ax++)
+#pragma unroll
+    for (int a = 0; a < NS; a++) {
+      v[ax * NS + a] = S.w[ax][a];
+      if (FRC) v[3 * NS + ax * NS + a] = S.d[ax][a];
+    }
+}
+
+// after the stage (keys and records) is written and the block synchronised:
+// compact the run heads (first particle of every maximal run of equal keys)
+__device__ __forceinline__ void run_heads(const int* key, int* head, int* nh, int n) {
+  const int j = threadIdx.x;
+  const bool h = j < n && key[j] >= 0 && (j == 0 || key[j - 1] != key[j]);
+  const unsigned long long m = __ballot(h);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0 && m) base = atomicAdd(nh, __popcll(m));
+  base = __shfl(base, 0);
+  if (h) head[base + __popcll(m & ((1ull << lane) - 1))] = j;
+}
+
+// run sums of the mass (FP64) and NQ velocity-weighted mass components
+template <int NS, typename T, int NV, int OM, bool MASS, int NQ, typename A>
+__device__ __forceinline__ void run_scatter_w(const RunStage<T, NV>& R, const Box2& B, double* tm, A* tv,
+                                              int cap) {
+  const int k = threadIdx.x;
+  if (k >= *R.nh) return;
+  const int j0 = R.head[k], e0 = R.key[j0];
+  const Box& Bk = B.b[B.side(j0)];
+  const int sb = Bk.bx, sc = Bk.bx * Bk.by;
+  int j1 = j0 + 1;
+  while (j1 < R.n && R.key[j1] == e0) j1++;
+  constexpr int NVV = RunStage<T, NV>::NG * RunStage<T, NV>::GV;
+#pragma unroll
+  for (int c = 0; c < NS; c++) {
+    double am[NS][NS];
+    T aq[NS][NS][NQ > 0 ? NQ : 1];
+#pragma unroll
+    for (int b = 0; b < NS; b++)
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        am[b][a] = 0.0;
+#pragma unroll
+        for (int f = 0; f < NQ; f++) aq[b][a][f] = (T)0;
+      }
+    for (int j = j0; j < j1; j++) {
+      T v[NVV];
+      R.get(j, v);
+      const T m = v[OM];
+#pragma unroll
+      for (int b = 0; b < NS; b++) {
+        const T mw = m * (v[NS + b] * v[2 * NS + c]);
+#pragma unroll
+        for (int a = 0; a < NS; a++) {
+          const T shm = v[a] * mw;
+          if (MASS) am[b][a] += (double)shm;
+#pragma unroll
+          for (int f = 0; f < NQ; f++) aq[b][a][f] += v[OM + 1 + f] * shm;
+        }
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NS; b++)
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        const int e = e0 + a + b * sb + c * sc;
+        if (MASS) atomicAdd(&tm[e], am[b][a]);
+#pragma unroll
+        for (int f = 0; f < NQ; f++) atomicAdd(&tv[f * cap + e], (A)aq[b][a][f]);
+      }
+  }
+}
+
+// run sums of the nodal force -(sigma . grad N) vol + fext N
+template <int NS, typename T, int NV, int OD, int OS, typename A>
+__device__ __forceinline__ void run_scatter_force(const RunStage<T, NV>& R, const Box2& B, A* tv, int cap) {
+  const int k = threadIdx.x;
+  if (k >= *R.nh) return;
+  const int j0 = R.head[k], e0 = R.key[j0];
+  const Box& Bk = B.b[B.side(j0)];
+  const int sb = Bk.bx, sc = Bk.bx * Bk.by;
+  int j1 = j0 + 1;
+  while (j1 < R.n && R.key[j1] == e0) j1++;
+  constexpr int NVV = RunStage<T, NV>::NG * RunStage<T, NV>::GV;
+#pragma unroll
+  for (int c = 0; c < NS; c++) {
+    T af[NS][NS][3];
+#pragma unroll
+    for (int b = 0; b < NS; b++)
+#pragma unroll
+      for (int a = 0; a < NS; a++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) af[b][a][i] = (T)0;
+    for (int j = j0; j < j1; j++) {
+      T v[NVV];
+      R.get(j, v);
+      const T sv[3][3] = {{v[OS + 0], v[OS + 3], v[OS + 5]},
+                          {v[OS + 3], v[OS + 1], v[OS + 4]},
+                          {v[OS + 5], v[OS + 4], v[OS + 2]}};
+      const T w2 = v[2 * NS + c], d2 = v[OD + 2 * NS + c];
+#pragma unroll
+      for (int b = 0; b < NS; b++) {
+        const T wyz = v[NS + b] * w2, dyz = v[OD + NS + b] * w2, wdz = v[NS + b] * d2;
+        T Af[3], Q[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          Af[i] = sv[i][0] * wyz;
+          Q[i] = sv[i][1] * dyz + sv[i][2] * wdz + v[OS + 6 + i] * wyz;
+        }
+#pragma unroll
+        for (int a = 0; a < NS; a++)
+#pragma unroll
+          for (int i = 0; i < 3; i++) af[b][a][i] += v[OD + a] * Af[i] + v[a] * Q[i];
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NS; b++)
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        const int e = e0 + a + b * sb + c * sc;
+#pragma unroll
+        for (int i = 0; i < 3; i++) atomicAdd(&tv[i * cap + e], (A)af[b][a][i]);
+      }
+  }
+}
+
 // ================================================================ kernels
 
 // GridMomentumInitial / GridMomentumUpdate / GridMomentumMUSL scatters
@@ -628,7 +912,7 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
   int gran;
   const long i = part_index(P.ilv, glist, bpg, gran);
   const int ii = ginst[gran];
-  const InstC& I = P.inst[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   if (P.ts[ii].done) return;
   constexpr bool MASS = (MODE & P2G_MASS) != 0, MOM = (MODE & P2G_MOM) != 0, FRC = (MODE & P2G_FORCE) != 0;
   constexpr int NT = (MOM ? 3 : 0) + (FRC ? 3 : 0);
@@ -643,13 +927,28 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
   const bool live = i < I.p_end;
   int c = -1;
   double x = 0, y = 0, z = 0;
+  // every particle load is issued before the box reduction (two barriers),
+  // so that their latencies overlap
+  T m = (T)0, vx = (T)0, vy = (T)0, vz = (T)0;
+  T fx[3] = {(T)0, (T)0, (T)0};
+  T sxx = 0, syy = 0, szz = 0, sxy = 0, syz = 0, sxz = 0, vol = 0;
   if (live) {
     x = p.d(D_X); y = p.d(D_Y); z = p.d(D_Z);
+    if (!(MODE & P2G_ICELL)) c = p.n(I_CELL);
+    if (MASS || MOM) {
+      m = p.t(T_MASS);
+      vx = p.t(T_VX); vy = p.t(T_VY); vz = p.t(T_VZ);
+    }
+    if (FRC) {
+      external_force(P, p, I, P.ts[ii].time, fx);
+      const T smq = p.t(T_SM) - p.t(T_Q);
+      sxx = smq + p.t(T_SD0); syy = smq + p.t(T_SD1); szz = smq + p.t(T_SD2);
+      sxy = p.t(T_SD5); syz = p.t(T_SD3); sxz = p.t(T_SD4);
+      vol = (T)p.d(D_VOL);
+    }
     if (MODE & P2G_ICELL) {
       c = in_which_cell(I, x, y, z);
       p.n(I_CELL) = c;
-    } else {
-      c = p.n(I_CELL);
     }
   } else if (MODE & P2G_ICELL) {
     p.n(I_CELL) = -1;
@@ -657,7 +956,8 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
   const bool act = c >= 0;
   int cx = 0, cy = 0, cz = 0;
   if (act) cell_coords(I, c, cx, cy, cz);
-  const Box B = block_box<SHAPE>(I, act, cx, cy, cz, CAP);
+  const int jl = (int)(i - ((long)gran * PGRAN + (long)(blockIdx.x % bpg) * blockDim.x));
+  const Box2 B = block_box2<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem);
   const long nn = P.nn;
   if (B.ok) {
     for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
@@ -670,37 +970,41 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
   double* Mg = P.mg + I.node_off;
   T* Pg = ((MODE & P2G_TO_P2) ? P.p2 : P.pg) + I.node_off;
   T* Fg = P.fg + I.node_off;
-  if (act) {
-    T m = (T)0, vx = (T)0, vy = (T)0, vz = (T)0;
-    if (MASS || MOM) {
-      m = p.t(T_MASS);
-      vx = p.t(T_VX); vy = p.t(T_VY); vz = p.t(T_VZ);
-    }
-    T fx[3] = {(T)0, (T)0, (T)0};
-    T sxx = 0, syy = 0, szz = 0, sxy = 0, syz = 0, sxz = 0, vol = 0;
-    if (FRC) {
-      external_force(P, p, I, P.ts[ii].time, fx);
-      const T smq = p.t(T_SM) - p.t(T_Q);
-      sxx = smq + p.t(T_SD0); syy = smq + p.t(T_SD1); szz = smq + p.t(T_SD2);
-      sxy = p.t(T_SD5); syz = p.t(T_SD3); sxz = p.t(T_SD4);
-      vol = (T)p.d(D_VOL);
-    }
-    if (B.ok && B.inner) {
-      constexpr int NS = NSlot<SHAPE>::n;
+  if (B.ok && B.inner) {   // block-uniform: staged run sums
+    constexpr int NS = NSlot<SHAPE>::n;
+    using RR = RunRec<NS, MASS || MOM, FRC>;
+    extern __shared__ __align__(16) char run_mem[];
+    const RunStage<T, RR::NV> R(run_mem, blockDim.x);
+    constexpr int NVV = RunStage<T, RR::NV>::NG * RunStage<T, RR::NV>::GV;
+    int key = -1;
+    if (act) {
       Slots<T> S;
-      make_slots<SHAPE, T>(I, B, x, y, z, c, S);
-      const int sb = B.bx, sc = B.bx * B.by;
+      const int kb = B.side(jl);
+      make_slots<SHAPE, T>(I, B.b[kb], x, y, z, c, S);
+      S.e0 += B.off(kb);
+      T v[NVV];
+#pragma unroll
+      for (int q = 0; q < NVV; q++) v[q] = (T)0;
+      run_rec_slots<NS, T, NVV, FRC>(S, v);
       if (MASS || MOM) {
-        const T q[3] = {vx, vy, vz};
-        scatter_w<NS, T, MASS, MOM ? 3 : 0>(S, sb, sc, m, q, tm, tf, CAP);
+        v[RR::M] = m; v[RR::M + 1] = vx; v[RR::M + 2] = vy; v[RR::M + 3] = vz;
       }
       if (FRC) {
-        const T sv[3][3] = {{-(sxx * vol), -(sxy * vol), -(sxz * vol)},
-                            {-(sxy * vol), -(syy * vol), -(syz * vol)},
-                            {-(sxz * vol), -(syz * vol), -(szz * vol)}};
-        scatter_force<NS, T>(S, sb, sc, sv, fx, tf + (MOM ? 3 : 0) * CAP, CAP);
+        v[RR::S + 0] = -(sxx * vol); v[RR::S + 1] = -(syy * vol); v[RR::S + 2] = -(szz * vol);
+        v[RR::S + 3] = -(sxy * vol); v[RR::S + 4] = -(syz * vol); v[RR::S + 5] = -(sxz * vol);
+        v[RR::S + 6] = fx[0]; v[RR::S + 7] = fx[1]; v[RR::S + 8] = fx[2];
       }
-    } else {
+      R.put(jl, v);
+      key = S.e0;
+    }
+    R.key[jl] = key;
+    if (threadIdx.x == 0) *R.nh = 0;
+    __syncthreads();
+    run_heads(R.key, R.head, R.nh, R.n);
+    __syncthreads();
+    if (MASS || MOM) run_scatter_w<NS, T, RR::NV, RR::M, MASS, MOM ? 3 : 0>(R, B, tm, tf, CAP);
+    if (FRC) run_scatter_force<NS, T, RR::NV, RR::D, RR::S>(R, B, tf + (MOM ? 3 : 0) * CAP, CAP);
+  } else if (act) {
     Stencil<T> S;
     build_stencil<SHAPE, T>(I, x, y, z, c, S);
     for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T dX, T dY, T dZ) {
@@ -725,7 +1029,6 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
         if (FRC) { atomic_add(&Fg[k], val[q]); atomic_add(&Fg[nn + k], val[q + 1]); atomic_add(&Fg[2 * nn + k], val[q + 2]); }
       }
     });
-    }
   }
   if (B.ok && B.n) {
     __syncthreads();
@@ -752,8 +1055,8 @@ __global__ void __launch_bounds__(256) k_integrate(PoolPtrs<T> P, const int* ngl
   int gran;
   const long j = node_index(nglist, bpg, gran);
   const int ii = nginst[gran];
-  const InstC& I = P.inst[ii];
-  const InstT& ts = P.ts[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
+  const InstT ts = P.ts[ii];
   if (ts.done) return;
   if (j >= (long)I.node_off + I.nnode) return;
   const long nn = P.nn;
@@ -790,7 +1093,7 @@ __global__ void __launch_bounds__(256) k_bc(PoolPtrs<T> P, const int* nglist, co
   int gran;
   const long j = node_index(nglist, bpg, gran);
   const int ii = nginst[gran];
-  const InstC& I = P.inst[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   if (P.ts[ii].done) return;
   if (j >= (long)I.node_off + I.nnode) return;
   const uint8_t fx = bc ? P.fix[j] : 0;
@@ -817,7 +1120,7 @@ __global__ void __launch_bounds__(256) k_node_va(PoolPtrs<T> P, const int* nglis
   int gran;
   const long j = node_index(nglist, bpg, gran);
   const int ii = nginst[gran];
-  const InstC& I = P.inst[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   if (P.ts[ii].done) return;
   if (j >= (long)I.node_off + I.nnode) return;
   const long nn = P.nn;
@@ -839,8 +1142,8 @@ __global__ void __launch_bounds__(256) k_zero_nodes(T* a, int ncomp, long stride
 }
 
 // Gather the block's node box (NT fields of T) into LDS.
-template <typename T, int NT>
-__device__ __forceinline__ void load_tile(const InstC& I, const Box& B, T* tf, int cap, const T* const* src,
+template <typename T, int NT, typename BX>
+__device__ __forceinline__ void load_tile(const InstC& I, const BX& B, T* tf, int cap, const T* const* src,
                                           long nn) {
   for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
     const int k = B.node(I, e);
@@ -849,16 +1152,16 @@ __device__ __forceinline__ void load_tile(const InstC& I, const Box& B, T* tf, i
   }
 }
 
-template <int NT, typename A>
-__device__ __forceinline__ void zero_tile(const Box& B, A* ta, int cap) {
+template <int NT, typename A, typename BX>
+__device__ __forceinline__ void zero_tile(const BX& B, A* ta, int cap) {
   for (int e = threadIdx.x; e < B.n; e += blockDim.x)
 #pragma unroll
     for (int f = 0; f < NT; f++) ta[f * cap + e] = (A)0;
 }
 
 // flush a 3-component accumulation tile into a grid field of type T
-template <typename T, typename A>
-__device__ __forceinline__ void flush_tile3(const InstC& I, const Box& B, const A* ta, int cap, T* dst, long nn) {
+template <typename T, typename A, typename BX>
+__device__ __forceinline__ void flush_tile3(const InstC& I, const BX& B, const A* ta, int cap, T* dst, long nn) {
   for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
     const int k = B.node(I, e);
 #pragma unroll
@@ -877,8 +1180,8 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
   int gran;
   const long i = part_index(P.ilv, glist, bpg, gran);
   const int ii = ginst[gran];
-  const InstC& I = P.inst[ii];
-  const InstT& ts = P.ts[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
+  const InstT ts = P.ts[ii];
   if (ts.done) return;
   constexpr int NS = NSlot<SHAPE>::n;
   using A = double;   // re-map accumulation type (see k_p2g)
@@ -889,13 +1192,26 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
   A* ta = (A*)smem;   // re-map accumulation, reuses the gather tile
   PV<T> p{P, i};
   const bool live = i < I.p_end;
-  const int c = live ? p.n(I_CELL) : -1;
+  // the particle's loads are issued before the box reduction (two barriers)
+  // and the tile load, so that their latencies overlap
+  int c = -1;
+  double x = 0.0, y = 0.0, z = 0.0;
+  T v0[3] = {(T)0, (T)0, (T)0}, mp = (T)0;
+  if (live) {
+    c = p.n(I_CELL);
+    x = p.d(D_X); y = p.d(D_Y); z = p.d(D_Z);
+#pragma unroll
+    for (int d = 0; d < 3; d++) v0[d] = p.t(T_VX + d);
+    if (MODE & G2P_REMAP) mp = p.t(T_MASS);
+  }
   const bool act = c >= 0;
   int cx = 0, cy = 0, cz = 0;
   if (act) cell_coords(I, c, cx, cy, cz);
-  const Box B = block_box<SHAPE>(I, act, cx, cy, cz, CAP);
+  const int jl = (int)(i - ((long)gran * PGRAN + (long)(blockIdx.x % bpg) * blockDim.x));
+  const Box2 B = block_box2<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem);
   const bool fast = B.ok && B.inner;
-  const int sb = B.bx, sc = B.bx * B.by;
+  const int kb = B.side(jl);
+  const int sb = B.b[kb].bx, sc = B.b[kb].bx * B.b[kb].by;
   const long nn = P.nn;
   const T* Vg = P.vg + I.node_off;
   const T* Ag = P.ag + I.node_off;
@@ -909,11 +1225,11 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
   Stencil<T> S;
   Slots<T> F;
   T vn[3] = {(T)0, (T)0, (T)0};
-  const double x = act ? p.d(D_X) : 0.0, y = act ? p.d(D_Y) : 0.0, z = act ? p.d(D_Z) : 0.0;
   if (act) {
     T vx[3] = {(T)0, (T)0, (T)0}, ax[3] = {(T)0, (T)0, (T)0};
     if (fast) {
-      make_slots<SHAPE, T>(I, B, x, y, z, c, F);
+      make_slots<SHAPE, T>(I, B.b[kb], x, y, z, c, F);
+      F.e0 += B.off(kb);
       T va[6];
       gather_w<NS, T, 6>(F, tf, sb, sc, va);
 #pragma unroll
@@ -938,7 +1254,7 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
     const T tdt = (T)dt;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      const T v = p.t(T_VX + d);
+      const T v = v0[d];
       vn[d] = (ts.istep == 1) ? v + ax[d] * tdt * (T)0.5 : v + ax[d] * tdt;
       p.t(T_VX + d) = vn[d];
       p.t(T_BX + d) = vx[d];
@@ -957,7 +1273,7 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
       __syncthreads();
     }
     if (act) {
-      const T m = p.t(T_MASS);
+      const T m = mp;
       if (fast) {
         scatter_w<NS, T, false, 3>(F, sb, sc, m, vn, nullptr, ta, CAP);
       } else {
@@ -992,8 +1308,24 @@ struct Cst {
   T dinc[6];
   double vold, vol, dvol, iener;   // FP64 in both paths
   T sm, smold, sd[6], sig[6], sold[6], bqf;
-  T seqv, epeff, sigy, ieinc, den0, den, vol0, cp;
+  T seqv, epeff, sigy, ieinc, den0, den, vol0, cp, mp;
 };
+
+// the particle state the constitutive update reads (loaded ahead of the
+// strain-rate gather so that the loads overlap the block's other latencies)
+template <typename T>
+__device__ __forceinline__ void cst_load(const PV<T>& p, Cst<T>& m) {
+  m.vold = p.d(D_VOL);
+  m.epeff = p.t(T_EPEFF);
+  m.sigy = p.t(T_SIGY);
+  m.seqv = p.t(T_SEQV);
+  m.iener = p.d(D_IE);
+  m.mp = p.t(T_MASS);
+  m.cp = p.t(T_CP);
+  m.sm = p.t(T_SM);
+#pragma unroll
+  for (int k = 0; k < 6; k++) m.sd[k] = p.t(T_SD0 + k);
+}
 
 template <typename T>
 __device__ __forceinline__ T eq_stress(const Cst<T>& m) {   // EquivalentStress, FP32 result
@@ -1098,11 +1430,9 @@ __device__ __forceinline__ double det3(const double F[3][3]) {
 template <typename T, int MM = MM_ALL>
 __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, const MatC& mc, T dt,
                                              const T de[6], const T vort[3], const double F[3][3],
-                                             const double deF[3][3]) {
-  Cst<T> m;
+                                             const double deF[3][3], Cst<T>& m) {
 #pragma unroll
   for (int k = 0; k < 6; k++) m.dinc[k] = de[k];
-  m.vold = p.d(D_VOL);
   if (I.flags & IF_UVFP) {
     const double detF = det3(deF);
     m.vol = m.vold * detF;
@@ -1111,22 +1441,13 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
   }
   p.d(D_VOL) = m.vol;
   m.dvol = 0.5 * (m.vol - m.vold);
-  m.epeff = p.t(T_EPEFF);
-  m.sigy = p.t(T_SIGY);
-  m.seqv = p.t(T_SEQV);
-  m.iener = p.d(D_IE);
-  const T mp = p.t(T_MASS);
+  const T mp = m.mp;
   m.den0 = (T)mc.density;
   m.vol0 = mp / m.den0;
   m.den = (T)((double)mp / m.vol);
-  m.cp = p.t(T_CP);
-  m.sm = p.t(T_SM);
   m.smold = m.sm;
 #pragma unroll
-  for (int k = 0; k < 6; k++) {
-    m.sd[k] = p.t(T_SD0 + k);
-    m.sold[k] = m.sd[k];
-  }
+  for (int k = 0; k < 6; k++) m.sold[k] = m.sd[k];
   m.sig[0] = m.sd[0] + m.sm;
   m.sig[1] = m.sd[1] + m.sm;
   m.sig[2] = m.sd[2] + m.sm;
@@ -1303,7 +1624,7 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
 template <typename T, int MM>
 __device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p, const InstC& I, const InstT& ts,
                                             T de[6], T vort[3], const T L[3][3], bool uvfp, double x, double y,
-                                            double z, bool move) {
+                                            double z, bool move, const MatC& mc, Cst<T>& m) {
     const double dtd = ts.dt;
     const T dt = (T)dtd;
 #pragma unroll
@@ -1335,8 +1656,7 @@ __device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p
         F[a][a] = F[a][a] + 1.0;
       }
     }
-    const MatC& mc = P.mats[P.bodies[p.n(I_BODY)].mat];
-    constitution<T, MM>(p, I, mc, dt, de, vort, F, deF);
+    constitution<T, MM>(p, I, mc, dt, de, vort, F, deF, m);
     if (move) {   // Xp = XX (update_step.f90:642)
       p.d(D_X) = x + (double)p.t(T_BX) * dtd;
       p.d(D_Y) = y + (double)p.t(T_BY) * dtd;
@@ -1352,8 +1672,8 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
   int gran;
   const long i = part_index(P.ilv, glist, bpg, gran);
   const int ii = ginst[gran];
-  const InstC& I = P.inst[ii];
-  const InstT& ts = P.ts[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
+  const InstT ts = P.ts[ii];
   if (ts.done) return;
   constexpr int NS = NSlot<SHAPE>::n;
   using A = double;   // force accumulation type (see k_p2g)
@@ -1364,13 +1684,25 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
   A* ta = (A*)smem;   // force accumulation after the gather
   PV<T> p{P, i};
   const bool live = i < I.p_end;
-  const int c = live ? p.n(I_CELL) : -1;
+  // the particle's loads are issued before the box reduction (two barriers)
+  // and the tile load, so that their latencies overlap
+  int c = -1, body = 0;
+  double x = 0.0, y = 0.0, z = 0.0;
+  Cst<T> cs;
+  if (live) {
+    c = p.n(I_CELL);
+    x = p.d(D_X); y = p.d(D_Y); z = p.d(D_Z);
+    body = p.n(I_BODY);
+    cst_load(p, cs);
+  }
   const bool act = c >= 0;
   int cx = 0, cy = 0, cz = 0;
   if (act) cell_coords(I, c, cx, cy, cz);
-  const Box B = block_box<SHAPE>(I, act, cx, cy, cz, CAP);
+  const int jl = (int)(i - ((long)gran * PGRAN + (long)(blockIdx.x % bpg) * blockDim.x));
+  const Box2 B = block_box2<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem);
   const bool fast = B.ok && B.inner;
-  const int sb = B.bx, sc = B.bx * B.by;
+  const int kb = B.side(jl);
+  const int sb = B.b[kb].bx, sc = B.b[kb].bx * B.b[kb].by;
   const long nn = P.nn;
   const T* Vg = P.vg + I.node_off;
   if (B.ok && B.n) {
@@ -1382,13 +1714,13 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
   double ev[6] = {0, 0, 0, 0, 0, 0};
   Stencil<T> S;
   Slots<T> FS;
-  const double x = act ? p.d(D_X) : 0.0, y = act ? p.d(D_Y) : 0.0, z = act ? p.d(D_Z) : 0.0;
   if (act) {
     T de[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, vort[3] = {(T)0, (T)0, (T)0};
     T L[3][3] = {{(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}};
     const bool uvfp = (I.flags & IF_UVFP) != 0;
     if (fast) {
-      make_slots<SHAPE, T>(I, B, x, y, z, c, FS);
+      make_slots<SHAPE, T>(I, B.b[kb], x, y, z, c, FS);
+      FS.e0 += B.off(kb);
       gather_grad<NS, T>(FS, tf, sb, sc, L);
       de[0] = L[0][0];
       de[1] = L[1][1];
@@ -1422,28 +1754,50 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
       }
     });
     }
-    stress_tail<T, MM>(P, p, I, ts, de, vort, L, uvfp, x, y, z, (MODE & ST_MOVE) != 0);
+    stress_tail<T, MM>(P, p, I, ts, de, vort, L, uvfp, x, y, z, (MODE & ST_MOVE) != 0, P.mats[P.bodies[body].mat],
+                       cs);
   }
   if (MODE & ST_FORCE) {   // USF: GridMomentumUpdate scatter right after the update
     T* Fg = P.fg + I.node_off;
+    using RR = RunRec<NS, false, true>;
+    extern __shared__ __align__(16) char run_mem[];
+    const RunStage<T, RR::NV> R(run_mem, blockDim.x);
+    constexpr int NVV = RunStage<T, RR::NV>::NG * RunStage<T, RR::NV>::GV;
+    T fx[3] = {(T)0, (T)0, (T)0};
+    T sxx = 0, syy = 0, szz = 0, sxy = 0, syz = 0, sxz = 0, vol = 0;
+    if (act) {   // the updated stress, bulk viscosity and volume (as just stored)
+      external_force(P, p, I, ts.time, fx);
+      const T smq = cs.sm - cs.bqf;
+      sxx = smq + cs.sd[0]; syy = smq + cs.sd[1]; szz = smq + cs.sd[2];
+      sxy = cs.sd[5]; syz = cs.sd[3]; sxz = cs.sd[4];
+      vol = (T)cs.vol;
+    }
+    if (fast) {   // stage the run records (the stage does not alias the tile)
+      int key = -1;
+      if (act) {
+        T v[NVV];
+#pragma unroll
+        for (int q = 0; q < NVV; q++) v[q] = (T)0;
+        run_rec_slots<NS, T, NVV, true>(FS, v);
+        v[RR::S + 0] = -(sxx * vol); v[RR::S + 1] = -(syy * vol); v[RR::S + 2] = -(szz * vol);
+        v[RR::S + 3] = -(sxy * vol); v[RR::S + 4] = -(syz * vol); v[RR::S + 5] = -(sxz * vol);
+        v[RR::S + 6] = fx[0]; v[RR::S + 7] = fx[1]; v[RR::S + 8] = fx[2];
+        R.put(jl, v);
+        key = FS.e0;
+      }
+      R.key[jl] = key;
+      if (threadIdx.x == 0) *R.nh = 0;
+    }
     if (B.ok && B.n) {
       __syncthreads();
       zero_tile<3>(B, ta, CAP);
       __syncthreads();
     }
-    if (act) {
-      T fx[3];
-      external_force(P, p, I, ts.time, fx);
-      const T smq = p.t(T_SM) - p.t(T_Q);
-      const T sxx = smq + p.t(T_SD0), syy = smq + p.t(T_SD1), szz = smq + p.t(T_SD2);
-      const T sxy = p.t(T_SD5), syz = p.t(T_SD3), sxz = p.t(T_SD4);
-      const T vol = (T)p.d(D_VOL);
-      if (fast) {
-        const T sv[3][3] = {{-(sxx * vol), -(sxy * vol), -(sxz * vol)},
-                            {-(sxy * vol), -(syy * vol), -(syz * vol)},
-                            {-(sxz * vol), -(syz * vol), -(szz * vol)}};
-        scatter_force<NS, T>(FS, sb, sc, sv, fx, ta, CAP);
-      } else
+    if (fast) {
+      run_heads(R.key, R.head, R.nh, R.n);
+      __syncthreads();
+      run_scatter_force<NS, T, RR::NV, RR::D, RR::S>(R, B, ta, CAP);
+    } else if (act) {
       for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T dX, T dY, T dZ) {
         T f[3];
         f[0] = -(sxx * dX + sxy * dY + sxz * dZ) * vol + fx[0] * N;
@@ -1559,7 +1913,7 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
   int gran;
   const long i = part_index(P.ilv, glist, bpg, gran);
   const int ii = ginst[gran];
-  const InstC& I = P.inst[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   if (P.ts[ii].done) return;
   constexpr bool MASS = (MODE & P2G_MASS) != 0, MOM = (MODE & P2G_MOM) != 0, FRC = (MODE & P2G_FORCE) != 0;
   // LDS accumulation tile over the block's box of auxiliary nodes (FP64):
@@ -1685,7 +2039,7 @@ __global__ void __launch_bounds__(256) k_sgmp_nodes(PoolPtrs<T> P, const int* ng
   int gran;
   const long j = node_index(nglist, bpg, gran);
   const int ii = nginst[gran];
-  const InstC& I = P.inst[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   if (P.ts[ii].done) return;
   if (j >= (long)I.node_off + I.nnode) return;
   const int k = (int)(j - I.node_off);
@@ -1749,7 +2103,7 @@ __global__ void __launch_bounds__(256) k_sgmp_cells(PoolPtrs<T> P, const int* ng
   int gran;
   const long j = node_index(nglist, bpg, gran);
   const int ii = nginst[gran];
-  const InstC& I = P.inst[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   if (P.ts[ii].done) return;
   const int c = (int)(j - I.node_off);
   if (c >= I.ncell) return;
@@ -1842,8 +2196,8 @@ __global__ void __launch_bounds__(256) k_sgmp_stress(PoolPtrs<T> P, const int* g
   int gran;
   const long i = part_index(0, glist, bpg, gran);
   const int ii = ginst[gran];
-  const InstC& I = P.inst[ii];
-  const InstT& ts = P.ts[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
+  const InstT ts = P.ts[ii];
   if (ts.done) return;
   PV<T> p{P, i};
   const bool live = i < I.p_end;
@@ -1871,7 +2225,10 @@ __global__ void __launch_bounds__(256) k_sgmp_stress(PoolPtrs<T> P, const int* g
 #pragma unroll
         for (int b = 0; b < 3; b++) Lg[a][b] = Lg[a][b] + shp[n] * aux[(AX_DF + 3 * a + b) * nn + c];
     }
-    stress_tail<T, MM>(P, p, I, ts, de, vort, Lg, (I.flags & IF_UVFP) != 0, x, y, z, (MODE & ST_MOVE) != 0);
+    Cst<T> cs;
+    cst_load(p, cs);
+    const MatC& mc = P.mats[P.bodies[p.n(I_BODY)].mat];
+    stress_tail<T, MM>(P, p, I, ts, de, vort, Lg, (I.flags & IF_UVFP) != 0, x, y, z, (MODE & ST_MOVE) != 0, mc, cs);
   }
   if (MODE & ST_ENERGY) {
     double ev[6] = {0, 0, 0, 0, 0, 0};
@@ -1888,8 +2245,8 @@ __global__ void __launch_bounds__(256) k_sgmp_g2p(PoolPtrs<T> P, const int* glis
   int gran;
   const long i = part_index(0, glist, bpg, gran);
   const int ii = ginst[gran];
-  const InstC& I = P.inst[ii];
-  const InstT& ts = P.ts[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
+  const InstT ts = P.ts[ii];
   if (ts.done) return;
   PV<T> p{P, i};
   const bool live = i < I.p_end;
@@ -1960,7 +2317,7 @@ __global__ void __launch_bounds__(256) k_energy(PoolPtrs<T> P, const int* glist,
   int gran;
   const long i = part_index(P.ilv, glist, bpg, gran);
   const int ii = ginst[gran];
-  const InstC& I = P.inst[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   if (P.ts[ii].done) return;
   double ev[6] = {0, 0, 0, 0, 0, 0};
   if (i < I.p_end) particle_energy(PV<T>{P, i}, ev);
@@ -1973,7 +2330,7 @@ __global__ void __launch_bounds__(256) k_energy(PoolPtrs<T> P, const int* glist,
 template <typename T>
 __global__ void __launch_bounds__(256) k_finalize(PoolPtrs<T> P, const int* ilist, int bpg, int what) {
   const int ii = ilist[blockIdx.x];
-  const InstC& I = P.inst[ii];
+  const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   InstT& ts = P.ts[ii];
   if (ts.done) return;
   const long g0 = I.p_begin / PGRAN;
@@ -2125,11 +2482,38 @@ __global__ void __launch_bounds__(256) k_sort_keys(const double* X, long np, con
 
 // ---------------------------------------------------------- dispatch
 
+template <int SHAPE, typename T, int MODE>
+constexpr size_t p2g_stage_bytes(int bs) {
+  using RR = RunRec<NSlot<SHAPE>::n, (MODE & (P2G_MASS | P2G_MOM)) != 0, (MODE & P2G_FORCE) != 0>;
+  return run_stage_bytes<T, RR::NV>(bs);
+}
+
+// LDS of one block (static tile + box reduction + dynamic run stage) is
+// kept within 64 KiB by halving the block size of a scatter launch (the
+// particle blocks of a granule are independent; no per-block partials)
+#define RUN_LDS_MAX 65536
+template <typename F>
+static LaunchP fit_lds(const LaunchP& L0, size_t static_bytes, F&& stage_bytes) {
+  LaunchP L = L0;
+  while (L.bs > 64 && static_bytes + 1024 + stage_bytes(L.bs) > RUN_LDS_MAX) {
+    const int ngran = L.nblk / L.bpg;
+    L.bs /= 2;
+    L.bpg = PGRAN / L.bs;
+    L.nblk = ngran * L.bpg;
+  }
+  return L;
+}
+
 template <int SHAPE, typename T>
-void launch_p2g_s(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
-  dim3 g(L.nblk), b(L.bs);
-#define P2G_CASE(M) \
-  case M: hipLaunchKernelGGL((k_p2g<SHAPE, T, M>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+void launch_p2g_s(int mode, const PoolPtrs<T>& P, const LaunchP& L0, hipStream_t s) {
+#define P2G_CASE(M)                                                                                    \
+  case M: {                                                                                            \
+    constexpr int NT = ((M) & P2G_MOM ? 3 : 0) + ((M) & P2G_FORCE ? 3 : 0);                            \
+    const LaunchP L = fit_lds(L0, (size_t)TILE_NODES * (((M) & P2G_MASS ? 8 : 0) + 8 * NT),            \
+                              [](int bs) { return p2g_stage_bytes<SHAPE, T, M>(bs); });                \
+    const size_t sh = p2g_stage_bytes<SHAPE, T, M>(L.bs);                                              \
+    hipLaunchKernelGGL((k_p2g<SHAPE, T, M>), dim3(L.nblk), dim3(L.bs), sh, s, P, L.glist, L.ginst, L.bpg); \
+  } break;
   switch (mode) {
     P2G_CASE(P2G_ICELL | P2G_MASS | P2G_MOM)
     P2G_CASE(P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE)
@@ -2158,10 +2542,17 @@ void launch_g2p_s(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t 
 // the fused-step modes get one kernel per single material model, the
 // per-phase modes only the general one
 template <int SHAPE, typename T, int MODE>
-void launch_stress_m(int mm, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+void launch_stress_m(int mm, const PoolPtrs<T>& P, const LaunchP& L0, hipStream_t s) {
+  constexpr int NV = RunRec<NSlot<SHAPE>::n, false, true>::NV;
+  // only the force-scatter mode has no per-block energy partials to keep in step with L0
+  const LaunchP L = (MODE == ST_FORCE)
+                        ? fit_lds(L0, (size_t)TILE_NODES * (4 * sizeof(T) > 24 ? 4 * sizeof(T) : 24),
+                                  [](int bs) { return run_stage_bytes<T, NV>(bs); })
+                        : L0;
   dim3 g(L.nblk), b(L.bs);
+  const size_t sh = (MODE & ST_FORCE) ? run_stage_bytes<T, NV>(L.bs) : 0;
 #define ST_M(M) \
-  case M: hipLaunchKernelGGL((k_stress<SHAPE, T, MODE, M>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+  case M: hipLaunchKernelGGL((k_stress<SHAPE, T, MODE, M>), g, b, sh, s, P, L.glist, L.ginst, L.bpg); break;
   switch (mm) {
     ST_M(1 << MPM_MAT_ELASTIC)
     ST_M(1 << MPM_MAT_PLA1)
@@ -2169,7 +2560,7 @@ void launch_stress_m(int mm, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t
     ST_M(1 << MPM_MAT_NULL)
     ST_M(1 << MPM_MAT_DP)
     ST_M(1 << MPM_MAT_NEOH)
-    default: hipLaunchKernelGGL((k_stress<SHAPE, T, MODE, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+    default: hipLaunchKernelGGL((k_stress<SHAPE, T, MODE, MM_ALL>), g, b, sh, s, P, L.glist, L.ginst, L.bpg); break;
   }
 #undef ST_M
 }

@@ -229,6 +229,39 @@ def test_sort_is_the_stable_cell_order(mpm, oracle_mod, name, brick):
     ref.close()
 
 
+@pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
+                                  "dam_null_eos4_musl", "plate_neoh_musl_qulo"])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_small_tiles_take_the_split_and_direct_paths(mpm, name, prec, tol):
+    """A block whose node box does not fit its LDS tile is split in two boxes
+    at the largest jump of its particles' cells, or scatters and gathers
+    directly in global memory.  Small tile capacities send most blocks down
+    these paths (64 nodes: single, split and direct blocks mixed; 24 nodes:
+    every B-spline and GIMP block direct); the results equal the full-tile
+    run up to the summation order of the scatters."""
+    if prec == "fp32" and name == "dam_null_eos4_musl":
+        pytest.skip("FP32 water column: its early near-zero pressure is not reproducible to 1e-5 across "
+                    "summation orders (as in the FP32 parity test)")
+    p = mpm.read_deck(deck_path(name))
+    steps = 6 if name == "dam_null_eos4_musl" else 10
+    ref = mpm.Engine(max_particles=p.n + 256, max_nodes=p.nb_node + 256, max_instances=1, precision=prec)
+    j = ref.add(p)
+    ref.step(steps)
+    want = ref.particles(j)
+    ref.close()
+    loose = {"stress": 1e-9, "ie": 1e-9} if (prec == "fp64" and name == "dam_null_eos4_musl") else {}
+    for cap in (64, 24):
+        eng = mpm.Engine(max_particles=p.n + 256, max_nodes=p.nb_node + 256, max_instances=1, precision=prec)
+        i = eng.add(p)
+        eng.apply_tuning(tile_nodes=cap)
+        assert eng.tuning()["tile_nodes"] == cap
+        eng.step(steps)
+        errs = group_errors(eng.particles(i), want, mpm.PF_NAMES, PGROUPS)
+        for g, e in errs.items():
+            assert e <= loose.get(g, tol), (name, prec, cap, g, e, errs)
+        eng.close()
+
+
 def oracle_cells(p, ps):
     """InWhichCell (Grid.f90:123-155) of every particle, 1-based, -1 outside."""
     x, y, z = ps[0], ps[1], ps[2]
