@@ -428,6 +428,7 @@ struct Box2 {
   Box b[2];
   int split, n, ok, inner;
   __device__ __forceinline__ int side(int j) const { return j >= split; }
+  __device__ __forceinline__ Box get(int k) const { return k ? b[1] : b[0]; }   // no indexed access (registers)
   __device__ __forceinline__ int off(int k) const { return k ? b[0].n : 0; }
   __device__ __forceinline__ int node(const InstC& I, int e) const {
     return e < b[0].n ? b[0].node(I, e) : b[1].node(I, e - b[0].n);
@@ -440,7 +441,8 @@ struct Box2 {
 // jl: the particle's index in the block (particle order); scratch: 4 ints
 // per thread of LDS that is free until the function returns (the tile)
 template <int SHAPE>
-__device__ Box2 block_box2(const InstC& I, bool act, int cx, int cy, int cz, int jl, int cap, int* scratch) {
+__device__ __forceinline__ Box2 block_box2(const InstC& I, bool act, int cx, int cy, int cz, int jl, int cap,
+                                           int* scratch) {
   __shared__ int wbest[16];
   Box2 r;
   r.b[0] = block_box<SHAPE>(I, act, cx, cy, cz, cap);
@@ -798,20 +800,36 @@ __device__ __forceinline__ void run_heads(const int* key, int* head, int* nh, in
   if (h) head[base + __popcll(m & ((1ull << lane) - 1))] = j;
 }
 
+// Every (run, slab) pair is one work item: a thread sums one slab (the NS^2
+// nodes of one stencil layer c) over the particles of one run, reading each
+// particle's record once per slab (the next record is read while the
+// current one is summed), then adds the sums to the tile.
+template <typename T, int NV>
+__device__ __forceinline__ int run_span(const RunStage<T, NV>& R, int j0, int& e0) {
+  e0 = R.key[j0];
+  int j1 = j0 + 1;
+  while (j1 < R.n && R.key[j1] == e0) j1++;
+  return j1;
+}
+
+template <int NS, typename T>
+__device__ __forceinline__ T pick(const T* v, int c) {   // v[c] for a runtime c < NS, kept in registers
+  return NS == 2 ? (c ? v[1] : v[0]) : (c == 0 ? v[0] : (c == 1 ? v[1] : v[2]));
+}
+
 // run sums of the mass (FP64) and NQ velocity-weighted mass components
 template <int NS, typename T, int NV, int OM, bool MASS, int NQ, typename A>
 __device__ __forceinline__ void run_scatter_w(const RunStage<T, NV>& R, const Box2& B, double* tm, A* tv,
                                               int cap) {
-  const int k = threadIdx.x;
-  if (k >= *R.nh) return;
-  const int j0 = R.head[k], e0 = R.key[j0];
-  const Box& Bk = B.b[B.side(j0)];
-  const int sb = Bk.bx, sc = Bk.bx * Bk.by;
-  int j1 = j0 + 1;
-  while (j1 < R.n && R.key[j1] == e0) j1++;
   constexpr int NVV = RunStage<T, NV>::NG * RunStage<T, NV>::GV;
-#pragma unroll
-  for (int c = 0; c < NS; c++) {
+  const int nh = *R.nh;
+  for (int item = threadIdx.x; item < nh * NS; item += blockDim.x) {
+    const int k = item % nh, c = item / nh;
+    const int j0 = R.head[k];
+    int e0;
+    const int j1 = run_span(R, j0, e0);
+    const Box Bk = B.get(B.side(j0));
+    const int sb = Bk.bx, sc = Bk.bx * Bk.by;
     double am[NS][NS];
     T aq[NS][NS][NQ > 0 ? NQ : 1];
 #pragma unroll
@@ -822,19 +840,23 @@ __device__ __forceinline__ void run_scatter_w(const RunStage<T, NV>& R, const Bo
 #pragma unroll
         for (int f = 0; f < NQ; f++) aq[b][a][f] = (T)0;
       }
+    T v[NVV];
+    R.get(j0, v);
     for (int j = j0; j < j1; j++) {
-      T v[NVV];
-      R.get(j, v);
-      const T m = v[OM];
+      T u[NVV];
+#pragma unroll
+      for (int q = 0; q < NVV; q++) u[q] = v[q];
+      if (j + 1 < j1) R.get(j + 1, v);
+      const T m = u[OM], w2 = pick<NS>(u + 2 * NS, c);
 #pragma unroll
       for (int b = 0; b < NS; b++) {
-        const T mw = m * (v[NS + b] * v[2 * NS + c]);
+        const T mw = m * (u[NS + b] * w2);
 #pragma unroll
         for (int a = 0; a < NS; a++) {
-          const T shm = v[a] * mw;
+          const T shm = u[a] * mw;
           if (MASS) am[b][a] += (double)shm;
 #pragma unroll
-          for (int f = 0; f < NQ; f++) aq[b][a][f] += v[OM + 1 + f] * shm;
+          for (int f = 0; f < NQ; f++) aq[b][a][f] += u[OM + 1 + f] * shm;
         }
       }
     }
@@ -853,16 +875,15 @@ __device__ __forceinline__ void run_scatter_w(const RunStage<T, NV>& R, const Bo
 // run sums of the nodal force -(sigma . grad N) vol + fext N
 template <int NS, typename T, int NV, int OD, int OS, typename A>
 __device__ __forceinline__ void run_scatter_force(const RunStage<T, NV>& R, const Box2& B, A* tv, int cap) {
-  const int k = threadIdx.x;
-  if (k >= *R.nh) return;
-  const int j0 = R.head[k], e0 = R.key[j0];
-  const Box& Bk = B.b[B.side(j0)];
-  const int sb = Bk.bx, sc = Bk.bx * Bk.by;
-  int j1 = j0 + 1;
-  while (j1 < R.n && R.key[j1] == e0) j1++;
   constexpr int NVV = RunStage<T, NV>::NG * RunStage<T, NV>::GV;
-#pragma unroll
-  for (int c = 0; c < NS; c++) {
+  const int nh = *R.nh;
+  for (int item = threadIdx.x; item < nh * NS; item += blockDim.x) {
+    const int k = item % nh, c = item / nh;
+    const int j0 = R.head[k];
+    int e0;
+    const int j1 = run_span(R, j0, e0);
+    const Box Bk = B.get(B.side(j0));
+    const int sb = Bk.bx, sc = Bk.bx * Bk.by;
     T af[NS][NS][3];
 #pragma unroll
     for (int b = 0; b < NS; b++)
@@ -870,26 +891,30 @@ __device__ __forceinline__ void run_scatter_force(const RunStage<T, NV>& R, cons
       for (int a = 0; a < NS; a++)
 #pragma unroll
         for (int i = 0; i < 3; i++) af[b][a][i] = (T)0;
+    T v[NVV];
+    R.get(j0, v);
     for (int j = j0; j < j1; j++) {
-      T v[NVV];
-      R.get(j, v);
-      const T sv[3][3] = {{v[OS + 0], v[OS + 3], v[OS + 5]},
-                          {v[OS + 3], v[OS + 1], v[OS + 4]},
-                          {v[OS + 5], v[OS + 4], v[OS + 2]}};
-      const T w2 = v[2 * NS + c], d2 = v[OD + 2 * NS + c];
+      T u[NVV];
+#pragma unroll
+      for (int q = 0; q < NVV; q++) u[q] = v[q];
+      if (j + 1 < j1) R.get(j + 1, v);
+      const T sv[3][3] = {{u[OS + 0], u[OS + 3], u[OS + 5]},
+                          {u[OS + 3], u[OS + 1], u[OS + 4]},
+                          {u[OS + 5], u[OS + 4], u[OS + 2]}};
+      const T w2 = pick<NS>(u + 2 * NS, c), d2 = pick<NS>(u + OD + 2 * NS, c);
 #pragma unroll
       for (int b = 0; b < NS; b++) {
-        const T wyz = v[NS + b] * w2, dyz = v[OD + NS + b] * w2, wdz = v[NS + b] * d2;
+        const T wyz = u[NS + b] * w2, dyz = u[OD + NS + b] * w2, wdz = u[NS + b] * d2;
         T Af[3], Q[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
           Af[i] = sv[i][0] * wyz;
-          Q[i] = sv[i][1] * dyz + sv[i][2] * wdz + v[OS + 6 + i] * wyz;
+          Q[i] = sv[i][1] * dyz + sv[i][2] * wdz + u[OS + 6 + i] * wyz;
         }
 #pragma unroll
         for (int a = 0; a < NS; a++)
 #pragma unroll
-          for (int i = 0; i < 3; i++) af[b][a][i] += v[OD + a] * Af[i] + v[a] * Q[i];
+          for (int i = 0; i < 3; i++) af[b][a][i] += u[OD + a] * Af[i] + u[a] * Q[i];
       }
     }
 #pragma unroll
@@ -980,7 +1005,7 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
     if (act) {
       Slots<T> S;
       const int kb = B.side(jl);
-      make_slots<SHAPE, T>(I, B.b[kb], x, y, z, c, S);
+      make_slots<SHAPE, T>(I, B.get(kb), x, y, z, c, S);
       S.e0 += B.off(kb);
       T v[NVV];
 #pragma unroll
@@ -1007,7 +1032,7 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
   } else if (act) {
     Stencil<T> S;
     build_stencil<SHAPE, T>(I, x, y, z, c, S);
-    for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T dX, T dY, T dZ) {
+    for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T dX, T dY, T dZ) __attribute__((always_inline)) {
       const int e = B.ok ? B.idx(nx, ny, nz) : -1;
       T val[6];
       int q = 0;
@@ -1211,7 +1236,8 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
   const Box2 B = block_box2<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem);
   const bool fast = B.ok && B.inner;
   const int kb = B.side(jl);
-  const int sb = B.b[kb].bx, sc = B.b[kb].bx * B.b[kb].by;
+  const Box Bk = B.get(kb);
+  const int sb = Bk.bx, sc = Bk.bx * Bk.by;
   const long nn = P.nn;
   const T* Vg = P.vg + I.node_off;
   const T* Ag = P.ag + I.node_off;
@@ -1228,7 +1254,7 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
   if (act) {
     T vx[3] = {(T)0, (T)0, (T)0}, ax[3] = {(T)0, (T)0, (T)0};
     if (fast) {
-      make_slots<SHAPE, T>(I, B.b[kb], x, y, z, c, F);
+      make_slots<SHAPE, T>(I, Bk, x, y, z, c, F);
       F.e0 += B.off(kb);
       T va[6];
       gather_w<NS, T, 6>(F, tf, sb, sc, va);
@@ -1239,7 +1265,7 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
       }
     } else {
       build_stencil<SHAPE, T>(I, x, y, z, c, S);
-      for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) {
+      for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) __attribute__((always_inline)) {
         const int e = B.ok ? B.idx(nx, ny, nz) : -1;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -1277,7 +1303,7 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
       if (fast) {
         scatter_w<NS, T, false, 3>(F, sb, sc, m, vn, nullptr, ta, CAP);
       } else {
-        for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) {
+        for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) __attribute__((always_inline)) {
           const T shm = N * m;
           const int e = B.ok ? B.idx(nx, ny, nz) : -1;
           if (e >= 0) {
@@ -1702,7 +1728,8 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
   const Box2 B = block_box2<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem);
   const bool fast = B.ok && B.inner;
   const int kb = B.side(jl);
-  const int sb = B.b[kb].bx, sc = B.b[kb].bx * B.b[kb].by;
+  const Box Bk = B.get(kb);
+  const int sb = Bk.bx, sc = Bk.bx * Bk.by;
   const long nn = P.nn;
   const T* Vg = P.vg + I.node_off;
   if (B.ok && B.n) {
@@ -1719,7 +1746,7 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
     T L[3][3] = {{(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}};
     const bool uvfp = (I.flags & IF_UVFP) != 0;
     if (fast) {
-      make_slots<SHAPE, T>(I, B.b[kb], x, y, z, c, FS);
+      make_slots<SHAPE, T>(I, Bk, x, y, z, c, FS);
       FS.e0 += B.off(kb);
       gather_grad<NS, T>(FS, tf, sb, sc, L);
       de[0] = L[0][0];
@@ -1733,7 +1760,7 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
       vort[2] = L[1][0] - L[0][1];
     } else {
     build_stencil<SHAPE, T>(I, x, y, z, c, S);
-    for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T, T dX, T dY, T dZ) {
+    for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T, T dX, T dY, T dZ) __attribute__((always_inline)) {
       const int e = B.ok ? B.idx(nx, ny, nz) : -1;
       const T v0 = e >= 0 ? tf[e] : Vg[k];
       const T v1 = e >= 0 ? tf[CAP + e] : Vg[nn + k];
@@ -1798,7 +1825,7 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
       __syncthreads();
       run_scatter_force<NS, T, RR::NV, RR::D, RR::S>(R, B, ta, CAP);
     } else if (act) {
-      for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T dX, T dY, T dZ) {
+      for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T dX, T dY, T dZ) __attribute__((always_inline)) {
         T f[3];
         f[0] = -(sxx * dX + sxy * dY + sxz * dZ) * vol + fx[0] * N;
         f[1] = -(sxy * dX + syy * dY + syz * dZ) * vol + fx[1] * N;
