@@ -90,10 +90,26 @@ template <typename T>
 struct Stencil {
   T w[3][3];   // [axis][slot] weights
   T d[3][3];   // [axis][slot] derivative factors
+  T r[3];      // B-spline: shifted local coordinate per axis (w and d follow from it)
   int base[3]; // node coordinate of slot 0 per axis
   int act[3];  // active slots per axis (GIMP: 2 or 3)
   int node0;   // local linear index of the cell's first node
 };
+
+// quadratic B-spline weights and derivative factors of one axis from the
+// shifted local coordinate r = (Xp - Xg)/DCell*0.2 + 0.4 (NShape_Bspline,
+// Grid.f90:927-962, FP32 literals 0.2 and 0.4 as there)
+template <typename T>
+__device__ __forceinline__ void bspline_axis(T r, T idc, T* w, T* d) {
+  const T f02 = (T)(double)0.2f;
+  const T a = (T)3 - (T)5 * r, b = (T)5 * r - (T)2;
+  w[0] = (T)0.5 * (a * a);
+  w[1] = -((T)25 * (r * r)) + (T)25 * r - (T)5.5;
+  w[2] = (T)0.5 * (b * b);
+  d[0] = ((T)5 * r - (T)3) * (T)5 * f02 * idc;
+  d[1] = (-((T)50 * r) + (T)25) * f02 * idc;
+  d[2] = ((T)5 * r - (T)2) * (T)5 * f02 * idc;
+}
 
 // per-axis weights.  c: 0-based cell coordinate; s0: span start.
 template <int SHAPE, typename T>
@@ -144,14 +160,8 @@ __device__ __forceinline__ void axis_weights(const InstC& I, int ax, double x, i
   } else {  // B-spline: BFindInflNode + NShape_Bspline (Grid.f90:852-1049)
     const double c02 = (double)0.2f, c04 = (double)0.4f;   // FP32 literals
     const T r = (T)((x - xg1) * I.idcell * c02 + c04);
-    const T idc = (T)I.idcell, f02 = (T)c02;
-    const T a = (T)3 - (T)5 * r, b = (T)5 * r - (T)2;
-    S.w[ax][0] = (T)0.5 * (a * a);
-    S.w[ax][1] = -((T)25 * (r * r)) + (T)25 * r - (T)5.5;
-    S.w[ax][2] = (T)0.5 * (b * b);
-    S.d[ax][0] = ((T)5 * r - (T)3) * (T)5 * f02 * idc;
-    S.d[ax][1] = (-((T)50 * r) + (T)25) * f02 * idc;
-    S.d[ax][2] = ((T)5 * r - (T)2) * (T)5 * f02 * idc;
+    bspline_axis<T>(r, (T)I.idcell, S.w[ax], S.d[ax]);
+    S.r[ax] = r;
     S.base[ax] = c - 1;
     S.act[ax] = 3;
   }
@@ -508,6 +518,7 @@ template <typename T>
 struct Slots {
   T w[3][3];
   T d[3][3];
+  T r[3];      // B-spline only (see Stencil)
   int e0;
 };
 
@@ -527,6 +538,9 @@ __device__ __forceinline__ void make_slots(const InstC& I, const Box& B, double 
       S.d[ax][k] = st.d[ax][k] * ds;
     }
   S.e0 = (st.base[0] - B.x0) + B.bx * ((st.base[1] - B.y0) + B.by * (st.base[2] - B.z0));
+  if (SHAPE == MPM_SHAPE_BSPLINE)
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) S.r[ax] = st.r[ax];
 }
 
 // gather tiles in the fast path: NF fields per node, node-major with a
@@ -724,17 +738,41 @@ struct V16<double> {
   __device__ static void split(const type& u, double* v) { v[0] = u.x; v[1] = u.y; }
 };
 
-// record of one staged particle: the per-axis weights w[3][NS], their
-// derivative factors d[3][NS] (force scatters), mass and velocity (mass /
+// record of one staged particle: the per-axis weights w[3][NS] and their
+// derivative factors d[3][NS] (force scatters) -- or, compact (B-spline),
+// the three local coordinates they follow from --, mass and velocity (mass /
 // momentum scatters), -vol*sigma (xx yy zz xy yz xz) and the external force
-template <int NS, bool MQ, bool FRC>
+template <int NS, bool MQ, bool FRC, bool CR = false>
 struct RunRec {
   static constexpr int W = 0;
-  static constexpr int D = 3 * NS;                       // valid if FRC
-  static constexpr int M = D + (FRC ? 3 * NS : 0);       // m, q0, q1, q2 if MQ
-  static constexpr int S = M + (MQ ? 4 : 0);             // sv[6], fx[3] if FRC
+  static constexpr int D = CR ? 0 : 3 * NS;                      // valid if FRC && !CR
+  static constexpr int M = CR ? 3 : D + (FRC ? 3 * NS : 0);      // m, q0, q1, q2 if MQ
+  static constexpr int S = M + (MQ ? 4 : 0);                     // sv[6], fx[3] if FRC
   static constexpr int NV = S + (FRC ? 9 : 0);
 };
+
+// weights (and derivative factors) of a staged record
+template <int NS, typename T, bool CR, bool FRC, int OD>
+__device__ __forceinline__ void run_rec_weights(const T* u, T idc, T (&w)[3][NS], T (&d)[3][NS]) {
+#pragma unroll
+  for (int ax = 0; ax < 3; ax++) {
+    if (CR) {
+      T ww[3], dd[3];
+      bspline_axis<T>(u[ax], idc, ww, dd);
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        w[ax][a] = ww[a];
+        d[ax][a] = dd[a];
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < NS; a++) {
+        w[ax][a] = u[ax * NS + a];
+        d[ax][a] = FRC ? u[OD + ax * NS + a] : (T)0;
+      }
+    }
+  }
+}
 
 __host__ __device__ constexpr int run_slots(int bs) { return bs + bs / 8; }
 
@@ -776,8 +814,13 @@ struct RunStage {
 };
 
 // fill a record's weight part from the particle's slots
-template <int NS, typename T, int NVV, bool FRC>
+template <int NS, typename T, int NVV, bool FRC, bool CR>
 __device__ __forceinline__ void run_rec_slots(const Slots<T>& S, T (&v)[NVV]) {
+  if (CR) {
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) v[ax] = S.r[ax];
+    return;
+  }
 #pragma unroll
   for (int ax = 0; ax < 3; ax++)
 #pragma unroll
@@ -818,8 +861,8 @@ __device__ __forceinline__ T pick(const T* v, int c) {   // v[c] for a runtime c
 }
 
 // run sums of the mass (FP64) and NQ velocity-weighted mass components
-template <int NS, typename T, int NV, int OM, bool MASS, int NQ, typename A>
-__device__ __forceinline__ void run_scatter_w(const RunStage<T, NV>& R, const Box2& B, double* tm, A* tv,
+template <int NS, typename T, int NV, int OM, bool MASS, int NQ, bool CR, typename A>
+__device__ __forceinline__ void run_scatter_w(const RunStage<T, NV>& R, const Box2& B, T idc, double* tm, A* tv,
                                               int cap) {
   constexpr int NVV = RunStage<T, NV>::NG * RunStage<T, NV>::GV;
   const int nh = *R.nh;
@@ -847,13 +890,15 @@ __device__ __forceinline__ void run_scatter_w(const RunStage<T, NV>& R, const Bo
 #pragma unroll
       for (int q = 0; q < NVV; q++) u[q] = v[q];
       if (j + 1 < j1) R.get(j + 1, v);
-      const T m = u[OM], w2 = pick<NS>(u + 2 * NS, c);
+      T w[3][NS], d[3][NS];
+      run_rec_weights<NS, T, CR, false, 0>(u, idc, w, d);
+      const T m = u[OM], w2 = pick<NS>(w[2], c);
 #pragma unroll
       for (int b = 0; b < NS; b++) {
-        const T mw = m * (u[NS + b] * w2);
+        const T mw = m * (w[1][b] * w2);
 #pragma unroll
         for (int a = 0; a < NS; a++) {
-          const T shm = u[a] * mw;
+          const T shm = w[0][a] * mw;
           if (MASS) am[b][a] += (double)shm;
 #pragma unroll
           for (int f = 0; f < NQ; f++) aq[b][a][f] += u[OM + 1 + f] * shm;
@@ -873,8 +918,8 @@ __device__ __forceinline__ void run_scatter_w(const RunStage<T, NV>& R, const Bo
 }
 
 // run sums of the nodal force -(sigma . grad N) vol + fext N
-template <int NS, typename T, int NV, int OD, int OS, typename A>
-__device__ __forceinline__ void run_scatter_force(const RunStage<T, NV>& R, const Box2& B, A* tv, int cap) {
+template <int NS, typename T, int NV, int OD, int OS, bool CR, typename A>
+__device__ __forceinline__ void run_scatter_force(const RunStage<T, NV>& R, const Box2& B, T idc, A* tv, int cap) {
   constexpr int NVV = RunStage<T, NV>::NG * RunStage<T, NV>::GV;
   const int nh = *R.nh;
   for (int item = threadIdx.x; item < nh * NS; item += blockDim.x) {
@@ -901,10 +946,12 @@ __device__ __forceinline__ void run_scatter_force(const RunStage<T, NV>& R, cons
       const T sv[3][3] = {{u[OS + 0], u[OS + 3], u[OS + 5]},
                           {u[OS + 3], u[OS + 1], u[OS + 4]},
                           {u[OS + 5], u[OS + 4], u[OS + 2]}};
-      const T w2 = pick<NS>(u + 2 * NS, c), d2 = pick<NS>(u + OD + 2 * NS, c);
+      T w[3][NS], d[3][NS];
+      run_rec_weights<NS, T, CR, true, OD>(u, idc, w, d);
+      const T w2 = pick<NS>(w[2], c), d2 = pick<NS>(d[2], c);
 #pragma unroll
       for (int b = 0; b < NS; b++) {
-        const T wyz = u[NS + b] * w2, dyz = u[OD + NS + b] * w2, wdz = u[NS + b] * d2;
+        const T wyz = w[1][b] * w2, dyz = d[1][b] * w2, wdz = w[1][b] * d2;
         T Af[3], Q[3];
 #pragma unroll
         for (int i = 0; i < 3; i++) {
@@ -914,7 +961,7 @@ __device__ __forceinline__ void run_scatter_force(const RunStage<T, NV>& R, cons
 #pragma unroll
         for (int a = 0; a < NS; a++)
 #pragma unroll
-          for (int i = 0; i < 3; i++) af[b][a][i] += u[OD + a] * Af[i] + u[a] * Q[i];
+          for (int i = 0; i < 3; i++) af[b][a][i] += d[0][a] * Af[i] + w[0][a] * Q[i];
       }
     }
 #pragma unroll
@@ -997,7 +1044,8 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
   T* Fg = P.fg + I.node_off;
   if (B.ok && B.inner) {   // block-uniform: staged run sums
     constexpr int NS = NSlot<SHAPE>::n;
-    using RR = RunRec<NS, MASS || MOM, FRC>;
+    constexpr bool CR = SHAPE == MPM_SHAPE_BSPLINE;
+    using RR = RunRec<NS, MASS || MOM, FRC, CR>;
     extern __shared__ __align__(16) char run_mem[];
     const RunStage<T, RR::NV> R(run_mem, blockDim.x);
     constexpr int NVV = RunStage<T, RR::NV>::NG * RunStage<T, RR::NV>::GV;
@@ -1010,7 +1058,7 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
       T v[NVV];
 #pragma unroll
       for (int q = 0; q < NVV; q++) v[q] = (T)0;
-      run_rec_slots<NS, T, NVV, FRC>(S, v);
+      run_rec_slots<NS, T, NVV, FRC, CR>(S, v);
       if (MASS || MOM) {
         v[RR::M] = m; v[RR::M + 1] = vx; v[RR::M + 2] = vy; v[RR::M + 3] = vz;
       }
@@ -1027,8 +1075,9 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
     __syncthreads();
     run_heads(R.key, R.head, R.nh, R.n);
     __syncthreads();
-    if (MASS || MOM) run_scatter_w<NS, T, RR::NV, RR::M, MASS, MOM ? 3 : 0>(R, B, tm, tf, CAP);
-    if (FRC) run_scatter_force<NS, T, RR::NV, RR::D, RR::S>(R, B, tf + (MOM ? 3 : 0) * CAP, CAP);
+    const T idc = (T)I.idcell;
+    if (MASS || MOM) run_scatter_w<NS, T, RR::NV, RR::M, MASS, MOM ? 3 : 0, CR>(R, B, idc, tm, tf, CAP);
+    if (FRC) run_scatter_force<NS, T, RR::NV, RR::D, RR::S, CR>(R, B, idc, tf + (MOM ? 3 : 0) * CAP, CAP);
   } else if (act) {
     Stencil<T> S;
     build_stencil<SHAPE, T>(I, x, y, z, c, S);
@@ -1786,7 +1835,8 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
   }
   if (MODE & ST_FORCE) {   // USF: GridMomentumUpdate scatter right after the update
     T* Fg = P.fg + I.node_off;
-    using RR = RunRec<NS, false, true>;
+    constexpr bool CR = SHAPE == MPM_SHAPE_BSPLINE;
+    using RR = RunRec<NS, false, true, CR>;
     extern __shared__ __align__(16) char run_mem[];
     const RunStage<T, RR::NV> R(run_mem, blockDim.x);
     constexpr int NVV = RunStage<T, RR::NV>::NG * RunStage<T, RR::NV>::GV;
@@ -1805,7 +1855,7 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
         T v[NVV];
 #pragma unroll
         for (int q = 0; q < NVV; q++) v[q] = (T)0;
-        run_rec_slots<NS, T, NVV, true>(FS, v);
+        run_rec_slots<NS, T, NVV, true, CR>(FS, v);
         v[RR::S + 0] = -(sxx * vol); v[RR::S + 1] = -(syy * vol); v[RR::S + 2] = -(szz * vol);
         v[RR::S + 3] = -(sxy * vol); v[RR::S + 4] = -(syz * vol); v[RR::S + 5] = -(sxz * vol);
         v[RR::S + 6] = fx[0]; v[RR::S + 7] = fx[1]; v[RR::S + 8] = fx[2];
@@ -1823,7 +1873,7 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
     if (fast) {
       run_heads(R.key, R.head, R.nh, R.n);
       __syncthreads();
-      run_scatter_force<NS, T, RR::NV, RR::D, RR::S>(R, B, ta, CAP);
+      run_scatter_force<NS, T, RR::NV, RR::D, RR::S, CR>(R, B, (T)I.idcell, ta, CAP);
     } else if (act) {
       for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T dX, T dY, T dZ) __attribute__((always_inline)) {
         T f[3];
@@ -2511,7 +2561,8 @@ __global__ void __launch_bounds__(256) k_sort_keys(const double* X, long np, con
 
 template <int SHAPE, typename T, int MODE>
 constexpr size_t p2g_stage_bytes(int bs) {
-  using RR = RunRec<NSlot<SHAPE>::n, (MODE & (P2G_MASS | P2G_MOM)) != 0, (MODE & P2G_FORCE) != 0>;
+  using RR = RunRec<NSlot<SHAPE>::n, (MODE & (P2G_MASS | P2G_MOM)) != 0, (MODE & P2G_FORCE) != 0,
+                   SHAPE == MPM_SHAPE_BSPLINE>;
   return run_stage_bytes<T, RR::NV>(bs);
 }
 
@@ -2570,7 +2621,7 @@ void launch_g2p_s(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t 
 // per-phase modes only the general one
 template <int SHAPE, typename T, int MODE>
 void launch_stress_m(int mm, const PoolPtrs<T>& P, const LaunchP& L0, hipStream_t s) {
-  constexpr int NV = RunRec<NSlot<SHAPE>::n, false, true>::NV;
+  constexpr int NV = RunRec<NSlot<SHAPE>::n, false, true, SHAPE == MPM_SHAPE_BSPLINE>::NV;
   // only the force-scatter mode has no per-block energy partials to keep in step with L0
   const LaunchP L = (MODE == ST_FORCE)
                         ? fit_lds(L0, (size_t)TILE_NODES * (4 * sizeof(T) > 24 ? 4 * sizeof(T) : 24),
