@@ -374,7 +374,7 @@ def run_serving(mpm, decks, a, log):
         % (len(pool), sum(k == "slope" for _, _, k in pool), sum(k == "bar" for _, _, k in pool),
            min(p.n for p, _, _ in pool), max(p.n for p, _, _ in pool), time.perf_counter() - t0))
     eng = mpm.Engine(hbm_bytes=a.serve_budget, max_instances=512, precision="fp32")
-    srv = serve.Server(eng, chunk=8, graph=True)
+    srv = serve.Server(eng, chunk=8, graph="lazy")
     # warm-up: one request through the whole path (capture, admission, retire)
     srv.submit(serve.Request(-1, pool[0][0], 16, arrival=0.0, kind="warm"))
     srv.admit(0.0)

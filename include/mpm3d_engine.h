@@ -44,8 +44,11 @@ int mpm_engine_set_step_limit(mpm_engine* e, int id, int max_steps);
 /* largest free particle / node ranges and free instance slots */
 int mpm_engine_capacity(mpm_engine* e, long* free_particles, long* free_nodes, int* free_instances);
 
-/* advance every live, unfinished instance by nsteps; with use_graph the
- * step sequence of graph_steps steps is captured once and replayed */
+/* advance every live, unfinished instance by nsteps; with use_graph = 1 the
+ * step sequence of graph_steps steps is captured once and replayed (captured
+ * again after instances join or leave); use_graph = 2 (lazy) launches
+ * plainly on the first call after such a change and captures only when the
+ * next call finds the same instances (continuous batching) */
 int mpm_engine_step(mpm_engine* e, int nsteps, int use_graph, int graph_steps);
 int mpm_engine_sync(mpm_engine* e);
 /* stable re-sort of every instance's particles by cell */

@@ -254,8 +254,11 @@ class Engine:
         return fp.value, fn.value, fi.value
 
     def step(self, n=1, graph=False, graph_steps=1):
-        check(_lib.lib().mpm_engine_step(self._h, int(n), int(bool(graph)), int(graph_steps)),
-              "mpm_engine_step")
+        """graph: False (plain launches), True (capture once, replay) or
+        "lazy" (capture only when the resident set did not change since the
+        previous call)."""
+        g = 2 if graph == "lazy" else int(bool(graph))
+        check(_lib.lib().mpm_engine_step(self._h, int(n), g, int(graph_steps)), "mpm_engine_step")
 
     def sync(self):
         check(_lib.lib().mpm_engine_sync(self._h), "mpm_engine_sync")

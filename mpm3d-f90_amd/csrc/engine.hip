@@ -5,6 +5,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -170,6 +171,21 @@ struct mpm_engine {
   int* d_ginst = nullptr;         // instance of every particle granule
   int* d_nginst = nullptr;        // instance of every node granule
   std::vector<int> h_ginst, h_nginst;
+  int* d_seq = nullptr;           // 0, 1, 2, ...: granule range and instance lists without uploads
+  // host-to-device staging: a ring of pinned buffers; every upload is a
+  // copy ordered on the engine stream (so admissions and list changes never
+  // wait for queued steps), and a buffer is reused only after its copies ran
+  struct Pinned {
+    char* h = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+  };
+  std::vector<Pinned> ring;       // grows (up to 32) rather than wait for a busy buffer
+  int ring_next = 0;
+  char* d_up = nullptr;           // device staging of the raw particle rows of an admission
+  size_t d_up_cap = 0;
+  std::vector<std::array<int*, 3>> spare_lists;   // launch-list buffers of vanished groups (reused)
   // launch lists
   std::vector<Group> groups;
   bool lists_dirty = true;
@@ -200,6 +216,7 @@ struct mpm_engine {
   hipGraph_t graph = nullptr;
   int graph_steps = 0;
   bool graph_dirty = true;
+  bool graph_plain_once = false;  // lazy replay: the changed lists already ran once without a graph
   // profiling mode: HIP events around every launch of mpm_step / engine steps
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -251,6 +268,16 @@ static void free_engine(mpm_engine* e) {
     if (g.d_ngran) (void)hipFree(g.d_ngran);
     if (g.d_ilist) (void)hipFree(g.d_ilist);
   }
+  for (auto& l : e->spare_lists)
+    for (int* b : l)
+      if (b) (void)hipFree(b);
+  if (e->s) (void)hipStreamSynchronize(e->s);
+  for (auto& r : e->ring) {
+    if (r.h) (void)hipHostFree(r.h);
+    if (r.ev) (void)hipEventDestroy(r.ev);
+  }
+  if (e->d_up) (void)hipFree(e->d_up);
+  if (e->d_seq) (void)hipFree(e->d_seq);
   if (e->s) (void)hipStreamDestroy(e->s);
   delete e;
 }
@@ -299,6 +326,13 @@ static mpm_engine* new_engine(long np_slots, long nn_slots, int max_inst, int ma
   e->d_nginst = dalloc<int>(e->nn / NGRAN);
   e->h_ginst.assign(e->np / PGRAN, 0);
   e->h_nginst.assign(e->nn / NGRAN, 0);
+  {
+    const long nseq = std::max({e->np / PGRAN, e->nn / NGRAN, (long)max_inst});
+    std::vector<int> seq(nseq);
+    for (long k = 0; k < nseq; k++) seq[k] = (int)k;
+    e->d_seq = dalloc<int>(nseq);
+    HIPCHK(hipMemcpy(e->d_seq, seq.data(), nseq * sizeof(int), hipMemcpyHostToDevice));
+  }
   e->inst.resize(max_inst);
   for (int k = max_inst - 1; k >= 0; k--) e->free_ids.push_back(k);
   HIPCHK(hipMemsetAsync(e->t, 0, (size_t)T_COUNT * e->np * e->tsz, e->s));
@@ -371,8 +405,60 @@ static void check_problem(const mpm_problem* p) {
   if (nnode > 0x7fffffffL || ncell > 0x7fffffffL) fail(MPM_ERR_ARG, "grid too large");
 }
 
+// A pinned staging buffer of at least `bytes` that no queued copy still
+// reads (waits for the copies of the ring slot's previous use).
+static char* stage_get(mpm_engine* e, size_t bytes, int& slot) {
+  const int nr = (int)e->ring.size();
+  slot = -1;
+  for (int k = 0; k < nr && slot < 0; k++) {   // a buffer whose copies have run
+    const int q = (e->ring_next + k) % nr;
+    mpm_engine::Pinned& r = e->ring[q];
+    if (!r.pending) {
+      slot = q;
+    } else {
+      const hipError_t st = hipEventQuery(r.ev);
+      if (st == hipSuccess) {
+        r.pending = false;
+        slot = q;
+      } else if (st != hipErrorNotReady) {
+        HIPCHK(st);
+      }
+    }
+  }
+  if (slot < 0 && nr < 32) {
+    e->ring.emplace_back();
+    slot = nr;
+  }
+  if (slot < 0) {   // all 32 busy: wait for the next in turn
+    slot = e->ring_next % nr;
+    HIPCHK(hipEventSynchronize(e->ring[slot].ev));
+    e->ring[slot].pending = false;
+  }
+  e->ring_next = (slot + 1) % (int)e->ring.size();
+  mpm_engine::Pinned& r = e->ring[slot];
+  if (!r.ev) HIPCHK(hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
+  if (r.cap < bytes) {
+    if (r.h) HIPCHK(hipHostFree(r.h));
+    r.h = nullptr;
+    r.cap = 0;
+    const size_t cap = std::max<size_t>(bytes, (size_t)1 << 20);
+    HIPCHK(hipHostMalloc((void**)&r.h, cap, hipHostMallocDefault));
+    r.cap = cap;
+  }
+  return r.h;
+}
+
+// the copies out of a ring slot are enqueued: mark it busy until they ran
+static void stage_done(mpm_engine* e, int slot) {
+  HIPCHK(hipEventRecord(e->ring[slot].ev, e->s));
+  e->ring[slot].pending = true;
+}
+
+static size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+
 template <typename T>
 static void upload_particles_T(mpm_engine* e, const HostInst& h, const double* ps, int n, bool first) {
+  HIPCHK(hipStreamSynchronize(e->s));   // the copies below are not ordered on the engine stream
   const long np = e->np, pad = h.npg * PGRAN, b0 = h.pg0 * PGRAN;
   std::vector<T> tb((size_t)T_COUNT * pad, (T)0);
   std::vector<double> db((size_t)D_COUNT * pad, 0.0);
@@ -432,11 +518,17 @@ static void upload_particles(mpm_engine* e, const HostInst& h, const double* ps,
 }
 
 // add one instance to the pool; returns instance id
+// the launch lists or launch shape changed: the captured step graph is stale
+static void mark_graph_dirty(mpm_engine* e) {
+  e->graph_dirty = true;
+  e->graph_plain_once = false;
+}
+
 static void ensure_aux(mpm_engine* e) {
   if (e->aux) return;
   e->aux = dalloc<char>((size_t)AUX_COUNT * e->nn * e->tsz);
   e->cmg = dalloc<double>(e->nn);
-  e->graph_dirty = true;
+  mark_graph_dirty(e);
 }
 
 static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
@@ -520,84 +612,78 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   c.bq2 = p->bq2;
   c.dtscale = p->dtscale;
   c.endtime = p->endtime;
-  HIPCHK(hipMemcpy(e->d_inst + id, &c, sizeof(InstC), hipMemcpyHostToDevice));
   // materials and bodies
   std::vector<MatC> mats(p->nb_mat);
   for (int m = 0; m < p->nb_mat; m++) mats[m] = make_mat(p->mats[m]);
-  HIPCHK(hipMemcpy(e->d_mats + m0, mats.data(), sizeof(MatC) * p->nb_mat, hipMemcpyHostToDevice));
   std::vector<BodyC> bodies(p->nb_body);
   for (int b = 0; b < p->nb_body; b++) {
     bodies[b].mat = (int)m0 + p->bodies[b].mat - 1;
     for (int k = 0; k < 3; k++) bodies[b].grav[k] = p->bodies[b].grav[k];
   }
-  HIPCHK(hipMemcpy(e->d_bodies + b0, bodies.data(), sizeof(BodyC) * p->nb_body, hipMemcpyHostToDevice));
-  // time state: calcEnergy of the initial state is done on the device below
+  // time state: calcEnergy of the initial state is done on the device later
   InstT ts{};
   ts.dt = p->dt;
-  HIPCHK(hipMemcpy(e->d_ts + id, &ts, sizeof(InstT), hipMemcpyHostToDevice));
-  // particle arrays
-  upload_particles(e, h, p->pstate, n, true);
-  {
-    const long pad = npg * PGRAN, b0p = pg0 * PGRAN;
-    std::vector<int> ib((size_t)I_COUNT * pad, 0);
-    for (long j = 0; j < pad; j++) {
-      ib[(size_t)I_BODY * pad + j] = (j < n) ? (int)b0 + p->pbody[j] : (int)b0;
-      ib[(size_t)I_ORIG * pad + j] = (int)j;
-      ib[(size_t)I_CELL * pad + j] = -1;
-    }
-    for (int f = 0; f < I_COUNT; f++)
-      HIPCHK(hipMemcpy(e->ii + (size_t)f * e->np + b0p, &ib[(size_t)f * pad], pad * sizeof(int),
-                       hipMemcpyHostToDevice));
-  }
-  // node fixities (Grid.f90:291-359)
-  {
-    std::vector<uint8_t> fx(nng * NGRAN, 0);
-    const int* FixS = p->fix;
-    const bool wide = p->shape == MPM_SHAPE_GIMP || p->shape == MPM_SHAPE_BSPLINE || p->shape == MPM_SHAPE_SGMP;
-    const int NGx = c.ngx, NGy = c.ngy, NGz = c.ngz;
-    for (int iz = 1; iz <= NGz; iz++)
-      for (int iy = 1; iy <= NGy; iy++)
-        for (int ix = 1; ix <= NGx; ix++) {
-          int k = (iz - 1) * c.ngxy + (iy - 1) * NGx + ix - 1;
-          int bx = 0, by = 0, bz = 0;
-          if (wide) {
-            if (((ix == 1 || ix == 2) && FixS[0] == 1) || ((ix == NGx || ix == NGx - 1) && FixS[1] == 1) ||
-                ((iy == 1 || iy == 2) && FixS[2] == 1) || ((iy == NGy || iy == NGy - 1) && FixS[3] == 1) ||
-                ((iz == 1 || iz == 2) && FixS[4] == 1) || ((iz == NGz || iz == NGz - 1) && FixS[5] == 1))
-              bx = by = bz = 1;
-            if (((ix == 1 || ix == 2) && FixS[0] == 2) || ((ix == NGx || ix == NGx - 1) && FixS[1] == 2)) bx = 1;
-            if (((iy == 1 || iy == 2) && FixS[2] == 2) || ((iy == NGy || iy == NGy - 1) && FixS[3] == 2)) by = 1;
-            if (((iz == 1 || iz == 2) && FixS[4] == 2) || ((iz == NGz || iz == NGz - 1) && FixS[5] == 2)) bz = 1;
-          }
-          if ((ix == 1 && FixS[0] == 1) || (ix == NGx && FixS[1] == 1) || (iy == 1 && FixS[2] == 1) ||
-              (iy == NGy && FixS[3] == 1) || (iz == 1 && FixS[4] == 1) || (iz == NGz && FixS[5] == 1))
-            bx = by = bz = 1;
-          if ((ix == 1 && FixS[0] == 2) || (ix == NGx && FixS[1] == 2)) bx = 1;
-          if ((iy == 1 && FixS[2] == 2) || (iy == NGy && FixS[3] == 2)) by = 1;
-          if ((iz == 1 && FixS[4] == 2) || (iz == NGz && FixS[5] == 2)) bz = 1;
-          // SGMP boundary nodes: bd_type /= 0 (Grid.f90:303-316), first two layers of a face with FixS set
-          int qx = 0, qy = 0, qz = 0;
-          if (p->shape == MPM_SHAPE_SGMP) {
-            qx = ((ix == 1 || ix == 2) && FixS[0] != 0) || ((ix == NGx || ix == NGx - 1) && FixS[1] != 0);
-            qy = ((iy == 1 || iy == 2) && FixS[2] != 0) || ((iy == NGy || iy == NGy - 1) && FixS[3] != 0);
-            qz = ((iz == 1 || iz == 2) && FixS[4] != 0) || ((iz == NGz || iz == NGz - 1) && FixS[5] != 0);
-          }
-          fx[k] = (uint8_t)(bx | (by << 1) | (bz << 2) | (qx << 3) | (qy << 4) | (qz << 5));
-        }
-    HIPCHK(hipMemcpy(e->fix + ng0 * NGRAN, fx.data(), fx.size(), hipMemcpyHostToDevice));
-  }
   for (long g = 0; g < npg; g++) e->h_ginst[pg0 + g] = id;
   for (long g = 0; g < nng; g++) e->h_nginst[ng0 + g] = id;
-  HIPCHK(hipMemcpy(e->d_ginst + pg0, &e->h_ginst[pg0], npg * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(e->d_nginst + ng0, &e->h_nginst[ng0], nng * sizeof(int), hipMemcpyHostToDevice));
+  // Everything goes through one pinned staging buffer as copies ordered on
+  // the engine stream, behind any queued steps (which never touch the new
+  // ranges): constants, tables, granule maps, and the raw particle rows and
+  // body ids, which a kernel converts into the pool (k_admit) while another
+  // derives the node fixities (k_fixity).
+  const size_t o_m = align16(sizeof(InstC));
+  const size_t o_b = align16(o_m + sizeof(MatC) * p->nb_mat);
+  const size_t o_t = align16(o_b + sizeof(BodyC) * p->nb_body);
+  const size_t o_gi = align16(o_t + sizeof(InstT));
+  const size_t o_ng = align16(o_gi + sizeof(int) * npg);
+  const size_t o_pb = align16(o_ng + sizeof(int) * nng);
+  const size_t rows_pb = align16(sizeof(int) * (size_t)n);
+  const size_t rows = rows_pb + sizeof(double) * PF_COUNT * (size_t)n;
+  int slot;
+  char* st = stage_get(e, o_pb + rows, slot);
+  memcpy(st, &c, sizeof(InstC));
+  memcpy(st + o_m, mats.data(), sizeof(MatC) * p->nb_mat);
+  memcpy(st + o_b, bodies.data(), sizeof(BodyC) * p->nb_body);
+  memcpy(st + o_t, &ts, sizeof(InstT));
+  memcpy(st + o_gi, &e->h_ginst[pg0], sizeof(int) * npg);
+  memcpy(st + o_ng, &e->h_nginst[ng0], sizeof(int) * nng);
+  if (n) {
+    memcpy(st + o_pb, p->pbody, sizeof(int) * (size_t)n);
+    memcpy(st + o_pb + rows_pb, p->pstate, sizeof(double) * PF_COUNT * (size_t)n);
+  }
+  if (rows > e->d_up_cap) {   // grows with the largest admission so far
+    HIPCHK(hipStreamSynchronize(e->s));
+    if (e->d_up) HIPCHK(hipFree(e->d_up));
+    e->d_up = nullptr;
+    e->d_up_cap = 0;
+    e->d_up = dalloc<char>(rows);
+    e->d_up_cap = rows;
+  }
+  const hipMemcpyKind H2D = hipMemcpyHostToDevice;
+  HIPCHK(hipMemcpyAsync(e->d_inst + id, st, sizeof(InstC), H2D, e->s));
+  HIPCHK(hipMemcpyAsync(e->d_mats + m0, st + o_m, sizeof(MatC) * p->nb_mat, H2D, e->s));
+  HIPCHK(hipMemcpyAsync(e->d_bodies + b0, st + o_b, sizeof(BodyC) * p->nb_body, H2D, e->s));
+  HIPCHK(hipMemcpyAsync(e->d_ts + id, st + o_t, sizeof(InstT), H2D, e->s));
+  HIPCHK(hipMemcpyAsync(e->d_ginst + pg0, st + o_gi, sizeof(int) * npg, H2D, e->s));
+  HIPCHK(hipMemcpyAsync(e->d_nginst + ng0, st + o_ng, sizeof(int) * nng, H2D, e->s));
+  if (n) HIPCHK(hipMemcpyAsync(e->d_up, st + o_pb, rows, H2D, e->s));
+  stage_done(e, slot);
+  const int* d_pbody = (const int*)e->d_up;
+  const double* d_rows = (const double*)(e->d_up + rows_pb);
+  if (e->prec == MPM_PREC_FP32)
+    mpmk::launch_admit<float>(e->ptrs<float>(), d_rows, d_pbody, n, pg0 * PGRAN, npg * PGRAN, (int)b0, e->s);
+  else
+    mpmk::launch_admit<double>(e->ptrs<double>(), d_rows, d_pbody, n, pg0 * PGRAN, npg * PGRAN, (int)b0, e->s);
+  mpmk::launch_fixity(e->fix, e->d_inst, id, nng * NGRAN, e->s);
+  HIPCHK(hipGetLastError());
   e->lists_dirty = true;
-  e->graph_dirty = true;
+  mark_graph_dirty(e);
   return id;
 }
 
 static void remove_instance(mpm_engine* e, int id) {
   if (id < 0 || id >= e->max_inst || !e->inst[id].live) fail(MPM_ERR_ARG, "no live instance %d", id);
-  HIPCHK(hipStreamSynchronize(e->s));
+  // no wait: queued steps may still run over the instance; whatever is
+  // later written into its ranges is ordered behind them on the stream
   HostInst& h = e->inst[id];
   e->pa.release(h.pg0, h.npg);
   e->na.release(h.ng0, h.nng);
@@ -606,24 +692,20 @@ static void remove_instance(mpm_engine* e, int id) {
   h.live = false;
   e->free_ids.push_back(id);
   e->lists_dirty = true;
-  e->graph_dirty = true;
+  mark_graph_dirty(e);
 }
 
-template <typename V>
-static void upload_list(int*& dptr, size_t& cap, const V& v) {
-  if (v.size() > cap || !dptr) {
-    if (dptr) HIPCHK(hipFree(dptr));
-    cap = std::max<size_t>(v.size(), 16);
-    dptr = dalloc<int>(cap);
-  }
-  if (!v.empty()) HIPCHK(hipMemcpy(dptr, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
-}
-
+// Launch lists live in device buffers of the largest size they can take
+// (so they are never reallocated under queued launches) and are refilled by
+// copies ordered on the engine stream: launches already queued read the old
+// lists, later ones the new.  Buffers of groups that vanish are kept for
+// reuse until the engine is destroyed.
 static void rebuild_lists(mpm_engine* e) {
   if (!e->lists_dirty) return;
-  HIPCHK(hipStreamSynchronize(e->s));
   std::vector<Group> old;
   old.swap(e->groups);
+  for (auto& g : old)
+    if (g.d_pgran) e->spare_lists.push_back({g.d_pgran, g.d_ngran, g.d_ilist});
   std::map<std::pair<int, int>, int> gid;
   std::vector<int> all;
   for (int id = 0; id < e->max_inst; id++) {
@@ -637,13 +719,23 @@ static void rebuild_lists(mpm_engine* e) {
       g = (int)e->groups.size();
       gid[key] = g;
       e->groups.emplace_back();
-      e->groups[g].shape = h.c.shape;
-      e->groups[g].scheme = h.c.scheme;
-      if (g < (int)old.size()) {   // reuse device lists
-        e->groups[g].d_pgran = old[g].d_pgran; e->groups[g].cap_p = old[g].cap_p; old[g].d_pgran = nullptr;
-        e->groups[g].d_ngran = old[g].d_ngran; e->groups[g].cap_n = old[g].cap_n; old[g].d_ngran = nullptr;
-        e->groups[g].d_ilist = old[g].d_ilist; e->groups[g].cap_i = old[g].cap_i; old[g].d_ilist = nullptr;
+      Group& G = e->groups[g];
+      G.shape = h.c.shape;
+      G.scheme = h.c.scheme;
+      if (!e->spare_lists.empty()) {
+        const auto l = e->spare_lists.back();
+        e->spare_lists.pop_back();
+        G.d_pgran = l[0];
+        G.d_ngran = l[1];
+        G.d_ilist = l[2];
+      } else {
+        G.d_pgran = dalloc<int>(e->np / PGRAN);
+        G.d_ngran = dalloc<int>(e->nn / NGRAN);
+        G.d_ilist = dalloc<int>(e->max_inst);
       }
+      G.cap_p = e->np / PGRAN;
+      G.cap_n = e->nn / NGRAN;
+      G.cap_i = e->max_inst;
     } else {
       g = it->second;
     }
@@ -653,17 +745,30 @@ static void rebuild_lists(mpm_engine* e) {
     for (long k = 0; k < h.npg; k++) G.pgran.push_back((int)(h.pg0 + k));
     for (long k = 0; k < h.nng; k++) G.ngran.push_back((int)(h.ng0 + k));
   }
-  for (auto& g : old) {
-    if (g.d_pgran) (void)hipFree(g.d_pgran);
-    if (g.d_ngran) (void)hipFree(g.d_ngran);
-    if (g.d_ilist) (void)hipFree(g.d_ilist);
+  if (!e->d_ilist_all) {
+    e->d_ilist_all = dalloc<int>(e->max_inst);
+    e->cap_ilist = e->max_inst;
   }
+  size_t bytes = align16(all.size() * sizeof(int));
+  for (auto& G : e->groups)
+    bytes += align16(G.pgran.size() * sizeof(int)) + align16(G.ngran.size() * sizeof(int)) +
+             align16(G.insts.size() * sizeof(int));
+  int slot;
+  char* st = stage_get(e, bytes, slot);
+  size_t off = 0;
+  auto put = [&](int* dst, const std::vector<int>& v) {
+    if (v.empty()) return;
+    memcpy(st + off, v.data(), v.size() * sizeof(int));
+    HIPCHK(hipMemcpyAsync(dst, st + off, v.size() * sizeof(int), hipMemcpyHostToDevice, e->s));
+    off += align16(v.size() * sizeof(int));
+  };
   for (auto& G : e->groups) {
-    upload_list(G.d_pgran, G.cap_p, G.pgran);
-    upload_list(G.d_ngran, G.cap_n, G.ngran);
-    upload_list(G.d_ilist, G.cap_i, G.insts);
+    put(G.d_pgran, G.pgran);
+    put(G.d_ngran, G.ngran);
+    put(G.d_ilist, G.insts);
   }
-  upload_list(e->d_ilist_all, e->cap_ilist, all);
+  put(e->d_ilist_all, all);
+  stage_done(e, slot);
   e->n_active = (int)all.size();
   e->lists_dirty = false;
 }
@@ -937,11 +1042,20 @@ static void step_plain(mpm_engine* e, int nsteps, int fin_flags) {
   mark_after_step(e);
 }
 
-// graph of `per` fused steps, replayed; instances finish on their own
-static void step_graph(mpm_engine* e, int nsteps, int per, int fin_flags) {
+// graph of `per` fused steps, replayed; instances finish on their own.
+// lazy: after the launch lists change, the first call launches plainly and
+// only a second call with the same lists captures (continuous batching
+// changes the instance set every few steps, and a capture costs several
+// plain steps)
+static void step_graph(mpm_engine* e, int nsteps, int per, int fin_flags, bool lazy) {
   rebuild_lists(e);
   if (per < 1) per = 1;
   if (e->graph_dirty || e->graph_steps != per) {
+    if (lazy && !e->graph_plain_once) {
+      step_plain(e, nsteps, fin_flags);
+      e->graph_plain_once = true;
+      return;
+    }
     if (e->gexec) HIPCHK(hipGraphExecDestroy(e->gexec));
     if (e->graph) HIPCHK(hipGraphDestroy(e->graph));
     e->gexec = nullptr;
@@ -958,6 +1072,7 @@ static void step_graph(mpm_engine* e, int nsteps, int per, int fin_flags) {
     HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
     e->graph_steps = per;
     e->graph_dirty = false;
+    e->graph_plain_once = false;
   }
   int done = 0;
   while (done + per <= nsteps) {
@@ -996,6 +1111,17 @@ static void ensure_sort_buffers(mpm_engine* e) {
     e->t_alt = dalloc<char>((size_t)T_COUNT * np * e->tsz);
     e->d_alt = dalloc<double>((size_t)D_COUNT * np);
     e->i_alt = dalloc<int>((size_t)I_COUNT * np);
+    // temporary storage of a radix sort over the whole pool, which covers
+    // the single-instance sorts at admission (no reallocation there)
+    size_t need = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, e->k_in, e->k_out, e->v_in, e->v_out, (int)np, 0, 32,
+                                              e->s));
+    if (need > e->cub_bytes) {
+      if (e->cub_tmp) HIPCHK(hipFree(e->cub_tmp));
+      e->cub_tmp = nullptr;
+      e->cub_tmp = dalloc<char>(need);
+      e->cub_bytes = need;
+    }
   }
 }
 
@@ -1005,12 +1131,8 @@ static void sort_instance(mpm_engine* e, int id) {
   ensure_sort_buffers(e);
   const HostInst& h = e->inst[id];
   const long np = e->np, b0 = h.pg0 * PGRAN, pad = h.npg * PGRAN;
-  std::vector<int> gl(h.npg);
-  for (long g = 0; g < h.npg; g++) gl[g] = (int)(h.pg0 + g);
-  int* d_gl = dalloc<int>(gl.size());
-  HIPCHK(hipMemcpy(d_gl, gl.data(), gl.size() * sizeof(int), hipMemcpyHostToDevice));
   LaunchP L;
-  L.glist = d_gl;
+  L.glist = e->d_seq + h.pg0;   // the instance's granules pg0, pg0 + 1, ...
   L.ginst = e->d_ginst;
   L.bs = 256;
   L.bpg = PGRAN / 256;
@@ -1045,8 +1167,6 @@ static void sort_instance(mpm_engine* e, int id) {
   for (int f = 0; f < I_COUNT; f++)
     HIPCHK(hipMemcpyAsync(e->ii + (size_t)f * np + b0, e->i_alt + (size_t)f * np + b0, pad * sizeof(int),
                           hipMemcpyDeviceToDevice, e->s));
-  HIPCHK(hipStreamSynchronize(e->s));
-  HIPCHK(hipFree(d_gl));
 }
 
 static void sort_particles(mpm_engine* e) {
@@ -1179,7 +1299,7 @@ static void apply_tuning(mpm_engine* e, const mpm_tuning& t) {
     if (t.tile_nodes < 8 || t.tile_nodes > 512) fail(MPM_ERR_ARG, "bad tuning values");
     e->tile_cap = t.tile_nodes;
   }
-  e->graph_dirty = true;
+  mark_graph_dirty(e);
 }
 
 static mpm_tuning current_tuning(const mpm_engine* e) {
@@ -1435,18 +1555,13 @@ static int live_id(mpm_engine* e, int id) {
 // calcEnergy of one freshly added instance (MPM3D.f90:42)
 static void initial_energy(mpm_engine* e, int id) {
   const HostInst& h = e->inst[id];
-  std::vector<int> gl;
-  for (long g = 0; g < h.npg; g++) gl.push_back((int)(h.pg0 + g));
-  int* d_gl = dalloc<int>(gl.size());
-  int* d_il = dalloc<int>(1);
-  HIPCHK(hipMemcpyAsync(d_gl, gl.data(), gl.size() * sizeof(int), hipMemcpyHostToDevice, e->s));
-  HIPCHK(hipMemcpyAsync(d_il, &id, sizeof(int), hipMemcpyHostToDevice, e->s));
+  const int* d_il = e->d_seq + id;   // the one-instance list {id}
   LaunchP L;
-  L.glist = d_gl;
+  L.glist = e->d_seq + h.pg0;
   L.ginst = e->d_ginst;
   L.bs = e->bs_p;
   L.bpg = PGRAN / e->bs_p;
-  L.nblk = (int)gl.size() * L.bpg;
+  L.nblk = (int)h.npg * L.bpg;
   if (e->prec == MPM_PREC_FP32) {
     mpmk::launch_energy<float>(e->ptrs<float>(), L, e->s);
     mpmk::launch_finalize<float>(e->ptrs<float>(), d_il, 1, L.bpg, FIN_ENERGY, e->s);
@@ -1454,9 +1569,7 @@ static void initial_energy(mpm_engine* e, int id) {
     mpmk::launch_energy<double>(e->ptrs<double>(), L, e->s);
     mpmk::launch_finalize<double>(e->ptrs<double>(), d_il, 1, L.bpg, FIN_ENERGY, e->s);
   }
-  HIPCHK(hipStreamSynchronize(e->s));
-  HIPCHK(hipFree(d_gl));
-  HIPCHK(hipFree(d_il));
+  HIPCHK(hipGetLastError());
 }
 
 // ================================================================ C-ABI
@@ -1624,8 +1737,7 @@ extern "C" int mpm_engine_add(mpm_engine* e, const mpm_problem* p, int stop_at_e
     check_ctx(e);
     if (!id) fail(MPM_ERR_ARG, "id is NULL");
     *id = add_instance(e, p, stop_at_endtime ? IF_ENDTIME : 0);
-    e->lists_dirty = true;
-    rebuild_lists(e);
+    e->lists_dirty = true;    // rebuilt before the next step
     if (e->sort_every > 0) sort_instance(e, *id);
     initial_energy(e, *id);   // MPM3D.f90:42
   });
@@ -1637,9 +1749,13 @@ extern "C" int mpm_engine_set_step_limit(mpm_engine* e, int id, int max_steps) {
     HostInst& h = e->inst[live_id(e, id)];
     if (max_steps < 0) fail(MPM_ERR_ARG, "max_steps < 0");
     h.c.max_steps = max_steps;
-    HIPCHK(hipStreamSynchronize(e->s));
-    HIPCHK(hipMemcpy(e->d_inst + id, &h.c, sizeof(InstC), hipMemcpyHostToDevice));
-    e->graph_dirty = true;
+    // ordered behind queued work (kernels read the constants at run time,
+    // so a captured graph stays valid)
+    int slot;
+    char* st = stage_get(e, sizeof(InstC), slot);
+    memcpy(st, &h.c, sizeof(InstC));
+    HIPCHK(hipMemcpyAsync(e->d_inst + id, st, sizeof(InstC), hipMemcpyHostToDevice, e->s));
+    stage_done(e, slot);
   });
 }
 
@@ -1654,7 +1770,8 @@ extern "C" int mpm_engine_step(mpm_engine* e, int nsteps, int use_graph, int gra
   return guarded([&] {
     check_ctx(e);
     if (nsteps < 0) fail(MPM_ERR_ARG, "nsteps < 0");
-    if (use_graph) step_graph(e, nsteps, graph_steps, FIN_DONE);
+    if (use_graph < 0 || use_graph > 2) fail(MPM_ERR_ARG, "use_graph must be 0, 1 or 2");
+    if (use_graph) step_graph(e, nsteps, graph_steps, FIN_DONE, use_graph == 2);
     else step_plain(e, nsteps, FIN_DONE);
   });
 }
@@ -1677,7 +1794,7 @@ extern "C" int mpm_engine_set_interleave(mpm_engine* e, int enable) {
   return guarded([&] {
     check_ctx(e);
     e->ilv = enable != 0;
-    e->graph_dirty = true;
+    mark_graph_dirty(e);
   });
 }
 
@@ -1689,7 +1806,7 @@ extern "C" int mpm_engine_set_tuning(mpm_engine* e, int block_particles, int blo
     e->bs_p = block_particles;
     e->bs_n = block_nodes;
     e->sort_every = sort_every;
-    e->graph_dirty = true;
+    mark_graph_dirty(e);
   });
 }
 

@@ -69,6 +69,11 @@ void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int bpg,
 void launch_begin_step(InstT* ts, const int* ilist, int n, hipStream_t s);
 template <typename S>
 void launch_permute(S* dst, const S* src, const int* perm, const LaunchP& L, long np, int nfield, hipStream_t s);
+// admission: raw FP64 particle rows -> pool slots [b0, b0+pad); node fixities of instance id
+template <typename T>
+void launch_admit(const PoolPtrs<T>& P, const double* ps, const int* pbody, int n, long b0, long pad, int body0,
+                  hipStream_t s);
+void launch_fixity(uint8_t* fix, const InstC* inst, int id, long nslots, hipStream_t s);
 template <typename T> void launch_sgmp_p2g(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_sgmp_nodes(int what, const PoolPtrs<T>& P, const LaunchP& N, hipStream_t s);
 template <typename T> void launch_sgmp_cells(int what, const PoolPtrs<T>& P, const LaunchP& N, hipStream_t s);

@@ -2557,6 +2557,99 @@ __global__ void __launch_bounds__(256) k_sort_keys(const double* X, long np, con
 }
 #endif
 
+// ---------------------------------------------------------- admission
+
+// Scatter one admitted instance's particle state (field-major FP64 rows of
+// mpm_pfield, n particles, uploaded raw) into its padded slot range [b0,
+// b0 + pad) of the pool, converting the bulk fields to T; pad slots are zero.
+template <typename T>
+__global__ void __launch_bounds__(256) k_admit(PoolPtrs<T> P, const double* ps, const int* pbody, int n, long b0,
+                                               long pad, int body0) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= pad) return;
+  const bool in = j < n;
+  const long s = b0 + j, np = P.np;
+  auto F = [&](int f) { return in ? ps[(size_t)f * n + j] : 0.0; };
+  T* t = P.t;
+  double* d = P.d;
+  t[T_VX * np + s] = (T)F(PF_VX);
+  t[T_VY * np + s] = (T)F(PF_VY);
+  t[T_VZ * np + s] = (T)F(PF_VZ);
+  t[T_BX * np + s] = (T)0;
+  t[T_BY * np + s] = (T)0;
+  t[T_BZ * np + s] = (T)0;
+  t[T_FX * np + s] = (T)F(PF_FX);
+  t[T_FY * np + s] = (T)F(PF_FY);
+  t[T_FZ * np + s] = (T)F(PF_FZ);
+  t[T_MASS * np + s] = (T)F(PF_MASS);
+  t[T_SM * np + s] = (T)F(PF_SM);
+  t[T_SEQV * np + s] = (T)F(PF_SEQV);
+  t[T_SD0 * np + s] = (T)F(PF_SDXX);
+  t[T_SD1 * np + s] = (T)F(PF_SDYY);
+  t[T_SD2 * np + s] = (T)F(PF_SDZZ);
+  t[T_SD3 * np + s] = (T)F(PF_SDYZ);
+  t[T_SD4 * np + s] = (T)F(PF_SDXZ);
+  t[T_SD5 * np + s] = (T)F(PF_SDXY);
+  t[T_SIGY * np + s] = (T)F(PF_SIGY);
+  t[T_EPEFF * np + s] = (T)F(PF_EPEFF);
+  t[T_Q * np + s] = (T)F(PF_Q);
+  t[T_CP * np + s] = (T)F(PF_CP);
+  d[D_X * np + s] = F(PF_X);
+  d[D_Y * np + s] = F(PF_Y);
+  d[D_Z * np + s] = F(PF_Z);
+  d[D_VOL * np + s] = F(PF_VOL);
+  d[D_IE * np + s] = F(PF_IE);
+#pragma unroll
+  for (int k = 0; k < 9; k++) d[(D_F0 + k) * np + s] = F(PF_FP + k);
+  d[D_MASS * np + s] = F(PF_MASS);
+  P.i[I_BODY * np + s] = in ? body0 + pbody[j] : body0;
+  P.i[I_ORIG * np + s] = (int)j;
+  P.i[I_CELL * np + s] = -1;
+}
+
+#if MPMK_PART != 2
+// Fixity byte of every node of one instance (Grid.f90:291-359): bits 0-2
+// fixed x/y/z, bits 3-5 SGMP boundary node (bd_type /= 0, Grid.f90:303-316).
+// One outer layer of a face with FixS = 1 (fixed) or 2 (symmetry: normal
+// component only); two layers for GIMP, B-spline and SGMP.
+__global__ void __launch_bounds__(256) k_fixity(uint8_t* fix, const InstC* inst, int id, long nslots) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nslots) return;
+  const InstC& c = inst[id];
+  uint8_t v = 0;
+  if (k < c.nnode) {
+    const int NGx = c.ngx, NGy = c.ngy, NGz = c.ngz;
+    const int ix = (int)(k % NGx) + 1, iy = (int)((k / NGx) % NGy) + 1, iz = (int)(k / c.ngxy) + 1;
+    const int* FixS = c.fixs;
+    const bool wide = c.shape == MPM_SHAPE_GIMP || c.shape == MPM_SHAPE_BSPLINE || c.shape == MPM_SHAPE_SGMP;
+    int bx = 0, by = 0, bz = 0;
+    if (wide) {
+      if (((ix == 1 || ix == 2) && FixS[0] == 1) || ((ix == NGx || ix == NGx - 1) && FixS[1] == 1) ||
+          ((iy == 1 || iy == 2) && FixS[2] == 1) || ((iy == NGy || iy == NGy - 1) && FixS[3] == 1) ||
+          ((iz == 1 || iz == 2) && FixS[4] == 1) || ((iz == NGz || iz == NGz - 1) && FixS[5] == 1))
+        bx = by = bz = 1;
+      if (((ix == 1 || ix == 2) && FixS[0] == 2) || ((ix == NGx || ix == NGx - 1) && FixS[1] == 2)) bx = 1;
+      if (((iy == 1 || iy == 2) && FixS[2] == 2) || ((iy == NGy || iy == NGy - 1) && FixS[3] == 2)) by = 1;
+      if (((iz == 1 || iz == 2) && FixS[4] == 2) || ((iz == NGz || iz == NGz - 1) && FixS[5] == 2)) bz = 1;
+    }
+    if ((ix == 1 && FixS[0] == 1) || (ix == NGx && FixS[1] == 1) || (iy == 1 && FixS[2] == 1) ||
+        (iy == NGy && FixS[3] == 1) || (iz == 1 && FixS[4] == 1) || (iz == NGz && FixS[5] == 1))
+      bx = by = bz = 1;
+    if ((ix == 1 && FixS[0] == 2) || (ix == NGx && FixS[1] == 2)) bx = 1;
+    if ((iy == 1 && FixS[2] == 2) || (iy == NGy && FixS[3] == 2)) by = 1;
+    if ((iz == 1 && FixS[4] == 2) || (iz == NGz && FixS[5] == 2)) bz = 1;
+    int qx = 0, qy = 0, qz = 0;
+    if (c.shape == MPM_SHAPE_SGMP) {
+      qx = ((ix == 1 || ix == 2) && FixS[0] != 0) || ((ix == NGx || ix == NGx - 1) && FixS[1] != 0);
+      qy = ((iy == 1 || iy == 2) && FixS[2] != 0) || ((iy == NGy || iy == NGy - 1) && FixS[3] != 0);
+      qz = ((iz == 1 || iz == 2) && FixS[4] != 0) || ((iz == NGz || iz == NGz - 1) && FixS[5] != 0);
+    }
+    v = (uint8_t)(bx | (by << 1) | (bz << 2) | (qx << 3) | (qy << 4) | (qz << 5));
+  }
+  fix[c.node_off + k] = v;
+}
+#endif
+
 // ---------------------------------------------------------- dispatch
 
 template <int SHAPE, typename T, int MODE>
@@ -2778,7 +2871,20 @@ void launch_permute(S* dst, const S* src, const int* perm, const LaunchP& L, lon
   if (L.nblk) hipLaunchKernelGGL(k_permute<S>, dim3(L.nblk), dim3(L.bs), 0, s, dst, src, perm, L.glist, L.bpg, np, nfield);
 }
 
+template <typename T>
+void launch_admit(const PoolPtrs<T>& P, const double* ps, const int* pbody, int n, long b0, long pad, int body0,
+                  hipStream_t s) {
+  if (pad > 0)
+    hipLaunchKernelGGL(k_admit<T>, dim3((unsigned)((pad + 255) / 256)), dim3(256), 0, s, P, ps, pbody, n, b0, pad,
+                       body0);
+}
+
 #if MPMK_PART != 2
+void launch_fixity(uint8_t* fix, const InstC* inst, int id, long nslots, hipStream_t s) {
+  if (nslots > 0)
+    hipLaunchKernelGGL(k_fixity, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, fix, inst, id, nslots);
+}
+
 void launch_begin_step(InstT* ts, const int* ilist, int n, hipStream_t s) {
   if (n) hipLaunchKernelGGL(k_begin_step, dim3((n + 255) / 256), dim3(256), 0, s, ts, ilist, n);
 }
@@ -2804,7 +2910,8 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
   template void launch_permute<T>(T*, const T*, const int*, const LaunchP&, long, int, hipStream_t);  \
   template void launch_sgmp_p2g<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);         \
   template void launch_sgmp_nodes<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);       \
-  template void launch_sgmp_cells<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);
+  template void launch_sgmp_cells<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);                     \
+  template void launch_admit<T>(const PoolPtrs<T>&, const double*, const int*, int, long, long, int, hipStream_t);
 #if MPMK_PART != 2
 INST(double)
 template void launch_permute<int>(int*, const int*, const int*, const LaunchP&, long, int, hipStream_t);

@@ -42,10 +42,12 @@ class Request:
 
 class Server:
     """engine: an Engine; chunk: steps per launch round; graph: replay the
-    chunk from a captured HIP graph; keep_state: download the final particle
-    state of every run (else only its scalars)."""
+    chunk from a captured HIP graph ("lazy": capture only once the resident
+    set has stayed the same for a round, since a capture costs several plain
+    rounds and admissions change the set often); keep_state: download the
+    final particle state of every run (else only its scalars)."""
 
-    def __init__(self, engine, chunk=8, graph=True, keep_state=False, clock=time.perf_counter):
+    def __init__(self, engine, chunk=8, graph="lazy", keep_state=False, clock=time.perf_counter):
         self.eng = engine
         self.chunk = int(chunk)
         self.graph = graph

@@ -29,8 +29,9 @@ def pad(n):
     return -(-n // 256) * 256
 
 
+@pytest.mark.parametrize("graph", [True, "lazy"])
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
-def test_served_runs_equal_solo_runs(prec, tol):
+def test_served_runs_equal_solo_runs(prec, tol, graph):
     mpm = load_package()
     NAMES, STEPS = CASES[prec]
     probs = [mpm.read_deck(deck_path(n)) for n in NAMES]
@@ -45,7 +46,7 @@ def test_served_runs_equal_solo_runs(prec, tol):
     two = sorted(pad(p.n) for p in probs)[-2:]
     eng = mpm.Engine(max_particles=sum(two), max_nodes=sum(pad(p.nb_node) for p in probs), max_instances=4,
                      precision=prec)
-    srv = mpm.serve.Server(eng, chunk=4, graph=True, keep_state=True)
+    srv = mpm.serve.Server(eng, chunk=4, graph=graph, keep_state=True)
     for k, (p, s) in enumerate(zip(probs, STEPS)):
         srv.submit(mpm.serve.Request(k, p, s, arrival=0.0, kind=NAMES[k]))
     clock = [0.0]
