@@ -91,6 +91,14 @@ int mpm_engine_get_scalars(mpm_engine* e, int id, double* out8);
 int mpm_engine_download_order(mpm_engine* e, int id, int32_t* orig);
 /* done flag (1 finished, 0 running, -1 no such instance) and istep */
 int mpm_engine_status(mpm_engine* e, int n, const int* ids, int* done, int* istep);
+/* Non-blocking status: snapshot_status enqueues a copy of every instance's
+ * time state behind the work queued so far and returns at once;
+ * snapshot_read waits for that copy (not for work queued after it) and
+ * returns per id the done flag and the 8 scalars of mpm_engine_get_scalars
+ * as of the snapshot (out8 may be NULL).  A serving loop reads round k's
+ * finished runs while round k+1 runs. */
+int mpm_engine_snapshot_status(mpm_engine* e);
+int mpm_engine_snapshot_read(mpm_engine* e, int n, const int* ids, int* done, double* out8);
 
 /* Profiling: with enable != 0 every launch of the following steps is
  * bracketed by HIP events on the engine's stream; kernel_times returns the

@@ -326,6 +326,21 @@ class Engine:
         check(_lib.lib().mpm_engine_get_scalars(self._h, int(iid), dptr(out)), "mpm_engine_get_scalars")
         return out
 
+    def snapshot(self):
+        """Enqueue a copy of every instance's time state behind the queued
+        work (non-blocking); read it with snapshot_read."""
+        check(_lib.lib().mpm_engine_snapshot_status(self._h), "mpm_engine_snapshot_status")
+
+    def snapshot_read(self, ids):
+        """{id: (done, scalars)} as of the last snapshot (waits for it only)."""
+        ids = list(ids)
+        n = len(ids)
+        a = (C.c_int * max(n, 1))(*ids)
+        done = (C.c_int * max(n, 1))()
+        out = np.zeros((max(n, 1), 8))
+        check(_lib.lib().mpm_engine_snapshot_read(self._h, n, a, done, dptr(out)), "mpm_engine_snapshot_read")
+        return {ids[k]: (done[k], out[k].copy()) for k in range(n)}
+
     def status(self, ids=None):
         ids = list(self.inst) if ids is None else list(ids)
         n = len(ids)

@@ -139,6 +139,8 @@ def lib():
         "mpm_engine_get_scalars": (C.c_int, [vp, C.c_int, P(C.c_double)]),
         "mpm_engine_download_order": (C.c_int, [vp, C.c_int, P(C.c_int32)]),
         "mpm_engine_status": (C.c_int, [vp, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "mpm_engine_snapshot_status": (C.c_int, [vp]),
+        "mpm_engine_snapshot_read": (C.c_int, [vp, C.c_int, P(C.c_int), P(C.c_int), P(C.c_double)]),
         "mpm_ctx_engine": (C.c_int, [vp, P(vp)]),
         "mpm_engine_profile": (C.c_int, [vp, C.c_int]),
         "mpm_engine_kernel_times": (C.c_int, [vp, P(C.c_double), P(C.c_long)]),

@@ -186,6 +186,9 @@ struct mpm_engine {
   char* d_up = nullptr;           // device staging of the raw particle rows of an admission
   size_t d_up_cap = 0;
   std::vector<std::array<int*, 3>> spare_lists;   // launch-list buffers of vanished groups (reused)
+  InstT* h_snap = nullptr;        // pinned: time states as of the last status snapshot
+  hipEvent_t snap_ev = nullptr;
+  bool snap_pending = false;
   // launch lists
   std::vector<Group> groups;
   bool lists_dirty = true;
@@ -277,6 +280,8 @@ static void free_engine(mpm_engine* e) {
     if (r.ev) (void)hipEventDestroy(r.ev);
   }
   if (e->d_up) (void)hipFree(e->d_up);
+  if (e->h_snap) (void)hipHostFree(e->h_snap);
+  if (e->snap_ev) (void)hipEventDestroy(e->snap_ev);
   if (e->d_seq) (void)hipFree(e->d_seq);
   if (e->s) (void)hipStreamDestroy(e->s);
   delete e;
@@ -1911,6 +1916,45 @@ extern "C" int mpm_engine_status(mpm_engine* e, int n, const int* ids, int* done
       }
       done[k] = ts[id].done;
       if (istep) istep[k] = ts[id].istep;
+    }
+  });
+}
+
+extern "C" int mpm_engine_snapshot_status(mpm_engine* e) {
+  return guarded([&] {
+    check_ctx(e);
+    if (!e->h_snap) {
+      HIPCHK(hipHostMalloc((void**)&e->h_snap, sizeof(InstT) * e->max_inst, hipHostMallocDefault));
+      HIPCHK(hipEventCreateWithFlags(&e->snap_ev, hipEventDisableTiming));
+    }
+    if (e->snap_pending) HIPCHK(hipEventSynchronize(e->snap_ev));   // the buffer is being filled
+    HIPCHK(hipMemcpyAsync(e->h_snap, e->d_ts, sizeof(InstT) * e->max_inst, hipMemcpyDeviceToHost, e->s));
+    HIPCHK(hipEventRecord(e->snap_ev, e->s));
+    e->snap_pending = true;
+  });
+}
+
+extern "C" int mpm_engine_snapshot_read(mpm_engine* e, int n, const int* ids, int* done, double* out8) {
+  return guarded([&] {
+    check_ctx(e);
+    if (!e->h_snap || !e->snap_pending) fail(MPM_ERR_STATE, "no status snapshot taken");
+    HIPCHK(hipEventSynchronize(e->snap_ev));
+    for (int k = 0; k < n; k++) {
+      const int id = ids[k];
+      const bool ok = id >= 0 && id < e->max_inst && e->inst[id].live;
+      done[k] = ok ? e->h_snap[id].done : -1;
+      if (out8) {
+        double* o = out8 + 8 * k;
+        const InstT& t = e->h_snap[ok ? id : 0];
+        o[0] = ok ? t.istep : -1;
+        o[1] = t.time;
+        o[2] = t.dt;
+        o[3] = t.ek;
+        o[4] = t.ei;
+        o[5] = t.mom[0];
+        o[6] = t.mom[1];
+        o[7] = t.mom[2];
+      }
     }
   });
 }

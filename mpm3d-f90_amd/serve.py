@@ -45,14 +45,19 @@ class Server:
     chunk from a captured HIP graph ("lazy": capture only once the resident
     set has stayed the same for a round, since a capture costs several plain
     rounds and admissions change the set often); keep_state: download the
-    final particle state of every run (else only its scalars)."""
+    final particle state of every run (else only its scalars); pipeline:
+    retire from a status snapshot taken after the previous round, so the host
+    waits for round k-1 while round k runs (finished runs are seen one round
+    later; the device is never idle between rounds; needs keep_state off)."""
 
-    def __init__(self, engine, chunk=8, graph="lazy", keep_state=False, clock=time.perf_counter):
+    def __init__(self, engine, chunk=8, graph="lazy", keep_state=False, clock=time.perf_counter, pipeline=False):
         self.eng = engine
         self.chunk = int(chunk)
         self.graph = graph
         self.keep_state = keep_state
         self.clock = clock
+        self.pipeline = bool(pipeline) and not keep_state
+        self._snap_ids = None      # running ids covered by the pending status snapshot
         self.queue = collections.deque()
         self.running = {}          # iid -> Request
         self.done = []
@@ -99,6 +104,8 @@ class Server:
         return self.chunk
 
     def retire(self, now_fn):
+        if self.pipeline:
+            return self._retire_pipelined(now_fn)
         if not self.running:
             return 0
         ids = list(self.running)
@@ -117,6 +124,26 @@ class Server:
             req.finished = now_fn()
             self.done.append(req)
             out += 1
+        return out
+
+    def _retire_pipelined(self, now_fn):
+        out = 0
+        if self._snap_ids:
+            snap = self.eng.snapshot_read(self._snap_ids)   # waits for the previous round only
+            for iid, (done, sc) in snap.items():
+                if done != 1 or iid not in self.running:
+                    continue
+                req = self.running.pop(iid)
+                req.final_step = int(sc[0])
+                req.result = {"scalars": sc}
+                self.eng.remove(iid)
+                req.finished = now_fn()
+                self.done.append(req)
+                out += 1
+        self._snap_ids = None
+        if self.running:
+            self.eng.snapshot()                               # after the round just queued
+            self._snap_ids = list(self.running)
         return out
 
     def idle(self):

@@ -69,3 +69,48 @@ def test_served_runs_equal_solo_runs(prec, tol, graph):
         for g, e in errs.items():
             t = loose.get(g, tol) if NAMES[r.rid] == "dam_null_eos4_musl" else tol
             assert e <= t, (NAMES[r.rid], g, e, errs)
+
+
+def test_pipelined_serving_matches_solo_runs():
+    """The pipelined loop retires runs from a status snapshot of the previous
+    round (no device wait between rounds); each run still stops at its own
+    step count, and its final scalars (time, DT, kinetic and internal
+    energy, momentum) equal the run alone up to the summation order."""
+    mpm = load_package()
+    NAMES, STEPS = CASES["fp64"]
+    probs = [mpm.read_deck(deck_path(n)) for n in NAMES]
+    solo = []
+    for p, s in zip(probs, STEPS):
+        e = mpm.Engine(max_particles=pad(p.n), max_nodes=pad(p.nb_node), max_instances=1, precision="fp64")
+        i = e.add(p)
+        e.step(s)
+        solo.append(e.scalars(i))
+        e.close()
+    two = sorted(pad(p.n) for p in probs)[-2:]
+    eng = mpm.Engine(max_particles=sum(two), max_nodes=sum(pad(p.nb_node) for p in probs), max_instances=4,
+                     precision="fp64")
+    srv = mpm.serve.Server(eng, chunk=4, graph="lazy", pipeline=True)
+    assert srv.pipeline
+    for k, (p, s) in enumerate(zip(probs, STEPS)):
+        srv.submit(mpm.serve.Request(k, p, s, arrival=0.0, kind=NAMES[k]))
+    clock = [0.0]
+    rounds = 0
+    while not srv.idle():
+        clock[0] += 1.0
+        srv.admit(clock[0])
+        if srv.running:
+            srv.advance()
+        srv.retire(lambda: clock[0])
+        rounds += 1
+        assert rounds < 100
+    eng.close()
+    assert sorted(r.rid for r in srv.done) == [0, 1, 2, 3]
+    for r in srv.done:
+        sc, ref = r.result["scalars"], solo[r.rid]
+        assert r.final_step == STEPS[r.rid] == int(sc[0]) == int(ref[0])
+        tol = 1e-8 if NAMES[r.rid] == "dam_null_eos4_musl" else 1e-10
+        assert np.allclose(sc[1:3], ref[1:3], rtol=1e-12, atol=0), (NAMES[r.rid], sc, ref)
+        scale = max(abs(ref[3]) + abs(ref[4]), 1e-300)
+        assert abs(sc[3] - ref[3]) <= tol * scale and abs(sc[4] - ref[4]) <= tol * scale, (NAMES[r.rid], sc, ref)
+        mscale = max(np.abs(ref[5:8]).max(), 1e-300)
+        assert np.abs(sc[5:8] - ref[5:8]).max() <= tol * mscale + 1e-300, (NAMES[r.rid], sc, ref)

@@ -53,6 +53,15 @@ class FakeEngine:
     def remove(self, iid):
         self.log.append(("remove", self.live.pop(iid)[0]))
 
+    # status snapshot: the state after the work issued so far (the stand-in's
+    # steps complete at once)
+    def snapshot(self):
+        self.snap = {i: (int(v[2] and v[1] >= v[2]), v[1]) for i, v in self.live.items()}
+
+    def snapshot_read(self, ids):
+        return {i: (self.snap[i][0] if i in self.live else -1,
+                    np.array([self.snap[i][1], 0, 0, 0, 0, 0, 0, 0], dtype=float)) for i in ids}
+
 
 def test_first_fit_admission_and_retirement():
     serve = load_package().serve
@@ -106,3 +115,33 @@ def test_poisson_arrivals_and_summary():
     assert abs(st["p99_tts_s"] - np.percentile(tts, 99)) < 1e-12
     assert st["particle_steps"] == sum((10 + k) * 5 for k in range(100))
     assert st["max_concurrent"] == 7
+
+
+def test_pipelined_retirement_sees_each_round_one_round_later():
+    """Pipelined serving retires from the snapshot taken after the previous
+    round: every run still finishes with its own step count, one round later
+    than the synchronous loop sees it, and never twice."""
+    serve = load_package().serve
+    results = {}
+    for pipe in (False, True):
+        eng = FakeEngine(cap_p=1000, cap_i=3)
+        srv = serve.Server(eng, chunk=4, graph=False, clock=lambda: 0.0, pipeline=pipe)
+        for k, (n, s) in enumerate(zip([600, 500, 300, 200], [16, 8, 8, 8])):
+            srv.submit(serve.Request(k, FakeProblem(n), s))
+        t = [0.0]
+        rounds = 0
+        while not srv.idle():
+            t[0] += 1.0
+            srv.admit(t[0])
+            if srv.running:
+                srv.advance()
+            srv.retire(lambda: t[0])
+            rounds += 1
+            assert rounds < 50
+        assert sorted(r.rid for r in srv.done) == [0, 1, 2, 3]
+        assert all(r.final_step == r.steps for r in srv.done)
+        assert len(eng.live) == 0
+        results[pipe] = {r.rid: r.finished for r in srv.done}
+    # the first run to finish (rid 2: 8 steps from round 1) is seen in round 2
+    # synchronously and in round 3 pipelined
+    assert results[False][2] == 2.0 and results[True][2] == 3.0
