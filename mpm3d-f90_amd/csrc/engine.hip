@@ -258,6 +258,7 @@ struct mpm_engine {
 
 static void free_engine(mpm_engine* e) {
   if (!e) return;
+  if (e->s) (void)hipStreamSynchronize(e->s);   // queued work may still use any buffer below
   for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
@@ -274,7 +275,6 @@ static void free_engine(mpm_engine* e) {
   for (auto& l : e->spare_lists)
     for (int* b : l)
       if (b) (void)hipFree(b);
-  if (e->s) (void)hipStreamSynchronize(e->s);
   for (auto& r : e->ring) {
     if (r.h) (void)hipHostFree(r.h);
     if (r.ev) (void)hipEventDestroy(r.ev);
