@@ -114,3 +114,54 @@ def test_pipelined_serving_matches_solo_runs():
         assert abs(sc[3] - ref[3]) <= tol * scale and abs(sc[4] - ref[4]) <= tol * scale, (NAMES[r.rid], sc, ref)
         mscale = max(np.abs(ref[5:8]).max(), 1e-300)
         assert np.abs(sc[5:8] - ref[5:8]).max() <= tol * mscale + 1e-300, (NAMES[r.rid], sc, ref)
+
+
+@pytest.mark.parametrize("graph", [False, "lazy"])
+def test_churn_of_admissions_and_removals(graph):
+    """Admissions, removals and launch-list changes are ordered on the device
+    stream behind queued steps and never wait for them.  Twelve requests over
+    a pool that holds two or three runs at a time, two steps per round, so
+    slots, granules and instance ids are reused while earlier rounds may still
+    be queued: every run equals the same run alone (FP64, 1e-12 normwise per
+    group; 1e-9 for the water column's pressure and energy)."""
+    mpm = load_package()
+    NAMES, STEPS = CASES["fp64"]
+    probs = [mpm.read_deck(deck_path(n)) for n in NAMES]
+    solo = {}
+    for k, p in enumerate(probs):
+        for s in (3, 5, 6, 8, 10):
+            e = mpm.Engine(max_particles=pad(p.n), max_nodes=pad(p.nb_node), max_instances=1, precision="fp64")
+            i = e.add(p)
+            e.step(s)
+            solo[(k, s)] = e.particles(i)
+            e.close()
+    two = sorted(pad(p.n) for p in probs)[-2:]
+    eng = mpm.Engine(max_particles=sum(two) + 512, max_nodes=sum(pad(p.nb_node) for p in probs), max_instances=3,
+                     precision="fp64")
+    srv = mpm.serve.Server(eng, chunk=2, graph=graph, keep_state=True)
+    plan = [(k % 4, (3, 5, 8, 10)[(k + k // 4) % 4]) for k in range(12)]
+    plan = [(k, min(s, 6) if NAMES[k] == "dam_null_eos4_musl" else s) for k, s in plan]
+    for r, (k, s) in enumerate(plan):
+        srv.submit(mpm.serve.Request(r, probs[k], s, arrival=0.0, kind=NAMES[k]))
+    clock = [0.0]
+    rounds = 0
+    while not srv.idle():
+        clock[0] += 1.0
+        srv.admit(clock[0])
+        srv.advance()
+        srv.retire(lambda: clock[0])
+        rounds += 1
+        assert rounds < 400
+    eng.close()
+    assert sorted(r.rid for r in srv.done) == list(range(12))
+    assert len({r.admitted for r in srv.done}) >= 4
+    for r in srv.done:
+        k, s = plan[r.rid]
+        assert r.final_step == s
+        ref = solo.get((k, s))
+        if ref is None:
+            continue
+        errs = group_errors(r.result["particles"], ref, mpm.PF_NAMES, PGROUPS)
+        loose = {"stress": 1e-9, "ie": 1e-9} if NAMES[k] == "dam_null_eos4_musl" else {}
+        for g, e in errs.items():
+            assert e <= loose.get(g, 1e-12), (NAMES[k], s, g, e)
