@@ -1332,7 +1332,7 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
       const T v = v0[d];
       vn[d] = (ts.istep == 1) ? v + ax[d] * tdt * (T)0.5 : v + ax[d] * tdt;
       p.t(T_VX + d) = vn[d];
-      p.t(T_BX + d) = vx[d];
+      if (!(MODE & G2P_MOVE)) p.t(T_BX + d) = vx[d];   // XX = Xp + vbar DT, pending only without MOVE
     }
     if (MODE & G2P_MOVE) {   // USF: Xp = XX = Xp + vx*DT
       p.d(D_X) = x + (double)vx[0] * dt;
@@ -1386,14 +1386,19 @@ struct Cst {
   T seqv, epeff, sigy, ieinc, den0, den, vol0, cp, mp;
 };
 
+// material models a kernel instance may meet (MM: bit 1 << model type)
+__host__ __device__ constexpr bool mm_has(int MM, int type) { return ((MM >> type) & 1) != 0; }
+
 // the particle state the constitutive update reads (loaded ahead of the
-// strain-rate gather so that the loads overlap the block's other latencies)
-template <typename T>
+// strain-rate gather so that the loads overlap the block's other latencies).
+// sig_y is read only by pla1/pla2 and Seqv only kept by the null material
+// (the others recompute it), so the loads depend on the models MM.
+template <typename T, int MM>
 __device__ __forceinline__ void cst_load(const PV<T>& p, Cst<T>& m) {
   m.vold = p.d(D_VOL);
   m.epeff = p.t(T_EPEFF);
-  m.sigy = p.t(T_SIGY);
-  m.seqv = p.t(T_SEQV);
+  m.sigy = (mm_has(MM, MPM_MAT_PLA1) || mm_has(MM, MPM_MAT_PLA2)) ? p.t(T_SIGY) : (T)0;
+  m.seqv = mm_has(MM, MPM_MAT_NULL) ? p.t(T_SEQV) : (T)0;
   m.iener = p.d(D_IE);
   m.mp = p.t(T_MASS);
   m.cp = p.t(T_CP);
@@ -1686,10 +1691,9 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
   p.t(T_SEQV) = m.seqv;
 #pragma unroll
   for (int k = 0; k < 6; k++) p.t(T_SD0 + k) = m.sd[k];
-  p.t(T_SIGY) = m.sigy;
+  if (mm_has(MM, MPM_MAT_PLA2)) p.t(T_SIGY) = m.sigy;   // only pla2 hardens; cp never changes
   p.t(T_EPEFF) = m.epeff;
   p.d(D_IE) = m.iener;
-  p.t(T_CP) = m.cp;
   p.t(T_Q) = m.bqf;
 }
 
@@ -1768,7 +1772,7 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
     c = p.n(I_CELL);
     x = p.d(D_X); y = p.d(D_Y); z = p.d(D_Z);
     body = p.n(I_BODY);
-    cst_load(p, cs);
+    cst_load<T, MM>(p, cs);
   }
   const bool act = c >= 0;
   int cx = 0, cy = 0, cz = 0;
@@ -2303,7 +2307,7 @@ __global__ void __launch_bounds__(256) k_sgmp_stress(PoolPtrs<T> P, const int* g
         for (int b = 0; b < 3; b++) Lg[a][b] = Lg[a][b] + shp[n] * aux[(AX_DF + 3 * a + b) * nn + c];
     }
     Cst<T> cs;
-    cst_load(p, cs);
+    cst_load<T, MM>(p, cs);
     const MatC& mc = P.mats[P.bodies[p.n(I_BODY)].mat];
     stress_tail<T, MM>(P, p, I, ts, de, vort, Lg, (I.flags & IF_UVFP) != 0, x, y, z, (MODE & ST_MOVE) != 0, mc, cs);
   }
@@ -2373,7 +2377,7 @@ __global__ void __launch_bounds__(256) k_sgmp_g2p(PoolPtrs<T> P, const int* glis
     for (int d = 0; d < 3; d++) {
       const T v = p.t(T_VX + d);
       p.t(T_VX + d) = (ts.istep == 1) ? v + ax[d] * tdt * (T)0.5 : v + ax[d] * tdt;
-      p.t(T_BX + d) = vx[d];
+      if (!(MODE & G2P_MOVE)) p.t(T_BX + d) = vx[d];   // XX = Xp + vbar DT, pending only without MOVE
     }
     if (MODE & G2P_MOVE) {
       p.d(D_X) = x + (double)vx[0] * dt;
