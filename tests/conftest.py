@@ -33,7 +33,9 @@ if ORACLE_DIR not in sys.path:
     sys.path.insert(0, ORACLE_DIR)
 
 DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
-         "plate_neoh_musl_qulo", "dam_null_eos4_musl"]
+         "plate_neoh_musl_qulo", "dam_null_eos4_musl",
+         # the remaining shape x scheme pairs of the fused kernels
+         "cube_elastic_linear_usf", "bar_j2_gimp_usf", "slope_dp_bspline_musl", "slope_dp_bspline_usl"]
 # auxiliary-grid (SGMP) decks: clamped plate (MUSL), water column with
 # symmetric walls (MUSL), J2 block on a clamped floor (USF)
 SGMP_DECKS = ["plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf"]
