@@ -169,9 +169,9 @@ def test_fp32_bulk_within_1e5_of_fp64_oracle(mpm, oracle_mod, name):
 
 
 def test_batched_engine_matches_solo_runs(mpm):
-    """All non-SGMP decks (three shape families x three schemes, seven
-    launch groups) in one engine: each instance equals its solo context run
-    (to FP64 round-off)."""
+    """All twelve decks (four shape families, every scheme, SGMP included)
+    in one engine: each instance equals its solo context run (to FP64
+    round-off)."""
     probs = {n: mpm.read_deck(deck_path(n)) for n in DECKS}
     eng = mpm.Engine(max_particles=sum(p.n for p in probs.values()) + 16 * 256,
                      max_nodes=sum(p.nb_node for p in probs.values()) + 16 * 256, max_instances=16)
