@@ -781,7 +781,7 @@ __host__ __device__ constexpr int run_groups() { return (NV + 16 / (int)sizeof(T
 
 template <typename T, int NV>
 __host__ __device__ constexpr size_t run_stage_bytes(int bs) {
-  return (size_t)run_groups<T, NV>() * run_slots(bs) * 16 + (size_t)(2 * bs + 4) * sizeof(int);
+  return (size_t)run_groups<T, NV>() * run_slots(bs) * 16 + (size_t)(3 * bs + 4) * sizeof(int);
 }
 
 template <typename T, int NV>
@@ -790,7 +790,8 @@ struct RunStage {
   static constexpr int GV = V16<T>::n;
   typename V16<T>::type* g;
   int* key;
-  int* head;
+  int* head;   // first particle of every run (compacted)
+  int* hend;   // one past its last particle
   int* nh;
   int slots, n;
   __device__ RunStage(char* mem, int bs) {
@@ -799,7 +800,8 @@ struct RunStage {
     g = (typename V16<T>::type*)mem;
     key = (int*)(mem + (size_t)NG * slots * 16);
     head = key + bs;
-    nh = head + bs;
+    hend = head + bs;
+    nh = hend + bs;
   }
   __device__ void put(int j, const T (&v)[NG * GV]) const {
     const int s = j + (j >> 3);
@@ -832,74 +834,135 @@ __device__ __forceinline__ void run_rec_slots(const Slots<T>& S, T (&v)[NVV]) {
 
 // after the stage (keys and records) is written and the block synchronised:
 // compact the run heads (first particle of every maximal run of equal keys)
-__device__ __forceinline__ void run_heads(const int* key, int* head, int* nh, int n) {
-  const int j = threadIdx.x;
-  const bool h = j < n && key[j] >= 0 && (j == 0 || key[j - 1] != key[j]);
-  const unsigned long long m = __ballot(h);
+// with their run ends.  A run's end is its first tail (last particle of a
+// run) at or after the head in the wave's tail mask; only a run that reaches
+// past the end of the wave scans the keys beyond it.
+template <typename T, int NV>
+__device__ __forceinline__ void run_heads(const RunStage<T, NV>& R) {
+  const int j = threadIdx.x, n = R.n;
+  const int kj = j < n ? R.key[j] : -1;
+  const int kp = (j > 0 && j < n) ? R.key[j - 1] : -1;
+  const int kn = j + 1 < n ? R.key[j + 1] : -1;
+  const bool h = kj >= 0 && (j == 0 || kp != kj);
+  const bool t = kj >= 0 && kn != kj;
+  const unsigned long long m = __ballot(h), tm = __ballot(t);
   const int lane = threadIdx.x & 63;
   int base = 0;
-  if (lane == 0 && m) base = atomicAdd(nh, __popcll(m));
+  if (lane == 0 && m) base = atomicAdd(R.nh, __popcll(m));
   base = __shfl(base, 0);
-  if (h) head[base + __popcll(m & ((1ull << lane) - 1))] = j;
+  if (h) {
+    const unsigned long long rest = tm >> lane;
+    int end;
+    if (rest) {
+      end = j + __ffsll((long long)rest);   // one past the tail
+    } else {
+      end = j - lane + 64;
+      while (end < n && R.key[end] == kj) end++;
+    }
+    const int q = base + __popcll(m & ((1ull << lane) - 1));
+    R.head[q] = j;
+    R.hend[q] = end;
+  }
 }
 
-// Every (run, slab) pair is one work item: a thread sums one slab (the NS^2
-// nodes of one stencil layer c) over the particles of one run, reading each
-// particle's record once per slab (the next record is read while the
-// current one is summed), then adds the sums to the tile.
-template <typename T, int NV>
-__device__ __forceinline__ int run_span(const RunStage<T, NV>& R, int j0, int& e0) {
-  e0 = R.key[j0];
-  int j1 = j0 + 1;
-  while (j1 < R.n && R.key[j1] == e0) j1++;
-  return j1;
+// A work item is a (run, slab, part) triple: a thread sums one slab (the
+// NS^2 nodes of one stencil layer c) over one part of the particles of one
+// run, reading each particle's record once per slab, then adds the sums to
+// the tile.  A block has few runs (one per occupied cell), so each run is
+// cut into as many parts as keep the item count within the block: the work
+// spreads over every wave of the block instead of the first one or two, and
+// each thread's loop is that much shorter.
+struct RunItem {
+  int j0, j1, e0, c;   // particle range [j0, j1) of the part, run key, slab
+  int run0;            // first particle of the whole run (its box side)
+};
+
+// number of parts per run for nh runs of NS slabs
+template <int NS>
+__device__ __forceinline__ int run_parts(int nh) {
+  const int per = nh * NS;
+  return per > 0 ? max(1, (int)blockDim.x / per) : 1;
 }
 
-template <int NS, typename T>
-__device__ __forceinline__ T pick(const T* v, int c) {   // v[c] for a runtime c < NS, kept in registers
-  return NS == 2 ? (c ? v[1] : v[0]) : (c == 0 ? v[0] : (c == 1 ? v[1] : v[2]));
+template <int NS, typename T, int NV>
+__device__ __forceinline__ RunItem run_item(const RunStage<T, NV>& R, int nh, int parts, int item) {
+  RunItem it;
+  const int k = item % nh, rest = item / nh;
+  it.c = rest % NS;
+  const int h = rest / NS;
+  it.run0 = R.head[k];
+  it.e0 = R.key[it.run0];
+  const int len = R.hend[k] - it.run0;
+  it.j0 = it.run0 + (len * h) / parts;
+  it.j1 = it.run0 + (len * (h + 1)) / parts;
+  return it;
 }
 
-// run sums of the mass (FP64) and NQ velocity-weighted mass components
+// weight (and derivative factor) of stencil slot c along one axis of a staged
+// record, without branching on c: B-spline from the local coordinate
+// (bspline_axis), otherwise a register select
+template <int NS, typename T, bool CR>
+__device__ __forceinline__ T slot_w(const T* u, int ax, int c) {
+  if (CR) {
+    const T r = u[ax];
+    const T a = (T)3 - (T)5 * r, b = (T)5 * r - (T)2;
+    const T wo = (T)0.5 * (c == 0 ? a * a : b * b);
+    const T wm = -((T)25 * (r * r)) + (T)25 * r - (T)5.5;
+    return c == 1 ? wm : wo;
+  }
+  const T* w = u + ax * NS;
+  return NS == 2 ? (c ? w[1] : w[0]) : (c == 0 ? w[0] : (c == 1 ? w[1] : w[2]));
+}
+
+template <int NS, typename T, bool CR, int OD>
+__device__ __forceinline__ T slot_d(const T* u, T idc, int ax, int c) {
+  if (CR) {
+    const T f02 = (T)(double)0.2f, r = u[ax];
+    const T dm = (-((T)50 * r) + (T)25) * f02 * idc;
+    const T t = c == 0 ? (T)5 * r - (T)3 : (T)5 * r - (T)2;
+    const T dout = t * (T)5 * f02 * idc;
+    return c == 1 ? dm : dout;
+  }
+  const T* d = u + OD + ax * NS;
+  return NS == 2 ? (c ? d[1] : d[0]) : (c == 0 ? d[0] : (c == 1 ? d[1] : d[2]));
+}
+
+// run sums of the mass and NQ velocity-weighted mass components; a part's
+// mass sum is kept in T (a few terms) and added to the FP64 tile
 template <int NS, typename T, int NV, int OM, bool MASS, int NQ, bool CR, typename A>
 __device__ __forceinline__ void run_scatter_w(const RunStage<T, NV>& R, const Box2& B, T idc, double* tm, A* tv,
                                               int cap) {
   constexpr int NVV = RunStage<T, NV>::NG * RunStage<T, NV>::GV;
   const int nh = *R.nh;
-  for (int item = threadIdx.x; item < nh * NS; item += blockDim.x) {
-    const int k = item % nh, c = item / nh;
-    const int j0 = R.head[k];
-    int e0;
-    const int j1 = run_span(R, j0, e0);
-    const Box Bk = B.get(B.side(j0));
+  const int parts = run_parts<NS>(nh);
+  for (int item = threadIdx.x; item < nh * NS * parts; item += blockDim.x) {
+    const RunItem it = run_item<NS>(R, nh, parts, item);
+    if (it.j0 >= it.j1) continue;
+    const Box Bk = B.get(B.side(it.run0));
     const int sb = Bk.bx, sc = Bk.bx * Bk.by;
-    double am[NS][NS];
+    T am[NS][NS];
     T aq[NS][NS][NQ > 0 ? NQ : 1];
 #pragma unroll
     for (int b = 0; b < NS; b++)
 #pragma unroll
       for (int a = 0; a < NS; a++) {
-        am[b][a] = 0.0;
+        am[b][a] = (T)0;
 #pragma unroll
         for (int f = 0; f < NQ; f++) aq[b][a][f] = (T)0;
       }
-    T v[NVV];
-    R.get(j0, v);
-    for (int j = j0; j < j1; j++) {
+    for (int j = it.j0; j < it.j1; j++) {
       T u[NVV];
-#pragma unroll
-      for (int q = 0; q < NVV; q++) u[q] = v[q];
-      if (j + 1 < j1) R.get(j + 1, v);
+      R.get(j, u);
       T w[3][NS], d[3][NS];
       run_rec_weights<NS, T, CR, false, 0>(u, idc, w, d);
-      const T m = u[OM], w2 = pick<NS>(w[2], c);
+      const T m = u[OM], w2 = slot_w<NS, T, CR>(u, 2, it.c);
 #pragma unroll
       for (int b = 0; b < NS; b++) {
         const T mw = m * (w[1][b] * w2);
 #pragma unroll
         for (int a = 0; a < NS; a++) {
           const T shm = w[0][a] * mw;
-          if (MASS) am[b][a] += (double)shm;
+          if (MASS) am[b][a] += shm;
 #pragma unroll
           for (int f = 0; f < NQ; f++) aq[b][a][f] += u[OM + 1 + f] * shm;
         }
@@ -909,8 +972,8 @@ __device__ __forceinline__ void run_scatter_w(const RunStage<T, NV>& R, const Bo
     for (int b = 0; b < NS; b++)
 #pragma unroll
       for (int a = 0; a < NS; a++) {
-        const int e = e0 + a + b * sb + c * sc;
-        if (MASS) atomicAdd(&tm[e], am[b][a]);
+        const int e = it.e0 + a + b * sb + it.c * sc;
+        if (MASS) atomicAdd(&tm[e], (double)am[b][a]);
 #pragma unroll
         for (int f = 0; f < NQ; f++) atomicAdd(&tv[f * cap + e], (A)aq[b][a][f]);
       }
@@ -922,12 +985,12 @@ template <int NS, typename T, int NV, int OD, int OS, bool CR, typename A>
 __device__ __forceinline__ void run_scatter_force(const RunStage<T, NV>& R, const Box2& B, T idc, A* tv, int cap) {
   constexpr int NVV = RunStage<T, NV>::NG * RunStage<T, NV>::GV;
   const int nh = *R.nh;
-  for (int item = threadIdx.x; item < nh * NS; item += blockDim.x) {
-    const int k = item % nh, c = item / nh;
-    const int j0 = R.head[k];
-    int e0;
-    const int j1 = run_span(R, j0, e0);
-    const Box Bk = B.get(B.side(j0));
+  const int parts = run_parts<NS>(nh);
+  for (int item = threadIdx.x; item < nh * NS * parts; item += blockDim.x) {
+    const RunItem it = run_item<NS>(R, nh, parts, item);
+    if (it.j0 >= it.j1) continue;
+    const int c = it.c;
+    const Box Bk = B.get(B.side(it.run0));
     const int sb = Bk.bx, sc = Bk.bx * Bk.by;
     T af[NS][NS][3];
 #pragma unroll
@@ -936,19 +999,15 @@ __device__ __forceinline__ void run_scatter_force(const RunStage<T, NV>& R, cons
       for (int a = 0; a < NS; a++)
 #pragma unroll
         for (int i = 0; i < 3; i++) af[b][a][i] = (T)0;
-    T v[NVV];
-    R.get(j0, v);
-    for (int j = j0; j < j1; j++) {
+    for (int j = it.j0; j < it.j1; j++) {
       T u[NVV];
-#pragma unroll
-      for (int q = 0; q < NVV; q++) u[q] = v[q];
-      if (j + 1 < j1) R.get(j + 1, v);
+      R.get(j, u);
       const T sv[3][3] = {{u[OS + 0], u[OS + 3], u[OS + 5]},
                           {u[OS + 3], u[OS + 1], u[OS + 4]},
                           {u[OS + 5], u[OS + 4], u[OS + 2]}};
       T w[3][NS], d[3][NS];
       run_rec_weights<NS, T, CR, true, OD>(u, idc, w, d);
-      const T w2 = pick<NS>(w[2], c), d2 = pick<NS>(d[2], c);
+      const T w2 = slot_w<NS, T, CR>(u, 2, c), d2 = slot_d<NS, T, CR, OD>(u, idc, 2, c);
 #pragma unroll
       for (int b = 0; b < NS; b++) {
         const T wyz = w[1][b] * w2, dyz = d[1][b] * w2, wdz = w[1][b] * d2;
@@ -968,7 +1027,7 @@ __device__ __forceinline__ void run_scatter_force(const RunStage<T, NV>& R, cons
     for (int b = 0; b < NS; b++)
 #pragma unroll
       for (int a = 0; a < NS; a++) {
-        const int e = e0 + a + b * sb + c * sc;
+        const int e = it.e0 + a + b * sb + c * sc;
 #pragma unroll
         for (int i = 0; i < 3; i++) atomicAdd(&tv[i * cap + e], (A)af[b][a][i]);
       }
@@ -1073,7 +1132,7 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
     R.key[jl] = key;
     if (threadIdx.x == 0) *R.nh = 0;
     __syncthreads();
-    run_heads(R.key, R.head, R.nh, R.n);
+    run_heads(R);
     __syncthreads();
     const T idc = (T)I.idcell;
     if (MASS || MOM) run_scatter_w<NS, T, RR::NV, RR::M, MASS, MOM ? 3 : 0, CR>(R, B, idc, tm, tf, CAP);
@@ -1746,8 +1805,19 @@ __device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p
 // ParticleStressUpdate (update_step.f90:456-647): strain-rate gather, F
 // update, constitution; optionally Xp = XX (USL/MUSL), the USF force
 // scatter, and the energy partials.
+// Waves per SIMD the stress kernel is compiled for: the fused gather,
+// constitutive update and force scatter need many registers, and the
+// B-spline and linear FP32 kernels (and the B-spline FP64 kernel, one wave
+// fewer) fit the bound with little or no spilling, which lets another block
+// stream its particles while one sums its force runs.  GIMP and linear FP64
+// spill heavily under a bound and are left unbounded.
+template <int SHAPE, typename T>
+__host__ __device__ constexpr int stress_waves() {
+  return SHAPE == MPM_SHAPE_GIMP ? 1 : (sizeof(T) == 4 ? 4 : (SHAPE == MPM_SHAPE_BSPLINE ? 3 : 1));
+}
+
 template <int SHAPE, typename T, int MODE, int MM>
-__global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
+__global__ void __launch_bounds__(256, (stress_waves<SHAPE, T>())) k_stress(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
   int gran;
   const long i = part_index(P.ilv, glist, bpg, gran);
   const int ii = ginst[gran];
@@ -1875,7 +1945,7 @@ __global__ void __launch_bounds__(256) k_stress(PoolPtrs<T> P, const int* glist,
       __syncthreads();
     }
     if (fast) {
-      run_heads(R.key, R.head, R.nh, R.n);
+      run_heads(R);
       __syncthreads();
       run_scatter_force<NS, T, RR::NV, RR::D, RR::S, CR>(R, B, (T)I.idcell, ta, CAP);
     } else if (act) {
