@@ -985,11 +985,15 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
     if (G.shape == MPM_SHAPE_SGMP) {
       enqueue_sgmp<T>(e, G, P, L, LS, N);
     } else if (G.scheme == MPM_USF) {
+      // ApplyBoundaryConditions ahead of the stress update and the nodal
+      // velocity/acceleration of ParticlePositionUpdate are formed in the
+      // tile loads of the two gathers (the bc pass and IntegrateMomentum's
+      // V, A stores are not needed: integrate zeroes the fixed directions
+      // of PXg and FXg itself, and V, A are not grid state)
       timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, LS, s); });
-      timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 0, 1, 1, s); });
-      timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_FORCE, G.matmask, P, L, s); });
-      timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
-      timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, G2P_MOVE | G2P_ENERGY, P, L, s); });
+      timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_FORCE | ST_VFLY, G.matmask, P, L, s); });
+      timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 0, s); });
+      timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, G2P_MOVE | G2P_ENERGY | G2P_VAFLY, P, L, s); });
     } else if (G.scheme == MPM_USL) {
       timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE, P, LS, s); });
       timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });

@@ -16,11 +16,13 @@
 #define G2P_MOVE 1    // USF: Xp = XX
 #define G2P_REMAP 2   // MUSL: re-map momentum into p2
 #define G2P_ENERGY 4  // energy partials
+#define G2P_VAFLY 8   // nodal velocity/acceleration formed from PXg, FXg, Mg in the tile load
 
 // k_stress modes
 #define ST_MOVE 1     // USL/MUSL: Xp = XX
 #define ST_FORCE 2    // USF: force scatter with the updated stress
 #define ST_ENERGY 4   // energy partials
+#define ST_VFLY 8     // nodal velocity formed from PXg, Mg and the fixities in the tile load
 
 // SGMP: spreads from the auxiliary nodes to the grid nodes
 #define SGN_MASSMOM 1   // Mg, PXg from Cmg, Cpxg (CutOff test)

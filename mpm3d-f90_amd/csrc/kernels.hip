@@ -561,6 +561,50 @@ __device__ __forceinline__ void load_tile_aos(const InstC& I, const BX& B, T* tf
   }
 }
 
+// Nodal velocity (NF = 3) or velocity and acceleration (NF = 6) formed from
+// the grid sums where a gather needs them, instead of by a node pass:
+// PXg/Mg and FXg/Mg where Mg > CutOff, 0 elsewhere (the values
+// IntegrateMomentum's pass would store, update_step.f90:1194-1239, 1506-1509);
+// with FIX the momentum of fixed directions counts as 0 (ApplyBoundaryConditions,
+// update_step.f90:1795-1830, ahead of the strain-rate gather).  k: pool node.
+// The quotients are formed as products with one FP64 reciprocal of Mg per
+// node (within an FP64 ulp of the quotient).
+template <typename T, int NF, bool FIX>
+__device__ __forceinline__ void node_fly(const PoolPtrs<T>& P, const InstC& I, long k, T* v) {
+  const long nn = P.nn;
+  const double mg = P.mg[k];
+  const bool heavy = mg > I.cutoff;
+  const double rm = heavy ? 1.0 / mg : 0.0;
+  const int fx = FIX ? (int)P.fix[k] : 0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const T pd = P.pg[d * nn + k];
+    v[d] = (fx & (1 << d)) ? (T)0 : (T)((double)pd * rm);
+  }
+  if (NF == 6)
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const T fd = P.fg[d * nn + k];
+      v[3 + d] = (T)((double)fd * rm);
+    }
+}
+
+// node_fly values of a block's node box into the tile, node-major (AOS, the
+// fast-path layout) or field-major with stride cap
+template <typename T, int NF, bool FIX, bool AOS, typename BX>
+__device__ __forceinline__ void load_tile_fly(const PoolPtrs<T>& P, const InstC& I, const BX& B, T* tf, int cap) {
+  constexpr int S = AosTile<T, NF>::S;
+  for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
+    T v[6];
+    node_fly<T, NF, FIX>(P, I, (long)I.node_off + B.node(I, e), v);
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      if (AOS) tf[e * S + f] = v[f];
+      else tf[f * cap + e] = v[f];
+    }
+  }
+}
+
 template <typename T, int NF>
 __device__ __forceinline__ void read_node(const T* tf, int e, T v[NF]) {
   constexpr int S = AosTile<T, NF>::S;
@@ -1308,8 +1352,15 @@ __device__ __forceinline__ void flush_tile3(const InstC& I, const BX& B, const A
 // ParticlePositionUpdate (update_step.f90:1434-1522), optionally fused with
 // the MUSL re-map of momentum (GridMomentumMUSL, :1705-1758) into p2 and
 // with the energy partials (when it is the last particle pass).
+// Waves per SIMD the particle-update kernel is compiled for: the FP32
+// GIMP and B-spline kernels fit six waves without spilling
+template <int SHAPE, typename T>
+__host__ __device__ constexpr int g2p_waves() {
+  return sizeof(T) == 4 && SHAPE != MPM_SHAPE_LINEAR ? 6 : 1;
+}
+
 template <int SHAPE, typename T, int MODE>
-__global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
+__global__ void __launch_bounds__(256, (g2p_waves<SHAPE, T>())) k_g2p(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
   int gran;
   const long i = part_index(P.ilv, glist, bpg, gran);
   const int ii = ginst[gran];
@@ -1347,12 +1398,18 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
   const Box Bk = B.get(kb);
   const int sb = Bk.bx, sc = Bk.bx * Bk.by;
   const long nn = P.nn;
+  constexpr bool FLY = (MODE & G2P_VAFLY) != 0;
   const T* Vg = P.vg + I.node_off;
   const T* Ag = P.ag + I.node_off;
   if (B.ok && B.n) {
     const T* srcs[2] = {Vg, Ag};
-    if (fast) load_tile_aos<T, 6>(I, B, tf, srcs, nn);
-    else load_tile<T, 6>(I, B, tf, CAP, srcs, nn);
+    if (FLY) {
+      if (fast) load_tile_fly<T, 6, false, true>(P, I, B, tf, CAP);
+      else load_tile_fly<T, 6, false, false>(P, I, B, tf, CAP);
+    } else {
+      if (fast) load_tile_aos<T, 6>(I, B, tf, srcs, nn);
+      else load_tile<T, 6>(I, B, tf, CAP, srcs, nn);
+    }
     __syncthreads();
   }
   double ev[6] = {0, 0, 0, 0, 0, 0};
@@ -1375,10 +1432,12 @@ __global__ void __launch_bounds__(256) k_g2p(PoolPtrs<T> P, const int* glist, co
       build_stencil<SHAPE, T>(I, x, y, z, c, S);
       for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) __attribute__((always_inline)) {
         const int e = B.ok ? B.idx(nx, ny, nz) : -1;
+        T g[6];
+        if (FLY && e < 0) node_fly<T, 6, false>(P, I, (long)I.node_off + k, g);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-          const T v = e >= 0 ? tf[d * CAP + e] : Vg[d * nn + k];
-          const T a = e >= 0 ? tf[(3 + d) * CAP + e] : Ag[d * nn + k];
+          const T v = e >= 0 ? tf[d * CAP + e] : (FLY ? g[d] : Vg[d * nn + k]);
+          const T a = e >= 0 ? tf[(3 + d) * CAP + e] : (FLY ? g[3 + d] : Ag[d * nn + k]);
           vx[d] = vx[d] + N * v;
           ax[d] = ax[d] + N * a;
         }
@@ -1854,11 +1913,17 @@ __global__ void __launch_bounds__(256, (stress_waves<SHAPE, T>())) k_stress(Pool
   const Box Bk = B.get(kb);
   const int sb = Bk.bx, sc = Bk.bx * Bk.by;
   const long nn = P.nn;
+  constexpr bool FLY = (MODE & ST_VFLY) != 0;
   const T* Vg = P.vg + I.node_off;
   if (B.ok && B.n) {
     const T* srcs[1] = {Vg};
-    if (fast) load_tile_aos<T, 3>(I, B, tf, srcs, nn);
-    else load_tile<T, 3>(I, B, tf, CAP, srcs, nn);
+    if (FLY) {
+      if (fast) load_tile_fly<T, 3, true, true>(P, I, B, tf, CAP);
+      else load_tile_fly<T, 3, true, false>(P, I, B, tf, CAP);
+    } else {
+      if (fast) load_tile_aos<T, 3>(I, B, tf, srcs, nn);
+      else load_tile<T, 3>(I, B, tf, CAP, srcs, nn);
+    }
     __syncthreads();
   }
   double ev[6] = {0, 0, 0, 0, 0, 0};
@@ -1885,9 +1950,11 @@ __global__ void __launch_bounds__(256, (stress_waves<SHAPE, T>())) k_stress(Pool
     build_stencil<SHAPE, T>(I, x, y, z, c, S);
     for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T, T dX, T dY, T dZ) __attribute__((always_inline)) {
       const int e = B.ok ? B.idx(nx, ny, nz) : -1;
-      const T v0 = e >= 0 ? tf[e] : Vg[k];
-      const T v1 = e >= 0 ? tf[CAP + e] : Vg[nn + k];
-      const T v2 = e >= 0 ? tf[2 * CAP + e] : Vg[2 * nn + k];
+      T g[3];
+      if (FLY && e < 0) node_fly<T, 3, true>(P, I, (long)I.node_off + k, g);
+      const T v0 = e >= 0 ? tf[e] : (FLY ? g[0] : Vg[k]);
+      const T v1 = e >= 0 ? tf[CAP + e] : (FLY ? g[1] : Vg[nn + k]);
+      const T v2 = e >= 0 ? tf[2 * CAP + e] : (FLY ? g[2] : Vg[2 * nn + k]);
       de[0] = de[0] + dX * v0;
       de[1] = de[1] + dY * v1;
       de[2] = de[2] + dZ * v2;
@@ -2777,7 +2844,7 @@ void launch_g2p_s(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t 
   switch (mode) {
     G2P_CASE(0)
     G2P_CASE(G2P_MOVE)
-    G2P_CASE(G2P_MOVE | G2P_ENERGY)
+    G2P_CASE(G2P_MOVE | G2P_ENERGY | G2P_VAFLY)
     G2P_CASE(G2P_REMAP)
     default: break;
   }
@@ -2790,7 +2857,7 @@ template <int SHAPE, typename T, int MODE>
 void launch_stress_m(int mm, const PoolPtrs<T>& P, const LaunchP& L0, hipStream_t s) {
   constexpr int NV = RunRec<NSlot<SHAPE>::n, false, true, SHAPE == MPM_SHAPE_BSPLINE>::NV;
   // only the force-scatter mode has no per-block energy partials to keep in step with L0
-  const LaunchP L = (MODE == ST_FORCE)
+  const LaunchP L = ((MODE & ~ST_VFLY) == ST_FORCE)
                         ? fit_lds(L0, (size_t)TILE_NODES * (4 * sizeof(T) > 24 ? 4 * sizeof(T) : 24),
                                   [](int bs) { return run_stage_bytes<T, NV>(bs); })
                         : L0;
@@ -2814,7 +2881,7 @@ template <int SHAPE, typename T>
 void launch_stress_s(int mode, int mm, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
   dim3 g(L.nblk), b(L.bs);
   switch (mode) {
-    case ST_FORCE: launch_stress_m<SHAPE, T, ST_FORCE>(mm, P, L, s); break;
+    case ST_FORCE | ST_VFLY: launch_stress_m<SHAPE, T, ST_FORCE | ST_VFLY>(mm, P, L, s); break;
     case ST_MOVE | ST_ENERGY: launch_stress_m<SHAPE, T, ST_MOVE | ST_ENERGY>(mm, P, L, s); break;
     case ST_MOVE: hipLaunchKernelGGL((k_stress<SHAPE, T, ST_MOVE, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
     case 0: hipLaunchKernelGGL((k_stress<SHAPE, T, 0, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
