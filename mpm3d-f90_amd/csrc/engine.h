@@ -10,6 +10,10 @@
 #include <cstdint>
 
 #define PGRAN 256
+// node-box cache of the particle kernels: one entry per 64 particle slots
+#define BOX_INTS 24
+#define BOX_LOAD 1
+#define BOX_STORE 2
 #define NGRAN 256
 
 // per-instance constants (device memory, read through the scalar cache)
@@ -121,4 +125,6 @@ struct PoolPtrs {
   double* part;  // energy partials: 6 per particle block
   int ilv;       // interleaved lane -> particle mapping in particle kernels
   int tile_cap;  // nodes a block's LDS tile may hold (<= TILE_NODES; smaller values exercise the fallbacks)
+  int* boxes;    // node-box cache of the particle kernels: BOX_INTS ints per 64 particle slots
+  int box_mode;  // BOX_LOAD | BOX_STORE
 };

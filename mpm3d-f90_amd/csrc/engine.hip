@@ -198,6 +198,8 @@ struct mpm_engine {
   int bs_p = 128, bs_n = 256;     // block sizes (autotuned)
   int bs_s = 256;                 // block size of the particle-to-grid scatter launches
   int tile_cap = 512;             // nodes per LDS tile of the particle kernels (TILE_NODES)
+  int* boxes = nullptr;           // node-box cache of the particle kernels (BOX_INTS per 64 slots)
+  int box_mode = BOX_LOAD | BOX_STORE;
   int ilv = 1;                    // interleaved lane mapping in particle kernels
   int sort_every = 20;            // steps between re-sorts in mpm_step (0: never)
   int brick = 0;                  // sort order: 0 Hilbert, B > 0 bricks of B^3 cells
@@ -252,6 +254,8 @@ struct mpm_engine {
     P.part = part;
     P.ilv = ilv;
     P.tile_cap = tile_cap;
+    P.boxes = boxes;
+    P.box_mode = box_mode;
     return P;
   }
 };
@@ -264,7 +268,7 @@ static void free_engine(mpm_engine* e) {
   if (e->graph) (void)hipGraphDestroy(e->graph);
   void* bufs[] = {e->t, e->d, e->ii, e->mg, e->pg, e->fg, e->p2, e->vg, e->ag, e->fix, e->d_inst, e->d_ts, e->d_mats,
                   e->d_bodies, e->part, e->d_ginst, e->d_nginst, e->d_ilist_all, e->k_in, e->k_out, e->v_in,
-                  e->v_out, e->cub_tmp, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg};
+                  e->v_out, e->cub_tmp, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg, e->boxes};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& g : e->groups) {
@@ -327,6 +331,7 @@ static mpm_engine* new_engine(long np_slots, long nn_slots, int max_inst, int ma
   e->d_mats = dalloc<MatC>(max_mat);
   e->d_bodies = dalloc<BodyC>(max_body);
   e->part = dalloc<double>((size_t)(e->np / PGRAN) * (PGRAN / 64) * 6);
+  e->boxes = dalloc<int>((size_t)(e->np / 64) * BOX_INTS);
   e->d_ginst = dalloc<int>(e->np / PGRAN);
   e->d_nginst = dalloc<int>(e->nn / NGRAN);
   e->h_ginst.assign(e->np / PGRAN, 0);
@@ -348,6 +353,7 @@ static mpm_engine* new_engine(long np_slots, long nn_slots, int max_inst, int ma
   HIPCHK(hipMemsetAsync(e->p2, 0, 3 * e->nn * e->tsz, e->s));
   HIPCHK(hipMemsetAsync(e->vg, 0, 3 * e->nn * e->tsz, e->s));
   HIPCHK(hipMemsetAsync(e->ag, 0, 3 * e->nn * e->tsz, e->s));
+  HIPCHK(hipMemsetAsync(e->boxes, 0, (size_t)(e->np / 64) * BOX_INTS * sizeof(int), e->s));
   HIPCHK(hipStreamSynchronize(e->s));
   return e.release();
 }
@@ -679,6 +685,8 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   else
     mpmk::launch_admit<double>(e->ptrs<double>(), d_rows, d_pbody, n, pg0 * PGRAN, npg * PGRAN, (int)b0, e->s);
   mpmk::launch_fixity(e->fix, e->d_inst, id, nng * NGRAN, e->s);
+  // the slots' cached node boxes belong to earlier occupants
+  mpmk::launch_clear_ints(e->boxes + pg0 * (PGRAN / 64) * BOX_INTS, npg * (PGRAN / 64) * BOX_INTS, e->s);
   HIPCHK(hipGetLastError());
   e->lists_dirty = true;
   mark_graph_dirty(e);
@@ -1236,6 +1244,8 @@ static void sort_particles(mpm_engine* e) {
     mpmk::launch_permute<double>(e->d_alt, e->d, e->v_out, L, np, D_COUNT, e->s);
     mpmk::launch_permute<int>(e->i_alt, e->ii, e->v_out, L, np, I_COUNT, e->s);
   }
+  // the cached node boxes describe the old particle order
+  mpmk::launch_clear_ints(e->boxes, (e->np / 64) * BOX_INTS, e->s);
   // copy back so the particle arrays keep their addresses (captured graphs stay valid)
   HIPCHK(hipMemcpyAsync(e->t, e->t_alt, (size_t)T_COUNT * np * e->tsz, hipMemcpyDeviceToDevice, e->s));
   HIPCHK(hipMemcpyAsync(e->d, e->d_alt, (size_t)D_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
@@ -1724,8 +1734,9 @@ extern "C" int mpm_engine_create(long max_particles, long max_nodes, int max_ins
 extern "C" int mpm_engine_create_budget(double hbm_bytes, int max_instances, int precision, mpm_engine** out) {
   return guarded([&] {
     if (!out || !(hbm_bytes > 0) || max_instances <= 0) fail(MPM_ERR_ARG, "bad arguments");
-    // split the budget: particles with sort double-buffers, nodes ~ particles/2
-    const double bp = 2.0 * bytes_per_particle(precision) + 16, bn = bytes_per_node(precision);
+    // split the budget: particles with sort double-buffers (and 1.5 B of
+    // node-box cache), nodes ~ particles/2
+    const double bp = 2.0 * bytes_per_particle(precision) + 18, bn = bytes_per_node(precision);
     const long np = (long)(hbm_bytes / (bp + 0.5 * bn));
     const long nn = np / 2;
     // decks built from many particle generators have one body per generator
@@ -1974,7 +1985,7 @@ extern "C" int mpm_engine_capacity(mpm_engine* e, long* free_particles, long* fr
 
 extern "C" long mpm_engine_device_bytes(mpm_engine* e) {
   if (!e) return -1;
-  return (long)(e->np * (2 * bytes_per_particle(e->prec) + 16) + e->nn * bytes_per_node(e->prec) +
+  return (long)(e->np * (2 * bytes_per_particle(e->prec) + 16) + e->np / 64 * BOX_INTS * 4 + e->nn * bytes_per_node(e->prec) +
                 (e->aux ? e->nn * (AUX_COUNT * e->tsz + 8) : 0));
 }
 

@@ -76,6 +76,7 @@ template <typename T>
 void launch_admit(const PoolPtrs<T>& P, const double* ps, const int* pbody, int n, long b0, long pad, int body0,
                   hipStream_t s);
 void launch_fixity(uint8_t* fix, const InstC* inst, int id, long nslots, hipStream_t s);
+void launch_clear_ints(int* a, long n, hipStream_t s);
 template <typename T> void launch_sgmp_p2g(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_sgmp_nodes(int what, const PoolPtrs<T>& P, const LaunchP& N, hipStream_t s);
 template <typename T> void launch_sgmp_cells(int what, const PoolPtrs<T>& P, const LaunchP& N, hipStream_t s);

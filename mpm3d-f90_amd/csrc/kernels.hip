@@ -433,7 +433,7 @@ __device__ __forceinline__ Box block_box(const InstC& I, bool act, int cx, int c
 // exceeds the tile and the block would fall back to global atomics.  Such a
 // block is split at the largest jump between consecutive particles and each
 // side gets a box of its own; the two share the tile (box 1 from slot
-// b[0].n).  A single-box block has split = blockDim.x.
+// b[0].n).  A single-box block has split = INT_MAX.
 struct Box2 {
   Box b[2];
   int split, n, ok, inner;
@@ -458,7 +458,7 @@ __device__ __forceinline__ Box2 block_box2(const InstC& I, bool act, int cx, int
   r.b[0] = block_box<SHAPE>(I, act, cx, cy, cz, cap);
   r.b[1] = r.b[0];
   r.b[1].n = 0;
-  r.split = blockDim.x;
+  r.split = 0x7fffffff;   // every particle on side 0, whatever the block size of a later user
   r.n = r.b[0].n;
   r.ok = r.b[0].ok;
   r.inner = r.b[0].inner;
@@ -494,6 +494,42 @@ __device__ __forceinline__ Box2 block_box2(const InstC& I, bool act, int cx, int
     r.ok = 1;
     r.inner = 1;
   }
+  return r;
+}
+
+// Box of a block through the per-block cache.  With BOX_LOAD the block's
+// cached entry (the box its particles had in an earlier particle kernel of
+// the step, or in an earlier step: cells move by at most one per step) is
+// taken when it is an inner box that holds the stencil of every active
+// particle (of its side) and fits the tile; any such box gives the same
+// tile sums and gathers as the tight one, so a stale entry is either
+// correct or rejected.  Otherwise the box is computed, and with BOX_STORE
+// written back.  blk: the block's first particle slot / 64.
+template <int SHAPE>
+__device__ __forceinline__ Box2 block_box_cached(const InstC& I, bool act, int cx, int cy, int cz, int jl, int cap,
+                                                 int* scratch, int* boxes, int mode, long blk) {
+  static_assert(sizeof(Box2) <= BOX_INTS * sizeof(int), "box cache entry");
+  Box2* entry = (Box2*)(boxes + blk * BOX_INTS);
+  if (mode & BOX_LOAD) {
+    __shared__ int wbad[16];
+    const Box2 C = *entry;   // block-uniform
+    bool bad = !(C.ok && C.inner && C.n <= cap && C.n == C.b[0].n + C.b[1].n);
+    if (!bad && act) {   // the particle's side box is a whole box and holds its stencil
+      const Box Bk = C.get(C.side(jl));
+      constexpr int lo = SHAPE == MPM_SHAPE_LINEAR ? 0 : -1;
+      constexpr int hi = SHAPE == MPM_SHAPE_GIMP ? 2 : 1;
+      bad = Bk.n <= 0 || Bk.n != Bk.bx * Bk.by * Bk.bz || cx + lo < Bk.x0 || cy + lo < Bk.y0 || cz + lo < Bk.z0 ||
+            cx + hi >= Bk.x0 + Bk.bx || cy + hi >= Bk.y0 + Bk.by || cz + hi >= Bk.z0 + Bk.bz;
+    }
+    const unsigned long long m = __ballot(bad);
+    if ((threadIdx.x & 63) == 0) wbad[threadIdx.x >> 6] = m != 0ull;
+    __syncthreads();
+    int any = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) any |= wbad[w];
+    if (!any) return C;
+  }
+  const Box2 r = block_box2<SHAPE>(I, act, cx, cy, cz, jl, cap, scratch);
+  if ((mode & BOX_STORE) && threadIdx.x == 0) *entry = r;
   return r;
 }
 
@@ -1132,7 +1168,8 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
   int cx = 0, cy = 0, cz = 0;
   if (act) cell_coords(I, c, cx, cy, cz);
   const int jl = (int)(i - ((long)gran * PGRAN + (long)(blockIdx.x % bpg) * blockDim.x));
-  const Box2 B = block_box2<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem);
+  const Box2 B = block_box_cached<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem, P.boxes,
+                                         P.box_mode, (i - jl) / 64);
   const long nn = P.nn;
   if (B.ok) {
     for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
@@ -1392,7 +1429,8 @@ __global__ void __launch_bounds__(256, (g2p_waves<SHAPE, T>())) k_g2p(PoolPtrs<T
   int cx = 0, cy = 0, cz = 0;
   if (act) cell_coords(I, c, cx, cy, cz);
   const int jl = (int)(i - ((long)gran * PGRAN + (long)(blockIdx.x % bpg) * blockDim.x));
-  const Box2 B = block_box2<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem);
+  const Box2 B = block_box_cached<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem, P.boxes,
+                                         P.box_mode, (i - jl) / 64);
   const bool fast = B.ok && B.inner;
   const int kb = B.side(jl);
   const Box Bk = B.get(kb);
@@ -1907,7 +1945,8 @@ __global__ void __launch_bounds__(256, (stress_waves<SHAPE, T>())) k_stress(Pool
   int cx = 0, cy = 0, cz = 0;
   if (act) cell_coords(I, c, cx, cy, cz);
   const int jl = (int)(i - ((long)gran * PGRAN + (long)(blockIdx.x % bpg) * blockDim.x));
-  const Box2 B = block_box2<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem);
+  const Box2 B = block_box_cached<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem, P.boxes,
+                                         P.box_mode, (i - jl) / 64);
   const bool fast = B.ok && B.inner;
   const int kb = B.side(jl);
   const Box Bk = B.get(kb);
@@ -3021,6 +3060,18 @@ void launch_admit(const PoolPtrs<T>& P, const double* ps, const int* pbody, int 
 }
 
 #if MPMK_PART != 2
+// zero n ints (box-cache invalidation; a kernel so that it may be captured)
+__global__ void __launch_bounds__(256) k_clear_ints(int* a, long n) {
+  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) a[j] = 0;
+}
+
+void launch_clear_ints(int* a, long n, hipStream_t s) {
+  if (n > 0) {
+    const long nb = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+    hipLaunchKernelGGL(k_clear_ints, dim3((unsigned)nb), dim3(256), 0, s, a, n);
+  }
+}
+
 void launch_fixity(uint8_t* fix, const InstC* inst, int id, long nslots, hipStream_t s) {
   if (nslots > 0)
     hipLaunchKernelGGL(k_fixity, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, fix, inst, id, nslots);

@@ -263,6 +263,31 @@ def test_small_tiles_take_the_split_and_direct_paths(mpm, name, prec, tol):
         eng.close()
 
 
+@pytest.mark.parametrize("name", ["dam_null_eos4_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl"])
+@pytest.mark.parametrize("bp,bsc,tile", [(64, 256, 512), (256, 64, 512), (128, 256, 64), (256, 128, 512)])
+def test_block_sizes_share_the_node_box_cache(mpm, oracle_mod, name, bp, bsc, tile):
+    """The particle kernels keep each block's node box in a cache keyed by the
+    block's first particle slot; the scatter launch and the other particle
+    kernels may run with different block sizes (and so read entries written
+    for other particle ranges), and a re-sort or a smaller tile capacity
+    leaves stale entries.  Whatever the mix, every step equals the oracle to
+    FP64 round-off."""
+    p = mpm.read_deck(deck_path(name))
+    eng = mpm.Engine(max_particles=p.n + 256, max_nodes=p.nb_node + 256, max_instances=1, precision="fp64")
+    i = eng.add(p)
+    eng.apply_tuning(block_particles=bp, block_scatter=bsc, sort_every=3)
+    o = oracle_mod.Oracle(p)
+    loose = TOL64_LOOSE.get(name)
+    for k, n in enumerate((2, 3, 4)):
+        if k == 1:
+            eng.apply_tuning(tile_nodes=tile)
+        eng.step(n)
+        o.step(n)
+        assert_close(perr(mpm, eng.particles(i), o.particles()), 1e-12, loose, "%s %d/%d/%d" % (name, bp, bsc, tile))
+        assert np.array_equal(eng.icell(i), o.icell())
+    eng.close()
+
+
 def oracle_cells(p, ps):
     """InWhichCell (Grid.f90:123-155) of every particle, 1-based, -1 outside."""
     x, y, z = ps[0], ps[1], ps[2]
