@@ -879,11 +879,11 @@ static void run_phase(mpm_engine* e, int ph) {
         break;
       case PH_ENERGY:
         mpmk::launch_energy<T>(P, L, s);
-        mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), L.bpg, FIN_ENERGY, s);
+        mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), FIN_ENERGY, s);
         break;
       case PH_DT:
         mpmk::launch_energy<T>(P, L, s);
-        mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), L.bpg, FIN_DT, s);
+        mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), FIN_DT, s);
         break;
     }
   }
@@ -1015,7 +1015,7 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
       timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_MOVE | ST_ENERGY, G.matmask, P, L, s); });
     }
     timed(e, KI_FINALIZE, [&] {
-      mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), L.bpg, FIN_ENERGY | FIN_DT | fin_flags, s);
+      mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), FIN_ENERGY | FIN_DT | fin_flags, s);
     });
   }
 }
@@ -1583,10 +1583,10 @@ static void initial_energy(mpm_engine* e, int id) {
   L.nblk = (int)h.npg * L.bpg;
   if (e->prec == MPM_PREC_FP32) {
     mpmk::launch_energy<float>(e->ptrs<float>(), L, e->s);
-    mpmk::launch_finalize<float>(e->ptrs<float>(), d_il, 1, L.bpg, FIN_ENERGY, e->s);
+    mpmk::launch_finalize<float>(e->ptrs<float>(), d_il, 1, FIN_ENERGY, e->s);
   } else {
     mpmk::launch_energy<double>(e->ptrs<double>(), L, e->s);
-    mpmk::launch_finalize<double>(e->ptrs<double>(), d_il, 1, L.bpg, FIN_ENERGY, e->s);
+    mpmk::launch_finalize<double>(e->ptrs<double>(), d_il, 1, FIN_ENERGY, e->s);
   }
   HIPCHK(hipGetLastError());
 }

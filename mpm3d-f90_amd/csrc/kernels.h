@@ -67,7 +67,7 @@ template <typename T> void launch_node_va(const PoolPtrs<T>& P, const LaunchP& L
 template <typename T> void launch_zero_nodes(T* a, int ncomp, long stride, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_energy(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
 template <typename T>
-void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int bpg, int what, hipStream_t s);
+void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int what, hipStream_t s);
 void launch_begin_step(InstT* ts, const int* ilist, int n, hipStream_t s);
 template <typename S>
 void launch_permute(S* dst, const S* src, const int* perm, const LaunchP& L, long np, int nfield, hipStream_t s);

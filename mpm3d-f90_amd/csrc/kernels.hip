@@ -296,26 +296,45 @@ __device__ __forceinline__ void external_force(const PoolPtrs<T>& P, const PV<T>
 // ------------------------------------------------- energy block partials
 
 // block-level deterministic reduction of (ke, ie, mx, my, mz, maxv)
+// DPP lane move of a double within a row of 16 lanes (CTRL: quad_perm,
+// row_half_mirror 0x141, row_mirror 0x140); every lane of the wave active
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// sum (or max) of v over the wave, in every lane: quads by DPP (xor 1, xor
+// 2), then the mirrored quad and half-row (each lane already holds its
+// group's value), then the other half-wave rows by lane permutes
+template <bool MAX>
+__device__ __forceinline__ double wave_reduce(double v) {
+  auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+  v = op(v, dpp_f64<0xB1>(v));
+  v = op(v, dpp_f64<0x4E>(v));
+  v = op(v, dpp_f64<0x141>(v));
+  v = op(v, dpp_f64<0x140>(v));
+  v = op(v, __shfl_xor(v, 16));
+  v = op(v, __shfl_xor(v, 32));
+  return v;
+}
+
+// Energy partials of a block: every wave writes the totals of its 64
+// particle slots to its own slot of the partials (granule * PGRAN/64 +
+// first particle of the wave / 64), so no block-wide step is needed;
+// k_finalize sums the PGRAN/64 slots of every granule.
 __device__ __forceinline__ void block_energy(double* part, int gran, int sub, double v[6]) {
-  __shared__ double red[6][16];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
+  for (int q = 0; q < 5; q++) v[q] = wave_reduce<false>(v[q]);
+  v[5] = wave_reduce<true>(v[5]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane < 6) {
+    double mine = v[0];
 #pragma unroll
-    for (int q = 0; q < 5; q++) v[q] += __shfl_xor(v[q], off);
-    v[5] = fmax(v[5], __shfl_xor(v[5], off));
-  }
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if (lane == 0)
-    for (int q = 0; q < 6; q++) red[q][wave] = v[q];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    for (int w = 0; w < nw; w++) {
-      for (int q = 0; q < 5; q++) s[q] += red[q][w];
-      s[5] = fmax(s[5], red[5][w]);
-    }
-    double* o = part + ((long)gran * (PGRAN / 64) + sub) * 6;
-    for (int q = 0; q < 6; q++) o[q] = s[q];
+    for (int q = 1; q < 6; q++) mine = lane == q ? v[q] : mine;
+    part[((long)gran * (PGRAN / 64) + (long)sub * (blockDim.x >> 6) + wave) * 6 + lane] = mine;
   }
 }
 
@@ -2585,7 +2604,8 @@ __global__ void __launch_bounds__(256) k_energy(PoolPtrs<T> P, const int* glist,
 // its block partials in a fixed order; DT from max |v| (update_step.f90:1845-1872);
 // optional end-of-run test (MPM3D.f90:152-158).
 template <typename T>
-__global__ void __launch_bounds__(256) k_finalize(PoolPtrs<T> P, const int* ilist, int bpg, int what) {
+__global__ void __launch_bounds__(256) k_finalize(PoolPtrs<T> P, const int* ilist, int what) {
+  constexpr int bpg = PGRAN / 64;   // energy partial slots per granule (one per 64 particles)
   const int ii = ilist[blockIdx.x];
   const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   InstT& ts = P.ts[ii];
@@ -3042,8 +3062,8 @@ void launch_energy(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
 }
 
 template <typename T>
-void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int bpg, int what, hipStream_t s) {
-  if (ninst) hipLaunchKernelGGL(k_finalize<T>, dim3(ninst), dim3(256), 0, s, P, ilist, bpg, what);
+void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int what, hipStream_t s) {
+  if (ninst) hipLaunchKernelGGL(k_finalize<T>, dim3(ninst), dim3(256), 0, s, P, ilist, what);
 }
 
 template <typename S>
@@ -3098,7 +3118,7 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
   template void launch_node_va<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                \
   template void launch_zero_nodes<T>(T*, int, long, const LaunchP&, hipStream_t);                  \
   template void launch_energy<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                 \
-  template void launch_finalize<T>(const PoolPtrs<T>&, const int*, int, int, int, hipStream_t);    \
+  template void launch_finalize<T>(const PoolPtrs<T>&, const int*, int, int, hipStream_t);         \
   template void launch_permute<T>(T*, const T*, const int*, const LaunchP&, long, int, hipStream_t);  \
   template void launch_sgmp_p2g<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);         \
   template void launch_sgmp_nodes<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);       \
