@@ -62,17 +62,18 @@ def load_package():
 # the USF step (B-spline, Drucker-Prager), by bulk type size t (4 or 8):
 # every field the kernel must read plus every field it must write, once.
 # Particle: x,y,z 8 B each; volume and internal energy 8 B; cell and body
-# ids 4 B; every other particle field t B.  Grid: velocity V, acceleration
-# A, momentum and force 3t B each, mass 8 B.
+# ids 4 B; every other particle field t B.  Grid: momentum and force 3t B
+# each, mass 8 B, fixity 1 B.  The gathers form the nodal velocity (and
+# acceleration) from momentum, force and mass as they load their tiles.
 def kernel_bytes(kid, t, loads=True):
     if kid == "p2g":        # read x, m, v; write icell | write Mg, PXg
         return 24 + t + 3 * t + 4, 8 + 3 * t
     if kid == "stress":     # read x, cell, body, vol, ie, sm, seqv, sd(6), sig_y, epeff, cp, loads
         rd = 24 + 4 + 4 + 8 + 8 + 11 * t + (3 * t if loads else 0)
         wr = 8 + 8 + 12 * t  # write vol, ie, sm, seqv, sd(6), sig_y, epeff, q, cp
-        return rd + wr, 6 * t            # read V | write FXg
-    if kid == "g2p":        # read x, cell, v, m, ie; write v, vbar, x | read V, A
-        return 24 + 4 + 3 * t + t + 8 + 3 * t + 3 * t + 24, 6 * t
+        return rd + wr, 3 * t + 8 + 1 + 3 * t   # read PXg, Mg, fixity | write FXg
+    if kid == "g2p":        # read x, cell, v, m, ie; write v, x | read PXg, FXg, Mg
+        return 24 + 4 + 3 * t + t + 8 + 3 * t + 24, 6 * t + 8
     return 0, 0
 
 
@@ -399,7 +400,7 @@ def run_serving(mpm, decks, a, log):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=64, help="timed steps (default: one re-sort period of the tuned shape)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--ensemble", type=int, default=256, help="instances of the batched slope ensemble")
     ap.add_argument("--seed", type=int, default=0, help="seed of the ensemble generator")
