@@ -204,6 +204,7 @@ struct mpm_engine {
   int sort_every = 20;            // steps between re-sorts in mpm_step (0: never)
   int brick = 0;                  // sort order: 0 Hilbert, B > 0 bricks of B^3 cells
   long steps_done = 0;
+  long since_sort = 1L << 40;     // steps since the last re-sort of the whole pool (none yet: sort first)
   // sort scratch (lazily allocated)
   unsigned* k_in = nullptr;
   unsigned* k_out = nullptr;
@@ -1050,10 +1051,11 @@ static void sort_particles(mpm_engine* e);
 static void step_plain(mpm_engine* e, int nsteps, int fin_flags) {
   rebuild_lists(e);
   for (int k = 0; k < nsteps; k++) {
-    if (e->sort_every > 0 && e->steps_done % e->sort_every == 0) sort_particles(e);
+    if (e->sort_every > 0 && e->since_sort >= e->sort_every) sort_particles(e);
     if (e->prec == MPM_PREC_FP32) enqueue_step<float>(e, fin_flags);
     else enqueue_step<double>(e, fin_flags);
     e->steps_done++;
+    e->since_sort++;
   }
   HIPCHK(hipGetLastError());
   mark_after_step(e);
@@ -1094,11 +1096,11 @@ static void step_graph(mpm_engine* e, int nsteps, int per, int fin_flags, bool l
   int done = 0;
   while (done + per <= nsteps) {
     const long se = e->sort_every;
-    if (se > 0 && (e->steps_done % se == 0 || e->steps_done / se != (e->steps_done + per - 1) / se))
-      sort_particles(e);
+    if (se > 0 && e->since_sort + per > se) sort_particles(e);   // at most one period between re-sorts
     HIPCHK(hipGraphLaunch(e->gexec, e->s));
     done += per;
     e->steps_done += per;
+    e->since_sort += per;
   }
   if (done < nsteps) step_plain(e, nsteps - done, fin_flags);
   mark_after_step(e);
@@ -1188,6 +1190,7 @@ static void sort_instance(mpm_engine* e, int id) {
 
 static void sort_particles(mpm_engine* e) {
   rebuild_lists(e);
+  e->since_sort = 0;
   const long np = e->np;
   ensure_sort_buffers(e);
   if (!e->d_seg) e->d_seg = dalloc<int>(2 * (size_t)e->max_inst);
@@ -1261,7 +1264,7 @@ struct Snapshot {
   double* d = nullptr;
   int* i = nullptr;
   InstT* ts = nullptr;
-  long steps = 0;
+  long steps = 0, since_sort = 0;
   std::vector<std::pair<bool, bool>> flags;
 };
 
@@ -1283,6 +1286,7 @@ static void snap_take(mpm_engine* e, Snapshot& S) {
   HIPCHK(hipMemcpyAsync(S.i, e->ii, (size_t)I_COUNT * np * sizeof(int), hipMemcpyDeviceToDevice, e->s));
   HIPCHK(hipMemcpyAsync(S.ts, e->d_ts, sizeof(InstT) * e->max_inst, hipMemcpyDeviceToDevice, e->s));
   S.steps = e->steps_done;
+  S.since_sort = e->since_sort;
   S.flags.clear();
   for (const HostInst& h : e->inst) S.flags.emplace_back(h.p_in_p2, h.xx_pending);
 }
@@ -1294,6 +1298,7 @@ static void snap_restore(mpm_engine* e, const Snapshot& S) {
   HIPCHK(hipMemcpyAsync(e->ii, S.i, (size_t)I_COUNT * np * sizeof(int), hipMemcpyDeviceToDevice, e->s));
   HIPCHK(hipMemcpyAsync(e->d_ts, S.ts, sizeof(InstT) * e->max_inst, hipMemcpyDeviceToDevice, e->s));
   e->steps_done = S.steps;
+  e->since_sort = S.since_sort;
   for (size_t k = 0; k < e->inst.size() && k < S.flags.size(); k++) {
     e->inst[k].p_in_p2 = S.flags[k].first;
     e->inst[k].xx_pending = S.flags[k].second;
@@ -1454,6 +1459,10 @@ static mpm_tuning autotune(mpm_engine* e, int trial_steps) {
     best.trials = trials + 1;
     snap_restore(e, S);
     apply_tuning(e, best);
+    // the restored particles are in the order of the last sort before
+    // tuning: re-sort in the chosen order now (this also starts a full
+    // re-sort period and clears the node-box cache of the trials)
+    sort_particles(e);
     HIPCHK(hipStreamSynchronize(e->s));
   } catch (...) {
     e->prof = was_prof;
