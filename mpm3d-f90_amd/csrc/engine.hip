@@ -1212,8 +1212,27 @@ static void sort_particles(mpm_engine* e) {
       const HostInst& h = e->inst[id];
       seg.push_back((int)((h.pg0 + h.npg) * PGRAN));
     }
+  // segments in pool order (begins, then ends)
+  {
+    std::vector<std::pair<int, int>> sr(nseg);
+    for (int k = 0; k < nseg; k++) sr[k] = {seg[k], seg[nseg + k]};
+    std::sort(sr.begin(), sr.end());
+    for (int k = 0; k < nseg; k++) {
+      seg[k] = sr[k].first;
+      seg[nseg + k] = sr[k].second;
+    }
+  }
   int end_bit = 1;
   while (end_bit < 32 && (1u << end_bit) <= maxkey) end_bit++;
+  // When the segments tile one slot range without gaps, one device-wide
+  // radix sort of (segment rank, cell key) gives the permutation of the
+  // segmented sort (both are stable), at a fraction of its time: the
+  // segmented sort gives each segment (an instance) little parallelism.
+  bool tiled = true;
+  for (int k = 1; k < nseg; k++) tiled = tiled && seg[k] == seg[nseg + k - 1];
+  int seg_bits = 0;
+  while ((1 << seg_bits) < nseg) seg_bits++;
+  const bool global = nseg > 1 && tiled && end_bit + seg_bits <= 32;
   if (seg != e->h_seg) {
     // the previous upload may still read h_seg: wait before replacing it
     HIPCHK(hipStreamSynchronize(e->s));
@@ -1221,12 +1240,15 @@ static void sort_particles(mpm_engine* e) {
     HIPCHK(hipMemcpyAsync(e->d_seg, e->h_seg.data(), seg.size() * sizeof(int), hipMemcpyHostToDevice, e->s));
   }
   size_t need = 0;
-  const int b0 = seg[0], n0 = seg[nseg] - seg[0];
-  // one instance: plain radix sort of its slot range; several: segmented
+  // one instance, or tiled segments: plain radix sort of the slot range
+  // (the segment rank above the cell key); otherwise segmented
+  const int b0 = seg[0], n0 = (nseg == 1 || global) ? seg[2 * nseg - 1] - seg[0] : 0;
+  const int nbits = global ? end_bit + seg_bits : end_bit;
+  if (global) mpmk::launch_seg_prefix(e->k_in, e->d_seg, nseg, b0, n0, end_bit, e->s);
   auto run_sort = [&](void* tmp, size_t& bytes) {
-    if (nseg == 1)
+    if (nseg == 1 || global)
       return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, e->k_in + b0, e->k_out + b0, e->v_in + b0,
-                                                e->v_out + b0, n0, 0, end_bit, e->s);
+                                                e->v_out + b0, n0, 0, nbits, e->s);
     return hipcub::DeviceSegmentedRadixSort::SortPairs(tmp, bytes, e->k_in, e->k_out, e->v_in, e->v_out, (int)np,
                                                        nseg, e->d_seg, e->d_seg + nseg, 0, end_bit, e->s);
   };

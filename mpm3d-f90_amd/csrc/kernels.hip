@@ -3085,6 +3085,27 @@ __global__ void __launch_bounds__(256) k_clear_ints(int* a, long n) {
   for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) a[j] = 0;
 }
 
+// keys[j] |= (rank of the segment holding slot j) << shift, for j in
+// [b0, b0 + n); begins: the segments' first slots, ascending
+__global__ void __launch_bounds__(256) k_seg_prefix(unsigned* keys, const int* begins, int nseg, long b0, long n,
+                                                    int shift) {
+  const long j = b0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= b0 + n) return;
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (begins[mid] <= j) lo = mid;
+    else hi = mid - 1;
+  }
+  keys[j] |= (unsigned)lo << shift;
+}
+
+void launch_seg_prefix(unsigned* keys, const int* begins, int nseg, long b0, long n, int shift, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_seg_prefix, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keys, begins, nseg, b0, n,
+                       shift);
+}
+
 void launch_clear_ints(int* a, long n, hipStream_t s) {
   if (n > 0) {
     const long nb = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
