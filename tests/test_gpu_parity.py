@@ -342,6 +342,30 @@ def test_hilbert_key_walks_face_neighbours():
     assert np.all(step == 1)
 
 
+@pytest.mark.parametrize("gap", [False, True])
+def test_pool_sort_of_several_instances(mpm, gap):
+    """The pool re-sort orders every instance's slots by its cell key, stably:
+    one device-wide sort of (segment rank, key) when the instances tile the
+    pool, the segmented sort when a removed instance leaves a gap."""
+    names = ("slope_dp_bspline_usf", "bar_j2_gimp_musl", "dam_null_eos4_musl", "cube_elastic_linear_usl")
+    probs = [mpm.read_deck(deck_path(n)) for n in names]
+    eng = mpm.Engine(max_particles=sum(p.n for p in probs) + 16 * 256,
+                     max_nodes=sum(p.nb_node for p in probs) + 16 * 256, max_instances=8)
+    ids = [eng.add(p) for p in probs]
+    if gap:
+        eng.remove(ids[1])
+        probs, ids = [probs[0]] + probs[2:], [ids[0]] + ids[2:]
+    eng.apply_tuning(sort_brick=2)
+    eng.step(3)
+    before = [(eng.order(i), eng.particles(i)) for i in ids]
+    eng.sort()
+    for p, i, (order0, ps) in zip(probs, ids, before):
+        key = brick_key(p, oracle_cells(p, ps), 2)
+        assert np.array_equal(eng.order(i), order0[np.argsort(key[order0], kind="stable")])
+        assert np.array_equal(eng.particles(i), ps)
+    eng.close()
+
+
 def brick_key(p, c, B=4):
     """Sort key of the cell order: B = 0 Hilbert (falls back to B = 4 above
     1024 cells per axis), B > 0 brick-major with B^3-cell bricks."""
@@ -383,9 +407,13 @@ def test_autotune_leaves_the_state_and_picks_valid_shapes(mpm):
     assert 1 <= t["sort_every"] <= 64 and t["trials"] == 18 + 2 + 2 + 1
     assert t["step_ms"] > 0 and t["sort_ms"] > 0
     assert a.tuning()["sort_brick"] == t["sort_brick"]
-    for x, p0 in zip(ia, before):
+    for x, p0, p in zip(ia, before, probs):
         assert a.scalars(x)[0] == 3
         assert np.array_equal(a.particles(x), p0)
+        # the pool is re-sorted in the chosen cell order when tuning ends
+        key = brick_key(p, oracle_cells(p, p0), t["sort_brick"])
+        o = a.order(x)
+        assert np.all(np.diff(key[o]) >= 0)
     a.step(6, graph=True, graph_steps=2)
     b.step(6)
     for x, y, n in zip(ia, ib, names):
