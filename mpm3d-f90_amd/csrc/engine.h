@@ -122,7 +122,7 @@ struct PoolPtrs {
   InstT* ts;
   const MatC* mats;
   const BodyC* bodies;
-  double* part;  // energy partials: 6 per particle block
+  double* part;  // energy partials: 6 per 64 particle slots (one per wave of the particle kernels)
   int ilv;       // interleaved lane -> particle mapping in particle kernels
   int tile_cap;  // nodes a block's LDS tile may hold (<= TILE_NODES; smaller values exercise the fallbacks)
   int* boxes;    // node-box cache of the particle kernels: BOX_INTS ints per 64 particle slots

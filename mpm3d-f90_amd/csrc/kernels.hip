@@ -2915,7 +2915,8 @@ void launch_g2p_s(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t 
 template <int SHAPE, typename T, int MODE>
 void launch_stress_m(int mm, const PoolPtrs<T>& P, const LaunchP& L0, hipStream_t s) {
   constexpr int NV = RunRec<NSlot<SHAPE>::n, false, true, SHAPE == MPM_SHAPE_BSPLINE>::NV;
-  // only the force-scatter mode has no per-block energy partials to keep in step with L0
+  // only the force-scatter mode stages runs in dynamic LDS: it runs with
+  // smaller blocks when the stage does not fit
   const LaunchP L = ((MODE & ~ST_VFLY) == ST_FORCE)
                         ? fit_lds(L0, (size_t)TILE_NODES * (4 * sizeof(T) > 24 ? 4 * sizeof(T) : 24),
                                   [](int bs) { return run_stage_bytes<T, NV>(bs); })
