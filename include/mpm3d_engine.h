@@ -79,7 +79,8 @@ int mpm_engine_get_tuning(mpm_engine* e, mpm_tuning* out);
 int mpm_engine_apply_tuning(mpm_engine* e, const mpm_tuning* t);
 /* Time launch shapes on the resident instances (trial_steps steps each,
  * from a device snapshot of the state that is restored
- * afterwards, so the simulation is unchanged) and apply the fastest. */
+ * afterwards, so the simulation is unchanged), apply the fastest and
+ * re-sort the pool in its cell order (a full re-sort period starts). */
 int mpm_engine_autotune(mpm_engine* e, int trial_steps, mpm_tuning* best);
 
 /* per-instance state (synchronising); layouts as in mpm3d_hip.h */
