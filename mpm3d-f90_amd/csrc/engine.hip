@@ -985,12 +985,7 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
   timed(e, KI_BEGIN, [&] { mpmk::launch_begin_step(e->d_ts, e->d_ilist_all, e->n_active, s); });
   for (Group& G : e->groups) {
     const LaunchP L = plist(e, G), N = nlist(e, G), LS = plist(e, G, e->bs_s);
-    timed(e, KI_ZERO, [&] {
-      mpmk::launch_zero_nodes<double>(e->mg, 1, e->nn, N, s);
-      mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
-      mpmk::launch_zero_nodes<T>((T*)e->fg, 3, e->nn, N, s);
-      if (G.scheme == MPM_MUSL) mpmk::launch_zero_nodes<T>((T*)e->p2, 3, e->nn, N, s);
-    });
+    timed(e, KI_ZERO, [&] { mpmk::launch_zero_grid<T>(P, G.scheme == MPM_MUSL, N, s); });
     if (G.shape == MPM_SHAPE_SGMP) {
       enqueue_sgmp<T>(e, G, P, L, LS, N);
     } else if (G.scheme == MPM_USF) {

@@ -65,6 +65,7 @@ template <typename T>
 void launch_bc(const PoolPtrs<T>& P, const LaunchP& L, int use_p2, int bc, int vel, hipStream_t s);
 template <typename T> void launch_node_va(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_zero_nodes(T* a, int ncomp, long stride, const LaunchP& L, hipStream_t s);
+template <typename T> void launch_zero_grid(const PoolPtrs<T>& P, bool p2, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_energy(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
 template <typename T>
 void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int what, hipStream_t s);

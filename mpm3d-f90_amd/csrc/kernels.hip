@@ -1366,6 +1366,22 @@ __global__ void __launch_bounds__(256) k_node_va(PoolPtrs<T> P, const int* nglis
   }
 }
 
+// zero the grid sums of a step in one pass: Mg, PXg, FXg and, when given,
+// the MUSL re-map buffer (a kernel, so that steps can be graph-captured)
+template <typename T>
+__global__ void __launch_bounds__(256) k_zero_grid(double* mg, T* pg, T* fg, T* p2, long nn, const int* nglist,
+                                                   int bpg) {
+  int gran;
+  const long j = node_index(nglist, bpg, gran);
+  mg[j] = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    pg[d * nn + j] = (T)0;
+    fg[d * nn + j] = (T)0;
+    if (p2) p2[d * nn + j] = (T)0;
+  }
+}
+
 // fill kernel (used instead of hipMemsetAsync so steps can be graph-captured)
 template <typename T>
 __global__ void __launch_bounds__(256) k_zero_nodes(T* a, int ncomp, long stride, const int* nglist, int bpg) {
@@ -3058,6 +3074,13 @@ void launch_zero_nodes(T* a, int ncomp, long stride, const LaunchP& L, hipStream
 }
 
 template <typename T>
+void launch_zero_grid(const PoolPtrs<T>& P, bool p2, const LaunchP& L, hipStream_t s) {
+  if (L.nblk)
+    hipLaunchKernelGGL(k_zero_grid<T>, dim3(L.nblk), dim3(L.bs), 0, s, P.mg, P.pg, P.fg, p2 ? P.p2 : (T*)nullptr,
+                       P.nn, L.glist, L.bpg);
+}
+
+template <typename T>
 void launch_energy(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
   if (L.nblk) hipLaunchKernelGGL(k_energy<T>, dim3(L.nblk), dim3(L.bs), 0, s, P, L.glist, L.ginst, L.bpg);
 }
@@ -3139,6 +3162,7 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
   template void launch_bc<T>(const PoolPtrs<T>&, const LaunchP&, int, int, int, hipStream_t);      \
   template void launch_node_va<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                \
   template void launch_zero_nodes<T>(T*, int, long, const LaunchP&, hipStream_t);                  \
+  template void launch_zero_grid<T>(const PoolPtrs<T>&, bool, const LaunchP&, hipStream_t);        \
   template void launch_energy<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                 \
   template void launch_finalize<T>(const PoolPtrs<T>&, const int*, int, int, hipStream_t);         \
   template void launch_permute<T>(T*, const T*, const int*, const LaunchP&, long, int, hipStream_t);  \
