@@ -60,8 +60,9 @@ int mpm_engine_set_interleave(mpm_engine* e, int enable);
 
 /* Launch-shape settings.  sort_brick selects the cell order of the particle
  * sort ("cell bins" that keep a block's particles inside a compact box of
- * nodes): 0 the 3D Hilbert curve (default), B = 1, 2, 4, 8 brick-major order
- * with B^3-cell bricks.  step_ms / sort_ms / trials are filled by
+ * nodes): 0 the 3D Hilbert curve, B = 1, 2, 4, 8 brick-major order with
+ * B^3-cell bricks.  Defaults: 256-particle and 256-node blocks, 2^3 bricks,
+ * contiguous lanes, 256-particle scatter blocks, a re-sort every 64 steps.  step_ms / sort_ms / trials are filled by
  * mpm_engine_autotune and ignored on input. */
 typedef struct mpm_tuning {
   int32_t block_particles;   /* 64, 128, 256 */

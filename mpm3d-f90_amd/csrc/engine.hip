@@ -195,14 +195,17 @@ struct mpm_engine {
   int* d_ilist_all = nullptr;
   size_t cap_ilist = 0;
   int n_active = 0;
-  int bs_p = 128, bs_n = 256;     // block sizes (autotuned)
+  // Launch shape.  The defaults are what the autotuner picks for the slope
+  // decks (single slope and ensemble, FP32 and FP64) and the Taylor bar:
+  // contexts and the Fortran drop-in, which never autotune, run with them.
+  int bs_p = 256, bs_n = 256;     // block sizes (autotuned)
   int bs_s = 256;                 // block size of the particle-to-grid scatter launches
   int tile_cap = 512;             // nodes per LDS tile of the particle kernels (TILE_NODES)
   int* boxes = nullptr;           // node-box cache of the particle kernels (BOX_INTS per 64 slots)
   int box_mode = BOX_LOAD | BOX_STORE;
-  int ilv = 1;                    // interleaved lane mapping in particle kernels
-  int sort_every = 20;            // steps between re-sorts in mpm_step (0: never)
-  int brick = 0;                  // sort order: 0 Hilbert, B > 0 bricks of B^3 cells
+  int ilv = 0;                    // interleaved lane mapping in particle kernels
+  int sort_every = 64;            // steps between re-sorts in mpm_step (0: never)
+  int brick = 2;                  // sort order: 0 Hilbert, B > 0 bricks of B^3 cells
   long steps_done = 0;
   long since_sort = 1L << 40;     // steps since the last re-sort of the whole pool (none yet: sort first)
   // sort scratch (lazily allocated)
@@ -1631,6 +1634,10 @@ extern "C" int mpm_ctx_create(const mpm_problem* prob, int precision, mpm_ctx** 
     check_problem(prob);
     const long nnode = (long)(prob->ncx + 1) * (prob->ncy + 1) * (prob->ncz + 1);
     mpm_engine* e = new_engine(prob->nb_particle, nnode, 1, prob->nb_mat, prob->nb_body, precision);
+    // a problem below ~1M particles fills the GPU with fewer 256-particle
+    // blocks than it needs to hide latency: 128 (what the autotuner picks
+    // for the 1.3e5-particle Taylor bar; 256 for the 1.3e6-particle slope)
+    if (prob->nb_particle < (1 << 20)) e->bs_p = 128;
     try {
       add_instance(e, prob, 0);
       rebuild_lists(e);
