@@ -33,8 +33,10 @@
 #define SGC_GRAD 1      // strain rate, spin, velocity gradient
 #define SGC_KIN 2       // velocity, acceleration
 // SGMP auxiliary-node fields per cell (the passes alias the storage); the
-// force pass keeps Cfext, Cfint in slots 3..11, behind the momentum slots
-#define AUX_COUNT 18
+// force pass keeps Cfext, Cfint in slots 3..11, behind the momentum slots;
+// the gradient pass stores its 18 values centre-major with a stride of 20
+// (16-byte aligned in both precisions)
+#define AUX_COUNT 20
 #define AUX_FORCE_OFF 3
 
 // all material models (general stress kernel)

@@ -2130,6 +2130,23 @@ __global__ void __launch_bounds__(256, (stress_waves<SHAPE, T>())) k_stress(Pool
 // auxiliary-node field slots (the three passes alias the same storage)
 enum { AX_P = 0, AX_FE = 3, AX_FI = 6 };   // momentum, external and internal force
 enum { AX_CO = 0, AX_CW = 6, AX_DF = 9 };  // strain rate, spin, velocity gradient
+// The gradient pass stores its 18 values centre-major (AUX_COUNT values per
+// centre, the last two padding) so that the particles' 8-centre gathers are
+// a few 16-byte loads per centre instead of one load per field.
+template <typename T>
+__device__ __forceinline__ void grad_store(T* aux, long cell, const T (&v)[AUX_COUNT]) {
+  using VT = typename V16<T>::type;
+  VT* q = (VT*)(aux + cell * AUX_COUNT);
+#pragma unroll
+  for (int g = 0; g < AUX_COUNT / V16<T>::n; g++) q[g] = V16<T>::make(v + g * V16<T>::n);
+}
+template <typename T>
+__device__ __forceinline__ void grad_load(const T* aux, long cell, T (&v)[AUX_COUNT]) {
+  using VT = typename V16<T>::type;
+  const VT* q = (const VT*)(aux + cell * AUX_COUNT);
+#pragma unroll
+  for (int g = 0; g < AUX_COUNT / V16<T>::n; g++) V16<T>::split(q[g], v + g * V16<T>::n);
+}
 enum { AX_V = 0, AX_A = 3 };               // velocity, acceleration
 
 __device__ constexpr int SGN[3][8] = {{-1, 1, 1, -1, -1, 1, 1, -1}, {-1, -1, 1, 1, -1, -1, 1, 1},
@@ -2431,12 +2448,16 @@ __global__ void __launch_bounds__(256) k_sgmp_cells(PoolPtrs<T> P, const int* ng
         df[3 * n + 2] = df[3 * n + 2] + DZ * v[n];
       }
     }
+    T g[AUX_COUNT];
 #pragma unroll
-    for (int q = 0; q < 6; q++) aux[(AX_CO + q) * nn + c] = co[q];
+    for (int q = 0; q < AUX_COUNT; q++) g[q] = (T)0;
 #pragma unroll
-    for (int q = 0; q < 3; q++) aux[(AX_CW + q) * nn + c] = cw[q];
+    for (int q = 0; q < 6; q++) g[AX_CO + q] = co[q];
 #pragma unroll
-    for (int q = 0; q < 9; q++) aux[(AX_DF + q) * nn + c] = df[q];
+    for (int q = 0; q < 3; q++) g[AX_CW + q] = cw[q];
+#pragma unroll
+    for (int q = 0; q < 9; q++) g[AX_DF + q] = df[q];
+    grad_store<T>(P.aux, (long)I.node_off + c, g);
   }
   if (WHAT & SGC_KIN) {
     // a centre on a fixed or symmetric face takes the nodal values at its
@@ -2499,23 +2520,22 @@ __global__ void __launch_bounds__(256) k_sgmp_stress(PoolPtrs<T> P, const int* g
   if (act) {
     T shp[8];
     sgn_shape<T>(L.xxn, shp);
-    const long nn = P.nn;
-    const T* aux = P.aux + I.node_off;
     T de[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, vort[3] = {(T)0, (T)0, (T)0};
     T Lg[3][3] = {{(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}, {(T)0, (T)0, (T)0}};
 #pragma unroll
     for (int n = 0; n < 8; n++) {
       const int k = sgmp_node(I, L, n);
       if (!k) continue;
-      const long c = k - 1;
+      T g[AUX_COUNT];
+      grad_load<T>(P.aux, (long)I.node_off + k - 1, g);
 #pragma unroll
-      for (int q = 0; q < 6; q++) de[q] = de[q] + aux[(AX_CO + q) * nn + c] * shp[n];
+      for (int q = 0; q < 6; q++) de[q] = de[q] + g[AX_CO + q] * shp[n];
 #pragma unroll
-      for (int q = 0; q < 3; q++) vort[q] = vort[q] + aux[(AX_CW + q) * nn + c] * shp[n];
+      for (int q = 0; q < 3; q++) vort[q] = vort[q] + g[AX_CW + q] * shp[n];
 #pragma unroll
       for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++) Lg[a][b] = Lg[a][b] + shp[n] * aux[(AX_DF + 3 * a + b) * nn + c];
+        for (int b = 0; b < 3; b++) Lg[a][b] = Lg[a][b] + shp[n] * g[AX_DF + 3 * a + b];
     }
     Cst<T> cs;
     cst_load<T, MM>(p, cs);
