@@ -2130,23 +2130,26 @@ __global__ void __launch_bounds__(256, (stress_waves<SHAPE, T>())) k_stress(Pool
 // auxiliary-node field slots (the three passes alias the same storage)
 enum { AX_P = 0, AX_FE = 3, AX_FI = 6 };   // momentum, external and internal force
 enum { AX_CO = 0, AX_CW = 6, AX_DF = 9 };  // strain rate, spin, velocity gradient
-// The gradient pass stores its 18 values centre-major (AUX_COUNT values per
-// centre, the last two padding) so that the particles' 8-centre gathers are
-// a few 16-byte loads per centre instead of one load per field.
-template <typename T>
-__device__ __forceinline__ void grad_store(T* aux, long cell, const T (&v)[AUX_COUNT]) {
+// The gradient and kinematics passes store their values centre-major (NV
+// values per centre: AUX_COUNT for the 18 gradient values, 8 for velocity
+// and acceleration, padded so that a centre is a few 16-byte words) so that
+// the particles' 8-centre gathers are a few vector loads per centre instead
+// of one load per field.
+template <typename T, int NV>
+__device__ __forceinline__ void centre_store(T* aux, long cell, const T (&v)[NV]) {
   using VT = typename V16<T>::type;
-  VT* q = (VT*)(aux + cell * AUX_COUNT);
+  VT* q = (VT*)(aux + cell * NV);
 #pragma unroll
-  for (int g = 0; g < AUX_COUNT / V16<T>::n; g++) q[g] = V16<T>::make(v + g * V16<T>::n);
+  for (int g = 0; g < NV / V16<T>::n; g++) q[g] = V16<T>::make(v + g * V16<T>::n);
 }
-template <typename T>
-__device__ __forceinline__ void grad_load(const T* aux, long cell, T (&v)[AUX_COUNT]) {
+template <typename T, int NV>
+__device__ __forceinline__ void centre_load(const T* aux, long cell, T (&v)[NV]) {
   using VT = typename V16<T>::type;
-  const VT* q = (const VT*)(aux + cell * AUX_COUNT);
+  const VT* q = (const VT*)(aux + cell * NV);
 #pragma unroll
-  for (int g = 0; g < AUX_COUNT / V16<T>::n; g++) V16<T>::split(q[g], v + g * V16<T>::n);
+  for (int g = 0; g < NV / V16<T>::n; g++) V16<T>::split(q[g], v + g * V16<T>::n);
 }
+constexpr int KIN_NV = 8;   // velocity (3), acceleration (3), padding
 enum { AX_V = 0, AX_A = 3 };               // velocity, acceleration
 
 __device__ constexpr int SGN[3][8] = {{-1, 1, 1, -1, -1, 1, 1, -1}, {-1, -1, 1, 1, -1, -1, 1, 1},
@@ -2421,7 +2424,6 @@ __global__ void __launch_bounds__(256) k_sgmp_cells(PoolPtrs<T> P, const int* ng
   const long n1 = I.node_off + (long)cz * I.ngxy + (long)cy * I.ngx + cx;
   const long nodes[8] = {n1, n1 + 1, n1 + 1 + I.ngx, n1 + I.ngx,
                          n1 + I.ngxy, n1 + I.ngxy + 1, n1 + I.ngxy + 1 + I.ngx, n1 + I.ngxy + I.ngx};
-  T* aux = P.aux + I.node_off;
   if (WHAT & SGC_GRAD) {
     const T ij4 = (T)I.ijac4;
     T co[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, cw[3] = {(T)0, (T)0, (T)0};
@@ -2457,7 +2459,7 @@ __global__ void __launch_bounds__(256) k_sgmp_cells(PoolPtrs<T> P, const int* ng
     for (int q = 0; q < 3; q++) g[AX_CW + q] = cw[q];
 #pragma unroll
     for (int q = 0; q < 9; q++) g[AX_DF + q] = df[q];
-    grad_store<T>(P.aux, (long)I.node_off + c, g);
+    centre_store<T, AUX_COUNT>(P.aux, (long)I.node_off + c, g);
   }
   if (WHAT & SGC_KIN) {
     // a centre on a fixed or symmetric face takes the nodal values at its
@@ -2494,11 +2496,14 @@ __global__ void __launch_bounds__(256) k_sgmp_cells(PoolPtrs<T> P, const int* ng
         ca[d] = ca[d] + w[jl] * P.ag[d * nn + k];
       }
     }
+    T g[KIN_NV];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      aux[(AX_V + d) * nn + c] = cv[d];
-      aux[(AX_A + d) * nn + c] = ca[d];
+      g[AX_V + d] = cv[d];
+      g[AX_A + d] = ca[d];
     }
+    g[6] = g[7] = (T)0;
+    centre_store<T, KIN_NV>(P.aux, (long)I.node_off + c, g);
   }
 }
 
@@ -2527,7 +2532,7 @@ __global__ void __launch_bounds__(256) k_sgmp_stress(PoolPtrs<T> P, const int* g
       const int k = sgmp_node(I, L, n);
       if (!k) continue;
       T g[AUX_COUNT];
-      grad_load<T>(P.aux, (long)I.node_off + k - 1, g);
+      centre_load<T, AUX_COUNT>(P.aux, (long)I.node_off + k - 1, g);
 #pragma unroll
       for (int q = 0; q < 6; q++) de[q] = de[q] + g[AX_CO + q] * shp[n];
 #pragma unroll
@@ -2588,18 +2593,17 @@ __global__ void __launch_bounds__(256) k_sgmp_g2p(PoolPtrs<T> P, const int* glis
     }
     T shp[8];
     sgn_shape<T>(xi, shp);
-    const long nn = P.nn;
-    const T* aux = P.aux + I.node_off;
     T vx[3] = {(T)0, (T)0, (T)0}, ax[3] = {(T)0, (T)0, (T)0};
 #pragma unroll
     for (int n = 0; n < 8; n++) {
       const int k = sgmp_node(I, L, n);
       if (!k) continue;
-      const long c = k - 1;
+      T g[KIN_NV];
+      centre_load<T, KIN_NV>(P.aux, (long)I.node_off + k - 1, g);
 #pragma unroll
       for (int d = 0; d < 3; d++) {
-        vx[d] = vx[d] + shp[n] * aux[(AX_V + d) * nn + c];
-        ax[d] = ax[d] + shp[n] * aux[(AX_A + d) * nn + c];
+        vx[d] = vx[d] + shp[n] * g[AX_V + d];
+        ax[d] = ax[d] + shp[n] * g[AX_A + d];
       }
     }
     const double dt = ts.dt;
