@@ -1638,6 +1638,10 @@ extern "C" int mpm_ctx_create(const mpm_problem* prob, int precision, mpm_ctx** 
     // blocks than it needs to hide latency: 128 (what the autotuner picks
     // for the 1.3e5-particle Taylor bar; 256 for the 1.3e6-particle slope)
     if (prob->nb_particle < (1 << 20)) e->bs_p = 128;
+    // the auxiliary-grid scatter of SGMP runs 2.2x faster in Hilbert order
+    // than in 2^3 bricks (measured on the Cook-plate batch, which the
+    // autotuner also sorts in Hilbert order)
+    if (prob->shape == MPM_SHAPE_SGMP) e->brick = 0;
     try {
       add_instance(e, prob, 0);
       rebuild_lists(e);
