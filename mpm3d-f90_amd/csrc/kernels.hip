@@ -2260,15 +2260,13 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
   const long nn = P.nn;
   T* aux = P.aux + I.node_off;
   double* cmg = P.cmg + I.node_off;
+  T m = (T)0, v[3] = {(T)0, (T)0, (T)0};
+  T fx[3] = {(T)0, (T)0, (T)0}, sg[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, vol = (T)0;
   if (act) {
-    T shp[8];
-    sgn_shape<T>(L.xxn, shp);
-    T m = (T)0, v[3] = {(T)0, (T)0, (T)0};
     if (MASS || MOM) {
       m = p.t(T_MASS);
       v[0] = p.t(T_VX); v[1] = p.t(T_VY); v[2] = p.t(T_VZ);
     }
-    T fx[3] = {(T)0, (T)0, (T)0}, sg[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, vol = (T)0;
     if (FRC) {
       external_force(P, p, I, P.ts[ii].time, fx);
       const T smq = p.t(T_SM) - p.t(T_Q);
@@ -2276,7 +2274,85 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
       sg[3] = p.t(T_SD5); sg[4] = p.t(T_SD3); sg[5] = p.t(T_SD4);   // Cfint(4..6): sxy, syz, sxz
       vol = (T)p.d(D_VOL);
     }
-    const int eb = fast ? (ax - B.x0) + B.bx * ((ay - B.y0) + B.by * (az - B.z0)) : 0;
+  }
+  if (fast) {
+    // run sums: particles in the nearest-node order share their auxiliary
+    // cell (and so its 8 nodes) in runs; a (run, node, part) item sums one
+    // node's values over its part of the run in registers and adds the sums
+    // to the tile, instead of one LDS atomic per particle, node and value
+    constexpr int OM = 3, OF = OM + ((MASS || MOM) ? 4 : 0);
+    constexpr int NVR = OF + (FRC ? 10 : 0);   // fext (3), sigma (6), vol
+    extern __shared__ __align__(16) char run_mem[];
+    const RunStage<T, NVR> R(run_mem, blockDim.x);
+    constexpr int NVV = RunStage<T, NVR>::NG * RunStage<T, NVR>::GV;
+    const int jl = (int)(i - ((long)gran * PGRAN + (long)(blockIdx.x % bpg) * blockDim.x));
+    int key = -1;
+    if (act) {
+      T u[NVV];
+#pragma unroll
+      for (int q = 0; q < NVV; q++) u[q] = (T)0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) u[d] = (T)L.xxn[d];
+      if (MASS || MOM) {
+        u[OM] = m;
+#pragma unroll
+        for (int d = 0; d < 3; d++) u[OM + 1 + d] = v[d];
+      }
+      if (FRC) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) u[OF + d] = fx[d];
+#pragma unroll
+        for (int k = 0; k < 6; k++) u[OF + 3 + k] = sg[k];
+        u[OF + 9] = vol;
+      }
+      R.put(jl, u);
+      key = (ax - B.x0) + B.bx * ((ay - B.y0) + B.by * (az - B.z0));
+    }
+    R.key[jl] = key;
+    if (threadIdx.x == 0) *R.nh = 0;
+    __syncthreads();
+    run_heads(R);
+    __syncthreads();
+    const int nh = *R.nh;
+    const int parts = run_parts<8>(nh);
+    for (int item = threadIdx.x; item < nh * 8 * parts; item += blockDim.x) {
+      const RunItem it = run_item<8>(R, nh, parts, item);
+      if (it.j0 >= it.j1) continue;
+      const int n = it.c;
+      const T gx = (T)SGN[0][n], gy = (T)SGN[1][n], gz = (T)SGN[2][n];
+      // each particle's term as in the per-particle path, summed in FP64
+      double acc[NF];
+#pragma unroll
+      for (int f = 0; f < NF; f++) acc[f] = 0.0;
+      for (int j = it.j0; j < it.j1; j++) {
+        T u[NVV];
+        R.get(j, u);
+        // SGNShape weight of node n (sgn_shape)
+        const T N = (gx * u[0] + (T)1) * (gy * u[1] + (T)1) * (gz * u[2] + (T)1) * (T)0.125;
+        int q = 0;
+        if (MASS || MOM) {
+          const T shm = N * u[OM];
+          if (MASS) acc[q++] += (double)shm;
+          if (MOM)
+#pragma unroll
+            for (int d = 0; d < 3; d++) acc[q++] += (double)(u[OM + 1 + d] * shm);
+        }
+        if (FRC) {
+#pragma unroll
+          for (int d = 0; d < 3; d++) acc[q++] += (double)(u[OF + d] * N);
+#pragma unroll
+          for (int k = 0; k < 6; k++) acc[q++] += (double)(N * u[OF + 3 + k] * u[OF + 9]);
+        }
+      }
+      // node n of the SNX/SNY/SNZ order: x + 1 for n & 3 in {1, 2}, y + 1 for {2, 3}, z + 1 for n >= 4
+      const int e = it.e0 + ((n & 3) == 1 || (n & 3) == 2 ? 1 : 0) + ((n & 3) >= 2 ? B.bx : 0) +
+                    (n >= 4 ? B.bx * B.by : 0);
+#pragma unroll
+      for (int f = 0; f < NF; f++) atomicAdd(&tile[f * CAP + e], acc[f]);
+    }
+  } else if (act) {
+    T shp[8];
+    sgn_shape<T>(L.xxn, shp);
 #pragma unroll
     for (int n = 0; n < 8; n++) {
       const T N = shp[n];
@@ -2295,25 +2371,17 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
 #pragma unroll
         for (int k = 0; k < 6; k++) val[q++] = N * sg[k] * vol;
       }
-      if (fast) {
-        // node n of the SNX/SNY/SNZ order: x + 1 for n & 3 in {1, 2}, y + 1 for {2, 3}, z + 1 for n >= 4
-        const int e = eb + ((n & 3) == 1 || (n & 3) == 2 ? 1 : 0) + ((n & 3) >= 2 ? B.bx : 0) +
-                      (n >= 4 ? B.bx * B.by : 0);
+      const int k = sgmp_node(I, L, n);
+      if (!k) continue;
+      const long c = k - 1;
+      q = 0;
+      if (MASS) atomic_add(&cmg[c], (double)val[q++]);
+      if (MOM)
 #pragma unroll
-        for (int f = 0; f < NF; f++) atomicAdd(&tile[f * CAP + e], (double)val[f]);
-      } else {
-        const int k = sgmp_node(I, L, n);
-        if (!k) continue;
-        const long c = k - 1;
-        q = 0;
-        if (MASS) atomic_add(&cmg[c], (double)val[q++]);
-        if (MOM)
+        for (int d = 0; d < 3; d++) atomic_add(&aux[(AX_P + d) * nn + c], val[q++]);
+      if (FRC)
 #pragma unroll
-          for (int d = 0; d < 3; d++) atomic_add(&aux[(AX_P + d) * nn + c], val[q++]);
-        if (FRC)
-#pragma unroll
-          for (int f = 0; f < 9; f++) atomic_add(&aux[(AX_FE + f) * nn + c], val[q++]);
-      }
+        for (int f = 0; f < 9; f++) atomic_add(&aux[(AX_FE + f) * nn + c], val[q++]);
     }
   }
   if (fast) {
@@ -2728,13 +2796,17 @@ __global__ void __launch_bounds__(256) k_permute(S* dst, const S* src, const int
 // particles covers a compact box of nodes.  The cell is located from the
 // current position (InWhichCell); particles outside the grid and padding
 // slots get the largest key of their instance.  B = 1 is the plain cell order.
-__device__ __forceinline__ unsigned brick_key(const InstC& I, int c, int B) {
-  int cx, cy, cz;
-  cell_coords(I, c, cx, cy, cz);
+__device__ __forceinline__ unsigned brick_key_xyz(const InstC& I, int cx, int cy, int cz, int B) {
   const int nbx = (I.ncx + B - 1) / B, nby = (I.ncy + B - 1) / B;
   const unsigned brick = (unsigned)(((cz / B) * nby + cy / B) * nbx + cx / B);
   const unsigned local = (unsigned)(((cz % B) * B + cy % B) * B + cx % B);
   return brick * (unsigned)(B * B * B) + local;
+}
+
+__device__ __forceinline__ unsigned brick_key(const InstC& I, int c, int B) {
+  int cx, cy, cz;
+  cell_coords(I, c, cx, cy, cz);
+  return brick_key_xyz(I, cx, cy, cz, B);
 }
 
 // Index of cell (x, y, z) along the 3D Hilbert curve of 2^bits cells per
@@ -2777,21 +2849,41 @@ __global__ void __launch_bounds__(256) k_sort_keys(const double* X, long np, con
   int gran;
   const long j = part_index(0, glist, bpg, gran);
   const InstC& I = inst[ginst[gran]];
-  int c = -1;
-  if (j < I.p_end) c = in_which_cell(I, X[j], X[np + j], X[2 * np + j]);
+  // key coordinates: the cell, or for SGMP the auxiliary cell (the cell of
+  // the nearest node in the centre grid, decomposed as sgmp_locate does), so
+  // that particles sharing their 8 auxiliary nodes are consecutive
+  bool in = false;
+  int cx = 0, cy = 0, cz = 0;
+  if (j < I.p_end) {
+    if (I.shape == MPM_SHAPE_SGMP) {
+      const double h = I.dcell;
+      const int ix = (int)((X[j] - I.span0[0] - h / 2.0) / h);
+      const int iy = (int)((X[np + j] - I.span0[1] - h / 2.0) / h);
+      const int iz = (int)((X[2 * np + j] - I.span0[2] - h / 2.0) / h);
+      const int ac = iz * I.cncxy + iy * I.cncx + ix + 1;
+      if (ac >= 1 && ac <= I.cncell) {
+        const int a0 = ac - 1;
+        cz = a0 / I.cncxy;
+        const int r = a0 - cz * I.cncxy;
+        cy = r / I.cncx;
+        cx = r - cy * I.cncx;
+        in = true;
+      }
+    } else {
+      const int c = in_which_cell(I, X[j], X[np + j], X[2 * np + j]);
+      if (c > 0) {
+        cell_coords(I, c, cx, cy, cz);
+        in = true;
+      }
+    }
+  }
   const int hb = B == 0 ? hilbert_bits(I) : 0;
   if (hb > 0) {
-    unsigned key = 1u << (3 * hb);
-    if (c > 0) {
-      int cx, cy, cz;
-      cell_coords(I, c, cx, cy, cz);
-      key = hilbert_key((unsigned)cx, (unsigned)cy, (unsigned)cz, hb);
-    }
-    keys[j] = key;
+    keys[j] = in ? hilbert_key((unsigned)cx, (unsigned)cy, (unsigned)cz, hb) : 1u << (3 * hb);
   } else {
     if (B == 0) B = 4;
     const int nbx = (I.ncx + B - 1) / B, nby = (I.ncy + B - 1) / B, nbz = (I.ncz + B - 1) / B;
-    keys[j] = c > 0 ? brick_key(I, c, B) : (unsigned)nbx * nby * nbz * (unsigned)(B * B * B);
+    keys[j] = in ? brick_key_xyz(I, cx, cy, cz, B) : (unsigned)nbx * nby * nbz * (unsigned)(B * B * B);
   }
   vals[j] = (int)j;
 }
@@ -3038,9 +3130,15 @@ void launch_stress(int shape, int mode, int mm, const PoolPtrs<T>& P, const Laun
 template <typename T>
 void launch_sgmp_p2g(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
   if (!L.nblk) return;
-  dim3 g(L.nblk), b(L.bs);
-#define SG_CASE(M) \
-  case M: hipLaunchKernelGGL((k_sgmp_p2g<T, M>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+#define SG_CASE(M)                                                                                       \
+  case M: {                                                                                              \
+    constexpr int NF = ((M) & P2G_MASS ? 1 : 0) + ((M) & P2G_MOM ? 3 : 0) + ((M) & P2G_FORCE ? 9 : 0);   \
+    constexpr int NVR = 3 + ((M) & (P2G_MASS | P2G_MOM) ? 4 : 0) + ((M) & P2G_FORCE ? 10 : 0);           \
+    const LaunchP L2 = fit_lds(L, (size_t)(NF > 4 ? TILE_NODES / 2 : TILE_NODES) * NF * 8,               \
+                               [](int bs) { return run_stage_bytes<T, NVR>(bs); });                      \
+    const size_t sh = run_stage_bytes<T, NVR>(L2.bs);                                                    \
+    hipLaunchKernelGGL((k_sgmp_p2g<T, M>), dim3(L2.nblk), dim3(L2.bs), sh, s, P, L2.glist, L2.ginst, L2.bpg); \
+  } break;
   switch (mode) {
     SG_CASE(P2G_ICELL | P2G_MASS | P2G_MOM)
     SG_CASE(P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE)
