@@ -76,6 +76,33 @@ __device__ __forceinline__ int in_which_cell(const InstC& I, double x, double y,
   return c;
 }
 
+// InWhichCell with the cell's coordinates: the axis indices of the division,
+// carried the way the reference's linear index wraps at the upper faces
+// (x == SpanX(2) gives the first cell of the next row), equal to
+// cell_coords of the returned index without its two integer divisions.
+__device__ __forceinline__ int in_which_cell3(const InstC& I, double x, double y, double z, int& cx, int& cy,
+                                              int& cz) {
+  if (x < I.span0[0] || x > I.span1[0] || y < I.span0[1] || y > I.span1[1] || z < I.span0[2] || z > I.span1[2])
+    return -1;
+  int ix = (int)((x - I.span0[0]) / I.dcell);
+  int iy = (int)((y - I.span0[1]) / I.dcell);
+  int iz = (int)((z - I.span0[2]) / I.dcell);
+  const int c = iz * I.ncxy + iy * I.ncx + ix + 1;
+  if (c > I.ncell || c <= 0) return -1;
+  if (ix >= I.ncx) {
+    ix -= I.ncx;
+    iy++;
+  }
+  if (iy >= I.ncy) {
+    iy -= I.ncy;
+    iz++;
+  }
+  cx = ix;
+  cy = iy;
+  cz = iz;
+  return c;
+}
+
 __device__ __forceinline__ void cell_coords(const InstC& I, int c, int& cx, int& cy, int& cz) {
   const int c0 = c - 1;
   cz = c0 / I.ncxy;
@@ -1155,7 +1182,7 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
   A* tf = (A*)(smem + (MASS ? 8 * CAP : 0));        // NT * CAP
   PV<T> p{P, i};
   const bool live = i < I.p_end;
-  int c = -1;
+  int c = -1, cx = 0, cy = 0, cz = 0;
   double x = 0, y = 0, z = 0;
   // every particle load is issued before the box reduction (two barriers),
   // so that their latencies overlap
@@ -1177,15 +1204,14 @@ __global__ void __launch_bounds__(256) k_p2g(PoolPtrs<T> P, const int* glist, co
       vol = (T)p.d(D_VOL);
     }
     if (MODE & P2G_ICELL) {
-      c = in_which_cell(I, x, y, z);
+      c = in_which_cell3(I, x, y, z, cx, cy, cz);
       p.n(I_CELL) = c;
     }
   } else if (MODE & P2G_ICELL) {
     p.n(I_CELL) = -1;
   }
   const bool act = c >= 0;
-  int cx = 0, cy = 0, cz = 0;
-  if (act) cell_coords(I, c, cx, cy, cz);
+  if (act && !(MODE & P2G_ICELL)) cell_coords(I, c, cx, cy, cz);
   const int jl = (int)(i - ((long)gran * PGRAN + (long)(blockIdx.x % bpg) * blockDim.x));
   const Box2 B = block_box_cached<SHAPE>(I, act, cx, cy, cz, jl, min(P.tile_cap, CAP), (int*)smem, P.boxes,
                                          P.box_mode, (i - jl) / 64);
