@@ -946,7 +946,7 @@ static void enqueue_sgmp(mpm_engine* e, Group& G, const PoolPtrs<T>& P, const La
     timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 0, 1, 1, s); });
     timed(e, KI_STRESS, [&] {
       mpmk::launch_sgmp_cells<T>(SGC_GRAD, P, N, s);
-      mpmk::launch_stress<T>(MPM_SHAPE_SGMP, 0, MM_ALL, P, L, s);
+      mpmk::launch_stress<T>(MPM_SHAPE_SGMP, 0, G.matmask, P, L, s);
     });
     timed(e, KI_P2G, [&] {
       mpmk::launch_zero_nodes<T>(aux + AUX_FORCE_OFF * nn, 9, nn, N, s);
@@ -975,7 +975,7 @@ static void enqueue_sgmp(mpm_engine* e, Group& G, const PoolPtrs<T>& P, const La
     }
     timed(e, KI_STRESS, [&] {
       mpmk::launch_sgmp_cells<T>(SGC_GRAD, P, N, s);
-      mpmk::launch_stress<T>(MPM_SHAPE_SGMP, ST_MOVE | ST_ENERGY, MM_ALL, P, L, s);
+      mpmk::launch_stress<T>(MPM_SHAPE_SGMP, ST_MOVE | ST_ENERGY, G.matmask, P, L, s);
     });
   }
 }

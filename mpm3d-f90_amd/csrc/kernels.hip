@@ -3135,15 +3135,32 @@ void launch_g2p(int shape, int mode, const PoolPtrs<T>& P, const LaunchP& L, hip
   }
 }
 
+// the fused SGMP modes get one kernel per single material model and for the
+// neo-Hookean + null pair, the per-phase mode only the general one
+template <typename T, int MODE>
+void launch_sgmp_stress_m(int mm, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  dim3 g(L.nblk), b(L.bs);
+#define SGS_M(M) \
+  case M: hipLaunchKernelGGL((k_sgmp_stress<T, MODE, M>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+  switch (mm) {
+    SGS_M(1 << MPM_MAT_ELASTIC)
+    SGS_M(1 << MPM_MAT_PLA2)
+    SGS_M(1 << MPM_MAT_NULL)
+    SGS_M(1 << MPM_MAT_NEOH)
+    SGS_M((1 << MPM_MAT_NEOH) | (1 << MPM_MAT_NULL))
+    default: hipLaunchKernelGGL((k_sgmp_stress<T, MODE, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg); break;
+  }
+#undef SGS_M
+}
+
 template <typename T>
 void launch_stress(int shape, int mode, int mm, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
   if (!L.nblk) return;
   if (shape == MPM_SHAPE_SGMP) {
     dim3 g(L.nblk), b(L.bs);
     if (mode == ST_MOVE) hipLaunchKernelGGL((k_sgmp_stress<T, ST_MOVE, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
-    else if (mode == (ST_MOVE | ST_ENERGY))
-      hipLaunchKernelGGL((k_sgmp_stress<T, ST_MOVE | ST_ENERGY, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
-    else hipLaunchKernelGGL((k_sgmp_stress<T, 0, MM_ALL>), g, b, 0, s, P, L.glist, L.ginst, L.bpg);
+    else if (mode == (ST_MOVE | ST_ENERGY)) launch_sgmp_stress_m<T, ST_MOVE | ST_ENERGY>(mm, P, L, s);
+    else launch_sgmp_stress_m<T, 0>(mm, P, L, s);
     return;
   }
   switch (shape) {
