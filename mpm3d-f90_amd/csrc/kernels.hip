@@ -103,8 +103,33 @@ __device__ __forceinline__ int in_which_cell3(const InstC& I, double x, double y
   return c;
 }
 
+// Quotient and remainder of 0 <= a < 2^22 by b >= 1 from the FP32
+// reciprocal, exact: a and b (b < 2^24; a larger b gives a quotient of 0
+// either way) convert exactly, v_rcp_f32 is within 1 ulp, so a * rcp(b) is
+// within 1.5 * 2^-23 * a / b < 0.4 of a / b for b >= 2 (exact for b = 1),
+// and one correction step lands on the integer quotient.  About a third of
+// the instructions of the 32-bit integer division sequence.
+__device__ __forceinline__ int fdivmod(int a, int b, int& r) {
+  int q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
+  r = a - q * b;
+  if (r < 0) {
+    q -= 1;
+    r += b;
+  } else if (r >= b) {
+    q += 1;
+    r -= b;
+  }
+  return q;
+}
+
 __device__ __forceinline__ void cell_coords(const InstC& I, int c, int& cx, int& cy, int& cz) {
   const int c0 = c - 1;
+  if ((unsigned)c0 < (1u << 22)) {
+    int r;
+    cz = fdivmod(c0, I.ncxy, r);
+    cy = fdivmod(r, I.ncx, cx);
+    return;
+  }
   cz = c0 / I.ncxy;
   const int r = c0 - cz * I.ncxy;
   cy = r / I.ncx;
@@ -407,7 +432,8 @@ struct Box {
     return ex + bx * (ey + by * ez);
   }
   __device__ __forceinline__ int node(const InstC& I, int e) const {
-    const int ex = e % bx, r = e / bx, ey = r % by, ez = r / by;
+    int ex, ey;   // e < box size <= tile capacity
+    const int r = fdivmod(e, bx, ex), ez = fdivmod(r, by, ey);
     return (z0 + ez) * I.ngxy + (y0 + ey) * I.ngx + (x0 + ex);
   }
 };
@@ -1013,14 +1039,15 @@ __device__ __forceinline__ int run_parts(int nh) {
 template <int NS, typename T, int NV>
 __device__ __forceinline__ RunItem run_item(const RunStage<T, NV>& R, int nh, int parts, int item) {
   RunItem it;
-  const int k = item % nh, rest = item / nh;
+  int k, dump;   // item < nh * NS * parts <= 3 * blockDim.x; len * parts <= blockDim.x^2
+  const int rest = fdivmod(item, nh, k);
   it.c = rest % NS;
   const int h = rest / NS;
   it.run0 = R.head[k];
   it.e0 = R.key[it.run0];
   const int len = R.hend[k] - it.run0;
-  it.j0 = it.run0 + (len * h) / parts;
-  it.j1 = it.run0 + (len * (h + 1)) / parts;
+  it.j0 = it.run0 + fdivmod(len * h, parts, dump);
+  it.j1 = it.run0 + fdivmod(len * (h + 1), parts, dump);
   return it;
 }
 
@@ -2266,13 +2293,7 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
   const bool act = live && sgmp_locate(I, P.fix, p.d(D_X), p.d(D_Y), p.d(D_Z), L);
   // auxiliary coordinates of node 0 of the particle's auxiliary cell
   int ax = 0, ay = 0, az = 0;
-  if (act) {
-    const int c0 = L.cn1 - 1;
-    az = c0 / I.ncxy;
-    const int r = c0 - az * I.ncxy;
-    ay = r / I.ncx;
-    ax = r - ay * I.ncx;
-  }
+  if (act) cell_coords(I, L.cn1, ax, ay, az);
   const Box B = block_box_n(act, ax, ay, az, 0, 1, I.ncx - 1, I.ncy - 1, I.ncz - 1, CAP);
   // with fewer than 2 cells along an axis the reference's linear index of
   // the +1 neighbour leaves the axis: direct path
@@ -2413,7 +2434,8 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
   if (fast) {
     __syncthreads();
     for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
-      const int ex = e % B.bx, r = e / B.bx, ey = r % B.by, ez = r / B.by;
+      int ex, ey;
+      const int r = fdivmod(e, B.bx, ex), ez = fdivmod(r, B.by, ey);
       const long c = (long)(B.z0 + ez) * I.ncxy + (long)(B.y0 + ey) * I.ncx + (B.x0 + ex);
       int q = 0;
       if (MASS) {
@@ -2513,7 +2535,8 @@ __global__ void __launch_bounds__(256) k_sgmp_cells(PoolPtrs<T> P, const int* ng
   if (P.ts[ii].done) return;
   const int c = (int)(j - I.node_off);
   if (c >= I.ncell) return;
-  const int cz = c / I.ncxy, r = c - cz * I.ncxy, cy = r / I.ncx, cx = r - cy * I.ncx;
+  int cx, cy, cz;
+  cell_coords(I, c + 1, cx, cy, cz);
   const long nn = P.nn;
   const long n1 = I.node_off + (long)cz * I.ngxy + (long)cy * I.ngx + cx;
   const long nodes[8] = {n1, n1 + 1, n1 + 1 + I.ngx, n1 + I.ngx,
