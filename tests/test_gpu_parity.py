@@ -435,3 +435,34 @@ def test_instances_stop_at_their_own_endtime(mpm):
     assert st[a][0] == 1 and st[b][0] == 1
     assert st[b][1] < st[a][1] <= 8
     eng.close()
+
+
+def test_cell_indices_across_two_to_the_22(mpm, oracle_mod, tmp_path):
+    """Cell coordinates are split off the cell index with an FP32-reciprocal
+    quotient below 2^22 and with integer division above.  The small B-spline
+    slope stretched along y to 40,001 cells (800,020 cells per z layer)
+    puts its base layer below 2^22 and its terraces above: both paths, FP64
+    and FP32, against the oracle."""
+    text = open(deck_path("slope_dp_bspline_usf")).read()
+    assert text.count("spy -0.2 0.6") == 1
+    path = tmp_path / "slope_wide.mpm"
+    path.write_text(text.replace("spy -0.2 0.6", "spy -0.2 8000.0"))
+    prob = mpm.read_deck(str(path))
+    o = oracle_mod.Oracle(prob)
+    a = mpm.Context(prob, "fp64")
+    b = mpm.Context(prob, "fp32")
+    o.step(1)
+    a.step(1)
+    b.step(1)
+    ic = o.icell() - 1
+    assert (ic >= 1 << 22).sum() > 50 and ((ic >= 0) & (ic < 1 << 22)).sum() > 50
+    assert np.array_equal(a.icell(), o.icell()) and np.array_equal(b.icell(), o.icell())
+    o.step(9)
+    a.step(9)
+    b.step(9)
+    assert np.array_equal(a.icell(), o.icell())
+    assert_close(perr(mpm, a.particles(), o.particles()), 1e-12, None, "wide fp64")
+    assert_close(gerr(mpm, a.grid(), o.grid()), 1e-12, None, "wide fp64 grid")
+    assert_close(perr(mpm, b.particles(), o.particles()), 1e-5, None, "wide fp32")
+    a.close()
+    b.close()
