@@ -1478,10 +1478,11 @@ __device__ __forceinline__ void flush_tile3(const InstC& I, const BX& B, const A
 // the MUSL re-map of momentum (GridMomentumMUSL, :1705-1758) into p2 and
 // with the energy partials (when it is the last particle pass).
 // Waves per SIMD the particle-update kernel is compiled for: the FP32
-// GIMP and B-spline kernels fit six waves without spilling
+// B-spline kernel fits eight waves (64 VGPRs) without spilling, the FP32
+// GIMP kernel six (it spills at eight)
 template <int SHAPE, typename T>
 __host__ __device__ constexpr int g2p_waves() {
-  return sizeof(T) == 4 && SHAPE != MPM_SHAPE_LINEAR ? 6 : 1;
+  return sizeof(T) != 4 || SHAPE == MPM_SHAPE_LINEAR ? 1 : (SHAPE == MPM_SHAPE_BSPLINE ? 8 : 6);
 }
 
 template <int SHAPE, typename T, int MODE>
