@@ -45,6 +45,10 @@ struct oracle {
   int istep;
   double time, dt;
   double ek, ei, mom[3], ek_prev;
+  /* diagnostics of M3DM9 (not reference state): return-map branch of every
+   * particle's last stress update and the running count of each branch */
+  int8_t* dpbr;
+  long dpcount[4];
 };
 
 #define P(o, f, i) ((o)->ps[(size_t)(f) * (o)->n + (i)])
@@ -462,6 +466,7 @@ oracle* oracle_create(const mpm_problem* p) {
   memcpy(o->pbody, p->pbody, sizeof(int32_t) * o->n);
   o->icell = (int32_t*)calloc(o->n, sizeof(int32_t));
   o->xn = (double*)calloc((size_t)3 * o->n, sizeof(double));
+  o->dpbr = (int8_t*)calloc(o->n, 1);
   o->ncx = p->ncx; o->ncy = p->ncy; o->ncz = p->ncz;
   o->ncxy = o->ncx * o->ncy;
   o->ncell = o->ncxy * o->ncz;
@@ -513,7 +518,7 @@ oracle* oracle_create(const mpm_problem* p) {
 void oracle_destroy(oracle* o) {
   if (!o) return;
   free(o->mats); free(o->bodies); free(o->ps); free(o->pbody); free(o->icell);
-  free(o->xn); free(o->g); free(o->fix);
+  free(o->xn); free(o->g); free(o->fix); free(o->dpbr);
   free(o->bdt); free(o->cicell); free(o->inode); free(o->xxn);
   free(o->cmg); free(o->cp); free(o->cfint); free(o->cfext); free(o->co); free(o->cw); free(o->cdefp);
   free(o->cv); free(o->ca);
@@ -956,6 +961,7 @@ typedef struct {
   double seqv, epeff_, sig_y_, depeff, ratio, G2, K3, PlaMod, lamda, miu2;
   double iener, specen, ieinc, den0_, den_, vol0_, cp, yield0_;
   double F[3][3];
+  int br;   /* M3DM9 branch taken (DP_BR_*), diagnostics only */
 } cst_t;
 
 /* EquivalentStress returns a default REAL (Constitution.f90:679-699) */
@@ -1123,8 +1129,9 @@ static void m3dm9(cst_t* m, const mpm_material* mat, double young, double poisso
     double Taup = kfai - qfai * tenf;
     double dp_hfai = Tau - Taup - alphap * dpsig;
     if (dp_hfai > 0.0) {
-      shear = 1;
+      shear = 2;
     } else {
+      m->br = DP_BR_TENSION;
       double dlamd = (m->sm - tenf) / Kmod;
       m->sm = tenf;
       m->depeff = dlamd * third32 * (double)sqrtf(2.0f);
@@ -1132,6 +1139,7 @@ static void m3dm9(cst_t* m, const mpm_material* mat, double young, double poisso
     }
   }
   if (shear) {
+    m->br = shear == 1 ? DP_BR_SHEAR : DP_BR_SHEAR_TENSILE;
     double dlamd = dpFi / (Gmod + Kmod * qfai * qpsi);
     m->sm = m->sm - Kmod * qpsi * dlamd;
     double newTau = kfai - qfai * m->sm;
@@ -1256,7 +1264,13 @@ static void constitution(oracle* o, int p, const double de[6], const double vort
     case MPM_MAT_PLA1: sigrot(&m, vort); m3dm2(&m); lieupd(&m, o); break;
     case MPM_MAT_PLA2: sigrot(&m, vort); m3dm3(&m); lieupd(&m, o); break;
     case MPM_MAT_NULL: m3dm7(&m, o->dt); eos4(&m, o, mat); break;
-    case MPM_MAT_DP: sigrot(&m, vort); m3dm9(&m, mat, young, poisson); lieupd(&m, o); break;
+    case MPM_MAT_DP:
+      sigrot(&m, vort);
+      m3dm9(&m, mat, young, poisson);
+      lieupd(&m, o);
+      o->dpbr[p] = (int8_t)m.br;
+      o->dpcount[m.br]++;
+      break;
     case MPM_MAT_NEOH: m3dm10(&m); lieupd(&m, o); break;
     default: break;
   }
@@ -1404,6 +1418,32 @@ void oracle_step(oracle* o, int nsteps) {
 
 void oracle_get_particles(const oracle* o, double* out) {
   memcpy(out, o->ps, sizeof(double) * PF_COUNT * o->n);
+}
+
+void oracle_get_dp_branches(const oracle* o, int8_t* last, long* counts4) {
+  if (last) memcpy(last, o->dpbr, (size_t)o->n);
+  if (counts4)
+    for (int i = 0; i < 4; i++) counts4[i] = o->dpcount[i];
+}
+
+/* M3DM9 on one stress point with a zero strain increment (so the trial
+ * state is the given stress): sd[6] (xx yy zz yz xz xy) and *sm in, the
+ * returned stress, *seqv and the plastic strain increment out; returns the
+ * branch (DP_BR_*).  Test entry for the return map's closed forms. */
+int oracle_dp_point(const mpm_material* mat, double young, double poisson, double sd[6], double* sm, double* seqv,
+                    double* depeff) {
+  cst_t m;
+  memset(&m, 0, sizeof m);
+  for (int i = 0; i < 6; i++) m.sd[i] = sd[i];
+  m.sm = *sm;
+  m.G2 = young / (1 + poisson);
+  m.K3 = young / (1 - 2 * poisson);
+  m3dm9(&m, mat, young, poisson);
+  for (int i = 0; i < 6; i++) sd[i] = m.sd[i];
+  *sm = m.sm;
+  *seqv = m.seqv;
+  *depeff = m.epeff_;
+  return m.br;
 }
 
 void oracle_set_particles(oracle* o, const double* in) {

@@ -50,6 +50,16 @@ void oracle_get_fix(const oracle* o, int8_t* out);         /* 3 * nb_node */
 void oracle_get_scalars(const oracle* o, double* out);
 void oracle_set_particles(oracle* o, const double* in);
 
+/* Diagnostics of the Drucker-Prager return map (M3DM9, Constitution.f90:
+ * 1263-1321), not reference state: the branch each particle took in its
+ * last stress update, and running counts per branch over all updates. */
+enum { DP_BR_ELASTIC = 0, DP_BR_SHEAR = 1, DP_BR_SHEAR_TENSILE = 2, DP_BR_TENSION = 3 };
+void oracle_get_dp_branches(const oracle* o, int8_t* last, long* counts4);
+/* M3DM9 on one stress point with a zero strain increment (trial = input):
+ * returns the branch; sd[6], *sm updated, *seqv and *depeff written. */
+int oracle_dp_point(const mpm_material* mat, double young, double poisson, double sd[6], double* sm, double* seqv,
+                    double* depeff);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,10 @@ def lib():
             getattr(L, name).argtypes = [vp, C.POINTER(C.c_double)]
         L.oracle_get_icell.argtypes = [vp, C.POINTER(C.c_int32)]
         L.oracle_get_fix.argtypes = [vp, C.POINTER(C.c_int8)]
+        L.oracle_get_dp_branches.argtypes = [vp, C.POINTER(C.c_int8), C.POINTER(C.c_long)]
+        dp = C.POINTER(C.c_double)
+        L.oracle_dp_point.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
+        L.oracle_dp_point.restype = C.c_int
         _L = L
     return _L
 
@@ -98,10 +102,32 @@ class Oracle:
         lib().oracle_get_fix(self.h, out.ctypes.data_as(C.POINTER(C.c_int8)))
         return out
 
+    def dp_branches(self):
+        """(branch of every particle's last Drucker-Prager update, running
+        counts [elastic, shear with sm < tenf, shear with sm >= tenf, tension
+        cap]) -- oracle diagnostics of M3DM9's return map."""
+        last = np.zeros(self.n, dtype=np.int8)
+        cnt = np.zeros(4, dtype=np.int64)
+        lib().oracle_get_dp_branches(self.h, last.ctypes.data_as(C.POINTER(C.c_int8)),
+                                     cnt.ctypes.data_as(C.POINTER(C.c_long)))
+        return last, cnt
+
     def scalars(self):
         out = np.empty(8)
         lib().oracle_get_scalars(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
+
+
+def dp_point(material, young, poisson, sd, sm):
+    """M3DM9 on one stress point with a zero strain increment.
+    material: an mpm_material ctypes struct (q_fai, k_fai, q_psi, ten_f set);
+    sd: 6 deviatoric components (xx yy zz yz xz xy).  Returns
+    (branch, sd_new, sm_new, seqv, depeff)."""
+    sdv = (C.c_double * 6)(*[float(v) for v in sd])
+    smv, seqv, dep = C.c_double(sm), C.c_double(0.0), C.c_double(0.0)
+    br = lib().oracle_dp_point(C.cast(C.pointer(material), C.c_void_p), young, poisson, sdv, C.byref(smv),
+                               C.byref(seqv), C.byref(dep))
+    return br, np.array(sdv[:]), smv.value, seqv.value, dep.value
 
 
 # ---------------------------------------------------------------- ref dumps

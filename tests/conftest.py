@@ -35,7 +35,10 @@ if ORACLE_DIR not in sys.path:
 DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
          "plate_neoh_musl_qulo", "dam_null_eos4_musl",
          # the remaining shape x scheme pairs of the fused kernels
-         "cube_elastic_linear_usf", "bar_j2_gimp_usf", "slope_dp_bspline_musl", "slope_dp_bspline_usl"]
+         "cube_elastic_linear_usf", "bar_j2_gimp_usf", "slope_dp_bspline_musl", "slope_dp_bspline_usl",
+         # Drucker-Prager blocks that yield on every branch of the return map
+         # within 10 steps (sheared and squeezed: USF; pulled apart: MUSL)
+         "block_dp_bspline_shear_usf", "block_dp_bspline_pull_musl"]
 # auxiliary-grid (SGMP) decks: clamped plate (MUSL), water column with
 # symmetric walls (MUSL), J2 block on a clamped floor (USF)
 SGMP_DECKS = ["plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf"]

@@ -32,7 +32,8 @@ import pyoracle  # noqa: E402
 DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
          "plate_neoh_musl_qulo", "dam_null_eos4_musl",
          "plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf",
-         "cube_elastic_linear_usf", "bar_j2_gimp_usf", "slope_dp_bspline_musl", "slope_dp_bspline_usl"]
+         "cube_elastic_linear_usf", "bar_j2_gimp_usf", "slope_dp_bspline_musl", "slope_dp_bspline_usl",
+         "block_dp_bspline_shear_usf", "block_dp_bspline_pull_musl"]
 S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp"]
 
 
