@@ -39,6 +39,11 @@ extern "C" {
 /* arithmetic of the bulk particle/grid state */
 #define MPM_PREC_FP64 0           /* plain FP64 everywhere */
 #define MPM_PREC_FP32 1           /* FP32 bulk, FP64 positions and small-mass division */
+/* plain FP64 with every grid sum taken in the reference's order (ascending
+ * particle index per node, update_step.f90:335-392, 946-988, 1705-1758) and
+ * every gather in its node-list order: bit-identical to the reference and
+ * reproducible from run to run (no floating-point atomics) */
+#define MPM_PREC_FP64_EXACT 2
 
 typedef struct mpm_ctx mpm_ctx;
 

@@ -16,12 +16,12 @@ import numpy as np
 
 from . import _lib
 from ._lib import (PF, PF_NAMES, PF_COUNT, GF, GF_NAMES, GF_COUNT, SHAPES, SCHEMES, PREC_FP64,
-                   PREC_FP32, MpmError, check, dptr, iptr)
+                   PREC_FP32, PREC_FP64_EXACT, MpmError, check, dptr, iptr)
 from .problem import Problem
 from . import serve
 
 __all__ = ["Problem", "Context", "Engine", "MpmError", "PF", "PF_NAMES", "PF_COUNT", "GF",
-           "GF_NAMES", "GF_COUNT", "PREC_FP64", "PREC_FP32", "read_deck"]
+           "GF_NAMES", "GF_COUNT", "PREC_FP64", "PREC_FP32", "PREC_FP64_EXACT", "read_deck"]
 
 
 def read_deck(path):
@@ -34,7 +34,9 @@ def _prec(p):
         return PREC_FP64
     if p in (PREC_FP32, "fp32", "f32", "float"):
         return PREC_FP32
-    raise ValueError("precision must be fp64 or fp32")
+    if p in (PREC_FP64_EXACT, "fp64exact", "exact"):
+        return PREC_FP64_EXACT
+    raise ValueError("precision must be fp64, fp32 or fp64exact")
 
 
 class Context:

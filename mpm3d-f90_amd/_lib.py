@@ -25,7 +25,7 @@ GF_COUNT = len(GF_NAMES)
 
 SHAPES = {"linear": 0, "gimp": 1, "bspline": 2, "sgmp": 3}
 SCHEMES = {"usf": 0, "usl": 1, "musl": 2}
-PREC_FP64, PREC_FP32 = 0, 1
+PREC_FP64, PREC_FP32, PREC_FP64_EXACT = 0, 1, 2
 
 MPM_OK = 0
 ERRORS = {-1: "MPM_ERR_ARG", -2: "MPM_ERR_IO", -3: "MPM_ERR_DECK", -4: "MPM_ERR_UNSUPPORTED",

@@ -36,7 +36,8 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-I" + 
 HIP_SOURCES = [("kernels.hip", "kernels_f64.o", ["-DMPMK_PART=1"]),
                ("kernels.hip", "kernels_f32.o", ["-DMPMK_PART=2", "-ffp-contract=fast",
                                                  "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
-               ("engine.hip", "engine.o", [])]
+               ("engine.hip", "engine.o", []),
+               ("radix.hip", "radix.o", [])]
 CXX_SOURCES = ["deck.cpp"]
 
 

@@ -101,6 +101,20 @@ enum DField { D_X = 0, D_Y, D_Z, D_VOL, D_IE, D_F0, D_F1, D_F2, D_F3, D_F4, D_F5
 // particle integer fields
 enum IField { I_BODY = 0, I_ORIG, I_CELL, I_COUNT };
 
+// Reference-order (exact FP64) scatter lists: the stencil entries of every
+// particle slot (entry q = j * np + slot, j the position in the shape
+// function's node list), their sort keys and, after the sort, the range of
+// sorted entries of every pool node.
+struct XPtrs {
+  unsigned long long* key;   // m sort keys: pool node << obits | original index << 5 | j (invalid: all ones)
+  int* rnode;                // m: instance-local node (SGMP: auxiliary node) of entry q, -1 if none
+  double* rw;                // 4 * m: N, dN/dx, dN/dy, dN/dz of entry q (field-major, stride m)
+  int* seg0;                 // nn: first sorted entry of every pool node
+  int* seg1;                 // nn: one past its last
+  long m;                    // entries: 27 * np
+  int obits;                 // key bits below the node index
+};
+
 template <typename T>
 struct PoolPtrs {
   T* t;          // T_COUNT * np

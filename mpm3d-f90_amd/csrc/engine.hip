@@ -2,7 +2,6 @@
 // tables, step sequences, graph capture, and the extern "C" entry points of
 // include/mpm3d_hip.h.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <array>
@@ -213,8 +212,7 @@ struct mpm_engine {
   unsigned* k_out = nullptr;
   int* v_in = nullptr;
   int* v_out = nullptr;
-  void* cub_tmp = nullptr;
-  size_t cub_bytes = 0;
+  unsigned* rs_scr = nullptr;     // radix-sort scratch for a sort over the whole pool
   void* t_alt = nullptr;
   double* d_alt = nullptr;
   int* i_alt = nullptr;
@@ -226,6 +224,20 @@ struct mpm_engine {
   int graph_steps = 0;
   bool graph_dirty = true;
   bool graph_plain_once = false;  // lazy replay: the changed lists already ran once without a graph
+  // reference-order FP64 mode (MPM_PREC_FP64_EXACT): stencil entries, their
+  // sort double buffers and the radix scratch, allocated before the first
+  // exact step
+  bool exact = false;
+  unsigned long long* x_key[2] = {nullptr, nullptr};
+  int* x_val[2] = {nullptr, nullptr};
+  int* x_rnode = nullptr;
+  double* x_rw = nullptr;
+  int* x_seg0 = nullptr;
+  int* x_seg1 = nullptr;
+  unsigned* x_scr = nullptr;
+  long x_m = 0;
+  int x_obits = 0, x_nbits = 0;
+  int x_sorted = 0;               // buffer holding the step's sorted entries
   // profiling mode: HIP events around every launch of mpm_step / engine steps
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -272,7 +284,9 @@ static void free_engine(mpm_engine* e) {
   if (e->graph) (void)hipGraphDestroy(e->graph);
   void* bufs[] = {e->t, e->d, e->ii, e->mg, e->pg, e->fg, e->p2, e->vg, e->ag, e->fix, e->d_inst, e->d_ts, e->d_mats,
                   e->d_bodies, e->part, e->d_ginst, e->d_nginst, e->d_ilist_all, e->k_in, e->k_out, e->v_in,
-                  e->v_out, e->cub_tmp, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg, e->boxes};
+                  e->v_out, e->rs_scr, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg, e->boxes,
+                  e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1], e->x_rnode, e->x_rw, e->x_seg0, e->x_seg1,
+                  e->x_scr};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& g : e->groups) {
@@ -307,6 +321,8 @@ static size_t bytes_per_node(int prec) {
 
 static mpm_engine* new_engine(long np_slots, long nn_slots, int max_inst, int max_mat, int max_body, int prec) {
   std::unique_ptr<mpm_engine, void (*)(mpm_engine*)> e(new mpm_engine, free_engine);
+  e->exact = prec == MPM_PREC_FP64_EXACT;
+  if (e->exact) prec = MPM_PREC_FP64;
   e->prec = prec;
   e->tsz = prec == MPM_PREC_FP32 ? 4 : 8;
   HIPCHK(hipGetDevice(&e->device));
@@ -816,6 +832,137 @@ enum Phase {
   PH_BEGIN, PH_GMI, PH_BC, PH_STRESS, PH_GMU, PH_INTEGRATE, PH_G2P, PH_MUSL, PH_ENERGY, PH_DT
 };
 
+// ---- reference-order FP64 mode (MPM_PREC_FP64_EXACT)
+
+static int bits_for(long v) {
+  int b = 0;
+  while ((1L << b) < v) b++;
+  return b;
+}
+
+// stencil-entry buffers of the whole pool: 27 entries per particle slot
+static void ensure_exact(mpm_engine* e) {
+  if (!e->exact || e->x_key[0]) return;
+  const long m = 27 * e->np;
+  if (m >= (1L << 31))
+    fail(MPM_ERR_UNSUPPORTED, "reference-order mode: %ld particle slots exceed 2^31 stencil entries", e->np);
+  const int obits = 5 + bits_for(e->np), nbits = obits + bits_for(e->nn);
+  if (nbits > 64) fail(MPM_ERR_UNSUPPORTED, "reference-order mode: sort keys of %d bits", nbits);
+  e->x_m = m;
+  e->x_obits = obits;
+  e->x_nbits = nbits;
+  for (int k = 0; k < 2; k++) {
+    e->x_key[k] = dalloc<unsigned long long>(m);
+    e->x_val[k] = dalloc<int>(m);
+  }
+  e->x_rnode = dalloc<int>(m);
+  e->x_rw = dalloc<double>(4 * m);
+  e->x_seg0 = dalloc<int>(e->nn);
+  e->x_seg1 = dalloc<int>(e->nn);
+  e->x_scr = dalloc<unsigned>(mpmk::radix_scratch_words(m));
+}
+
+static XPtrs xptrs(const mpm_engine* e) {
+  XPtrs X;
+  X.key = e->x_key[0];
+  X.rnode = e->x_rnode;
+  X.rw = e->x_rw;
+  X.seg0 = e->x_seg0;
+  X.seg1 = e->x_seg1;
+  X.m = e->x_m;
+  X.obits = e->x_obits;
+  return X;
+}
+
+// GridMomentumInitial of every group: ParticleInitial and the stencil
+// entries, one stable sort of all entries by (node, original particle
+// index, list position), the node ranges, then every node's sums in that
+// order (update_step.f90:288-406)
+static void exact_gmi(mpm_engine* e) {
+  ensure_exact(e);
+  const PoolPtrs<double> P = e->ptrs<double>();
+  const XPtrs X = xptrs(e);
+  hipStream_t s = e->s;
+  mpmk::launch_x_fill_keys(e->x_key[0], e->x_val[0], e->x_m, s);
+  for (Group& G : e->groups) mpmk::launch_x_pairs(G.shape, P, X, plist(e, G), s);
+  e->x_sorted = mpmk::radix_sort_pairs<unsigned long long>(e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1],
+                                                           e->x_m, 0, e->x_nbits, e->x_scr, s);
+  mpmk::launch_clear_ints(e->x_seg0, e->nn, s);
+  mpmk::launch_clear_ints(e->x_seg1, e->nn, s);
+  mpmk::launch_x_segments(e->x_key[e->x_sorted], e->x_m, e->x_obits, e->x_seg0, e->x_seg1, s);
+  const int* sval = e->x_val[e->x_sorted];
+  for (Group& G : e->groups) {
+    const LaunchP N = nlist(e, G);
+    if (G.shape == MPM_SHAPE_SGMP) {
+      mpmk::launch_x_scatter(XS_SGMP | XS_MASS | XS_MOM, P, X, sval, N, s);
+      mpmk::launch_sgmp_nodes<double>(SGN_MASSMOM, P, N, s);
+    } else {
+      mpmk::launch_x_scatter(XS_MASS | XS_MOM, P, X, sval, N, s);
+    }
+    for (int id : G.insts) e->inst[id].p_in_p2 = false;
+  }
+}
+
+// one later phase of one group in the reference-order mode: the particle
+// scatters reuse the step's sorted entries, the gathers walk each
+// particle's entries in list order; node passes and the SGMP centre <->
+// node passes are the general kernels
+static void exact_group_phase(mpm_engine* e, Group& G, int ph) {
+  const PoolPtrs<double> P = e->ptrs<double>();
+  const XPtrs X = xptrs(e);
+  hipStream_t s = e->s;
+  const LaunchP L = plist(e, G), N = nlist(e, G);
+  const bool sg = G.shape == MPM_SHAPE_SGMP, usf = G.scheme == MPM_USF;
+  const int* sval = e->x_val[e->x_sorted];
+  switch (ph) {
+    case PH_BC:
+      mpmk::launch_bc<double>(P, N, 0, 1, 0, s);
+      break;
+    case PH_STRESS:
+      if (sg) {
+        mpmk::launch_bc<double>(P, N, 0, 0, 1, s);
+        mpmk::launch_sgmp_cells<double>(SGC_GRAD, P, N, s);
+        mpmk::launch_x_sgmp_stress(usf ? 0 : 1, P, L, s);
+      } else {
+        mpmk::launch_x_stress(G.shape, usf ? 0 : 1, P, X, L, s);
+      }
+      for (int id : G.insts) e->inst[id].xx_pending = false;
+      break;
+    case PH_GMU:
+      if (sg) {
+        mpmk::launch_x_scatter(XS_SGMP | XS_FORCE, P, X, sval, N, s);
+        mpmk::launch_sgmp_nodes<double>(SGN_FORCE, P, N, s);
+      } else {
+        mpmk::launch_x_scatter(XS_FORCE, P, X, sval, N, s);
+      }
+      break;
+    case PH_INTEGRATE:
+      mpmk::launch_integrate<double>(P, N, 0, s);
+      break;
+    case PH_G2P:
+      if (sg) {
+        mpmk::launch_node_va<double>(P, N, s);
+        mpmk::launch_sgmp_cells<double>(SGC_KIN, P, N, s);
+        mpmk::launch_g2p<double>(MPM_SHAPE_SGMP, usf ? G2P_MOVE : 0, P, L, s);
+      } else {
+        mpmk::launch_x_g2p(G.shape, usf ? 1 : 0, P, X, L, s);
+      }
+      for (int id : G.insts) e->inst[id].xx_pending = !usf;
+      break;
+    case PH_MUSL:
+      if (sg) {
+        mpmk::launch_x_scatter(XS_SGMP | XS_MOM, P, X, sval, N, s);
+        mpmk::launch_sgmp_nodes<double>(SGN_MOM, P, N, s);
+      } else {
+        mpmk::launch_x_scatter(XS_MOM, P, X, sval, N, s);
+      }
+      mpmk::launch_bc<double>(P, N, 0, 1, 0, s);
+      break;
+    default:
+      break;
+  }
+}
+
 template <typename T>
 static void run_phase(mpm_engine* e, int ph) {
   rebuild_lists(e);
@@ -823,6 +970,13 @@ static void run_phase(mpm_engine* e, int ph) {
   hipStream_t s = e->s;
   if (ph == PH_BEGIN) {
     mpmk::launch_begin_step(e->d_ts, e->d_ilist_all, e->n_active, s);
+    return;
+  }
+  if (e->exact && ph != PH_ENERGY && ph != PH_DT) {
+    if (ph == PH_GMI) exact_gmi(e);
+    else
+      for (Group& G : e->groups) exact_group_phase(e, G, ph);
+    HIPCHK(hipGetLastError());
     return;
   }
   for (Group& G : e->groups) {
@@ -980,9 +1134,40 @@ static void enqueue_sgmp(mpm_engine* e, Group& G, const PoolPtrs<T>& P, const La
   }
 }
 
+// one step of every active instance in the reference-order mode: the
+// phases of update_step.f90 one by one, in the call order of
+// MPM3D.f90:69-113 (no fusion: each scatter is its own sorted-order pass)
+static void enqueue_step_exact(mpm_engine* e, int fin_flags) {
+  const PoolPtrs<double> P = e->ptrs<double>();
+  hipStream_t s = e->s;
+  timed(e, KI_BEGIN, [&] { mpmk::launch_begin_step(e->d_ts, e->d_ilist_all, e->n_active, s); });
+  timed(e, KI_P2G, [&] { exact_gmi(e); });
+  for (Group& G : e->groups) {
+    const LaunchP L = plist(e, G);
+    auto ph = [&](int kid, int p) { timed(e, kid, [&] { exact_group_phase(e, G, p); }); };
+    if (G.scheme == MPM_USF) {
+      ph(KI_BC, PH_BC);
+      ph(KI_STRESS, PH_STRESS);
+    }
+    ph(KI_P2G, PH_GMU);
+    ph(KI_INTEGRATE, PH_INTEGRATE);
+    ph(KI_G2P, PH_G2P);
+    if (G.scheme == MPM_MUSL) ph(KI_P2G, PH_MUSL);
+    if (G.scheme != MPM_USF) ph(KI_STRESS, PH_STRESS);
+    timed(e, KI_FINALIZE, [&] {
+      mpmk::launch_energy<double>(P, L, s);
+      mpmk::launch_finalize<double>(P, G.d_ilist, (int)G.insts.size(), FIN_ENERGY | FIN_DT | fin_flags, s);
+    });
+  }
+}
+
 // one fused step of every active instance (MPM3D.f90:69-113)
 template <typename T>
 static void enqueue_step(mpm_engine* e, int fin_flags) {
+  if (e->exact) {
+    enqueue_step_exact(e, fin_flags);
+    return;
+  }
   const PoolPtrs<T> P = e->ptrs<T>();
   hipStream_t s = e->s;
   timed(e, KI_BEGIN, [&] { mpmk::launch_begin_step(e->d_ts, e->d_ilist_all, e->n_active, s); });
@@ -1036,7 +1221,7 @@ static void collect_times(mpm_engine* e) {
 static void mark_after_step(mpm_engine* e) {
   for (auto& G : e->groups)
     for (int id : G.insts) {
-      e->inst[id].p_in_p2 = G.scheme == MPM_MUSL;
+      e->inst[id].p_in_p2 = G.scheme == MPM_MUSL && !e->exact;   // the exact step re-maps into PXg
       e->inst[id].xx_pending = false;
     }
 }
@@ -1079,6 +1264,7 @@ static void step_graph(mpm_engine* e, int nsteps, int per, int fin_flags, bool l
     e->graph = nullptr;
     const bool was_prof = e->prof;
     e->prof = false;
+    ensure_exact(e);   // no allocation inside the capture
     HIPCHK(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < per; k++) {
       if (e->prec == MPM_PREC_FP32) enqueue_step<float>(e, fin_flags);
@@ -1128,17 +1314,9 @@ static void ensure_sort_buffers(mpm_engine* e) {
     e->t_alt = dalloc<char>((size_t)T_COUNT * np * e->tsz);
     e->d_alt = dalloc<double>((size_t)D_COUNT * np);
     e->i_alt = dalloc<int>((size_t)I_COUNT * np);
-    // temporary storage of a radix sort over the whole pool, which covers
-    // the single-instance sorts at admission (no reallocation there)
-    size_t need = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, e->k_in, e->k_out, e->v_in, e->v_out, (int)np, 0, 32,
-                                              e->s));
-    if (need > e->cub_bytes) {
-      if (e->cub_tmp) HIPCHK(hipFree(e->cub_tmp));
-      e->cub_tmp = nullptr;
-      e->cub_tmp = dalloc<char>(need);
-      e->cub_bytes = need;
-    }
+    // scratch of a radix sort over the whole pool, which covers the
+    // single-instance sorts at admission (no reallocation there)
+    e->rs_scr = dalloc<unsigned>(mpmk::radix_scratch_words(np));
   }
 }
 
@@ -1158,23 +1336,15 @@ static void sort_instance(mpm_engine* e, int id) {
   int end_bit = 1;
   const unsigned maxkey = max_sort_key(h.c, e->brick);
   while (end_bit < 32 && (1u << end_bit) <= maxkey) end_bit++;
-  size_t need = 0;
-  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, e->k_in + b0, e->k_out + b0, e->v_in + b0, e->v_out + b0,
-                                            (int)pad, 0, end_bit, e->s));
-  if (need > e->cub_bytes) {
-    HIPCHK(hipStreamSynchronize(e->s));
-    if (e->cub_tmp) HIPCHK(hipFree(e->cub_tmp));
-    e->cub_tmp = dalloc<char>(need);
-    e->cub_bytes = need;
-  }
-  HIPCHK(hipcub::DeviceRadixSort::SortPairs(e->cub_tmp, need, e->k_in + b0, e->k_out + b0, e->v_in + b0,
-                                            e->v_out + b0, (int)pad, 0, end_bit, e->s));
+  const int par = mpmk::radix_sort_pairs<unsigned>(e->k_in + b0, e->k_out + b0, e->v_in + b0, e->v_out + b0, pad, 0,
+                                                   end_bit, e->rs_scr, e->s);
+  const int* perm = par ? e->v_out : e->v_in;
   if (e->prec == MPM_PREC_FP32)
-    mpmk::launch_permute<float>((float*)e->t_alt, (const float*)e->t, e->v_out, L, np, T_COUNT, e->s);
+    mpmk::launch_permute<float>((float*)e->t_alt, (const float*)e->t, perm, L, np, T_COUNT, e->s);
   else
-    mpmk::launch_permute<double>((double*)e->t_alt, (const double*)e->t, e->v_out, L, np, T_COUNT, e->s);
-  mpmk::launch_permute<double>(e->d_alt, e->d, e->v_out, L, np, D_COUNT, e->s);
-  mpmk::launch_permute<int>(e->i_alt, e->ii, e->v_out, L, np, I_COUNT, e->s);
+    mpmk::launch_permute<double>((double*)e->t_alt, (const double*)e->t, perm, L, np, T_COUNT, e->s);
+  mpmk::launch_permute<double>(e->d_alt, e->d, perm, L, np, D_COUNT, e->s);
+  mpmk::launch_permute<int>(e->i_alt, e->ii, perm, L, np, I_COUNT, e->s);
   for (int f = 0; f < T_COUNT; f++)
     HIPCHK(hipMemcpyAsync((char*)e->t + ((size_t)f * np + b0) * e->tsz, (char*)e->t_alt + ((size_t)f * np + b0) * e->tsz,
                           pad * e->tsz, hipMemcpyDeviceToDevice, e->s));
@@ -1237,35 +1407,33 @@ static void sort_particles(mpm_engine* e) {
     e->h_seg = seg;
     HIPCHK(hipMemcpyAsync(e->d_seg, e->h_seg.data(), seg.size() * sizeof(int), hipMemcpyHostToDevice, e->s));
   }
-  size_t need = 0;
-  // one instance, or tiled segments: plain radix sort of the slot range
-  // (the segment rank above the cell key); otherwise segmented
-  const int b0 = seg[0], n0 = (nseg == 1 || global) ? seg[2 * nseg - 1] - seg[0] : 0;
-  const int nbits = global ? end_bit + seg_bits : end_bit;
-  if (global) mpmk::launch_seg_prefix(e->k_in, e->d_seg, nseg, b0, n0, end_bit, e->s);
-  auto run_sort = [&](void* tmp, size_t& bytes) {
-    if (nseg == 1 || global)
-      return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, e->k_in + b0, e->k_out + b0, e->v_in + b0,
-                                                e->v_out + b0, n0, 0, nbits, e->s);
-    return hipcub::DeviceSegmentedRadixSort::SortPairs(tmp, bytes, e->k_in, e->k_out, e->v_in, e->v_out, (int)np,
-                                                       nseg, e->d_seg, e->d_seg + nseg, 0, end_bit, e->s);
-  };
-  HIPCHK(run_sort(nullptr, need));
-  if (need > e->cub_bytes) {
-    HIPCHK(hipStreamSynchronize(e->s));
-    if (e->cub_tmp) HIPCHK(hipFree(e->cub_tmp));
-    e->cub_tmp = dalloc<char>(need);
-    e->cub_bytes = need;
+  // one instance, or tiled segments: one radix sort of the slot range
+  // (the segment rank above the cell key); otherwise one sort per segment
+  // (every segment takes the same number of passes, so all results land in
+  // the same buffer)
+  int par = 0;
+  if (nseg == 1 || global) {
+    const int b0 = seg[0], n0 = seg[2 * nseg - 1] - seg[0];
+    const int nbits = global ? end_bit + seg_bits : end_bit;
+    if (global) mpmk::launch_seg_prefix(e->k_in, e->d_seg, nseg, b0, n0, end_bit, e->s);
+    par = mpmk::radix_sort_pairs<unsigned>(e->k_in + b0, e->k_out + b0, e->v_in + b0, e->v_out + b0, n0, 0, nbits,
+                                           e->rs_scr, e->s);
+  } else {
+    for (int k = 0; k < nseg; k++) {
+      const int b0 = seg[k], n0 = seg[nseg + k] - seg[k];
+      par = mpmk::radix_sort_pairs<unsigned>(e->k_in + b0, e->k_out + b0, e->v_in + b0, e->v_out + b0, n0, 0, end_bit,
+                                             e->rs_scr, e->s);
+    }
   }
-  HIPCHK(run_sort(e->cub_tmp, need));
+  const int* perm = par ? e->v_out : e->v_in;
   for (auto& G : e->groups) {
     const LaunchP L = plist(e, G);
     if (e->prec == MPM_PREC_FP32)
-      mpmk::launch_permute<float>((float*)e->t_alt, (const float*)e->t, e->v_out, L, np, T_COUNT, e->s);
+      mpmk::launch_permute<float>((float*)e->t_alt, (const float*)e->t, perm, L, np, T_COUNT, e->s);
     else
-      mpmk::launch_permute<double>((double*)e->t_alt, (const double*)e->t, e->v_out, L, np, T_COUNT, e->s);
-    mpmk::launch_permute<double>(e->d_alt, e->d, e->v_out, L, np, D_COUNT, e->s);
-    mpmk::launch_permute<int>(e->i_alt, e->ii, e->v_out, L, np, I_COUNT, e->s);
+      mpmk::launch_permute<double>((double*)e->t_alt, (const double*)e->t, perm, L, np, T_COUNT, e->s);
+    mpmk::launch_permute<double>(e->d_alt, e->d, perm, L, np, D_COUNT, e->s);
+    mpmk::launch_permute<int>(e->i_alt, e->ii, perm, L, np, I_COUNT, e->s);
   }
   // the cached node boxes describe the old particle order
   mpmk::launch_clear_ints(e->boxes, (e->np / 64) * BOX_INTS, e->s);
@@ -1630,7 +1798,8 @@ extern "C" int mpm_ctx_create(const mpm_problem* prob, int precision, mpm_ctx** 
   return guarded([&] {
     if (!out) fail(MPM_ERR_ARG, "out is NULL");
     *out = nullptr;
-    if (precision != MPM_PREC_FP64 && precision != MPM_PREC_FP32) fail(MPM_ERR_ARG, "bad precision");
+    if (precision != MPM_PREC_FP64 && precision != MPM_PREC_FP32 && precision != MPM_PREC_FP64_EXACT)
+      fail(MPM_ERR_ARG, "bad precision");
     check_problem(prob);
     const long nnode = (long)(prob->ncx + 1) * (prob->ncy + 1) * (prob->ncz + 1);
     mpm_engine* e = new_engine(prob->nb_particle, nnode, 1, prob->nb_mat, prob->nb_body, precision);
@@ -1763,7 +1932,8 @@ extern "C" int mpm_engine_create(long max_particles, long max_nodes, int max_ins
   return guarded([&] {
     if (!out || max_particles <= 0 || max_nodes <= 0 || max_instances <= 0 || max_materials <= 0 || max_bodies <= 0)
       fail(MPM_ERR_ARG, "bad arguments");
-    if (precision != MPM_PREC_FP64 && precision != MPM_PREC_FP32) fail(MPM_ERR_ARG, "bad precision");
+    if (precision != MPM_PREC_FP64 && precision != MPM_PREC_FP32 && precision != MPM_PREC_FP64_EXACT)
+      fail(MPM_ERR_ARG, "bad precision");
     *out = new_engine(max_particles, max_nodes, max_instances, max_materials, max_bodies, precision);
   });
 }

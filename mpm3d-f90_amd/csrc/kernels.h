@@ -39,8 +39,17 @@
 #define AUX_COUNT 20
 #define AUX_FORCE_OFF 3
 
+// reference-order scatter (exact FP64 mode): what k_x_scatter sums
+#define XS_MASS 1     // Mg (SGMP: Cmg)
+#define XS_MOM 2      // PXg (SGMP: Cpxg)
+#define XS_FORCE 4    // FXg (SGMP: Cfext, Cfint)
+#define XS_SGMP 8     // to the auxiliary nodes
+
 // all material models (general stress kernel)
 #define MM_ALL 0xffff
+// flag bit of the material mask: the neo-Hookean ln J is the correctly
+// rounded logarithm (crlog.h) instead of the device libm's
+#define MM_CRLOG (1 << 30)
 
 // k_finalize parts
 #define FIN_ENERGY 1
@@ -86,4 +95,23 @@ template <typename T> void launch_sgmp_nodes(int what, const PoolPtrs<T>& P, con
 template <typename T> void launch_sgmp_cells(int what, const PoolPtrs<T>& P, const LaunchP& N, hipStream_t s);
 void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP& L, int brick, unsigned* keys,
                       int* vals, hipStream_t s);
+
+// stable LSD radix sort of (key, value) pairs over key bits [begin_bit,
+// end_bit) (radix.hip); returns 0 when the result is in (k0, v0), 1 when in
+// (k1, v1); scratch: radix_scratch_words(n) words
+size_t radix_scratch_words(long n);
+template <typename K>
+int radix_sort_pairs(K* k0, K* k1, int* v0, int* v1, long n, int begin_bit, int end_bit, unsigned* scratch,
+                     hipStream_t s);
+
+// reference-order FP64 step (exact.inc)
+void launch_x_fill_keys(unsigned long long* key, int* val, long m, hipStream_t s);
+void launch_x_pairs(int shape, const PoolPtrs<double>& P, const XPtrs& X, const LaunchP& L, hipStream_t s);
+void launch_x_segments(const unsigned long long* key, long m, int obits, int* seg0, int* seg1, hipStream_t s);
+void launch_x_scatter(int what, const PoolPtrs<double>& P, const XPtrs& X, const int* sval, const LaunchP& N,
+                      hipStream_t s);
+void launch_x_stress(int shape, int move, const PoolPtrs<double>& P, const XPtrs& X, const LaunchP& L,
+                     hipStream_t s);
+void launch_x_g2p(int shape, int move, const PoolPtrs<double>& P, const XPtrs& X, const LaunchP& L, hipStream_t s);
+void launch_x_sgmp_stress(int move, const PoolPtrs<double>& P, const LaunchP& L, hipStream_t s);
 }  // namespace mpmk

@@ -11,6 +11,7 @@
 // grid mass and FP64 momentum/mass division).
 #include <hip/hip_runtime.h>
 
+#include "crlog.h"
 #include "engine.h"
 #include "kernels.h"
 
@@ -1908,7 +1909,10 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
       B[5] = F[0][0] * F[2][0] + F[0][1] * F[2][1] + F[0][2] * F[2][2];
       m.ieinc = sig_dot(m);
       const double iJ = 1.0 / J;
-      const double lj = mc.lamda * log(J) * 1.0;
+      // ln J: the correctly rounded logarithm in the reference-order mode
+      // (reproducible on any libm), the device libm's otherwise
+      const double lnJ = (MM & MM_CRLOG) ? mpm_log_cr(J) : log(J);
+      const double lj = mc.lamda * lnJ * 1.0;
       double s[6];
       s[0] = iJ * (G * B[0] + lj);
       s[1] = iJ * (G * B[1] + lj);
@@ -3341,6 +3345,10 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
     hipLaunchKernelGGL(k_sort_keys, dim3(L.nblk), dim3(L.bs), 0, s, X, np, inst, L.glist, L.ginst, L.bpg, brick, keys,
                        vals);
 }
+#endif
+
+#if MPMK_PART != 2
+#include "exact.inc"
 #endif
 
 #define INST(T)                                                                                     \

@@ -14,6 +14,8 @@
  */
 #include "mpm_oracle.h"
 
+#include "../mpm3d-f90_amd/csrc/crlog.h"
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -49,6 +51,9 @@ struct oracle {
    * particle's last stress update and the running count of each branch */
   int8_t* dpbr;
   long dpcount[4];
+  /* ln J of M3DM10: 0 the C library's log (as the reference's Fortran
+   * runtime), 1 the correctly rounded logarithm of crlog.h */
+  int log_cr;
 };
 
 #define P(o, f, i) ((o)->ps[(size_t)(f) * (o)->n + (i)])
@@ -681,6 +686,14 @@ static int gimp_offsets(int zc, int yc, int xc, int off[27][3]) {
   return n;
 }
 
+/* test entry: the node list of one GIMP case, n offsets into off */
+int oracle_gimp_offsets(int zc, int yc, int xc, int* off) {
+  int o3[27][3];
+  int n = gimp_offsets(zc, yc, xc, o3);
+  memcpy(off, o3, sizeof(int) * 3 * (size_t)n);
+  return n;
+}
+
 /* FindInflNode + NShape_GIMP (Grid.f90:546-599, 602-849) */
 static void gimp_shape(const oracle* o, int p, int c, shape_t* s) {
   double x[3] = {P(o, PF_X, p), P(o, PF_Y, p), P(o, PF_Z, p)};
@@ -962,6 +975,7 @@ typedef struct {
   double iener, specen, ieinc, den0_, den_, vol0_, cp, yield0_;
   double F[3][3];
   int br;   /* M3DM9 branch taken (DP_BR_*), diagnostics only */
+  int log_cr;
 } cst_t;
 
 /* EquivalentStress returns a default REAL (Constitution.f90:679-699) */
@@ -1170,7 +1184,7 @@ static void m3dm10(cst_t* m) {         /* :725-780 neo-Hookean */
   B[5] = F[0][0] * F[2][0] + F[0][1] * F[2][1] + F[0][2] * F[2][2];
   m->ieinc = sig_dot_dinc(m);
   double iJ = 1.0 / J;
-  double lj = m->lamda * log(J) * 1.0;
+  double lj = m->lamda * (m->log_cr ? mpm_log_cr(J) : log(J)) * 1.0;
   m->sig[0] = iJ * (G * B[0] + lj);
   m->sig[1] = iJ * (G * B[1] + lj);
   m->sig[2] = iJ * (G * B[2] + lj);
@@ -1203,6 +1217,7 @@ static void constitution(oracle* o, int p, const double de[6], const double vort
   cst_t m;
   memset(&m, 0, sizeof m);
   for (int i = 0; i < 6; i++) m.dinc[i] = de[i];
+  m.log_cr = o->log_cr;
   m.vold = P(o, PF_VOL, p);
   if (o->prob.update_v_by_fp) {
     double deF[3][3];
@@ -1419,6 +1434,10 @@ void oracle_step(oracle* o, int nsteps) {
 void oracle_get_particles(const oracle* o, double* out) {
   memcpy(out, o->ps, sizeof(double) * PF_COUNT * o->n);
 }
+
+void oracle_set_log_cr(oracle* o, int on) { o->log_cr = on != 0; }
+
+double oracle_log_cr(double x) { return mpm_log_cr(x); }
 
 void oracle_get_dp_branches(const oracle* o, int8_t* last, long* counts4) {
   if (last) memcpy(last, o->dpbr, (size_t)o->n);

@@ -50,6 +50,13 @@ void oracle_get_fix(const oracle* o, int8_t* out);         /* 3 * nb_node */
 void oracle_get_scalars(const oracle* o, double* out);
 void oracle_set_particles(oracle* o, const double* in);
 
+/* ln J of the neo-Hookean model (M3DM10): on = 0 the C library's log (the
+ * default, as the reference's Fortran runtime), 1 the correctly rounded
+ * logarithm that the device's reference-order mode uses; oracle_log_cr is
+ * that logarithm (test entry) */
+void oracle_set_log_cr(oracle* o, int on);
+double oracle_log_cr(double x);
+
 /* Diagnostics of the Drucker-Prager return map (M3DM9, Constitution.f90:
  * 1263-1321), not reference state: the branch each particle took in its
  * last stress update, and running counts per branch over all updates. */
@@ -57,6 +64,9 @@ enum { DP_BR_ELASTIC = 0, DP_BR_SHEAR = 1, DP_BR_SHEAR_TENSILE = 2, DP_BR_TENSIO
 void oracle_get_dp_branches(const oracle* o, int8_t* last, long* counts4);
 /* M3DM9 on one stress point with a zero strain increment (trial = input):
  * returns the branch; sd[6], *sm updated, *seqv and *depeff written. */
+/* FindInflNode's node list for the axis cases zc, yc, xc (-1, 0 or 2):
+ * writes n (dx, dy, dz) offsets from the cell's first node, returns n. */
+int oracle_gimp_offsets(int zc, int yc, int xc, int* off);
 int oracle_dp_point(const mpm_material* mat, double young, double poisson, double sd[6], double* sm, double* seqv,
                     double* depeff);
 

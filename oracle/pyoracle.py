@@ -50,6 +50,9 @@ def lib():
         L.oracle_get_icell.argtypes = [vp, C.POINTER(C.c_int32)]
         L.oracle_get_fix.argtypes = [vp, C.POINTER(C.c_int8)]
         L.oracle_get_dp_branches.argtypes = [vp, C.POINTER(C.c_int8), C.POINTER(C.c_long)]
+        L.oracle_set_log_cr.argtypes = [vp, C.c_int]
+        L.oracle_log_cr.argtypes = [C.c_double]
+        L.oracle_log_cr.restype = C.c_double
         dp = C.POINTER(C.c_double)
         L.oracle_dp_point.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp]
         L.oracle_dp_point.restype = C.c_int
@@ -60,10 +63,14 @@ def lib():
 class Oracle:
     """FP64 CPU restatement of one instance; method names follow update_step.f90."""
 
-    def __init__(self, problem):
+    def __init__(self, problem, log_cr=False):
+        """log_cr: the neo-Hookean ln J from the correctly rounded logarithm
+        (the device's reference-order mode) instead of the C library's."""
         self._s = problem.struct()
         self._prob = problem
         self.h = lib().oracle_create(C.cast(C.pointer(self._s), C.c_void_p))
+        if log_cr:
+            lib().oracle_set_log_cr(self.h, 1)
         self.n = problem.n
         self.nnode = lib().oracle_nb_node(self.h)
 
