@@ -16,8 +16,14 @@ Workloads:
 Reported:
 * value: particle-steps/s of the FP32-bulk batched ensemble (FP32
   particle/grid arithmetic; FP64 positions, volume, internal energy, F and
-  grid mass), inputs resident in HBM, K timed steps after W warm-up steps;
-* fp64_plain / fp32_plain / fp64_batched and the speedup over fp64_plain;
+  grid mass), inputs resident in HBM, K timed steps after W warm-up steps,
+  at the steady-state step time: the window's time without the re-sorts
+  that fell in it, plus one separately timed re-sort per re-sort period
+  (timed_ms_per_step keeps the raw window);
+* fp64_plain (FP64, scatter sums in atomic order), fp64_exact_plain (FP64 in
+  the reference's summation order, bit-identical to it), fp32_plain,
+  fp64_batched, and the speedups over both FP64 paths; every leg timed with
+  profiling off, per-kernel times from a separate profiled pass;
 * parity: the single slope after a few steps and three ensemble instances
   after every step the engine ran, against the CPU FP64 oracle (normwise per
   physical group);
@@ -27,7 +33,8 @@ Reported:
 * cpu_baseline: the reference's own Fortran build (oracle/_ref/ref_driver)
   timed on the host on the first three ensemble instances for a bounded
   number of steps, 1 core; the CPU oracle is timed instead when that binary
-  cannot run.
+  cannot run; cpu_baseline_all_cores: one such run per host core at once;
+* taylor_bar and sgmp lines with their parity against the FP64 oracle.
 """
 import argparse
 import json
@@ -93,19 +100,45 @@ def measured_traffic(kernel, prec, workload):
     return None
 
 
-def time_steps(ctx, steps, warmup, profile):
-    ctx.step(warmup)
-    ctx.sync()
-    if profile:
-        ctx.profile(True)
+def time_sort(obj, reps=3):
+    """mean milliseconds of one whole-pool re-sort (synchronised)"""
+    obj.sync()
     t0 = time.perf_counter()
-    ctx.step(steps)
-    ctx.sync()
-    dt = time.perf_counter() - t0
-    kt = ctx.kernel_times() if profile else {}
-    if profile:
-        ctx.profile(False)
-    return dt, kt
+    for _ in range(reps):
+        obj.sort()
+    obj.sync()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def timed_window(obj, steps, step, period):
+    """Exactly `steps` steps timed with the host clock between
+    synchronisations (profiling off), then the steady-state step time: the
+    re-sorts that fell in the window are counted, one re-sort is timed on its
+    own, and the step time without re-sorts gets one re-sort per period
+    added, whatever the window length."""
+    s0 = obj.counters()[1]
+    obj.sync()
+    t0 = time.perf_counter()
+    step(steps)
+    obj.sync()
+    sec = time.perf_counter() - t0
+    nsort = obj.counters()[1] - s0
+    sort_ms = time_sort(obj)
+    step_ms = max(sec * 1e3 - nsort * sort_ms, 0.0) / steps
+    amort = step_ms + (sort_ms / period if period > 0 else 0.0)
+    return {"sec": sec, "timed_ms_per_step": sec * 1e3 / steps, "sorts_in_window": nsort, "sort_ms": sort_ms,
+            "sort_every": period, "step_ms_without_sort": step_ms, "ms_per_step": amort}
+
+
+def profiled_kernels(obj, steps, step):
+    """per-kernel (total ms, launches) from HIP events around every launch of
+    a separate run of `steps` steps (not the timed window)"""
+    obj.sync()
+    obj.profile(True)
+    step(steps)
+    kt = obj.kernel_times()
+    obj.profile(False)
+    return kt
 
 
 def group_err(mpm, a, b, groups):
@@ -168,6 +201,46 @@ def cpu_baseline(sample, mpm, decks, nsteps, log):
                       % (nsteps, len(sample), "+".join(map(str, sizes)))}
 
 
+def cpu_baseline_all_cores(sample, nsteps, log):
+    """The reference Fortran build as k independent single-threaded runs at
+    once, one per host core of this process (up to 16): aggregate
+    particle-steps/s of the k runs over the wall time of the slowest."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(ref):
+        return None
+    cores = sorted(os.sched_getaffinity(0))[:16]
+    d = tempfile.mkdtemp(prefix="mpmrefk_")
+    try:
+        for k, (text, n, _) in enumerate(sample):
+            with open(os.path.join(d, "s%d.mpm" % k), "w") as f:
+                f.write(text)
+        procs = []
+        t0 = time.perf_counter()
+        for j, c in enumerate(cores):
+            k = j % len(sample)
+            procs.append((k, subprocess.Popen(["taskset", "-c", str(c), ref, "s%d.mpm" % k, str(nsteps),
+                                               os.path.join(d, "o%d" % j), "-1"], cwd=d,
+                                              stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+        tot_ps, loops = 0, []
+        for k, p in procs:
+            so, _ = p.communicate(timeout=900)
+            m = re.search(r"STEPLOOP\s+(\S+)\s+(\d+)\s+(\d+)", so)
+            if p.returncode != 0 or not m:
+                return None
+            tot_ps += int(m.group(2)) * int(m.group(3))
+            loops.append(float(m.group(1)))
+        wall = time.perf_counter() - t0
+    except (OSError, subprocess.SubprocessError) as e:
+        log("all-core reference run failed: %s" % e)
+        return None
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"value": tot_ps / max(loops), "unit": "particle-steps/s", "cores": len(cores), "kind": "reference",
+            "sample": "%d concurrent single-threaded reference runs (one per core), %d steps of one of the first %d "
+                      "ensemble instances each; particle-steps of all runs over the slowest run's step loop "
+                      "(%.1f s; %.1f s wall with input)" % (len(cores), nsteps, len(sample), max(loops), wall)}
+
+
 def largest_divisor_upto(p, cap):
     return max(d for d in range(1, min(p, cap) + 1) if p % d == 0)
 
@@ -184,18 +257,29 @@ def roofline(kt, n_particles, n_touched, tsz, traffic_of):
     achieved = alg / (ms_avg * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_of(dom), "algorithmic_bytes": alg,
-            "avg_launch_ms": ms_avg}
+            "avg_launch_ms": ms_avg,
+            "traffic_source": "committed rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, "
+                              "profiles/slope_ensemble_pmc_traffic.md"}
 
 
-def run_plain(mpm, prob, a, log):
-    """The single slope on the plain (one-instance, fused-step) path."""
+PLAIN_SORT_EVERY = 64   # the engine's default re-sort period (contexts never autotune)
+
+
+def run_plain(mpm, prob, a, log, precs=("fp64", "fp32", "fp64exact")):
+    """The single slope on the plain (one-instance, fused-step) path:
+    FP64 (atomic scatter order), FP32 bulk, and the reference-order FP64
+    mode (bit-identical to the reference)."""
     res = {}
-    for prec in ("fp64", "fp32"):
+    for prec in precs:
         ctx = mpm.Context(prob, prec)
-        sec, kt = time_steps(ctx, a.steps, a.warmup, profile=True)
+        ctx.step(a.warmup)
+        w = timed_window(ctx, a.steps, ctx.step, PLAIN_SORT_EVERY)
+        kt = profiled_kernels(ctx, a.profile_steps, ctx.step)
         touched = int(np.count_nonzero(ctx.grid()[0]))
-        res[prec] = {"sec": sec, "kt": kt, "touched": touched, "value": prob.n * a.steps / sec}
-        log("plain %s: %.3f ms/step, %.4g particle-steps/s" % (prec, sec / a.steps * 1e3, res[prec]["value"]))
+        res[prec] = dict(w, kt=kt, touched=touched, value=prob.n / (w["ms_per_step"] * 1e-3))
+        log("plain %s: %.3f ms/step (timed %.3f, %d re-sorts of %.2f ms in the window), %.4g particle-steps/s"
+            % (prec, w["ms_per_step"], w["timed_ms_per_step"], w["sorts_in_window"], w["sort_ms"],
+               res[prec]["value"]))
         ctx.close()
     return res
 
@@ -232,18 +316,12 @@ def run_ensemble(mpm, decks, a, log):
         log("ensemble %s tuning: %s, graph of %d steps" % (prec, json.dumps(tune), per))
         eng.step(per, graph=a.graph, graph_steps=per)   # capture outside the timed region
         eng.sync()
-        t0 = time.perf_counter()
-        eng.step(a.steps, graph=a.graph, graph_steps=per)
-        eng.sync()
-        sec = time.perf_counter() - t0
-        eng.profile(True)
-        eng.step(a.profile_steps)
-        kt = eng.kernel_times()
-        eng.profile(False)
-        out[prec] = {"sec": sec, "kt": kt, "tuning": tune, "graph_steps": per,
-                     "value": total_p * a.steps / sec, "ms_per_step": sec / a.steps * 1e3}
-        log("ensemble %s: %.3f ms/step, %.4g particle-steps/s" % (prec, out[prec]["ms_per_step"],
-                                                                 out[prec]["value"]))
+        w = timed_window(eng, a.steps, lambda n: eng.step(n, graph=a.graph, graph_steps=per), tune["sort_every"])
+        kt = profiled_kernels(eng, a.profile_steps, eng.step)
+        out[prec] = dict(w, kt=kt, tuning=tune, graph_steps=per, value=total_p / (w["ms_per_step"] * 1e-3))
+        log("ensemble %s: %.3f ms/step (timed %.3f, %d re-sorts of %.2f ms in the window), %.4g particle-steps/s"
+            % (prec, w["ms_per_step"], w["timed_ms_per_step"], w["sorts_in_window"], w["sort_ms"],
+               out[prec]["value"]))
     out["touched"] = touched_nodes(engs["fp32"], ids["fp32"])
     out["particles"] = total_p
     out["nodes"] = total_n
@@ -285,11 +363,30 @@ def run_bar(mpm, decks, a, log):
     prob = decks.load(text, mpm, "taylor")
     out = {"workload": "Taylor bar R=3.2 mm L=32.4 mm, 190 m/s, copper J2 (400 MPa, H=100 MPa), GIMP, MUSL, "
                        "h=0.4 mm, 2x2x2 particles/cell", "particles": prob.n}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", "fp32", "fp64exact"):
         ctx = mpm.Context(prob, prec)
-        sec, kt = time_steps(ctx, a.steps, a.warmup, profile=False)
-        out[prec] = {"value": prob.n * a.steps / sec, "ms_per_step": sec / a.steps * 1e3}
+        ctx.step(a.warmup)
+        w = timed_window(ctx, a.steps, ctx.step, PLAIN_SORT_EVERY)
+        out[prec] = {"value": prob.n / (w["ms_per_step"] * 1e-3), "ms_per_step": w["ms_per_step"],
+                     "timed_ms_per_step": w["timed_ms_per_step"]}
         ctx.close()
+    if not a.no_cpu:   # every path against the FP64 oracle after a few full-size steps
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        o = pyoracle.Oracle(prob)
+        o.step(a.check_steps)
+        ref = o.particles()
+        par = {"steps": a.check_steps}
+        for prec in ("fp64", "fp32", "fp64exact"):
+            ctx = mpm.Context(prob, prec)
+            ctx.step(a.check_steps)
+            ps = ctx.particles()
+            par[prec + "_vs_fp64_oracle"] = group_err(mpm, ps, ref, GROUPS)
+            if prec == "fp64exact":
+                par["fp64exact_bitwise"] = bool(np.array_equal(ps, ref, equal_nan=True)
+                                                and np.array_equal(ctx.grid(), o.grid(), equal_nan=True))
+            ctx.close()
+        out["parity"] = par
     log("taylor bar: %s" % json.dumps(out))
     if not a.no_cpu:
         ref = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
@@ -332,35 +429,28 @@ def run_sgmp(mpm, decks, a, log):
     per = largest_divisor_upto(tune["sort_every"], 10) if tune["sort_every"] > 0 else 10
     eng.step(per, graph=a.graph, graph_steps=per)
     eng.sync()
-    t0 = time.perf_counter()
-    eng.step(a.sgmp_steps, graph=a.graph, graph_steps=per)
-    eng.sync()
-    sec = time.perf_counter() - t0
-    eng.profile(True)
-    eng.step(3)
-    kt = eng.kernel_times()
-    eng.profile(False)
+    w = timed_window(eng, a.sgmp_steps, lambda n: eng.step(n, graph=a.graph, graph_steps=per), tune["sort_every"])
+    kt = profiled_kernels(eng, 3, eng.step)
+    par = None
+    if not a.no_cpu:
+        # a batched Cook plate against the CPU FP64 oracle after the same steps
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        steps = int(eng.scalars(ids[0])[0])
+        o = pyoracle.Oracle(cook)
+        o.step(steps)
+        par = {"instance": "Cook plate 0 of the batch", "steps": steps,
+               "fp32_vs_fp64_oracle": group_err(mpm, eng.particles(ids[0]), o.particles(), GROUPS)}
+        log("sgmp parity:", json.dumps(par))
     eng.close()
     out = {"workload": "%d Cook-membrane plates 48x44(->16)x10 mm, h=0.5 mm, 2x2x2 particles/cell, neo-Hookean "
                        "E=240 MPa nu=0.4999 rho=1e-3 g/mm^3, shear traction ramp, + 3D dam break 0.6x0.3x0.3 m, "
                        "h=5 mm, c=50 m/s; SGMP, MUSL, FP32 bulk" % nc,
-           "particles": tot_p, "steps": a.sgmp_steps, "ms_per_step": sec / a.sgmp_steps * 1e3,
-           "value": tot_p * a.sgmp_steps / sec, "unit": "particle-steps/s", "tuning": tune,
-           "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()}}
+           "particles": tot_p, "steps": a.sgmp_steps, "ms_per_step": w["ms_per_step"],
+           "timed_ms_per_step": w["timed_ms_per_step"], "sort_ms": w["sort_ms"],
+           "value": tot_p / (w["ms_per_step"] * 1e-3), "unit": "particle-steps/s", "tuning": tune,
+           "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()}, "parity": par}
     log("sgmp: %.3f ms/step, %.4g particle-steps/s" % (out["ms_per_step"], out["value"]))
-    if not a.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle
-        # 10 steps: in the first steps the ramped load and the stress are
-        # still near zero, and the relative error of a near-zero state is large
-        ctx = mpm.Context(cook, "fp32")
-        ctx.step(10)
-        o = pyoracle.Oracle(cook)
-        o.step(10)
-        out["parity_cook_fp32_vs_fp64_oracle"] = group_err(mpm, ctx.particles(), o.particles(), GROUPS)
-        out["parity_steps"] = 10
-        ctx.close()
-        log("sgmp parity:", json.dumps(out["parity_cook_fp32_vs_fp64_oracle"]))
     return out
 
 
@@ -432,7 +522,7 @@ def main():
     E = run_ensemble(mpm, decks, a, log)
 
     parity = {}
-    cpu = None
+    cpu = cpu_all = None
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
@@ -440,10 +530,14 @@ def main():
         o.step(a.check_steps)
         ref = o.particles()
         single = {}
-        for prec in ("fp64", "fp32"):
+        for prec in ("fp64", "fp32", "fp64exact"):
             ctx = mpm.Context(prob, prec)
             ctx.step(a.check_steps)
-            single[prec + "_vs_fp64_oracle"] = group_err(mpm, ctx.particles(), ref, GROUPS)
+            ps = ctx.particles()
+            single[prec + "_vs_fp64_oracle"] = group_err(mpm, ps, ref, GROUPS)
+            if prec == "fp64exact":   # the reference-order mode: bit for bit
+                single["fp64exact_bitwise"] = bool(np.array_equal(ps, ref, equal_nan=True)
+                                                   and np.array_equal(ctx.grid(), o.grid(), equal_nan=True))
             ctx.close()
         single["steps"] = a.check_steps
         parity["single_slope"] = single
@@ -459,11 +553,16 @@ def main():
         sample = [E["ens"][k] for k in range(3)]
         cpu = cpu_baseline(sample, mpm, decks, a.cpu_steps, log)
         log("cpu baseline:", json.dumps(cpu))
+        cpu_all = cpu_baseline_all_cores(sample, a.cpu_steps, log)
+        log("cpu baseline, all cores:", json.dumps(cpu_all))
 
     e32 = E["fp32"]
     kt = e32["kt"]
     traffic = lambda k: measured_traffic(k, "fp32", "ensemble")
     v64p = plain["fp64"]["value"]
+    v64x = plain["fp64exact"]["value"]
+    leg = lambda r: {"value": r["value"], "ms_per_step": r["ms_per_step"], "timed_ms_per_step": r["timed_ms_per_step"],
+                     "sort_ms": r["sort_ms"], "sort_every": r["sort_every"], "sorts_in_window": r["sorts_in_window"]}
     out = {
         "metric": "particle-steps/s FP32-batched vs FP64-plain @<=1e-5 rel err; p50/p99 latency",
         "value": e32["value"],
@@ -472,6 +571,12 @@ def main():
         "steps": a.steps,
         "warmup": a.warmup,
         "ms_per_step": e32["ms_per_step"],
+        "timed_ms_per_step": e32["timed_ms_per_step"],
+        "resort": {"sort_ms": e32["sort_ms"], "sort_every": e32["sort_every"],
+                   "sorts_in_window": e32["sorts_in_window"],
+                   "note": "value and ms_per_step: step time of the timed window without its re-sorts plus one "
+                           "measured re-sort per sort_every steps (steady state, whatever --steps is); "
+                           "timed_ms_per_step: the raw window"},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -485,11 +590,14 @@ def main():
                    "instances": a.ensemble, "particles": E["particles"], "nodes": E["nodes"],
                    "nodes_touched": E["touched"],
                    "precision": "FP32 bulk; FP64 positions, volume, internal energy, F, grid mass"},
-        "fp64_plain": {"workload": "single slope, %d particles, %d nodes" % (prob.n, prob.nb_node),
-                       "value": v64p, "ms_per_step": plain["fp64"]["sec"] / a.steps * 1e3},
+        "fp64_plain": dict(leg(plain["fp64"]), workload="single slope, %d particles, %d nodes, FP64 (scatter "
+                                                           "sums in atomic order)" % (prob.n, prob.nb_node)),
+        "fp64_exact_plain": dict(leg(plain["fp64exact"]), workload="single slope, FP64 in the reference's "
+                                                                    "summation order (bit-identical to it)"),
         "speedup_vs_fp64_plain": e32["value"] / v64p,
-        "fp32_plain": {"value": plain["fp32"]["value"], "ms_per_step": plain["fp32"]["sec"] / a.steps * 1e3},
-        "fp64_batched": {"value": E["fp64"]["value"], "ms_per_step": E["fp64"]["ms_per_step"]},
+        "speedup_vs_fp64_exact_plain": e32["value"] / v64x,
+        "fp32_plain": leg(plain["fp32"]),
+        "fp64_batched": leg(E["fp64"]),
         "tuning": {"fp32": e32["tuning"], "fp64": E["fp64"]["tuning"], "graph_steps": e32["graph_steps"]},
         "kernels_ms_avg": {k: (v[0] / v[1] if v[1] else 0.0) for k, v in kt.items()},
         "roofline": roofline(kt, E["particles"], E["touched"], 4, traffic),
@@ -500,6 +608,7 @@ def main():
         "sgmp": sgmp,
         "serving": serving,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all if not a.no_cpu else None,
     }
     print(json.dumps(out))
 

@@ -53,6 +53,8 @@ int mpm_engine_step(mpm_engine* e, int nsteps, int use_graph, int graph_steps);
 int mpm_engine_sync(mpm_engine* e);
 /* stable re-sort of every instance's particles by cell */
 int mpm_engine_sort(mpm_engine* e);
+/* steps advanced and whole-pool re-sorts run so far (either may be NULL) */
+int mpm_engine_counters(mpm_engine* e, long* steps_done, long* sorts_done);
 /* block sizes for particle and node kernels (64/128/256), re-sort period */
 int mpm_engine_set_tuning(mpm_engine* e, int block_particles, int block_nodes, int sort_every);
 /* interleaved (1) or contiguous (0) lane -> particle mapping in particle kernels */

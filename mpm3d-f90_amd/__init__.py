@@ -173,6 +173,21 @@ class Context:
         """{kernel class: (total ms, launches)} since profile(True)."""
         return _kernel_times(self.engine_handle())
 
+    def sort(self):
+        """Stable re-sort of the particles by cell (as the step does every
+        few steps)."""
+        check(_lib.lib().mpm_engine_sort(self.engine_handle()), "mpm_engine_sort")
+
+    def counters(self):
+        """(steps advanced, whole-pool re-sorts run) so far."""
+        return _counters(self.engine_handle())
+
+
+def _counters(h):
+    st, so = C.c_long(), C.c_long()
+    check(_lib.lib().mpm_engine_counters(h, C.byref(st), C.byref(so)), "mpm_engine_counters")
+    return st.value, so.value
+
 
 def _profile(h, enable):
     check(_lib.lib().mpm_engine_profile(h, int(bool(enable))), "mpm_engine_profile")
@@ -267,6 +282,10 @@ class Engine:
 
     def sort(self):
         check(_lib.lib().mpm_engine_sort(self._h), "mpm_engine_sort")
+
+    def counters(self):
+        """(steps advanced, whole-pool re-sorts run) so far."""
+        return _counters(self._h)
 
     def profile(self, enable=True):
         _profile(self._h, enable)

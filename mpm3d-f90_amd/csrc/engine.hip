@@ -206,6 +206,7 @@ struct mpm_engine {
   int sort_every = 64;            // steps between re-sorts in mpm_step (0: never)
   int brick = 2;                  // sort order: 0 Hilbert, B > 0 bricks of B^3 cells
   long steps_done = 0;
+  long sorts_done = 0;            // whole-pool re-sorts so far (admission sorts excluded)
   long since_sort = 1L << 40;     // steps since the last re-sort of the whole pool (none yet: sort first)
   // sort scratch (lazily allocated)
   unsigned* k_in = nullptr;
@@ -1359,6 +1360,7 @@ static void sort_instance(mpm_engine* e, int id) {
 static void sort_particles(mpm_engine* e) {
   rebuild_lists(e);
   e->since_sort = 0;
+  e->sorts_done++;
   const long np = e->np;
   ensure_sort_buffers(e);
   if (!e->d_seg) e->d_seg = dalloc<int>(2 * (size_t)e->max_inst);
@@ -2014,6 +2016,14 @@ extern "C" int mpm_engine_sort(mpm_engine* e) {
   return guarded([&] {
     check_ctx(e);
     sort_particles(e);
+  });
+}
+
+extern "C" int mpm_engine_counters(mpm_engine* e, long* steps_done, long* sorts_done) {
+  return guarded([&] {
+    if (!e) fail(MPM_ERR_ARG, "null engine");
+    if (steps_done) *steps_done = e->steps_done;
+    if (sorts_done) *sorts_done = e->sorts_done;
   });
 }
 
