@@ -88,6 +88,23 @@ int mpm_update_dt(mpm_ctx* ctx, double* dt);
  * kernels, no host synchronisation inside */
 int mpm_step(mpm_ctx* ctx, int nsteps);
 
+/* ---- particles by cell -------------------------------------------------
+ * The cell of every particle as ParticleInitial finds it (InWhichCell,
+ * update_step.f90:408-431, Grid.f90:123-155: 1-based, -1 outside the grid)
+ * and the stable order of the particles by cell (ascending cell, equal
+ * cells in ascending index, particles outside the grid last) from the
+ * engine's radix sort, which reorders particle arrays for locality where
+ * the reference walks them in index order.  x, y, z (FP64), icell and order
+ * are device arrays of n entries owned by the caller; the grid is prob's
+ * (span, dcell, ncx, ncy, ncz; its particles are not read).  Enqueued on
+ * stream (a hipStream_t; NULL: the null stream); returns when enqueued.
+ * Scratch of 12 B per particle plus the sort's counts is held per device by
+ * the library, grown on demand (hipMalloc) and kept; calls on one device
+ * share it, so calls that overlap in time on different streams race. */
+int mpm_cell_order(const mpm_problem* prob, const double* x, const double* y, const double* z, long n,
+                   int32_t* icell, int32_t* order, void* stream);
+
+
 /* ---- state exchange (synchronising) ---------------------------------- */
 /* pstate: PF_COUNT * nb_particle doubles, original particle order */
 int mpm_download_particles(mpm_ctx* ctx, double* pstate);

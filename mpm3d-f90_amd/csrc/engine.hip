@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2212,3 +2213,87 @@ extern "C" int mpm_ctx_engine(mpm_ctx* ctx, mpm_engine** out) {
     *out = ctx->e;
   });
 }
+
+// ------------------------------------------------ particles by cell (C-ABI)
+
+// the grid constants InWhichCell reads, from a problem's grid description
+static InstC cell_grid(const mpm_problem* p) {
+  if (!p || p->ncx <= 0 || p->ncy <= 0 || p->ncz <= 0 || !(p->dcell > 0))
+    fail(MPM_ERR_ARG, "bad grid description");
+  InstC c{};
+  c.ncx = p->ncx;
+  c.ncy = p->ncy;
+  c.ncz = p->ncz;
+  c.ncxy = c.ncx * c.ncy;
+  c.ncell = c.ncxy * c.ncz;
+  for (int k = 0; k < 3; k++) {
+    c.span0[k] = p->span[2 * k];
+    c.span1[k] = p->span[2 * k + 1];
+  }
+  c.dcell = p->dcell;
+  return c;
+}
+
+// scratch words of one cell-order call over n particles: two key buffers,
+// one value buffer (the other is the caller's order array) and the radix
+// sort's counts
+static size_t cell_order_words(long n) { return 3 * (size_t)n + mpmk::radix_scratch_words(n); }
+
+static void check_cell_order_args(const mpm_problem* p, const double* x, const double* y, const double* z, long n,
+                                  const int32_t* icell, const int32_t* order) {
+  if (!p || n < 0) fail(MPM_ERR_ARG, "bad arguments");
+  if (n > 0 && (!x || !y || !z || !icell || !order)) fail(MPM_ERR_ARG, "null device array");
+  if (n > (1L << 30)) fail(MPM_ERR_ARG, "at most 2^30 particles per call");
+}
+
+// InWhichCell keys, then the stable radix sort of (key, index); the value
+// buffers are arranged so that the last pass writes the caller's order array
+static void cell_order_run(const InstC& g, const double* x, const double* y, const double* z, long n, int32_t* icell,
+                           int32_t* order, unsigned* scr, hipStream_t s) {
+  unsigned* k0 = scr;
+  unsigned* k1 = scr + n;
+  int* vt = (int*)(scr + 2 * n);
+  unsigned* rs = scr + 3 * n;
+  const int nbits = bits_for((long)g.ncell + 2);
+  const int odd = ((nbits + 7) / 8) % 2;   // passes of 8 bits: the result lands in v1 when odd
+  int* v0 = odd ? vt : order;
+  int* v1 = odd ? order : vt;
+  mpmk::launch_cell_keys(g, x, y, z, n, icell, k0, v0, s);
+  const int par = mpmk::radix_sort_pairs<unsigned>(k0, k1, v0, v1, n, 0, nbits, rs, s);
+  if (par != odd) fail(MPM_ERR_STATE, "cell order: unexpected sort parity");
+  HIPCHK(hipGetLastError());
+}
+
+// per-device scratch of mpm_cell_order, grown on demand and kept
+struct CellOrderScratch {
+  unsigned* p = nullptr;
+  size_t words = 0;
+};
+static std::mutex g_cell_order_mu;
+static std::map<int, CellOrderScratch> g_cell_order_scr;
+
+extern "C" int mpm_cell_order(const mpm_problem* prob, const double* x, const double* y, const double* z, long n,
+                              int32_t* icell, int32_t* order, void* stream) {
+  return guarded([&] {
+    check_cell_order_args(prob, x, y, z, n, icell, order);
+    const InstC g = cell_grid(prob);
+    if (n == 0) return;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_cell_order_mu);
+    CellOrderScratch& sc = g_cell_order_scr[dev];
+    const size_t need = cell_order_words(n);
+    if (sc.words < need) {
+      if (sc.p) {
+        HIPCHK(hipDeviceSynchronize());   // earlier calls may still use it
+        HIPCHK(hipFree(sc.p));
+        sc.p = nullptr;
+        sc.words = 0;
+      }
+      sc.p = dalloc<unsigned>(need);
+      sc.words = need;
+    }
+    cell_order_run(g, x, y, z, n, icell, order, sc.p, (hipStream_t)stream);
+  });
+}
+

@@ -89,6 +89,9 @@ void launch_admit(const PoolPtrs<T>& P, const double* ps, const int* pbody, int 
                   hipStream_t s);
 void launch_fixity(uint8_t* fix, const InstC* inst, int id, long nslots, hipStream_t s);
 void launch_clear_ints(int* a, long n, hipStream_t s);
+// InWhichCell of n particles (x, y, z), their cell-order sort keys and identity values
+void launch_cell_keys(const InstC& I, const double* x, const double* y, const double* z, long n, int* icell,
+                      unsigned* key, int* val, hipStream_t s);
 void launch_seg_prefix(unsigned* keys, const int* begins, int nseg, long b0, long n, int shift, hipStream_t s);
 template <typename T> void launch_sgmp_p2g(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_sgmp_nodes(int what, const PoolPtrs<T>& P, const LaunchP& N, hipStream_t s);

@@ -3323,6 +3323,26 @@ void launch_seg_prefix(unsigned* keys, const int* begins, int nseg, long b0, lon
                        shift);
 }
 
+// ParticleInitial's cell location (InWhichCell) of caller-owned particle
+// arrays, the sort key of the stable cell order (particles outside the grid
+// after every cell) and the identity as the sort values
+__global__ void __launch_bounds__(256) k_cell_keys(InstC I, const double* x, const double* y, const double* z,
+                                                   long n, int* icell, unsigned* key, int* val) {
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int c = in_which_cell(I, x[j], y[j], z[j]);
+  icell[j] = c;
+  key[j] = c < 0 ? (unsigned)I.ncell + 1u : (unsigned)c;
+  val[j] = (int)j;
+}
+
+void launch_cell_keys(const InstC& I, const double* x, const double* y, const double* z, long n, int* icell,
+                      unsigned* key, int* val, hipStream_t s) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_cell_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, I, x, y, z, n, icell, key,
+                       val);
+}
+
 void launch_clear_ints(int* a, long n, hipStream_t s) {
   if (n > 0) {
     const long nb = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
