@@ -104,6 +104,17 @@ int mpm_step(mpm_ctx* ctx, int nsteps);
 int mpm_cell_order(const mpm_problem* prob, const double* x, const double* y, const double* z, long n,
                    int32_t* icell, int32_t* order, void* stream);
 
+/* The same computation without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch is one fixed device array of the library (about 25 MiB, held by
+ * every GPU the library runs on until the process ends), so n is at most
+ * MPM_CELL_ORDER_MAX (larger n: MPM_ERR_ARG).  The caller passes inputs and
+ * outputs as above.  All calls on one GPU share that array: calls that
+ * overlap in time on one GPU race (order them on one stream). */
+#define MPM_CELL_ORDER_MAX (1L << 21)
+int mpm_cell_order_noalloc(const mpm_problem* prob, const double* x, const double* y, const double* z, long n,
+                           int32_t* icell, int32_t* order, void* stream);
 
 /* ---- state exchange (synchronising) ---------------------------------- */
 /* pstate: PF_COUNT * nb_particle doubles, original particle order */

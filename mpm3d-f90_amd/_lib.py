@@ -130,6 +130,7 @@ def lib():
         "mpm_engine_sort": (C.c_int, [vp]),
         "mpm_engine_counters": (C.c_int, [vp, P(C.c_long), P(C.c_long)]),
         "mpm_cell_order": (C.c_int, [P(ProblemStruct), vp, vp, vp, C.c_long, vp, vp, vp]),
+        "mpm_cell_order_noalloc": (C.c_int, [P(ProblemStruct), vp, vp, vp, C.c_long, vp, vp, vp]),
         "mpm_engine_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "mpm_engine_set_interleave": (C.c_int, [vp, C.c_int]),
         "mpm_engine_get_tuning": (C.c_int, [vp, P(Tuning)]),

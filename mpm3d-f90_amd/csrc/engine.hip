@@ -2297,3 +2297,41 @@ extern "C" int mpm_cell_order(const mpm_problem* prob, const double* x, const do
   });
 }
 
+// Scratch of mpm_cell_order_noalloc: one fixed device array, no allocation
+// at call time.  Every GPU the library runs on holds its own copy (at its
+// own address) until the process ends.  Sized for MPM_CELL_ORDER_MAX
+// particles: two key buffers and one value buffer of that many entries and
+// the radix sort's counts (256 per 4096-particle tile, plus 256 totals).
+constexpr long CO_MAX = MPM_CELL_ORDER_MAX;
+constexpr size_t CO_WORDS = 3 * (size_t)CO_MAX + 256 * (size_t)((CO_MAX + 4095) / 4096) + 256;
+static __device__ __align__(256) unsigned scratch[CO_WORDS];
+
+// the device address of `scratch` on the current device, looked up once
+// per device (the host-side name is a stand-in, not a device address)
+static std::mutex g_scratch_mu;
+static std::map<int, unsigned*> g_scratch_ptr;
+
+static unsigned* scratch_ptr() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void* p = nullptr;
+  const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (rc != hipSuccess || !p) fail(MPM_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(rc));
+  g_scratch_ptr[dev] = (unsigned*)p;
+  return (unsigned*)p;
+}
+
+extern "C" int mpm_cell_order_noalloc(const mpm_problem* prob, const double* x, const double* y, const double* z,
+                                      long n, int32_t* icell, int32_t* order, void* stream) {
+  return guarded([&] {
+    check_cell_order_args(prob, x, y, z, n, icell, order);
+    if (n > CO_MAX) fail(MPM_ERR_ARG, "mpm_cell_order_noalloc: %ld particles exceed its maximum of %ld", n, CO_MAX);
+    const InstC g = cell_grid(prob);
+    if (n == 0) return;
+    if (cell_order_words(n) > CO_WORDS) fail(MPM_ERR_STATE, "cell order scratch size");
+    cell_order_run(g, x, y, z, n, icell, order, scratch_ptr(), (hipStream_t)stream);
+  });
+}
