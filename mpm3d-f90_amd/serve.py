@@ -219,19 +219,22 @@ def summarize(done, wall, rate, duration, arrived, server):
     }
 
 
-def request_pool(mpm, decks, count=64, seed=0, slope_frac=0.7, steps=(100, 300)):
-    """Mixed request pool: slopes as in the batched ensemble (random size and
-    strength) and Taylor bars (impact speed U(100, 250) m/s, radius U(2, 5)
-    mm); every request asks for a step count drawn from U(steps)."""
+def request_pool(mpm, decks, count=64, seed=0, slope_frac=0.7, steps=(100, 300), slope_n=(2e4, 4e5),
+                 bar_radius=(2.0, 5.0), bar_length=32.4):
+    """Mixed request pool: slopes as in the batched ensemble (random size in
+    slope_n particles and random strength) and Taylor bars (impact speed
+    U(100, 250) m/s, radius U(bar_radius) mm, length bar_length mm); every
+    request asks for a step count drawn from U(steps)."""
     rng = np.random.default_rng(seed)
     n_slope = int(round(slope_frac * count))
     pool = []
-    ens = decks.slope_ensemble(n_slope, seed=seed + 1)
+    ens = decks.slope_ensemble(n_slope, seed=seed + 1, n_min=slope_n[0], n_max=slope_n[1])
     for k, (text, n, meta) in enumerate(ens):
         p = decks.load(text, mpm, "req_slope%d" % k)
         pool.append((p, int(rng.integers(steps[0], steps[1] + 1)), "slope"))
     for k in range(count - n_slope):
-        text, n = decks.taylor_bar(radius=float(rng.uniform(2.0, 5.0)), speed=float(rng.uniform(100.0, 250.0)))
+        text, n = decks.taylor_bar(radius=float(rng.uniform(*bar_radius)), speed=float(rng.uniform(100.0, 250.0)),
+                                   length=bar_length)
         p = decks.load(text, mpm, "req_bar%d" % k)
         pool.append((p, int(rng.integers(steps[0], steps[1] + 1)), "bar"))
     order = rng.permutation(len(pool))
