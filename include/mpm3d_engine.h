@@ -93,6 +93,12 @@ int mpm_engine_download_icell(mpm_engine* e, int id, int32_t* icell);
 int mpm_engine_get_scalars(mpm_engine* e, int id, double* out8);
 /* original particle index held by each device slot (the sort permutation) */
 int mpm_engine_download_order(mpm_engine* e, int id, int32_t* orig);
+/* per instance: particles whose stress update gave a negative volume
+ * (a warning in the reference, Constitution.f90:135-137) and a non-finite
+ * volume or stress, counted over all steps so far (-1: no such instance);
+ * an instance's NaN stays in its own particle and node ranges, so the other
+ * instances are unaffected (synchronising; either array may be NULL) */
+int mpm_engine_numeric_flags(mpm_engine* e, int n, const int* ids, int* negvol, int* nonfinite);
 /* done flag (1 finished, 0 running, -1 no such instance) and istep */
 int mpm_engine_status(mpm_engine* e, int n, const int* ids, int* done, int* istep);
 /* Non-blocking status: snapshot_status enqueues a copy of every instance's

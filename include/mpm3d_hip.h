@@ -81,8 +81,16 @@ int mpm_particle_position_update(mpm_ctx* ctx);
 int mpm_grid_momentum_musl(mpm_ctx* ctx);
 /* calcEnergy (Particle.f90:204-230); any output pointer may be NULL */
 int mpm_calc_energy(mpm_ctx* ctx, double* eng_kinetic, double* eng_internal, double* momentum3);
-/* UpdateDT (update_step.f90:1833-1874); *dt may be NULL */
+/* UpdateDT (update_step.f90:1833-1874); *dt may be NULL.  Returns
+ * MPM_ERR_NUMERIC (DT still written) once a stress update has produced a
+ * non-finite volume or stress. */
 int mpm_update_dt(mpm_ctx* ctx, double* dt);
+/* Particles whose stress update gave a negative volume (the reference warns
+ * and goes on, Constitution.f90:135-137) and a non-finite volume or stress,
+ * counted over all steps so far (synchronising; either pointer may be NULL).
+ * mpm_step does not synchronise: its flags are read here or by the next
+ * mpm_update_dt. */
+int mpm_numeric_flags(mpm_ctx* ctx, int* negvol, int* nonfinite);
 
 /* nsteps whole steps in the reference order (MPM3D.f90:69-113), fused
  * kernels, no host synchronisation inside */

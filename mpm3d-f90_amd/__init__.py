@@ -182,6 +182,13 @@ class Context:
         """(steps advanced, whole-pool re-sorts run) so far."""
         return _counters(self.engine_handle())
 
+    def numeric_flags(self):
+        """(negative-volume count, non-finite volume/stress count) of the
+        stress updates so far (the reference only warns on the first)."""
+        nv, nf = C.c_int(), C.c_int()
+        check(_lib.lib().mpm_numeric_flags(self._h, C.byref(nv), C.byref(nf)), "mpm_numeric_flags")
+        return nv.value, nf.value
+
 
 def _counters(h):
     st, so = C.c_long(), C.c_long()
@@ -286,6 +293,14 @@ class Engine:
     def counters(self):
         """(steps advanced, whole-pool re-sorts run) so far."""
         return _counters(self._h)
+
+    def numeric_flags(self, ids):
+        """{id: (negative-volume count, non-finite volume/stress count)}"""
+        ids = [int(i) for i in ids]
+        a = (C.c_int * len(ids))(*ids)
+        nv, nf = (C.c_int * len(ids))(), (C.c_int * len(ids))()
+        check(_lib.lib().mpm_engine_numeric_flags(self._h, len(ids), a, nv, nf), "mpm_engine_numeric_flags")
+        return {i: (nv[k], nf[k]) for k, i in enumerate(ids)}
 
     def profile(self, enable=True):
         _profile(self._h, enable)

@@ -129,6 +129,8 @@ def lib():
         "mpm_engine_sync": (C.c_int, [vp]),
         "mpm_engine_sort": (C.c_int, [vp]),
         "mpm_engine_counters": (C.c_int, [vp, P(C.c_long), P(C.c_long)]),
+        "mpm_engine_numeric_flags": (C.c_int, [vp, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int)]),
+        "mpm_numeric_flags": (C.c_int, [vp, P(C.c_int), P(C.c_int)]),
         "mpm_cell_order": (C.c_int, [P(ProblemStruct), vp, vp, vp, C.c_long, vp, vp, vp]),
         "mpm_cell_order_noalloc": (C.c_int, [P(ProblemStruct), vp, vp, vp, C.c_long, vp, vp, vp]),
         "mpm_engine_set_tuning": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),

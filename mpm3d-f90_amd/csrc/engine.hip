@@ -1793,6 +1793,8 @@ static void initial_energy(mpm_engine* e, int id) {
 
 // ================================================================ C-ABI
 
+static void numeric_flags(mpm_engine* e, int n, const int* ids, int* negvol, int* nonfinite);
+
 struct mpm_ctx {
   mpm_engine* e;
 };
@@ -1868,11 +1870,23 @@ extern "C" int mpm_update_dt(mpm_ctx* ctx, double* dt) {
   return guarded([&] {
     if (!ctx) fail(MPM_ERR_ARG, "null context");
     phase(ctx->e, PH_DT);
-    if (dt) {
-      double s[8];
-      get_scalars(ctx->e, 0, s);
-      *dt = s[2];
-    }
+    double s[8];
+    get_scalars(ctx->e, 0, s);
+    if (dt) *dt = s[2];
+    const int id = 0;
+    int nonfinite = 0;
+    numeric_flags(ctx->e, 1, &id, nullptr, &nonfinite);
+    if (nonfinite > 0)
+      fail(MPM_ERR_NUMERIC, "%d particle stress updates gave a non-finite volume or stress (DT was still computed)",
+           nonfinite);
+  });
+}
+
+extern "C" int mpm_numeric_flags(mpm_ctx* ctx, int* negvol, int* nonfinite) {
+  return guarded([&] {
+    if (!ctx) fail(MPM_ERR_ARG, "null context");
+    const int id = 0;
+    numeric_flags(ctx->e, 1, &id, negvol, nonfinite);
   });
 }
 
@@ -2150,6 +2164,27 @@ extern "C" int mpm_engine_status(mpm_engine* e, int n, const int* ids, int* done
       done[k] = ts[id].done;
       if (istep) istep[k] = ts[id].istep;
     }
+  });
+}
+
+// per-instance counts of particles with a negative or non-finite state
+static void numeric_flags(mpm_engine* e, int n, const int* ids, int* negvol, int* nonfinite) {
+  HIPCHK(hipStreamSynchronize(e->s));
+  std::vector<InstT> ts(e->max_inst);
+  HIPCHK(hipMemcpy(ts.data(), e->d_ts, sizeof(InstT) * e->max_inst, hipMemcpyDeviceToHost));
+  for (int k = 0; k < n; k++) {
+    const int id = ids[k];
+    const bool ok = id >= 0 && id < e->max_inst && e->inst[id].live;
+    if (negvol) negvol[k] = ok ? ts[id].negvol_count : -1;
+    if (nonfinite) nonfinite[k] = ok ? ts[id].nan_count : -1;
+  }
+}
+
+extern "C" int mpm_engine_numeric_flags(mpm_engine* e, int n, const int* ids, int* negvol, int* nonfinite) {
+  return guarded([&] {
+    check_ctx(e);
+    if (n < 0 || (n > 0 && !ids)) fail(MPM_ERR_ARG, "bad arguments");
+    numeric_flags(e, n, ids, negvol, nonfinite);
   });
 }
 

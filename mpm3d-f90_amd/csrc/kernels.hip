@@ -66,8 +66,11 @@ __device__ __forceinline__ double div_mass<double>(double p, double mg) {
 // InWhichCell (Grid.f90:123-155).  Returns the 1-based cell or -1; the
 // cell's coordinates are decomposed from the linear index so that the
 // reference's wrap at x == SpanX(2) is reproduced.
+// A non-finite coordinate is outside as well (the comparisons are written so
+// that NaN fails them), which keeps the integer conversions below defined.
 __device__ __forceinline__ int in_which_cell(const InstC& I, double x, double y, double z) {
-  if (x < I.span0[0] || x > I.span1[0] || y < I.span0[1] || y > I.span1[1] || z < I.span0[2] || z > I.span1[2])
+  if (!(x >= I.span0[0] && x <= I.span1[0] && y >= I.span0[1] && y <= I.span1[1] && z >= I.span0[2] &&
+        z <= I.span1[2]))
     return -1;
   const int ix = (int)((x - I.span0[0]) / I.dcell) + 1;
   const int iy = (int)((y - I.span0[1]) / I.dcell) + 1;
@@ -83,7 +86,8 @@ __device__ __forceinline__ int in_which_cell(const InstC& I, double x, double y,
 // cell_coords of the returned index without its two integer divisions.
 __device__ __forceinline__ int in_which_cell3(const InstC& I, double x, double y, double z, int& cx, int& cy,
                                               int& cz) {
-  if (x < I.span0[0] || x > I.span1[0] || y < I.span0[1] || y > I.span1[1] || z < I.span0[2] || z > I.span1[2])
+  if (!(x >= I.span0[0] && x <= I.span1[0] && y >= I.span0[1] && y <= I.span1[1] && z >= I.span0[2] &&
+        z <= I.span1[2]))
     return -1;
   int ix = (int)((x - I.span0[0]) / I.dcell);
   int iy = (int)((y - I.span0[1]) / I.dcell);
@@ -1952,7 +1956,7 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
 template <typename T, int MM>
 __device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p, const InstC& I, const InstT& ts,
                                             T de[6], T vort[3], const T L[3][3], bool uvfp, double x, double y,
-                                            double z, bool move, const MatC& mc, Cst<T>& m) {
+                                            double z, bool move, const MatC& mc, Cst<T>& m, InstT* tsp) {
     const double dtd = ts.dt;
     const T dt = (T)dtd;
 #pragma unroll
@@ -1985,6 +1989,11 @@ __device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p
       }
     }
     constitution<T, MM>(p, I, mc, dt, de, vort, F, deF, m);
+    // counted per instance: a negative volume (the reference writes a
+    // warning and goes on, Constitution.f90:135-137) and a non-finite volume
+    // or stress (NaN or inf, which the reference would carry on silently)
+    if (m.vol < 0.0) atomicAdd(&tsp->negvol_count, 1);
+    if (!isfinite(m.vol) || !isfinite((double)m.sm) || !isfinite((double)m.seqv)) atomicAdd(&tsp->nan_count, 1);
     if (move) {   // Xp = XX (update_step.f90:642)
       p.d(D_X) = x + (double)p.t(T_BX) * dtd;
       p.d(D_Y) = y + (double)p.t(T_BY) * dtd;
@@ -2104,7 +2113,7 @@ __global__ void __launch_bounds__(256, (stress_waves<SHAPE, T>())) k_stress(Pool
     });
     }
     stress_tail<T, MM>(P, p, I, ts, de, vort, L, uvfp, x, y, z, (MODE & ST_MOVE) != 0, P.mats[P.bodies[body].mat],
-                       cs);
+                       cs, P.ts + ii);
   }
   if (MODE & ST_FORCE) {   // USF: GridMomentumUpdate scatter right after the update
     T* Fg = P.fg + I.node_off;
@@ -2223,6 +2232,7 @@ struct SgLoc {
 // CenterInWhichCell (Grid.f90:157-190) and CenterCellNode (Grid.f90:386-403)
 __device__ __forceinline__ bool sgmp_locate(const InstC& I, const uint8_t* fix, double x, double y, double z,
                                             SgLoc& L) {
+  if (!(isfinite(x) && isfinite(y) && isfinite(z))) return false;   // keeps the conversions below defined
   const double h = I.dcell;
   const double X[3] = {x, y, z};
   const int ix = (int)((x - I.span0[0] - h / 2.0) / h) + 1;
@@ -2667,7 +2677,8 @@ __global__ void __launch_bounds__(256) k_sgmp_stress(PoolPtrs<T> P, const int* g
     Cst<T> cs;
     cst_load<T, MM>(p, cs);
     const MatC& mc = P.mats[P.bodies[p.n(I_BODY)].mat];
-    stress_tail<T, MM>(P, p, I, ts, de, vort, Lg, (I.flags & IF_UVFP) != 0, x, y, z, (MODE & ST_MOVE) != 0, mc, cs);
+    stress_tail<T, MM>(P, p, I, ts, de, vort, Lg, (I.flags & IF_UVFP) != 0, x, y, z, (MODE & ST_MOVE) != 0, mc, cs,
+                       P.ts + ii);
   }
   if (MODE & ST_ENERGY) {
     double ev[6] = {0, 0, 0, 0, 0, 0};
@@ -2911,11 +2922,13 @@ __global__ void __launch_bounds__(256) k_sort_keys(const double* X, long np, con
   if (j < I.p_end) {
     if (I.shape == MPM_SHAPE_SGMP) {
       const double h = I.dcell;
+      // a non-finite position sorts last (and keeps the conversions defined)
+      const bool fin = isfinite(X[j]) && isfinite(X[np + j]) && isfinite(X[2 * np + j]);
       const int ix = (int)((X[j] - I.span0[0] - h / 2.0) / h);
       const int iy = (int)((X[np + j] - I.span0[1] - h / 2.0) / h);
       const int iz = (int)((X[2 * np + j] - I.span0[2] - h / 2.0) / h);
       const int ac = iz * I.cncxy + iy * I.cncx + ix + 1;
-      if (ac >= 1 && ac <= I.cncell) {
+      if (fin && ac >= 1 && ac <= I.cncell) {
         const int a0 = ac - 1;
         cz = a0 / I.cncxy;
         const int r = a0 - cz * I.cncxy;

@@ -276,8 +276,19 @@ subroutine UpdateDT()
   use update_step_hip_state
   use ParticleData, only: DT
   implicit none
+  integer(c_int), parameter :: MPM_ERR_NUMERIC = -8
   real(c_double) :: dt_dev
-  call mpm_check(mpm_update_dt(ctx, dt_dev), 'UpdateDT')
+  integer(c_int) :: rc
+  logical, save :: warned = .false.
+  rc = mpm_update_dt(ctx, dt_dev)
+  if (rc == MPM_ERR_NUMERIC) then
+     ! a non-finite volume or stress: warned once, the run goes on as the
+     ! reference's would (it warns only on negative volume)
+     if (.not. warned) write(*, '(a)') ' === warning: non-finite particle volume or stress on the device'
+     warned = .true.
+  else
+     call mpm_check(rc, 'UpdateDT')
+  end if
   DT = dt_dev
   nsteps_done = nsteps_done + 1
   call maybe_dump()

@@ -61,6 +61,7 @@ class Server:
         self.queue = collections.deque()
         self.running = {}          # iid -> Request
         self.done = []
+        self.failed = []           # runs evicted because their state went non-finite
         self.max_running = 0
         self.steps_issued = 0
 
@@ -111,6 +112,19 @@ class Server:
         ids = list(self.running)
         st = self.eng.status(ids)
         out = 0
+        # a run whose stress update went non-finite is evicted and reported;
+        # its NaNs stay in its own particle and node ranges, so the others
+        # go on unaffected
+        for iid, (negvol, nonfinite) in self.eng.numeric_flags(ids).items():
+            if nonfinite > 0:
+                req = self.running.pop(iid)
+                req.final_step = st[iid][1]
+                req.result = {"error": "non-finite state", "nonfinite": nonfinite, "negvol": negvol,
+                              "scalars": self.eng.scalars(iid)}
+                self.eng.remove(iid)
+                req.finished = now_fn()
+                self.failed.append(req)
+        ids = list(self.running)
         for iid in ids:
             done, istep = st[iid]
             if done != 1:
@@ -214,6 +228,7 @@ def summarize(done, wall, rate, duration, arrived, server):
         "particle_steps": psteps,
         "particle_steps_per_s": psteps / span,
         "max_concurrent": server.max_running,
+        "failed": len(getattr(server, "failed", ())),
         "wall_s": wall,
         "kinds": dict(collections.Counter(r.kind for r in done)),
     }

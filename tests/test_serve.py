@@ -53,6 +53,11 @@ class FakeEngine:
     def remove(self, iid):
         self.log.append(("remove", self.live.pop(iid)[0]))
 
+    bad = ()   # instance ids whose state the stand-in reports non-finite
+
+    def numeric_flags(self, ids):
+        return {i: (0, 3 if i in self.bad else 0) for i in ids}
+
     # status snapshot: the state after the work issued so far (the stand-in's
     # steps complete at once)
     def snapshot(self):
@@ -145,3 +150,22 @@ def test_pipelined_retirement_sees_each_round_one_round_later():
     # the first run to finish (rid 2: 8 steps from round 1) is seen in round 2
     # synchronously and in round 3 pipelined
     assert results[False][2] == 2.0 and results[True][2] == 3.0
+
+
+def test_a_non_finite_run_is_evicted_and_reported():
+    serve = load_package().serve
+    eng = FakeEngine(cap_p=1000, cap_i=4)
+    eng.bad = (1,)
+    srv = serve.Server(eng, chunk=4, graph=False, clock=lambda: 0.0)
+    for k, n in enumerate([100, 200, 300]):
+        srv.submit(serve.Request(k, FakeProblem(n), 12))
+    srv.admit(0.0)
+    while not srv.idle():
+        srv.advance()
+        srv.retire(lambda: 0.0)
+    assert [r.rid for r in srv.failed] == [1]
+    assert srv.failed[0].result["error"] == "non-finite state" and srv.failed[0].result["nonfinite"] == 3
+    assert sorted(r.rid for r in srv.done) == [0, 2] and all(r.final_step == 12 for r in srv.done)
+    assert ("remove", 200) in eng.log
+    st = serve.summarize(srv.done, 1.0, 1.0, 1.0, 3, srv)
+    assert st["failed"] == 1 and st["completed"] == 2
