@@ -127,6 +127,11 @@ int mpm_cell_order_noalloc(const mpm_problem* prob, const double* x, const doubl
 /* ---- state exchange (synchronising) ---------------------------------- */
 /* pstate: PF_COUNT * nb_particle doubles, original particle order */
 int mpm_download_particles(mpm_ctx* ctx, double* pstate);
+/* the state of k particles chosen by their 1-based original index (as the
+ * reference's CurvePoint): out holds PF_COUNT * k doubles, field-major
+ * (out[f * k + q] for ids[q]); a gather on the device and one small copy,
+ * for per-step curve output without downloading every particle */
+int mpm_download_particle_subset(mpm_ctx* ctx, int k, const int32_t* ids, double* out);
 /* gstate: GF_COUNT * nb_node doubles, node order of the reference */
 int mpm_download_grid(mpm_ctx* ctx, double* gstate);
 /* icell: nb_particle, 1-based cell index of the last GridMomentumInitial, -1 outside */

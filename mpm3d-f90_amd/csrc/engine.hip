@@ -240,6 +240,10 @@ struct mpm_engine {
   long x_m = 0;
   int x_obits = 0, x_nbits = 0;
   int x_sorted = 0;               // buffer holding the step's sorted entries
+  // subset downloads (a few particles per step for the drop-in's curves)
+  int* d_sub_ids = nullptr;
+  double* d_sub_rows = nullptr;
+  int sub_cap = 0;
   // profiling mode: HIP events around every launch of mpm_step / engine steps
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -288,7 +292,7 @@ static void free_engine(mpm_engine* e) {
                   e->d_bodies, e->part, e->d_ginst, e->d_nginst, e->d_ilist_all, e->k_in, e->k_out, e->v_in,
                   e->v_out, e->rs_scr, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg, e->boxes,
                   e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1], e->x_rnode, e->x_rw, e->x_seg0, e->x_seg1,
-                  e->x_scr};
+                  e->x_scr, e->d_sub_ids, e->d_sub_rows};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& g : e->groups) {
@@ -1714,6 +1718,58 @@ static void download_particles(mpm_engine* e, int id, double* out) {
   else download_particles_T<double>(e, id, out);
 }
 
+// PF_COUNT x k doubles (field-major) of the particles with 1-based original
+// indices ids[0..k): one gather kernel and one small copy, not the whole pool
+template <typename T>
+static void download_subset_T(mpm_engine* e, int id, int k, const int32_t* ids, double* out) {
+  HostInst& h = e->inst[id];
+  std::vector<int> u(k);
+  for (int q = 0; q < k; q++) {
+    if (ids[q] < 1 || ids[q] > h.n) fail(MPM_ERR_ARG, "particle %d outside 1..%d", ids[q], h.n);
+    u[q] = ids[q] - 1;
+  }
+  std::sort(u.begin(), u.end());
+  u.erase(std::unique(u.begin(), u.end()), u.end());
+  const int nu = (int)u.size();
+  constexpr int NF = T_COUNT + D_COUNT + I_COUNT;
+  HIPCHK(hipStreamSynchronize(e->s));
+  if (nu > e->sub_cap) {
+    if (e->d_sub_ids) HIPCHK(hipFree(e->d_sub_ids));
+    if (e->d_sub_rows) HIPCHK(hipFree(e->d_sub_rows));
+    e->d_sub_ids = nullptr;
+    e->d_sub_rows = nullptr;
+    e->sub_cap = 0;
+    e->d_sub_ids = dalloc<int>(nu);
+    e->d_sub_rows = dalloc<double>((size_t)nu * NF);
+    e->sub_cap = nu;
+  }
+  HIPCHK(hipMemcpy(e->d_sub_ids, u.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
+  mpmk::launch_pack_subset<T>(e->ptrs<T>(), id, h.n, e->d_sub_ids, nu, e->d_sub_rows, e->s);
+  HIPCHK(hipGetLastError());
+  std::vector<double> rows((size_t)nu * NF);
+  HIPCHK(hipStreamSynchronize(e->s));
+  HIPCHK(hipMemcpy(rows.data(), e->d_sub_rows, sizeof(double) * rows.size(), hipMemcpyDeviceToHost));
+  InstT ts;
+  HIPCHK(hipMemcpy(&ts, e->d_ts + id, sizeof ts, hipMemcpyDeviceToHost));
+  for (int q = 0; q < k; q++) {
+    const double* r = &rows[(size_t)(std::lower_bound(u.begin(), u.end(), ids[q] - 1) - u.begin()) * NF];
+    auto O = [&](int f) -> double& { return out[(size_t)f * k + q]; };
+    auto Tv = [&](int f) { return r[f]; };
+    auto Dv = [&](int f) { return r[T_COUNT + f]; };
+    O(PF_X) = Dv(D_X); O(PF_Y) = Dv(D_Y); O(PF_Z) = Dv(D_Z);
+    O(PF_VX) = Tv(T_VX); O(PF_VY) = Tv(T_VY); O(PF_VZ) = Tv(T_VZ);
+    O(PF_FX) = Tv(T_FX); O(PF_FY) = Tv(T_FY); O(PF_FZ) = Tv(T_FZ);
+    O(PF_MASS) = Dv(D_MASS); O(PF_SM) = Tv(T_SM); O(PF_SEQV) = Tv(T_SEQV);
+    O(PF_SDXX) = Tv(T_SD0); O(PF_SDYY) = Tv(T_SD1); O(PF_SDZZ) = Tv(T_SD2);
+    O(PF_SDYZ) = Tv(T_SD3); O(PF_SDXZ) = Tv(T_SD4); O(PF_SDXY) = Tv(T_SD5);
+    O(PF_SIGY) = Tv(T_SIGY); O(PF_EPEFF) = Tv(T_EPEFF); O(PF_Q) = Tv(T_Q); O(PF_CP) = Tv(T_CP);
+    for (int c = 0; c < 9; c++) O(PF_FP + c) = Dv(D_F0 + c);
+    O(PF_VOL) = Dv(D_VOL);
+    O(PF_IE) = Dv(D_IE);
+    for (int c = 0; c < 3; c++) O(PF_XX + c) = h.xx_pending ? Dv(D_X + c) + Tv(T_BX + c) * ts.dt : Dv(D_X + c);
+  }
+}
+
 template <typename T>
 static void download_grid_T(mpm_engine* e, int id, double* out) {
   HostInst& h = e->inst[id];
@@ -1901,6 +1957,15 @@ extern "C" int mpm_download_particles(mpm_ctx* ctx, double* ps) {
   return guarded([&] {
     if (!ctx || !ps) fail(MPM_ERR_ARG, "bad arguments");
     download_particles(ctx->e, 0, ps);
+  });
+}
+
+extern "C" int mpm_download_particle_subset(mpm_ctx* ctx, int k, const int32_t* ids, double* out) {
+  return guarded([&] {
+    if (!ctx || k < 0 || (k > 0 && (!ids || !out))) fail(MPM_ERR_ARG, "bad arguments");
+    if (k == 0) return;
+    if (ctx->e->prec == MPM_PREC_FP32) download_subset_T<float>(ctx->e, 0, k, ids, out);
+    else download_subset_T<double>(ctx->e, 0, k, ids, out);
   });
 }
 

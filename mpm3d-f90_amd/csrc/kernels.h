@@ -88,6 +88,10 @@ template <typename T>
 void launch_admit(const PoolPtrs<T>& P, const double* ps, const int* pbody, int n, long b0, long pad, int body0,
                   hipStream_t s);
 void launch_fixity(uint8_t* fix, const InstC* inst, int id, long nslots, hipStream_t s);
+// raw fields of the particles of instance id with original index in want (ascending, k of them):
+// T_COUNT + D_COUNT + I_COUNT doubles per particle; n: the instance's particle count
+template <typename T>
+void launch_pack_subset(const PoolPtrs<T>& P, int id, long n, const int* want, int k, double* rows, hipStream_t s);
 void launch_clear_ints(int* a, long n, hipStream_t s);
 // InWhichCell of n particles (x, y, z), their cell-order sort keys and identity values
 void launch_cell_keys(const InstC& I, const double* x, const double* y, const double* z, long n, int* icell,

@@ -3301,6 +3301,34 @@ void launch_permute(S* dst, const S* src, const int* perm, const LaunchP& L, lon
   if (L.nblk) hipLaunchKernelGGL(k_permute<S>, dim3(L.nblk), dim3(L.bs), 0, s, dst, src, perm, L.glist, L.bpg, np, nfield);
 }
 
+// the raw fields (bulk, FP64 and integer, as doubles) of the particles of
+// instance `id` whose original index is in want[0..k) (ascending), one row
+// of T_COUNT + D_COUNT + I_COUNT values per wanted particle
+template <typename T>
+__global__ void __launch_bounds__(256) k_pack_subset(PoolPtrs<T> P, int id, const int* want, int k, double* rows) {
+  const InstC& I = P.inst[id];
+  const long i = (long)I.p_begin + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I.p_end) return;
+  const int o = P.i[(long)I_ORIG * P.np + i];
+  int lo = 0, hi = k - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (want[mid] < o) lo = mid + 1;
+    else hi = mid;
+  }
+  if (k == 0 || want[lo] != o) return;
+  double* r = rows + (long)lo * (T_COUNT + D_COUNT + I_COUNT);
+  for (int f = 0; f < T_COUNT; f++) r[f] = (double)P.t[(long)f * P.np + i];
+  for (int f = 0; f < D_COUNT; f++) r[T_COUNT + f] = P.d[(long)f * P.np + i];
+  for (int f = 0; f < I_COUNT; f++) r[T_COUNT + D_COUNT + f] = (double)P.i[(long)f * P.np + i];
+}
+
+template <typename T>
+void launch_pack_subset(const PoolPtrs<T>& P, int id, long n, const int* want, int k, double* rows, hipStream_t s) {
+  if (n > 0 && k > 0)
+    hipLaunchKernelGGL(k_pack_subset<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, id, want, k, rows);
+}
+
 template <typename T>
 void launch_admit(const PoolPtrs<T>& P, const double* ps, const int* pbody, int n, long b0, long pad, int body0,
                   hipStream_t s) {
@@ -3399,7 +3427,8 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
   template void launch_sgmp_p2g<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);         \
   template void launch_sgmp_nodes<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);       \
   template void launch_sgmp_cells<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);                     \
-  template void launch_admit<T>(const PoolPtrs<T>&, const double*, const int*, int, long, long, int, hipStream_t);
+  template void launch_admit<T>(const PoolPtrs<T>&, const double*, const int*, int, long, long, int, hipStream_t); \
+  template void launch_pack_subset<T>(const PoolPtrs<T>&, int, long, const int*, int, double*, hipStream_t);
 #if MPMK_PART != 2
 INST(double)
 template void launch_permute<int>(int*, const int*, const int*, const LaunchP&, long, int, hipStream_t);

@@ -9,7 +9,7 @@ module mpm_hip
   use, intrinsic :: iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: MPM_PREC_FP64 = 0, MPM_PREC_FP32 = 1
+  integer(c_int), parameter :: MPM_PREC_FP64 = 0, MPM_PREC_FP32 = 1, MPM_PREC_FP64_EXACT = 2
   integer(c_int), parameter :: MPM_USF = 0, MPM_USL = 1, MPM_MUSL = 2
   integer(c_int), parameter :: MPM_SHAPE_LINEAR = 0, MPM_SHAPE_GIMP = 1, &
                                MPM_SHAPE_BSPLINE = 2, MPM_SHAPE_SGMP = 3
@@ -153,6 +153,23 @@ module mpm_hip
        real(c_double), intent(out) :: pstate(*)
        integer(c_int) :: rc
      end function mpm_download_particles
+
+     function mpm_download_particle_subset(ctx, k, ids, out) bind(C, name="mpm_download_particle_subset") &
+          result(rc)
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: k
+       integer(c_int32_t), intent(in) :: ids(*)
+       real(c_double), intent(out) :: out(*)
+       integer(c_int) :: rc
+     end function mpm_download_particle_subset
+
+     function mpm_download_icell(ctx, icell) bind(C, name="mpm_download_icell") result(rc)
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: icell(*)
+       integer(c_int) :: rc
+     end function mpm_download_icell
 
      function mpm_upload_particles(ctx, pstate) bind(C, name="mpm_upload_particles") result(rc)
        import :: c_int, c_ptr, c_double

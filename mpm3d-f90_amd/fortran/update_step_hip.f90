@@ -10,12 +10,16 @@
 ! State: on the first call the module globals built by InputPara
 ! (particle_list, body_list, mat_list, grid description, flags, DT) are
 ! packed into an mpm_problem and uploaded; from then on the device owns the
-! state.  After the last particle phase of each step the particle state is
-! copied back into particle_list so that the reference's own calcEnergy,
-! OutCurve and writers see current values; UpdateDT returns the device DT
-! into the module variable DT.
+! state.  UpdateDT, the last routine of a step, returns the device DT into
+! DT and refreshes what the driver reads next: the kinetic and internal
+! energies and momentum (replacing the host sums calcEnergy just made from
+! stale particles), the curve particles of OutCurve (a gather of those few
+! particles on the device), and, on the steps where the driver writes its
+! TecPlot / ParaView output (its own schedule, plt += OutTime, mirrored
+! here), the whole particle state and the cell indices.
 !
-! Environment: MPM3D_PRECISION=fp32 selects the FP32-bulk path;
+! Environment: MPM3D_PRECISION=fp32 selects the FP32-bulk path, =fp64exact
+! the reference-order FP64 path (bit-identical to the reference);
 ! MPM3D_DUMP=<file> writes the particle state after every step (stream,
 ! n, istep, 8 scalars, PF_COUNT x n doubles).
 !
@@ -34,6 +38,11 @@ module update_step_hip_state
   real(c_double), allocatable, target, save :: ps(:)
   integer(c_int32_t), allocatable, target, save :: pbody(:)
   integer, save :: nsteps_done = 0
+  ! the driver's output schedule (MPM3D.f90:35, 47, 125-126): a default REAL
+  real, save :: plt_mirror = 0.0
+  logical, save :: plt_started = .false.
+  real(c_double), allocatable, target, save :: sub(:)
+  integer(c_int32_t), allocatable, target, save :: icell_buf(:)
 
 contains
 
@@ -143,6 +152,7 @@ contains
     prec = MPM_PREC_FP64
     call get_environment_variable('MPM3D_PRECISION', env)
     if (trim(env) == 'fp32') prec = MPM_PREC_FP32
+    if (trim(env) == 'fp64exact') prec = MPM_PREC_FP64_EXACT
     call mpm_check(mpm_ctx_create(c_loc(prob), prec, ctx), 'mpm_ctx_create')
     j = 0
   end subroutine ensure_context
@@ -207,6 +217,62 @@ contains
     end do
   end subroutine unpack_particles
 
+  ! what the driver reads after UpdateDT (see the module header)
+  subroutine refresh_host_view()
+    use ParticleData
+    use DataOut, only: nCurves, CurvePoint, WriteTecPlot, WriteParaView, OutTime
+    real(c_double) :: ek, ei, mom(3)
+    character(len=8) :: f
+    integer :: q, k, n, a, c
+    ! calcEnergy on the device (Particle.f90:204-230)
+    call mpm_check(mpm_calc_energy(ctx, ek, ei, mom), 'mpm_calc_energy')
+    EngKinetic = ek
+    EngInternal = ei
+    Momentum = mom
+    if (.not. plt_started) then
+       plt_mirror = plt_mirror + OutTime
+       plt_started = .true.
+    end if
+    call get_environment_variable('MPM3D_DUMP', f)
+    if (len_trim(f) > 0 .or. ((WriteTecPlot .or. WriteParaView) .and. CurrentTime >= plt_mirror)) then
+       if ((WriteTecPlot .or. WriteParaView) .and. CurrentTime >= plt_mirror) plt_mirror = plt_mirror + OutTime
+       call unpack_particles()
+       n = nb_particle
+       if (.not. allocated(icell_buf)) allocate(icell_buf(n))
+       call mpm_check(mpm_download_icell(ctx, icell_buf), 'mpm_download_icell')
+       do q = 1, n
+          particle_list(q)%icell = icell_buf(q)
+       end do
+       return
+    end if
+    k = nCurves
+    if (k <= 0) return
+    if (.not. allocated(sub)) allocate(sub(PF_COUNT * k))
+    call mpm_check(mpm_download_particle_subset(ctx, int(k, c_int), int(CurvePoint(1:k), c_int32_t), sub), &
+                   'mpm_download_particle_subset')
+    do q = 1, k
+       associate (p => particle_list(CurvePoint(q)))
+         p%Xp = [sub(PF_X * k + q), sub(PF_Y * k + q), sub(PF_Z * k + q)]
+         p%VXp = [sub(PF_VX * k + q), sub(PF_VY * k + q), sub(PF_VZ * k + q)]
+         p%VOL = sub(PF_VOL * k + q)
+         p%SM = sub(PF_SM * k + q)
+         p%Seqv = sub(PF_SEQV * k + q)
+         p%SDxx = sub(PF_SDXX * k + q); p%SDyy = sub(PF_SDYY * k + q); p%SDzz = sub(PF_SDZZ * k + q)
+         p%SDyz = sub(PF_SDYZ * k + q); p%SDxz = sub(PF_SDXZ * k + q); p%SDxy = sub(PF_SDXY * k + q)
+         p%sig_y = sub(PF_SIGY * k + q)
+         p%epeff = sub(PF_EPEFF * k + q)
+         p%ie = sub(PF_IE * k + q)
+         p%q = sub(PF_Q * k + q)
+         p%cp = sub(PF_CP * k + q)
+         do a = 1, 3
+            do c = 1, 3
+               p%Fp(a, c) = sub((PF_FP + 3 * (a - 1) + (c - 1)) * k + q)
+            end do
+         end do
+       end associate
+    end do
+  end subroutine refresh_host_view
+
   subroutine maybe_dump()
     use ParticleData
     character(len=1024) :: f
@@ -240,10 +306,8 @@ end subroutine ApplyBoundaryConditions
 
 subroutine ParticleStressUpdate()
   use update_step_hip_state
-  use ParticleData, only: USF
   implicit none
   call mpm_check(mpm_particle_stress_update(ctx), 'ParticleStressUpdate')
-  if (.not. USF) call unpack_particles()
 end subroutine ParticleStressUpdate
 
 subroutine GridMomentumUpdate()
@@ -260,10 +324,8 @@ end subroutine IntegrateMomentum
 
 subroutine ParticlePositionUpdate()
   use update_step_hip_state
-  use ParticleData, only: USF
   implicit none
   call mpm_check(mpm_particle_position_update(ctx), 'ParticlePositionUpdate')
-  if (USF) call unpack_particles()
 end subroutine ParticlePositionUpdate
 
 subroutine GridMomentumMUSL()
@@ -290,6 +352,7 @@ subroutine UpdateDT()
      call mpm_check(rc, 'UpdateDT')
   end if
   DT = dt_dev
+  call refresh_host_view()
   nsteps_done = nsteps_done + 1
   call maybe_dump()
 end subroutine UpdateDT

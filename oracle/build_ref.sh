@@ -41,6 +41,11 @@ for f in "$REF/src/FFI.f90" "$REF/src/Material.f90" "$REF/src/Particle.f90" \
   b=$(basename "$f" .f90)
   $FC $FFLAGS -w -c "$f" -o "$OUT/$b.o"
 done
+$FC $FFLAGS -w -c "$REF/src/MPM3D.f90" -o "$OUT/MPM3D.o"
+# the reference program itself (its own main loop, writers and step)
+$FC $FFLAGS -o "$OUT/mpm3d_ref" MPM3D.o FFI.o Material.o Particle.o Grid.o DataOut.o DataIn.o Constitution.o \
+    update_step.o
+echo "built $OUT/mpm3d_ref"
 $FC $FFLAGS -c "$HERE/ref_driver.f90" -o "$OUT/ref_driver.o"
 $FC $FFLAGS -o "$OUT/ref_driver" "$OUT/ref_driver.o" FFI.o Material.o Particle.o \
     Grid.o DataOut.o DataIn.o Constitution.o update_step.o
@@ -53,7 +58,6 @@ LIBDIR="$HERE/../mpm3d-f90_amd"
 if [ -f "$LIBDIR/libmpm3d_hip.so" ]; then
   $FC $FFLAGS -c "$LIBDIR/fortran/mpm_hip.f90" -o "$OUT/mpm_hip.o"
   $FC $FFLAGS -w -c "$LIBDIR/fortran/update_step_hip.f90" -o "$OUT/update_step_hip.o"
-  $FC $FFLAGS -w -c "$REF/src/MPM3D.f90" -o "$OUT/MPM3D.o"
   $FC $FFLAGS -o "$OUT/mpm3d_dropin" MPM3D.o update_step_hip.o mpm_hip.o FFI.o Material.o Particle.o \
       Grid.o DataOut.o DataIn.o Constitution.o -L"$LIBDIR" -lmpm3d_hip -Wl,-rpath,"$LIBDIR"
   echo "built $OUT/mpm3d_dropin"

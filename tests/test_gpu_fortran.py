@@ -90,3 +90,56 @@ def test_reference_driver_runs_the_device_step(oracle_mod, name):
         assert max(errs.values()) <= 1e-12, errs
     finally:
         shutil.rmtree(tmp)
+
+
+REFPROG = os.path.join(ROOT, "oracle", "_ref", "mpm3d_ref")
+
+
+@pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "block_dp_bspline_pull_musl", "dam_null_sgmp_musl"])
+def test_dropin_writes_the_reference_output_files(name):
+    """The reference program itself (oracle/_ref/mpm3d_ref: MPM3D.f90 with its
+    own update_step.f90, on the CPU) and the drop-in (the same main program
+    over the device step in the reference-order mode) on one deck with curve
+    and TecPlot output every few steps.  The drop-in downloads only the curve
+    particles per step and everything at the output times; its TecPlot file
+    is the reference's, character for character, and so is its curve file
+    but for the energy columns (the device's block sums against the
+    reference's serial sums), which agree to the printed precision."""
+    if not (os.path.exists(DROPIN) and os.path.exists(REFPROG)):
+        pytest.skip("reference program / drop-in not built (needs the reference sources at build time)")
+    mpm = load_package()
+    prob = mpm.read_deck(deck_path(name))
+    n, dt = prob.n, prob.dt
+    ids = [5, n // 3, n - 1, 7]
+    extra = ("outt %.9e\ntecp\noutr seqv\noutr velz\noutr epef\ncurv seqv %d\ncurv velz %d\ncurv epef %d\n"
+             "curv engk\ncurv volu %d\n" % ((10.5 * dt,) + tuple(ids)))
+    text = open(deck_path(name)).read()
+    text = re.sub(r"(?mi)^endtime .*$", "endtime %.9e" % (40.5 * dt), text)
+    text = re.sub(r"(?mi)^endi\s*$", extra + "endi", text)
+    out = {}
+    tmp = tempfile.mkdtemp(prefix="outputs_")
+    try:
+        for prog, env in ((REFPROG, {}), (DROPIN, {"MPM3D_PRECISION": "fp64exact"})):
+            d = os.path.join(tmp, os.path.basename(prog))
+            os.makedirs(d)
+            with open(os.path.join(d, name + ".mpm"), "w") as f:
+                f.write(text)
+            r = subprocess.run([prog, name], cwd=d, env=dict(os.environ, **env), capture_output=True, text=True,
+                               timeout=300)
+            assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+            out[prog] = {k: open(os.path.join(d, name + k)).read().splitlines() for k in ("_anim.dat", "_curv.dat")}
+        ref, dev = out[REFPROG], out[DROPIN]
+        assert len(ref["_anim.dat"]) > 4 * n and ref["_anim.dat"] == dev["_anim.dat"]
+        rc, dc = ref["_curv.dat"], dev["_curv.dat"]
+        assert len(rc) >= 40 and len(rc) == len(dc) and rc[0] == dc[0]
+        # energy columns (the reference's two default curves and ours), by header name
+        energy = [k for k, h in enumerate(rc[0].split()) if h.startswith(("engk", "engi"))]
+        assert len(energy) == 3
+        for a, b in zip(rc[1:], dc[1:]):
+            ta, tb = a.split(), b.split()
+            assert len(ta) == len(tb) == 8
+            assert [t for k, t in enumerate(ta) if k not in energy] == [t for k, t in enumerate(tb) if k not in energy]
+            for k in energy:
+                assert abs(float(ta[k]) - float(tb[k])) <= 1e-4 * abs(float(ta[k])) + 1e-300
+    finally:
+        shutil.rmtree(tmp)
