@@ -55,6 +55,26 @@ const char* mpm_version(void);
 int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, int errlen);
 void mpm_problem_free(mpm_problem* p);
 
+/* Output options of a deck (keywords mpm3 (title), endt/outt, rptt, tecp,
+ * para, curv, curx, outr; DataIn.f90:130-330, defaults of DataOut.f90:39-59),
+ * for hosts that write the reference's result files themselves
+ * (mpm3d-f90_amd/output.py); the drop-in runs the reference's own writers. */
+#define MPM_MAX_CURVES 15
+#define MPM_MAX_ANIM 10
+typedef struct mpm_outputs {
+  char title[257];          /* the deck's first line as FFI's ReadLine leaves it: 256 characters, blank-padded */
+  double out_time;          /* OutTime: outt, else EndTime */
+  double report_time;       /* ReportTime (rptt) */
+  int32_t tecplot;          /* tecp */
+  int32_t paraview;         /* para */
+  int32_t n_curves;         /* 2 defaults (engk, engi of particle 1), then curv / curx lines */
+  int32_t curve_option[MPM_MAX_CURVES];   /* result option, 1-based: seqv epef mat pres volu engk engi */
+  int32_t curve_point[MPM_MAX_CURVES];    /*   velx vely velz cels fail sspd damg stsx disx disy disz; particle, 1-based */
+  int32_t n_anim;           /* 1 default (mat), then outr lines */
+  int32_t anim_option[MPM_MAX_ANIM];
+} mpm_outputs;
+int mpm_deck_read_outputs(const char* path, mpm_outputs* out, char* errbuf, int errlen);
+
 /* ---- context -------------------------------------------------------------
  * Uploads the problem (particles as SoA, grid description, materials,
  * flags) to the current HIP device.  The device keeps DT, time and step

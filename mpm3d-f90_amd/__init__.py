@@ -137,6 +137,15 @@ class Context:
         check(_lib.lib().mpm_download_particles(self._h, dptr(out)), "mpm_download_particles")
         return out
 
+    def particle_subset(self, ids):
+        """PF_COUNT x len(ids) state of the particles ids (1-based, as the
+        reference numbers them), gathered on the device."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        out = np.empty((PF_COUNT, len(ids)))
+        check(_lib.lib().mpm_download_particle_subset(self._h, len(ids), iptr(ids), dptr(out)),
+              "mpm_download_particle_subset")
+        return out
+
     def upload_particles(self, ps):
         ps = np.ascontiguousarray(ps, dtype=np.float64)
         check(_lib.lib().mpm_upload_particles(self._h, dptr(ps)), "mpm_upload_particles")

@@ -59,6 +59,17 @@ class ProblemStruct(C.Structure):
                 ("pstate", C.POINTER(C.c_double)), ("pbody", C.POINTER(C.c_int32))]
 
 
+MAX_CURVES, MAX_ANIM = 15, 10
+
+
+class Outputs(C.Structure):
+    """mpm_outputs (include/mpm3d_hip.h): the deck's result-file options"""
+    _fields_ = [("title", C.c_char * 257), ("out_time", C.c_double), ("report_time", C.c_double),
+                ("tecplot", C.c_int32), ("paraview", C.c_int32), ("n_curves", C.c_int32),
+                ("curve_option", C.c_int32 * MAX_CURVES), ("curve_point", C.c_int32 * MAX_CURVES),
+                ("n_anim", C.c_int32), ("anim_option", C.c_int32 * MAX_ANIM)]
+
+
 class Tuning(C.Structure):
     """mpm_tuning (include/mpm3d_engine.h)"""
     _fields_ = [("block_particles", C.c_int32), ("block_nodes", C.c_int32), ("sort_every", C.c_int32),
@@ -112,6 +123,8 @@ def lib():
         "mpm_download_particles": (C.c_int, [vp, P(C.c_double)]),
         "mpm_download_grid": (C.c_int, [vp, P(C.c_double)]),
         "mpm_download_icell": (C.c_int, [vp, P(C.c_int32)]),
+        "mpm_download_particle_subset": (C.c_int, [vp, C.c_int, P(C.c_int32), P(C.c_double)]),
+        "mpm_deck_read_outputs": (C.c_int, [C.c_char_p, P(Outputs), C.c_char_p, C.c_int]),
         "mpm_upload_particles": (C.c_int, [vp, P(C.c_double)]),
         "mpm_get_scalars": (C.c_int, [vp, P(C.c_double)]),
         "mpm_nb_node": (C.c_int, [vp]),

@@ -16,6 +16,7 @@
 // outside the explicit updated-Lagrangian step (contact, TLMPM, dynamic
 // relaxation, Johnson-Cook family, HE burn, EOS 1-3) are rejected with a
 // message naming the keyword.
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -33,11 +34,48 @@ struct DeckError {
   std::string msg;
 };
 
+// The line as FFI's ReadLine leaves it in sss (FFI.f90:191-233), which
+// GetString hands out (the deck title): tabs to blanks, the comment cut,
+// leading blanks removed, every blank followed by a blank or a comma
+// removed, then every blank right after a comma removed; 256 characters,
+// blank-padded.
+static std::string ffi_line(const std::string& line) {
+  std::string s = line;
+  for (auto& c : s)
+    if (c == '\t') c = ' ';
+  const size_t bang = s.find('!');
+  if (bang != std::string::npos) s.resize(bang);
+  const size_t a = s.find_first_not_of(' ');
+  s = a == std::string::npos ? std::string() : s.substr(a);
+  const size_t z = s.find_last_not_of(' ');
+  int len = z == std::string::npos ? 0 : (int)z + 1;
+  std::string b = s;
+  b.resize(258, ' ');
+  auto ch = [&](int i) { return b[i - 1]; };   // 1-based, as sss(i:i)
+  auto del = [&](int i) {                     // sss(i:len-1) = sss(i+1:len); sss(len:len) = ' '
+    b.erase((size_t)i - 1, 1);
+    b.push_back(' ');
+  };
+  for (int i = 1; i < len; i++)
+    while (ch(i) == ' ' && (ch(i + 1) == ' ' || ch(i + 1) == ',')) {
+      del(i);
+      len--;
+    }
+  for (int i = 1; i < len; i++)
+    if (ch(i) == ',' && ch(i + 1) == ' ') {
+      del(i + 1);
+      len--;
+    }
+  b.resize(256);
+  return b;
+}
+
 struct Tokenizer {
   FILE* f = nullptr;
   int line_nb = 0;
   std::vector<std::string> words;  // tokens of the current line
   size_t next = 0;                 // next unread token
+  std::string raw;                 // the current line as read
 
   size_t remaining() const { return words.size() - next; }
 
@@ -53,6 +91,7 @@ struct Tokenizer {
     std::string s(buf);
     while (!s.empty() && (s.back() == '\n' || s.back() == '\r')) s.pop_back();
     if (s.size() > 256) s.resize(256);  // character(256) sss
+    raw = s;
     for (auto& c : s)
       if (c == '\t') c = ' ';
     size_t bang = s.find('!');
@@ -139,6 +178,16 @@ struct Tokenizer {
 };
 
 struct Deck {
+  mpm_outputs outs{};   // output options (DataOut.f90:39-59 defaults set in the constructor)
+  Deck() {
+    memset(outs.title, ' ', 256);
+    outs.n_curves = 2;                // engk and engi of particle 1
+    outs.curve_option[0] = 6;
+    outs.curve_option[1] = 7;
+    outs.curve_point[0] = outs.curve_point[1] = 1;
+    outs.n_anim = 1;                  // the material number
+    outs.anim_option[0] = 3;
+  }
   Tokenizer tk;
   int nb_particle = 0, nb_body = 0, nb_mat = 0, nb_component = 1;
   std::vector<mpm_material> mats;
@@ -459,7 +508,13 @@ struct Deck {
       int key = tk.keyword(kw, 43);
       switch (key) {
         case 1: return;
-        case 2: tk.next = tk.words.size(); break;  // title: rest of line
+        case 2: {   // title: GetString hands out the whole line (DataIn.f90:134)
+          const std::string t = ffi_line(tk.raw);
+          memcpy(outs.title, t.data(), 256);
+          outs.title[256] = 0;
+          tk.next = tk.words.size();
+          break;
+        }
         case 3: {
           nb_particle = tk.get_int();
           if (nb_particle <= 0) fail("nbmp must be positive");
@@ -471,7 +526,7 @@ struct Deck {
           mass.assign(nb_particle, 0.0);
           break;
         }
-        case 4: endtime = tk.get_real(); break;
+        case 4: endtime = tk.get_real(); outs.out_time = endtime; break;   // OutTime = EndTime
         case 5: for (int i = 0; i < 6; i++) span[i] = tk.get_real(); break;
         case 6: span[0] = tk.get_real(); span[1] = tk.get_real(); break;
         case 7: span[2] = tk.get_real(); span[3] = tk.get_real(); break;
@@ -481,8 +536,8 @@ struct Deck {
           dtscale = tk.get_real();
           if (dtscale > 2.5) fail("DTScale too large");
           break;
-        case 11: (void)tk.get_real(); break;  // outtime
-        case 12: (void)tk.get_real(); break;  // rpttime
+        case 11: outs.out_time = tk.get_real(); break;
+        case 12: outs.report_time = tk.get_real(); break;
         case 13: for (int i = 0; i < 6; i++) fix[i] = tk.get_int(); break;
         case 14:
           nb_mat = tk.get_int();
@@ -495,23 +550,45 @@ struct Deck {
         case 18: (void)on_off(); break;  // jaum: Jaumann rotation is always applied
         case 19: set_load(); break;
         case 20: set_velocity(); break;
-        case 21: (void)res_option(); break;
-        case 22:
-          (void)res_option();
+        case 21:
+          if (outs.n_anim >= MPM_MAX_ANIM) fail("more than 10 'outr' variables");
+          outs.anim_option[outs.n_anim++] = res_option();
+          break;
+        case 22: {
+          if (outs.n_curves >= MPM_MAX_CURVES) fail("more than 15 curves");
+          const int k = outs.n_curves++;
+          outs.curve_option[k] = res_option();
+          outs.curve_point[k] = 1;
           if (tk.remaining() > 0) {
             int t = tk.get_int();
             if (t <= 0 || t > nb_particle) fail("invalid curve particle ID");
+            outs.curve_point[k] = t;
           }
           break;
+        }
         case 23: set_eos(); break;
         case 24: for (int i = 0; i < 6; i++) (void)tk.get_real(); break;  // pt2d (plot only)
-        case 25:
+        case 25: {   // SetCurX (DataIn.f90:1158-1192): the last body's first particle within DCell/4
           if (par_counter != nb_particle) fail("curx should be used after particle input");
-          (void)res_option();
-          for (int i = 0; i < 3; i++) (void)tk.get_real();
+          if (outs.n_curves >= MPM_MAX_CURVES) fail("more than 15 curves");
+          const int k = outs.n_curves++;
+          outs.curve_option[k] = res_option();
+          outs.curve_point[k] = 1;
+          double t[3];
+          for (int i = 0; i < 3; i++) t[i] = tk.get_real();
+          for (size_t b = 0; b < bodies.size(); b++)
+            for (int p = bodies[b].par_begin; p < bodies[b].par_end; p++) {
+              const double m = std::max({std::fabs(xp[0][p] - t[0]), std::fabs(xp[1][p] - t[1]),
+                                         std::fabs(xp[2][p] - t[2])});
+              if (m <= 0.25 * dcell) {
+                outs.curve_point[k] = p + 1;
+                break;
+              }
+            }
           break;
+        }
         case 26: for (int i = 0; i < 3; i++) (void)tk.get_real(); break;  // deto (HE only)
-        case 27: break;                                                    // tecp
+        case 27: outs.tecplot = 1; break;
         case 28: bq1 = tk.get_real(); bq2 = tk.get_real(); break;
         case 29: gimp = true; break;
         case 30: fail("contact ('cont') is outside the explicit single-body step");
@@ -526,7 +603,7 @@ struct Deck {
           if (nb_body <= 0) fail("nbbo must be positive");
           bodies.assign(nb_body, mpm_body{});
           break;
-        case 35: break;  // para
+        case 35: outs.paraview = 1; break;
         case 36: fail("dynamic relaxation damping ('drda') is not supported");
         case 37: bspline = true; break;
         case 38: sgmp = true; break;
@@ -666,4 +743,24 @@ extern "C" void mpm_problem_free(mpm_problem* p) {
   free(p->pstate);
   free(p->pbody);
   free(p);
+}
+
+extern "C" int mpm_deck_read_outputs(const char* path, mpm_outputs* out, char* errbuf, int errlen) {
+  if (!out || !path) return MPM_ERR_ARG;
+  Deck d;
+  d.tk.f = fopen(path, "r");
+  if (!d.tk.f) {
+    if (errbuf && errlen > 0) snprintf(errbuf, errlen, "cannot open deck '%s'", path);
+    return MPM_ERR_IO;
+  }
+  try {
+    d.parse();
+  } catch (const DeckError& e) {
+    fclose(d.tk.f);
+    if (errbuf && errlen > 0) snprintf(errbuf, errlen, "%s", e.msg.c_str());
+    return MPM_ERR_DECK;
+  }
+  fclose(d.tk.f);
+  *out = d.outs;
+  return MPM_OK;
 }

@@ -1,0 +1,165 @@
+"""Result files (mpm3d-f90_amd/output.py) against the reference program's own.
+
+* Ew.d editing: known answers printed by the flang-built reference's
+  runtime (ties of the exact binary value to even, signed zero, three-digit
+  exponents without the letter, Inf/NaN);
+* the deck's output options as the reference reads them (mpm_deck_read_outputs:
+  defaults, appended curves and animation variables, curx, the title after
+  ReadLine);
+* the writers driven by the oracle on the reference's schedule produce the
+  files the reference program (oracle/_ref/mpm3d_ref) writes for the same
+  deck, character for character: curve, energy, momentum and contact files,
+  TecPlot frames, ParaView frames, collection and grid.  The oracle's state
+  and energies are the reference's bit for bit, so nothing is compared with
+  a tolerance.
+"""
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, deck_path, load_package
+
+REFPROG = os.path.join(ROOT, "oracle", "_ref", "mpm3d_ref")
+
+
+@pytest.fixture(scope="module")
+def out_mod():
+    load_package()
+    import importlib
+    return importlib.import_module("mpm3d_f90_amd.output")
+
+
+# flang 'E12.4' / 'E10.3' of these values (the reference's runtime)
+E_CASES = [(0.0, 12, 4, "  0.0000E+00"), (-0.0, 12, 4, " -0.0000E+00"), (1.0625, 12, 4, "  0.1062E+01"),
+           (1.00005, 12, 4, "  0.1000E+01"), (0.99995, 12, 4, "  0.1000E+01"), (1e100, 12, 4, "  0.1000+101"),
+           (1e-100, 12, 4, "  0.1000E-99"), (2.5e-310, 12, 4, "  0.2500-309"), (1e300, 12, 4, "  0.1000+301"),
+           (-3.14159, 12, 4, " -0.3142E+01"), (9.99995, 12, 4, "  0.1000E+02"),
+           (0.000123456, 12, 4, "  0.1235E-03"), (1.2345, 12, 4, "  0.1234E+01"), (99999.5, 12, 4, "  0.1000E+06"),
+           (float("inf"), 12, 4, "         Inf"), (float("-inf"), 12, 4, "        -Inf"),
+           (float("nan"), 12, 4, "         NaN"), (0.00009159102723, 10, 3, " 0.916E-04"),
+           (1.0625, 10, 3, " 0.106E+01"), (1e100, 10, 3, " 0.100+101"), (0.15625, 12, 4, "  0.1562E+00"),
+           (0.00015625, 12, 4, "  0.1563E-03"), (2.25, 12, 4, "  0.2250E+01"), (2.75e-3, 12, 4, "  0.2750E-02")]
+
+
+@pytest.mark.parametrize("v,w,d,s", E_CASES)
+def test_e_editing_matches_flang(out_mod, v, w, d, s):
+    assert out_mod.fe(v, w, d) == s
+
+
+def test_a_editing(out_mod):
+    # '(a2,a4,a1)' of '"mat"', '"x"', '"yy"' as flang writes it
+    assert out_mod.fa('"mat"', 2) + out_mod.fa('"x"', 4) + out_mod.fa('"yy"', 1) == '"m "x""'
+
+
+def with_outputs(name, lines, n_end=40.5, n_out=10.5):
+    mpm = load_package()
+    dt = mpm.read_deck(deck_path(name)).dt
+    text = open(deck_path(name)).read()
+    text = re.sub(r"(?mi)^endtime .*$", "endtime %.9e" % (n_end * dt), text)
+    return re.sub(r"(?mi)^endi\s*$", ("outt %.9e\n" % (n_out * dt)) + lines + "endi", text)
+
+
+def write_deck(d, name, text):
+    os.makedirs(d, exist_ok=True)
+    p = os.path.join(d, name + ".mpm")
+    with open(p, "w") as f:
+        f.write(text)
+    return p
+
+
+def test_deck_output_options(out_mod, tmp_path):
+    text = with_outputs("bar_j2_gimp_musl", "tecp\npara\noutr seqv\noutr volu\ncurv velz 666\ncurv engk\n"
+                        "curx pres -0.9 -0.9 0.5\n")
+    o = out_mod.read_outputs(write_deck(str(tmp_path), "bar", text))
+    assert o.title.decode().rstrip() == "mpm3d *** small bar impact: J2 linear hardening,GIMP,MUSL"
+    assert len(o.title) == 256
+    assert o.tecplot == 1 and o.paraview == 1
+    assert [o.anim_option[i] for i in range(o.n_anim)] == [3, 1, 5]
+    assert [(o.curve_option[i], o.curve_point[i]) for i in range(o.n_curves)] == \
+        [(6, 1), (7, 1), (10, 666), (6, 1), (4, 1)]
+    mpm = load_package()
+    # GetReal returns a default REAL (FFI.f90:110)
+    assert o.out_time == float(np.float32(float("%.9e" % (10.5 * mpm.read_deck(deck_path("bar_j2_gimp_musl")).dt))))
+
+
+def test_deck_output_limits(out_mod, tmp_path):
+    text = with_outputs("bar_j2_gimp_musl", "curv seqv 2\n" * 14)
+    with pytest.raises(Exception, match="15 curves"):
+        out_mod.read_outputs(write_deck(str(tmp_path), "bar", text))
+
+
+class OracleHost:
+    """the oracle with the attributes output.run reads"""
+
+    def __init__(self, oracle_mod, problem):
+        self.problem = problem
+        self.o = oracle_mod.Oracle(problem)
+
+    def step(self, n=1):
+        self.o.step(n)
+
+    def scalars(self):
+        return self.o.scalars()
+
+    def particles(self):
+        return self.o.particles()
+
+    def icell(self):
+        return self.o.icell()
+
+
+# two sets of options: between them every curve option but 'fail' (which the
+# flang-built reference cannot write) and every animation option
+OPTION_SETS = {
+    "bar_j2_gimp_musl": "tecp\npara\n" + "".join("outr %s\n" % k for k in
+                                                ("seqv", "epef", "pres", "volu", "engk", "engi", "velx", "vely",
+                                                 "velz"))
+    + "".join("curv %s %d\n" % kv for kv in (("seqv", 5), ("epef", 1999), ("mat", 7), ("pres", 100), ("volu", 7),
+                                             ("engk", 3), ("engi", 4), ("velx", 11), ("vely", 12), ("velz", 666),
+                                             ("cels", 3), ("sspd", 400), ("damg", 2))),
+    "block_dp_bspline_pull_musl": "tecp\npara\n" + "".join("outr %s\n" % k for k in
+                                                          ("cels", "sspd", "damg", "stsx", "disx", "disy", "disz",
+                                                           "fail", "mat"))
+    + "".join("curv %s %d\n" % kv for kv in (("stsx", 5), ("disx", 17), ("disy", 300), ("disz", 511),
+                                             ("pres", 256), ("seqv", 1)))
+    + "curx velz 0.5 0.5 0.5\n",
+}
+
+
+def ref_files(d):
+    return {f: open(os.path.join(d, f)).read() for f in sorted(os.listdir(d)) if not f.endswith((".mpm", ".out"))}
+
+
+@pytest.mark.parametrize("name", sorted(OPTION_SETS))
+def test_writers_match_the_reference_program(out_mod, oracle_mod, name):
+    if not os.path.exists(REFPROG):
+        pytest.skip("reference program not built (oracle/build_ref.sh needs the reference sources)")
+    mpm = load_package()
+    text = with_outputs(name, OPTION_SETS[name])
+    tmp = tempfile.mkdtemp(prefix="outputs_")
+    try:
+        dref, dours = os.path.join(tmp, "ref"), os.path.join(tmp, "ours")
+        write_deck(dref, name, text)
+        p = write_deck(dours, name, text)
+        r = subprocess.run([REFPROG, name], cwd=dref, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        prob = mpm.read_deck(p)
+        steps = out_mod.run(OracleHost(oracle_mod, prob), out_mod.read_outputs(p), name, dours)
+        os.remove(p)
+        ref, ours = ref_files(dref), ref_files(dours)
+        assert sorted(ref) == sorted(ours)
+        assert len([f for f in ref if f.endswith(".vtu")]) == 5      # grid + frames at 0 and 3 output times
+        assert ref[name + "_curv.dat"].count("\n") == steps + 1
+        for f in ref:
+            if ref[f] != ours[f]:
+                a, b = ref[f].splitlines(), ours[f].splitlines()
+                k = next((i for i, (x, y) in enumerate(zip(a, b)) if x != y), min(len(a), len(b)))
+                pytest.fail("%s differs at line %d:\n ref  %r\n ours %r" % (f, k + 1, a[k] if k < len(a) else None,
+                                                                            b[k] if k < len(b) else None))
+    finally:
+        shutil.rmtree(tmp)
