@@ -11,6 +11,7 @@
 // grid mass and FP64 momentum/mass division).
 #include <hip/hip_runtime.h>
 
+#include "boxcheck.h"
 #include "crlog.h"
 #include "engine.h"
 #include "kernels.h"
@@ -586,17 +587,18 @@ template <int SHAPE>
 __device__ __forceinline__ Box2 block_box_cached(const InstC& I, bool act, int cx, int cy, int cz, int jl, int cap,
                                                  int* scratch, int* boxes, int mode, long blk) {
   static_assert(sizeof(Box2) <= BOX_INTS * sizeof(int), "box cache entry");
+  static_assert(sizeof(Box2) == 22 * sizeof(int), "the 22-int record tests/box_cache_shim.cpp reads");
   Box2* entry = (Box2*)(boxes + blk * BOX_INTS);
   if (mode & BOX_LOAD) {
     __shared__ int wbad[16];
     const Box2 C = *entry;   // block-uniform
-    bool bad = !(C.ok && C.inner && C.n <= cap && C.n == C.b[0].n + C.b[1].n);
-    if (!bad && act) {   // the particle's side box is a whole box and holds its stencil
-      const Box Bk = C.get(C.side(jl));
+    // the rules of boxcheck.h: a consistent entry whose boxes lie in the grid,
+    // and every active particle's stencil inside its side's box
+    bool bad = !box_entry_consistent(C, cap, I.ncx, I.ncy, I.ncz);
+    if (!bad && act) {
       constexpr int lo = SHAPE == MPM_SHAPE_LINEAR ? 0 : -1;
       constexpr int hi = SHAPE == MPM_SHAPE_GIMP ? 2 : 1;
-      bad = Bk.n <= 0 || Bk.n != Bk.bx * Bk.by * Bk.bz || cx + lo < Bk.x0 || cy + lo < Bk.y0 || cz + lo < Bk.z0 ||
-            cx + hi >= Bk.x0 + Bk.bx || cy + hi >= Bk.y0 + Bk.by || cz + hi >= Bk.z0 + Bk.bz;
+      bad = !box_entry_holds<lo, hi>(C, jl, cx, cy, cz);
     }
     const unsigned long long m = __ballot(bad);
     if ((threadIdx.x & 63) == 0) wbad[threadIdx.x >> 6] = m != 0ull;

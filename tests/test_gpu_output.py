@@ -6,8 +6,9 @@ frames are equal character for character, and so is every curve column but
 the energies.  Energies and momentum are the device's deterministic block
 sums (the reference sums serially): they agree to the printed 4 digits up to
 one unit in the last place where a value sits on a rounding boundary, and
-momentum components that cancel to round-off (about 1e-17 of the total) are
-compared against the column's scale.
+momentum components that cancel to round-off (1e-13 of the momentum's
+largest component, where the reference and the device differ by 1e-16) are
+compared against the file's largest value.
 """
 import os
 import shutil
@@ -62,7 +63,7 @@ def test_device_writers_match_the_reference_program(name):
         for f, header in ((name + "_curv.dat", True), ("EnergyPlot.dat", False), ("MomentumPlot.dat", False)):
             a, b = numeric_rows(ref[f], header), numeric_rows(ours[f], header)
             assert a.shape == b.shape
-            scale = np.abs(a).max(axis=0)
+            scale = np.abs(a[:, 1:]).max()      # the file's largest energy or momentum component
             assert np.all(np.abs(a - b) <= 1e-4 * np.abs(a) + 1e-12 * scale), f
     finally:
         shutil.rmtree(tmp)
