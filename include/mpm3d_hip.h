@@ -72,6 +72,9 @@ typedef struct mpm_outputs {
   int32_t curve_point[MPM_MAX_CURVES];    /*   velx vely velz cels fail sspd damg stsx disx disy disz; particle, 1-based */
   int32_t n_anim;           /* 1 default (mat), then outr lines */
   int32_t anim_option[MPM_MAX_ANIM];
+  int32_t plot2d;           /* pt2d given: TecPlot frames leave out particles outside plot2d_box */
+  int32_t skip_first;       /*   that were generated after it (0-based index of the first such particle) */
+  double plot2d_box[6];     /*   x0 x1 y0 y1 z0 z1 (a default REAL array in the reference) */
 } mpm_outputs;
 int mpm_deck_read_outputs(const char* path, mpm_outputs* out, char* errbuf, int errlen);
 

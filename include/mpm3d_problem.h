@@ -23,6 +23,11 @@ enum mpm_mat_type {
   MPM_MAT_ELASTIC = 1,   /* elas  -> M3DM1 */
   MPM_MAT_PLA1 = 2,      /* pla1  -> M3DM2, elastic-perfectly plastic */
   MPM_MAT_PLA2 = 3,      /* pla2  -> M3DM3, J2 with linear hardening */
+  MPM_MAT_JOHN = 4,      /* john  -> M3DM4, Johnson-Cook (read; not run, see mpm_deck_read) */
+  MPM_MAT_SJC = 5,       /* sjc   -> M3DM5 + EOS, simplified Johnson-Cook */
+  MPM_MAT_SJCF = 6,      /* sjcf  -> M3DM5 + EOS with failure strain */
+  MPM_MAT_JCF = 7,       /* jcf   -> M3DM8 + EOS with damage (read; not run) */
+  MPM_MAT_HIEX = 8,      /* hiex  -> M3DM6 + EOS (JWL), high explosive burn */
   MPM_MAT_NULL = 9,      /* null  -> M3DM7 + EOS */
   MPM_MAT_DP = 10,       /* dpm   -> M3DM9, Drucker-Prager */
   MPM_MAT_NEOH = 11      /* neoh  -> M3DM10, neo-Hookean */
@@ -36,7 +41,7 @@ enum mpm_scheme { MPM_USF = 0, MPM_USL = 1, MPM_MUSL = 2 };
 
 typedef struct mpm_material {
   int32_t type;           /* enum mpm_mat_type */
-  int32_t eos_type;       /* 0 none, 4 weakly compressible (seos ... 4 c) */
+  int32_t eos_type;       /* 0 none; seos: 1 linear polynomial, 2 Mie-Gruneisen, 3 JWL, 4 weakly compressible */
   double density;
   double young, poisson;
   double yield0, tangmod;
@@ -44,8 +49,12 @@ typedef struct mpm_material {
   double bulk_modulus;    /* null */
   double wavespd;         /* null: sqrt(K/rho) */
   double q_fai, k_fai, q_psi, ten_f;   /* Drucker-Prager */
-  double ceos[10];        /* EOS constants; ceos[0] is c for eos 4 */
-  double det_speed;       /* D (hiex only; kept for SetDT parity) */
+  double ceos[10];        /* cEos(1..10) as SetEos stores them (Gruneisen: C1..C5 derived; E0 in ceos[9]) */
+  double det_speed;       /* D: detonation velocity (hiex; SetDT/UpdateDT take it as a sound speed) */
+  double b_jc, n_jc, c_jc, m_jc;   /* Johnson-Cook B, n, C, m */
+  double epso;            /* reference strain rate */
+  double epf;             /* failure strain (sjcf, jcf) */
+  double roomt, melt, spec_heat;   /* john, jcf: room and melting temperature, specific heat */
 } mpm_material;
 
 typedef struct mpm_body {
@@ -70,6 +79,10 @@ enum mpm_pfield {
   PF_SIGY, PF_EPEFF, PF_IE, PF_Q, PF_CP,
   PF_FP,                        /* 9 entries */
   PF_XX = PF_FP + 9, PF_XY, PF_XZ,   /* XX: position at t+1 */
+  PF_TEMP,                      /* celsius_t: temperature (293 at start, Particle.f90:131) */
+  PF_DMG,                       /* damage */
+  PF_LT,                        /* lighting time of high explosive (Initial, DataIn.f90:1252-1260) */
+  PF_FAIL,                      /* failure flag, 0 or 1 */
   PF_COUNT
 };
 
@@ -90,7 +103,7 @@ typedef struct mpm_problem {
   int32_t fix[6];         /* FixS(1..6): x0 xn y0 yn z0 zn */
   /* grid, after SetGridData (Grid.f90:250-263) */
   int32_t ncx, ncy, ncz;  /* NumCellx/y/z */
-  int32_t pad0;
+  int32_t dr_damping;     /* drda: dynamic relaxation damping after UpdateDT (Particle.f90:172-202) */
   double span[6];         /* SpanX(1), SpanX(2), SpanY(1), ... (upper ends adjusted) */
   double dcell;
   double cutoff;          /* grid mass cut-off (DataIn.f90:517-524) */

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libmpm3d_hip.so")
 PF_NAMES = (["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sm", "seqv",
              "sdxx", "sdyy", "sdzz", "sdyz", "sdxz", "sdxy", "sig_y", "epeff", "ie", "q", "cp"]
             + ["fp%d%d" % (i + 1, j + 1) for i in range(3) for j in range(3)]
-            + ["xx", "xy", "xz"])
+            + ["xx", "xy", "xz", "temp", "dmg", "lt", "fail"])
 PF = {n: i for i, n in enumerate(PF_NAMES)}
 PF_COUNT = len(PF_NAMES)
 GF_NAMES = ["mg", "px", "py", "pz", "fx", "fy", "fz"]
@@ -38,7 +38,9 @@ class Material(C.Structure):
                 ("tangmod", C.c_double), ("miu", C.c_double), ("bulk_modulus", C.c_double),
                 ("wavespd", C.c_double), ("q_fai", C.c_double), ("k_fai", C.c_double),
                 ("q_psi", C.c_double), ("ten_f", C.c_double), ("ceos", C.c_double * 10),
-                ("det_speed", C.c_double)]
+                ("det_speed", C.c_double), ("b_jc", C.c_double), ("n_jc", C.c_double), ("c_jc", C.c_double),
+                ("m_jc", C.c_double), ("epso", C.c_double), ("epf", C.c_double), ("roomt", C.c_double),
+                ("melt", C.c_double), ("spec_heat", C.c_double)]
 
 
 class Body(C.Structure):
@@ -51,7 +53,7 @@ class ProblemStruct(C.Structure):
                 ("shape", C.c_int32), ("scheme", C.c_int32), ("quasi_load", C.c_int32),
                 ("quasi_damp", C.c_int32), ("gravity", C.c_int32), ("update_v_by_fp", C.c_int32),
                 ("fix", C.c_int32 * 6), ("ncx", C.c_int32), ("ncy", C.c_int32), ("ncz", C.c_int32),
-                ("pad0", C.c_int32), ("span", C.c_double * 6), ("dcell", C.c_double),
+                ("dr_damping", C.c_int32), ("span", C.c_double * 6), ("dcell", C.c_double),
                 ("cutoff", C.c_double), ("dt", C.c_double), ("dtscale", C.c_double),
                 ("endtime", C.c_double), ("quasi_load_time", C.c_double),
                 ("damp_factor", C.c_double), ("bq1", C.c_double), ("bq2", C.c_double),
@@ -67,7 +69,8 @@ class Outputs(C.Structure):
     _fields_ = [("title", C.c_char * 257), ("out_time", C.c_double), ("report_time", C.c_double),
                 ("tecplot", C.c_int32), ("paraview", C.c_int32), ("n_curves", C.c_int32),
                 ("curve_option", C.c_int32 * MAX_CURVES), ("curve_point", C.c_int32 * MAX_CURVES),
-                ("n_anim", C.c_int32), ("anim_option", C.c_int32 * MAX_ANIM)]
+                ("n_anim", C.c_int32), ("anim_option", C.c_int32 * MAX_ANIM), ("plot2d", C.c_int32),
+                ("skip_first", C.c_int32), ("plot2d_box", C.c_double * 6)]
 
 
 class Tuning(C.Structure):

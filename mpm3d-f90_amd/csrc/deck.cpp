@@ -196,6 +196,9 @@ struct Deck {
   double span[6] = {0, 0, 0, 0, 0, 0};
   double dcell = 0, cutoff = 0, dtscale = (double)0.9f, endtime = 0;
   double qlt = 0, damp = 0, bq1 = 1.5, bq2 = (double)0.06f;
+  bool dr_damping = false;
+  int n_deto = 0;
+  double deto[10][3] = {};
   int fix[6] = {0, 0, 0, 0, 0, 0};
   bool musl = false, usl = false, usf = false, update_method = false;
   bool gimp = false, bspline = false, sgmp = false, quasi_load = false, quasi_damp = false;
@@ -285,9 +288,27 @@ struct Deck {
           m.density = tk.get_real(); m.young = tk.get_real(); m.poisson = tk.get_real();
           uvfp = true;
           break;
-        case 4: case 5: case 6: case 7: case 8:
-          fail(std::string("material model '") + kw[t - 1] +
-               "' (Johnson-Cook family / HE burn) is outside the explicit-step build");
+        case 4: case 7:   // john / jcf
+          m.type = t;
+          m.density = tk.get_real(); m.young = tk.get_real(); m.poisson = tk.get_real();
+          m.yield0 = tk.get_real();
+          m.b_jc = tk.get_real(); m.n_jc = tk.get_real(); m.c_jc = tk.get_real(); m.m_jc = tk.get_real();
+          m.roomt = tk.get_real(); m.melt = tk.get_real(); m.spec_heat = tk.get_real();
+          m.epso = tk.get_real();
+          if (t == 7) m.epf = tk.get_real();
+          break;
+        case 5: case 6:   // sjc / sjcf
+          m.type = t;
+          m.density = tk.get_real(); m.young = tk.get_real(); m.poisson = tk.get_real();
+          m.yield0 = tk.get_real();
+          m.b_jc = tk.get_real(); m.n_jc = tk.get_real(); m.c_jc = tk.get_real();
+          m.epso = tk.get_real();
+          if (t == 6) m.epf = tk.get_real();
+          break;
+        case 8:   // hiex
+          m.type = 8;
+          m.density = tk.get_real(); m.det_speed = tk.get_real();
+          break;
         default:
           fail("invalid material type");
       }
@@ -491,11 +512,32 @@ struct Deck {
     int mset = tk.get_int();
     if (mset <= 0 || mset > nb_mat) fail("seos: material id out of range");
     int et = tk.get_int();
-    mats[mset - 1].eos_type = et;
-    if (et == 4) {
-      mats[mset - 1].ceos[0] = tk.get_real();
-    } else {
-      fail("seos: only the weakly compressible EOS (type 4) is on the explicit-step path");
+    mpm_material& m = mats[mset - 1];
+    m.eos_type = et;
+    switch (et) {   // SetEos (DataIn.f90:1095-1155)
+      case 1:       // linear polynomial: C0..C6, E0
+        for (int k = 0; k < 7; k++) m.ceos[k] = tk.get_real();
+        m.ceos[9] = tk.get_real();
+        break;
+      case 2: {     // Mie-Gruneisen from c0, lambda (s), gamma0, E0
+        const double c0 = tk.get_real(), lambda = tk.get_real(), gamma0 = tk.get_real();
+        m.ceos[0] = m.density * (c0 * c0);
+        m.ceos[1] = m.ceos[0] * (2 * lambda - 1);
+        m.ceos[2] = m.ceos[0] * (lambda - 1) * (3 * lambda - 1);
+        m.ceos[3] = m.density * gamma0;
+        m.ceos[4] = gamma0;
+        m.ceos[9] = tk.get_real();
+        break;
+      }
+      case 3:       // JWL: A, B, R1, R2, omega, E0
+        for (int k = 0; k < 5; k++) m.ceos[k] = tk.get_real();
+        m.ceos[9] = tk.get_real();
+        break;
+      case 4:       // weakly compressible: c
+        m.ceos[0] = tk.get_real();
+        break;
+      default:
+        fail("error in EOS type (seos)");
     }
   }
 
@@ -567,7 +609,11 @@ struct Deck {
           break;
         }
         case 23: set_eos(); break;
-        case 24: for (int i = 0; i < 6; i++) (void)tk.get_real(); break;  // pt2d (plot only)
+        case 24:   // pt2d: particles generated from here on outside the box are not plotted
+          outs.plot2d = 1;
+          outs.skip_first = par_counter;
+          for (int i = 0; i < 6; i++) outs.plot2d_box[i] = (double)(float)tk.get_real();   // real plot2d(6)
+          break;
         case 25: {   // SetCurX (DataIn.f90:1158-1192): the last body's first particle within DCell/4
           if (par_counter != nb_particle) fail("curx should be used after particle input");
           if (outs.n_curves >= MPM_MAX_CURVES) fail("more than 15 curves");
@@ -587,7 +633,11 @@ struct Deck {
             }
           break;
         }
-        case 26: for (int i = 0; i < 3; i++) (void)tk.get_real(); break;  // deto (HE only)
+        case 26:   // deto: a detonation point (DataIn.f90:265-269)
+          if (n_deto >= 10) fail("more than 10 detonation points");
+          for (int i = 0; i < 3; i++) deto[n_deto][i] = tk.get_real();
+          n_deto++;
+          break;
         case 27: outs.tecplot = 1; break;
         case 28: bq1 = tk.get_real(); bq2 = tk.get_real(); break;
         case 29: gimp = true; break;
@@ -604,7 +654,7 @@ struct Deck {
           bodies.assign(nb_body, mpm_body{});
           break;
         case 35: outs.paraview = 1; break;
-        case 36: fail("dynamic relaxation damping ('drda') is not supported");
+        case 36: dr_damping = true; break;
         case 37: bspline = true; break;
         case 38: sgmp = true; break;
         case 39: break;  // smoo: smoothed pressure feeds only the output writers
@@ -640,9 +690,21 @@ extern "C" int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, 
     if (d.dcell == 0.0) d.fail("DCell must be defined");
     for (int m = 0; m < d.nb_mat; m++)
       if (d.mats[m].type == 0) d.fail("material " + std::to_string(m + 1) + " not defined");
-    for (int m = 0; m < d.nb_mat; m++)
-      if (d.mats[m].type == MPM_MAT_NULL && d.mats[m].eos_type != 4)
-        d.fail("null material needs 'seos <mat> 4 <c>'");
+    for (int m = 0; m < d.nb_mat; m++) {
+      const mpm_material& mt = d.mats[m];
+      const std::string id = "material " + std::to_string(m + 1);
+      if (mt.type == MPM_MAT_JOHN || mt.type == MPM_MAT_JCF)
+        d.fail(id + ": 'john'/'jcf' heat (and damage) each particle with the plastic strain increment the "
+               "last yielding particle left in a module variable (Constitution.f90:237, 290, 314), an order "
+               "dependence this build does not reproduce");
+      // seleos is called by sjc, sjcf, hiex and null (Constitution.f90:239-330)
+      const bool needs_eos = mt.type == MPM_MAT_SJC || mt.type == MPM_MAT_SJCF || mt.type == MPM_MAT_HIEX ||
+                             mt.type == MPM_MAT_NULL;
+      if (needs_eos && (mt.eos_type < 1 || mt.eos_type > 4)) d.fail(id + " needs an equation of state ('seos')");
+      // eos3 scales by the burn fraction only HE burn sets (for others the
+      // reference reads the last HE particle's value)
+      if (mt.eos_type == 3 && mt.type != MPM_MAT_HIEX) d.fail(id + ": the JWL EOS is used with 'hiex' only");
+    }
 
     mpm_problem* p = (mpm_problem*)calloc(1, sizeof(mpm_problem));
     const int n = d.nb_particle;
@@ -669,6 +731,7 @@ extern "C" int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, 
     p->damp_factor = d.damp;
     p->bq1 = d.bq1;
     p->bq2 = d.bq2;
+    p->dr_damping = d.dr_damping;
     p->mats = (mpm_material*)calloc(d.nb_mat, sizeof(mpm_material));
     p->bodies = (mpm_body*)calloc(d.nb_body, sizeof(mpm_body));
     for (int m = 0; m < d.nb_mat; m++) p->mats[m] = d.mats[m];
@@ -702,6 +765,17 @@ extern "C" int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, 
           ps[(size_t)PF_CP * n + i] = mt.wavespd;
           mt.ceos[0] = mt.wavespd;
         }
+        // lighting time: the nearest detonation point's distance over D,
+        // through a default REAL (DataIn.f90:1252-1260)
+        double lt = (double)10e6f;
+        for (int j = 0; j < d.n_deto; j++) {
+          const double dx = ps[(size_t)PF_X * n + i] - d.deto[j][0], dy = ps[(size_t)PF_Y * n + i] - d.deto[j][1],
+                       dz = ps[(size_t)PF_Z * n + i] - d.deto[j][2];
+          const float temp = (float)(std::sqrt(dx * dx + dy * dy + dz * dz) / mt.det_speed);
+          lt = (double)temp < lt ? (double)temp : lt;
+        }
+        ps[(size_t)PF_LT * n + i] = lt;
+        ps[(size_t)PF_TEMP * n + i] = (double)293.0f;   // InitParticle (Particle.f90:131)
       }
     }
     // SetGridData (Grid.f90:238-263)

@@ -51,7 +51,8 @@ enum InstFlags {
   IF_GRAVITY = 4,
   IF_UVFP = 8,
   IF_LOADS = 16,         // some particle carries a prescribed load FXp
-  IF_ENDTIME = 32        // instance stops itself when time > endtime
+  IF_ENDTIME = 32,       // instance stops itself when time > endtime
+  IF_DRDA = 64           // dynamic relaxation damping after UpdateDT (drda)
 };
 
 // per-instance time state, advanced on the device (graph-capturable)
@@ -63,6 +64,7 @@ struct InstT {
   int done;
   int nan_count;         // particles whose stress update produced NaN
   int negvol_count;      // particles with negative volume (warning in the reference)
+  int damped;            // drda: this step's kinetic energy did not grow, velocities are zeroed
 };
 
 // per-material constants, all derived exactly as the reference derives them
@@ -73,6 +75,8 @@ struct MatC {
   double Gmod, Kmod, q_fai, k_fai, q_psi, tenf;   // Drucker-Prager (tenf = min(ten_f, k/q))
   double c0;             // EOS 4 sound speed
   double csound;         // SetDT/UpdateDT sound speed max(c, D, Wavespd)
+  double ce[7];          // EOS 1, 2: cEos(1..7) through a default REAL; EOS 3: A, B, R1, R2, omega
+  double bjc, njc, cjc, epso, epf;   // (simplified) Johnson-Cook
 };
 
 struct BodyC {
@@ -93,10 +97,15 @@ enum TField {
 };
 
 // particle fields always kept in FP64: position, volume and internal energy
-// (accumulated every step), F (det F, ln J of the neo-Hookean model) and the
+// (accumulated every step), F (det F, ln J of the neo-Hookean model), the
 // mass as read (the weakly compressible EOS takes the small difference
-// m/V - rho0: an FP32 mass alone would offset it by 1e-5 of itself)
-enum DField { D_X = 0, D_Y, D_Z, D_VOL, D_IE, D_F0, D_F1, D_F2, D_F3, D_F4, D_F5, D_F6, D_F7, D_F8, D_MASS, D_COUNT };
+// m/V - rho0: an FP32 mass alone would offset it by 1e-5 of itself), and
+// the state only some models change: temperature, damage, the HE lighting
+// time and the failure flag (0/1)
+enum DField {
+  D_X = 0, D_Y, D_Z, D_VOL, D_IE, D_F0, D_F1, D_F2, D_F3, D_F4, D_F5, D_F6, D_F7, D_F8, D_MASS,
+  D_TEMP, D_DMG, D_LT, D_FAIL, D_COUNT
+};
 
 // particle integer fields
 enum IField { I_BODY = 0, I_ORIG, I_CELL, I_COUNT };

@@ -132,6 +132,7 @@ struct HostInst {
 struct Group {
   int shape = 0, scheme = 0;
   int matmask = 0;   // material models of the group's instances (bits 1 << type)
+  bool drda = false; // some instance damps by dynamic relaxation
   std::vector<int> insts;
   std::vector<int> pgran, ngran;
   int* d_pgran = nullptr;
@@ -413,6 +414,13 @@ static MatC make_mat(const mpm_material& m) {
     c.tenf = m.ten_f < tmax ? m.ten_f : tmax;
   }
   c.c0 = m.ceos[0];
+  for (int k = 0; k < 7; k++)   // eos1/eos2 hold their constants in default REALs
+    c.ce[k] = (m.eos_type == 1 || m.eos_type == 2) ? (double)(float)m.ceos[k] : m.ceos[k];
+  c.bjc = m.b_jc;
+  c.njc = m.n_jc;
+  c.cjc = m.c_jc;
+  c.epso = m.epso;
+  c.epf = m.epf;
   // SetDT / UpdateDT sound speed (DataIn.f90:1285-1290, update_step.f90:1861-1866)
   double E = m.young, nu = m.poisson, ro = m.density;
   double cp = std::sqrt(E * (1 - nu) / (1 + nu) / (1 - 2 * nu) / ro);
@@ -428,11 +436,14 @@ static void check_problem(const mpm_problem* p) {
     fail(MPM_ERR_ARG, "incomplete problem description");
   if (p->shape < 0 || p->shape > 3 || p->scheme < 0 || p->scheme > 2) fail(MPM_ERR_ARG, "bad shape/scheme");
   for (int m = 0; m < p->nb_mat; m++) {
-    int t = p->mats[m].type;
-    if (t != MPM_MAT_ELASTIC && t != MPM_MAT_PLA1 && t != MPM_MAT_PLA2 && t != MPM_MAT_NULL && t != MPM_MAT_DP &&
-        t != MPM_MAT_NEOH)
-      fail(MPM_ERR_UNSUPPORTED, "material type %d is not on the device path", t);
-    if (t == MPM_MAT_NULL && p->mats[m].eos_type != 4) fail(MPM_ERR_UNSUPPORTED, "null material needs EOS 4");
+    const int t = p->mats[m].type, et = p->mats[m].eos_type;
+    if (t == MPM_MAT_JOHN || t == MPM_MAT_JCF)
+      fail(MPM_ERR_UNSUPPORTED, "material type %d (john/jcf) is not on the device path", t);
+    if (t < MPM_MAT_ELASTIC || t > MPM_MAT_NEOH) fail(MPM_ERR_ARG, "material type %d", t);
+    // models that call seleos need an EOS; JWL only with HE burn (its burn fraction)
+    const bool needs_eos = t == MPM_MAT_SJC || t == MPM_MAT_SJCF || t == MPM_MAT_HIEX || t == MPM_MAT_NULL;
+    if (needs_eos && (et < 1 || et > 4)) fail(MPM_ERR_ARG, "material %d needs an equation of state", m + 1);
+    if (et == 3 && t != MPM_MAT_HIEX) fail(MPM_ERR_UNSUPPORTED, "material %d: JWL EOS without HE burn", m + 1);
   }
   for (int b = 0; b < p->nb_body; b++)
     if (p->bodies[b].mat < 1 || p->bodies[b].mat > p->nb_mat) fail(MPM_ERR_ARG, "body %d has no material", b);
@@ -527,6 +538,10 @@ static void upload_particles_T(mpm_engine* e, const HostInst& h, const double* p
     db[(size_t)D_Z * pad + i] = P(PF_Z, i);
     for (int k = 0; k < 9; k++) db[(size_t)(D_F0 + k) * pad + i] = P(PF_FP + k, i);
     db[(size_t)D_MASS * pad + i] = P(PF_MASS, i);
+    db[(size_t)D_TEMP * pad + i] = P(PF_TEMP, i);
+    db[(size_t)D_DMG * pad + i] = P(PF_DMG, i);
+    db[(size_t)D_LT * pad + i] = P(PF_LT, i);
+    db[(size_t)D_FAIL * pad + i] = P(PF_FAIL, i);
   }
   if (!first) {
     // upload keeps the device permutation: scatter host rows to their slots
@@ -618,7 +633,8 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   c.shape = p->shape;
   c.scheme = p->scheme;
   c.flags = (p->quasi_load ? IF_QUASI_LOAD : 0) | (p->quasi_damp ? IF_QUASI_DAMP : 0) |
-            (p->gravity ? IF_GRAVITY : 0) | (p->update_v_by_fp ? IF_UVFP : 0) | flags_extra;
+            (p->gravity ? IF_GRAVITY : 0) | (p->update_v_by_fp ? IF_UVFP : 0) | (p->dr_damping ? IF_DRDA : 0) |
+            flags_extra;
   bool loads = false;
   for (int i = 0; i < n && !loads; i++)
     for (int d = 0; d < 3; d++)
@@ -781,6 +797,7 @@ static void rebuild_lists(mpm_engine* e) {
     Group& G = e->groups[g];
     G.insts.push_back(id);
     G.matmask |= h.matmask;
+    G.drda = G.drda || (h.c.flags & IF_DRDA) != 0;
     for (long k = 0; k < h.npg; k++) G.pgran.push_back((int)(h.pg0 + k));
     for (long k = 0; k < h.nng; k++) G.ngran.push_back((int)(h.ng0 + k));
   }
@@ -1048,6 +1065,7 @@ static void run_phase(mpm_engine* e, int ph) {
       case PH_DT:
         mpmk::launch_energy<T>(P, L, s);
         mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), FIN_DT, s);
+        if (G.drda) mpmk::launch_damp<T>(P, L, s);
         break;
     }
   }
@@ -1163,6 +1181,7 @@ static void enqueue_step_exact(mpm_engine* e, int fin_flags) {
     timed(e, KI_FINALIZE, [&] {
       mpmk::launch_energy<double>(P, L, s);
       mpmk::launch_finalize<double>(P, G.d_ilist, (int)G.insts.size(), FIN_ENERGY | FIN_DT | fin_flags, s);
+      if (G.drda) mpmk::launch_damp<double>(P, L, s);
     });
   }
 }
@@ -1206,6 +1225,7 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
     }
     timed(e, KI_FINALIZE, [&] {
       mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), FIN_ENERGY | FIN_DT | fin_flags, s);
+      if (G.drda) mpmk::launch_damp<T>(P, L, s);
     });
   }
 }
@@ -1706,6 +1726,10 @@ static void download_particles_T(mpm_engine* e, int id, double* out) {
     for (int k = 0; k < 9; k++) O(PF_FP + k) = db[(size_t)(D_F0 + k) * pad + j];
     O(PF_VOL) = db[(size_t)D_VOL * pad + j];
     O(PF_IE) = db[(size_t)D_IE * pad + j];
+    O(PF_TEMP) = db[(size_t)D_TEMP * pad + j];
+    O(PF_DMG) = db[(size_t)D_DMG * pad + j];
+    O(PF_LT) = db[(size_t)D_LT * pad + j];
+    O(PF_FAIL) = db[(size_t)D_FAIL * pad + j];
     for (int k = 0; k < 3; k++) {
       const double x = db[(size_t)(D_X + k) * pad + j];
       O(PF_XX + k) = h.xx_pending ? x + Tv(T_BX + k) * ts.dt : x;
@@ -1766,6 +1790,10 @@ static void download_subset_T(mpm_engine* e, int id, int k, const int32_t* ids, 
     for (int c = 0; c < 9; c++) O(PF_FP + c) = Dv(D_F0 + c);
     O(PF_VOL) = Dv(D_VOL);
     O(PF_IE) = Dv(D_IE);
+    O(PF_TEMP) = Dv(D_TEMP);
+    O(PF_DMG) = Dv(D_DMG);
+    O(PF_LT) = Dv(D_LT);
+    O(PF_FAIL) = Dv(D_FAIL);
     for (int c = 0; c < 3; c++) O(PF_XX + c) = h.xx_pending ? Dv(D_X + c) + Tv(T_BX + c) * ts.dt : Dv(D_X + c);
   }
 }

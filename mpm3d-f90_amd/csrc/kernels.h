@@ -78,6 +78,7 @@ template <typename T> void launch_node_va(const PoolPtrs<T>& P, const LaunchP& L
 template <typename T> void launch_zero_nodes(T* a, int ncomp, long stride, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_zero_grid(const PoolPtrs<T>& P, bool p2, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_energy(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
+template <typename T> void launch_damp(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
 template <typename T>
 void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int what, hipStream_t s);
 void launch_begin_step(InstT* ts, const int* ilist, int n, hipStream_t s);

@@ -1643,14 +1643,16 @@ __host__ __device__ constexpr bool mm_has(int MM, int type) { return ((MM >> typ
 
 // the particle state the constitutive update reads (loaded ahead of the
 // strain-rate gather so that the loads overlap the block's other latencies).
-// sig_y is read only by pla1/pla2 and Seqv only kept by the null material
-// (the others recompute it), so the loads depend on the models MM.
+// sig_y is read only by the hardening models and Seqv only kept by the
+// null, HE burn and (failed) sjcf models (the others recompute it), so the
+// loads depend on the models MM.
 template <typename T, int MM>
 __device__ __forceinline__ void cst_load(const PV<T>& p, Cst<T>& m) {
   m.vold = p.d(D_VOL);
   m.epeff = p.t(T_EPEFF);
-  m.sigy = (mm_has(MM, MPM_MAT_PLA1) || mm_has(MM, MPM_MAT_PLA2)) ? p.t(T_SIGY) : (T)0;
-  m.seqv = mm_has(MM, MPM_MAT_NULL) ? p.t(T_SEQV) : (T)0;
+  m.sigy = (mm_has(MM, MPM_MAT_PLA1) || mm_has(MM, MPM_MAT_PLA2) || mm_has(MM, MPM_MAT_SJC) ||
+            mm_has(MM, MPM_MAT_SJCF)) ? p.t(T_SIGY) : (T)0;
+  m.seqv = (mm_has(MM, MPM_MAT_NULL) || mm_has(MM, MPM_MAT_HIEX) || mm_has(MM, MPM_MAT_SJCF)) ? p.t(T_SEQV) : (T)0;
   m.iener = p.d(D_IE);
   m.mp = p.t(T_MASS);
   m.cp = p.t(T_CP);
@@ -1739,13 +1741,130 @@ __device__ __forceinline__ void lieupd(Cst<T>& m, const InstC& I, T dt) {   // :
 }
 
 template <typename T>
-__device__ __forceinline__ void hieupd(Cst<T>& m) {   // :649-676, non-failed branch
-  const float vavg = (float)(m.vol + m.vold);
-  const T e = m.dinc[0] * (m.sold[0] + m.sd[0]) + m.dinc[1] * (m.sold[1] + m.sd[1]) +
-              m.dinc[2] * (m.sold[2] + m.sd[2]) + m.dinc[3] * (m.sold[3] + m.sd[3]) +
-              m.dinc[4] * (m.sold[4] + m.sd[4]) + m.dinc[5] * (m.sold[5] + m.sd[5]);
-  m.ieinc = (T)0.25 * e * (T)vavg + (T)m.dvol * (m.smold - (T)2 * m.bqf);
+__device__ __forceinline__ void hieupd(Cst<T>& m, bool failure) {   // :649-676
+  if (failure) {
+    m.ieinc = (T)m.dvol * (m.smold - (T)2 * m.bqf);
+  } else {
+    const float vavg = (float)(m.vol + m.vold);
+    const T e = m.dinc[0] * (m.sold[0] + m.sd[0]) + m.dinc[1] * (m.sold[1] + m.sd[1]) +
+                m.dinc[2] * (m.sold[2] + m.sd[2]) + m.dinc[3] * (m.sold[3] + m.sd[3]) +
+                m.dinc[4] * (m.sold[4] + m.sd[4]) + m.dinc[5] * (m.sold[5] + m.sd[5]);
+    m.ieinc = (T)0.25 * e * (T)vavg + (T)m.dvol * (m.smold - (T)2 * m.bqf);
+  }
   m.iener = m.iener + m.ieinc;
+}
+
+// log, exp and real powers: in the FP64 paths the correctly rounded ones of
+// crlog.h (reproducible on any libm, and what the reference's glibc returns
+// in all but rare cases: a one-ulp difference in the Johnson-Cook yield
+// stress can flip its return-map test), the device libm's in the FP32 path
+template <typename T, int MM>
+__device__ __forceinline__ bool m_cr() { return (MM & MM_CRLOG) || sizeof(T) == 8; }
+template <typename T, int MM>
+__device__ __forceinline__ double m_log(double x) { return m_cr<T, MM>() ? mpm_log_cr(x) : log(x); }
+template <typename T, int MM>
+__device__ __forceinline__ double m_exp(double x) { return m_cr<T, MM>() ? mpm_exp_cr(x) : exp(x); }
+template <typename T, int MM>
+__device__ __forceinline__ double m_pow(double x, double y) { return m_cr<T, MM>() ? mpm_pow_cr(x, y) : pow(x, y); }
+
+// M3DM5, simplified Johnson-Cook (Constitution.f90:1043-1100): the hardening
+// modulus at epeff + 1e-4 and the rate-dependent yield stress in FP64 (the
+// literals 0.0001 and 0.001 are default REALs), the radial return in T
+template <typename T, int MM>
+__device__ __forceinline__ void m3dm5(Cst<T>& m, const MatC& mc, T G2, double dtd) {
+  const double e4 = (double)0.0001f, rmin = (double)0.001f;
+  double ep = (double)m.epeff + e4;
+  const T plamod = (T)(mc.bjc * mc.njc * m_pow<T, MM>(ep, mc.njc - 1));
+  ep = ep - e4;
+  m.epeff = (T)ep;
+  elastic_devi(m, G2);
+  m.seqv = eq_stress(m);
+  if (m.seqv > m.sigy) {
+    const T depeff = (m.seqv - m.sigy) / ((T)1.5 * G2 + plamod);
+    m.epeff = m.epeff + depeff;
+    double srate = (double)depeff / mc.epso / dtd;
+    if (srate < rmin) srate = rmin;
+    m.sigy = (T)((mc.yield0 + mc.bjc * m_pow<T, MM>((double)m.epeff, mc.njc)) * (1 + mc.cjc * m_log<T, MM>(srate)));
+    if (m.sigy > m.seqv) {
+      m.epeff = m.epeff - depeff;
+    } else {
+      const T ratio = m.sigy / m.seqv;
+#pragma unroll
+      for (int k = 0; k < 6; k++) m.sd[k] = m.sd[k] * ratio;
+      m.seqv = m.seqv * ratio;
+    }
+  }
+}
+
+// seleos (Constitution.f90:1333-1541) in FP64 from the FP64 mass and volume
+// (the reference's mu, relative volume and specific energy); EOS 1 and 2
+// keep A, B and the pressure in default REALs, as the reference declares
+// them; bfac: the HE burn fraction (EOS 3)
+template <typename T, int MM>
+__device__ __forceinline__ void seleos(Cst<T>& m, const PV<T>& p, const MatC& mc, const InstC& I, T dt,
+                                       bool failure, double bfac) {
+  const double mass = p.d(D_MASS);
+  const double vol0 = mass / mc.density, den = mass / m.vol;
+  const double mu = den / mc.density - 1, rv = m.vol / vol0;
+  const double* c = mc.ce;
+  switch (mc.eos) {
+    case 1: {   // linear polynomial
+      const float A = (float)(c[0] + mu * (c[1] + mu * (c[2] + mu * c[3])));
+      const float B = (float)(c[4] + mu * (c[5] + mu * c[6]));
+      hieupd(m, failure);
+      const double specen = m.iener / vol0;
+      float pnew = (float)((A + B * specen) / (1 + B * m.dvol / vol0));
+      if (failure && pnew < 0.0f) pnew = 0.0f;
+      m.ieinc = (T)((double)m.ieinc - m.dvol * pnew);
+      m.iener = m.iener - m.dvol * pnew;
+      m.sm = (T)(-(double)pnew);
+      break;
+    }
+    case 2: {   // Mie-Gruneisen; mu**2 = mu*mu, mu**3 = (mu*mu)*mu
+      float A, B;
+      if (mu > 0) {
+        A = (float)((c[0] * mu + c[1] * (mu * mu) + c[2] * ((mu * mu) * mu)) * (1 - c[3] * mu * 0.5 / den));
+        B = (float)c[4];
+      } else {
+        A = (float)(c[0] * mu);
+        B = 0.0f;
+      }
+      if (m.sm < (T)0) bulkq(m, I, dt);
+      hieupd(m, failure);
+      const double specen = m.iener / vol0;
+      float pnew = (float)((A + B * specen) / (1 + B * m.dvol / vol0));
+      if (failure && pnew < 0.0f) pnew = 0.0f;
+      m.ieinc = (T)((double)m.ieinc - m.dvol * pnew);
+      m.iener = m.iener - m.dvol * pnew;
+      m.sm = (T)(-(double)pnew);
+      break;
+    }
+    case 3: {   // JWL
+      const double r1v = c[2] * rv, r2v = c[3] * rv;
+      const double wr1v = c[0] * c[4] / r1v, wr2v = c[1] * c[4] / r2v;
+      const double wdr1v = c[0] - wr1v, wdr2v = c[1] - wr2v;
+      double A = wdr1v * m_exp<T, MM>(-r1v) + wdr2v * m_exp<T, MM>(-r2v) + (double)m.bqf;
+      double B = c[4] / rv;
+      hieupd(m, failure);
+      const double specen = m.iener / vol0;
+      A = A * bfac;
+      B = B * bfac;
+      double pnew = (A + B * specen) / (1 + B * m.dvol / vol0);
+      if (failure && pnew < 0) pnew = 0.0;
+      m.iener = m.iener - m.dvol * pnew;
+      m.sm = (T)(-pnew);
+      break;
+    }
+    default: {   // 4: weakly compressible
+      if (m.sm < (T)0) bulkq(m, I, dt);
+      hieupd(m, failure);
+      double pnew = mc.c0 * mc.c0 * (den - mc.density);
+      if (failure && pnew < 0) pnew = 0.0;
+      m.iener = m.iener - m.dvol * pnew;
+      m.sm = (T)(-pnew);
+      break;
+    }
+  }
 }
 
 __device__ __forceinline__ double det3(const double F[3][3]) {
@@ -1753,15 +1872,29 @@ __device__ __forceinline__ double det3(const double F[3][3]) {
          F[1][0] * F[0][2] * F[2][1] + F[2][0] * F[0][1] * F[1][2] - F[2][0] * F[0][2] * F[1][1];
 }
 
+// M3DM7, the null material's viscous deviator (Constitution.f90:1124-1142)
+template <typename T>
+__device__ __forceinline__ void m3dm7(Cst<T>& m, const MatC& mc, T dt) {
+  const T miu2 = (T)mc.miu2;
+  const T tri = (T)0 * (m.dinc[0] / dt + m.dinc[1] / dt + m.dinc[2] / dt);   // 1/3 is integer 0
+  m.sd[0] = miu2 * (m.dinc[0] / dt - tri);
+  m.sd[1] = miu2 * (m.dinc[1] / dt - tri);
+  m.sd[2] = miu2 * (m.dinc[2] / dt - tri);
+  m.sd[3] = (T)0.5 * miu2 * m.dinc[3] / dt;
+  m.sd[4] = (T)0.5 * miu2 * m.dinc[4] / dt;
+  m.sd[5] = (T)0.5 * miu2 * m.dinc[5] / dt;
+}
+
 // Constitution (Constitution.f90:79-368) for the models on the path:
-// elas (M3DM1), pla1 (M3DM2), pla2 (M3DM3), null+EOS4 (M3DM7/eos4),
-// Drucker-Prager (M3DM9), neo-Hookean (M3DM10).
+// elas (M3DM1), pla1 (M3DM2), pla2 (M3DM3), sjc and sjcf (M3DM5 + EOS),
+// HE burn (M3DM6 + JWL), null (M3DM7 + EOS), Drucker-Prager (M3DM9),
+// neo-Hookean (M3DM10).
 // material models compiled into a stress kernel: bit (1 << type); a launch
 // over a group whose instances use one model gets the kernel with only that
 // model (fewer registers, higher occupancy), mixed groups the general one
 template <typename T, int MM = MM_ALL>
-__device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, const MatC& mc, T dt,
-                                             const T de[6], const T vort[3], const double F[3][3],
+__device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, const MatC& mc, T dt, double dtd,
+                                             double time, const T de[6], const T vort[3], const double F[3][3],
                                              const double deF[3][3], Cst<T>& m) {
 #pragma unroll
   for (int k = 0; k < 6; k++) m.dinc[k] = de[k];
@@ -1842,23 +1975,45 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
       lieupd(m, I, dt);
       break;
     }
-    case MPM_MAT_NULL: {
-      const T miu2 = (T)mc.miu2;
-      const T tri = (T)0 * (m.dinc[0] / dt + m.dinc[1] / dt + m.dinc[2] / dt);   // 1/3 is integer 0
-      m.sd[0] = miu2 * (m.dinc[0] / dt - tri);
-      m.sd[1] = miu2 * (m.dinc[1] / dt - tri);
-      m.sd[2] = miu2 * (m.dinc[2] / dt - tri);
-      m.sd[3] = (T)0.5 * miu2 * m.dinc[3] / dt;
-      m.sd[4] = (T)0.5 * miu2 * m.dinc[4] / dt;
-      m.sd[5] = (T)0.5 * miu2 * m.dinc[5] / dt;
-      // eos4 (:1513-1541)
-      if (m.sm < (T)0) bulkq(m, I, dt);
-      hieupd(m);
-      const double pnew = mc.c0 * mc.c0 * (p.d(D_MASS) / m.vol - mc.density);
-      m.iener = m.iener - m.dvol * pnew;
-      m.sm = (T)(-pnew);
+    case MPM_MAT_NULL:
+      m3dm7(m, mc, dt);
+      seleos<T, MM>(m, p, mc, I, dt, false, 0.0);
+      break;
+    case MPM_MAT_SJC:
+      sigrot(m, vort);
+      m3dm5<T, MM>(m, mc, G2, dtd);
+      seleos<T, MM>(m, p, mc, I, dt, false, 0.0);
+      break;
+    case MPM_MAT_SJCF: {   // Constitution.f90:245-276
+      sigrot(m, vort);
+      bool failure = p.d(D_FAIL) != 0.0;
+      if (!failure) {
+        m3dm5<T, MM>(m, mc, G2, dtd);
+        seleos<T, MM>(m, p, mc, I, dt, false, 0.0);
+        if ((double)m.epeff > mc.epf) {
+          m.epeff = (T)(mc.epf + (double)0.0000001f);
+          failure = true;
+        }
+        if (failure && m.sm < (T)0) {
+          m.sm = (T)0;
+          m.bqf = (T)0;
+          p.d(D_VOL) = m.vold;
+        }
+        if (failure) {
+#pragma unroll
+          for (int k = 0; k < 6; k++) m.sd[k] = (T)0;
+          m.iener = m.iener - (double)m.ieinc + m.dvol * (double)(m.smold + m.sm - (T)2 * m.bqf);
+          p.d(D_FAIL) = 1.0;
+        }
+      } else {   // a failed particle flows as the null material (M3DM7)
+        m3dm7(m, mc, dt);
+        seleos<T, MM>(m, p, mc, I, dt, true, 0.0);
+      }
       break;
     }
+    case MPM_MAT_HIEX:   // M3DM6 (:1103-1121): burnt once the time passes the lighting time
+      seleos<T, MM>(m, p, mc, I, dt, p.d(D_FAIL) != 0.0, time > p.d(D_LT) ? 1.0 : 0.0);
+      break;
     case MPM_MAT_DP: {   // M3DM9 (:1214-1330)
       sigrot(m, vort);
       const T Gmod = (T)mc.Gmod, Kmod = (T)mc.Kmod;
@@ -1915,9 +2070,8 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
       B[5] = F[0][0] * F[2][0] + F[0][1] * F[2][1] + F[0][2] * F[2][2];
       m.ieinc = sig_dot(m);
       const double iJ = 1.0 / J;
-      // ln J: the correctly rounded logarithm in the reference-order mode
-      // (reproducible on any libm), the device libm's otherwise
-      const double lnJ = (MM & MM_CRLOG) ? mpm_log_cr(J) : log(J);
+      // ln J: the correctly rounded logarithm in the FP64 paths (see m_log)
+      const double lnJ = m_log<T, MM>(J);
       const double lj = mc.lamda * lnJ * 1.0;
       double s[6];
       s[0] = iJ * (G * B[0] + lj);
@@ -1946,7 +2100,8 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
   p.t(T_SEQV) = m.seqv;
 #pragma unroll
   for (int k = 0; k < 6; k++) p.t(T_SD0 + k) = m.sd[k];
-  if (mm_has(MM, MPM_MAT_PLA2)) p.t(T_SIGY) = m.sigy;   // only pla2 hardens; cp never changes
+  // only pla2 and the Johnson-Cook models harden; cp never changes
+  if (mm_has(MM, MPM_MAT_PLA2) || mm_has(MM, MPM_MAT_SJC) || mm_has(MM, MPM_MAT_SJCF)) p.t(T_SIGY) = m.sigy;
   p.t(T_EPEFF) = m.epeff;
   p.d(D_IE) = m.iener;
   p.t(T_Q) = m.bqf;
@@ -1990,7 +2145,7 @@ __device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p
         F[a][a] = F[a][a] + 1.0;
       }
     }
-    constitution<T, MM>(p, I, mc, dt, de, vort, F, deF, m);
+    constitution<T, MM>(p, I, mc, dt, dtd, ts.time, de, vort, F, deF, m);
     // counted per instance: a negative volume (the reference writes a
     // warning and goes on, Constitution.f90:135-137) and a non-finite volume
     // or stress (NaN or inf, which the reference would carry on silently)
@@ -2775,6 +2930,19 @@ __global__ void __launch_bounds__(256) k_energy(PoolPtrs<T> P, const int* glist,
   block_energy(P.part, gran, blockIdx.x % bpg, ev);
 }
 
+// the velocities of the particles of instances that k_finalize damped
+// (dynamicRelaxationDamping, Particle.f90:190-197)
+template <typename T>
+__global__ void __launch_bounds__(256) k_damp(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
+  int gran;
+  const long i = part_index(P.ilv, glist, bpg, gran);
+  const int ii = ginst[gran];
+  if (!P.ts[ii].damped || i >= P.inst[ii].p_end) return;
+  P.t[T_VX * P.np + i] = (T)0;
+  P.t[T_VY * P.np + i] = (T)0;
+  P.t[T_VZ * P.np + i] = (T)0;
+}
+
 // calcEnergy / UpdateDT finalisation: one block per active instance sums
 // its block partials in a fixed order; DT from max |v| (update_step.f90:1845-1872);
 // optional end-of-run test (MPM3D.f90:152-158).
@@ -2822,6 +2990,10 @@ __global__ void __launch_bounds__(256) k_finalize(PoolPtrs<T> P, const int* ilis
       dt = fmin(dt, I.dcell / vv);
     }
     ts.dt = dt * I.dtscale;
+    if (I.flags & IF_DRDA) {   // dynamicRelaxationDamping (Particle.f90:172-202), after UpdateDT
+      ts.damped = ts.ek - ts.ek_prev <= 0.0;
+      if (ts.damped) ts.ek = 0.0;   // k_damp zeroes the velocities
+    }
   }
   if ((what & FIN_DONE) && (I.flags & IF_ENDTIME)) {   // MPM3D.f90:152-158
     bool stop;
@@ -3003,6 +3175,10 @@ __global__ void __launch_bounds__(256) k_admit(PoolPtrs<T> P, const double* ps, 
 #pragma unroll
   for (int k = 0; k < 9; k++) d[(D_F0 + k) * np + s] = F(PF_FP + k);
   d[D_MASS * np + s] = F(PF_MASS);
+  d[D_TEMP * np + s] = F(PF_TEMP);
+  d[D_DMG * np + s] = F(PF_DMG);
+  d[D_LT * np + s] = F(PF_LT);
+  d[D_FAIL * np + s] = F(PF_FAIL);
   P.i[I_BODY * np + s] = in ? body0 + pbody[j] : body0;
   P.i[I_ORIG * np + s] = (int)j;
   P.i[I_CELL * np + s] = -1;
@@ -3298,6 +3474,11 @@ void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int what
   if (ninst) hipLaunchKernelGGL(k_finalize<T>, dim3(ninst), dim3(256), 0, s, P, ilist, what);
 }
 
+template <typename T>
+void launch_damp(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
+  if (L.nblk) hipLaunchKernelGGL(k_damp<T>, dim3(L.nblk), dim3(L.bs), 0, s, P, L.glist, L.ginst, L.bpg);
+}
+
 template <typename S>
 void launch_permute(S* dst, const S* src, const int* perm, const LaunchP& L, long np, int nfield, hipStream_t s) {
   if (L.nblk) hipLaunchKernelGGL(k_permute<S>, dim3(L.nblk), dim3(L.bs), 0, s, dst, src, perm, L.glist, L.bpg, np, nfield);
@@ -3425,6 +3606,7 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
   template void launch_zero_grid<T>(const PoolPtrs<T>&, bool, const LaunchP&, hipStream_t);        \
   template void launch_energy<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                 \
   template void launch_finalize<T>(const PoolPtrs<T>&, const int*, int, int, hipStream_t);         \
+  template void launch_damp<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                    \
   template void launch_permute<T>(T*, const T*, const int*, const LaunchP&, long, int, hipStream_t);  \
   template void launch_sgmp_p2g<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);         \
   template void launch_sgmp_nodes<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);       \

@@ -18,7 +18,8 @@ module mpm_hip
                         PF_FX = 6, PF_FY = 7, PF_FZ = 8, PF_MASS = 9, PF_VOL = 10, PF_SM = 11, &
                         PF_SEQV = 12, PF_SDXX = 13, PF_SDYY = 14, PF_SDZZ = 15, PF_SDYZ = 16, &
                         PF_SDXZ = 17, PF_SDXY = 18, PF_SIGY = 19, PF_EPEFF = 20, PF_IE = 21, &
-                        PF_Q = 22, PF_CP = 23, PF_FP = 24, PF_XX = 33, PF_COUNT = 36
+                        PF_Q = 22, PF_CP = 23, PF_FP = 24, PF_XX = 33, PF_TEMP = 36, PF_DMG = 37, &
+                        PF_LT = 38, PF_FAIL = 39, PF_COUNT = 40
 
   type, bind(C) :: mpm_material_t
      integer(c_int32_t) :: mtype, eos_type
@@ -26,6 +27,7 @@ module mpm_hip
      real(c_double) :: q_fai, k_fai, q_psi, ten_f
      real(c_double) :: ceos(10)
      real(c_double) :: det_speed
+     real(c_double) :: b_jc, n_jc, c_jc, m_jc, epso, epf, roomt, melt, spec_heat
   end type mpm_material_t
 
   type, bind(C) :: mpm_body_t
@@ -37,7 +39,7 @@ module mpm_hip
      integer(c_int32_t) :: nb_particle, nb_body, nb_mat, shape, scheme
      integer(c_int32_t) :: quasi_load, quasi_damp, gravity, update_v_by_fp
      integer(c_int32_t) :: fix(6)
-     integer(c_int32_t) :: ncx, ncy, ncz, pad0
+     integer(c_int32_t) :: ncx, ncy, ncz, dr_damping
      real(c_double) :: span(6), dcell, cutoff, dt, dtscale, endtime
      real(c_double) :: quasi_load_time, damp_factor, bq1, bq2
      type(c_ptr) :: mats, bodies, pstate, pbody

@@ -81,7 +81,8 @@ contains
     prob%gravity = merge(1, 0, Gravity)
     prob%update_v_by_fp = merge(1, 0, UpdateVbyFp)
     prob%fix = FixS
-    prob%ncx = NumCellx; prob%ncy = NumCelly; prob%ncz = NumCellz; prob%pad0 = 0
+    prob%ncx = NumCellx; prob%ncy = NumCelly; prob%ncz = NumCellz
+    prob%dr_damping = merge(1, 0, DR_DAMPING)
     prob%span = [SpanX(1), SpanX(2), SpanY(1), SpanY(2), SpanZ(1), SpanZ(2)]
     prob%dcell = DCell
     prob%cutoff = CutOff
@@ -109,12 +110,23 @@ contains
        mats(m)%q_psi = mat_list(m)%q_psi
        mats(m)%ten_f = mat_list(m)%ten_f
        mats(m)%ceos = mat_list(m)%cEos
+       mats(m)%b_jc = mat_list(m)%B_jc
+       mats(m)%n_jc = mat_list(m)%n_jc
+       mats(m)%c_jc = mat_list(m)%C_jc
+       mats(m)%m_jc = mat_list(m)%m_jc
+       mats(m)%epso = mat_list(m)%epso
+       mats(m)%epf = mat_list(m)%epf
+       mats(m)%roomt = mat_list(m)%roomt
+       mats(m)%melt = mat_list(m)%Melt
+       mats(m)%spec_heat = mat_list(m)%SpecHeat
        ! SetDT/UpdateDT take max(c, D, Wavespd) (DataIn.f90:1289-1290), but the
-       ! input phase sets D only for HE burn and Wavespd only for the null
-       ! material; otherwise the reference reads whatever the allocation held.
-       ! Those undefined values are replaced by 0 here.
+       ! input phase sets D only for HE burn and Wavespd and miu only for the
+       ! null material; otherwise the reference reads whatever the allocation
+       ! held.  Those undefined values are replaced by 0 here.
        mats(m)%det_speed = 0d0
+       if (mat_list(m)%MatType == 8) mats(m)%det_speed = mat_list(m)%D
        if (mat_list(m)%MatType /= 9) mats(m)%wavespd = 0d0
+       if (mat_list(m)%MatType /= 9) mats(m)%miu = 0d0
        ! the null material reads no Young's modulus or Poisson ratio
        ! (DataIn.f90:486-491), yet SetDT/UpdateDT evaluate the elastic wave
        ! speed from them: 0 here, so its wave speed is Wavespd
@@ -183,6 +195,10 @@ contains
             end do
          end do
          ps((PF_XX + 0) * n + i) = p%XX(1); ps((PF_XX + 1) * n + i) = p%XX(2); ps((PF_XX + 2) * n + i) = p%XX(3)
+         ps(PF_TEMP * n + i) = p%celsius_t
+         ps(PF_DMG * n + i) = p%DMG
+         ps(PF_LT * n + i) = p%LT
+         ps(PF_FAIL * n + i) = merge(1d0, 0d0, p%failure)
        end associate
     end do
   end subroutine pack_particles
@@ -213,6 +229,9 @@ contains
                p%Fp(a, c) = ps((PF_FP + 3 * (a - 1) + (c - 1)) * n + i)
             end do
          end do
+         p%celsius_t = ps(PF_TEMP * n + i)
+         p%DMG = ps(PF_DMG * n + i)
+         p%failure = ps(PF_FAIL * n + i) /= 0d0
        end associate
     end do
   end subroutine unpack_particles
@@ -221,14 +240,14 @@ contains
   subroutine refresh_host_view()
     use ParticleData
     use DataOut, only: nCurves, CurvePoint, WriteTecPlot, WriteParaView, OutTime
-    real(c_double) :: ek, ei, mom(3)
+    real(c_double) :: sc(8)
     character(len=8) :: f
     integer :: q, k, n, a, c
-    ! calcEnergy on the device (Particle.f90:204-230)
-    call mpm_check(mpm_calc_energy(ctx, ek, ei, mom), 'mpm_calc_energy')
-    EngKinetic = ek
-    EngInternal = ei
-    Momentum = mom
+    ! the device's calcEnergy (after its damping, if any)
+    call mpm_check(mpm_get_scalars(ctx, sc), 'mpm_get_scalars')
+    EngKinetic = sc(4)
+    EngInternal = sc(5)
+    Momentum = sc(6:8)
     if (.not. plt_started) then
        plt_mirror = plt_mirror + OutTime
        plt_started = .true.
@@ -269,6 +288,9 @@ contains
                p%Fp(a, c) = sub((PF_FP + 3 * (a - 1) + (c - 1)) * k + q)
             end do
          end do
+         p%celsius_t = sub(PF_TEMP * k + q)
+         p%DMG = sub(PF_DMG * k + q)
+         p%failure = sub(PF_FAIL * k + q) /= 0d0
        end associate
     end do
   end subroutine refresh_host_view
@@ -342,6 +364,11 @@ subroutine UpdateDT()
   real(c_double) :: dt_dev
   integer(c_int) :: rc
   logical, save :: warned = .false.
+  real(c_double) :: ek, ei, mom(3)
+  ! calcEnergy (Particle.f90:204-230) ahead of UpdateDT, as the driver calls
+  ! them: the driver's own calcEnergy ran on the host view; the device's
+  ! energies are the ones dynamicRelaxationDamping (drda) compares
+  call mpm_check(mpm_calc_energy(ctx, ek, ei, mom), 'mpm_calc_energy')
   rc = mpm_update_dt(ctx, dt_dev)
   if (rc == MPM_ERR_NUMERIC) then
      ! a non-finite volume or stress: warned once, the run goes on as the

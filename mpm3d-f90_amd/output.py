@@ -26,11 +26,12 @@ end time writing every file; ``ResultWriter`` is the writer alone, for hosts
 with their own loop.  Particle state comes from the device: per step only
 the curve particles (mpm_download_particle_subset), everything at frames.
 
-Fields the built materials never change print their initial values, as the
-reference's do: temperature 293 (celsius_t, Particle.f90:131), damage and
-smoothed pressure 0, no failed or skipped particles.  One difference: a curve
-of option 12 (fail) writes the logical with an I edit descriptor, which the
-flang-built reference refuses at run time; here it prints 0.
+Temperature, damage and the failure flag are particle state (PF temp, dmg,
+fail); the smoothed pressure (smoo) is not computed and prints 0.  TecPlot
+frames leave out the particles pt2d marks (SkipThis, DataIn.f90:867-880).
+One difference: a curve of option 12 (fail) writes the logical with an I
+edit descriptor, which the flang-built reference refuses at run time; here it
+prints the flag as 0 or 1.
 """
 import ctypes as C
 import math
@@ -44,7 +45,6 @@ from ._lib import PF, Outputs
 # DataOut.f90:30-34
 OUTPUT_NAMES = ("seqv", "epef", "mat ", "pres", "volu", "engk", "engi", "velx", "vely", "velz",
                 "cels", "fail", "sspd", "damg", "stsx", "disx", "disy", "disz")
-CELSIUS0 = 293.0      # Particle.f90:131
 INDENT = "    "
 
 
@@ -125,6 +125,13 @@ class ResultWriter:
             if not 1 <= opt <= len(OUTPUT_NAMES) or not 1 <= pt <= problem.n:
                 raise ValueError("curve option %d of particle %d out of range" % (opt, pt))
         self.xo = problem.pstate[[PF["x"], PF["y"], PF["z"]]].copy()   # Xo (DataIn.f90:859-863)
+        # SkipThis (setskip, DataIn.f90:867-880): outside the pt2d box at generation
+        self.skip = np.zeros(problem.n, dtype=bool)
+        if outputs.plot2d:
+            b = np.array(outputs.plot2d_box[:])
+            x = self.xo[:, outputs.skip_first:]
+            self.skip[outputs.skip_first:] = ((x[0] < b[0]) | (x[0] > b[1]) | (x[1] < b[2]) | (x[1] > b[3])
+                                              | (x[2] < b[4]) | (x[2] > b[5]))
         self.mat = np.zeros(problem.n, dtype=np.int64)
         for b in problem.bodies:
             self.mat[b.par_begin:b.par_end] = b.mat
@@ -190,11 +197,11 @@ class ResultWriter:
     def _curve_value(self, opt, p, xo, ek, ei):
         # DataOut.f90:94-137
         if opt == 12:
-            return fi(0, 12)
+            return fi(int(p[PF["fail"]] != 0), 12)
         v = {1: lambda: p[PF["seqv"]], 2: lambda: p[PF["epeff"]], 3: lambda: p[PF["epeff"]],
              4: lambda: -p[PF["sm"]], 5: lambda: p[PF["vol"]], 6: lambda: ek, 7: lambda: ei,
-             8: lambda: p[PF["vx"]], 9: lambda: p[PF["vy"]], 10: lambda: p[PF["vz"]], 11: lambda: CELSIUS0,
-             13: lambda: p[PF["sdzz"]], 14: lambda: 0.0, 15: lambda: p[PF["sdxx"]] + p[PF["sm"]],
+             8: lambda: p[PF["vx"]], 9: lambda: p[PF["vy"]], 10: lambda: p[PF["vz"]], 11: lambda: p[PF["temp"]],
+             13: lambda: p[PF["sdzz"]], 14: lambda: p[PF["dmg"]], 15: lambda: p[PF["sdxx"]] + p[PF["sm"]],
              16: lambda: p[PF["x"]] - xo[0], 17: lambda: p[PF["y"]] - xo[1], 18: lambda: p[PF["z"]] - xo[2]}
         return fe(v[opt](), 12, 4)
 
@@ -220,7 +227,7 @@ class ResultWriter:
             out.write('VARIABLES= "X"   "Y"   "Z"  ' + "".join(fa(s, widths[k % 3]) for k, s in enumerate(items))
                       + "\n")
         out.write('ZONE T="time' + fe(time, 10, 3) + '"\n')
-        keep = np.ones(self.problem.n, dtype=bool) if icell is None else np.asarray(icell) >= 0
+        keep = ~self.skip if icell is None else (np.asarray(icell) >= 0) & ~self.skip
         cols = []
         for a in self.anim:
             cols.append(self._anim_column(a, ps))
@@ -238,12 +245,13 @@ class ResultWriter:
         if a == 3:
             return lambda p: fi(self.mat[p], 3)
         if a == 12:
-            return lambda p: fi(0, 3)
+            fail = ps[PF["fail"]] != 0
+            return lambda p: fi(int(fail[p]), 3)
         arr = {1: lambda: g("seqv"), 2: lambda: g("epeff"), 4: lambda: -g("sm"), 5: lambda: g("vol"),
                6: lambda: 0.5 * g("mass") * (g("vx") ** 2 + g("vy") ** 2 + g("vz") ** 2),
                7: lambda: g("ie"), 8: lambda: g("vx"), 9: lambda: g("vy"), 10: lambda: g("vz"),
-               11: lambda: np.full(self.problem.n, CELSIUS0), 13: lambda: g("cp"),
-               14: lambda: np.zeros(self.problem.n), 15: lambda: g("sdxx") + g("sm"),
+               11: lambda: g("temp"), 13: lambda: g("cp"),
+               14: lambda: g("dmg"), 15: lambda: g("sdxx") + g("sm"),
                16: lambda: g("x") - self.xo[0]}.get(a)
         if arr is None:
             return None
@@ -270,7 +278,7 @@ class ResultWriter:
                   ("MeanNormalStress", g("sm")), ("SmoothMeanNormalStress", -zero), ("vx", g("vx")),
                   ("vy", g("vy")), ("vz", g("vz")), ("disx", x - self.xo[0]), ("disy", y - self.xo[1]),
                   ("disz", z - self.xo[2]), ("fxpx", g("fx")), ("fxpy", g("fy")), ("fxpz", g("fz")),
-                  ("vol", g("vol")), ("celsius_t", np.full(n, CELSIUS0)), ("damage", zero),
+                  ("vol", g("vol")), ("celsius_t", g("temp")), ("damage", g("dmg")),
                   ("stressx", g("sdxx") + g("sm")), ("disx", x - self.xo[0])]
         for label, col in fields:
             w.append('%s<DataArray type="Float32" Name="%s" Format="ascii">\n' % (i3, label))

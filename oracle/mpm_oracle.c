@@ -973,10 +973,17 @@ typedef struct {
   double dinc[6], vold, vol_, dvol, sm, smold, sd[6], sig[6], sold[6], bqf;
   double seqv, epeff_, sig_y_, depeff, ratio, G2, K3, PlaMod, lamda, miu2;
   double iener, specen, ieinc, den0_, den_, vol0_, cp, yield0_;
+  double mu, rv, bfac;
   double F[3][3];
   int br;   /* M3DM9 branch taken (DP_BR_*), diagnostics only */
   int log_cr;
 } cst_t;
+
+/* log, exp and real powers: the C library's (as the reference's Fortran
+ * runtime calls them), or the correctly rounded ones of crlog.h */
+static double o_log(const cst_t* m, double x) { return m->log_cr ? mpm_log_cr(x) : log(x); }
+static double o_exp(const cst_t* m, double x) { return m->log_cr ? mpm_exp_cr(x) : exp(x); }
+static double o_pow(const cst_t* m, double x, double y) { return m->log_cr ? mpm_pow_cr(x, y) : pow(x, y); }
 
 /* EquivalentStress returns a default REAL (Constitution.f90:679-699) */
 static double equivalent_stress(const cst_t* m) {
@@ -1052,13 +1059,17 @@ static void lieupd(cst_t* m, const oracle* o) {
   m->iener = m->iener + 0.25 * m->ieinc * (double)vavg;
 }
 
-/* hieupd, non-failed branch (Constitution.f90:649-676) */
-static void hieupd(cst_t* m) {
-  float vavg = (float)(m->vol_ + m->vold);
-  double e = m->dinc[0] * (m->sold[0] + m->sd[0]) + m->dinc[1] * (m->sold[1] + m->sd[1]) +
-             m->dinc[2] * (m->sold[2] + m->sd[2]) + m->dinc[3] * (m->sold[3] + m->sd[3]) +
-             m->dinc[4] * (m->sold[4] + m->sd[4]) + m->dinc[5] * (m->sold[5] + m->sd[5]);
-  m->ieinc = 0.25 * e * (double)vavg + m->dvol * (m->smold - 2.0 * m->bqf);
+/* hieupd (Constitution.f90:649-676); vavg is a default REAL */
+static void hieupd(cst_t* m, int failure) {
+  if (failure) {
+    m->ieinc = m->dvol * (m->smold - 2.0 * m->bqf);
+  } else {
+    float vavg = (float)(m->vol_ + m->vold);
+    double e = m->dinc[0] * (m->sold[0] + m->sd[0]) + m->dinc[1] * (m->sold[1] + m->sd[1]) +
+               m->dinc[2] * (m->sold[2] + m->sd[2]) + m->dinc[3] * (m->sold[3] + m->sd[3]) +
+               m->dinc[4] * (m->sold[4] + m->sd[4]) + m->dinc[5] * (m->sold[5] + m->sd[5]);
+    m->ieinc = 0.25 * e * (double)vavg + m->dvol * (m->smold - 2.0 * m->bqf);
+  }
   m->iener = m->iener + m->ieinc;
   m->specen = m->iener / m->vol0_;
 }
@@ -1202,14 +1213,115 @@ static void m3dm10(cst_t* m) {         /* :725-780 neo-Hookean */
   m->ieinc = new_stress_dot_dinc(m, m->ieinc);
 }
 
-/* eos4 (Constitution.f90:1513-1541) via seleos, non-failed particle */
-static void eos4(cst_t* m, const oracle* o, const mpm_material* mat) {
-  double c0 = mat->ceos[0];
+/* M3DM5, simplified Johnson-Cook (Constitution.f90:1043-1100); 0.0001 and
+ * 0.001 are default REAL literals */
+static void m3dm5(cst_t* m, const mpm_material* mat, double DT) {
+  const double Bjc = mat->b_jc, njc = mat->n_jc, Cjc = mat->c_jc, epso = mat->epso;
+  const double e4 = (double)0.0001f, rmin = (double)0.001f;
+  m->epeff_ = m->epeff_ + e4;
+  m->PlaMod = Bjc * njc * o_pow(m, m->epeff_, njc - 1);
+  m->epeff_ = m->epeff_ - e4;
+  elastic_devi(m);
+  m->seqv = equivalent_stress(m);
+  if (m->seqv > m->sig_y_) {
+    m->depeff = (m->seqv - m->sig_y_) / (1.5 * m->G2 + m->PlaMod);
+    m->epeff_ = m->epeff_ + m->depeff;
+    double srate = m->depeff / epso / DT;
+    if (srate < rmin) srate = rmin;
+    m->sig_y_ = (m->yield0_ + Bjc * o_pow(m, m->epeff_, njc)) * (1 + Cjc * o_log(m, srate));
+    if (m->sig_y_ > m->seqv) {
+      m->epeff_ = m->epeff_ - m->depeff;
+      m->depeff = 0.0;
+    } else {
+      m->ratio = m->sig_y_ / m->seqv;
+      for (int i = 0; i < 6; i++) m->sd[i] = m->sd[i] * m->ratio;
+      m->seqv = m->seqv * m->ratio;
+    }
+  }
+}
+
+/* pressure from an EOS's A and B (Eq. 5.18 of the reference's EOS routines),
+ * a default REAL */
+static float eos_pnew(const cst_t* m, double A, double B) {
+  return (float)((A + B * m->specen) / (1 + B * m->dvol / m->vol0_));
+}
+
+/* eos1, linear polynomial (Constitution.f90:1364-1405): constants, A, B and
+ * pnew are default REALs */
+static void eos1(cst_t* m, const mpm_material* mat, int failure) {
+  float c[7];
+  for (int k = 0; k < 7; k++) c[k] = (float)mat->ceos[k];
+  const double mu = m->mu;
+  float A = (float)(c[0] + mu * (c[1] + mu * (c[2] + mu * c[3])));
+  float B = (float)(c[4] + mu * (c[5] + mu * c[6]));
+  hieupd(m, failure);
+  float pnew = eos_pnew(m, A, B);
+  if (failure && pnew < 0) pnew = 0.0f;
+  m->ieinc = m->ieinc - m->dvol * pnew;
+  m->iener = m->iener - m->dvol * pnew;
+  m->sm = -(double)pnew;
+}
+
+/* eos2, Mie-Gruneisen (Constitution.f90:1408-1455): constants, A, B and
+ * pnew are default REALs; mu**2 = mu*mu, mu**3 = (mu*mu)*mu */
+static void eos2(cst_t* m, const oracle* o, const mpm_material* mat, int failure) {
+  const float c1 = (float)mat->ceos[0], c2 = (float)mat->ceos[1], c3 = (float)mat->ceos[2];
+  const float c4 = (float)mat->ceos[3], c5 = (float)mat->ceos[4];
+  const double mu = m->mu;
+  float A, B;
+  if (mu > 0) {
+    A = (float)((c1 * mu + c2 * (mu * mu) + c3 * ((mu * mu) * mu)) * (1 - c4 * mu * 0.5 / m->den_));
+    B = c5;
+  } else {
+    A = (float)(c1 * mu);
+    B = 0.0f;
+  }
   if (m->sm < 0) bulkq(m, o);
-  hieupd(m);
-  double pnew = c0 * c0 * (m->den_ - m->den0_);
+  hieupd(m, failure);
+  float pnew = eos_pnew(m, A, B);
+  if (failure && pnew < 0) pnew = 0.0f;
+  m->ieinc = m->ieinc - m->dvol * pnew;
+  m->iener = m->iener - m->dvol * pnew;
+  m->sm = -(double)pnew;
+}
+
+/* eos3, JWL (Constitution.f90:1458-1511), scaled by the burn fraction */
+static void eos3(cst_t* m, const mpm_material* mat, int failure) {
+  const double c1 = mat->ceos[0], c2 = mat->ceos[1], r1 = mat->ceos[2], r2 = mat->ceos[3], w1 = mat->ceos[4];
+  const double r1v = r1 * m->rv, r2v = r2 * m->rv;
+  const double wr1v = c1 * w1 / r1v, wr2v = c2 * w1 / r2v;
+  const double wdr1v = c1 - wr1v, wdr2v = c2 - wr2v;
+  const double er1v = o_exp(m, -r1v), er2v = o_exp(m, -r2v);
+  double A = wdr1v * er1v + wdr2v * er2v + m->bqf;
+  double B = w1 / m->rv;
+  hieupd(m, failure);
+  A = A * m->bfac;
+  B = B * m->bfac;
+  double pnew = (A + B * m->specen) / (1 + B * m->dvol / m->vol0_);
+  if (failure && pnew < 0) pnew = 0.0;
   m->iener = m->iener - m->dvol * pnew;
   m->sm = -pnew;
+}
+
+/* eos4, weakly compressible (Constitution.f90:1513-1541) */
+static void eos4(cst_t* m, const oracle* o, const mpm_material* mat, int failure) {
+  double c0 = mat->ceos[0];
+  if (m->sm < 0) bulkq(m, o);
+  hieupd(m, failure);
+  double pnew = c0 * c0 * (m->den_ - m->den0_);
+  if (failure && pnew < 0) pnew = 0.0;
+  m->iener = m->iener - m->dvol * pnew;
+  m->sm = -pnew;
+}
+
+/* seleos (Constitution.f90:1333-1361) */
+static void seleos(cst_t* m, const oracle* o, const mpm_material* mat, int failure) {
+  switch (mat->eos_type) {
+    case 1: eos1(m, mat, failure); break;
+    case 2: eos2(m, o, mat, failure); break;
+    case 3: eos3(m, mat, failure); break;
+    default: eos4(m, o, mat, failure); break;
+  }
 }
 
 /* Constitution (Constitution.f90:79-368) */
@@ -1251,7 +1363,10 @@ static void constitution(oracle* o, int p, const double de[6], const double vort
   m.vol0_ = mp / m.den0_;
   m.den_ = mp / m.vol_;
   m.specen = m.iener / m.vol0_;
-  const double DMG = 0.0;
+  m.mu = m.den_ / m.den0_ - 1;
+  m.rv = m.vol_ / m.vol0_;
+  const double DMG = P(o, PF_DMG, p);
+  int failure = P(o, PF_FAIL, p) != 0.0;
   young = young * (1.0 - DMG + 0x1p-52);                 /* epsilon(1.0d0) */
   m.G2 = young / (1 + poisson);
   m.K3 = young / (1 - 2 * poisson);
@@ -1278,7 +1393,36 @@ static void constitution(oracle* o, int p, const double de[6], const double vort
     case MPM_MAT_ELASTIC: sigrot(&m, vort); m3dm1(&m); lieupd(&m, o); break;
     case MPM_MAT_PLA1: sigrot(&m, vort); m3dm2(&m); lieupd(&m, o); break;
     case MPM_MAT_PLA2: sigrot(&m, vort); m3dm3(&m); lieupd(&m, o); break;
-    case MPM_MAT_NULL: m3dm7(&m, o->dt); eos4(&m, o, mat); break;
+    case MPM_MAT_SJC: sigrot(&m, vort); m3dm5(&m, mat, o->dt); seleos(&m, o, mat, failure); break;
+    case MPM_MAT_SJCF:   /* Constitution.f90:245-276 */
+      sigrot(&m, vort);
+      if (!failure) {
+        m3dm5(&m, mat, o->dt);
+        seleos(&m, o, mat, failure);
+        if (m.epeff_ > mat->epf) {
+          m.epeff_ = mat->epf + (double)0.0000001f;
+          failure = 1;
+        }
+        if (failure && m.sm < 0) {
+          m.sm = 0.0;
+          m.bqf = 0.0;
+          P(o, PF_VOL, p) = m.vold;
+        }
+        if (failure) {
+          for (int i = 0; i < 6; i++) m.sd[i] = 0.0;
+          m.iener = m.iener - m.ieinc + m.dvol * (m.smold + m.sm - 2 * m.bqf);
+        }
+        P(o, PF_FAIL, p) = failure;
+      } else {
+        m3dm7(&m, o->dt);
+        seleos(&m, o, mat, failure);
+      }
+      break;
+    case MPM_MAT_HIEX:   /* M3DM6 (:1103-1121): burnt once the time passes the lighting time */
+      m.bfac = o->time > P(o, PF_LT, p) ? 1.0 : 0.0;
+      seleos(&m, o, mat, failure);
+      break;
+    case MPM_MAT_NULL: m3dm7(&m, o->dt); seleos(&m, o, mat, failure); break;
     case MPM_MAT_DP:
       sigrot(&m, vort);
       m3dm9(&m, mat, young, poisson);
@@ -1384,6 +1528,18 @@ void oracle_calc_energy(oracle* o) {
   }
 }
 
+/* dynamicRelaxationDamping (Particle.f90:172-202), after UpdateDT with drda (run
+ * at the end of oracle_update_dt):
+ * a step whose kinetic energy did not grow stops every particle */
+static void dr_damping(oracle* o) {
+  if (!o->prob.dr_damping) return;
+  double delta = o->ek - o->ek_prev;
+  if (delta <= 0.0) {
+    for (int p = 0; p < o->n; p++) P(o, PF_VX, p) = P(o, PF_VY, p) = P(o, PF_VZ, p) = 0.0;
+    o->ek = 0.0;
+  }
+}
+
 void oracle_update_dt(oracle* o) {
   double maxv = 0.0;
   for (int p = 0; p < o->n; p++) {
@@ -1406,6 +1562,7 @@ void oracle_update_dt(oracle* o) {
     dt = fmin(dt, o->prob.dcell / vv);
   }
   o->dt = dt * o->prob.dtscale;
+  dr_damping(o);
 }
 
 void oracle_step(oracle* o, int nsteps) {
@@ -1438,6 +1595,8 @@ void oracle_get_particles(const oracle* o, double* out) {
 void oracle_set_log_cr(oracle* o, int on) { o->log_cr = on != 0; }
 
 double oracle_log_cr(double x) { return mpm_log_cr(x); }
+double oracle_exp_cr(double x) { return mpm_exp_cr(x); }
+double oracle_pow_cr(double x, double y) { return mpm_pow_cr(x, y); }
 
 void oracle_get_dp_branches(const oracle* o, int8_t* last, long* counts4) {
   if (last) memcpy(last, o->dpbr, (size_t)o->n);

@@ -56,6 +56,8 @@ void oracle_set_particles(oracle* o, const double* in);
  * that logarithm (test entry) */
 void oracle_set_log_cr(oracle* o, int on);
 double oracle_log_cr(double x);
+double oracle_exp_cr(double x);
+double oracle_pow_cr(double x, double y);
 
 /* Diagnostics of the Drucker-Prager return map (M3DM9, Constitution.f90:
  * 1263-1321), not reference state: the branch each particle took in its

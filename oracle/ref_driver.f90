@@ -7,7 +7,9 @@
 ! [USF: ApplyBoundaryConditions, ParticleStressUpdate], GridMomentumUpdate,
 ! IntegrateMomentum, ParticlePositionUpdate, [MUSL: GridMomentumMUSL,
 ! ApplyBoundaryConditions], [USL/MUSL: ParticleStressUpdate], calcEnergy,
-! UpdateDT) without the per-step file output, and writes the complete
+! UpdateDT, [drda: dynamicRelaxationDamping]) without the per-step file
+! output (and without SmoothStressbyGrid, whose smoothed pressure only the
+! writers read), and writes the complete
 ! particle and grid state as little-endian binary after chosen steps.
 !
 ! usage:  ref_driver <deck> <nsteps> <outprefix> <dumpsteps> [phasestep]
@@ -103,6 +105,7 @@ program ref_driver
      end if
      call calcEnergy()
      call UpdateDT()
+     if (DR_DAMPING) call dynamicRelaxationDamping()
      if (dophase) call dump_state(istep, 8)
 
      do n = 1, ndump
@@ -130,7 +133,7 @@ contains
 
     hdr = 0
     hdr(1) = 1146113101          ! 'MPMD' tag
-    hdr(2) = 1                   ! layout version
+    hdr(2) = 2                   ! layout version
     hdr(3) = nb_particle
     hdr(4) = nb_gridnode
     hdr(5) = istep
@@ -144,7 +147,7 @@ contains
     rh(14) = SpanZ(1); rh(15) = SpanZ(2); rh(16) = DTScale
     write(u) hdr, rh
 
-    ! particle arrays, one field after another (see oracle/refdump.py)
+    ! particle arrays, one field after another (oracle/pyoracle.py: read_ref_dump)
     do j = 1, 3
        write(u) (particle_list(i)%Xp(j), i = 1, nb_particle)
     end do
@@ -172,6 +175,10 @@ contains
     write(u) (particle_list(i)%ie, i = 1, nb_particle)
     write(u) (particle_list(i)%q, i = 1, nb_particle)
     write(u) (particle_list(i)%cp, i = 1, nb_particle)
+    write(u) (particle_list(i)%celsius_t, i = 1, nb_particle)
+    write(u) (particle_list(i)%DMG, i = 1, nb_particle)
+    write(u) (particle_list(i)%LT, i = 1, nb_particle)
+    write(u) (merge(1d0, 0d0, particle_list(i)%failure), i = 1, nb_particle)
     do j = 1, 3          ! Fp(i,j) column-major: (1,1),(2,1),(3,1),(1,2),...
        do n = 1, 3
           write(u) (particle_list(i)%Fp(n, j), i = 1, nb_particle)

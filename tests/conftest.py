@@ -38,7 +38,13 @@ DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
          "cube_elastic_linear_usf", "bar_j2_gimp_usf", "slope_dp_bspline_musl", "slope_dp_bspline_usl",
          # Drucker-Prager blocks that yield on every branch of the return map
          # within 10 steps (sheared and squeezed: USF; pulled apart: MUSL)
-         "block_dp_bspline_shear_usf", "block_dp_bspline_pull_musl"]
+         "block_dp_bspline_shear_usf", "block_dp_bspline_pull_musl",
+         # the EOS models: simplified Johnson-Cook with the Mie-Gruneisen EOS
+         # (Taylor impact) and the linear polynomial EOS (GIMP), with failure
+         # (B-spline impact), HE burn with the JWL EOS and a detonation point,
+         # and dynamic relaxation damping
+         "taylor_sjc_eos2_musl", "cube_sjc_eos1_gimp_usl", "impact_sjcf_eos2_bspline_usf", "tnt_hiex_jwl_usl",
+         "cube_drda_linear_usf"]
 # auxiliary-grid (SGMP) decks: clamped plate (MUSL), water column with
 # symmetric walls (MUSL), J2 block on a clamped floor (USF)
 SGMP_DECKS = ["plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf"]

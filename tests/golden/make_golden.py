@@ -33,8 +33,10 @@ DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
          "plate_neoh_musl_qulo", "dam_null_eos4_musl",
          "plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf",
          "cube_elastic_linear_usf", "bar_j2_gimp_usf", "slope_dp_bspline_musl", "slope_dp_bspline_usl",
-         "block_dp_bspline_shear_usf", "block_dp_bspline_pull_musl"]
-S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp"]
+         "block_dp_bspline_shear_usf", "block_dp_bspline_pull_musl",
+         "taylor_sjc_eos2_musl", "cube_sjc_eos1_gimp_usl", "impact_sjcf_eos2_bspline_usf", "tnt_hiex_jwl_usl",
+         "cube_drda_linear_usf"]
+S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp", "temp", "lt"]
 
 
 def scal(d):

@@ -1,9 +1,13 @@
-"""The correctly rounded logarithm (mpm3d-f90_amd/csrc/crlog.h) that the
-reference-order FP64 mode uses for the neo-Hookean ln J.
+"""The correctly rounded logarithm, exponential and power
+(mpm3d-f90_amd/csrc/crlog.h) that the reference-order FP64 mode uses for the
+neo-Hookean ln J, the Johnson-Cook powers and rate logarithm and the JWL
+exponentials.
 
-* it equals the 60-digit decimal logarithm rounded to the nearest double on
-  random arguments near 1 (the range det F takes), across [1e-3, 1e3], on
-  powers of two and on subnormal and extreme arguments;
+* each equals the 60-digit decimal value rounded to the nearest double: the
+  logarithm on random arguments near 1 (the range det F takes), across
+  [1e-3, 1e3], on powers of two and on subnormal and extreme arguments; the
+  exponential across [-700, 700]; the power over the Johnson-Cook range, and
+  C99's special values;
 * the oracle run with it stays within 1e-13 of the oracle run with the C
   library's log (the reference's own) on the neo-Hookean decks after 200
   steps.  The two differ because glibc's log is accurate to about half an
@@ -40,6 +44,41 @@ def test_log_cr_is_correctly_rounded(oracle_mod):
     bad = [(x, L.oracle_log_cr(x), cr_ln(x)) for x in _samples() if L.oracle_log_cr(x) != cr_ln(x)]
     assert not bad, bad[:5]
     assert math.isnan(L.oracle_log_cr(float("nan"))) and L.oracle_log_cr(0.0) == -math.inf
+
+
+def cr_exp(x):
+    return float(decimal.Decimal(x).exp())
+
+
+def cr_pow(x, y):
+    return float(decimal.Decimal(x) ** decimal.Decimal(y))
+
+
+def test_exp_cr_is_correctly_rounded(oracle_mod):
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(20261205)
+    xs = list(rng.uniform(-1, 1, 3000)) + list(rng.uniform(-40, 40, 3000)) + list(rng.uniform(-700, 700, 1000))
+    xs += [0.0, -0.0, 1.0, -1.0, 0.5 * math.log(2), -0.5 * math.log(2), 709.0, -700.0, 1e-300, -1e-20]
+    bad = [(x, L.oracle_exp_cr(x), cr_exp(x)) for x in xs if L.oracle_exp_cr(x) != cr_exp(x)]
+    assert not bad, bad[:5]
+    assert math.isnan(L.oracle_exp_cr(float("nan"))) and L.oracle_exp_cr(1000.0) == math.inf
+    assert L.oracle_exp_cr(-1000.0) == 0.0
+
+
+def test_pow_cr_is_correctly_rounded(oracle_mod):
+    """the Johnson-Cook arguments: plastic strains (1e-6..2) to hardening
+    exponents (0.1..1.5, and those minus one), homologous temperatures"""
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(20261206)
+    xs = np.exp(rng.uniform(math.log(1e-6), math.log(2.0), 4000))
+    ys = rng.uniform(-0.9, 1.5, 4000)
+    cases = list(zip(xs, ys)) + [(1e-4, 0.3 - 1), (1e-4 + 1e-7, 0.3), (2.0, 10.0), (10.0, -3.0), (0.5, 0.5)]
+    bad = [(x, y, L.oracle_pow_cr(x, y), cr_pow(x, y)) for x, y in cases if L.oracle_pow_cr(x, y) != cr_pow(x, y)]
+    assert not bad, bad[:5]
+    # C99 special values
+    assert L.oracle_pow_cr(0.0, 0.3) == 0.0 and L.oracle_pow_cr(0.0, -0.7) == math.inf
+    assert L.oracle_pow_cr(3.0, 0.0) == 1.0 and L.oracle_pow_cr(1.0, float("nan")) == 1.0
+    assert math.isnan(L.oracle_pow_cr(-2.0, 0.5)) and L.oracle_pow_cr(-2.0, 3.0) == -8.0
 
 
 def test_libc_log_is_not_always_correctly_rounded():

@@ -11,7 +11,7 @@ import pytest
 
 from conftest import ALL_DECKS as DECKS, GOLDEN_DIR, deck_path, load_package
 
-S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp"]
+S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp", "temp", "lt"]
 
 
 @pytest.mark.parametrize("name", DECKS)
@@ -93,7 +93,6 @@ def test_default_scheme_is_musl_and_flags(tmp_path):
 @pytest.mark.parametrize("bad,msg", [
     ("cont lagr 0.1 1", "contact"),
     ("TLmp", "total-Lagrangian"),
-    ("drda", "dynamic relaxation"),
     ("frob 1 2", "invalid keyword"),
     ("spx a 1", "non-numeric"),
 ])
@@ -105,12 +104,45 @@ def test_unsupported_or_bad_input_is_rejected(tmp_path, bad, msg):
     assert msg in str(e.value)
 
 
-def test_johnson_cook_family_rejected(tmp_path):
+@pytest.mark.parametrize("mat,extra,msg", [
+    # john / jcf read the plastic strain increment another particle left behind
+    (" 1 john 8.93e-3 117e3 0.35 90 292 0.3 0.025 1.0 20 1800 385 1e-3", "", "order"),
+    (" 1 jcf 8.93e-3 117e3 0.35 90 292 0.3 0.025 1.0 20 1800 385 1e-3 2.0", "seos 1 2 3940 1.49 1.96 0", "order"),
+    # models that call seleos need an EOS; JWL only with HE burn
+    (" 1 sjc 8.93e-3 117e3 0.35 157 425 1.0 0.0 1e-3", "", "equation of state"),
+    (" 1 hiex 1.63e-3 6930", "", "equation of state"),
+    (" 1 sjc 8.93e-3 117e3 0.35 157 425 1.0 0.0 1e-3", "seos 1 3 3.7e5 3.2e3 4.15 0.95 0.3 6993", "JWL"),
+    (" 1 null 1000 2.2e9 1e-3", "seos 1 5 1", "EOS type"),
+])
+def test_material_and_eos_combinations_rejected(tmp_path, mat, extra, msg):
     mpm = load_package()
-    text = BASE.replace(" 1 elas 1000 1e6 0.25", " 1 sjc 8.93e-3 117e3 0.35 157 425 1.0 0.0 1e-3")
+    text = BASE.replace(" 1 elas 1000 1e6 0.25", mat).replace("endi", extra + "\nendi")
     with pytest.raises(mpm.MpmError) as e:
         mpm.read_deck(write(tmp_path, text))
-    assert "Johnson-Cook" in str(e.value)
+    assert msg in str(e.value)
+
+
+def test_eos_constants_and_lighting_time(tmp_path):
+    """SetEos's Gruneisen constants (DataIn.f90:1123-1138) from FP32-read
+    c0, lambda, gamma0, and Initial's lighting time through a default REAL
+    (DataIn.f90:1252-1260); particles start at 293 degrees"""
+    mpm = load_package()
+    text = BASE.replace(" 1 elas 1000 1e6 0.25", " 1 sjc 8.93e-3 117e3 0.35 157 425 1.0 0.0 1e-3")
+    p = mpm.read_deck(write(tmp_path, text.replace("endi", "seos 1 2 3940 1.49 1.96 7\nendi")))
+    m = p.mats[0]
+    f = lambda v: float(np.float32(v))
+    c1 = f(8.93e-3) * (f(3940) * f(3940))
+    assert (m.eos_type, m.ceos[0], m.ceos[1]) == (2, c1, c1 * (2 * f(1.49) - 1))
+    assert m.ceos[2] == c1 * (f(1.49) - 1) * (3 * f(1.49) - 1)
+    assert (m.ceos[3], m.ceos[4], m.ceos[9]) == (f(8.93e-3) * f(1.96), f(1.96), 7.0)
+    assert np.all(p.field("temp") == 293.0) and np.all(p.field("lt") == 1e7)
+    text = BASE.replace(" 1 elas 1000 1e6 0.25", " 1 hiex 1.63e-3 6930")
+    p = mpm.read_deck(write(tmp_path, text.replace("endi", "seos 1 3 3.7e5 3.2e3 4.15 0.95 0.3 6993\n"
+                                                           "deto 0 0 0\ndeto 1 1 1\nendi")))
+    x, y, z = p.field("x"), p.field("y"), p.field("z")
+    d = np.minimum(np.sqrt(x * x + y * y + z * z), np.sqrt((x - 1) ** 2 + (y - 1) ** 2 + (z - 1) ** 2))
+    lt = (d / f(6930)).astype(np.float32).astype(np.float64)
+    assert np.array_equal(p.field("lt"), lt) and p.mats[0].det_speed == 6930.0
 
 
 def test_too_many_particles_rejected(tmp_path):

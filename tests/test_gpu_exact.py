@@ -156,9 +156,10 @@ def test_exact_batched_engine_with_graphs_and_resorts(mpm, oracle_mod):
     bitwise its oracle run (the scatter order does not depend on where the
     pool keeps a particle)."""
     probs = {n: mpm.read_deck(deck_path(n)) for n in ALL_DECKS}
-    eng = mpm.Engine(max_particles=sum(p.n for p in probs.values()) + 16 * 256,
-                     max_nodes=sum(p.nb_node for p in probs.values()) + 16 * 256, max_instances=16,
-                     precision="fp64exact")
+    eng = mpm.Engine(max_particles=sum(p.n for p in probs.values()) + len(probs) * 256,
+                     max_nodes=sum(p.nb_node for p in probs.values()) + len(probs) * 256, max_instances=len(probs),
+                     max_materials=sum(len(p.mats) for p in probs.values()),
+                     max_bodies=sum(len(p.bodies) for p in probs.values()), precision="fp64exact")
     eng.apply_tuning(sort_every=8)
     ids = {n: eng.add(p) for n, p in probs.items()}
     for _ in range(3):

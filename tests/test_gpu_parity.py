@@ -11,6 +11,16 @@ Tolerances (normwise per physical group, see conftest.group_errors):
 * cell indices (icell) and the sort permutation: bit-exact.
 * FP32 bulk path (FP64 positions, volume, energy, F, grid mass): 1e-5
   after 10 steps.
+* Decks on which the reference's own algorithm turns round-off into O(1)
+  changes are compared only bit for bit, in the reference-order mode
+  (tests/test_gpu_exact.py), not here (GATED below; tests/test_oracle.py
+  shows the oracle itself moving by more than 1e-5 under a one-ulp change
+  of the initial velocities).  In the Taylor impact with the Mie-Gruneisen
+  EOS the artificial bulk viscosity applies only where the pressure before
+  the step is compressive (eos2, Constitution.f90:1428-1430); particles
+  ahead of the compression wave carry a pressure of round-off size, so its
+  sign, not its value, decides an O(1) viscous term.  The Johnson-Cook
+  block deck is compared to step 10 (SHORT below).
 """
 import os
 
@@ -25,6 +35,15 @@ TOL64 = {name: 1e-12 for name in DECKS}
 TOL64_LOOSE = {"dam_null_eos4_musl": {"stress": 1e-9, "ie": 1e-9},
                "dam_null_sgmp_musl": {"stress": 1e-9, "ie": 1e-9}}
 WATER = ("dam_null_eos4_musl", "dam_null_sgmp_musl")
+GATED = {"taylor_sjc_eos2_musl": "bulk viscosity gated on the sign of a round-off-sized pressure: compared bit "
+                                 "for bit in the reference-order mode (test_gpu_exact.py)"}
+TOL_DECKS = [n for n in DECKS if n not in GATED]
+# compared to 10 steps only: by step 50 a particle yields by a round-off
+# amount, and the Johnson-Cook rate factor 1 + C ln(max(rate, 0.001))
+# (Constitution.f90:1077-1081) then cuts its yield stress by C ln 1000 (17%);
+# the oracle jumps by the same 2e-8 under one-ulp velocity changes
+# (tests/test_oracle.py)
+SHORT = {"cube_sjc_eos1_gimp_usl": 10}
 
 
 def perr(mpm, a, b):
@@ -58,13 +77,15 @@ def test_icell_bitexact(mpm, oracle_mod, name):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", DECKS)
+@pytest.mark.parametrize("name", TOL_DECKS)
 def test_fp64_fused_steps_match_oracle(mpm, oracle_mod, name):
     prob = mpm.read_deck(deck_path(name))
     ctx = mpm.Context(prob, "fp64")
     o = oracle_mod.Oracle(prob)
     done = 0
     for s in (2, 10, 50):
+        if s > SHORT.get(name, 50):
+            break
         ctx.step(s - done)
         o.step(s - done)
         done = s
@@ -77,7 +98,7 @@ def test_fp64_fused_steps_match_oracle(mpm, oracle_mod, name):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", DECKS)
+@pytest.mark.parametrize("name", TOL_DECKS)
 def test_fp64_matches_reference_fixture(mpm, name):
     """Directly against the Fortran reference's own state after 10 steps."""
     g = np.load(os.path.join(GOLDEN_DIR, name + ".npz"))
@@ -104,7 +125,7 @@ CTX_NAME = {"grid_momentum_initial": "GridMomentumInitial", "apply_bc": "ApplyBo
             "grid_momentum_musl": "GridMomentumMUSL"}
 
 
-@pytest.mark.parametrize("name", DECKS)
+@pytest.mark.parametrize("name", TOL_DECKS)
 def test_each_phase_entry_point_matches_oracle(mpm, oracle_mod, name):
     """Drive both sides phase by phase in the reference's call order and
     compare particles and grid after every phase of steps 1 and 2."""
@@ -147,7 +168,7 @@ def test_each_phase_entry_point_matches_oracle(mpm, oracle_mod, name):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", DECKS)
+@pytest.mark.parametrize("name", TOL_DECKS)
 def test_fp32_bulk_within_1e5_of_fp64_oracle(mpm, oracle_mod, name):
     """FP32 bulk after 10 steps.  The water columns are compared later: their
     pressure c^2 (m/V - rho0) starts from zero and during the first steps is
@@ -169,12 +190,14 @@ def test_fp32_bulk_within_1e5_of_fp64_oracle(mpm, oracle_mod, name):
 
 
 def test_batched_engine_matches_solo_runs(mpm):
-    """All twelve decks (four shape families, every scheme, SGMP included)
-    in one engine: each instance equals its solo context run (to FP64
+    """All decks (four shape families, every scheme, SGMP included, every
+    material model) in one engine: each instance equals its solo context run (to FP64
     round-off)."""
-    probs = {n: mpm.read_deck(deck_path(n)) for n in DECKS}
-    eng = mpm.Engine(max_particles=sum(p.n for p in probs.values()) + 16 * 256,
-                     max_nodes=sum(p.nb_node for p in probs.values()) + 16 * 256, max_instances=16)
+    probs = {n: mpm.read_deck(deck_path(n)) for n in TOL_DECKS}
+    eng = mpm.Engine(max_particles=sum(p.n for p in probs.values()) + len(probs) * 256,
+                     max_nodes=sum(p.nb_node for p in probs.values()) + len(probs) * 256,
+                     max_instances=len(probs), max_materials=sum(len(p.mats) for p in probs.values()),
+                     max_bodies=sum(len(p.bodies) for p in probs.values()))
     ids = {n: eng.add(p) for n, p in probs.items()}
     eng.step(10)
     for n, p in probs.items():

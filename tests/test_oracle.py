@@ -161,3 +161,55 @@ def test_update_dt_closed_form(tmp_path, oracle_mod):
     c = np.sqrt(E * (1 - nu) / (1 + nu) / (1 - 2 * nu) / ro)
     vmax = np.max(np.abs(o.particles()[mpm.PF["vx"]:mpm.PF["vz"] + 1]))
     assert o.scalars()[2] == prob.dcell / (c + vmax) * prob.dtscale
+
+
+@pytest.mark.parametrize("name,gated", [("taylor_sjc_eos2_musl", True), ("impact_sjcf_eos2_bspline_usf", False),
+                                        ("cube_sjc_eos1_gimp_usl", False), ("bar_j2_gimp_musl", False)])
+def test_round_off_sensitivity_of_the_reference_algorithm(name, gated, oracle_mod):
+    """Why one deck is compared only bit for bit: one ulp more on every
+    particle's initial z-velocity moves the oracle's stresses by more than
+    1e-5 after two steps on the Taylor impact with the Mie-Gruneisen EOS (its
+    bulk viscosity applies only where the pressure before the step is
+    negative, and ahead of the compression wave that pressure is round-off),
+    and by round-off on the others."""
+    from conftest import PGROUPS, group_errors
+    mpm = load_package()
+    p = mpm.read_deck(deck_path(name))
+    q = p.copy()
+    k = mpm.PF["vz"]
+    q.pstate[k] = np.nextafter(q.pstate[k], -np.inf)
+    a, b = oracle_mod.Oracle(p), oracle_mod.Oracle(q)
+    a.step(2)
+    b.step(2)
+    e = group_errors(b.particles(), a.particles(), mpm.PF_NAMES, PGROUPS)
+    if gated:
+        assert e["stress"] > 1e-5, e
+    else:
+        assert max(e.values()) < 1e-13, e
+
+
+def test_johnson_cook_yield_onset_is_discontinuous(oracle_mod):
+    """Why the Johnson-Cook block deck is compared to step 10 only: one-ulp
+    changes of some particles' velocities (seed 2: about half of them, each
+    component up or down) leave the oracle within round-off for 50 steps
+    except for one particle that yields by a round-off amount, whose rate
+    factor 1 + C ln(max(rate, 0.001)) then cuts its yield stress; the
+    stresses jump by 2e-8 (seeds 0 and 1 stay within 1e-15)."""
+    from conftest import PGROUPS, group_errors
+    mpm = load_package()
+    p = mpm.read_deck(deck_path("cube_sjc_eos1_gimp_usl"))
+    a = oracle_mod.Oracle(p)
+    a.step(50)
+    ref = a.particles()
+    jumps = {}
+    for seed in (0, 1, 2):
+        rng = np.random.default_rng(seed)
+        q = p.copy()
+        for f in ("vx", "vy", "vz"):
+            k = mpm.PF[f]
+            up = rng.choice([-1, 1], size=p.n) * np.inf
+            q.pstate[k] = np.where(rng.random(p.n) < 0.5, np.nextafter(q.pstate[k], up), q.pstate[k])
+        b = oracle_mod.Oracle(q)
+        b.step(50)
+        jumps[seed] = group_errors(b.particles(), ref, mpm.PF_NAMES, PGROUPS)["stress"]
+    assert jumps[0] < 1e-14 and jumps[1] < 1e-14 and jumps[2] > 1e-9, jumps

@@ -128,6 +128,11 @@ OPTION_SETS = {
     + "".join("curv %s %d\n" % kv for kv in (("stsx", 5), ("disx", 17), ("disy", 300), ("disz", 511),
                                              ("pres", 256), ("seqv", 1)))
     + "curx velz 0.5 0.5 0.5\n",
+    # pt2d (TecPlot frames leave out the particles outside its box) and damping
+    "cube_drda_linear_usf": "tecp\npara\noutr vely\noutr velz\ncurv velz 3\ncurv disz 400\n",
+    # failure flags and pressures of the failing impact, burn of the TNT bar
+    "impact_sjcf_eos2_bspline_usf": "tecp\noutr fail\noutr pres\noutr epef\ncurv pres 10\ncurv epef 100\n",
+    "tnt_hiex_jwl_usl": "tecp\npara\noutr pres\noutr velx\ncurv pres 5\ncurv cels 5\ncurv volu 9\n",
 }
 
 
@@ -153,7 +158,8 @@ def test_writers_match_the_reference_program(out_mod, oracle_mod, name):
         os.remove(p)
         ref, ours = ref_files(dref), ref_files(dours)
         assert sorted(ref) == sorted(ours)
-        assert len([f for f in ref if f.endswith(".vtu")]) == 5      # grid + frames at 0 and 3 output times
+        if "para" in OPTION_SETS[name]:
+            assert len([f for f in ref if f.endswith(".vtu")]) >= 4   # grid + frames at 0 and the output times
         assert ref[name + "_curv.dat"].count("\n") == steps + 1
         for f in ref:
             if ref[f] != ours[f]:
