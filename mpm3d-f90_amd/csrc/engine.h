@@ -65,6 +65,9 @@ struct InstT {
   int nan_count;         // particles whose stress update produced NaN
   int negvol_count;      // particles with negative volume (warning in the reference)
   int damped;            // drda: this step's kinetic energy did not grow, velocities are zeroed
+  int integ_pending;     // the last step's g2p integrated on the fly: PXg, FXg are not integrated yet
+  int integ_half;        //   that step was the first (half step)
+  double integ_dt;       //   and its DT
 };
 
 // per-material constants, all derived exactly as the reference derives them

@@ -127,6 +127,7 @@ struct HostInst {
   bool p_in_p2 = false;    // latest PXg lives in the re-map buffer
   int matmask = 0;         // material models used (bits 1 << type)
   bool xx_pending = false; // G2P done, Xp not yet moved (per-phase mode)
+  bool integ_pending = false;   // fused USF step: g2p integrated on the fly, PXg/FXg not yet
 };
 
 struct Group {
@@ -986,8 +987,24 @@ static void exact_group_phase(mpm_engine* e, Group& G, int ph) {
   }
 }
 
+// the grid state IntegrateMomentum leaves, for instances whose last fused
+// USF step integrated on the fly inside g2p (before any read of the grid or
+// any per-phase call): the host flag says "maybe", the instance's device
+// flag (set by that step's finalize, so not for instances that had already
+// stopped) decides
+template <typename T>
+static void settle_integrate(mpm_engine* e, int only = -1) {
+  for (size_t id = 0; id < e->inst.size(); id++) {
+    HostInst& h = e->inst[id];
+    if (!h.live || !h.integ_pending || (only >= 0 && (int)id != only)) continue;
+    mpmk::launch_integrate_range<T>(e->ptrs<T>(), (int)id, h.ng0 * NGRAN, h.c.nnode, e->s);
+    h.integ_pending = false;
+  }
+}
+
 template <typename T>
 static void run_phase(mpm_engine* e, int ph) {
+  settle_integrate<T>(e);
   rebuild_lists(e);
   const PoolPtrs<T> P = e->ptrs<T>();
   hipStream_t s = e->s;
@@ -1207,10 +1224,15 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
       // tile loads of the two gathers (the bc pass and IntegrateMomentum's
       // V, A stores are not needed: integrate zeroes the fixed directions
       // of PXg and FXg itself, and V, A are not grid state)
+      // IntegrateMomentum is folded into g2p's tile load as well (the node
+      // pass that stored the integrated momentum is gone from the step; the
+      // grid is integrated only when it is read, settle_integrate)
       timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, LS, s); });
       timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_FORCE | ST_VFLY, G.matmask, P, L, s); });
-      timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 0, s); });
-      timed(e, KI_G2P, [&] { mpmk::launch_g2p<T>(G.shape, G2P_MOVE | G2P_ENERGY | G2P_VAFLY, P, L, s); });
+      timed(e, KI_G2P, [&] {
+        mpmk::launch_g2p<T>(G.shape, G2P_MOVE | G2P_ENERGY | G2P_VAFLY | G2P_INTFLY, P, L, s);
+      });
+      for (int id : G.insts) e->inst[id].integ_pending = true;   // host: maybe (the device flag decides)
     } else if (G.scheme == MPM_USL) {
       timed(e, KI_P2G, [&] { mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM | P2G_FORCE, P, LS, s); });
       timed(e, KI_INTEGRATE, [&] { mpmk::launch_integrate<T>(P, N, 1, s); });
@@ -1223,8 +1245,9 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
       timed(e, KI_BC, [&] { mpmk::launch_bc<T>(P, N, 1, 1, 1, s); });
       timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_MOVE | ST_ENERGY, G.matmask, P, L, s); });
     }
+    const int intpend = (G.shape != MPM_SHAPE_SGMP && G.scheme == MPM_USF) ? FIN_INTPEND : 0;
     timed(e, KI_FINALIZE, [&] {
-      mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), FIN_ENERGY | FIN_DT | fin_flags, s);
+      mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), FIN_ENERGY | FIN_DT | fin_flags | intpend, s);
       if (G.drda) mpmk::launch_damp<T>(P, L, s);
     });
   }
@@ -1802,6 +1825,7 @@ template <typename T>
 static void download_grid_T(mpm_engine* e, int id, double* out) {
   HostInst& h = e->inst[id];
   const long nn = h.c.nnode, n0 = h.ng0 * NGRAN;
+  settle_integrate<T>(e, id);
   HIPCHK(hipStreamSynchronize(e->s));
   std::vector<double> m(nn);
   std::vector<T> v(nn);

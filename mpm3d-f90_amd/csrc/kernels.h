@@ -17,6 +17,7 @@
 #define G2P_REMAP 2   // MUSL: re-map momentum into p2
 #define G2P_ENERGY 4  // energy partials
 #define G2P_VAFLY 8   // nodal velocity/acceleration formed from PXg, FXg, Mg in the tile load
+#define G2P_INTFLY 16 // with VAFLY: IntegrateMomentum folded into the tile load (the grid stays un-integrated)
 
 // k_stress modes
 #define ST_MOVE 1     // USL/MUSL: Xp = XX
@@ -55,6 +56,7 @@
 #define FIN_ENERGY 1
 #define FIN_DT 2
 #define FIN_DONE 4
+#define FIN_INTPEND 8   // the step's IntegrateMomentum was folded into g2p: mark the grid un-integrated
 
 // a launch over a list of granules (particle or node)
 struct LaunchP {
@@ -78,6 +80,8 @@ template <typename T> void launch_node_va(const PoolPtrs<T>& P, const LaunchP& L
 template <typename T> void launch_zero_nodes(T* a, int ncomp, long stride, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_zero_grid(const PoolPtrs<T>& P, bool p2, const LaunchP& L, hipStream_t s);
 template <typename T> void launch_energy(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
+template <typename T>
+void launch_integrate_range(const PoolPtrs<T>& P, int inst, long node0, long nnode, hipStream_t s);
 template <typename T> void launch_damp(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s);
 template <typename T>
 void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int what, hipStream_t s);

@@ -684,13 +684,33 @@ __device__ __forceinline__ void load_tile_aos(const InstC& I, const BX& B, T* tf
 // update_step.f90:1795-1830, ahead of the strain-rate gather).  k: pool node.
 // The quotients are formed as products with one FP64 reciprocal of Mg per
 // node (within an FP64 ulp of the quotient).
-template <typename T, int NF, bool FIX>
-__device__ __forceinline__ void node_fly(const PoolPtrs<T>& P, const InstC& I, long k, T* v) {
+// With INT (NF = 6) the momentum is integrated first, with k_integrate's
+// expressions: PXg + FXg DT (DT/2 in the first step), both 0 on fixed
+// directions (IntegrateMomentum, update_step.f90:1194-1239).
+template <typename T, int NF, bool FIX, bool INT = false>
+__device__ __forceinline__ void node_fly(const PoolPtrs<T>& P, const InstC& I, long k, T* v, T dt = (T)0,
+                                         bool half = false) {
   const long nn = P.nn;
   const double mg = P.mg[k];
   const bool heavy = mg > I.cutoff;
   const double rm = heavy ? 1.0 / mg : 0.0;
-  const int fx = FIX ? (int)P.fix[k] : 0;
+  const int fx = (FIX || INT) ? (int)P.fix[k] : 0;
+  if (INT) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      T pg = P.pg[d * nn + k];
+      T f = P.fg[d * nn + k];
+      if (half) pg = pg + f * dt * (T)0.5;
+      else pg = pg + f * dt;
+      if (fx & (1 << d)) {
+        pg = (T)0;
+        f = (T)0;
+      }
+      v[d] = (T)((double)pg * rm);
+      v[3 + d] = (T)((double)f * rm);
+    }
+    return;
+  }
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const T pd = P.pg[d * nn + k];
@@ -706,12 +726,13 @@ __device__ __forceinline__ void node_fly(const PoolPtrs<T>& P, const InstC& I, l
 
 // node_fly values of a block's node box into the tile, node-major (AOS, the
 // fast-path layout) or field-major with stride cap
-template <typename T, int NF, bool FIX, bool AOS, typename BX>
-__device__ __forceinline__ void load_tile_fly(const PoolPtrs<T>& P, const InstC& I, const BX& B, T* tf, int cap) {
+template <typename T, int NF, bool FIX, bool AOS, typename BX, bool INT = false>
+__device__ __forceinline__ void load_tile_fly(const PoolPtrs<T>& P, const InstC& I, const BX& B, T* tf, int cap,
+                                              T dt = (T)0, bool half = false) {
   constexpr int S = AosTile<T, NF>::S;
   for (int e = threadIdx.x; e < B.n; e += blockDim.x) {
     T v[6];
-    node_fly<T, NF, FIX>(P, I, (long)I.node_off + B.node(I, e), v);
+    node_fly<T, NF, FIX, INT>(P, I, (long)I.node_off + B.node(I, e), v, dt, half);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
       if (AOS) tf[e * S + f] = v[f];
@@ -1377,6 +1398,32 @@ __global__ void __launch_bounds__(256) k_integrate(PoolPtrs<T> P, const int* ngl
   }
 }
 
+// IntegrateMomentum of one instance's nodes [node0, node0 + nnode), without
+// the nodal V/A (the grid of a fused USF step whose g2p integrated on the fly)
+template <typename T>
+__global__ void __launch_bounds__(256) k_integrate_range(PoolPtrs<T> P, int ii, long node0, long nnode) {
+  const long j = node0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= node0 + nnode) return;
+  const long nn = P.nn;
+  const InstT ts = P.ts[ii];
+  if (!ts.integ_pending) return;
+  const T dt = (T)ts.integ_dt;
+  const bool half = ts.integ_half != 0;
+  const uint8_t fx = P.fix[j];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    T pg = P.pg[d * nn + j];
+    T f = P.fg[d * nn + j];
+    if (half) pg = pg + f * dt * (T)0.5;
+    else pg = pg + f * dt;
+    if (fx & (1 << d)) {
+      pg = (T)0;
+      P.fg[d * nn + j] = (T)0;
+    }
+    P.pg[d * nn + j] = pg;
+  }
+}
+
 // ApplyBoundaryConditions (update_step.f90:1795-1830) on PXg (or the
 // re-mapped momentum) and FXg; with VEL also the nodal velocity the
 // strain-rate gather of ParticleStressUpdate reads (update_step.f90:586-587).
@@ -1533,13 +1580,16 @@ __global__ void __launch_bounds__(256, (g2p_waves<SHAPE, T>())) k_g2p(PoolPtrs<T
   const int sb = Bk.bx, sc = Bk.bx * Bk.by;
   const long nn = P.nn;
   constexpr bool FLY = (MODE & G2P_VAFLY) != 0;
+  constexpr bool INT = (MODE & G2P_INTFLY) != 0;
+  const T idt = (T)ts.dt;
+  const bool ihalf = ts.istep == 1;
   const T* Vg = P.vg + I.node_off;
   const T* Ag = P.ag + I.node_off;
   if (B.ok && B.n) {
     const T* srcs[2] = {Vg, Ag};
     if (FLY) {
-      if (fast) load_tile_fly<T, 6, false, true>(P, I, B, tf, CAP);
-      else load_tile_fly<T, 6, false, false>(P, I, B, tf, CAP);
+      if (fast) load_tile_fly<T, 6, false, true, Box2, INT>(P, I, B, tf, CAP, idt, ihalf);
+      else load_tile_fly<T, 6, false, false, Box2, INT>(P, I, B, tf, CAP, idt, ihalf);
     } else {
       if (fast) load_tile_aos<T, 6>(I, B, tf, srcs, nn);
       else load_tile<T, 6>(I, B, tf, CAP, srcs, nn);
@@ -1567,7 +1617,7 @@ __global__ void __launch_bounds__(256, (g2p_waves<SHAPE, T>())) k_g2p(PoolPtrs<T
       for_nodes<SHAPE, T>(I, S, [&](int k, int nx, int ny, int nz, T N, T, T, T) __attribute__((always_inline)) {
         const int e = B.ok ? B.idx(nx, ny, nz) : -1;
         T g[6];
-        if (FLY && e < 0) node_fly<T, 6, false>(P, I, (long)I.node_off + k, g);
+        if (FLY && e < 0) node_fly<T, 6, false, INT>(P, I, (long)I.node_off + k, g, idt, ihalf);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           const T v = e >= 0 ? tf[d * CAP + e] : (FLY ? g[d] : Vg[d * nn + k]);
@@ -2974,6 +3024,11 @@ __global__ void __launch_bounds__(256) k_finalize(PoolPtrs<T> P, const int* ilis
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
+  if (what & FIN_INTPEND) {   // ahead of the DT update: the step's own DT
+    ts.integ_pending = 1;
+    ts.integ_dt = ts.dt;
+    ts.integ_half = ts.istep == 1;
+  }
   if (what & FIN_ENERGY) {
     ts.ek_prev = ts.ek;
     ts.ek = red[0][0];
@@ -3281,6 +3336,7 @@ void launch_g2p_s(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream_t 
     G2P_CASE(0)
     G2P_CASE(G2P_MOVE)
     G2P_CASE(G2P_MOVE | G2P_ENERGY | G2P_VAFLY)
+    G2P_CASE(G2P_MOVE | G2P_ENERGY | G2P_VAFLY | G2P_INTFLY)
     G2P_CASE(G2P_REMAP)
     default: break;
   }
@@ -3474,6 +3530,18 @@ void launch_finalize(const PoolPtrs<T>& P, const int* ilist, int ninst, int what
   if (ninst) hipLaunchKernelGGL(k_finalize<T>, dim3(ninst), dim3(256), 0, s, P, ilist, what);
 }
 
+// (a template: each precision's object holds its own instance)
+template <typename T>
+__global__ void k_clear_integ(InstT* ts, int ii) { ts[ii].integ_pending = 0; }
+
+template <typename T>
+void launch_integrate_range(const PoolPtrs<T>& P, int inst, long node0, long nnode, hipStream_t s) {
+  if (nnode > 0)
+    hipLaunchKernelGGL(k_integrate_range<T>, dim3((unsigned)((nnode + 255) / 256)), dim3(256), 0, s, P, inst, node0,
+                       nnode);
+  hipLaunchKernelGGL(k_clear_integ<T>, dim3(1), dim3(1), 0, s, P.ts, inst);
+}
+
 template <typename T>
 void launch_damp(const PoolPtrs<T>& P, const LaunchP& L, hipStream_t s) {
   if (L.nblk) hipLaunchKernelGGL(k_damp<T>, dim3(L.nblk), dim3(L.bs), 0, s, P, L.glist, L.ginst, L.bpg);
@@ -3607,6 +3675,7 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
   template void launch_energy<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                 \
   template void launch_finalize<T>(const PoolPtrs<T>&, const int*, int, int, hipStream_t);         \
   template void launch_damp<T>(const PoolPtrs<T>&, const LaunchP&, hipStream_t);                    \
+  template void launch_integrate_range<T>(const PoolPtrs<T>&, int, long, long, hipStream_t);        \
   template void launch_permute<T>(T*, const T*, const int*, const LaunchP&, long, int, hipStream_t);  \
   template void launch_sgmp_p2g<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);         \
   template void launch_sgmp_nodes<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);       \
