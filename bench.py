@@ -79,8 +79,8 @@ def kernel_bytes(kid, t, loads=True):
         rd = 24 + 4 + 4 + 8 + 8 + 11 * t + (3 * t if loads else 0)
         wr = 8 + 8 + 12 * t  # write vol, ie, sm, seqv, sd(6), sig_y, epeff, q, cp
         return rd + wr, 3 * t + 8 + 1 + 3 * t   # read PXg, Mg, fixity | write FXg
-    if kid == "g2p":        # read x, cell, v, m, ie; write v, x | read PXg, FXg, Mg
-        return 24 + 4 + 3 * t + t + 8 + 3 * t + 24, 6 * t + 8
+    if kid == "g2p":        # read x, cell, v, m, ie; write v, x | read PXg, FXg, Mg, fixity (integrated in the load)
+        return 24 + 4 + 3 * t + t + 8 + 3 * t + 24, 6 * t + 8 + 1
     return 0, 0
 
 
@@ -406,6 +406,60 @@ def run_bar(mpm, decks, a, log):
     return out
 
 
+def run_dropin(mpm, decks, a, log):
+    """The reference's own main program over the device step (the Fortran
+    drop-in: MPM3D.f90 and its input phase linked with update_step_hip.f90,
+    oracle/_ref/mpm3d_dropin) on the Taylor bar and the single slope, with
+    the reference program itself (oracle/_ref/mpm3d_ref, one core) on the same
+    decks.  Reported: the programs' own "Time per particle cycle" line (CPU
+    time of the step loop per particle and step, MPM3D.f90:259-279; the
+    drop-in's host waits for the device inside that time), the loop's wall
+    time as the program prints it (whole seconds) and the process's wall
+    time (input phase included); steps from the curve file.  The drop-in downloads per step only the energies and the curve
+    particles; these decks write no frames.  The slope runs without its
+    gravity ramp (QuLo): with it the reference's loop only ends after
+    CurrentTime > 100 (MPM3D.f90:152-155); the work per step is unchanged."""
+    progs = {"dropin": os.path.join(ROOT, "oracle", "_ref", "mpm3d_dropin"),
+             "reference": os.path.join(ROOT, "oracle", "_ref", "mpm3d_ref")}
+    if not all(os.path.exists(p) for p in progs.values()):
+        return None
+    out = {}
+    for key, (text, _) in (("taylor_bar", decks.taylor_bar()), ("single_slope", decks.slope())):
+        prob = decks.load(text, mpm, key)
+        res = {"particles": prob.n}
+        runs = [("dropin_fp64", "dropin", 100, {"MPM3D_PRECISION": "fp64"}),
+                ("dropin_fp32", "dropin", 100, {"MPM3D_PRECISION": "fp32"}),
+                ("reference_1core", "reference", 3 if prob.n > 500000 else 20, {})]
+        for name, prog, steps, env in runs:
+            deck = re.sub(r"(?m)^endtime .*$", "endtime %.9e" % ((steps - 0.5) * prob.dt), text)
+            deck = re.sub(r"(?m)^QuLo .*\n", "", deck)
+            d = tempfile.mkdtemp(prefix="mpmdropin_")
+            try:
+                with open(os.path.join(d, "run.mpm"), "w") as f:
+                    f.write(deck)
+                cmd = ([progs[prog], "run"] if prog == "dropin" else ["taskset", "-c", "0", progs[prog], "run"])
+                t0 = time.time()
+                r = subprocess.run(cmd, cwd=d, env=dict(os.environ, **env), capture_output=True, text=True,
+                                   timeout=600, stdin=subprocess.DEVNULL)
+                t1 = time.time() - t0
+                tp = re.search(r"Time per particle cycle:\s+(\S+)", r.stdout)
+                wall = re.search(r"Total elasped time :\s+(\S+)", r.stdout)
+                nst = max(0, len(open(os.path.join(d, "run_curv.dat")).read().splitlines()) - 1)
+                if r.returncode == 0 and tp and wall and nst:
+                    res[name] = {"ns_per_particle_cycle": float(tp.group(1)),
+                                 "particle_steps_per_s_cpu_time": 1e9 / float(tp.group(1)),
+                                 "steps": nst, "loop_wall_s": float(wall.group(1)),
+                                 "process_wall_s": t1}
+                else:
+                    res[name] = {"error": (r.stdout + r.stderr)[-300:]}
+                log("drop-in %s %s: %s" % (key, name, json.dumps(res[name])))
+            finally:
+                shutil.rmtree(d, ignore_errors=True)
+        out[key] = res
+    log("drop-in: %s" % json.dumps(out))
+    return out
+
+
 def run_sgmp(mpm, decks, a, log):
     """SGMP workloads: 64 Cook-membrane plates (nearly incompressible
     neo-Hookean) and one 3D dam break in one engine, FP32 bulk (FP64 F, det F,
@@ -498,6 +552,8 @@ def main():
     ap.add_argument("--check-steps", type=int, default=2, help="steps of the single slope compared with the oracle")
     ap.add_argument("--cpu-steps", type=int, default=30, help="steps of the CPU baseline sample per instance")
     ap.add_argument("--no-cpu", action="store_true", help="skip oracle parity and CPU baseline")
+    ap.add_argument("--no-dropin", dest="dropin", action="store_false",
+                    help="skip the Fortran drop-in timing on the Taylor bar and the single slope")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="timed steps as plain launches instead of graph replays (for tracing tools)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false", help="keep the default launch shape")
@@ -547,6 +603,7 @@ def main():
     for eng in E["engs"].values():
         eng.close()
     bar = run_bar(mpm, decks, a, log)
+    dropin = run_dropin(mpm, decks, a, log) if a.dropin else None
     sgmp = run_sgmp(mpm, decks, a, log) if a.sgmp_cooks > 0 else None
     serving = run_serving(mpm, decks, a, log) if a.serve_window > 0 else None
     if not a.no_cpu:
@@ -605,6 +662,7 @@ def main():
                                         lambda k: measured_traffic(k, "fp32", "slope")),
         "parity": parity or None,
         "taylor_bar": bar,
+        "dropin": dropin,
         "sgmp": sgmp,
         "serving": serving,
         "cpu_baseline": cpu,
