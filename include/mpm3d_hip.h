@@ -102,6 +102,11 @@ int mpm_integrate_momentum(mpm_ctx* ctx);
 int mpm_particle_position_update(mpm_ctx* ctx);
 /* GridMomentumMUSL (update_step.f90:1668-1793) */
 int mpm_grid_momentum_musl(mpm_ctx* ctx);
+/* SmoothStressbyGrid (update_step.f90:1876-2055, called after UpdateDT when
+ * the deck has 'smoo', MPM3D.f90:114): the smoothed pressure Spre (PF_SPRE)
+ * of every particle; a no-op for a problem without smooth_stress.  mpm_step
+ * runs it inside each step of such a problem. */
+int mpm_smooth_stress_by_grid(mpm_ctx* ctx);
 /* calcEnergy (Particle.f90:204-230); any output pointer may be NULL */
 int mpm_calc_energy(mpm_ctx* ctx, double* eng_kinetic, double* eng_internal, double* momentum3);
 /* UpdateDT (update_step.f90:1833-1874); *dt may be NULL.  Returns

@@ -83,6 +83,7 @@ enum mpm_pfield {
   PF_DMG,                       /* damage */
   PF_LT,                        /* lighting time of high explosive (Initial, DataIn.f90:1252-1260) */
   PF_FAIL,                      /* failure flag, 0 or 1 */
+  PF_SPRE,                      /* Spre: smoothed pressure of SmoothStressbyGrid (update_step.f90:1876-2055) */
   PF_COUNT
 };
 
@@ -118,6 +119,14 @@ typedef struct mpm_problem {
   mpm_body* bodies;       /* nb_body */
   double* pstate;         /* PF_COUNT * nb_particle, field-major */
   int32_t* pbody;         /* nb_particle, 0-based body id */
+  /* appended after the fields above (whose layout stays as it was) */
+  int32_t smooth_stress;  /* smoo: SmoothStressbyGrid after UpdateDT every step (MPM3D.f90:114) */
+  int32_t tlmpm;          /* TLmp: total-Lagrangian step sequence (MPM3D.f90:166-250) */
+  int32_t contact;        /* cont lagr: Contact_type 1, Lagr_NodContact after IntegrateMomentum */
+  int32_t normbody;       /* contact normal: 0 from both bodies, 1 from component 1, 2 from component 2 */
+  int32_t nb_component;   /* nbco: grid components (bodies carry their component in com) */
+  int32_t reserved0;
+  double fricfa;          /* contact friction coefficient */
 } mpm_problem;
 
 #ifdef __cplusplus

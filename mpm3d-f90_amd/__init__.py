@@ -96,6 +96,9 @@ class Context:
     def GridMomentumMUSL(self):
         self._call("mpm_grid_momentum_musl")
 
+    def SmoothStressbyGrid(self):
+        self._call("mpm_smooth_stress_by_grid")
+
     def calcEnergy(self):
         ek, ei, mom = C.c_double(), C.c_double(), (C.c_double * 3)()
         check(_lib.lib().mpm_calc_energy(self._h, C.byref(ek), C.byref(ei), mom), "mpm_calc_energy")
@@ -130,6 +133,8 @@ class Context:
                 self.ParticleStressUpdate()
             self.calcEnergy()
             self.UpdateDT()
+            if self.problem.smooth_stress:
+                self.SmoothStressbyGrid()
 
     # ---- state ---------------------------------------------------------
     def particles(self):

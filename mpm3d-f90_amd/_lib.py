@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libmpm3d_hip.so")
 PF_NAMES = (["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sm", "seqv",
              "sdxx", "sdyy", "sdzz", "sdyz", "sdxz", "sdxy", "sig_y", "epeff", "ie", "q", "cp"]
             + ["fp%d%d" % (i + 1, j + 1) for i in range(3) for j in range(3)]
-            + ["xx", "xy", "xz", "temp", "dmg", "lt", "fail"])
+            + ["xx", "xy", "xz", "temp", "dmg", "lt", "fail", "spre"])
 PF = {n: i for i, n in enumerate(PF_NAMES)}
 PF_COUNT = len(PF_NAMES)
 GF_NAMES = ["mg", "px", "py", "pz", "fx", "fy", "fz"]
@@ -58,7 +58,10 @@ class ProblemStruct(C.Structure):
                 ("endtime", C.c_double), ("quasi_load_time", C.c_double),
                 ("damp_factor", C.c_double), ("bq1", C.c_double), ("bq2", C.c_double),
                 ("mats", C.POINTER(Material)), ("bodies", C.POINTER(Body)),
-                ("pstate", C.POINTER(C.c_double)), ("pbody", C.POINTER(C.c_int32))]
+                ("pstate", C.POINTER(C.c_double)), ("pbody", C.POINTER(C.c_int32)),
+                ("smooth_stress", C.c_int32), ("tlmpm", C.c_int32), ("contact", C.c_int32),
+                ("normbody", C.c_int32), ("nb_component", C.c_int32), ("reserved0", C.c_int32),
+                ("fricfa", C.c_double)]
 
 
 MAX_CURVES, MAX_ANIM = 15, 10
@@ -120,6 +123,7 @@ def lib():
         "mpm_integrate_momentum": (C.c_int, [vp]),
         "mpm_particle_position_update": (C.c_int, [vp]),
         "mpm_grid_momentum_musl": (C.c_int, [vp]),
+        "mpm_smooth_stress_by_grid": (C.c_int, [vp]),
         "mpm_calc_energy": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_double)]),
         "mpm_update_dt": (C.c_int, [vp, P(C.c_double)]),
         "mpm_step": (C.c_int, [vp, C.c_int]),

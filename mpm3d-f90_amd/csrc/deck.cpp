@@ -197,6 +197,7 @@ struct Deck {
   double dcell = 0, cutoff = 0, dtscale = (double)0.9f, endtime = 0;
   double qlt = 0, damp = 0, bq1 = 1.5, bq2 = (double)0.06f;
   bool dr_damping = false;
+  bool smooth_stress = false;   // smoo: SmoothStressbyGrid every step (Spre, read by the ParaView writer)
   int n_deto = 0;
   double deto[10][3] = {};
   int fix[6] = {0, 0, 0, 0, 0, 0};
@@ -645,8 +646,10 @@ struct Deck {
         case 31: usf = on_off(); if (usf) update_method = true; break;
         case 32: usl = on_off(); if (usl) update_method = true; break;
         case 33:
+          // without contact every body maps to component 1 (update_step.f90:56);
+          // the other components' grids stay empty
           nb_component = tk.get_int();
-          if (nb_component != 1) fail("multi-component grids are used only by contact");
+          if (nb_component < 1) fail("nbco must be positive");
           break;
         case 34:
           nb_body = tk.get_int();
@@ -657,7 +660,7 @@ struct Deck {
         case 36: dr_damping = true; break;
         case 37: bspline = true; break;
         case 38: sgmp = true; break;
-        case 39: break;  // smoo: smoothed pressure feeds only the output writers
+        case 39: smooth_stress = true; break;
         case 40: fail("total-Lagrangian MPM ('TLmp') is not supported");
         case 41: uvfp = true; break;
         case 42: quasi_load = true; qlt = tk.get_real(); break;
@@ -732,6 +735,8 @@ extern "C" int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, 
     p->bq1 = d.bq1;
     p->bq2 = d.bq2;
     p->dr_damping = d.dr_damping;
+    p->smooth_stress = d.smooth_stress;
+    p->nb_component = d.nb_component;
     p->mats = (mpm_material*)calloc(d.nb_mat, sizeof(mpm_material));
     p->bodies = (mpm_body*)calloc(d.nb_body, sizeof(mpm_body));
     for (int m = 0; m < d.nb_mat; m++) p->mats[m] = d.mats[m];

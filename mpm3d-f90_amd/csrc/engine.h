@@ -52,7 +52,8 @@ enum InstFlags {
   IF_UVFP = 8,
   IF_LOADS = 16,         // some particle carries a prescribed load FXp
   IF_ENDTIME = 32,       // instance stops itself when time > endtime
-  IF_DRDA = 64           // dynamic relaxation damping after UpdateDT (drda)
+  IF_DRDA = 64,          // dynamic relaxation damping after UpdateDT (drda)
+  IF_SMOOTH = 128        // SmoothStressbyGrid at the end of every step (smoo)
 };
 
 // per-instance time state, advanced on the device (graph-capturable)
@@ -107,7 +108,9 @@ enum TField {
 // time and the failure flag (0/1)
 enum DField {
   D_X = 0, D_Y, D_Z, D_VOL, D_IE, D_F0, D_F1, D_F2, D_F3, D_F4, D_F5, D_F6, D_F7, D_F8, D_MASS,
-  D_TEMP, D_DMG, D_LT, D_FAIL, D_COUNT
+  D_TEMP, D_DMG, D_LT, D_FAIL,
+  D_SPRE,                // smoothed pressure (SmoothStressbyGrid; output only)
+  D_COUNT
 };
 
 // particle integer fields
@@ -125,6 +128,21 @@ struct XPtrs {
   int* seg1;                 // nn: one past its last
   long m;                    // entries: 27 * np
   int obits;                 // key bits below the node index
+};
+
+// SmoothStressbyGrid buffers, FP64 in every precision (allocated with the
+// first instance that smooths): node sums of -SM V and V, the nodal pressure,
+// the SGMP cell-centre sums (stored at the cell's index inside the node
+// range), the mapped flags, and the step-start particle positions (the
+// linear and SGMP shape functions of the general modes are those of the
+// step's own ParticleInitial)
+struct SmPtrs {
+  double* gpre;   // nn: Gpre
+  double* gv;     // nn: GV_all
+  double* cgpre;  // nn: CGpre (SGMP)
+  double* cgv;    // nn: CGV_all (SGMP)
+  uint8_t* map;   // nn: mapped
+  double* xs;     // 3 * np: positions at the start of the step (x, y, z field-major)
 };
 
 template <typename T>

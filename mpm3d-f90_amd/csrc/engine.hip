@@ -134,6 +134,7 @@ struct Group {
   int shape = 0, scheme = 0;
   int matmask = 0;   // material models of the group's instances (bits 1 << type)
   bool drda = false; // some instance damps by dynamic relaxation
+  bool smooth = false; // some instance smooths its stress by the grid (smoo)
   std::vector<int> insts;
   std::vector<int> pgran, ngran;
   int* d_pgran = nullptr;
@@ -164,6 +165,7 @@ struct mpm_engine {
   void* ag = nullptr;
   uint8_t* fix = nullptr;
   void* aux = nullptr;            // SGMP auxiliary-node fields (allocated with the first SGMP instance)
+  SmPtrs sm{};                    // SmoothStressbyGrid buffers (allocated with the first instance that smooths)
   double* cmg = nullptr;
   InstC* d_inst = nullptr;
   InstT* d_ts = nullptr;
@@ -294,7 +296,8 @@ static void free_engine(mpm_engine* e) {
                   e->d_bodies, e->part, e->d_ginst, e->d_nginst, e->d_ilist_all, e->k_in, e->k_out, e->v_in,
                   e->v_out, e->rs_scr, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg, e->boxes,
                   e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1], e->x_rnode, e->x_rw, e->x_seg0, e->x_seg1,
-                  e->x_scr, e->d_sub_ids, e->d_sub_rows};
+                  e->x_scr, e->d_sub_ids, e->d_sub_rows, e->sm.gpre, e->sm.gv, e->sm.cgpre, e->sm.cgv, e->sm.map,
+                  e->sm.xs};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& g : e->groups) {
@@ -543,6 +546,7 @@ static void upload_particles_T(mpm_engine* e, const HostInst& h, const double* p
     db[(size_t)D_DMG * pad + i] = P(PF_DMG, i);
     db[(size_t)D_LT * pad + i] = P(PF_LT, i);
     db[(size_t)D_FAIL * pad + i] = P(PF_FAIL, i);
+    db[(size_t)D_SPRE * pad + i] = P(PF_SPRE, i);
   }
   if (!first) {
     // upload keeps the device permutation: scatter host rows to their slots
@@ -581,6 +585,17 @@ static void ensure_aux(mpm_engine* e) {
   if (e->aux) return;
   e->aux = dalloc<char>((size_t)AUX_COUNT * e->nn * e->tsz);
   e->cmg = dalloc<double>(e->nn);
+  mark_graph_dirty(e);
+}
+
+static void ensure_smooth(mpm_engine* e) {
+  if (e->sm.gpre) return;
+  e->sm.gpre = dalloc<double>(e->nn);
+  e->sm.gv = dalloc<double>(e->nn);
+  e->sm.cgpre = dalloc<double>(e->nn);
+  e->sm.cgv = dalloc<double>(e->nn);
+  e->sm.map = dalloc<uint8_t>(e->nn);
+  e->sm.xs = dalloc<double>(3 * e->np);
   mark_graph_dirty(e);
 }
 
@@ -635,7 +650,7 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   c.scheme = p->scheme;
   c.flags = (p->quasi_load ? IF_QUASI_LOAD : 0) | (p->quasi_damp ? IF_QUASI_DAMP : 0) |
             (p->gravity ? IF_GRAVITY : 0) | (p->update_v_by_fp ? IF_UVFP : 0) | (p->dr_damping ? IF_DRDA : 0) |
-            flags_extra;
+            (p->smooth_stress ? IF_SMOOTH : 0) | flags_extra;
   bool loads = false;
   for (int i = 0; i < n && !loads; i++)
     for (int d = 0; d < 3; d++)
@@ -651,6 +666,7 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   c.cncell = c.cncxy * std::max(c.ncz - 1, 1);
   for (int k = 0; k < 6; k++) c.fixs[k] = p->fix[k];
   if (p->shape == MPM_SHAPE_SGMP) ensure_aux(e);
+  if (p->smooth_stress) ensure_smooth(e);
   for (int k = 0; k < 3; k++) {
     c.span0[k] = p->span[2 * k];
     c.span1[k] = p->span[2 * k + 1];
@@ -799,6 +815,7 @@ static void rebuild_lists(mpm_engine* e) {
     G.insts.push_back(id);
     G.matmask |= h.matmask;
     G.drda = G.drda || (h.c.flags & IF_DRDA) != 0;
+    G.smooth = G.smooth || (h.c.flags & IF_SMOOTH) != 0;
     for (long k = 0; k < h.npg; k++) G.pgran.push_back((int)(h.pg0 + k));
     for (long k = 0; k < h.nng; k++) G.ngran.push_back((int)(h.ng0 + k));
   }
@@ -853,8 +870,14 @@ static LaunchP nlist(const mpm_engine* e, const Group& G) {
 // --------------------------------------------------------- phases
 
 enum Phase {
-  PH_BEGIN, PH_GMI, PH_BC, PH_STRESS, PH_GMU, PH_INTEGRATE, PH_G2P, PH_MUSL, PH_ENERGY, PH_DT
+  PH_BEGIN, PH_GMI, PH_BC, PH_STRESS, PH_GMU, PH_INTEGRATE, PH_G2P, PH_MUSL, PH_ENERGY, PH_DT, PH_SMOOTH
 };
+
+// the smoothing shape functions of the general modes need the step-start
+// positions for the linear and SGMP shapes (their XN, XXN)
+static bool smooth_saves_x(const Group& G) {
+  return G.smooth && (G.shape == MPM_SHAPE_LINEAR || G.shape == MPM_SHAPE_SGMP);
+}
 
 // ---- reference-order FP64 mode (MPM_PREC_FP64_EXACT)
 
@@ -987,6 +1010,41 @@ static void exact_group_phase(mpm_engine* e, Group& G, int ph) {
   }
 }
 
+// SmoothStressbyGrid of every smoothing group in the reference-order mode,
+// after all of the step's other phases: linear and SGMP take the step's
+// entries (GridMomentumInitial's XN, XXN), B-spline rewrites its entries'
+// weights at the current position (same node list, so the sorted order
+// stands), GIMP rebuilds its entries at the current position (its case may
+// change) and sorts them again -- last, since that sort replaces every
+// group's entries
+static void exact_smooth(mpm_engine* e) {
+  const PoolPtrs<double> P = e->ptrs<double>();
+  const XPtrs X = xptrs(e);
+  hipStream_t s = e->s;
+  bool gimp = false;
+  for (Group& G : e->groups) {
+    if (!G.smooth) continue;
+    if (G.shape == MPM_SHAPE_GIMP) {
+      gimp = true;
+      continue;
+    }
+    if (G.shape == MPM_SHAPE_BSPLINE) mpmk::launch_x_pairs_smooth(G.shape, P, X, plist(e, G), s);
+    mpmk::launch_x_smooth(G.shape, P, X, e->sm, e->x_val[e->x_sorted], plist(e, G), nlist(e, G), s);
+  }
+  if (!gimp) return;
+  mpmk::launch_x_fill_keys(e->x_key[0], e->x_val[0], e->x_m, s);
+  for (Group& G : e->groups)
+    if (G.smooth && G.shape == MPM_SHAPE_GIMP) mpmk::launch_x_pairs_smooth(G.shape, P, X, plist(e, G), s);
+  e->x_sorted = mpmk::radix_sort_pairs<unsigned long long>(e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1],
+                                                           e->x_m, 0, e->x_nbits, e->x_scr, s);
+  mpmk::launch_clear_ints(e->x_seg0, e->nn, s);
+  mpmk::launch_clear_ints(e->x_seg1, e->nn, s);
+  mpmk::launch_x_segments(e->x_key[e->x_sorted], e->x_m, e->x_obits, e->x_seg0, e->x_seg1, s);
+  for (Group& G : e->groups)
+    if (G.smooth && G.shape == MPM_SHAPE_GIMP)
+      mpmk::launch_x_smooth(G.shape, P, X, e->sm, e->x_val[e->x_sorted], plist(e, G), nlist(e, G), s);
+}
+
 // the grid state IntegrateMomentum leaves, for instances whose last fused
 // USF step integrated on the fly inside g2p (before any read of the grid or
 // any per-phase call): the host flag says "maybe", the instance's device
@@ -1014,6 +1072,7 @@ static void run_phase(mpm_engine* e, int ph) {
   }
   if (e->exact && ph != PH_ENERGY && ph != PH_DT) {
     if (ph == PH_GMI) exact_gmi(e);
+    else if (ph == PH_SMOOTH) exact_smooth(e);
     else
       for (Group& G : e->groups) exact_group_phase(e, G, ph);
     HIPCHK(hipGetLastError());
@@ -1034,6 +1093,7 @@ static void run_phase(mpm_engine* e, int ph) {
           mpmk::launch_zero_nodes<T>((T*)e->pg, 3, e->nn, N, s);
           mpmk::launch_p2g<T>(G.shape, P2G_ICELL | P2G_MASS | P2G_MOM, P, LS, s);
         }
+        if (smooth_saves_x(G)) mpmk::launch_smooth_save_x<T>(P, e->sm, L, s);
         for (int id : G.insts) e->inst[id].p_in_p2 = false;
         break;
       case PH_BC:
@@ -1083,6 +1143,9 @@ static void run_phase(mpm_engine* e, int ph) {
         mpmk::launch_energy<T>(P, L, s);
         mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), FIN_DT, s);
         if (G.drda) mpmk::launch_damp<T>(P, L, s);
+        break;
+      case PH_SMOOTH:
+        if (G.smooth) mpmk::launch_smooth<T>(G.shape, P, e->sm, L, N, s);
         break;
     }
   }
@@ -1195,6 +1258,14 @@ static void enqueue_step_exact(mpm_engine* e, int fin_flags) {
     ph(KI_G2P, PH_G2P);
     if (G.scheme == MPM_MUSL) ph(KI_P2G, PH_MUSL);
     if (G.scheme != MPM_USF) ph(KI_STRESS, PH_STRESS);
+  }
+  // SmoothStressbyGrid (timed with finalize) before finalize, which may
+  // mark an instance done
+  bool smooth = false;
+  for (Group& G : e->groups) smooth = smooth || G.smooth;
+  if (smooth) timed(e, KI_FINALIZE, [&] { exact_smooth(e); });
+  for (Group& G : e->groups) {
+    const LaunchP L = plist(e, G);
     timed(e, KI_FINALIZE, [&] {
       mpmk::launch_energy<double>(P, L, s);
       mpmk::launch_finalize<double>(P, G.d_ilist, (int)G.insts.size(), FIN_ENERGY | FIN_DT | fin_flags, s);
@@ -1216,6 +1287,7 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
   for (Group& G : e->groups) {
     const LaunchP L = plist(e, G), N = nlist(e, G), LS = plist(e, G, e->bs_s);
     timed(e, KI_ZERO, [&] { mpmk::launch_zero_grid<T>(P, G.scheme == MPM_MUSL, N, s); });
+    if (smooth_saves_x(G)) timed(e, KI_FINALIZE, [&] { mpmk::launch_smooth_save_x<T>(P, e->sm, L, s); });
     if (G.shape == MPM_SHAPE_SGMP) {
       enqueue_sgmp<T>(e, G, P, L, LS, N);
     } else if (G.scheme == MPM_USF) {
@@ -1246,6 +1318,9 @@ static void enqueue_step(mpm_engine* e, int fin_flags) {
       timed(e, KI_STRESS, [&] { mpmk::launch_stress<T>(G.shape, ST_MOVE | ST_ENERGY, G.matmask, P, L, s); });
     }
     const int intpend = (G.shape != MPM_SHAPE_SGMP && G.scheme == MPM_USF) ? FIN_INTPEND : 0;
+    // SmoothStressbyGrid (after UpdateDT in the reference; it reads none of
+    // finalize's results, and finalize may mark an instance done)
+    if (G.smooth) timed(e, KI_FINALIZE, [&] { mpmk::launch_smooth<T>(G.shape, P, e->sm, L, N, s); });
     timed(e, KI_FINALIZE, [&] {
       mpmk::launch_finalize<T>(P, G.d_ilist, (int)G.insts.size(), FIN_ENERGY | FIN_DT | fin_flags | intpend, s);
       if (G.drda) mpmk::launch_damp<T>(P, L, s);
@@ -1753,6 +1828,7 @@ static void download_particles_T(mpm_engine* e, int id, double* out) {
     O(PF_DMG) = db[(size_t)D_DMG * pad + j];
     O(PF_LT) = db[(size_t)D_LT * pad + j];
     O(PF_FAIL) = db[(size_t)D_FAIL * pad + j];
+    O(PF_SPRE) = db[(size_t)D_SPRE * pad + j];
     for (int k = 0; k < 3; k++) {
       const double x = db[(size_t)(D_X + k) * pad + j];
       O(PF_XX + k) = h.xx_pending ? x + Tv(T_BX + k) * ts.dt : x;
@@ -1817,6 +1893,7 @@ static void download_subset_T(mpm_engine* e, int id, int k, const int32_t* ids, 
     O(PF_DMG) = Dv(D_DMG);
     O(PF_LT) = Dv(D_LT);
     O(PF_FAIL) = Dv(D_FAIL);
+    O(PF_SPRE) = Dv(D_SPRE);
     for (int c = 0; c < 3; c++) O(PF_XX + c) = h.xx_pending ? Dv(D_X + c) + Tv(T_BX + c) * ts.dt : Dv(D_X + c);
   }
 }
@@ -1960,6 +2037,7 @@ CTX_PHASE(mpm_grid_momentum_update, PH_GMU)
 CTX_PHASE(mpm_integrate_momentum, PH_INTEGRATE)
 CTX_PHASE(mpm_particle_position_update, PH_G2P)
 CTX_PHASE(mpm_grid_momentum_musl, PH_MUSL)
+CTX_PHASE(mpm_smooth_stress_by_grid, PH_SMOOTH)
 #undef CTX_PHASE
 
 extern "C" int mpm_calc_energy(mpm_ctx* ctx, double* ek, double* ei, double* mom3) {

@@ -46,6 +46,10 @@
 #define XS_FORCE 4    // FXg (SGMP: Cfext, Cfint)
 #define XS_SGMP 8     // to the auxiliary nodes
 
+// SmoothStressbyGrid node passes (launch_smooth_nodes)
+#define SM_ZERO 1     // clear the sums and flags
+#define SM_NODES 2    // nodal pressure (SGMP: through the cell centres, and back to them)
+
 // all material models (general stress kernel)
 #define MM_ALL 0xffff
 // flag bit of the material mask: the neo-Hookean ln J is the correctly
@@ -126,4 +130,14 @@ void launch_x_stress(int shape, int move, const PoolPtrs<double>& P, const XPtrs
                      hipStream_t s);
 void launch_x_g2p(int shape, int move, const PoolPtrs<double>& P, const XPtrs& X, const LaunchP& L, hipStream_t s);
 void launch_x_sgmp_stress(int move, const PoolPtrs<double>& P, const LaunchP& L, hipStream_t s);
+// SmoothStressbyGrid (smooth.inc; exact mode: exact.inc)
+template <typename T> void launch_smooth_save_x(const PoolPtrs<T>& P, const SmPtrs& S, const LaunchP& L, hipStream_t s);
+template <typename T>
+void launch_smooth_nodes(int shape, int what, const PoolPtrs<T>& P, const SmPtrs& S, const LaunchP& N, hipStream_t s);
+template <typename T>
+void launch_smooth(int shape, const PoolPtrs<T>& P, const SmPtrs& S, const LaunchP& L, const LaunchP& N,
+                   hipStream_t s);
+void launch_x_pairs_smooth(int shape, const PoolPtrs<double>& P, const XPtrs& X, const LaunchP& L, hipStream_t s);
+void launch_x_smooth(int shape, const PoolPtrs<double>& P, const XPtrs& X, const SmPtrs& S, const int* sval,
+                     const LaunchP& L, const LaunchP& N, hipStream_t s);
 }  // namespace mpmk

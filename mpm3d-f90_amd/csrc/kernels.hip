@@ -3234,6 +3234,7 @@ __global__ void __launch_bounds__(256) k_admit(PoolPtrs<T> P, const double* ps, 
   d[D_DMG * np + s] = F(PF_DMG);
   d[D_LT * np + s] = F(PF_LT);
   d[D_FAIL * np + s] = F(PF_FAIL);
+  d[D_SPRE * np + s] = F(PF_SPRE);
   P.i[I_BODY * np + s] = in ? body0 + pbody[j] : body0;
   P.i[I_ORIG * np + s] = (int)j;
   P.i[I_CELL * np + s] = -1;
@@ -3659,6 +3660,8 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
 }
 #endif
 
+#include "smooth.inc"
+
 #if MPMK_PART != 2
 #include "exact.inc"
 #endif
@@ -3681,7 +3684,10 @@ void launch_sort_keys(const double* X, long np, const InstC* inst, const LaunchP
   template void launch_sgmp_nodes<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);       \
   template void launch_sgmp_cells<T>(int, const PoolPtrs<T>&, const LaunchP&, hipStream_t);                     \
   template void launch_admit<T>(const PoolPtrs<T>&, const double*, const int*, int, long, long, int, hipStream_t); \
-  template void launch_pack_subset<T>(const PoolPtrs<T>&, int, long, const int*, int, double*, hipStream_t);
+  template void launch_pack_subset<T>(const PoolPtrs<T>&, int, long, const int*, int, double*, hipStream_t); \
+  template void launch_smooth_save_x<T>(const PoolPtrs<T>&, const SmPtrs&, const LaunchP&, hipStream_t);    \
+  template void launch_smooth_nodes<T>(int, int, const PoolPtrs<T>&, const SmPtrs&, const LaunchP&, hipStream_t); \
+  template void launch_smooth<T>(int, const PoolPtrs<T>&, const SmPtrs&, const LaunchP&, const LaunchP&, hipStream_t);
 #if MPMK_PART != 2
 INST(double)
 template void launch_permute<int>(int*, const int*, const int*, const LaunchP&, long, int, hipStream_t);

@@ -77,6 +77,8 @@ program mpm3d_host
      if (.not. usf) call mpm_check(mpm_particle_stress_update(ctx), 'ParticleStressUpdate')
      call mpm_check(mpm_calc_energy(ctx, ek, ei, mom), 'calcEnergy')            ! :112
      call mpm_check(mpm_update_dt(ctx, dt), 'UpdateDT')                         ! :113
+     if (prob%smooth_stress /= 0) &
+        call mpm_check(mpm_smooth_stress_by_grid(ctx), 'SmoothStressbyGrid')     ! :114
      call mpm_check(mpm_get_scalars(ctx, sc), 'get_scalars')
      if (nsteps > 0) then
         if (istep >= nsteps) exit

@@ -19,7 +19,7 @@ module mpm_hip
                         PF_SEQV = 12, PF_SDXX = 13, PF_SDYY = 14, PF_SDZZ = 15, PF_SDYZ = 16, &
                         PF_SDXZ = 17, PF_SDXY = 18, PF_SIGY = 19, PF_EPEFF = 20, PF_IE = 21, &
                         PF_Q = 22, PF_CP = 23, PF_FP = 24, PF_XX = 33, PF_TEMP = 36, PF_DMG = 37, &
-                        PF_LT = 38, PF_FAIL = 39, PF_COUNT = 40
+                        PF_LT = 38, PF_FAIL = 39, PF_SPRE = 40, PF_COUNT = 41
 
   type, bind(C) :: mpm_material_t
      integer(c_int32_t) :: mtype, eos_type
@@ -43,6 +43,8 @@ module mpm_hip
      real(c_double) :: span(6), dcell, cutoff, dt, dtscale, endtime
      real(c_double) :: quasi_load_time, damp_factor, bq1, bq2
      type(c_ptr) :: mats, bodies, pstate, pbody
+     integer(c_int32_t) :: smooth_stress, tlmpm, contact, normbody, nb_component, reserved0
+     real(c_double) :: fricfa
   end type mpm_problem_t
 
   interface
@@ -102,6 +104,12 @@ module mpm_hip
        type(c_ptr), value :: ctx
        integer(c_int) :: rc
      end function mpm_particle_stress_update
+
+     function mpm_smooth_stress_by_grid(ctx) bind(C, name="mpm_smooth_stress_by_grid") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: rc
+     end function mpm_smooth_stress_by_grid
 
      function mpm_grid_momentum_update(ctx) bind(C, name="mpm_grid_momentum_update") result(rc)
        import :: c_int, c_ptr

@@ -23,9 +23,13 @@
 ! MPM3D_DUMP=<file> writes the particle state after every step (stream,
 ! n, istep, 8 scalars, PF_COUNT x n doubles).
 !
+! SmoothStressbyGrid, which the driver calls after UpdateDT, smooths on the
+! device and, on output steps, refreshes the particles' Spre for the
+! ParaView writer.
+!
 ! Compiles against the reference's module files (ParticleData, GridData,
-! MaterialData); total-Lagrangian, contact and stress-smoothing routines are
-! present only to satisfy the link and stop when called.
+! MaterialData); total-Lagrangian and contact routines are present only to
+! satisfy the link and stop when called.
 ! ---------------------------------------------------------------------------
 module update_step_hip_state
   use, intrinsic :: iso_c_binding
@@ -43,6 +47,8 @@ module update_step_hip_state
   logical, save :: plt_started = .false.
   real(c_double), allocatable, target, save :: sub(:)
   integer(c_int32_t), allocatable, target, save :: icell_buf(:)
+  ! the last UpdateDT refreshed the whole particle view (an output step)
+  logical, save :: full_view = .false.
 
 contains
 
@@ -83,6 +89,13 @@ contains
     prob%fix = FixS
     prob%ncx = NumCellx; prob%ncy = NumCelly; prob%ncz = NumCellz
     prob%dr_damping = merge(1, 0, DR_DAMPING)
+    prob%smooth_stress = merge(1, 0, SmoothStress)
+    prob%nb_component = nb_component
+    prob%tlmpm = 0
+    prob%contact = 0
+    prob%normbody = 0
+    prob%reserved0 = 0
+    prob%fricfa = 0d0
     prob%span = [SpanX(1), SpanX(2), SpanY(1), SpanY(2), SpanZ(1), SpanZ(2)]
     prob%dcell = DCell
     prob%cutoff = CutOff
@@ -199,6 +212,7 @@ contains
          ps(PF_DMG * n + i) = p%DMG
          ps(PF_LT * n + i) = p%LT
          ps(PF_FAIL * n + i) = merge(1d0, 0d0, p%failure)
+         ps(PF_SPRE * n + i) = p%Spre
        end associate
     end do
   end subroutine pack_particles
@@ -232,6 +246,7 @@ contains
          p%celsius_t = ps(PF_TEMP * n + i)
          p%DMG = ps(PF_DMG * n + i)
          p%failure = ps(PF_FAIL * n + i) /= 0d0
+         p%Spre = ps(PF_SPRE * n + i)
        end associate
     end do
   end subroutine unpack_particles
@@ -253,8 +268,10 @@ contains
        plt_started = .true.
     end if
     call get_environment_variable('MPM3D_DUMP', f)
+    full_view = .false.
     if (len_trim(f) > 0 .or. ((WriteTecPlot .or. WriteParaView) .and. CurrentTime >= plt_mirror)) then
        if ((WriteTecPlot .or. WriteParaView) .and. CurrentTime >= plt_mirror) plt_mirror = plt_mirror + OutTime
+       full_view = .true.
        call unpack_particles()
        n = nb_particle
        if (.not. allocated(icell_buf)) allocate(icell_buf(n))
@@ -384,10 +401,23 @@ subroutine UpdateDT()
   call maybe_dump()
 end subroutine UpdateDT
 
-! --- routines of the reference outside the device path (link only) --------
 subroutine SmoothStressbyGrid()
-  stop 'update_step_hip: stress smoothing (smoo) is not available in the device build'
+  use update_step_hip_state
+  use ParticleData, only: particle_list, nb_particle
+  implicit none
+  integer :: i, n
+  call mpm_check(mpm_smooth_stress_by_grid(ctx), 'SmoothStressbyGrid')
+  if (.not. full_view) return
+  ! an output step: the writers read Spre of every particle
+  n = nb_particle
+  call mpm_check(mpm_download_particles(ctx, ps), 'mpm_download_particles')
+  do i = 1, n
+     particle_list(i)%Spre = ps(PF_SPRE * n + i)
+  end do
+  call maybe_dump()
 end subroutine SmoothStressbyGrid
+
+! --- routines of the reference outside the device path (link only) --------
 subroutine TLSmoothStressbyGrid()
   stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
 end subroutine TLSmoothStressbyGrid

@@ -26,8 +26,8 @@ end time writing every file; ``ResultWriter`` is the writer alone, for hosts
 with their own loop.  Particle state comes from the device: per step only
 the curve particles (mpm_download_particle_subset), everything at frames.
 
-Temperature, damage and the failure flag are particle state (PF temp, dmg,
-fail); the smoothed pressure (smoo) is not computed and prints 0.  TecPlot
+Temperature, damage, the failure flag and the smoothed pressure (smoo) are
+particle state (PF temp, dmg, fail, spre; spre stays 0 without smoo).  TecPlot
 frames leave out the particles pt2d marks (SkipThis, DataIn.f90:867-880).
 One difference: a curve of option 12 (fail) writes the logical with an I
 edit descriptor, which the flang-built reference refuses at run time; here it
@@ -273,9 +273,8 @@ class ResultWriter:
         w.append(i3 + "</DataArray>\n" + i2 + "</Points>\n")
         w.append(i2 + '<PointData Scalars="scalars">\n')
         g = lambda k: ps[PF[k]]
-        zero = np.zeros(n)
-        fields = [("seqv", g("seqv")), ("epeff", g("epeff")), ("pressure", -g("sm")), ("Smoothpressure", zero),
-                  ("MeanNormalStress", g("sm")), ("SmoothMeanNormalStress", -zero), ("vx", g("vx")),
+        fields = [("seqv", g("seqv")), ("epeff", g("epeff")), ("pressure", -g("sm")), ("Smoothpressure", g("spre")),
+                  ("MeanNormalStress", g("sm")), ("SmoothMeanNormalStress", -g("spre")), ("vx", g("vx")),
                   ("vy", g("vy")), ("vz", g("vz")), ("disx", x - self.xo[0]), ("disy", y - self.xo[1]),
                   ("disz", z - self.xo[2]), ("fxpx", g("fx")), ("fxpy", g("fy")), ("fxpz", g("fz")),
                   ("vol", g("vol")), ("celsius_t", g("temp")), ("damage", g("dmg")),

@@ -15,7 +15,8 @@ from ._lib import PF, PF_COUNT, ProblemStruct, Material, Body
 
 SCALARS = ("shape", "scheme", "quasi_load", "quasi_damp", "gravity", "update_v_by_fp",
            "ncx", "ncy", "ncz", "dr_damping", "dcell", "cutoff", "dt", "dtscale", "endtime",
-           "quasi_load_time", "damp_factor", "bq1", "bq2")
+           "quasi_load_time", "damp_factor", "bq1", "bq2", "smooth_stress", "tlmpm", "contact", "normbody",
+           "nb_component", "fricfa")
 
 
 class Problem:

@@ -1565,6 +1565,96 @@ void oracle_update_dt(oracle* o) {
   dr_damping(o);
 }
 
+/* SmoothStressbyGrid (update_step.f90:1876-2055): -SM V and V spread to
+ * the grid (every in-range node a particle's list reaches is "mapped"),
+ * Gpre = sum / sum at mapped nodes, then gathered back to Spre.  The
+ * shape functions are those of the later phases: linear from the XN of
+ * this step's ParticleInitial, GIMP and B-spline from the particle's current
+ * position with this step's cell, SGMP from XXN and centericell.  SGMP
+ * spreads through the auxiliary nodes (every cell's 8 nodes are mapped) and
+ * gathers from the cell centres, each the mean of its nodes with Mg >
+ * CutOff.  Spre only feeds the output writers. */
+void oracle_smooth_stress(oracle* o) {
+  const int nn = o->nnode;
+  double* gpre = (double*)calloc(nn, sizeof(double));
+  double* gv = (double*)calloc(nn, sizeof(double));
+  char* mapped = (char*)calloc(nn, 1);
+  double *cgpre = NULL, *cgv = NULL;
+  int cn[8];
+  double shp[8];
+  shape_t s;
+  if (o->sgmp) {
+    cgpre = (double*)calloc(o->ncell, sizeof(double));
+    cgv = (double*)calloc(o->ncell, sizeof(double));
+  }
+  for (int p = 0; p < o->n; p++) {
+    const double msm = -1 * P(o, PF_SM, p), vol = P(o, PF_VOL, p);
+    if (o->sgmp) {
+      if (o->cicell[p] < 0) continue;
+      center_cell_nodes(o, o->cicell[p], cn);
+      sgn_shape(&o->xxn[3 * p], shp);
+      for (int i = 0; i < 8; i++) {
+        if (!in_aux(o, cn[i])) continue;
+        const int k = cn[i] - 1;
+        cgpre[k] = cgpre[k] + shp[i] * msm * vol;
+        cgv[k] = cgv[k] + shp[i] * vol;
+      }
+    } else {
+      if (o->icell[p] < 0) continue;
+      shape_of(o, p, &s);
+      for (int i = 0; i < s.nb; i++) {
+        if (!in_grid(o, s.node[i])) continue;
+        const int k = s.node[i] - 1;
+        mapped[k] = 1;
+        gpre[k] = gpre[k] + s.shp[i] * msm * vol;
+        gv[k] = gv[k] + s.shp[i] * vol;
+      }
+    }
+  }
+  if (o->sgmp)
+    for (int c = 1; c <= o->ncell; c++) {
+      cell_nodes(o, c, cn);
+      for (int j = 0; j < 8; j++) {
+        const int k = cn[j] - 1;
+        mapped[k] = 1;
+        gpre[k] = gpre[k] + 0.125 * cgpre[c - 1];
+        gv[k] = gv[k] + 0.125 * cgv[c - 1];
+      }
+    }
+  for (int k = 0; k < nn; k++)
+    if (mapped[k]) gpre[k] = gpre[k] / gv[k];
+  if (o->sgmp)
+    for (int c = 1; c <= o->ncell; c++) {
+      cgpre[c - 1] = 0.0;
+      cell_nodes(o, c, cn);
+      for (int j = 0; j < 8; j++) {
+        const int k = cn[j] - 1;
+        if (G(o, GF_MG, k) > o->prob.cutoff) cgpre[c - 1] = cgpre[c - 1] + 0.125 * gpre[k];
+      }
+    }
+  for (int p = 0; p < o->n; p++) {
+    double sp = 0.0;
+    if (o->sgmp) {
+      if (o->cicell[p] >= 0) {
+        center_cell_nodes(o, o->cicell[p], cn);
+        sgn_shape(&o->xxn[3 * p], shp);
+        for (int i = 0; i < 8; i++)
+          if (in_aux(o, cn[i])) sp = sp + shp[i] * cgpre[cn[i] - 1];
+      }
+    } else if (o->icell[p] >= 0) {
+      shape_of(o, p, &s);
+      for (int i = 0; i < s.nb; i++)
+        if (in_grid(o, s.node[i])) sp = sp + s.shp[i] * gpre[s.node[i] - 1];
+    }
+    P(o, PF_SPRE, p) = sp;
+  }
+  free(gpre);
+  free(gv);
+  free(mapped);
+  free(cgpre);
+  free(cgv);
+}
+
 void oracle_step(oracle* o, int nsteps) {
   for (int k = 0; k < nsteps; k++) {
     oracle_begin_step(o);
@@ -1583,6 +1673,7 @@ void oracle_step(oracle* o, int nsteps) {
     if (o->prob.scheme != MPM_USF) oracle_particle_stress_update(o);
     oracle_calc_energy(o);
     oracle_update_dt(o);
+    if (o->prob.smooth_stress) oracle_smooth_stress(o);
   }
 }
 

@@ -39,6 +39,7 @@ void oracle_particle_position_update(oracle* o);  /* :1378-1526 */
 void oracle_grid_momentum_musl(oracle* o);        /* :1668-1793 */
 void oracle_calc_energy(oracle* o);               /* Particle.f90:204-230 */
 void oracle_update_dt(oracle* o);                 /* update_step.f90:1833-1874 */
+void oracle_smooth_stress(oracle* o);             /* :1876-2055 SmoothStressbyGrid (Spre) */
 /* whole steps in the order of MPM3D.f90:69-113 */
 void oracle_step(oracle* o, int nsteps);
 

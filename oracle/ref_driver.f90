@@ -7,10 +7,9 @@
 ! [USF: ApplyBoundaryConditions, ParticleStressUpdate], GridMomentumUpdate,
 ! IntegrateMomentum, ParticlePositionUpdate, [MUSL: GridMomentumMUSL,
 ! ApplyBoundaryConditions], [USL/MUSL: ParticleStressUpdate], calcEnergy,
-! UpdateDT, [drda: dynamicRelaxationDamping]) without the per-step file
-! output (and without SmoothStressbyGrid, whose smoothed pressure only the
-! writers read), and writes the complete
-! particle and grid state as little-endian binary after chosen steps.
+! UpdateDT, [smoo: SmoothStressbyGrid], [drda: dynamicRelaxationDamping])
+! without the per-step file output, and writes the complete particle and
+! grid state as little-endian binary after chosen steps.
 !
 ! usage:  ref_driver <deck> <nsteps> <outprefix> <dumpsteps> [phasestep]
 !   <deck>       .mpm input deck (read by the reference's InputPara)
@@ -39,7 +38,7 @@ program ref_driver
 
   external GridMomentumInitial, ParticleStressUpdate, GridMomentumUpdate, &
            IntegrateMomentum, ParticlePositionUpdate, GridMomentumMUSL, &
-           ApplyBoundaryConditions, UpdateDT
+           ApplyBoundaryConditions, UpdateDT, SmoothStressbyGrid
 
   if (command_argument_count() < 4) then
      print *, 'usage: ref_driver <deck> <nsteps> <outprefix> <dumpsteps> [phasestep]'
@@ -105,6 +104,7 @@ program ref_driver
      end if
      call calcEnergy()
      call UpdateDT()
+     if (SmoothStress) call SmoothStressbyGrid()
      if (DR_DAMPING) call dynamicRelaxationDamping()
      if (dophase) call dump_state(istep, 8)
 
@@ -133,7 +133,7 @@ contains
 
     hdr = 0
     hdr(1) = 1146113101          ! 'MPMD' tag
-    hdr(2) = 2                   ! layout version
+    hdr(2) = 3                   ! layout version
     hdr(3) = nb_particle
     hdr(4) = nb_gridnode
     hdr(5) = istep
@@ -179,6 +179,7 @@ contains
     write(u) (particle_list(i)%DMG, i = 1, nb_particle)
     write(u) (particle_list(i)%LT, i = 1, nb_particle)
     write(u) (merge(1d0, 0d0, particle_list(i)%failure), i = 1, nb_particle)
+    write(u) (particle_list(i)%Spre, i = 1, nb_particle)
     do j = 1, 3          ! Fp(i,j) column-major: (1,1),(2,1),(3,1),(1,2),...
        do n = 1, 3
           write(u) (particle_list(i)%Fp(n, j), i = 1, nb_particle)

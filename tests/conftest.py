@@ -44,10 +44,13 @@ DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
          # (B-spline impact), HE burn with the JWL EOS and a detonation point,
          # and dynamic relaxation damping
          "taylor_sjc_eos2_musl", "cube_sjc_eos1_gimp_usl", "impact_sjcf_eos2_bspline_usf", "tnt_hiex_jwl_usl",
-         "cube_drda_linear_usf"]
+         "cube_drda_linear_usf",
+         # SmoothStressbyGrid (smoo) on every shape family; the cube also
+         # carries two grid components without contact
+         "cube_smoo_linear_usl", "bar_smoo_gimp_usf", "slope_smoo_bspline_musl"]
 # auxiliary-grid (SGMP) decks: clamped plate (MUSL), water column with
 # symmetric walls (MUSL), J2 block on a clamped floor (USF)
-SGMP_DECKS = ["plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf"]
+SGMP_DECKS = ["plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf", "block_smoo_sgmp_usf"]
 ALL_DECKS = DECKS + SGMP_DECKS
 
 
@@ -104,6 +107,7 @@ PGROUPS = {
     "epeff": ["epeff"],
     "ie": ["ie"],
     "fp": ["fp%d%d" % (i, j) for i in (1, 2, 3) for j in (1, 2, 3)],
+    "spre": ["spre"],
 }
 GGROUPS = {"mg": ["mg"], "p": ["px", "py", "pz"], "f": ["fx", "fy", "fz"]}
 

@@ -35,7 +35,8 @@ DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
          "cube_elastic_linear_usf", "bar_j2_gimp_usf", "slope_dp_bspline_musl", "slope_dp_bspline_usl",
          "block_dp_bspline_shear_usf", "block_dp_bspline_pull_musl",
          "taylor_sjc_eos2_musl", "cube_sjc_eos1_gimp_usl", "impact_sjcf_eos2_bspline_usf", "tnt_hiex_jwl_usl",
-         "cube_drda_linear_usf"]
+         "cube_drda_linear_usf", "cube_smoo_linear_usl", "bar_smoo_gimp_usf", "slope_smoo_bspline_musl",
+         "block_smoo_sgmp_usf"]
 S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp", "temp", "lt"]
 
 

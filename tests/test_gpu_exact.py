@@ -135,6 +135,10 @@ def test_exact_each_phase_entry_point_bitwise(mpm, oracle_mod, name):
         ctx.calcEnergy()
         o.phase("calc_energy")
         assert ctx.UpdateDT() == (o.phase("update_dt") or o.scalars()[2])
+        if prob.smooth_stress:
+            ctx.SmoothStressbyGrid()
+            o.phase("smooth_stress")
+            assert_bitwise(mpm, ctx.particles(), o.particles(), what="%s step %d smoothed" % (name, step))
     ctx.close()
 
 

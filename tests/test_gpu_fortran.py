@@ -59,7 +59,7 @@ def test_fortran_host_matches_oracle(oracle_mod, name):
 
 
 @pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "dam_null_sgmp_musl",
-                                  "block_j2_sgmp_usf"])
+                                  "block_j2_sgmp_usf", "slope_smoo_bspline_musl", "block_smoo_sgmp_usf"])
 def test_reference_driver_runs_the_device_step(oracle_mod, name):
     """The reference's MPM3D.f90 loop, unchanged, over update_step_hip."""
     if not os.path.exists(DROPIN):
@@ -95,7 +95,8 @@ def test_reference_driver_runs_the_device_step(oracle_mod, name):
 REFPROG = os.path.join(ROOT, "oracle", "_ref", "mpm3d_ref")
 
 
-@pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "block_dp_bspline_pull_musl", "dam_null_sgmp_musl"])
+@pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "block_dp_bspline_pull_musl", "dam_null_sgmp_musl",
+                                  "cube_smoo_linear_usl"])
 def test_dropin_writes_the_reference_output_files(name):
     """The reference program itself (oracle/_ref/mpm3d_ref: MPM3D.f90 with its
     own update_step.f90, on the CPU) and the drop-in (the same main program
@@ -104,7 +105,9 @@ def test_dropin_writes_the_reference_output_files(name):
     particles per step and everything at the output times; its TecPlot file
     is the reference's, character for character, and so is its curve file
     but for the energy columns (the device's block sums against the
-    reference's serial sums), which agree to the printed precision."""
+    reference's serial sums), which agree to the printed precision.  On the
+    smoothing deck the ParaView frames (with the smoothed pressure the driver's
+    SmoothStressbyGrid call leaves) are the reference's too."""
     if not (os.path.exists(DROPIN) and os.path.exists(REFPROG)):
         pytest.skip("reference program / drop-in not built (needs the reference sources at build time)")
     mpm = load_package()
@@ -113,6 +116,8 @@ def test_dropin_writes_the_reference_output_files(name):
     ids = [5, n // 3, n - 1, 7]
     extra = ("outt %.9e\ntecp\noutr seqv\noutr velz\noutr epef\ncurv seqv %d\ncurv velz %d\ncurv epef %d\n"
              "curv engk\ncurv volu %d\n" % ((10.5 * dt,) + tuple(ids)))
+    if prob.smooth_stress:
+        extra += "para\n"
     text = open(deck_path(name)).read()
     text = re.sub(r"(?mi)^endtime .*$", "endtime %.9e" % (40.5 * dt), text)
     text = re.sub(r"(?mi)^endi\s*$", extra + "endi", text)
@@ -128,8 +133,14 @@ def test_dropin_writes_the_reference_output_files(name):
                                timeout=300)
             assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
             out[prog] = {k: open(os.path.join(d, name + k)).read().splitlines() for k in ("_anim.dat", "_curv.dat")}
+            out[prog].update({f: open(os.path.join(d, f)).read() for f in os.listdir(d) if f.endswith(".vtu")})
         ref, dev = out[REFPROG], out[DROPIN]
         assert len(ref["_anim.dat"]) > 4 * n and ref["_anim.dat"] == dev["_anim.dat"]
+        vtu = sorted(f for f in ref if f.endswith(".vtu"))
+        assert vtu == sorted(f for f in dev if f.endswith(".vtu"))
+        assert len(vtu) >= (4 if prob.smooth_stress else 0)
+        for f in vtu:
+            assert ref[f] == dev[f], f
         rc, dc = ref["_curv.dat"], dev["_curv.dat"]
         assert len(rc) >= 40 and len(rc) == len(dc) and rc[0] == dc[0]
         # energy columns (the reference's two default curves and ours), by header name

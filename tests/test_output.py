@@ -133,6 +133,9 @@ OPTION_SETS = {
     # failure flags and pressures of the failing impact, burn of the TNT bar
     "impact_sjcf_eos2_bspline_usf": "tecp\noutr fail\noutr pres\noutr epef\ncurv pres 10\ncurv epef 100\n",
     "tnt_hiex_jwl_usl": "tecp\npara\noutr pres\noutr velx\ncurv pres 5\ncurv cels 5\ncurv volu 9\n",
+    # the smoothed pressure (smoo) in the ParaView frames: B-spline and SGMP
+    "slope_smoo_bspline_musl": "para\noutr pres\ncurv pres 100\n",
+    "block_smoo_sgmp_usf": "tecp\npara\noutr pres\ncurv pres 7\n",
 }
 
 
