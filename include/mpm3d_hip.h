@@ -98,6 +98,11 @@ int mpm_particle_stress_update(mpm_ctx* ctx);
 int mpm_grid_momentum_update(mpm_ctx* ctx);
 /* IntegrateMomentum (update_step.f90:1194-1239) */
 int mpm_integrate_momentum(mpm_ctx* ctx);
+/* Lagr_NodContact (update_step.f90:1241-1376, called after IntegrateMomentum
+ * when the deck has 'cont lagr', MPM3D.f90:94-96): contact forces between
+ * grid components 1 and 2; a no-op without contact.  Contact problems run in
+ * MPM_PREC_FP64_EXACT (mpm_ctx_create refuses them in the other modes). */
+int mpm_lagr_nod_contact(mpm_ctx* ctx);
 /* ParticlePositionUpdate (update_step.f90:1378-1526) */
 int mpm_particle_position_update(mpm_ctx* ctx);
 /* GridMomentumMUSL (update_step.f90:1668-1793) */
@@ -162,6 +167,12 @@ int mpm_download_particles(mpm_ctx* ctx, double* pstate);
 int mpm_download_particle_subset(mpm_ctx* ctx, int k, const int32_t* ids, double* out);
 /* gstate: GF_COUNT * nb_node doubles, node order of the reference */
 int mpm_download_grid(mpm_ctx* ctx, double* gstate);
+/* the grid of component comp (0-based; 0 is what mpm_download_grid gives) of
+ * a contact problem, same layout */
+int mpm_download_grid_component(mpm_ctx* ctx, int comp, double* gstate);
+/* tot_cont_for, the contact force of the last Lagr_NodContact summed over the
+ * nodes (DataOut.f90:148-151 writes it every step); 0 without contact */
+int mpm_get_contact_force(mpm_ctx* ctx, double* out3);
 /* icell: nb_particle, 1-based cell index of the last GridMomentumInitial, -1 outside */
 int mpm_download_icell(mpm_ctx* ctx, int32_t* icell);
 /* overwrite particle state (e.g. a host-side restart); same layout as download */

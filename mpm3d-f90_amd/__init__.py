@@ -90,6 +90,9 @@ class Context:
     def IntegrateMomentum(self):
         self._call("mpm_integrate_momentum")
 
+    def LagrNodContact(self):
+        self._call("mpm_lagr_nod_contact")
+
     def ParticlePositionUpdate(self):
         self._call("mpm_particle_position_update")
 
@@ -125,6 +128,8 @@ class Context:
                 self.ParticleStressUpdate()
             self.GridMomentumUpdate()
             self.IntegrateMomentum()
+            if self.problem.contact == 1:
+                self.LagrNodContact()
             self.ParticlePositionUpdate()
             if scheme == SCHEMES["musl"]:
                 self.GridMomentumMUSL()
@@ -158,6 +163,18 @@ class Context:
     def grid(self):
         out = np.empty((GF_COUNT, self.nb_node))
         check(_lib.lib().mpm_download_grid(self._h, dptr(out)), "mpm_download_grid")
+        return out
+
+    def grid_component(self, comp):
+        """grid of component comp (0-based) of a contact problem"""
+        out = np.empty((GF_COUNT, self.nb_node))
+        check(_lib.lib().mpm_download_grid_component(self._h, int(comp), dptr(out)), "mpm_download_grid_component")
+        return out
+
+    def contact_force(self):
+        """tot_cont_for of the last contact pass"""
+        out = np.zeros(3)
+        check(_lib.lib().mpm_get_contact_force(self._h, dptr(out)), "mpm_get_contact_force")
         return out
 
     def icell(self):

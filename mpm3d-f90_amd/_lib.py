@@ -124,6 +124,9 @@ def lib():
         "mpm_particle_position_update": (C.c_int, [vp]),
         "mpm_grid_momentum_musl": (C.c_int, [vp]),
         "mpm_smooth_stress_by_grid": (C.c_int, [vp]),
+        "mpm_lagr_nod_contact": (C.c_int, [vp]),
+        "mpm_download_grid_component": (C.c_int, [vp, C.c_int, P(C.c_double)]),
+        "mpm_get_contact_force": (C.c_int, [vp, P(C.c_double)]),
         "mpm_calc_energy": (C.c_int, [vp, P(C.c_double), P(C.c_double), P(C.c_double)]),
         "mpm_update_dt": (C.c_int, [vp, P(C.c_double)]),
         "mpm_step": (C.c_int, [vp, C.c_int]),

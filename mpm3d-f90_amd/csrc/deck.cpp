@@ -198,6 +198,9 @@ struct Deck {
   double qlt = 0, damp = 0, bq1 = 1.5, bq2 = (double)0.06f;
   bool dr_damping = false;
   bool smooth_stress = false;   // smoo: SmoothStressbyGrid every step (Spre, read by the ParaView writer)
+  int contact = 0;              // cont: Contact_type (1 lagr: Lagr_NodContact; 2 others: no contact force)
+  double fricfa = 0;            // friction coefficient
+  int normbody = 0;             // contact normal from both bodies (0), component 1 (1) or 2 (2)
   int n_deto = 0;
   double deto[10][3] = {};
   int fix[6] = {0, 0, 0, 0, 0, 0};
@@ -642,7 +645,14 @@ struct Deck {
         case 27: outs.tecplot = 1; break;
         case 28: bq1 = tk.get_real(); bq2 = tk.get_real(); break;
         case 29: gimp = true; break;
-        case 30: fail("contact ('cont') is outside the explicit single-body step");
+        case 30: {   // Setcontact (DataIn.f90:1348-1393): lagr (Bardenhagen) or others, friction, normal
+          static const char* ck[] = {"lagr", "othe"};
+          contact = tk.keyword(ck, 2);
+          if (contact <= 0) fail("Error set contact parameter (lagr/pena)");
+          fricfa = tk.get_real();
+          normbody = tk.get_int();
+          break;
+        }
         case 31: usf = on_off(); if (usf) update_method = true; break;
         case 32: usl = on_off(); if (usl) update_method = true; break;
         case 33:
@@ -737,6 +747,21 @@ extern "C" int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, 
     p->dr_damping = d.dr_damping;
     p->smooth_stress = d.smooth_stress;
     p->nb_component = d.nb_component;
+    p->contact = d.contact;
+    p->fricfa = d.fricfa;
+    p->normbody = d.normbody;
+    if (d.contact) {
+      // the reference leaves these cases undefined: SGMP's cell-centre passes
+      // share one auxiliary grid between the components (comID keeps the
+      // last body's value there), Lagr_NodContact reads components 1 and 2,
+      // and another normbody leaves the normal unset
+      if (d.sgmp) d.fail("contact with SGMP is not supported");
+      if (d.nb_component < 2) d.fail("contact needs nbco >= 2");
+      if (d.normbody < 0 || d.normbody > 2) d.fail("contact normal (normbody) must be 0, 1 or 2");
+      for (int b = 0; b < d.nb_body; b++)
+        if (d.bodies[b].com < 1 || d.bodies[b].com > d.nb_component)
+          d.fail("body " + std::to_string(b + 1) + ": component outside 1..nbco");
+    }
     p->mats = (mpm_material*)calloc(d.nb_mat, sizeof(mpm_material));
     p->bodies = (mpm_body*)calloc(d.nb_body, sizeof(mpm_body));
     for (int m = 0; m < d.nb_mat; m++) p->mats[m] = d.mats[m];

@@ -29,6 +29,11 @@ struct InstC {
   int max_steps;         // > 0: the instance stops itself after this many steps
   int cncx, cncxy, cncell;   // SGMP auxiliary cells: CenterNumCellx, ..xy, CenterNumCell
   int fixs[6];           // FixS of the deck (x0 xn y0 yn z0 zn)
+  int ncomp;             // grid components (contact: nbco, else 1); component c's node k at c * nnode + k
+  int nnode_all;         // ncomp * nnode: the instance's node range
+  int contact;           // Contact_type: 1 Lagr_NodContact after IntegrateMomentum, 2 components only
+  int normbody;          // contact normal: 0 both bodies, 1 component 1, 2 component 2
+  double fricfa;         // contact friction coefficient
   double span0[3];       // SpanX(1), SpanY(1), SpanZ(1)
   double span1[3];       // adjusted upper ends
   double dcell, idcell, ijac, ijac4, cutoff;
@@ -69,6 +74,7 @@ struct InstT {
   int integ_pending;     // the last step's g2p integrated on the fly: PXg, FXg are not integrated yet
   int integ_half;        //   that step was the first (half step)
   double integ_dt;       //   and its DT
+  double cont_force[3];  // tot_cont_for of the last contact pass
 };
 
 // per-material constants, all derived exactly as the reference derives them
@@ -85,7 +91,7 @@ struct MatC {
 
 struct BodyC {
   int mat;               // global material index
-  int pad;
+  int com;               // grid component of the body, 0-based (0 without contact)
   double grav[3];
 };
 
@@ -126,6 +132,8 @@ struct XPtrs {
   double* rw;                // 4 * m: N, dN/dx, dN/dy, dN/dz of entry q (field-major, stride m)
   int* seg0;                 // nn: first sorted entry of every pool node
   int* seg1;                 // nn: one past its last
+  double* ndir;              // 3 * nn: contact normals (mass gradients) of every pool node (contact only)
+  double* cf;                // 3 * nn: contact force of every node of component 1 (contact only)
   long m;                    // entries: 27 * np
   int obits;                 // key bits below the node index
 };

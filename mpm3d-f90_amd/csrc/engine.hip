@@ -135,6 +135,7 @@ struct Group {
   int matmask = 0;   // material models of the group's instances (bits 1 << type)
   bool drda = false; // some instance damps by dynamic relaxation
   bool smooth = false; // some instance smooths its stress by the grid (smoo)
+  bool contact = false; // some instance has contact between grid components (Lagr_NodContact)
   std::vector<int> insts;
   std::vector<int> pgran, ngran;
   int* d_pgran = nullptr;
@@ -240,6 +241,8 @@ struct mpm_engine {
   double* x_rw = nullptr;
   int* x_seg0 = nullptr;
   int* x_seg1 = nullptr;
+  double* x_ndir = nullptr;       // contact normals (allocated with the first contact instance)
+  double* x_cf = nullptr;         // contact force per node
   unsigned* x_scr = nullptr;
   long x_m = 0;
   int x_obits = 0, x_nbits = 0;
@@ -297,7 +300,7 @@ static void free_engine(mpm_engine* e) {
                   e->v_out, e->rs_scr, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg, e->boxes,
                   e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1], e->x_rnode, e->x_rw, e->x_seg0, e->x_seg1,
                   e->x_scr, e->d_sub_ids, e->d_sub_rows, e->sm.gpre, e->sm.gv, e->sm.cgpre, e->sm.cgv, e->sm.map,
-                  e->sm.xs};
+                  e->sm.xs, e->x_ndir, e->x_cf};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& g : e->groups) {
@@ -452,6 +455,16 @@ static void check_problem(const mpm_problem* p) {
   for (int b = 0; b < p->nb_body; b++)
     if (p->bodies[b].mat < 1 || p->bodies[b].mat > p->nb_mat) fail(MPM_ERR_ARG, "body %d has no material", b);
   if (p->ncx <= 0 || p->ncy <= 0 || p->ncz <= 0 || !(p->dcell > 0)) fail(MPM_ERR_ARG, "bad grid");
+  if (p->contact) {
+    if (p->contact != 1 && p->contact != 2) fail(MPM_ERR_ARG, "contact type %d", p->contact);
+    if (p->nb_component < 2) fail(MPM_ERR_ARG, "contact needs at least 2 grid components");
+    if (p->normbody < 0 || p->normbody > 2) fail(MPM_ERR_ARG, "contact normal %d", p->normbody);
+    for (int b = 0; b < p->nb_body; b++)
+      if (p->bodies[b].com < 1 || p->bodies[b].com > p->nb_component)
+        fail(MPM_ERR_ARG, "body %d: component %d outside 1..%d", b + 1, p->bodies[b].com, p->nb_component);
+    if (p->shape == MPM_SHAPE_SGMP) fail(MPM_ERR_UNSUPPORTED, "contact with SGMP");
+    if (p->smooth_stress) fail(MPM_ERR_UNSUPPORTED, "stress smoothing (smoo) together with contact");
+  }
   long ncell = (long)p->ncx * p->ncy * p->ncz;
   long nnode = (long)(p->ncx + 1) * (p->ncy + 1) * (p->ncz + 1);
   if (nnode > 0x7fffffffL || ncell > 0x7fffffffL) fail(MPM_ERR_ARG, "grid too large");
@@ -599,13 +612,25 @@ static void ensure_smooth(mpm_engine* e) {
   mark_graph_dirty(e);
 }
 
+static void ensure_contact(mpm_engine* e) {
+  if (e->x_ndir) return;
+  e->x_ndir = dalloc<double>(3 * e->nn);
+  e->x_cf = dalloc<double>(3 * e->nn);
+  mark_graph_dirty(e);
+}
+
 static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   check_problem(p);
+  if (p->contact && !e->exact)
+    fail(MPM_ERR_UNSUPPORTED, "contact between grid components runs in the reference-order FP64 mode "
+                              "(MPM_PREC_FP64_EXACT)");
   if (e->free_ids.empty()) fail(MPM_ERR_NOMEM, "no free instance slot");
   const int n = p->nb_particle;
   const long npg = (n + PGRAN - 1) / PGRAN;
   const long nnode = (long)(p->ncx + 1) * (p->ncy + 1) * (p->ncz + 1);
-  const long nng = (nnode + NGRAN - 1) / NGRAN;
+  const int ncomp = p->contact ? p->nb_component : 1;
+  if (nnode * ncomp > 0x7fffffffL) fail(MPM_ERR_ARG, "grid components too large");
+  const long nng = (nnode * ncomp + NGRAN - 1) / NGRAN;
   long pg0 = e->pa.alloc(npg);
   if (pg0 < 0) fail(MPM_ERR_NOMEM, "particle pool full (%d particles requested)", n);
   long ng0 = e->na.alloc(nng);
@@ -643,6 +668,11 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   c.ngx = c.ncx + 1; c.ngy = c.ncy + 1; c.ngz = c.ncz + 1;
   c.ngxy = c.ngx * c.ngy;
   c.nnode = (int)nnode;
+  c.ncomp = ncomp;
+  c.nnode_all = (int)(nnode * ncomp);
+  c.contact = p->contact;
+  c.normbody = p->normbody;
+  c.fricfa = p->fricfa;
   c.node_off = (int)(ng0 * NGRAN);
   c.p_begin = (int)(pg0 * PGRAN);
   c.p_end = c.p_begin + n;
@@ -667,6 +697,7 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   for (int k = 0; k < 6; k++) c.fixs[k] = p->fix[k];
   if (p->shape == MPM_SHAPE_SGMP) ensure_aux(e);
   if (p->smooth_stress) ensure_smooth(e);
+  if (p->contact == 1) ensure_contact(e);
   for (int k = 0; k < 3; k++) {
     c.span0[k] = p->span[2 * k];
     c.span1[k] = p->span[2 * k + 1];
@@ -688,6 +719,7 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   std::vector<BodyC> bodies(p->nb_body);
   for (int b = 0; b < p->nb_body; b++) {
     bodies[b].mat = (int)m0 + p->bodies[b].mat - 1;
+    bodies[b].com = p->contact ? p->bodies[b].com - 1 : 0;
     for (int k = 0; k < 3; k++) bodies[b].grav[k] = p->bodies[b].grav[k];
   }
   // time state: calcEnergy of the initial state is done on the device later
@@ -816,6 +848,7 @@ static void rebuild_lists(mpm_engine* e) {
     G.matmask |= h.matmask;
     G.drda = G.drda || (h.c.flags & IF_DRDA) != 0;
     G.smooth = G.smooth || (h.c.flags & IF_SMOOTH) != 0;
+    G.contact = G.contact || h.c.contact == 1;
     for (long k = 0; k < h.npg; k++) G.pgran.push_back((int)(h.pg0 + k));
     for (long k = 0; k < h.nng; k++) G.ngran.push_back((int)(h.ng0 + k));
   }
@@ -870,7 +903,8 @@ static LaunchP nlist(const mpm_engine* e, const Group& G) {
 // --------------------------------------------------------- phases
 
 enum Phase {
-  PH_BEGIN, PH_GMI, PH_BC, PH_STRESS, PH_GMU, PH_INTEGRATE, PH_G2P, PH_MUSL, PH_ENERGY, PH_DT, PH_SMOOTH
+  PH_BEGIN, PH_GMI, PH_BC, PH_STRESS, PH_GMU, PH_INTEGRATE, PH_G2P, PH_MUSL, PH_ENERGY, PH_DT, PH_SMOOTH,
+  PH_CONTACT
 };
 
 // the smoothing shape functions of the general modes need the step-start
@@ -916,6 +950,8 @@ static XPtrs xptrs(const mpm_engine* e) {
   X.rw = e->x_rw;
   X.seg0 = e->x_seg0;
   X.seg1 = e->x_seg1;
+  X.ndir = e->x_ndir;
+  X.cf = e->x_cf;
   X.m = e->x_m;
   X.obits = e->x_obits;
   return X;
@@ -980,8 +1016,11 @@ static void exact_group_phase(mpm_engine* e, Group& G, int ph) {
         mpmk::launch_x_scatter(XS_SGMP | XS_FORCE, P, X, sval, N, s);
         mpmk::launch_sgmp_nodes<double>(SGN_FORCE, P, N, s);
       } else {
-        mpmk::launch_x_scatter(XS_FORCE, P, X, sval, N, s);
+        mpmk::launch_x_scatter(G.contact ? XS_FORCE | XS_NDIR : XS_FORCE, P, X, sval, N, s);
       }
+      break;
+    case PH_CONTACT:
+      if (G.contact) mpmk::launch_x_contact(P, X, N, G.d_ilist, (int)G.insts.size(), s);
       break;
     case PH_INTEGRATE:
       mpmk::launch_integrate<double>(P, N, 0, s);
@@ -1055,7 +1094,7 @@ static void settle_integrate(mpm_engine* e, int only = -1) {
   for (size_t id = 0; id < e->inst.size(); id++) {
     HostInst& h = e->inst[id];
     if (!h.live || !h.integ_pending || (only >= 0 && (int)id != only)) continue;
-    mpmk::launch_integrate_range<T>(e->ptrs<T>(), (int)id, h.ng0 * NGRAN, h.c.nnode, e->s);
+    mpmk::launch_integrate_range<T>(e->ptrs<T>(), (int)id, h.ng0 * NGRAN, h.c.nnode_all, e->s);
     h.integ_pending = false;
   }
 }
@@ -1255,6 +1294,7 @@ static void enqueue_step_exact(mpm_engine* e, int fin_flags) {
     }
     ph(KI_P2G, PH_GMU);
     ph(KI_INTEGRATE, PH_INTEGRATE);
+    if (G.contact) ph(KI_INTEGRATE, PH_CONTACT);   // Lagr_NodContact (MPM3D.f90:94-96)
     ph(KI_G2P, PH_G2P);
     if (G.scheme == MPM_MUSL) ph(KI_P2G, PH_MUSL);
     if (G.scheme != MPM_USF) ph(KI_STRESS, PH_STRESS);
@@ -1899,9 +1939,10 @@ static void download_subset_T(mpm_engine* e, int id, int k, const int32_t* ids, 
 }
 
 template <typename T>
-static void download_grid_T(mpm_engine* e, int id, double* out) {
+static void download_grid_T(mpm_engine* e, int id, double* out, int comp = 0) {
   HostInst& h = e->inst[id];
-  const long nn = h.c.nnode, n0 = h.ng0 * NGRAN;
+  if (comp < 0 || comp >= h.c.ncomp) fail(MPM_ERR_ARG, "grid component %d outside 0..%d", comp, h.c.ncomp - 1);
+  const long nn = h.c.nnode, n0 = h.ng0 * NGRAN + (long)comp * nn;
   settle_integrate<T>(e, id);
   HIPCHK(hipStreamSynchronize(e->s));
   std::vector<double> m(nn);
@@ -1917,9 +1958,9 @@ static void download_grid_T(mpm_engine* e, int id, double* out) {
   }
 }
 
-static void download_grid(mpm_engine* e, int id, double* out) {
-  if (e->prec == MPM_PREC_FP32) download_grid_T<float>(e, id, out);
-  else download_grid_T<double>(e, id, out);
+static void download_grid(mpm_engine* e, int id, double* out, int comp = 0) {
+  if (e->prec == MPM_PREC_FP32) download_grid_T<float>(e, id, out, comp);
+  else download_grid_T<double>(e, id, out, comp);
 }
 
 static void download_icell(mpm_engine* e, int id, int32_t* out) {
@@ -1991,7 +2032,7 @@ extern "C" int mpm_ctx_create(const mpm_problem* prob, int precision, mpm_ctx** 
     if (precision != MPM_PREC_FP64 && precision != MPM_PREC_FP32 && precision != MPM_PREC_FP64_EXACT)
       fail(MPM_ERR_ARG, "bad precision");
     check_problem(prob);
-    const long nnode = (long)(prob->ncx + 1) * (prob->ncy + 1) * (prob->ncz + 1);
+    const long nnode = (long)(prob->ncx + 1) * (prob->ncy + 1) * (prob->ncz + 1) * (prob->contact ? prob->nb_component : 1);
     mpm_engine* e = new_engine(prob->nb_particle, nnode, 1, prob->nb_mat, prob->nb_body, precision);
     // a problem below ~1M particles fills the GPU with fewer 256-particle
     // blocks than it needs to hide latency: 128 (what the autotuner picks
@@ -2038,6 +2079,7 @@ CTX_PHASE(mpm_integrate_momentum, PH_INTEGRATE)
 CTX_PHASE(mpm_particle_position_update, PH_G2P)
 CTX_PHASE(mpm_grid_momentum_musl, PH_MUSL)
 CTX_PHASE(mpm_smooth_stress_by_grid, PH_SMOOTH)
+CTX_PHASE(mpm_lagr_nod_contact, PH_CONTACT)
 #undef CTX_PHASE
 
 extern "C" int mpm_calc_energy(mpm_ctx* ctx, double* ek, double* ei, double* mom3) {
@@ -2111,6 +2153,24 @@ extern "C" int mpm_download_grid(mpm_ctx* ctx, double* g) {
   return guarded([&] {
     if (!ctx || !g) fail(MPM_ERR_ARG, "bad arguments");
     download_grid(ctx->e, 0, g);
+  });
+}
+
+extern "C" int mpm_download_grid_component(mpm_ctx* ctx, int comp, double* g) {
+  return guarded([&] {
+    if (!ctx || !g) fail(MPM_ERR_ARG, "bad arguments");
+    download_grid(ctx->e, 0, g, comp);
+  });
+}
+
+extern "C" int mpm_get_contact_force(mpm_ctx* ctx, double* out3) {
+  return guarded([&] {
+    if (!ctx || !out3) fail(MPM_ERR_ARG, "bad arguments");
+    mpm_engine* e = ctx->e;
+    InstT ts;
+    HIPCHK(hipStreamSynchronize(e->s));
+    HIPCHK(hipMemcpy(&ts, e->d_ts, sizeof ts, hipMemcpyDeviceToHost));
+    for (int d = 0; d < 3; d++) out3[d] = e->inst[0].c.contact == 1 ? ts.cont_force[d] : 0.0;
   });
 }
 

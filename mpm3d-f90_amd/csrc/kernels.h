@@ -45,6 +45,7 @@
 #define XS_MOM 2      // PXg (SGMP: Cpxg)
 #define XS_FORCE 4    // FXg (SGMP: Cfext, Cfint)
 #define XS_SGMP 8     // to the auxiliary nodes
+#define XS_NDIR 16    // with XS_FORCE: the contact normals of contact instances (mass gradients)
 
 // SmoothStressbyGrid node passes (launch_smooth_nodes)
 #define SM_ZERO 1     // clear the sums and flags
@@ -130,6 +131,9 @@ void launch_x_stress(int shape, int move, const PoolPtrs<double>& P, const XPtrs
                      hipStream_t s);
 void launch_x_g2p(int shape, int move, const PoolPtrs<double>& P, const XPtrs& X, const LaunchP& L, hipStream_t s);
 void launch_x_sgmp_stress(int move, const PoolPtrs<double>& P, const LaunchP& L, hipStream_t s);
+// Lagr_NodContact over the node list N, then each instance's tot_cont_for (ilist: the group's instances)
+void launch_x_contact(const PoolPtrs<double>& P, const XPtrs& X, const LaunchP& N, const int* ilist, int ninst,
+                      hipStream_t s);
 // SmoothStressbyGrid (smooth.inc; exact mode: exact.inc)
 template <typename T> void launch_smooth_save_x(const PoolPtrs<T>& P, const SmPtrs& S, const LaunchP& L, hipStream_t s);
 template <typename T>

@@ -1372,7 +1372,7 @@ __global__ void __launch_bounds__(256) k_integrate(PoolPtrs<T> P, const int* ngl
   const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   const InstT ts = P.ts[ii];
   if (ts.done) return;
-  if (j >= (long)I.node_off + I.nnode) return;
+  if (j >= (long)I.node_off + I.nnode_all) return;
   const long nn = P.nn;
   const T dt = (T)ts.dt;
   const bool half = ts.istep == 1;
@@ -1435,7 +1435,7 @@ __global__ void __launch_bounds__(256) k_bc(PoolPtrs<T> P, const int* nglist, co
   const int ii = nginst[gran];
   const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   if (P.ts[ii].done) return;
-  if (j >= (long)I.node_off + I.nnode) return;
+  if (j >= (long)I.node_off + I.nnode_all) return;
   const uint8_t fx = bc ? P.fix[j] : 0;
   T* pg = use_p2 ? P.p2 : P.pg;
   const long nn = P.nn;
@@ -1462,7 +1462,7 @@ __global__ void __launch_bounds__(256) k_node_va(PoolPtrs<T> P, const int* nglis
   const int ii = nginst[gran];
   const InstC I = P.inst[ii];   // by value: read once, ahead of the kernel's stores
   if (P.ts[ii].done) return;
-  if (j >= (long)I.node_off + I.nnode) return;
+  if (j >= (long)I.node_off + I.nnode_all) return;
   const long nn = P.nn;
   const double mg = P.mg[j];
   const bool heavy = mg > I.cutoff;
@@ -3250,9 +3250,10 @@ __global__ void __launch_bounds__(256) k_fixity(uint8_t* fix, const InstC* inst,
   if (k >= nslots) return;
   const InstC& c = inst[id];
   uint8_t v = 0;
-  if (k < c.nnode) {
+  if (k < c.nnode_all) {   // every grid component has the fixities of the grid
+    const long kb = k % c.nnode;
     const int NGx = c.ngx, NGy = c.ngy, NGz = c.ngz;
-    const int ix = (int)(k % NGx) + 1, iy = (int)((k / NGx) % NGy) + 1, iz = (int)(k / c.ngxy) + 1;
+    const int ix = (int)(kb % NGx) + 1, iy = (int)((kb / NGx) % NGy) + 1, iz = (int)(kb / c.ngxy) + 1;
     const int* FixS = c.fixs;
     const bool wide = c.shape == MPM_SHAPE_GIMP || c.shape == MPM_SHAPE_BSPLINE || c.shape == MPM_SHAPE_SGMP;
     int bx = 0, by = 0, bz = 0;

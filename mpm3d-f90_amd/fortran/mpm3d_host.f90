@@ -69,6 +69,7 @@ program mpm3d_host
      end if
      call mpm_check(mpm_grid_momentum_update(ctx), 'GridMomentumUpdate')        ! :87
      call mpm_check(mpm_integrate_momentum(ctx), 'IntegrateMomentum')           ! :90
+     if (prob%contact == 1) call mpm_check(mpm_lagr_nod_contact(ctx), 'Lagr_NodContact')   ! :94-96
      call mpm_check(mpm_particle_position_update(ctx), 'ParticlePositionUpdate') ! :99
      if (musl) then
         call mpm_check(mpm_grid_momentum_musl(ctx), 'GridMomentumMUSL')

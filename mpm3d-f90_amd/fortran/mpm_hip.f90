@@ -111,6 +111,19 @@ module mpm_hip
        integer(c_int) :: rc
      end function mpm_smooth_stress_by_grid
 
+     function mpm_lagr_nod_contact(ctx) bind(C, name="mpm_lagr_nod_contact") result(rc)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: rc
+     end function mpm_lagr_nod_contact
+
+     function mpm_get_contact_force(ctx, out3) bind(C, name="mpm_get_contact_force") result(rc)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: out3(3)
+       integer(c_int) :: rc
+     end function mpm_get_contact_force
+
      function mpm_grid_momentum_update(ctx) bind(C, name="mpm_grid_momentum_update") result(rc)
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

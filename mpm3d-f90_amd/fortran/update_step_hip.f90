@@ -25,11 +25,13 @@
 !
 ! SmoothStressbyGrid, which the driver calls after UpdateDT, smooths on the
 ! device and, on output steps, refreshes the particles' Spre for the
-! ParaView writer.
+! ParaView writer.  Lagr_NodContact runs the contact pass on the device;
+! contact decks run in the reference-order FP64 mode (the only mode with
+! grid components), and tot_cont_for is refreshed for OutCurve.
 !
 ! Compiles against the reference's module files (ParticleData, GridData,
-! MaterialData); total-Lagrangian and contact routines are present only to
-! satisfy the link and stop when called.
+! MaterialData); total-Lagrangian routines are present only to satisfy the
+! link and stop when called.
 ! ---------------------------------------------------------------------------
 module update_step_hip_state
   use, intrinsic :: iso_c_binding
@@ -93,9 +95,10 @@ contains
     prob%nb_component = nb_component
     prob%tlmpm = 0
     prob%contact = 0
-    prob%normbody = 0
+    if (contact) prob%contact = Contact_type
+    prob%normbody = normbody
     prob%reserved0 = 0
-    prob%fricfa = 0d0
+    prob%fricfa = fricfa
     prob%span = [SpanX(1), SpanX(2), SpanY(1), SpanY(2), SpanZ(1), SpanZ(2)]
     prob%dcell = DCell
     prob%cutoff = CutOff
@@ -178,6 +181,10 @@ contains
     call get_environment_variable('MPM3D_PRECISION', env)
     if (trim(env) == 'fp32') prec = MPM_PREC_FP32
     if (trim(env) == 'fp64exact') prec = MPM_PREC_FP64_EXACT
+    if (contact .and. prec /= MPM_PREC_FP64_EXACT) then
+       write(*, '(a)') ' update_step_hip: contact runs in the reference-order FP64 mode (fp64exact)'
+       prec = MPM_PREC_FP64_EXACT
+    end if
     call mpm_check(mpm_ctx_create(c_loc(prob), prec, ctx), 'mpm_ctx_create')
     j = 0
   end subroutine ensure_context
@@ -254,6 +261,7 @@ contains
   ! what the driver reads after UpdateDT (see the module header)
   subroutine refresh_host_view()
     use ParticleData
+    use GridData, only: tot_cont_for, Contact_type
     use DataOut, only: nCurves, CurvePoint, WriteTecPlot, WriteParaView, OutTime
     real(c_double) :: sc(8)
     character(len=8) :: f
@@ -263,6 +271,8 @@ contains
     EngKinetic = sc(4)
     EngInternal = sc(5)
     Momentum = sc(6:8)
+    if (contact .and. Contact_type == 1) &
+       call mpm_check(mpm_get_contact_force(ctx, tot_cont_for), 'mpm_get_contact_force')
     if (.not. plt_started) then
        plt_mirror = plt_mirror + OutTime
        plt_started = .true.
@@ -401,6 +411,12 @@ subroutine UpdateDT()
   call maybe_dump()
 end subroutine UpdateDT
 
+subroutine Lagr_NodContact()
+  use update_step_hip_state
+  implicit none
+  call mpm_check(mpm_lagr_nod_contact(ctx), 'Lagr_NodContact')
+end subroutine Lagr_NodContact
+
 subroutine SmoothStressbyGrid()
   use update_step_hip_state
   use ParticleData, only: particle_list, nb_particle
@@ -421,9 +437,6 @@ end subroutine SmoothStressbyGrid
 subroutine TLSmoothStressbyGrid()
   stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
 end subroutine TLSmoothStressbyGrid
-subroutine Lagr_NodContact()
-  stop 'update_step_hip: contact is not available in the device build'
-end subroutine Lagr_NodContact
 subroutine GridMassInitial()
   stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
 end subroutine GridMassInitial

@@ -5,7 +5,7 @@ The reference's driver writes, besides its message file (<name>.out):
                        (OutCurve, DataOut.f90:66-154)
   EnergyPlot.dat       time, total, kinetic and internal energy per step
   MomentumPlot.dat     time and momentum per step
-  ContforcPlot.dat     contact force per step (empty: contact is not built)
+  ContforcPlot.dat     contact force per step (contact decks; empty otherwise)
   <name>_anim.dat      TecPlot frames at the output times (OutAnim,
                        DataOut.f90:157-247), with ``tecp``
   <name>_anim_<k>.vtu  ParaView frames, collected in <name>_anim.pvd, and
@@ -177,9 +177,9 @@ class ResultWriter:
         """1-based particles whose state curve() reads, in curve order"""
         return [pt for _, pt in self.curves]
 
-    def curve(self, time, eng_kinetic, eng_internal, momentum, points):
+    def curve(self, time, eng_kinetic, eng_internal, momentum, points, contact_force=None):
         """one step's line; points: PF_COUNT x n_curves state of
-        curve_points()"""
+        curve_points(); contact_force: tot_cont_for of a contact problem"""
         out = self.files["curv"]
         self.n_cvs += 1
         if self.n_cvs == 1:
@@ -193,6 +193,9 @@ class ResultWriter:
         self.files["energy"].write("".join(fe(v, 12, 4) for v in (time, eng_kinetic + eng_internal, eng_kinetic,
                                                                   eng_internal)) + "\n")
         self.files["momentum"].write("".join(fe(v, 12, 4) for v in (time,) + tuple(momentum)) + "\n")
+        if self.problem.contact:   # DataOut.f90:148-151
+            cf = tuple(contact_force) if contact_force is not None else (0.0, 0.0, 0.0)
+            self.files["contact"].write("".join(fe(v, 12, 4) for v in (time,) + cf) + "\n")
 
     def _curve_value(self, opt, p, xo, ek, ei):
         # DataOut.f90:94-137
@@ -338,7 +341,8 @@ def run(ctx, outputs, name, directory=".", max_steps=None):
             sc = ctx.scalars()
             t, ek, ei = sc[1], sc[3], sc[4]
             sub = subset(pts) if subset is not None else ctx.particles()[:, pts - 1]
-            w.curve(t, ek, ei, sc[5:8], sub)
+            cf = ctx.contact_force() if prob.contact == 1 else None
+            w.curve(t, ek, ei, sc[5:8], sub, cf)
             if t >= np.float64(plt):
                 plt = np.float32(np.float64(plt) + outputs.out_time)
                 w.frame(t, ctx.particles(), ctx.icell())

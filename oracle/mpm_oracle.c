@@ -32,7 +32,15 @@ struct oracle {
   /* grid */
   int ncx, ncy, ncz, ncxy, ncell;
   int ngx, ngy, ngz, ngxy, nnode;
-  double* g;            /* GF_COUNT * nnode */
+  /* grid components (Grid.f90:284): with contact every body maps to the
+   * grid of its component (update_step.f90:56), stored one after another
+   * (node k of component c at c * nnode + k); without contact one grid */
+  int ncomp, nntot;
+  int8_t* pcomp;        /* per particle: its body's component - 1 (0 without contact) */
+  double* ndir;         /* 3 * nntot: CP_list%ndir, the contact normals (mass gradients) */
+  double cont_force[3]; /* tot_cont_for of the last Lagr_NodContact */
+  long cont_count[3];   /* diagnostics: contact nodes without friction, sticking, slipping (all steps) */
+  double* g;            /* GF_COUNT * nntot */
   int8_t* fix;          /* 3 * nnode */
   double ijac, ijac4, idcell;
   /* SGMP auxiliary grid: one node per cell centre (Grid.f90:50-65, 252-268) */
@@ -57,7 +65,9 @@ struct oracle {
 };
 
 #define P(o, f, i) ((o)->ps[(size_t)(f) * (o)->n + (i)])
-#define G(o, f, k) ((o)->g[(size_t)(f) * (o)->nnode + (k)])
+#define G(o, f, k) ((o)->g[(size_t)(f) * (o)->nntot + (k)])
+/* grid index of 0-based node k of particle p's component */
+#define PNODE(o, p, k) ((int)(o)->pcomp[p] * (o)->nnode + (k))
 
 /* node coordinate of 1-based linear node index k (Grid.f90:296-302) */
 static void node_xg(const oracle* o, int k, double xg[3]) {
@@ -478,7 +488,13 @@ oracle* oracle_create(const mpm_problem* p) {
   o->ngx = o->ncx + 1; o->ngy = o->ncy + 1; o->ngz = o->ncz + 1;
   o->ngxy = o->ngx * o->ngy;
   o->nnode = o->ngxy * o->ngz;
-  o->g = (double*)calloc((size_t)GF_COUNT * o->nnode, sizeof(double));
+  o->ncomp = p->contact ? p->nb_component : 1;
+  o->nntot = o->nnode * o->ncomp;
+  o->pcomp = (int8_t*)calloc(o->n, 1);
+  if (p->contact)
+    for (int i = 0; i < o->n; i++) o->pcomp[i] = (int8_t)(p->bodies[p->pbody[i]].com - 1);
+  o->ndir = (double*)calloc((size_t)3 * o->nntot, sizeof(double));
+  o->g = (double*)calloc((size_t)GF_COUNT * o->nntot, sizeof(double));
   o->fix = (int8_t*)calloc((size_t)3 * o->nnode, 1);
   /* fixities (Grid.f90:291-359) */
   const int* FixS = p->fix;
@@ -523,7 +539,7 @@ oracle* oracle_create(const mpm_problem* p) {
 void oracle_destroy(oracle* o) {
   if (!o) return;
   free(o->mats); free(o->bodies); free(o->ps); free(o->pbody); free(o->icell);
-  free(o->xn); free(o->g); free(o->fix); free(o->dpbr);
+  free(o->xn); free(o->g); free(o->fix); free(o->dpbr); free(o->pcomp); free(o->ndir);
   free(o->bdt); free(o->cicell); free(o->inode); free(o->xxn);
   free(o->cmg); free(o->cp); free(o->cfint); free(o->cfext); free(o->co); free(o->cw); free(o->cdefp);
   free(o->cv); free(o->ca);
@@ -804,7 +820,7 @@ static inline int in_grid(const oracle* o, int k) { return k <= o->nnode && k > 
 /* ----------------------------------------------------------- grid phases */
 
 void oracle_grid_momentum_initial(oracle* o) {
-  for (int k = 0; k < o->nnode; k++) G(o, GF_MG, k) = G(o, GF_PX, k) = G(o, GF_PY, k) = G(o, GF_PZ, k) = 0.0;
+  for (int k = 0; k < o->nntot; k++) G(o, GF_MG, k) = G(o, GF_PX, k) = G(o, GF_PY, k) = G(o, GF_PZ, k) = 0.0;
   particle_initial(o);
   if (o->sgmp) {
     sgmp_momentum_initial(o);
@@ -818,7 +834,7 @@ void oracle_grid_momentum_initial(oracle* o) {
     for (int i = 0; i < s.nb; i++) {
       int k = s.node[i];
       if (!in_grid(o, k)) continue;
-      k -= 1;
+      k = PNODE(o, p, k - 1);
       double shm = s.shp[i] * mp;
       G(o, GF_MG, k) = G(o, GF_MG, k) + shm;
       G(o, GF_PX, k) = G(o, GF_PX, k) + P(o, PF_VX, p) * shm;
@@ -829,9 +845,9 @@ void oracle_grid_momentum_initial(oracle* o) {
 }
 
 void oracle_apply_bc(oracle* o) {
-  for (int k = 0; k < o->nnode; k++)
+  for (int k = 0; k < o->nntot; k++)
     for (int d = 0; d < 3; d++)
-      if (o->fix[d * o->nnode + k]) {
+      if (o->fix[d * o->nnode + k % o->nnode]) {
         G(o, GF_PX + d, k) = 0.0;
         G(o, GF_FX + d, k) = 0.0;
       }
@@ -842,7 +858,8 @@ void oracle_grid_momentum_update(oracle* o) {
     memset(o->cfint, 0, 6 * (size_t)o->ncell * sizeof(double));
     memset(o->cfext, 0, 3 * (size_t)o->ncell * sizeof(double));
   }
-  for (int k = 0; k < o->nnode; k++) G(o, GF_FX, k) = G(o, GF_FY, k) = G(o, GF_FZ, k) = 0.0;
+  for (int k = 0; k < o->nntot; k++) G(o, GF_FX, k) = G(o, GF_FY, k) = G(o, GF_FZ, k) = 0.0;
+  if (o->prob.contact) memset(o->ndir, 0, (size_t)3 * o->nntot * sizeof(double));
   const mpm_problem* pr = &o->prob;
   shape_t s;
   for (int p = 0; p < o->n; p++) {
@@ -880,7 +897,7 @@ void oracle_grid_momentum_update(oracle* o) {
     for (int i = 0; i < s.nb; i++) {
       int k = s.node[i];
       if (!in_grid(o, k)) continue;
-      k -= 1;
+      k = PNODE(o, p, k - 1);
       double dX = s.dx[i], dY = s.dy[i], dZ = s.dz[i], N = s.shp[i];
       double fi0 = -(sxx * dX + sxy * dY + sxz * dZ) * vol;
       double fi1 = -(sxy * dX + syy * dY + syz * dZ) * vol;
@@ -888,22 +905,94 @@ void oracle_grid_momentum_update(oracle* o) {
       G(o, GF_FX, k) = G(o, GF_FX, k) + fi0 + fx[0] * N;
       G(o, GF_FY, k) = G(o, GF_FY, k) + fi1 + fx[1] * N;
       G(o, GF_FZ, k) = G(o, GF_FZ, k) + fi2 + fx[2] * N;
+      if (pr->contact) {   /* update_step.f90:977-982 */
+        const double mp = P(o, PF_MASS, p);
+        o->ndir[3 * k] = o->ndir[3 * k] + dX * mp;
+        o->ndir[3 * k + 1] = o->ndir[3 * k + 1] + dY * mp;
+        o->ndir[3 * k + 2] = o->ndir[3 * k + 2] + dZ * mp;
+      }
     }
   }
   if (o->sgmp) sgmp_force_cells(o);
 }
 
 void oracle_integrate_momentum(oracle* o) {
-  for (int k = 0; k < o->nnode; k++) {
+  for (int k = 0; k < o->nntot; k++) {
     for (int d = 0; d < 3; d++) {
       if (o->istep == 1) G(o, GF_PX + d, k) = G(o, GF_PX + d, k) + G(o, GF_FX + d, k) * o->dt * 0.5;
       else G(o, GF_PX + d, k) = G(o, GF_PX + d, k) + G(o, GF_FX + d, k) * o->dt;
     }
     for (int d = 0; d < 3; d++)
-      if (o->fix[d * o->nnode + k]) {
+      if (o->fix[d * o->nnode + k % o->nnode]) {
         G(o, GF_PX + d, k) = 0.0;
         G(o, GF_FX + d, k) = 0.0;
       }
+  }
+}
+
+/* Lagr_NodContact (update_step.f90:1241-1376): Bardenhagen's contact of
+ * components 1 and 2 at every node.  The normal is the mass gradient of
+ * GridMomentumUpdate (both bodies' normals averaged, or one body's, by
+ * normbody), made a unit vector; the bodies approach where
+ * (p1 m2 - p2 m1) . n > epsilon; the normal contact force and, with
+ * friction, the tangential force capped at fricfa |normal force| are taken
+ * off component 1 and given to component 2 (force and momentum, the fixed
+ * directions left alone), and summed over the nodes in ascending order. */
+void oracle_contact(oracle* o) {
+  const double eps = 2.220446049250313e-16;   /* epsilon(1d0) */
+  const double fricfa = o->prob.fricfa, cutoff = o->prob.cutoff;
+  const int nn = o->nnode;
+  for (int d = 0; d < 3; d++) o->cont_force[d] = 0.0;
+  for (int n = 0; n < nn; n++) {
+    double* n1 = &o->ndir[3 * n];
+    double* n2 = &o->ndir[3 * (nn + n)];
+    double nx, ny, nz;
+    if (o->prob.normbody == 1) {
+      nx = n1[0]; ny = n1[1]; nz = n1[2];
+    } else if (o->prob.normbody == 2) {
+      nx = -n2[0]; ny = -n2[1]; nz = -n2[2];
+    } else {
+      nx = n1[0] - n2[0]; ny = n1[1] - n2[1]; nz = n1[2] - n2[2];
+    }
+    double tt = sqrt(nx * nx + ny * ny + nz * nz);
+    if (tt > eps) {
+      nx = nx / tt;
+      ny = ny / tt;
+      nz = nz / tt;
+    }
+    n1[0] = nx; n1[1] = ny; n1[2] = nz;
+    n2[0] = -n1[0]; n2[1] = -n1[1]; n2[2] = -n1[2];
+    const int k1 = n, k2 = nn + n;
+    const double m1 = G(o, GF_MG, k1), m2 = G(o, GF_MG, k2);
+    double crit = 0.0;
+    if (m1 > cutoff && m2 > cutoff)
+      crit = (G(o, GF_PX, k1) * m2 - G(o, GF_PX, k2) * m1) * nx + (G(o, GF_PY, k1) * m2 - G(o, GF_PY, k2) * m1) * ny +
+             (G(o, GF_PZ, k1) * m2 - G(o, GF_PZ, k2) * m1) * nz;
+    if (!(crit > eps)) continue;
+    tt = (m1 + m2) * o->dt;
+    const double nomforce = crit / tt;
+    double cforce[3];
+    if (fricfa > eps) {
+      double fstick[3];
+      for (int d = 0; d < 3; d++) cforce[d] = (G(o, GF_PX + d, k1) * m2 - G(o, GF_PX + d, k2) * m1) / tt;
+      for (int d = 0; d < 3; d++) fstick[d] = cforce[d] - nomforce * n1[d];
+      const double val_fstick = sqrt(fstick[0] * fstick[0] + fstick[1] * fstick[1] + fstick[2] * fstick[2]);
+      const double val_fslip = fricfa * fabs(nomforce);
+      if (val_fslip < val_fstick)
+        for (int d = 0; d < 3; d++) cforce[d] = nomforce * n1[d] + val_fslip * (fstick[d] / val_fstick);
+      o->cont_count[val_fslip < val_fstick ? 2 : 1]++;
+    } else {
+      for (int d = 0; d < 3; d++) cforce[d] = nomforce * n1[d];
+      o->cont_count[0]++;
+    }
+    for (int d = 0; d < 3; d++) {
+      if (o->fix[d * nn + n]) continue;
+      G(o, GF_FX + d, k1) = G(o, GF_FX + d, k1) - cforce[d];
+      G(o, GF_FX + d, k2) = G(o, GF_FX + d, k2) + cforce[d];
+      G(o, GF_PX + d, k1) = G(o, GF_PX + d, k1) - cforce[d] * o->dt;
+      G(o, GF_PX + d, k2) = G(o, GF_PX + d, k2) + cforce[d] * o->dt;
+    }
+    for (int d = 0; d < 3; d++) o->cont_force[d] = o->cont_force[d] + cforce[d];
   }
 }
 
@@ -923,7 +1012,7 @@ void oracle_particle_position_update(oracle* o) {
     for (int i = 0; i < s.nb; i++) {
       int k = s.node[i];
       if (!in_grid(o, k)) continue;
-      k -= 1;
+      k = PNODE(o, p, k - 1);
       double mg = G(o, GF_MG, k);
       if (mg > cutoff) {
         double N = s.shp[i];
@@ -944,7 +1033,7 @@ void oracle_particle_position_update(oracle* o) {
 }
 
 void oracle_grid_momentum_musl(oracle* o) {
-  for (int k = 0; k < o->nnode; k++) G(o, GF_PX, k) = G(o, GF_PY, k) = G(o, GF_PZ, k) = 0.0;
+  for (int k = 0; k < o->nntot; k++) G(o, GF_PX, k) = G(o, GF_PY, k) = G(o, GF_PZ, k) = 0.0;
   if (o->sgmp) {
     sgmp_momentum_musl(o);
     oracle_apply_bc(o);
@@ -958,7 +1047,7 @@ void oracle_grid_momentum_musl(oracle* o) {
     for (int i = 0; i < s.nb; i++) {
       int k = s.node[i];
       if (!in_grid(o, k)) continue;
-      k -= 1;
+      k = PNODE(o, p, k - 1);
       double shm = s.shp[i] * mp;
       for (int d = 0; d < 3; d++) G(o, GF_PX + d, k) = G(o, GF_PX + d, k) + P(o, PF_VX + d, p) * shm;
     }
@@ -1466,7 +1555,7 @@ void oracle_particle_stress_update(oracle* o) {
     for (int i = 0; i < s.nb; i++) {
       int k = s.node[i];
       if (!in_grid(o, k)) continue;
-      k -= 1;
+      k = PNODE(o, p, k - 1);
       double mg = G(o, GF_MG, k);
       if (mg > cutoff) {
         double v[3] = {G(o, GF_PX, k) / mg, G(o, GF_PY, k) / mg, G(o, GF_PZ, k) / mg};
@@ -1665,6 +1754,7 @@ void oracle_step(oracle* o, int nsteps) {
     }
     oracle_grid_momentum_update(o);
     oracle_integrate_momentum(o);
+    if (o->prob.contact == 1) oracle_contact(o);
     oracle_particle_position_update(o);
     if (o->prob.scheme == MPM_MUSL) {
       oracle_grid_momentum_musl(o);
@@ -1723,8 +1813,21 @@ void oracle_get_icell(const oracle* o, int32_t* out) {
   memcpy(out, o->icell, sizeof(int32_t) * o->n);
 }
 
-void oracle_get_grid(const oracle* o, double* out) {
-  memcpy(out, o->g, sizeof(double) * GF_COUNT * o->nnode);
+void oracle_get_grid(const oracle* o, double* out) { oracle_get_grid_component(o, 0, out); }
+
+void oracle_get_grid_component(const oracle* o, int c, double* out) {
+  for (int f = 0; f < GF_COUNT; f++)
+    memcpy(out + (size_t)f * o->nnode, &G(o, f, c * o->nnode), sizeof(double) * o->nnode);
+}
+
+int oracle_nb_component(const oracle* o) { return o->ncomp; }
+
+void oracle_get_contact_force(const oracle* o, double* out3) {
+  for (int d = 0; d < 3; d++) out3[d] = o->cont_force[d];
+}
+
+void oracle_get_contact_counts(const oracle* o, long* out3) {
+  for (int d = 0; d < 3; d++) out3[d] = o->cont_count[d];
 }
 
 void oracle_get_fix(const oracle* o, int8_t* out) {

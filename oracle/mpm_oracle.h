@@ -35,6 +35,7 @@ void oracle_apply_bc(oracle* o);                  /* :1795-1830 */
 void oracle_particle_stress_update(oracle* o);    /* :456-647 */
 void oracle_grid_momentum_update(oracle* o);      /* :831-1013 */
 void oracle_integrate_momentum(oracle* o);        /* :1194-1239 */
+void oracle_contact(oracle* o);                   /* :1241-1376 Lagr_NodContact */
 void oracle_particle_position_update(oracle* o);  /* :1378-1526 */
 void oracle_grid_momentum_musl(oracle* o);        /* :1668-1793 */
 void oracle_calc_energy(oracle* o);               /* Particle.f90:204-230 */
@@ -45,7 +46,15 @@ void oracle_step(oracle* o, int nsteps);
 
 void oracle_get_particles(const oracle* o, double* out);   /* PF_COUNT * nb_particle */
 void oracle_get_icell(const oracle* o, int32_t* out);      /* nb_particle, 1-based, -1 outside */
-void oracle_get_grid(const oracle* o, double* out);        /* GF_COUNT * nb_node */
+void oracle_get_grid(const oracle* o, double* out);        /* GF_COUNT * nb_node, component 1 */
+/* grid of component c (0-based) of a contact problem, GF_COUNT * nb_node */
+void oracle_get_grid_component(const oracle* o, int c, double* out);
+int oracle_nb_component(const oracle* o);
+/* tot_cont_for of the last Lagr_NodContact (DataOut.f90:148-151) */
+void oracle_get_contact_force(const oracle* o, double* out3);
+/* diagnostics, not reference state: contact nodes over all steps so far,
+ * frictionless / sticking (full tangential force) / slipping (capped) */
+void oracle_get_contact_counts(const oracle* o, long* out3);
 void oracle_get_fix(const oracle* o, int8_t* out);         /* 3 * nb_node */
 /* out[0..7] = istep, CurrentTime, DT, EngKinetic, EngInternal, Momentum(3) */
 void oracle_get_scalars(const oracle* o, double* out);

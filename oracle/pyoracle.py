@@ -36,7 +36,7 @@ def lib():
                      "oracle_particle_stress_update", "oracle_grid_momentum_update",
                      "oracle_integrate_momentum", "oracle_particle_position_update",
                      "oracle_grid_momentum_musl", "oracle_calc_energy", "oracle_update_dt",
-                     "oracle_smooth_stress", "oracle_destroy"):
+                     "oracle_smooth_stress", "oracle_contact", "oracle_destroy"):
             getattr(L, name).argtypes = [vp]
             getattr(L, name).restype = None
         L.oracle_create.argtypes = [vp]
@@ -48,6 +48,11 @@ def lib():
                      "oracle_set_particles"):
             getattr(L, name).argtypes = [vp, C.POINTER(C.c_double)]
         L.oracle_get_icell.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.oracle_get_grid_component.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+        L.oracle_get_contact_force.argtypes = [vp, C.POINTER(C.c_double)]
+        L.oracle_get_contact_counts.argtypes = [vp, C.POINTER(C.c_long)]
+        L.oracle_nb_component.argtypes = [vp]
+        L.oracle_nb_component.restype = C.c_int
         L.oracle_get_fix.argtypes = [vp, C.POINTER(C.c_int8)]
         L.oracle_get_dp_branches.argtypes = [vp, C.POINTER(C.c_int8), C.POINTER(C.c_long)]
         L.oracle_set_log_cr.argtypes = [vp, C.c_int]
@@ -103,6 +108,28 @@ class Oracle:
         lib().oracle_get_grid(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out
 
+    def grid_component(self, c):
+        """grid of component c (0-based) of a contact problem"""
+        out = np.empty((GF_COUNT, self.nnode))
+        lib().oracle_get_grid_component(self.h, int(c), out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def nb_component(self):
+        return lib().oracle_nb_component(self.h)
+
+    def contact_force(self):
+        """tot_cont_for of the last Lagr_NodContact"""
+        out = np.zeros(3)
+        lib().oracle_get_contact_force(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
+
+    def contact_counts(self):
+        """contact nodes over all steps: [frictionless, sticking, slipping]
+        (oracle diagnostics of Lagr_NodContact's branches)"""
+        out = np.zeros(3, dtype=np.int64)
+        lib().oracle_get_contact_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_long)))
+        return out
+
     def icell(self):
         out = np.empty(self.n, dtype=np.int32)
         lib().oracle_get_icell(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)))
@@ -153,7 +180,8 @@ def read_ref_dump(path):
     """Parse one ref_driver dump -> dict(hdr, scal, particles{name: arr}, icell, grid[7,nn], fix[3,nn])."""
     raw = open(path, "rb").read()
     hdr = np.frombuffer(raw, dtype="<i4", count=16)
-    assert hdr[0] == 1146113101 and hdr[1] == 3, "not a ref_driver dump (layout 3): %s" % path
+    assert hdr[0] == 1146113101 and hdr[1] == 4, "not a ref_driver dump (layout 4): %s" % path
+    ncomp = int(hdr[11])
     n, nn = int(hdr[2]), int(hdr[3])
     scal = np.frombuffer(raw, dtype="<f8", count=16, offset=64)
     off = 64 + 128
@@ -169,12 +197,19 @@ def read_ref_dump(path):
     pos += 7 * nn
     fix = arrs[pos:pos + 3 * nn].reshape(3, nn).astype(np.int8)
     pos += 3 * nn
+    grids = [grid]
+    for _ in range(1, ncomp):
+        grids.append(arrs[pos:pos + 7 * nn].reshape(7, nn).copy())
+        pos += 7 * nn
+    cont_force = arrs[pos:pos + 3].copy()
+    pos += 3
     assert pos == arrs.size, "dump size mismatch"
     return {"hdr": hdr, "istep": int(hdr[4]), "phase": int(hdr[5]), "n": n, "nnode": nn,
             "ncell": (int(hdr[6]), int(hdr[7]), int(hdr[8])),
             "dt": scal[0], "time": scal[1], "ek": scal[2], "ei": scal[3], "mom": scal[4:7].copy(),
             "cutoff": scal[7], "dcell": scal[8], "span": scal[9:15].copy(), "dtscale": scal[15],
-            "particles": parts, "icell": icell, "grid": grid, "fix": fix}
+            "particles": parts, "icell": icell, "grid": grid, "fix": fix, "grids": grids,
+            "cont_force": cont_force}
 
 
 def run_ref(deck, nsteps, outprefix, dumpsteps, phasestep=None, cwd=None, timeout=600):

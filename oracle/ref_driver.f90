@@ -5,7 +5,8 @@
 ! Test infrastructure only.  It replays the updated-Lagrangian step sequence
 ! of the reference main program (MPM3D.f90:69-113: GridMomentumInitial,
 ! [USF: ApplyBoundaryConditions, ParticleStressUpdate], GridMomentumUpdate,
-! IntegrateMomentum, ParticlePositionUpdate, [MUSL: GridMomentumMUSL,
+! IntegrateMomentum, [cont lagr: Lagr_NodContact], ParticlePositionUpdate,
+! [MUSL: GridMomentumMUSL,
 ! ApplyBoundaryConditions], [USL/MUSL: ParticleStressUpdate], calcEnergy,
 ! UpdateDT, [smoo: SmoothStressbyGrid], [drda: dynamicRelaxationDamping])
 ! without the per-step file output, and writes the complete particle and
@@ -38,7 +39,7 @@ program ref_driver
 
   external GridMomentumInitial, ParticleStressUpdate, GridMomentumUpdate, &
            IntegrateMomentum, ParticlePositionUpdate, GridMomentumMUSL, &
-           ApplyBoundaryConditions, UpdateDT, SmoothStressbyGrid
+           ApplyBoundaryConditions, UpdateDT, SmoothStressbyGrid, Lagr_NodContact
 
   if (command_argument_count() < 4) then
      print *, 'usage: ref_driver <deck> <nsteps> <outprefix> <dumpsteps> [phasestep]'
@@ -69,7 +70,6 @@ program ref_driver
   call InputPara()
   call calcEnergy()
   if (TLMPM) stop 'ref_driver: total-Lagrangian decks are not replayed'
-  if (Contact_type /= 0) stop 'ref_driver: contact decks are not replayed'
 
   if (any(dumpat(1:ndump) == 0)) call dump_state(0, 0)
 
@@ -90,6 +90,7 @@ program ref_driver
      call GridMomentumUpdate()
      if (dophase) call dump_state(istep, 3)
      call IntegrateMomentum()
+     if (Contact_type == 1) call Lagr_NodContact()
      if (dophase) call dump_state(istep, 4)
      call ParticlePositionUpdate()
      if (dophase) call dump_state(istep, 5)
@@ -120,7 +121,7 @@ contains
   subroutine dump_state(step, phase)
     integer, intent(in) :: step, phase
     character(len=600) :: fname
-    integer :: u, i, j
+    integer :: u, i, j, c, ncomp
     integer(4) :: hdr(16)
     real(8) :: rh(16)
 
@@ -133,13 +134,16 @@ contains
 
     hdr = 0
     hdr(1) = 1146113101          ! 'MPMD' tag
-    hdr(2) = 3                   ! layout version
+    hdr(2) = 4                   ! layout version
     hdr(3) = nb_particle
     hdr(4) = nb_gridnode
     hdr(5) = istep
     hdr(6) = phase
     hdr(7) = NumCellx;  hdr(8) = NumCelly;  hdr(9) = NumCellz
     hdr(10) = nb_body;  hdr(11) = nb_mat
+    ncomp = 1
+    if (contact) ncomp = nb_component
+    hdr(12) = ncomp
     rh = 0d0
     rh(1) = DT;  rh(2) = CurrentTime;  rh(3) = EngKinetic;  rh(4) = EngInternal
     rh(5:7) = Momentum;  rh(8) = CutOff;  rh(9) = DCell
@@ -198,6 +202,21 @@ contains
     write(u) (merge(1d0, 0d0, node_list(i)%Fix_x), i = 1, nb_gridnode)
     write(u) (merge(1d0, 0d0, node_list(i)%Fix_y), i = 1, nb_gridnode)
     write(u) (merge(1d0, 0d0, node_list(i)%Fix_z), i = 1, nb_gridnode)
+    ! grids of the other components (contact), then tot_cont_for
+    do c = 2, ncomp
+       write(u) (grid_list(c, i)%Mg, i = 1, nb_gridnode)
+       do j = 1, 3
+          write(u) (grid_list(c, i)%PXg(j), i = 1, nb_gridnode)
+       end do
+       do j = 1, 3
+          write(u) (grid_list(c, i)%FXg(j), i = 1, nb_gridnode)
+       end do
+    end do
+    if (Contact_type == 1 .and. istep > 0) then
+       write(u) tot_cont_for
+    else
+       write(u) 0d0, 0d0, 0d0
+    end if
     close(u)
   end subroutine dump_state
 

@@ -48,6 +48,11 @@ DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
          # SmoothStressbyGrid (smoo) on every shape family; the cube also
          # carries two grid components without contact
          "cube_smoo_linear_usl", "bar_smoo_gimp_usf", "slope_smoo_bspline_musl"]
+# contact (cont lagr): two blocks pressed together on grids of their own
+# (linear, USF, frictionless, averaged normal), sliding with friction
+# (B-spline, MUSL, normal of body 1, a J2 block), and a third body on a third
+# component (GIMP, USL, normal of body 2, gravity); reference-order FP64 only
+CONTACT_DECKS = ["blocks_contact_linear_usf", "blocks_contact_bspline_musl", "blocks_contact_gimp_usl"]
 # auxiliary-grid (SGMP) decks: clamped plate (MUSL), water column with
 # symmetric walls (MUSL), J2 block on a clamped floor (USF)
 SGMP_DECKS = ["plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf", "block_smoo_sgmp_usf"]

@@ -10,6 +10,8 @@ every deck in tests/golden/decks/ and stores, per deck:
 * ``g10``         grid state (Mg, PXg, FXg) after 10 steps;
 * ``icell10``     cell index of every particle in step 10;
 * ``fix``         node fixities;
+* ``gc10``        contact decks: the grids of components 2.. after 10 steps,
+  ``cont10`` the contact force tot_cont_for of step 10;
 * ``scal0`` / ``scal10``: DT, time, EngKinetic, EngInternal, Momentum(3),
   CutOff, DCell, Span(6), DTScale.
 
@@ -36,7 +38,8 @@ DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
          "block_dp_bspline_shear_usf", "block_dp_bspline_pull_musl",
          "taylor_sjc_eos2_musl", "cube_sjc_eos1_gimp_usl", "impact_sjcf_eos2_bspline_usf", "tnt_hiex_jwl_usl",
          "cube_drda_linear_usf", "cube_smoo_linear_usl", "bar_smoo_gimp_usf", "slope_smoo_bspline_musl",
-         "block_smoo_sgmp_usf"]
+         "block_smoo_sgmp_usf", "blocks_contact_linear_usf", "blocks_contact_bspline_musl",
+         "blocks_contact_gimp_usl"]
 S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp", "temp", "lt"]
 
 
@@ -59,6 +62,9 @@ def main(names):
             shutil.rmtree(tmp)
         out = {"scal0": scal(d0), "scal10": scal(d10), "g10": d10["grid"], "icell10": d10["icell"],
                "fix": d10["fix"], "ncell": np.array(d10["ncell"])}
+        if len(d10["grids"]) > 1:
+            out["gc10"] = np.stack(d10["grids"][1:])
+            out["cont10"] = d10["cont_force"]
         for f in S0_FIELDS:
             out["s0_" + f] = d0["particles"][f]
         for f, v in d10["particles"].items():

@@ -59,7 +59,8 @@ def test_fortran_host_matches_oracle(oracle_mod, name):
 
 
 @pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "dam_null_sgmp_musl",
-                                  "block_j2_sgmp_usf", "slope_smoo_bspline_musl", "block_smoo_sgmp_usf"])
+                                  "block_j2_sgmp_usf", "slope_smoo_bspline_musl", "block_smoo_sgmp_usf",
+                                  "blocks_contact_bspline_musl"])
 def test_reference_driver_runs_the_device_step(oracle_mod, name):
     """The reference's MPM3D.f90 loop, unchanged, over update_step_hip."""
     if not os.path.exists(DROPIN):
@@ -96,7 +97,7 @@ REFPROG = os.path.join(ROOT, "oracle", "_ref", "mpm3d_ref")
 
 
 @pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "block_dp_bspline_pull_musl", "dam_null_sgmp_musl",
-                                  "cube_smoo_linear_usl"])
+                                  "cube_smoo_linear_usl", "blocks_contact_gimp_usl"])
 def test_dropin_writes_the_reference_output_files(name):
     """The reference program itself (oracle/_ref/mpm3d_ref: MPM3D.f90 with its
     own update_step.f90, on the CPU) and the drop-in (the same main program
@@ -107,7 +108,8 @@ def test_dropin_writes_the_reference_output_files(name):
     but for the energy columns (the device's block sums against the
     reference's serial sums), which agree to the printed precision.  On the
     smoothing deck the ParaView frames (with the smoothed pressure the driver's
-    SmoothStressbyGrid call leaves) are the reference's too."""
+    SmoothStressbyGrid call leaves) are the reference's too, and on the
+    contact deck the contact force file."""
     if not (os.path.exists(DROPIN) and os.path.exists(REFPROG)):
         pytest.skip("reference program / drop-in not built (needs the reference sources at build time)")
     mpm = load_package()
@@ -134,6 +136,7 @@ def test_dropin_writes_the_reference_output_files(name):
             assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
             out[prog] = {k: open(os.path.join(d, name + k)).read().splitlines() for k in ("_anim.dat", "_curv.dat")}
             out[prog].update({f: open(os.path.join(d, f)).read() for f in os.listdir(d) if f.endswith(".vtu")})
+            out[prog]["ContforcPlot.dat"] = open(os.path.join(d, "ContforcPlot.dat")).read()
         ref, dev = out[REFPROG], out[DROPIN]
         assert len(ref["_anim.dat"]) > 4 * n and ref["_anim.dat"] == dev["_anim.dat"]
         vtu = sorted(f for f in ref if f.endswith(".vtu"))
@@ -141,6 +144,9 @@ def test_dropin_writes_the_reference_output_files(name):
         assert len(vtu) >= (4 if prob.smooth_stress else 0)
         for f in vtu:
             assert ref[f] == dev[f], f
+        assert ref["ContforcPlot.dat"] == dev["ContforcPlot.dat"]
+        if prob.contact:
+            assert ref["ContforcPlot.dat"].count("\n") >= 40
         rc, dc = ref["_curv.dat"], dev["_curv.dat"]
         assert len(rc) >= 40 and len(rc) == len(dc) and rc[0] == dc[0]
         # energy columns (the reference's two default curves and ours), by header name

@@ -15,7 +15,9 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import ALL_DECKS as DECKS, GOLDEN_DIR, deck_path, load_package
+from conftest import ALL_DECKS, CONTACT_DECKS, GOLDEN_DIR, deck_path, load_package
+
+DECKS = ALL_DECKS + CONTACT_DECKS
 
 PF_SKIP_NAN = ()
 
@@ -37,6 +39,10 @@ def test_oracle_matches_reference_fixture_bitexact(name, oracle_mod):
             f = key[4:]
             assert arr_eq(ps[mpm.PF[f]], g[key]), f
     assert arr_eq(o.grid(), g["g10"])
+    if "gc10" in g.files:
+        for c in range(1, o.nb_component()):
+            assert arr_eq(o.grid_component(c), g["gc10"][c - 1]), c
+        assert arr_eq(o.contact_force(), g["cont10"])
     assert np.array_equal(o.icell(), g["icell10"])
     assert np.array_equal(o.fix(), g["fix"])
     sc = o.scalars()
@@ -65,6 +71,11 @@ def test_oracle_matches_live_reference_200_steps(name, oracle_mod):
             for f, v in d["particles"].items():
                 assert arr_eq(ps[mpm.PF[f]], v), (s, f)
             assert arr_eq(o.grid(), d["grid"]), s
+            assert len(d["grids"]) == o.nb_component()
+            for c in range(1, len(d["grids"])):
+                assert arr_eq(o.grid_component(c), d["grids"][c]), (s, c)
+            if prob.contact == 1:
+                assert arr_eq(o.contact_force(), d["cont_force"]), s
             assert np.array_equal(o.icell(), d["icell"]), s
             sc = o.scalars()
             assert sc[2] == d["dt"] and sc[3] == d["ek"] and sc[4] == d["ei"], s

@@ -112,6 +112,9 @@ class OracleHost:
     def icell(self):
         return self.o.icell()
 
+    def contact_force(self):
+        return self.o.contact_force()
+
 
 # two sets of options: between them every curve option but 'fail' (which the
 # flang-built reference cannot write) and every animation option
@@ -136,6 +139,9 @@ OPTION_SETS = {
     # the smoothed pressure (smoo) in the ParaView frames: B-spline and SGMP
     "slope_smoo_bspline_musl": "para\noutr pres\ncurv pres 100\n",
     "block_smoo_sgmp_usf": "tecp\npara\noutr pres\ncurv pres 7\n",
+    # the contact force file (ContforcPlot.dat) of two contact decks
+    "blocks_contact_linear_usf": "tecp\noutr velx\ncurv velx 100\ncurv velx 1000\n",
+    "blocks_contact_gimp_usl": "para\ncurv disx 5\n",
 }
 
 
