@@ -84,7 +84,11 @@ enum mpm_pfield {
   PF_LT,                        /* lighting time of high explosive (Initial, DataIn.f90:1252-1260) */
   PF_FAIL,                      /* failure flag, 0 or 1 */
   PF_SPRE,                      /* Spre: smoothed pressure of SmoothStressbyGrid (update_step.f90:1876-2055) */
-  PF_COUNT
+  PF_XO, PF_YO, PF_ZO,          /* Xo: position at t = 0 (DataIn.f90:859-863), TLMPM's reference configuration */
+  PF_VOL0,                      /* VOL0: initial volume (Initial, DataIn.f90:1240) */
+  PF_PK,                        /* 9 entries: TLMPM's first Piola-Kirchhoff stress PK(1..9), the reference's */
+                                /* order xx yy zz yz xz xy zy zx yx (Constitution.f90:806-815) */
+  PF_COUNT = PF_PK + 9
 };
 
 /* Grid node fields of a grid state array (each nb_node doubles). */

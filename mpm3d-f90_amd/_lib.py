@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "libmpm3d_hip.so")
 PF_NAMES = (["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sm", "seqv",
              "sdxx", "sdyy", "sdzz", "sdyz", "sdxz", "sdxy", "sig_y", "epeff", "ie", "q", "cp"]
             + ["fp%d%d" % (i + 1, j + 1) for i in range(3) for j in range(3)]
-            + ["xx", "xy", "xz", "temp", "dmg", "lt", "fail", "spre"])
+            + ["xx", "xy", "xz", "temp", "dmg", "lt", "fail", "spre", "xo", "yo", "zo", "vol0"]
+            + ["pk%d" % (i + 1) for i in range(9)])
 PF = {n: i for i, n in enumerate(PF_NAMES)}
 PF_COUNT = len(PF_NAMES)
 GF_NAMES = ["mg", "px", "py", "pz", "fx", "fy", "fz"]

@@ -198,6 +198,7 @@ struct Deck {
   double qlt = 0, damp = 0, bq1 = 1.5, bq2 = (double)0.06f;
   bool dr_damping = false;
   bool smooth_stress = false;   // smoo: SmoothStressbyGrid every step (Spre, read by the ParaView writer)
+  bool tlmpm = false;           // TLmp: total-Lagrangian step sequence
   int contact = 0;              // cont: Contact_type (1 lagr: Lagr_NodContact; 2 others: no contact force)
   double fricfa = 0;            // friction coefficient
   int normbody = 0;             // contact normal from both bodies (0), component 1 (1) or 2 (2)
@@ -671,7 +672,7 @@ struct Deck {
         case 37: bspline = true; break;
         case 38: sgmp = true; break;
         case 39: smooth_stress = true; break;
-        case 40: fail("total-Lagrangian MPM ('TLmp') is not supported");
+        case 40: tlmpm = true; break;
         case 41: uvfp = true; break;
         case 42: quasi_load = true; qlt = tk.get_real(); break;
         case 43: quasi_damp = true; damp = tk.get_real(); break;
@@ -747,6 +748,15 @@ extern "C" int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, 
     p->dr_damping = d.dr_damping;
     p->smooth_stress = d.smooth_stress;
     p->nb_component = d.nb_component;
+    p->tlmpm = d.tlmpm;
+    if (d.tlmpm) {
+      // TLConstitution stops for every model but the neo-Hookean one
+      // (Constitution.f90:487-497)
+      for (int m = 0; m < d.nb_mat; m++)
+        if (d.mats[m].type != MPM_MAT_NEOH)
+          d.fail("total-Lagrangian MPM supports the neo-Hookean material only (material " + std::to_string(m + 1) + ")");
+      if (d.contact) d.fail("total-Lagrangian MPM together with contact is not supported");
+    }
     p->contact = d.contact;
     p->fricfa = d.fricfa;
     p->normbody = d.normbody;
@@ -771,6 +781,7 @@ extern "C" int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, 
     double* ps = p->pstate;
     for (int i = 0; i < n; i++) {
       for (int c = 0; c < 3; c++) {
+        ps[(size_t)(PF_XO + c) * n + i] = d.xp[c][i];   // Xo = Xp (DataIn.f90:859-863)
         ps[(size_t)(PF_X + c) * n + i] = d.xp[c][i];
         ps[(size_t)(PF_VX + c) * n + i] = d.vxp[c][i];
         ps[(size_t)(PF_FX + c) * n + i] = d.fxp[c][i];
@@ -785,6 +796,7 @@ extern "C" int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, 
         p->pbody[i] = b;
         double vol = ps[(size_t)PF_MASS * n + i] / mt.density;
         ps[(size_t)PF_VOL * n + i] = vol;
+        ps[(size_t)PF_VOL0 * n + i] = vol;
         ps[(size_t)PF_SIGY * n + i] = mt.yield0;
         ps[(size_t)PF_IE * n + i] = mt.ceos[9] * vol;
         if (mt.type == MPM_MAT_ELASTIC || mt.type == MPM_MAT_NEOH) {

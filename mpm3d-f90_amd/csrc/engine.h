@@ -58,7 +58,8 @@ enum InstFlags {
   IF_LOADS = 16,         // some particle carries a prescribed load FXp
   IF_ENDTIME = 32,       // instance stops itself when time > endtime
   IF_DRDA = 64,          // dynamic relaxation damping after UpdateDT (drda)
-  IF_SMOOTH = 128        // SmoothStressbyGrid at the end of every step (smoo)
+  IF_SMOOTH = 128,       // SmoothStressbyGrid at the end of every step (smoo)
+  IF_TL = 256            // total-Lagrangian step (TLmp): cells of the initial positions, mass of the first step
 };
 
 // per-instance time state, advanced on the device (graph-capturable)
@@ -116,8 +117,15 @@ enum DField {
   D_X = 0, D_Y, D_Z, D_VOL, D_IE, D_F0, D_F1, D_F2, D_F3, D_F4, D_F5, D_F6, D_F7, D_F8, D_MASS,
   D_TEMP, D_DMG, D_LT, D_FAIL,
   D_SPRE,                // smoothed pressure (SmoothStressbyGrid; output only)
+  D_XO, D_YO, D_ZO,      // position at t = 0 (total-Lagrangian reference configuration)
+  D_VOL0,                // initial volume
   D_COUNT
 };
+
+// first Piola-Kirchhoff stress of total-Lagrangian instances, PK(1..9) in
+// the reference's order (xx yy zz yz xz xy zy zx yx): a field-major block of
+// 9 doubles per particle slot, allocated with the first such instance
+#define TL_PK_COUNT 9
 
 // particle integer fields
 enum IField { I_BODY = 0, I_ORIG, I_CELL, I_COUNT };
@@ -178,5 +186,6 @@ struct PoolPtrs {
   int ilv;       // interleaved lane -> particle mapping in particle kernels
   int tile_cap;  // nodes a block's LDS tile may hold (<= TILE_NODES; smaller values exercise the fallbacks)
   int* boxes;    // node-box cache of the particle kernels: BOX_INTS ints per 64 particle slots
+  double* pk;    // TL_PK_COUNT * np: PK of total-Lagrangian instances (null without any)
   int box_mode;  // BOX_LOAD | BOX_STORE
 };

@@ -167,6 +167,8 @@ struct mpm_engine {
   uint8_t* fix = nullptr;
   void* aux = nullptr;            // SGMP auxiliary-node fields (allocated with the first SGMP instance)
   SmPtrs sm{};                    // SmoothStressbyGrid buffers (allocated with the first instance that smooths)
+  double* pk = nullptr;           // PK of total-Lagrangian instances (allocated with the first one)
+  double* pk_alt = nullptr;       //   and its sort buffer
   double* cmg = nullptr;
   InstC* d_inst = nullptr;
   InstT* d_ts = nullptr;
@@ -284,6 +286,7 @@ struct mpm_engine {
     P.ilv = ilv;
     P.tile_cap = tile_cap;
     P.boxes = boxes;
+    P.pk = pk;
     P.box_mode = box_mode;
     return P;
   }
@@ -300,7 +303,7 @@ static void free_engine(mpm_engine* e) {
                   e->v_out, e->rs_scr, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg, e->boxes,
                   e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1], e->x_rnode, e->x_rw, e->x_seg0, e->x_seg1,
                   e->x_scr, e->d_sub_ids, e->d_sub_rows, e->sm.gpre, e->sm.gv, e->sm.cgpre, e->sm.cgv, e->sm.map,
-                  e->sm.xs, e->x_ndir, e->x_cf};
+                  e->sm.xs, e->x_ndir, e->x_cf, e->pk, e->pk_alt};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& g : e->groups) {
@@ -465,6 +468,12 @@ static void check_problem(const mpm_problem* p) {
     if (p->shape == MPM_SHAPE_SGMP) fail(MPM_ERR_UNSUPPORTED, "contact with SGMP");
     if (p->smooth_stress) fail(MPM_ERR_UNSUPPORTED, "stress smoothing (smoo) together with contact");
   }
+  if (p->tlmpm) {   // TLConstitution has the neo-Hookean model only (Constitution.f90:487-497)
+    for (int m = 0; m < p->nb_mat; m++)
+      if (p->mats[m].type != MPM_MAT_NEOH)
+        fail(MPM_ERR_UNSUPPORTED, "total-Lagrangian MPM with material type %d (neo-Hookean only)", p->mats[m].type);
+    if (p->contact) fail(MPM_ERR_UNSUPPORTED, "total-Lagrangian MPM together with contact");
+  }
   long ncell = (long)p->ncx * p->ncy * p->ncz;
   long nnode = (long)(p->ncx + 1) * (p->ncy + 1) * (p->ncz + 1);
   if (nnode > 0x7fffffffL || ncell > 0x7fffffffL) fail(MPM_ERR_ARG, "grid too large");
@@ -560,6 +569,8 @@ static void upload_particles_T(mpm_engine* e, const HostInst& h, const double* p
     db[(size_t)D_LT * pad + i] = P(PF_LT, i);
     db[(size_t)D_FAIL * pad + i] = P(PF_FAIL, i);
     db[(size_t)D_SPRE * pad + i] = P(PF_SPRE, i);
+    for (int k = 0; k < 3; k++) db[(size_t)(D_XO + k) * pad + i] = P(PF_XO + k, i);
+    db[(size_t)D_VOL0 * pad + i] = P(PF_VOL0, i);
   }
   if (!first) {
     // upload keeps the device permutation: scatter host rows to their slots
@@ -580,6 +591,19 @@ static void upload_particles_T(mpm_engine* e, const HostInst& h, const double* p
                      hipMemcpyHostToDevice));
   for (int f = 0; f < D_COUNT; f++)
     HIPCHK(hipMemcpy(e->d + (size_t)f * np + b0, &db[(size_t)f * pad], pad * sizeof(double), hipMemcpyHostToDevice));
+  if (e->pk && (h.c.flags & IF_TL)) {   // PK of a total-Lagrangian instance, in the slots' order
+    std::vector<int> orig(pad);
+    HIPCHK(hipMemcpy(orig.data(), e->ii + (size_t)I_ORIG * np + b0, pad * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<double> kb((size_t)TL_PK_COUNT * pad, 0.0);
+    for (long j = 0; j < pad; j++) {
+      const long o = first ? j : (j < n ? orig[j] : -1);
+      if (o < 0 || o >= n) continue;
+      for (int k = 0; k < TL_PK_COUNT; k++) kb[(size_t)k * pad + j] = P(PF_PK + k, (int)o);
+    }
+    for (int k = 0; k < TL_PK_COUNT; k++)
+      HIPCHK(hipMemcpy(e->pk + (size_t)k * np + b0, &kb[(size_t)k * pad], pad * sizeof(double),
+                       hipMemcpyHostToDevice));
+  }
 }
 
 static void upload_particles(mpm_engine* e, const HostInst& h, const double* ps, int n, bool first) {
@@ -612,6 +636,13 @@ static void ensure_smooth(mpm_engine* e) {
   mark_graph_dirty(e);
 }
 
+static void ensure_tl(mpm_engine* e) {
+  if (e->pk) return;
+  e->pk = dalloc<double>((size_t)TL_PK_COUNT * e->np);
+  if (e->k_in) e->pk_alt = dalloc<double>((size_t)TL_PK_COUNT * e->np);   // sort buffers exist already
+  mark_graph_dirty(e);
+}
+
 static void ensure_contact(mpm_engine* e) {
   if (e->x_ndir) return;
   e->x_ndir = dalloc<double>(3 * e->nn);
@@ -624,6 +655,9 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   if (p->contact && !e->exact)
     fail(MPM_ERR_UNSUPPORTED, "contact between grid components runs in the reference-order FP64 mode "
                               "(MPM_PREC_FP64_EXACT)");
+  if (p->tlmpm && !e->exact)
+    fail(MPM_ERR_UNSUPPORTED, "total-Lagrangian MPM runs in the reference-order FP64 mode (MPM_PREC_FP64_EXACT)");
+  if (p->tlmpm) ensure_tl(e);   // before the admission kernel, which fills it
   if (e->free_ids.empty()) fail(MPM_ERR_NOMEM, "no free instance slot");
   const int n = p->nb_particle;
   const long npg = (n + PGRAN - 1) / PGRAN;
@@ -680,7 +714,7 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   c.scheme = p->scheme;
   c.flags = (p->quasi_load ? IF_QUASI_LOAD : 0) | (p->quasi_damp ? IF_QUASI_DAMP : 0) |
             (p->gravity ? IF_GRAVITY : 0) | (p->update_v_by_fp ? IF_UVFP : 0) | (p->dr_damping ? IF_DRDA : 0) |
-            (p->smooth_stress ? IF_SMOOTH : 0) | flags_extra;
+            (p->smooth_stress ? IF_SMOOTH : 0) | (p->tlmpm ? IF_TL : 0) | flags_extra;
   bool loads = false;
   for (int i = 0; i < n && !loads; i++)
     for (int d = 0; d < 3; d++)
@@ -1478,6 +1512,7 @@ static void ensure_sort_buffers(mpm_engine* e) {
     e->t_alt = dalloc<char>((size_t)T_COUNT * np * e->tsz);
     e->d_alt = dalloc<double>((size_t)D_COUNT * np);
     e->i_alt = dalloc<int>((size_t)I_COUNT * np);
+    if (e->pk) e->pk_alt = dalloc<double>((size_t)TL_PK_COUNT * np);
     // scratch of a radix sort over the whole pool, which covers the
     // single-instance sorts at admission (no reallocation there)
     e->rs_scr = dalloc<unsigned>(mpmk::radix_scratch_words(np));
@@ -1509,6 +1544,12 @@ static void sort_instance(mpm_engine* e, int id) {
     mpmk::launch_permute<double>((double*)e->t_alt, (const double*)e->t, perm, L, np, T_COUNT, e->s);
   mpmk::launch_permute<double>(e->d_alt, e->d, perm, L, np, D_COUNT, e->s);
   mpmk::launch_permute<int>(e->i_alt, e->ii, perm, L, np, I_COUNT, e->s);
+  if (e->pk) {
+    mpmk::launch_permute<double>(e->pk_alt, e->pk, perm, L, np, TL_PK_COUNT, e->s);
+    for (int f = 0; f < TL_PK_COUNT; f++)
+      HIPCHK(hipMemcpyAsync(e->pk + (size_t)f * np + b0, e->pk_alt + (size_t)f * np + b0, pad * sizeof(double),
+                            hipMemcpyDeviceToDevice, e->s));
+  }
   for (int f = 0; f < T_COUNT; f++)
     HIPCHK(hipMemcpyAsync((char*)e->t + ((size_t)f * np + b0) * e->tsz, (char*)e->t_alt + ((size_t)f * np + b0) * e->tsz,
                           pad * e->tsz, hipMemcpyDeviceToDevice, e->s));
@@ -1599,6 +1640,7 @@ static void sort_particles(mpm_engine* e) {
       mpmk::launch_permute<double>((double*)e->t_alt, (const double*)e->t, perm, L, np, T_COUNT, e->s);
     mpmk::launch_permute<double>(e->d_alt, e->d, perm, L, np, D_COUNT, e->s);
     mpmk::launch_permute<int>(e->i_alt, e->ii, perm, L, np, I_COUNT, e->s);
+    if (e->pk) mpmk::launch_permute<double>(e->pk_alt, e->pk, perm, L, np, TL_PK_COUNT, e->s);
   }
   // the cached node boxes describe the old particle order
   mpmk::launch_clear_ints(e->boxes, (e->np / 64) * BOX_INTS, e->s);
@@ -1606,6 +1648,8 @@ static void sort_particles(mpm_engine* e) {
   HIPCHK(hipMemcpyAsync(e->t, e->t_alt, (size_t)T_COUNT * np * e->tsz, hipMemcpyDeviceToDevice, e->s));
   HIPCHK(hipMemcpyAsync(e->d, e->d_alt, (size_t)D_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
   HIPCHK(hipMemcpyAsync(e->ii, e->i_alt, (size_t)I_COUNT * np * sizeof(int), hipMemcpyDeviceToDevice, e->s));
+  if (e->pk)
+    HIPCHK(hipMemcpyAsync(e->pk, e->pk_alt, (size_t)TL_PK_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
 }
 
 // --------------------------------------------------------- autotuner
@@ -1617,12 +1661,13 @@ struct Snapshot {
   double* d = nullptr;
   int* i = nullptr;
   InstT* ts = nullptr;
+  double* pk = nullptr;
   long steps = 0, since_sort = 0;
   std::vector<std::pair<bool, bool>> flags;
 };
 
 static void snap_free(Snapshot& S) {
-  void* b[] = {S.t, S.d, S.i, S.ts};
+  void* b[] = {S.t, S.d, S.i, S.ts, S.pk};
   for (void* p : b)
     if (p) (void)hipFree(p);
   S = Snapshot();
@@ -1638,6 +1683,10 @@ static void snap_take(mpm_engine* e, Snapshot& S) {
   HIPCHK(hipMemcpyAsync(S.d, e->d, (size_t)D_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
   HIPCHK(hipMemcpyAsync(S.i, e->ii, (size_t)I_COUNT * np * sizeof(int), hipMemcpyDeviceToDevice, e->s));
   HIPCHK(hipMemcpyAsync(S.ts, e->d_ts, sizeof(InstT) * e->max_inst, hipMemcpyDeviceToDevice, e->s));
+  if (e->pk) {
+    S.pk = dalloc<double>((size_t)TL_PK_COUNT * np);
+    HIPCHK(hipMemcpyAsync(S.pk, e->pk, (size_t)TL_PK_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
+  }
   S.steps = e->steps_done;
   S.since_sort = e->since_sort;
   S.flags.clear();
@@ -1650,6 +1699,8 @@ static void snap_restore(mpm_engine* e, const Snapshot& S) {
   HIPCHK(hipMemcpyAsync(e->d, S.d, (size_t)D_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
   HIPCHK(hipMemcpyAsync(e->ii, S.i, (size_t)I_COUNT * np * sizeof(int), hipMemcpyDeviceToDevice, e->s));
   HIPCHK(hipMemcpyAsync(e->d_ts, S.ts, sizeof(InstT) * e->max_inst, hipMemcpyDeviceToDevice, e->s));
+  if (S.pk)
+    HIPCHK(hipMemcpyAsync(e->pk, S.pk, (size_t)TL_PK_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
   e->steps_done = S.steps;
   e->since_sort = S.since_sort;
   for (size_t k = 0; k < e->inst.size() && k < S.flags.size(); k++) {
@@ -1848,6 +1899,13 @@ static void download_particles_T(mpm_engine* e, int id, double* out) {
     HIPCHK(hipMemcpy(&ib[(size_t)f * pad], e->ii + (size_t)f * np + b0, pad * sizeof(int), hipMemcpyDeviceToHost));
   InstT ts;
   HIPCHK(hipMemcpy(&ts, e->d_ts + id, sizeof ts, hipMemcpyDeviceToHost));
+  std::vector<double> pkb;   // PK: total-Lagrangian instances only (0 otherwise, as the reference keeps it)
+  if (e->pk && (h.c.flags & IF_TL)) {
+    pkb.resize((size_t)TL_PK_COUNT * pad);
+    for (int k = 0; k < TL_PK_COUNT; k++)
+      HIPCHK(hipMemcpy(&pkb[(size_t)k * pad], e->pk + (size_t)k * np + b0, pad * sizeof(double),
+                       hipMemcpyDeviceToHost));
+  }
   for (long j = 0; j < n; j++) {
     const int o = ib[(size_t)I_ORIG * pad + j];
     auto O = [&](int f) -> double& { return out[(size_t)f * n + o]; };
@@ -1869,6 +1927,9 @@ static void download_particles_T(mpm_engine* e, int id, double* out) {
     O(PF_LT) = db[(size_t)D_LT * pad + j];
     O(PF_FAIL) = db[(size_t)D_FAIL * pad + j];
     O(PF_SPRE) = db[(size_t)D_SPRE * pad + j];
+    for (int k = 0; k < 3; k++) O(PF_XO + k) = db[(size_t)(D_XO + k) * pad + j];
+    O(PF_VOL0) = db[(size_t)D_VOL0 * pad + j];
+    for (int k = 0; k < TL_PK_COUNT; k++) O(PF_PK + k) = pkb.empty() ? 0.0 : pkb[(size_t)k * pad + j];
     for (int k = 0; k < 3; k++) {
       const double x = db[(size_t)(D_X + k) * pad + j];
       O(PF_XX + k) = h.xx_pending ? x + Tv(T_BX + k) * ts.dt : x;
@@ -1934,6 +1995,9 @@ static void download_subset_T(mpm_engine* e, int id, int k, const int32_t* ids, 
     O(PF_LT) = Dv(D_LT);
     O(PF_FAIL) = Dv(D_FAIL);
     O(PF_SPRE) = Dv(D_SPRE);
+    for (int c = 0; c < 3; c++) O(PF_XO + c) = Dv(D_XO + c);
+    O(PF_VOL0) = Dv(D_VOL0);
+    for (int c = 0; c < TL_PK_COUNT; c++) O(PF_PK + c) = 0.0;   // not gathered (mpm_download_particles has it)
     for (int c = 0; c < 3; c++) O(PF_XX + c) = h.xx_pending ? Dv(D_X + c) + Tv(T_BX + c) * ts.dt : Dv(D_X + c);
   }
 }

@@ -56,6 +56,9 @@
 // flag bit of the material mask: the neo-Hookean ln J is the correctly
 // rounded logarithm (crlog.h) instead of the device libm's
 #define MM_CRLOG (1 << 30)
+// flag bit of the material mask: total-Lagrangian instances may be met
+// (TLConstitution; reference-order mode only)
+#define MM_TL (1 << 29)
 
 // k_finalize parts
 #define FIN_ENERGY 1

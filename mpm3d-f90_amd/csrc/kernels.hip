@@ -2160,10 +2160,102 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
 // ParticleStressUpdate after the gather (update_step.f90:566-642): strain
 // and spin increments, F update (UVFp), Constitution, and Xp = XX for
 // USL/MUSL.  L is the velocity gradient sum (deFp before the DT scaling).
+// TLParticleStressUpdate's update of one particle (update_step.f90:785-826)
+// with TLConstitution and M3DM11 (Constitution.f90:369-525, 782-882): F +=
+// L DT (the gradient is material: no product), V = det(F) V0, the first
+// Piola-Kirchhoff stress P = G (F - F^-T) + lambda ln J F^-T and its Cauchy
+// stress P F^T / J, the internal energy from the mid-step P against the
+// increment of F.  TLConstitution stores back sig_y and epeff from module
+// variables no total-Lagrangian run sets (0), and q = 0.  ln J is the
+// correctly rounded logarithm (the mode's).
+template <typename T>
+__device__ __noinline__ void tl_stress_update(const PoolPtrs<T>& P, const PV<T>& p, const InstC& I, double dtd,
+                                              const T L[3][3], bool uvfp, const MatC& mc, InstT* tsp) {
+  double dF[3][3], F[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      dF[a][b] = (double)L[a][b] * dtd;
+      double f = p.d(D_F0 + 3 * a + b);
+      if (uvfp) {
+        f = f + dF[a][b];
+        p.d(D_F0 + 3 * a + b) = f;
+      }
+      F[a][b] = f;
+    }
+#pragma unroll
+  for (int a = 0; a < 3; a++) F[a][a] = F[a][a] + 1.0;
+  const double vold = p.d(D_VOL);
+  const double J = F[0][0] * F[1][1] * F[2][2] - F[0][0] * F[1][2] * F[2][1] - F[1][0] * F[0][1] * F[2][2] +
+                   F[1][0] * F[0][2] * F[2][1] + F[2][0] * F[0][1] * F[1][2] - F[2][0] * F[0][2] * F[1][1];
+  const double vol = J * p.d(D_VOL0);
+  p.d(D_VOL) = vol;
+  const double G = 0.5 * mc.G2, lamda = mc.lamda;
+  // slots 1..9: 11 22 33 23 13 12 32 31 21
+  const double f[9] = {F[0][0], F[1][1], F[2][2], F[1][2], F[0][2], F[0][1], F[2][1], F[2][0], F[1][0]};
+  double fit[9];
+  fit[0] = (f[1] * f[2] - f[3] * f[6]) / J;
+  fit[1] = (f[0] * f[2] - f[4] * f[7]) / J;
+  fit[2] = (f[0] * f[1] - f[5] * f[8]) / J;
+  fit[3] = (f[5] * f[7] - f[0] * f[6]) / J;
+  fit[4] = (f[8] * f[6] - f[1] * f[7]) / J;
+  fit[5] = (f[3] * f[7] - f[2] * f[8]) / J;
+  fit[6] = (f[4] * f[8] - f[3] * f[0]) / J;
+  fit[7] = (f[3] * f[5] - f[4] * f[1]) / J;
+  fit[8] = (f[4] * f[6] - f[5] * f[2]) / J;
+  const double lnJ = mpm_log_cr(J);
+  const long np = P.np, i = p.i;
+  double pk[9], ieinc = 0.0;
+  const double dfp[9] = {dF[0][0], dF[1][1], dF[2][2], dF[1][2], dF[0][2], dF[0][1], dF[2][1], dF[2][0], dF[1][0]};
+#pragma unroll
+  for (int q = 0; q < 9; q++) {
+    pk[q] = G * (f[q] - fit[q]) + lamda * lnJ * fit[q];
+    const double old = P.pk[q * np + i];
+    ieinc = ieinc + 0.5 * (old + pk[q]) * dfp[q];
+    P.pk[q * np + i] = pk[q];
+  }
+  const double ft[9] = {f[0], f[1], f[2], f[6], f[7], f[8], f[3], f[4], f[5]};
+  double sig[6];
+  sig[0] = (pk[0] * ft[0] + pk[5] * ft[8] + pk[4] * ft[7]) / J;
+  sig[1] = (pk[8] * ft[5] + pk[1] * ft[1] + pk[3] * ft[6]) / J;
+  sig[2] = (pk[7] * ft[4] + pk[6] * ft[3] + pk[2] * ft[2]) / J;
+  sig[3] = (pk[8] * ft[4] + pk[1] * ft[3] + pk[3] * ft[2]) / J;
+  sig[4] = (pk[0] * ft[4] + pk[5] * ft[3] + pk[4] * ft[2]) / J;
+  sig[5] = (pk[0] * ft[5] + pk[5] * ft[1] + pk[4] * ft[6]) / J;
+  Cst<T> m;
+  m.sm = (T)((sig[0] + sig[1] + sig[2]) / 3.0);
+  m.sd[0] = (T)(sig[0] - (double)m.sm);
+  m.sd[1] = (T)(sig[1] - (double)m.sm);
+  m.sd[2] = (T)(sig[2] - (double)m.sm);
+  m.sd[3] = (T)sig[3];
+  m.sd[4] = (T)sig[4];
+  m.sd[5] = (T)sig[5];
+  p.d(D_IE) = p.d(D_IE) + ieinc * 0.5 * (vol + vold);
+  p.t(T_SM) = m.sm;
+  p.t(T_SEQV) = eq_stress<T>(m);
+#pragma unroll
+  for (int k = 0; k < 6; k++) p.t(T_SD0 + k) = m.sd[k];
+  p.t(T_SIGY) = (T)0;
+  p.t(T_EPEFF) = (T)0;
+  p.t(T_Q) = (T)0;
+  if (vol < 0.0) atomicAdd(&tsp->negvol_count, 1);
+  if (!isfinite(vol) || !isfinite((double)m.sm)) atomicAdd(&tsp->nan_count, 1);
+}
+
 template <typename T, int MM>
 __device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p, const InstC& I, const InstT& ts,
                                             T de[6], T vort[3], const T L[3][3], bool uvfp, double x, double y,
                                             double z, bool move, const MatC& mc, Cst<T>& m, InstT* tsp) {
+  if ((MM & MM_TL) && (I.flags & IF_TL)) {
+    tl_stress_update<T>(P, p, I, ts.dt, L, uvfp, mc, tsp);
+    if (move) {   // Xp = XX (update_step.f90:824)
+      p.d(D_X) = x + (double)p.t(T_BX) * ts.dt;
+      p.d(D_Y) = y + (double)p.t(T_BY) * ts.dt;
+      p.d(D_Z) = z + (double)p.t(T_BZ) * ts.dt;
+    }
+    return;
+  }
     const double dtd = ts.dt;
     const T dt = (T)dtd;
 #pragma unroll
@@ -2721,12 +2813,22 @@ __global__ void __launch_bounds__(256) k_sgmp_nodes(PoolPtrs<T> P, const int* ng
         if (WHAT & SGN_FORCE) {
           const int jl = cell_local(nx - cx, ny - cy, nz - cz);
           const T DX = (T)SGN[0][jl] * ij4, DY = (T)SGN[1][jl] * ij4, DZ = (T)SGN[2][jl] * ij4;
-          T fi[6];
+          T f0, f1, f2;
+          if (I.flags & IF_TL) {   // CPKint (9, PK order xx yy zz yz xz xy zy zx yx), update_step.f90:1174-1178
+            T q[9];
 #pragma unroll
-          for (int q = 0; q < 6; q++) fi[q] = aux[(AX_FI + q) * nn + c];
-          const T f0 = -(fi[0] * DX + fi[3] * DY + fi[5] * DZ);
-          const T f1 = -(fi[3] * DX + fi[1] * DY + fi[4] * DZ);
-          const T f2 = -(fi[5] * DX + fi[4] * DY + fi[2] * DZ);
+            for (int k = 0; k < 9; k++) q[k] = aux[(AX_FI + k) * nn + c];
+            f0 = -(q[0] * DX + q[5] * DY + q[4] * DZ);
+            f1 = -(q[8] * DX + q[1] * DY + q[3] * DZ);
+            f2 = -(q[7] * DX + q[6] * DY + q[2] * DZ);
+          } else {
+            T fi[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) fi[q] = aux[(AX_FI + q) * nn + c];
+            f0 = -(fi[0] * DX + fi[3] * DY + fi[5] * DZ);
+            f1 = -(fi[3] * DX + fi[1] * DY + fi[4] * DZ);
+            f2 = -(fi[5] * DX + fi[4] * DY + fi[2] * DZ);
+          }
           fv[0] = fv[0] + (T)0.125 * aux[(AX_FE + 0) * nn + c] + f0;
           fv[1] = fv[1] + (T)0.125 * aux[(AX_FE + 1) * nn + c] + f1;
           fv[2] = fv[2] + (T)0.125 * aux[(AX_FE + 2) * nn + c] + f2;
@@ -2860,7 +2962,10 @@ __global__ void __launch_bounds__(256) k_sgmp_stress(PoolPtrs<T> P, const int* g
   const bool live = i < I.p_end;
   SgLoc L;
   const double x = live ? p.d(D_X) : 0.0, y = live ? p.d(D_Y) : 0.0, z = live ? p.d(D_Z) : 0.0;
-  const bool act = live && sgmp_locate(I, P.fix, x, y, z, L);
+  // a total-Lagrangian instance keeps the cells of its initial positions (TLParticleInitial)
+  const bool tl = (MM & MM_TL) && (I.flags & IF_TL);
+  const bool act = live && (tl ? sgmp_locate(I, P.fix, p.d(D_XO), p.d(D_YO), p.d(D_ZO), L)
+                               : sgmp_locate(I, P.fix, x, y, z, L));
   if (act) {
     T shp[8];
     sgn_shape<T>(L.xxn, shp);
@@ -2909,7 +3014,10 @@ __global__ void __launch_bounds__(256) k_sgmp_g2p(PoolPtrs<T> P, const int* glis
   const bool live = i < I.p_end;
   SgLoc L;
   const double x = live ? p.d(D_X) : 0.0, y = live ? p.d(D_Y) : 0.0, z = live ? p.d(D_Z) : 0.0;
-  const bool act = live && sgmp_locate(I, P.fix, x, y, z, L);
+  // a total-Lagrangian instance keeps the cells (and XN, XXN) of its initial positions
+  const bool tl = (I.flags & IF_TL) != 0;
+  const double lx = tl && live ? p.d(D_XO) : x, ly = tl && live ? p.d(D_YO) : y, lz = tl && live ? p.d(D_ZO) : z;
+  const bool act = live && sgmp_locate(I, P.fix, lx, ly, lz, L);
   if (act) {
     double xi[3] = {L.xxn[0], L.xxn[1], L.xxn[2]};
     if (L.bd) {
@@ -2920,7 +3028,7 @@ __global__ void __launch_bounds__(256) k_sgmp_g2p(PoolPtrs<T> P, const int* glis
         int cx, cy, cz;
         cell_coords(I, c, cx, cy, cz);
         const int cc[3] = {cx, cy, cz};
-        const double X[3] = {x, y, z};
+        const double X[3] = {lx, ly, lz};
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           const double cxg = ((double)cc[d] * I.dcell + I.span0[d]) + I.dcell / 2.0;
@@ -3235,6 +3343,12 @@ __global__ void __launch_bounds__(256) k_admit(PoolPtrs<T> P, const double* ps, 
   d[D_LT * np + s] = F(PF_LT);
   d[D_FAIL * np + s] = F(PF_FAIL);
   d[D_SPRE * np + s] = F(PF_SPRE);
+  d[D_XO * np + s] = F(PF_XO);
+  d[D_YO * np + s] = F(PF_YO);
+  d[D_ZO * np + s] = F(PF_ZO);
+  d[D_VOL0 * np + s] = F(PF_VOL0);
+  if (P.pk)
+    for (int k = 0; k < TL_PK_COUNT; k++) P.pk[k * np + s] = F(PF_PK + k);
   P.i[I_BODY * np + s] = in ? body0 + pbody[j] : body0;
   P.i[I_ORIG * np + s] = (int)j;
   P.i[I_CELL * np + s] = -1;

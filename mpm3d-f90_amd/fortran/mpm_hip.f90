@@ -19,7 +19,8 @@ module mpm_hip
                         PF_SEQV = 12, PF_SDXX = 13, PF_SDYY = 14, PF_SDZZ = 15, PF_SDYZ = 16, &
                         PF_SDXZ = 17, PF_SDXY = 18, PF_SIGY = 19, PF_EPEFF = 20, PF_IE = 21, &
                         PF_Q = 22, PF_CP = 23, PF_FP = 24, PF_XX = 33, PF_TEMP = 36, PF_DMG = 37, &
-                        PF_LT = 38, PF_FAIL = 39, PF_SPRE = 40, PF_COUNT = 41
+                        PF_LT = 38, PF_FAIL = 39, PF_SPRE = 40, PF_XO = 41, PF_VOL0 = 44, PF_PK = 45, &
+                        PF_COUNT = 54
 
   type, bind(C) :: mpm_material_t
      integer(c_int32_t) :: mtype, eos_type

@@ -29,9 +29,13 @@
 ! contact decks run in the reference-order FP64 mode (the only mode with
 ! grid components), and tot_cont_for is refreshed for OutCurve.
 !
+! The total-Lagrangian routines (TLmp decks, MPM3D.f90:162-250) map to the
+! same entry points: the context knows the problem is total-Lagrangian
+! (GridMassInitial only creates it: the device takes the initial mass in the
+! first step), and such decks run in the reference-order FP64 mode.
+!
 ! Compiles against the reference's module files (ParticleData, GridData,
-! MaterialData); total-Lagrangian routines are present only to satisfy the
-! link and stop when called.
+! MaterialData).
 ! ---------------------------------------------------------------------------
 module update_step_hip_state
   use, intrinsic :: iso_c_binding
@@ -93,7 +97,7 @@ contains
     prob%dr_damping = merge(1, 0, DR_DAMPING)
     prob%smooth_stress = merge(1, 0, SmoothStress)
     prob%nb_component = nb_component
-    prob%tlmpm = 0
+    prob%tlmpm = merge(1, 0, TLMPM)
     prob%contact = 0
     if (contact) prob%contact = Contact_type
     prob%normbody = normbody
@@ -181,8 +185,8 @@ contains
     call get_environment_variable('MPM3D_PRECISION', env)
     if (trim(env) == 'fp32') prec = MPM_PREC_FP32
     if (trim(env) == 'fp64exact') prec = MPM_PREC_FP64_EXACT
-    if (contact .and. prec /= MPM_PREC_FP64_EXACT) then
-       write(*, '(a)') ' update_step_hip: contact runs in the reference-order FP64 mode (fp64exact)'
+    if ((contact .or. TLMPM) .and. prec /= MPM_PREC_FP64_EXACT) then
+       write(*, '(a)') ' update_step_hip: contact and total-Lagrangian decks run in the reference-order FP64 mode'
        prec = MPM_PREC_FP64_EXACT
     end if
     call mpm_check(mpm_ctx_create(c_loc(prob), prec, ctx), 'mpm_ctx_create')
@@ -220,6 +224,11 @@ contains
          ps(PF_LT * n + i) = p%LT
          ps(PF_FAIL * n + i) = merge(1d0, 0d0, p%failure)
          ps(PF_SPRE * n + i) = p%Spre
+         ps((PF_XO + 0) * n + i) = p%Xo(1); ps((PF_XO + 1) * n + i) = p%Xo(2); ps((PF_XO + 2) * n + i) = p%Xo(3)
+         ps(PF_VOL0 * n + i) = p%VOL0
+         do a = 1, 9
+            ps((PF_PK + a - 1) * n + i) = p%PK(a)
+         end do
        end associate
     end do
   end subroutine pack_particles
@@ -254,6 +263,9 @@ contains
          p%DMG = ps(PF_DMG * n + i)
          p%failure = ps(PF_FAIL * n + i) /= 0d0
          p%Spre = ps(PF_SPRE * n + i)
+         do a = 1, 9
+            p%PK(a) = ps((PF_PK + a - 1) * n + i)
+         end do
        end associate
     end do
   end subroutine unpack_particles
@@ -411,6 +423,37 @@ subroutine UpdateDT()
   call maybe_dump()
 end subroutine UpdateDT
 
+subroutine GridMassInitial()
+  use update_step_hip_state
+  implicit none
+  call ensure_context()
+end subroutine GridMassInitial
+
+subroutine TLGridMomentumInitial()
+  use update_step_hip_state
+  implicit none
+  call ensure_context()
+  call mpm_check(mpm_begin_step(ctx), 'begin_step')
+  call mpm_check(mpm_grid_momentum_initial(ctx), 'TLGridMomentumInitial')
+end subroutine TLGridMomentumInitial
+
+subroutine TLParticleStressUpdate()
+  use update_step_hip_state
+  implicit none
+  call mpm_check(mpm_particle_stress_update(ctx), 'TLParticleStressUpdate')
+end subroutine TLParticleStressUpdate
+
+subroutine TLGridMomentumUpdate()
+  use update_step_hip_state
+  implicit none
+  call mpm_check(mpm_grid_momentum_update(ctx), 'TLGridMomentumUpdate')
+end subroutine TLGridMomentumUpdate
+
+subroutine TLSmoothStressbyGrid()
+  implicit none
+  call SmoothStressbyGrid()
+end subroutine TLSmoothStressbyGrid
+
 subroutine Lagr_NodContact()
   use update_step_hip_state
   implicit none
@@ -432,20 +475,3 @@ subroutine SmoothStressbyGrid()
   end do
   call maybe_dump()
 end subroutine SmoothStressbyGrid
-
-! --- routines of the reference outside the device path (link only) --------
-subroutine TLSmoothStressbyGrid()
-  stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
-end subroutine TLSmoothStressbyGrid
-subroutine GridMassInitial()
-  stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
-end subroutine GridMassInitial
-subroutine TLGridMomentumInitial()
-  stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
-end subroutine TLGridMomentumInitial
-subroutine TLParticleStressUpdate()
-  stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
-end subroutine TLParticleStressUpdate
-subroutine TLGridMomentumUpdate()
-  stop 'update_step_hip: total-Lagrangian MPM is not available in the device build'
-end subroutine TLGridMomentumUpdate

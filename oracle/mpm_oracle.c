@@ -40,6 +40,10 @@ struct oracle {
   double* ndir;         /* 3 * nntot: CP_list%ndir, the contact normals (mass gradients) */
   double cont_force[3]; /* tot_cont_for of the last Lagr_NodContact */
   long cont_count[3];   /* diagnostics: contact nodes without friction, sticking, slipping (all steps) */
+  /* total-Lagrangian MPM (TLmp): GridMassInitial has run; SGMP's per-cell
+   * first Piola-Kirchhoff force sums CPKint (9 per cell) */
+  int tl_started;
+  double* cpk;
   double* g;            /* GF_COUNT * nntot */
   int8_t* fix;          /* 3 * nnode */
   double ijac, ijac4, idcell;
@@ -539,7 +543,7 @@ oracle* oracle_create(const mpm_problem* p) {
 void oracle_destroy(oracle* o) {
   if (!o) return;
   free(o->mats); free(o->bodies); free(o->ps); free(o->pbody); free(o->icell);
-  free(o->xn); free(o->g); free(o->fix); free(o->dpbr); free(o->pcomp); free(o->ndir);
+  free(o->xn); free(o->g); free(o->fix); free(o->dpbr); free(o->pcomp); free(o->ndir); free(o->cpk);
   free(o->bdt); free(o->cicell); free(o->inode); free(o->xxn);
   free(o->cmg); free(o->cp); free(o->cfint); free(o->cfext); free(o->co); free(o->cw); free(o->cdefp);
   free(o->cv); free(o->ca);
@@ -1654,6 +1658,330 @@ void oracle_update_dt(oracle* o) {
   dr_damping(o);
 }
 
+
+/* ================================================================ TLMPM */
+
+/* TLParticleInitial (update_step.f90:127-172): the cells from the initial
+ * positions Xo, once; SGMP also the auxiliary cell, nearest node, XXN
+ * (snapped on boundary nodes) and XN.  Without SGMP the routine leaves XN
+ * as it was: the reference never sets it in a total-Lagrangian run, so its
+ * linear shape functions read the particle list's initial memory (zero for
+ * the freshly mapped allocation of every deck the reference runs here);
+ * the oracle's XN stays 0 likewise. */
+static void tl_particle_initial(oracle* o) {
+  const double* sp = o->prob.span;
+  const double h = o->prob.dcell;
+  for (int p = 0; p < o->n; p++) {
+    const double x[3] = {P(o, PF_XO, p), P(o, PF_YO, p), P(o, PF_ZO, p)};
+    o->icell[p] = in_which_cell(o, x);
+    if (!o->sgmp) continue;
+    o->cicell[p] = center_in_which_cell(o, x);
+    int ix = (int)((x[0] - sp[0]) / h + 0.5) + 1;
+    int iy = (int)((x[1] - sp[2]) / h + 0.5) + 1;
+    int iz = (int)((x[2] - sp[4]) / h + 0.5) + 1;
+    int nd = (iz - 1) * o->ngxy + (iy - 1) * o->ngx + ix;
+    o->inode[p] = nd;
+    double xg[3];
+    if (nd >= 1 && nd <= o->nnode) {
+      node_xg(o, nd, xg);
+      for (int d = 0; d < 3; d++) {
+        double v = (x[d] - xg[d]) * o->ijac;
+        if (o->bdt[(size_t)d * o->nnode + nd - 1] != 0) {
+          if (v > 0) v = 1;
+          else if (v < 0) v = -1;
+        }
+        o->xxn[3 * p + d] = v;
+      }
+    }
+    if (o->icell[p] > 0) {
+      node_xg(o, cell_node1(o, o->icell[p]), xg);
+      for (int d = 0; d < 3; d++) o->xn[3 * p + d] = (x[d] - (xg[d] + h / 2.0)) * o->ijac;
+    }
+  }
+}
+
+/* GridMassInitial (update_step.f90:22-125): the nodal mass of the initial
+ * configuration, once before the first step; it is never reset after */
+void oracle_tl_grid_mass_initial(oracle* o) {
+  for (int k = 0; k < o->nntot; k++) G(o, GF_MG, k) = 0.0;
+  tl_particle_initial(o);
+  int cn[8];
+  if (o->sgmp) {
+    memset(o->cmg, 0, (size_t)o->ncell * sizeof(double));
+    double shp[8];
+    for (int p = 0; p < o->n; p++) {
+      if (o->cicell[p] < 0) continue;
+      double mp = P(o, PF_MASS, p);
+      center_cell_nodes(o, o->cicell[p], cn);
+      sgn_shape(&o->xxn[3 * p], shp);
+      for (int i = 0; i < 8; i++)
+        if (in_aux(o, cn[i])) o->cmg[cn[i] - 1] = o->cmg[cn[i] - 1] + shp[i] * mp;
+    }
+    for (int c = 1; c <= o->ncell; c++) {
+      if (!(o->cmg[c - 1] > o->prob.cutoff)) continue;
+      cell_nodes(o, c, cn);
+      for (int j = 0; j < 8; j++) G(o, GF_MG, cn[j] - 1) = G(o, GF_MG, cn[j] - 1) + 0.125 * o->cmg[c - 1];
+    }
+  } else {
+    shape_t s;
+    for (int p = 0; p < o->n; p++) {
+      if (o->icell[p] < 0) continue;
+      double mp = P(o, PF_MASS, p);
+      shape_of(o, p, &s);
+      for (int i = 0; i < s.nb; i++) {
+        if (!in_grid(o, s.node[i])) continue;
+        const int k = PNODE(o, p, s.node[i] - 1);
+        G(o, GF_MG, k) = G(o, GF_MG, k) + s.shp[i] * mp;
+      }
+    }
+  }
+  o->tl_started = 1;
+}
+
+/* TLGridMomentumInitial (update_step.f90:174-286): the nodal momentum only
+ * (the mass is GridMassInitial's); shape functions of the fixed cells */
+void oracle_tl_grid_momentum_initial(oracle* o) {
+  for (int k = 0; k < o->nntot; k++) G(o, GF_PX, k) = G(o, GF_PY, k) = G(o, GF_PZ, k) = 0.0;
+  int cn[8];
+  if (o->sgmp) {
+    memset(o->cp, 0, 3 * (size_t)o->ncell * sizeof(double));
+    double shp[8];
+    for (int p = 0; p < o->n; p++) {
+      if (o->cicell[p] < 0) continue;
+      double mp = P(o, PF_MASS, p);
+      center_cell_nodes(o, o->cicell[p], cn);
+      sgn_shape(&o->xxn[3 * p], shp);
+      for (int i = 0; i < 8; i++) {
+        if (!in_aux(o, cn[i])) continue;
+        const int k = cn[i] - 1;
+        const double shm = shp[i] * mp;
+        for (int d = 0; d < 3; d++) o->cp[3 * k + d] = o->cp[3 * k + d] + P(o, PF_VX + d, p) * shm;
+      }
+    }
+    for (int c = 1; c <= o->ncell; c++) {
+      if (!(o->cmg[c - 1] > o->prob.cutoff)) continue;
+      cell_nodes(o, c, cn);
+      for (int j = 0; j < 8; j++)
+        for (int d = 0; d < 3; d++)
+          G(o, GF_PX + d, cn[j] - 1) = G(o, GF_PX + d, cn[j] - 1) + 0.125 * o->cp[3 * (c - 1) + d];
+    }
+    return;
+  }
+  shape_t s;
+  for (int p = 0; p < o->n; p++) {
+    if (o->icell[p] < 0) continue;
+    double mp = P(o, PF_MASS, p);
+    shape_of(o, p, &s);
+    for (int i = 0; i < s.nb; i++) {
+      if (!in_grid(o, s.node[i])) continue;
+      const int k = PNODE(o, p, s.node[i] - 1);
+      const double shm = s.shp[i] * mp;
+      for (int d = 0; d < 3; d++) G(o, GF_PX + d, k) = G(o, GF_PX + d, k) + P(o, PF_VX + d, p) * shm;
+    }
+  }
+}
+
+/* TLConstitution with M3DM11 (Constitution.f90:369-525, 782-882): the only
+ * total-Lagrangian model, neo-Hookean in the first Piola-Kirchhoff stress.
+ * V = det(F) V0; P = G (F - F^-T) + lambda ln J F^-T; Cauchy = P F^T / J;
+ * internal energy by the mid-step PK stress against the increment of F.
+ * The routine loads neither sig_y nor epeff from the particle but stores
+ * the module's values back, which no total-Lagrangian run ever sets: both
+ * become 0; q is 0. */
+static void tl_constitution(oracle* o, int p, const double deF[3][3]) {
+  const mpm_material* mat = &o->mats[o->bodies[o->pbody[p]].mat - 1];
+  double F[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) F[i][j] = P(o, PF_FP + 3 * i + j, p);
+  for (int i = 0; i < 3; i++) F[i][i] = F[i][i] + 1.0;
+  const double vold = P(o, PF_VOL, p);
+  const double detF = F[0][0] * F[1][1] * F[2][2] - F[0][0] * F[1][2] * F[2][1] - F[1][0] * F[0][1] * F[2][2] +
+                      F[1][0] * F[0][2] * F[2][1] + F[2][0] * F[0][1] * F[1][2] - F[2][0] * F[0][2] * F[1][1];
+  P(o, PF_VOL, p) = detF * P(o, PF_VOL0, p);
+  const double vol = P(o, PF_VOL, p);
+  const double DMG = P(o, PF_DMG, p);
+  const double young = mat->young * (1.0 - DMG + 0x1p-52), poisson = mat->poisson;
+  const double G2 = young / (1 + poisson);
+  const double lamda = (young * poisson) / ((1 + poisson) * (1 - 2 * poisson));
+  const double G = 0.5 * G2;
+  const double J = detF;
+  /* F(1..9) = F11 F22 F33 F23 F13 F12 F32 F31 F21 */
+  const double f[9] = {F[0][0], F[1][1], F[2][2], F[1][2], F[0][2], F[0][1], F[2][1], F[2][0], F[1][0]};
+  double fit[9];
+  fit[0] = (f[1] * f[2] - f[3] * f[6]) / J;
+  fit[1] = (f[0] * f[2] - f[4] * f[7]) / J;
+  fit[2] = (f[0] * f[1] - f[5] * f[8]) / J;
+  fit[3] = (f[5] * f[7] - f[0] * f[6]) / J;
+  fit[4] = (f[8] * f[6] - f[1] * f[7]) / J;
+  fit[5] = (f[3] * f[7] - f[2] * f[8]) / J;
+  fit[6] = (f[4] * f[8] - f[3] * f[0]) / J;
+  fit[7] = (f[3] * f[5] - f[4] * f[1]) / J;
+  fit[8] = (f[4] * f[6] - f[5] * f[2]) / J;
+  const double lnJ = o->log_cr ? mpm_log_cr(J) : log(J);
+  double pk[9], pko[9];
+  for (int i = 0; i < 9; i++) {
+    pko[i] = P(o, PF_PK + i, p);
+    pk[i] = G * (f[i] - fit[i]) + lamda * lnJ * fit[i];
+  }
+  /* FT = F^T in the same 9-slot order */
+  const double ft[9] = {f[0], f[1], f[2], f[6], f[7], f[8], f[3], f[4], f[5]};
+  double sig[6];
+  sig[0] = (pk[0] * ft[0] + pk[5] * ft[8] + pk[4] * ft[7]) / J;
+  sig[1] = (pk[8] * ft[5] + pk[1] * ft[1] + pk[3] * ft[6]) / J;
+  sig[2] = (pk[7] * ft[4] + pk[6] * ft[3] + pk[2] * ft[2]) / J;
+  sig[3] = (pk[8] * ft[4] + pk[1] * ft[3] + pk[3] * ft[2]) / J;
+  sig[4] = (pk[0] * ft[4] + pk[5] * ft[3] + pk[4] * ft[2]) / J;
+  sig[5] = (pk[0] * ft[5] + pk[5] * ft[1] + pk[4] * ft[6]) / J;
+  cst_t m;
+  memset(&m, 0, sizeof m);
+  m.sm = (sig[0] + sig[1] + sig[2]) / 3.0;
+  m.sd[0] = sig[0] - m.sm;
+  m.sd[1] = sig[1] - m.sm;
+  m.sd[2] = sig[2] - m.sm;
+  m.sd[3] = sig[3];
+  m.sd[4] = sig[4];
+  m.sd[5] = sig[5];
+  const double seqv = equivalent_stress(&m);
+  const double dfp[9] = {deF[0][0], deF[1][1], deF[2][2], deF[1][2], deF[0][2], deF[0][1], deF[2][1], deF[2][0],
+                         deF[1][0]};
+  double ieinc = 0.0;
+  for (int i = 0; i < 9; i++) ieinc = ieinc + 0.5 * (pko[i] + pk[i]) * dfp[i];
+  P(o, PF_IE, p) = P(o, PF_IE, p) + ieinc * 0.5 * (vol + vold);
+  for (int i = 0; i < 9; i++) P(o, PF_PK + i, p) = pk[i];
+  P(o, PF_SM, p) = m.sm;
+  P(o, PF_SEQV, p) = seqv;
+  P(o, PF_SDXX, p) = m.sd[0];
+  P(o, PF_SDYY, p) = m.sd[1];
+  P(o, PF_SDZZ, p) = m.sd[2];
+  P(o, PF_SDYZ, p) = m.sd[3];
+  P(o, PF_SDXZ, p) = m.sd[4];
+  P(o, PF_SDXY, p) = m.sd[5];
+  P(o, PF_SIGY, p) = 0.0;
+  P(o, PF_EPEFF, p) = 0.0;
+  P(o, PF_Q, p) = 0.0;
+}
+
+/* TLParticleStressUpdate (update_step.f90:649-828): the velocity gradient
+ * over the fixed cells' shape functions (the strain rate and spin are
+ * accumulated without a reset per particle in the reference, but M3DM11
+ * reads neither), Fp += deFp (no product: the gradient is material), then
+ * TLConstitution; USL/MUSL move Xp = XX */
+void oracle_tl_particle_stress_update(oracle* o) {
+  shape_t s;
+  const double cutoff = o->prob.cutoff;
+  if (o->sgmp) sgmp_cell_gradients(o);
+  for (int p = 0; p < o->n; p++) {
+    double de[6] = {0, 0, 0, 0, 0, 0}, vort[3] = {0, 0, 0}, deFp[3][3];
+    memset(deFp, 0, sizeof deFp);
+    if (o->sgmp) {
+      if (!sgmp_particle_strain(o, p, de, vort, deFp)) continue;
+    } else {
+      if (o->icell[p] < 0) continue;
+      shape_of(o, p, &s);
+      for (int i = 0; i < s.nb; i++) {
+        if (!in_grid(o, s.node[i])) continue;
+        const int k = PNODE(o, p, s.node[i] - 1);
+        const double mg = G(o, GF_MG, k);
+        if (!(mg > cutoff)) continue;
+        const double v[3] = {G(o, GF_PX, k) / mg, G(o, GF_PY, k) / mg, G(o, GF_PZ, k) / mg};
+        for (int d = 0; d < 3; d++) {
+          deFp[d][0] = deFp[d][0] + s.dx[i] * v[d];
+          deFp[d][1] = deFp[d][1] + s.dy[i] * v[d];
+          deFp[d][2] = deFp[d][2] + s.dz[i] * v[d];
+        }
+      }
+    }
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) deFp[i][j] = deFp[i][j] * o->dt;
+    if (o->prob.update_v_by_fp)
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) P(o, PF_FP + 3 * i + j, p) = P(o, PF_FP + 3 * i + j, p) + deFp[i][j];
+    tl_constitution(o, p, (const double(*)[3])deFp);
+    if (o->prob.scheme != MPM_USF)
+      for (int d = 0; d < 3; d++) P(o, PF_X + d, p) = P(o, PF_XX + d, p);
+  }
+}
+
+/* TLGridMomentumUpdate (update_step.f90:1015-1191): internal force from the
+ * first Piola-Kirchhoff stress and the initial volume, external force as in
+ * GridMomentumUpdate */
+void oracle_tl_grid_momentum_update(oracle* o) {
+  for (int k = 0; k < o->nntot; k++) G(o, GF_FX, k) = G(o, GF_FY, k) = G(o, GF_FZ, k) = 0.0;
+  if (o->sgmp) {
+    if (!o->cpk) o->cpk = (double*)calloc(9 * (size_t)o->ncell, sizeof(double));
+    memset(o->cpk, 0, 9 * (size_t)o->ncell * sizeof(double));
+    memset(o->cfext, 0, 3 * (size_t)o->ncell * sizeof(double));
+  }
+  const mpm_problem* pr = &o->prob;
+  shape_t s;
+  int cn[8];
+  for (int p = 0; p < o->n; p++) {
+    double pk[9];
+    for (int i = 0; i < 9; i++) pk[i] = P(o, PF_PK + i, p);
+    const double vol = P(o, PF_VOL0, p);
+    double fx[3];
+    for (int d = 0; d < 3; d++) {
+      double f = P(o, PF_FX + d, p);
+      if (pr->quasi_load) {
+        if (o->time >= pr->quasi_load_time) fx[d] = f;
+        else fx[d] = f / pr->quasi_load_time * o->time;
+      } else {
+        fx[d] = f;
+      }
+    }
+    if (pr->quasi_damp)
+      for (int d = 0; d < 3; d++) fx[d] = fx[d] - pr->damp_factor * P(o, PF_MASS, p) * P(o, PF_VX + d, p);
+    if (pr->gravity) {
+      const mpm_body* b = &o->bodies[o->pbody[p]];
+      for (int d = 0; d < 3; d++) fx[d] = fx[d] + P(o, PF_MASS, p) * b->grav[d];
+    }
+    if (o->sgmp) {
+      if (o->cicell[p] < 0) continue;
+      double shp[8];
+      center_cell_nodes(o, o->cicell[p], cn);
+      sgn_shape(&o->xxn[3 * p], shp);
+      for (int i = 0; i < 8; i++) {
+        if (!in_aux(o, cn[i])) continue;
+        const int k = cn[i] - 1;
+        for (int d = 0; d < 3; d++) o->cfext[3 * k + d] = o->cfext[3 * k + d] + fx[d] * shp[i];
+        for (int q = 0; q < 9; q++) o->cpk[9 * k + q] = o->cpk[9 * k + q] + shp[i] * pk[q] * vol;
+      }
+      continue;
+    }
+    if (o->icell[p] < 0) continue;
+    shape_of(o, p, &s);
+    for (int i = 0; i < s.nb; i++) {
+      if (!in_grid(o, s.node[i])) continue;
+      const int k = PNODE(o, p, s.node[i] - 1);
+      const double dX = s.dx[i], dY = s.dy[i], dZ = s.dz[i], N = s.shp[i];
+      /* PK(1..9) = xx yy zz yz xz xy zy zx yx */
+      const double fi0 = -(pk[0] * dX + pk[5] * dY + pk[4] * dZ) * vol;
+      const double fi1 = -(pk[8] * dX + pk[1] * dY + pk[3] * dZ) * vol;
+      const double fi2 = -(pk[7] * dX + pk[6] * dY + pk[2] * dZ) * vol;
+      G(o, GF_FX, k) = G(o, GF_FX, k) + fi0 + fx[0] * N;
+      G(o, GF_FY, k) = G(o, GF_FY, k) + fi1 + fx[1] * N;
+      G(o, GF_FZ, k) = G(o, GF_FZ, k) + fi2 + fx[2] * N;
+    }
+  }
+  if (o->sgmp) {
+    double DX[8], DY[8], DZ[8];
+    cn_shape(o, DX, DY, DZ);
+    for (int c = 1; c <= o->ncell; c++) {
+      const double* q = &o->cpk[9 * (c - 1)];
+      const double* fe = &o->cfext[3 * (c - 1)];
+      cell_nodes(o, c, cn);
+      for (int j = 0; j < 8; j++) {
+        const int k = cn[j] - 1;
+        double f[3];
+        f[0] = -(q[0] * DX[j] + q[5] * DY[j] + q[4] * DZ[j]);
+        f[1] = -(q[8] * DX[j] + q[1] * DY[j] + q[3] * DZ[j]);
+        f[2] = -(q[7] * DX[j] + q[6] * DY[j] + q[2] * DZ[j]);
+        for (int d = 0; d < 3; d++) G(o, GF_FX + d, k) = G(o, GF_FX + d, k) + 0.125 * fe[d] + f[d];
+      }
+    }
+  }
+}
+
 /* SmoothStressbyGrid (update_step.f90:1876-2055): -SM V and V spread to
  * the grid (every in-range node a particle's list reaches is "mapped"),
  * Gpre = sum / sum at mapped nodes, then gathered back to Spre.  The
@@ -1677,7 +2005,8 @@ void oracle_smooth_stress(oracle* o) {
     cgv = (double*)calloc(o->ncell, sizeof(double));
   }
   for (int p = 0; p < o->n; p++) {
-    const double msm = -1 * P(o, PF_SM, p), vol = P(o, PF_VOL, p);
+    /* TLSmoothStressbyGrid (update_step.f90:2057-2236) weighs with VOL0 */
+    const double msm = -1 * P(o, PF_SM, p), vol = o->prob.tlmpm ? P(o, PF_VOL0, p) : P(o, PF_VOL, p);
     if (o->sgmp) {
       if (o->cicell[p] < 0) continue;
       center_cell_nodes(o, o->cicell[p], cn);
@@ -1745,14 +2074,19 @@ void oracle_smooth_stress(oracle* o) {
 }
 
 void oracle_step(oracle* o, int nsteps) {
+  const int tl = o->prob.tlmpm;
+  if (tl && !o->tl_started) oracle_tl_grid_mass_initial(o);   /* MPM3D.f90:164 */
   for (int k = 0; k < nsteps; k++) {
     oracle_begin_step(o);
-    oracle_grid_momentum_initial(o);
+    if (tl) oracle_tl_grid_momentum_initial(o);
+    else oracle_grid_momentum_initial(o);
     if (o->prob.scheme == MPM_USF) {
       oracle_apply_bc(o);
-      oracle_particle_stress_update(o);
+      if (tl) oracle_tl_particle_stress_update(o);
+      else oracle_particle_stress_update(o);
     }
-    oracle_grid_momentum_update(o);
+    if (tl) oracle_tl_grid_momentum_update(o);
+    else oracle_grid_momentum_update(o);
     oracle_integrate_momentum(o);
     if (o->prob.contact == 1) oracle_contact(o);
     oracle_particle_position_update(o);
@@ -1760,7 +2094,10 @@ void oracle_step(oracle* o, int nsteps) {
       oracle_grid_momentum_musl(o);
       oracle_apply_bc(o);
     }
-    if (o->prob.scheme != MPM_USF) oracle_particle_stress_update(o);
+    if (o->prob.scheme != MPM_USF) {
+      if (tl) oracle_tl_particle_stress_update(o);
+      else oracle_particle_stress_update(o);
+    }
     oracle_calc_energy(o);
     oracle_update_dt(o);
     if (o->prob.smooth_stress) oracle_smooth_stress(o);

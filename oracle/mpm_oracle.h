@@ -40,7 +40,12 @@ void oracle_particle_position_update(oracle* o);  /* :1378-1526 */
 void oracle_grid_momentum_musl(oracle* o);        /* :1668-1793 */
 void oracle_calc_energy(oracle* o);               /* Particle.f90:204-230 */
 void oracle_update_dt(oracle* o);                 /* update_step.f90:1833-1874 */
-void oracle_smooth_stress(oracle* o);             /* :1876-2055 SmoothStressbyGrid (Spre) */
+void oracle_smooth_stress(oracle* o);             /* :1876-2055 SmoothStressbyGrid (Spre); TL: :2057-2236 */
+/* total-Lagrangian phases (TLmp) */
+void oracle_tl_grid_mass_initial(oracle* o);      /* :22-172 GridMassInitial + TLParticleInitial, once */
+void oracle_tl_grid_momentum_initial(oracle* o);  /* :174-286 */
+void oracle_tl_particle_stress_update(oracle* o); /* :649-828, Constitution.f90:369-525, 782-882 */
+void oracle_tl_grid_momentum_update(oracle* o);   /* :1015-1191 */
 /* whole steps in the order of MPM3D.f90:69-113 */
 void oracle_step(oracle* o, int nsteps);
 

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "libmpm_oracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
 
-PF_COUNT = 41   # enum mpm_pfield (include/mpm3d_problem.h)
+PF_COUNT = 54   # enum mpm_pfield (include/mpm3d_problem.h)
 GF_COUNT = 7
 
 
@@ -36,7 +36,9 @@ def lib():
                      "oracle_particle_stress_update", "oracle_grid_momentum_update",
                      "oracle_integrate_momentum", "oracle_particle_position_update",
                      "oracle_grid_momentum_musl", "oracle_calc_energy", "oracle_update_dt",
-                     "oracle_smooth_stress", "oracle_contact", "oracle_destroy"):
+                     "oracle_smooth_stress", "oracle_contact", "oracle_tl_grid_mass_initial",
+                     "oracle_tl_grid_momentum_initial", "oracle_tl_particle_stress_update",
+                     "oracle_tl_grid_momentum_update", "oracle_destroy"):
             getattr(L, name).argtypes = [vp]
             getattr(L, name).restype = None
         L.oracle_create.argtypes = [vp]
@@ -172,7 +174,8 @@ def dp_point(material, young, poisson, sd, sm):
 # particle field order written by ref_driver.f90 -> index in PF layout
 _REF_PFIELDS = (["x", "y", "z", "xx", "xy", "xz", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol",
                  "sm", "seqv", "sdxx", "sdyy", "sdzz", "sdyz", "sdxz", "sdxy", "sig_y", "epeff",
-                 "ie", "q", "cp", "temp", "dmg", "lt", "fail", "spre"]
+                 "ie", "q", "cp", "temp", "dmg", "lt", "fail", "spre", "xo", "yo", "zo", "vol0"]
+                + ["pk%d" % (i + 1) for i in range(9)]
                 + ["fp%d%d" % (i + 1, j + 1) for j in range(3) for i in range(3)])
 
 
@@ -180,7 +183,7 @@ def read_ref_dump(path):
     """Parse one ref_driver dump -> dict(hdr, scal, particles{name: arr}, icell, grid[7,nn], fix[3,nn])."""
     raw = open(path, "rb").read()
     hdr = np.frombuffer(raw, dtype="<i4", count=16)
-    assert hdr[0] == 1146113101 and hdr[1] == 4, "not a ref_driver dump (layout 4): %s" % path
+    assert hdr[0] == 1146113101 and hdr[1] == 5, "not a ref_driver dump (layout 5): %s" % path
     ncomp = int(hdr[11])
     n, nn = int(hdr[2]), int(hdr[3])
     scal = np.frombuffer(raw, dtype="<f8", count=16, offset=64)

@@ -10,7 +10,12 @@
 ! ApplyBoundaryConditions], [USL/MUSL: ParticleStressUpdate], calcEnergy,
 ! UpdateDT, [smoo: SmoothStressbyGrid], [drda: dynamicRelaxationDamping])
 ! without the per-step file output, and writes the complete particle and
-! grid state as little-endian binary after chosen steps.
+! grid state as little-endian binary after chosen steps.  A total-Lagrangian
+! deck (TLmp) replays MPM3D.f90:164-250 instead: GridMassInitial once, then
+! TLGridMomentumInitial, [USF: ApplyBoundaryConditions,
+! TLParticleStressUpdate], TLGridMomentumUpdate, IntegrateMomentum,
+! ParticlePositionUpdate, [MUSL ...], [USL/MUSL: TLParticleStressUpdate],
+! calcEnergy, UpdateDT, [smoo: TLSmoothStressbyGrid], [drda].
 !
 ! usage:  ref_driver <deck> <nsteps> <outprefix> <dumpsteps> [phasestep]
 !   <deck>       .mpm input deck (read by the reference's InputPara)
@@ -39,7 +44,9 @@ program ref_driver
 
   external GridMomentumInitial, ParticleStressUpdate, GridMomentumUpdate, &
            IntegrateMomentum, ParticlePositionUpdate, GridMomentumMUSL, &
-           ApplyBoundaryConditions, UpdateDT, SmoothStressbyGrid, Lagr_NodContact
+           ApplyBoundaryConditions, UpdateDT, SmoothStressbyGrid, Lagr_NodContact, &
+           GridMassInitial, TLGridMomentumInitial, TLParticleStressUpdate, TLGridMomentumUpdate, &
+           TLSmoothStressbyGrid
 
   if (command_argument_count() < 4) then
      print *, 'usage: ref_driver <deck> <nsteps> <outprefix> <dumpsteps> [phasestep]'
@@ -69,10 +76,10 @@ program ref_driver
   ! InputPara reads the deck named by the first command-line argument.
   call InputPara()
   call calcEnergy()
-  if (TLMPM) stop 'ref_driver: total-Lagrangian decks are not replayed'
 
   if (any(dumpat(1:ndump) == 0)) call dump_state(0, 0)
 
+  if (TLMPM) call GridMassInitial()
   call system_clock(c0, crate)
   do k = 1, nsteps
      istep = istep + 1
@@ -80,14 +87,26 @@ program ref_driver
      EngInternal = 0.0
      dophase = (istep == phasestep)
 
-     call GridMomentumInitial()
+     if (TLMPM) then
+        call TLGridMomentumInitial()
+     else
+        call GridMomentumInitial()
+     end if
      if (dophase) call dump_state(istep, 1)
      if (USF) then
         call ApplyBoundaryConditions()
-        call ParticleStressUpdate()
+        if (TLMPM) then
+           call TLParticleStressUpdate()
+        else
+           call ParticleStressUpdate()
+        end if
         if (dophase) call dump_state(istep, 2)
      end if
-     call GridMomentumUpdate()
+     if (TLMPM) then
+        call TLGridMomentumUpdate()
+     else
+        call GridMomentumUpdate()
+     end if
      if (dophase) call dump_state(istep, 3)
      call IntegrateMomentum()
      if (Contact_type == 1) call Lagr_NodContact()
@@ -100,12 +119,22 @@ program ref_driver
         if (dophase) call dump_state(istep, 6)
      end if
      if (.not. USF) then
-        call ParticleStressUpdate()
+        if (TLMPM) then
+           call TLParticleStressUpdate()
+        else
+           call ParticleStressUpdate()
+        end if
         if (dophase) call dump_state(istep, 7)
      end if
      call calcEnergy()
      call UpdateDT()
-     if (SmoothStress) call SmoothStressbyGrid()
+     if (SmoothStress) then
+        if (TLMPM) then
+           call TLSmoothStressbyGrid()
+        else
+           call SmoothStressbyGrid()
+        end if
+     end if
      if (DR_DAMPING) call dynamicRelaxationDamping()
      if (dophase) call dump_state(istep, 8)
 
@@ -134,7 +163,7 @@ contains
 
     hdr = 0
     hdr(1) = 1146113101          ! 'MPMD' tag
-    hdr(2) = 4                   ! layout version
+    hdr(2) = 5                   ! layout version
     hdr(3) = nb_particle
     hdr(4) = nb_gridnode
     hdr(5) = istep
@@ -184,6 +213,13 @@ contains
     write(u) (particle_list(i)%LT, i = 1, nb_particle)
     write(u) (merge(1d0, 0d0, particle_list(i)%failure), i = 1, nb_particle)
     write(u) (particle_list(i)%Spre, i = 1, nb_particle)
+    do j = 1, 3
+       write(u) (particle_list(i)%Xo(j), i = 1, nb_particle)
+    end do
+    write(u) (particle_list(i)%VOL0, i = 1, nb_particle)
+    do j = 1, 9
+       write(u) (particle_list(i)%PK(j), i = 1, nb_particle)
+    end do
     do j = 1, 3          ! Fp(i,j) column-major: (1,1),(2,1),(3,1),(1,2),...
        do n = 1, 3
           write(u) (particle_list(i)%Fp(n, j), i = 1, nb_particle)

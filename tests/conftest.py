@@ -53,6 +53,10 @@ DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
 # (B-spline, MUSL, normal of body 1, a J2 block), and a third body on a third
 # component (GIMP, USL, normal of body 2, gravity); reference-order FP64 only
 CONTACT_DECKS = ["blocks_contact_linear_usf", "blocks_contact_bspline_musl", "blocks_contact_gimp_usl"]
+# total-Lagrangian MPM (TLmp, neo-Hookean): a beam on a clamped base (linear,
+# MUSL, smoothed pressure), two sheared bodies (B-spline USF; GIMP USL with
+# gravity and smoothing; SGMP MUSL on a clamped floor); reference-order FP64 only
+TL_DECKS = ["beam_tl_linear_musl", "block_tl_bspline_usf", "block_tl_gimp_usl", "block_tl_sgmp_musl"]
 # auxiliary-grid (SGMP) decks: clamped plate (MUSL), water column with
 # symmetric walls (MUSL), J2 block on a clamped floor (USF)
 SGMP_DECKS = ["plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf", "block_smoo_sgmp_usf"]

@@ -9,7 +9,9 @@ import os
 import numpy as np
 import pytest
 
-from conftest import ALL_DECKS as DECKS, GOLDEN_DIR, deck_path, load_package
+from conftest import ALL_DECKS, CONTACT_DECKS, GOLDEN_DIR, TL_DECKS, deck_path, load_package
+
+DECKS = ALL_DECKS + CONTACT_DECKS + TL_DECKS
 
 S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp", "temp", "lt"]
 
@@ -19,8 +21,9 @@ def test_reader_matches_reference_input_phase(name):
     mpm = load_package()
     p = mpm.read_deck(deck_path(name))
     g = np.load(os.path.join(GOLDEN_DIR, name + ".npz"))
-    for f in S0_FIELDS:
-        assert np.array_equal(p.field(f), g["s0_" + f]), f
+    for f in S0_FIELDS + ["xo", "yo", "zo", "vol0"]:
+        if f in S0_FIELDS or "s0_" + f in g.files:
+            assert np.array_equal(p.field(f), g["s0_" + f]), f
     sc = g["scal0"]
     assert p.dt == sc[0]
     assert p.cutoff == sc[7]
