@@ -23,10 +23,10 @@ enum mpm_mat_type {
   MPM_MAT_ELASTIC = 1,   /* elas  -> M3DM1 */
   MPM_MAT_PLA1 = 2,      /* pla1  -> M3DM2, elastic-perfectly plastic */
   MPM_MAT_PLA2 = 3,      /* pla2  -> M3DM3, J2 with linear hardening */
-  MPM_MAT_JOHN = 4,      /* john  -> M3DM4, Johnson-Cook (read; not run, see mpm_deck_read) */
+  MPM_MAT_JOHN = 4,      /* john  -> M3DM4, Johnson-Cook with heating (reference-order FP64 mode only) */
   MPM_MAT_SJC = 5,       /* sjc   -> M3DM5 + EOS, simplified Johnson-Cook */
   MPM_MAT_SJCF = 6,      /* sjcf  -> M3DM5 + EOS with failure strain */
-  MPM_MAT_JCF = 7,       /* jcf   -> M3DM8 + EOS with damage (read; not run) */
+  MPM_MAT_JCF = 7,       /* jcf   -> M3DM8 + EOS with damage (reference-order FP64 mode only) */
   MPM_MAT_HIEX = 8,      /* hiex  -> M3DM6 + EOS (JWL), high explosive burn */
   MPM_MAT_NULL = 9,      /* null  -> M3DM7 + EOS */
   MPM_MAT_DP = 10,       /* dpm   -> M3DM9, Drucker-Prager */

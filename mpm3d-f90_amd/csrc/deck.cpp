@@ -707,13 +707,9 @@ extern "C" int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, 
     for (int m = 0; m < d.nb_mat; m++) {
       const mpm_material& mt = d.mats[m];
       const std::string id = "material " + std::to_string(m + 1);
-      if (mt.type == MPM_MAT_JOHN || mt.type == MPM_MAT_JCF)
-        d.fail(id + ": 'john'/'jcf' heat (and damage) each particle with the plastic strain increment the "
-               "last yielding particle left in a module variable (Constitution.f90:237, 290, 314), an order "
-               "dependence this build does not reproduce");
-      // seleos is called by sjc, sjcf, hiex and null (Constitution.f90:239-330)
-      const bool needs_eos = mt.type == MPM_MAT_SJC || mt.type == MPM_MAT_SJCF || mt.type == MPM_MAT_HIEX ||
-                             mt.type == MPM_MAT_NULL;
+      // seleos is called by sjc, sjcf, jcf, hiex and null (Constitution.f90:239-330)
+      const bool needs_eos = mt.type == MPM_MAT_SJC || mt.type == MPM_MAT_SJCF || mt.type == MPM_MAT_JCF ||
+                             mt.type == MPM_MAT_HIEX || mt.type == MPM_MAT_NULL;
       if (needs_eos && (mt.eos_type < 1 || mt.eos_type > 4)) d.fail(id + " needs an equation of state ('seos')");
       // eos3 scales by the burn fraction only HE burn sets (for others the
       // reference reads the last HE particle's value)

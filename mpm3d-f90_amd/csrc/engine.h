@@ -59,7 +59,8 @@ enum InstFlags {
   IF_ENDTIME = 32,       // instance stops itself when time > endtime
   IF_DRDA = 64,          // dynamic relaxation damping after UpdateDT (drda)
   IF_SMOOTH = 128,       // SmoothStressbyGrid at the end of every step (smoo)
-  IF_TL = 256            // total-Lagrangian step (TLmp): cells of the initial positions, mass of the first step
+  IF_TL = 256,           // total-Lagrangian step (TLmp): cells of the initial positions, mass of the first step
+  IF_JC = 512            // Johnson-Cook models (john, jcf): the plastic strain increment carried across particles
 };
 
 // per-instance time state, advanced on the device (graph-capturable)
@@ -76,6 +77,8 @@ struct InstT {
   int integ_half;        //   that step was the first (half step)
   double integ_dt;       //   and its DT
   double cont_force[3];  // tot_cont_for of the last contact pass
+  double depeff;         // Johnson-Cook instances: the plastic strain increment the last yielding particle
+                         // left (the reference's module variable), 0 before the first
 };
 
 // per-material constants, all derived exactly as the reference derives them
@@ -88,6 +91,8 @@ struct MatC {
   double csound;         // SetDT/UpdateDT sound speed max(c, D, Wavespd)
   double ce[7];          // EOS 1, 2: cEos(1..7) through a default REAL; EOS 3: A, B, R1, R2, omega
   double bjc, njc, cjc, epso, epf;   // (simplified) Johnson-Cook
+  double mjc, roomt, melt, spec_heat;   // Johnson-Cook temperature exponent, room and melting temperature,
+                                        // specific heat (john, jcf)
 };
 
 struct BodyC {
@@ -126,6 +131,10 @@ enum DField {
 // the reference's order (xx yy zz yz xz xy zy zx yx): a field-major block of
 // 9 doubles per particle slot, allocated with the first such instance
 #define TL_PK_COUNT 9
+
+// Johnson-Cook record fields (PoolPtrs::jc): the plastic strain increment,
+// then the jcf tail's V and SM at the step's start and its energy increment
+#define JC_REC_COUNT 4
 
 // particle integer fields
 enum IField { I_BODY = 0, I_ORIG, I_CELL, I_COUNT };
@@ -187,5 +196,12 @@ struct PoolPtrs {
   int tile_cap;  // nodes a block's LDS tile may hold (<= TILE_NODES; smaller values exercise the fallbacks)
   int* boxes;    // node-box cache of the particle kernels: BOX_INTS ints per 64 particle slots
   double* pk;    // TL_PK_COUNT * np: PK of total-Lagrangian instances (null without any)
+  // Johnson-Cook instances (null without any): every stress update's record,
+  // at the pool index p_begin + original particle index: JC_* flags (jcf),
+  // and jc = JC_REC_COUNT field-major doubles (the increment it assigned,
+  // or after the scan the one it reads; V, SM and the energy increment of
+  // the step's start for the jcf tail); flags are cleared by the tail
+  int* jcf;
+  double* jc;
   int box_mode;  // BOX_LOAD | BOX_STORE
 };

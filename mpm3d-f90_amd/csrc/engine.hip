@@ -136,6 +136,7 @@ struct Group {
   bool drda = false; // some instance damps by dynamic relaxation
   bool smooth = false; // some instance smooths its stress by the grid (smoo)
   bool contact = false; // some instance has contact between grid components (Lagr_NodContact)
+  bool jc = false;      // some instance has Johnson-Cook materials (john, jcf)
   std::vector<int> insts;
   std::vector<int> pgran, ngran;
   int* d_pgran = nullptr;
@@ -169,6 +170,8 @@ struct mpm_engine {
   SmPtrs sm{};                    // SmoothStressbyGrid buffers (allocated with the first instance that smooths)
   double* pk = nullptr;           // PK of total-Lagrangian instances (allocated with the first one)
   double* pk_alt = nullptr;       //   and its sort buffer
+  int* jcf = nullptr;             // Johnson-Cook records (allocated with the first Johnson-Cook instance)
+  double* jc = nullptr;
   double* cmg = nullptr;
   InstC* d_inst = nullptr;
   InstT* d_ts = nullptr;
@@ -287,6 +290,8 @@ struct mpm_engine {
     P.tile_cap = tile_cap;
     P.boxes = boxes;
     P.pk = pk;
+    P.jcf = jcf;
+    P.jc = jc;
     P.box_mode = box_mode;
     return P;
   }
@@ -303,7 +308,7 @@ static void free_engine(mpm_engine* e) {
                   e->v_out, e->rs_scr, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg, e->boxes,
                   e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1], e->x_rnode, e->x_rw, e->x_seg0, e->x_seg1,
                   e->x_scr, e->d_sub_ids, e->d_sub_rows, e->sm.gpre, e->sm.gv, e->sm.cgpre, e->sm.cgv, e->sm.map,
-                  e->sm.xs, e->x_ndir, e->x_cf, e->pk, e->pk_alt};
+                  e->sm.xs, e->x_ndir, e->x_cf, e->pk, e->pk_alt, e->jcf, e->jc};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto& g : e->groups) {
@@ -431,6 +436,10 @@ static MatC make_mat(const mpm_material& m) {
   c.cjc = m.c_jc;
   c.epso = m.epso;
   c.epf = m.epf;
+  c.mjc = m.m_jc;
+  c.roomt = m.roomt;
+  c.melt = m.melt;
+  c.spec_heat = m.spec_heat;
   // SetDT / UpdateDT sound speed (DataIn.f90:1285-1290, update_step.f90:1861-1866)
   double E = m.young, nu = m.poisson, ro = m.density;
   double cp = std::sqrt(E * (1 - nu) / (1 + nu) / (1 - 2 * nu) / ro);
@@ -447,11 +456,10 @@ static void check_problem(const mpm_problem* p) {
   if (p->shape < 0 || p->shape > 3 || p->scheme < 0 || p->scheme > 2) fail(MPM_ERR_ARG, "bad shape/scheme");
   for (int m = 0; m < p->nb_mat; m++) {
     const int t = p->mats[m].type, et = p->mats[m].eos_type;
-    if (t == MPM_MAT_JOHN || t == MPM_MAT_JCF)
-      fail(MPM_ERR_UNSUPPORTED, "material type %d (john/jcf) is not on the device path", t);
     if (t < MPM_MAT_ELASTIC || t > MPM_MAT_NEOH) fail(MPM_ERR_ARG, "material type %d", t);
     // models that call seleos need an EOS; JWL only with HE burn (its burn fraction)
-    const bool needs_eos = t == MPM_MAT_SJC || t == MPM_MAT_SJCF || t == MPM_MAT_HIEX || t == MPM_MAT_NULL;
+    const bool needs_eos = t == MPM_MAT_SJC || t == MPM_MAT_SJCF || t == MPM_MAT_JCF || t == MPM_MAT_HIEX ||
+                           t == MPM_MAT_NULL;
     if (needs_eos && (et < 1 || et > 4)) fail(MPM_ERR_ARG, "material %d needs an equation of state", m + 1);
     if (et == 3 && t != MPM_MAT_HIEX) fail(MPM_ERR_UNSUPPORTED, "material %d: JWL EOS without HE burn", m + 1);
   }
@@ -643,6 +651,16 @@ static void ensure_tl(mpm_engine* e) {
   mark_graph_dirty(e);
 }
 
+// Johnson-Cook records: flags cleared once (the tails clear what the stress
+// passes set), the doubles are written before they are read
+static void ensure_jc(mpm_engine* e) {
+  if (e->jcf) return;
+  e->jcf = dalloc<int>(e->np);
+  HIPCHK(hipMemsetAsync(e->jcf, 0, sizeof(int) * e->np, e->s));
+  e->jc = dalloc<double>((size_t)JC_REC_COUNT * e->np);
+  mark_graph_dirty(e);
+}
+
 static void ensure_contact(mpm_engine* e) {
   if (e->x_ndir) return;
   e->x_ndir = dalloc<double>(3 * e->nn);
@@ -657,7 +675,14 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
                               "(MPM_PREC_FP64_EXACT)");
   if (p->tlmpm && !e->exact)
     fail(MPM_ERR_UNSUPPORTED, "total-Lagrangian MPM runs in the reference-order FP64 mode (MPM_PREC_FP64_EXACT)");
+  bool jc = false;
+  for (int m = 0; m < p->nb_mat; m++) jc = jc || p->mats[m].type == MPM_MAT_JOHN || p->mats[m].type == MPM_MAT_JCF;
+  if (jc && !e->exact)
+    fail(MPM_ERR_UNSUPPORTED, "the Johnson-Cook models john and jcf run in the reference-order FP64 mode "
+                              "(MPM_PREC_FP64_EXACT): each particle that does not yield reads the plastic strain "
+                              "increment of the last particle before it that did");
   if (p->tlmpm) ensure_tl(e);   // before the admission kernel, which fills it
+  if (jc) ensure_jc(e);
   if (e->free_ids.empty()) fail(MPM_ERR_NOMEM, "no free instance slot");
   const int n = p->nb_particle;
   const long npg = (n + PGRAN - 1) / PGRAN;
@@ -714,7 +739,7 @@ static int add_instance(mpm_engine* e, const mpm_problem* p, int flags_extra) {
   c.scheme = p->scheme;
   c.flags = (p->quasi_load ? IF_QUASI_LOAD : 0) | (p->quasi_damp ? IF_QUASI_DAMP : 0) |
             (p->gravity ? IF_GRAVITY : 0) | (p->update_v_by_fp ? IF_UVFP : 0) | (p->dr_damping ? IF_DRDA : 0) |
-            (p->smooth_stress ? IF_SMOOTH : 0) | (p->tlmpm ? IF_TL : 0) | flags_extra;
+            (p->smooth_stress ? IF_SMOOTH : 0) | (p->tlmpm ? IF_TL : 0) | (jc ? IF_JC : 0) | flags_extra;
   bool loads = false;
   for (int i = 0; i < n && !loads; i++)
     for (int d = 0; d < 3; d++)
@@ -883,6 +908,7 @@ static void rebuild_lists(mpm_engine* e) {
     G.drda = G.drda || (h.c.flags & IF_DRDA) != 0;
     G.smooth = G.smooth || (h.c.flags & IF_SMOOTH) != 0;
     G.contact = G.contact || h.c.contact == 1;
+    G.jc = G.jc || (h.c.flags & IF_JC) != 0;
     for (long k = 0; k < h.npg; k++) G.pgran.push_back((int)(h.pg0 + k));
     for (long k = 0; k < h.nng; k++) G.ngran.push_back((int)(h.ng0 + k));
   }
@@ -1043,6 +1069,7 @@ static void exact_group_phase(mpm_engine* e, Group& G, int ph) {
       } else {
         mpmk::launch_x_stress(G.shape, usf ? 0 : 1, P, X, L, s);
       }
+      if (G.jc) mpmk::launch_x_jc(P, L, G.d_ilist, (int)G.insts.size(), s);
       for (int id : G.insts) e->inst[id].xx_pending = false;
       break;
     case PH_GMU:

@@ -59,6 +59,16 @@
 // flag bit of the material mask: total-Lagrangian instances may be met
 // (TLConstitution; reference-order mode only)
 #define MM_TL (1 << 29)
+// flag bit of the material mask: Johnson-Cook instances (john, jcf) may be
+// met, and every model's update records the plastic strain increment it
+// assigned (reference-order mode only)
+#define MM_JC (1 << 28)
+
+// Johnson-Cook record flags (PoolPtrs::jcf)
+#define JC_ASSIGNED 1   // the update assigned a plastic strain increment (any plastic model)
+#define JC_HEAT 2       // john: the temperature rise follows with the increment it reads
+#define JC_TAIL 4       // jcf (not failed before): failure, damage and temperature follow
+#define JC_MELTED 8     // jcf: failed by melting inside M3DM8
 
 // k_finalize parts
 #define FIN_ENERGY 1
@@ -134,6 +144,10 @@ void launch_x_stress(int shape, int move, const PoolPtrs<double>& P, const XPtrs
                      hipStream_t s);
 void launch_x_g2p(int shape, int move, const PoolPtrs<double>& P, const XPtrs& X, const LaunchP& L, hipStream_t s);
 void launch_x_sgmp_stress(int move, const PoolPtrs<double>& P, const LaunchP& L, hipStream_t s);
+// after a stress pass over Johnson-Cook instances: the increment every
+// particle reads (last assigned before it, in original order) and the
+// Johnson-Cook tails
+void launch_x_jc(const PoolPtrs<double>& P, const LaunchP& L, const int* ilist, int ninst, hipStream_t s);
 // Lagr_NodContact over the node list N, then each instance's tot_cont_for (ilist: the group's instances)
 void launch_x_contact(const PoolPtrs<double>& P, const XPtrs& X, const LaunchP& N, const int* ilist, int ninst,
                       hipStream_t s);

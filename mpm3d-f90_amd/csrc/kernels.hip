@@ -1686,6 +1686,8 @@ struct Cst {
   double vold, vol, dvol, iener;   // FP64 in both paths
   T sm, smold, sd[6], sig[6], sold[6], bqf;
   T seqv, epeff, sigy, ieinc, den0, den, vol0, cp, mp;
+  double dep;   // MM_JC: the plastic strain increment the update assigned (jcf & JC_ASSIGNED)
+  int jcf;      // MM_JC: JC_* record flags
 };
 
 // material models a kernel instance may meet (MM: bit 1 << model type)
@@ -1701,8 +1703,9 @@ __device__ __forceinline__ void cst_load(const PV<T>& p, Cst<T>& m) {
   m.vold = p.d(D_VOL);
   m.epeff = p.t(T_EPEFF);
   m.sigy = (mm_has(MM, MPM_MAT_PLA1) || mm_has(MM, MPM_MAT_PLA2) || mm_has(MM, MPM_MAT_SJC) ||
-            mm_has(MM, MPM_MAT_SJCF)) ? p.t(T_SIGY) : (T)0;
-  m.seqv = (mm_has(MM, MPM_MAT_NULL) || mm_has(MM, MPM_MAT_HIEX) || mm_has(MM, MPM_MAT_SJCF)) ? p.t(T_SEQV) : (T)0;
+            mm_has(MM, MPM_MAT_SJCF) || (MM & MM_JC)) ? p.t(T_SIGY) : (T)0;
+  m.seqv = (mm_has(MM, MPM_MAT_NULL) || mm_has(MM, MPM_MAT_HIEX) || mm_has(MM, MPM_MAT_SJCF) || (MM & MM_JC))
+               ? p.t(T_SEQV) : (T)0;
   m.iener = p.d(D_IE);
   m.mp = p.t(T_MASS);
   m.cp = p.t(T_CP);
@@ -1817,6 +1820,17 @@ __device__ __forceinline__ double m_exp(double x) { return m_cr<T, MM>() ? mpm_e
 template <typename T, int MM>
 __device__ __forceinline__ double m_pow(double x, double y) { return m_cr<T, MM>() ? mpm_pow_cr(x, y) : pow(x, y); }
 
+// an update that assigns the reference's module variable depeff (every
+// plastic model, Constitution.f90:902, 944, 1005, 1074, 1177, 1285-1322)
+// records it for the Johnson-Cook particles after it
+template <int MM, typename C>
+__device__ __forceinline__ void jc_assign(C& m, double depeff) {
+  if (MM & MM_JC) {
+    m.jcf |= JC_ASSIGNED;
+    m.dep = depeff;
+  }
+}
+
 // M3DM5, simplified Johnson-Cook (Constitution.f90:1043-1100): the hardening
 // modulus at epeff + 1e-4 and the rate-dependent yield stress in FP64 (the
 // literals 0.0001 and 0.001 are default REALs), the radial return in T
@@ -1835,7 +1849,8 @@ __device__ __forceinline__ void m3dm5(Cst<T>& m, const MatC& mc, T G2, double dt
     double srate = (double)depeff / mc.epso / dtd;
     if (srate < rmin) srate = rmin;
     m.sigy = (T)((mc.yield0 + mc.bjc * m_pow<T, MM>((double)m.epeff, mc.njc)) * (1 + mc.cjc * m_log<T, MM>(srate)));
-    if (m.sigy > m.seqv) {
+    const bool back = m.sigy > m.seqv;
+    if (back) {
       m.epeff = m.epeff - depeff;
     } else {
       const T ratio = m.sigy / m.seqv;
@@ -1843,6 +1858,56 @@ __device__ __forceinline__ void m3dm5(Cst<T>& m, const MatC& mc, T G2, double dt
       for (int k = 0; k < 6; k++) m.sd[k] = m.sd[k] * ratio;
       m.seqv = m.seqv * ratio;
     }
+    jc_assign<MM>(m, back ? 0.0 : (double)depeff);
+  }
+}
+
+// M3DM4 (john, Constitution.f90:969-1040) and M3DM8 (jcf, :1144-1212):
+// M3DM5's return with the temperature factor 1 - tstar**mjc, tstar from the
+// temperature the particle entered the step with (jcf: clamped to [0, 1],
+// and a particle past its melting point fails); john also takes the energy
+// terms and the elastic pressure, jcf gets its pressure from the EOS
+template <typename T, int MM, bool JCF>
+__device__ __forceinline__ void m3dm48(Cst<T>& m, const MatC& mc, T G2, T K3, double dtd, double tmprt,
+                                       bool& failure) {
+  const double e4 = (double)0.0001f, rmin = (double)0.001f;
+  if (!JCF) m.ieinc = sig_dot(m);
+  double ep = (double)m.epeff + e4;
+  const T plamod = (T)(mc.bjc * mc.njc * m_pow<T, MM>(ep, mc.njc - 1));
+  ep = ep - e4;
+  m.epeff = (T)ep;
+  elastic_devi(m, G2);
+  m.seqv = eq_stress(m);
+  if (m.seqv > m.sigy) {
+    const T depeff = (m.seqv - m.sigy) / ((T)1.5 * G2 + plamod);
+    m.epeff = m.epeff + depeff;
+    double tstar = (tmprt - mc.roomt) / (mc.melt - mc.roomt);
+    if (JCF) {
+      if (tstar > 1.0) {
+        tstar = 1.0;
+        failure = true;
+      } else if (tstar < 0.0) {
+        tstar = 0.0;
+      }
+    }
+    double srate = (double)depeff / mc.epso / dtd;
+    if (srate < rmin) srate = rmin;
+    m.sigy = (T)((mc.yield0 + mc.bjc * m_pow<T, MM>((double)m.epeff, mc.njc)) * (1 + mc.cjc * m_log<T, MM>(srate)) *
+                 (1 - m_pow<T, MM>(tstar, mc.mjc)));
+    const bool back = m.sigy > m.seqv;
+    if (back) {
+      m.epeff = m.epeff - depeff;
+    } else {
+      const T ratio = m.sigy / m.seqv;
+#pragma unroll
+      for (int k = 0; k < 6; k++) m.sd[k] = m.sd[k] * ratio;
+      m.seqv = m.seqv * ratio;
+    }
+    jc_assign<MM>(m, back ? 0.0 : (double)depeff);
+  }
+  if (!JCF) {
+    elastic_p(m, K3);
+    m.ieinc = new_dot(m, m.ieinc);
   }
 }
 
@@ -1938,7 +2003,9 @@ __device__ __forceinline__ void m3dm7(Cst<T>& m, const MatC& mc, T dt) {
 // Constitution (Constitution.f90:79-368) for the models on the path:
 // elas (M3DM1), pla1 (M3DM2), pla2 (M3DM3), sjc and sjcf (M3DM5 + EOS),
 // HE burn (M3DM6 + JWL), null (M3DM7 + EOS), Drucker-Prager (M3DM9),
-// neo-Hookean (M3DM10).
+// neo-Hookean (M3DM10); with MM_JC also john (M3DM4) and jcf (M3DM8 +
+// EOS), whose temperature, damage and failure read the increment another
+// particle may have assigned and follow in k_x_jc_tail.
 // material models compiled into a stress kernel: bit (1 << type); a launch
 // over a group whose instances use one model gets the kernel with only that
 // model (fewer registers, higher occupancy), mixed groups the general one
@@ -1971,6 +2038,7 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
   m.sig[5] = m.sd[5];
   m.bqf = (T)0;
   m.ieinc = (T)0;
+  if (MM & MM_JC) m.jcf = 0;
   const T G2 = (T)mc.G2, K3 = (T)mc.K3;
   // a model outside MM cannot occur here (the engine picks MM from the group)
   const int mtype = MM == (1 << MPM_MAT_ELASTIC) ? MPM_MAT_ELASTIC : MM == (1 << MPM_MAT_PLA1) ? MPM_MAT_PLA1
@@ -1995,6 +2063,7 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
       if (m.seqv > y0) {
         const T depeff = (m.seqv - m.sigy) / ((T)1.5 * G2);
         m.epeff = m.epeff + depeff;
+        jc_assign<MM>(m, (double)depeff);
         const T ratio = y0 / m.seqv;
 #pragma unroll
         for (int k = 0; k < 6; k++) m.sd[k] = m.sd[k] * ratio;
@@ -2014,6 +2083,7 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
         const T H = (T)mc.PlaMod;
         const T depeff = (m.seqv - m.sigy) / ((T)1.5 * G2 + H);
         m.epeff = m.epeff + depeff;
+        jc_assign<MM>(m, (double)depeff);
         m.sigy = m.sigy + H * depeff;
         const T ratio = m.sigy / m.seqv;
 #pragma unroll
@@ -2033,6 +2103,31 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
       sigrot(m, vort);
       m3dm5<T, MM>(m, mc, G2, dtd);
       seleos<T, MM>(m, p, mc, I, dt, false, 0.0);
+      break;
+    case MPM_MAT_JOHN:   // Constitution.f90:224-237: bulkq and its energy term twice (here and in lieupd)
+      if (MM & MM_JC) {
+        bool melted = false;
+        sigrot(m, vort);
+        m3dm48<T, MM, false>(m, mc, G2, K3, dtd, p.d(D_TEMP), melted);
+        if (m.sm < (T)0) bulkq(m, I, dt);
+        m.ieinc = m.ieinc - (T)2 * m.bqf * (m.dinc[0] + m.dinc[1] + m.dinc[2]);
+        lieupd(m, I, dt);
+        m.jcf |= JC_HEAT;   // the temperature rise needs the increment the particle reads
+      }
+      break;
+    case MPM_MAT_JCF:    // Constitution.f90:278-319 up to the EOS; the rest reads depeff (k_x_jc_tail)
+      if (MM & MM_JC) {
+        sigrot(m, vort);
+        bool failure = p.d(D_FAIL) != 0.0;
+        if (!failure) {
+          m3dm48<T, MM, true>(m, mc, G2, K3, dtd, p.d(D_TEMP), failure);
+          seleos<T, MM>(m, p, mc, I, dt, failure, 0.0);
+          m.jcf |= JC_TAIL | (failure ? JC_MELTED : 0);
+        } else {   // a failed particle flows as the null material (M3DM7)
+          m3dm7(m, mc, dt);
+          seleos<T, MM>(m, p, mc, I, dt, true, 0.0);
+        }
+      }
       break;
     case MPM_MAT_SJCF: {   // Constitution.f90:245-276
       sigrot(m, vort);
@@ -2091,7 +2186,9 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
         } else {
           const T dlamd = (m.sm - tenf) / Kmod;
           m.sm = tenf;
-          m.epeff = m.epeff + dlamd * third * (T)sqrtf(2.0f);
+          const T depeff = dlamd * third * (T)sqrtf(2.0f);
+          m.epeff = m.epeff + depeff;
+          jc_assign<MM>(m, (double)depeff);
         }
       }
       if (shear) {
@@ -2102,7 +2199,9 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
 #pragma unroll
         for (int k = 0; k < 6; k++) m.sd[k] = m.sd[k] * ratio;
         m.seqv = m.seqv * ratio;
-        m.epeff = m.epeff + dlamd * sqrt(third + twoninth * (qpsi * qpsi));
+        const T depeff = dlamd * sqrt(third + twoninth * (qpsi * qpsi));
+        m.epeff = m.epeff + depeff;
+        jc_assign<MM>(m, (double)depeff);
       }
       m.ieinc = new_dot(m, m.ieinc);
       lieupd(m, I, dt);
@@ -2151,7 +2250,8 @@ __device__ __forceinline__ void constitution(const PV<T>& p, const InstC& I, con
 #pragma unroll
   for (int k = 0; k < 6; k++) p.t(T_SD0 + k) = m.sd[k];
   // only pla2 and the Johnson-Cook models harden; cp never changes
-  if (mm_has(MM, MPM_MAT_PLA2) || mm_has(MM, MPM_MAT_SJC) || mm_has(MM, MPM_MAT_SJCF)) p.t(T_SIGY) = m.sigy;
+  if (mm_has(MM, MPM_MAT_PLA2) || mm_has(MM, MPM_MAT_SJC) || mm_has(MM, MPM_MAT_SJCF) || (MM & MM_JC))
+    p.t(T_SIGY) = m.sigy;
   p.t(T_EPEFF) = m.epeff;
   p.d(D_IE) = m.iener;
   p.t(T_Q) = m.bqf;
@@ -2288,6 +2388,14 @@ __device__ __forceinline__ void stress_tail(const PoolPtrs<T>& P, const PV<T>& p
       }
     }
     constitution<T, MM>(p, I, mc, dt, dtd, ts.time, de, vort, F, deF, m);
+    if ((MM & MM_JC) && (I.flags & IF_JC) && m.jcf) {   // the record the Johnson-Cook scan and tails read
+      const long r = (long)I.p_begin + p.n(I_ORIG), np = P.np;
+      P.jcf[r] = m.jcf;
+      P.jc[r] = m.dep;
+      P.jc[np + r] = m.vold;
+      P.jc[2 * np + r] = (double)m.smold;
+      P.jc[3 * np + r] = (double)m.ieinc;
+    }
     // counted per instance: a negative volume (the reference writes a
     // warning and goes on, Constitution.f90:135-137) and a non-finite volume
     // or stress (NaN or inf, which the reference would carry on silently)

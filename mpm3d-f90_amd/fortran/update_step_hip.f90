@@ -34,6 +34,11 @@
 ! (GridMassInitial only creates it: the device takes the initial mass in the
 ! first step), and such decks run in the reference-order FP64 mode.
 !
+! Decks with the Johnson-Cook models john and jcf also run in that mode: a
+! particle that does not yield heats (and jcf damages) with the plastic
+! strain increment of the last particle before it in the reference's order
+! that did, which that mode keeps.
+!
 ! Compiles against the reference's module files (ParticleData, GridData,
 ! MaterialData).
 ! ---------------------------------------------------------------------------
@@ -66,6 +71,7 @@ contains
     integer(c_int) :: prec
     character(len=16) :: env
     real(8) :: dt0, cp0
+    logical :: jc
     if (c_associated(ctx)) return
     n = nb_particle
     prob%nb_particle = n
@@ -185,8 +191,13 @@ contains
     call get_environment_variable('MPM3D_PRECISION', env)
     if (trim(env) == 'fp32') prec = MPM_PREC_FP32
     if (trim(env) == 'fp64exact') prec = MPM_PREC_FP64_EXACT
-    if ((contact .or. TLMPM) .and. prec /= MPM_PREC_FP64_EXACT) then
-       write(*, '(a)') ' update_step_hip: contact and total-Lagrangian decks run in the reference-order FP64 mode'
+    jc = .false.
+    do m = 1, nb_mat
+       jc = jc .or. mat_list(m)%MatType == 4 .or. mat_list(m)%MatType == 7
+    end do
+    if ((contact .or. TLMPM .or. jc) .and. prec /= MPM_PREC_FP64_EXACT) then
+       write(*, '(a)') ' update_step_hip: contact, total-Lagrangian and Johnson-Cook (john, jcf) decks run in ' // &
+                       'the reference-order FP64 mode'
        prec = MPM_PREC_FP64_EXACT
     end if
     call mpm_check(mpm_ctx_create(c_loc(prob), prec, ctx), 'mpm_ctx_create')

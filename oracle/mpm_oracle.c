@@ -63,6 +63,21 @@ struct oracle {
    * particle's last stress update and the running count of each branch */
   int8_t* dpbr;
   long dpcount[4];
+  /* MaterialModel's module variable depeff (Constitution.f90:55): the last
+   * plastic strain increment any particle's update assigned, read by the
+   * Johnson-Cook models of particles that do not assign it this step
+   * (Constitution.f90:237, 290, 314); 0 before the first assignment */
+  double depeff;
+  /* diagnostics of the Johnson-Cook models (not reference state), running
+   * counts: john/jcf updates that read a depeff another particle assigned
+   * (JC_STALE), jcf failures by melting (JC_MELT), by the failure strain
+   * (JC_EPF) and by damage (JC_DMG) */
+  long jccount[4];
+  /* test switch, not the reference: 1 starts every particle's update from
+   * depeff = 0 instead of the value the last update left (a plausible
+   * misreading of the module variable, for tests that show the fixtures
+   * tell the two apart) */
+  int fresh_depeff;
   /* ln J of M3DM10: 0 the C library's log (as the reference's Fortran
    * runtime), 1 the correctly rounded logarithm of crlog.h */
   int log_cr;
@@ -1069,6 +1084,7 @@ typedef struct {
   double mu, rv, bfac;
   double F[3][3];
   int br;   /* M3DM9 branch taken (DP_BR_*), diagnostics only */
+  int assigned;   /* this update assigned depeff (diagnostics only) */
   int log_cr;
 } cst_t;
 
@@ -1181,6 +1197,7 @@ static void m3dm2(cst_t* m) {          /* :885-923 elastic-perfectly plastic */
   m->seqv = equivalent_stress(m);
   if (m->seqv > m->yield0_) {
     m->depeff = (m->seqv - m->sig_y_) / (1.5 * m->G2);
+    m->assigned = 1;
     m->epeff_ = m->epeff_ + m->depeff;
     m->ratio = m->yield0_ / m->seqv;
     for (int i = 0; i < 6; i++) m->sd[i] = m->sd[i] * m->ratio;
@@ -1196,6 +1213,7 @@ static void m3dm3(cst_t* m) {          /* :926-966 J2 isotropic linear hardening
   m->seqv = equivalent_stress(m);
   if (m->seqv > m->sig_y_) {
     m->depeff = (m->seqv - m->sig_y_) / (1.5 * m->G2 + m->PlaMod);
+    m->assigned = 1;
     m->epeff_ = m->epeff_ + m->depeff;
     m->sig_y_ = m->sig_y_ + m->PlaMod * m->depeff;
     m->ratio = m->sig_y_ / m->seqv;
@@ -1253,6 +1271,7 @@ static void m3dm9(cst_t* m, const mpm_material* mat, double young, double poisso
       double dlamd = (m->sm - tenf) / Kmod;
       m->sm = tenf;
       m->depeff = dlamd * third32 * (double)sqrtf(2.0f);
+      m->assigned = 1;
       m->epeff_ = m->epeff_ + m->depeff;
     }
   }
@@ -1265,6 +1284,7 @@ static void m3dm9(cst_t* m, const mpm_material* mat, double young, double poisso
     for (int i = 0; i < 6; i++) m->sd[i] = m->sd[i] * m->ratio;
     m->seqv = m->seqv * m->ratio;
     m->depeff = dlamd * sqrt(third32 + twoninth32 * (qpsi * qpsi));
+    m->assigned = 1;
     m->epeff_ = m->epeff_ + m->depeff;
   }
   m->ieinc = new_stress_dot_dinc(m, m->ieinc);
@@ -1318,10 +1338,82 @@ static void m3dm5(cst_t* m, const mpm_material* mat, double DT) {
   m->seqv = equivalent_stress(m);
   if (m->seqv > m->sig_y_) {
     m->depeff = (m->seqv - m->sig_y_) / (1.5 * m->G2 + m->PlaMod);
+    m->assigned = 1;
     m->epeff_ = m->epeff_ + m->depeff;
     double srate = m->depeff / epso / DT;
     if (srate < rmin) srate = rmin;
     m->sig_y_ = (m->yield0_ + Bjc * o_pow(m, m->epeff_, njc)) * (1 + Cjc * o_log(m, srate));
+    if (m->sig_y_ > m->seqv) {
+      m->epeff_ = m->epeff_ - m->depeff;
+      m->depeff = 0.0;
+    } else {
+      m->ratio = m->sig_y_ / m->seqv;
+      for (int i = 0; i < 6; i++) m->sd[i] = m->sd[i] * m->ratio;
+      m->seqv = m->seqv * m->ratio;
+    }
+  }
+}
+
+/* M3DM4, Johnson-Cook (Constitution.f90:969-1040): M3DM5's return with the
+ * homologous temperature factor (1 - tstar**mjc), tstar from the
+ * temperature the particle entered the step with (not clamped here) */
+static void m3dm4(cst_t* m, const mpm_material* mat, double DT, double tmprt) {
+  const double Bjc = mat->b_jc, njc = mat->n_jc, Cjc = mat->c_jc, mjc = mat->m_jc, epso = mat->epso;
+  const double e4 = (double)0.0001f, rmin = (double)0.001f;
+  m->ieinc = sig_dot_dinc(m);
+  m->epeff_ = m->epeff_ + e4;
+  m->PlaMod = Bjc * njc * o_pow(m, m->epeff_, njc - 1);
+  m->epeff_ = m->epeff_ - e4;
+  elastic_devi(m);
+  m->seqv = equivalent_stress(m);
+  if (m->seqv > m->sig_y_) {
+    m->depeff = (m->seqv - m->sig_y_) / (1.5 * m->G2 + m->PlaMod);
+    m->assigned = 1;
+    m->epeff_ = m->epeff_ + m->depeff;
+    const double tstar = (tmprt - mat->roomt) / (mat->melt - mat->roomt);
+    double srate = m->depeff / epso / DT;
+    if (srate < rmin) srate = rmin;
+    m->sig_y_ = (m->yield0_ + Bjc * o_pow(m, m->epeff_, njc)) * (1 + Cjc * o_log(m, srate)) *
+                (1 - o_pow(m, tstar, mjc));
+    if (m->sig_y_ > m->seqv) {
+      m->epeff_ = m->epeff_ - m->depeff;
+      m->depeff = 0.0;
+    } else {
+      m->ratio = m->sig_y_ / m->seqv;
+      for (int i = 0; i < 6; i++) m->sd[i] = m->sd[i] * m->ratio;
+      m->seqv = m->seqv * m->ratio;
+    }
+  }
+  elastic_p(m);
+  m->ieinc = new_stress_dot_dinc(m, m->ieinc);
+}
+
+/* M3DM8, Johnson-Cook for jcf (Constitution.f90:1144-1212): M3DM4's return
+ * without the energy terms or the pressure update (the EOS follows), tstar
+ * clamped to [0, 1]; a particle past its melting temperature fails */
+static void m3dm8(cst_t* m, const mpm_material* mat, double DT, double tmprt, int* failure) {
+  const double Bjc = mat->b_jc, njc = mat->n_jc, Cjc = mat->c_jc, mjc = mat->m_jc, epso = mat->epso;
+  const double e4 = (double)0.0001f, rmin = (double)0.001f;
+  m->epeff_ = m->epeff_ + e4;
+  m->PlaMod = Bjc * njc * o_pow(m, m->epeff_, njc - 1);
+  m->epeff_ = m->epeff_ - e4;
+  elastic_devi(m);
+  m->seqv = equivalent_stress(m);
+  if (m->seqv > m->sig_y_) {
+    m->depeff = (m->seqv - m->sig_y_) / (1.5 * m->G2 + m->PlaMod);
+    m->assigned = 1;
+    m->epeff_ = m->epeff_ + m->depeff;
+    double tstar = (tmprt - mat->roomt) / (mat->melt - mat->roomt);
+    if (tstar > 1.0) {
+      tstar = 1.0;
+      *failure = 1;
+    } else if (tstar < 0.0) {
+      tstar = 0.0;
+    }
+    double srate = m->depeff / epso / DT;
+    if (srate < rmin) srate = rmin;
+    m->sig_y_ = (m->yield0_ + Bjc * o_pow(m, m->epeff_, njc)) * (1 + Cjc * o_log(m, srate)) *
+                (1 - o_pow(m, tstar, mjc));
     if (m->sig_y_ > m->seqv) {
       m->epeff_ = m->epeff_ - m->depeff;
       m->depeff = 0.0;
@@ -1423,6 +1515,7 @@ static void constitution(oracle* o, int p, const double de[6], const double vort
   memset(&m, 0, sizeof m);
   for (int i = 0; i < 6; i++) m.dinc[i] = de[i];
   m.log_cr = o->log_cr;
+  m.depeff = o->fresh_depeff ? 0.0 : o->depeff;   /* the module variable as the last update left it */
   m.vold = P(o, PF_VOL, p);
   if (o->prob.update_v_by_fp) {
     double deF[3][3];
@@ -1458,7 +1551,8 @@ static void constitution(oracle* o, int p, const double de[6], const double vort
   m.specen = m.iener / m.vol0_;
   m.mu = m.den_ / m.den0_ - 1;
   m.rv = m.vol_ / m.vol0_;
-  const double DMG = P(o, PF_DMG, p);
+  const double DMG = P(o, PF_DMG, p), oldDMG = DMG;   /* DMG is never stored back */
+  const double tmprt = P(o, PF_TEMP, p);
   int failure = P(o, PF_FAIL, p) != 0.0;
   young = young * (1.0 - DMG + 0x1p-52);                 /* epsilon(1.0d0) */
   m.G2 = young / (1 + poisson);
@@ -1486,6 +1580,52 @@ static void constitution(oracle* o, int p, const double de[6], const double vort
     case MPM_MAT_ELASTIC: sigrot(&m, vort); m3dm1(&m); lieupd(&m, o); break;
     case MPM_MAT_PLA1: sigrot(&m, vort); m3dm2(&m); lieupd(&m, o); break;
     case MPM_MAT_PLA2: sigrot(&m, vort); m3dm3(&m); lieupd(&m, o); break;
+    case MPM_MAT_JOHN:   /* Constitution.f90:224-237: bulkq and its energy term twice (here and in lieupd) */
+      sigrot(&m, vort);
+      m3dm4(&m, mat, o->dt, tmprt);
+      if (!m.assigned) o->jccount[JC_STALE]++;
+      if (m.sm < 0) bulkq(&m, o);
+      m.ieinc = m.ieinc - 2.0 * m.bqf * (m.dinc[0] + m.dinc[1] + m.dinc[2]);
+      lieupd(&m, o);
+      P(o, PF_TEMP, p) = P(o, PF_TEMP, p) + m.seqv * m.depeff / m.den_ / mat->spec_heat;
+      break;
+    case MPM_MAT_JCF:    /* Constitution.f90:278-319 */
+      sigrot(&m, vort);
+      if (!failure) {
+        m3dm8(&m, mat, o->dt, tmprt, &failure);
+        if (!m.assigned) o->jccount[JC_STALE]++;
+        if (failure) o->jccount[JC_MELT]++;
+        seleos(&m, o, mat, failure);
+        if (m.epeff_ > mat->epf) {
+          m.epeff_ = mat->epf + (double)0.0000001f;
+          if (!failure) o->jccount[JC_EPF]++;
+          failure = 1;
+        } else {
+          double dmg = oldDMG + m.depeff / mat->epf;
+          if (dmg >= 1.0) dmg = 1.0;
+          const double dmg_ratio = (1 - dmg) / (1 - oldDMG);
+          for (int i = 0; i < 6; i++) m.sd[i] = m.sd[i] * dmg_ratio;
+          if (fabs(dmg - 1.0) < (double)1e-5f) {
+            if (!failure) o->jccount[JC_DMG]++;
+            failure = 1;
+          }
+        }
+        if (failure && m.sm < 0) {
+          m.sm = 0.0;
+          m.bqf = 0.0;
+          P(o, PF_VOL, p) = m.vold;
+        }
+        if (failure) {
+          for (int i = 0; i < 6; i++) m.sd[i] = 0.0;
+          m.iener = m.iener - m.ieinc + m.dvol * (m.smold + m.sm - 2 * m.bqf);
+        }
+        P(o, PF_FAIL, p) = failure;
+        P(o, PF_TEMP, p) = P(o, PF_TEMP, p) + m.seqv * m.depeff / m.den_ / mat->spec_heat;
+      } else {
+        m3dm7(&m, o->dt);
+        seleos(&m, o, mat, failure);
+      }
+      break;
     case MPM_MAT_SJC: sigrot(&m, vort); m3dm5(&m, mat, o->dt); seleos(&m, o, mat, failure); break;
     case MPM_MAT_SJCF:   /* Constitution.f90:245-276 */
       sigrot(&m, vort);
@@ -1539,6 +1679,7 @@ static void constitution(oracle* o, int p, const double de[6], const double vort
   P(o, PF_IE, p) = m.iener;
   P(o, PF_CP, p) = m.cp;
   P(o, PF_Q, p) = m.bqf;
+  o->depeff = m.depeff;
 }
 
 /* ParticleStressUpdate (update_step.f90:456-647), non-SGMP part */
@@ -2115,6 +2256,12 @@ void oracle_set_log_cr(oracle* o, int on) { o->log_cr = on != 0; }
 double oracle_log_cr(double x) { return mpm_log_cr(x); }
 double oracle_exp_cr(double x) { return mpm_exp_cr(x); }
 double oracle_pow_cr(double x, double y) { return mpm_pow_cr(x, y); }
+
+void oracle_set_fresh_depeff(oracle* o, int on) { o->fresh_depeff = on != 0; }
+
+void oracle_get_jc_counts(const oracle* o, long* counts4) {
+  for (int i = 0; i < 4; i++) counts4[i] = o->jccount[i];
+}
 
 void oracle_get_dp_branches(const oracle* o, int8_t* last, long* counts4) {
   if (last) memcpy(last, o->dpbr, (size_t)o->n);

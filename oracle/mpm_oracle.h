@@ -79,6 +79,15 @@ double oracle_pow_cr(double x, double y);
  * last stress update, and running counts per branch over all updates. */
 enum { DP_BR_ELASTIC = 0, DP_BR_SHEAR = 1, DP_BR_SHEAR_TENSILE = 2, DP_BR_TENSION = 3 };
 void oracle_get_dp_branches(const oracle* o, int8_t* last, long* counts4);
+/* Diagnostics of the Johnson-Cook models (Constitution.f90:224-237,
+ * 278-319), not reference state: running counts of john/jcf updates that
+ * read the depeff another particle assigned, and of jcf failures by melting,
+ * by the failure strain and by damage. */
+enum { JC_STALE = 0, JC_MELT = 1, JC_EPF = 2, JC_DMG = 3 };
+void oracle_get_jc_counts(const oracle* o, long* counts4);
+/* Test switch, not the reference: on = 1 starts every particle's update
+ * from depeff = 0 instead of the last update's value. */
+void oracle_set_fresh_depeff(oracle* o, int on);
 /* M3DM9 on one stress point with a zero strain increment (trial = input):
  * returns the branch; sd[6], *sm updated, *seqv and *depeff written. */
 /* FindInflNode's node list for the axis cases zc, yc, xc (-1, 0 or 2):

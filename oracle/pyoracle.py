@@ -57,6 +57,8 @@ def lib():
         L.oracle_nb_component.restype = C.c_int
         L.oracle_get_fix.argtypes = [vp, C.POINTER(C.c_int8)]
         L.oracle_get_dp_branches.argtypes = [vp, C.POINTER(C.c_int8), C.POINTER(C.c_long)]
+        L.oracle_get_jc_counts.argtypes = [vp, C.POINTER(C.c_long)]
+        L.oracle_set_fresh_depeff.argtypes = [vp, C.c_int]
         L.oracle_set_log_cr.argtypes = [vp, C.c_int]
         L.oracle_log_cr.argtypes = [C.c_double]
         L.oracle_log_cr.restype = C.c_double
@@ -151,6 +153,19 @@ class Oracle:
         lib().oracle_get_dp_branches(self.h, last.ctypes.data_as(C.POINTER(C.c_int8)),
                                      cnt.ctypes.data_as(C.POINTER(C.c_long)))
         return last, cnt
+
+    def set_fresh_depeff(self, on=True):
+        """test switch (not the reference): start every particle's update
+        from depeff = 0 instead of the value the last update left"""
+        lib().oracle_set_fresh_depeff(self.h, 1 if on else 0)
+
+    def jc_counts(self):
+        """running counts [john/jcf updates that read another particle's
+        depeff, jcf failures by melting, by the failure strain, by damage]
+        -- oracle diagnostics of the Johnson-Cook models."""
+        cnt = np.zeros(4, dtype=np.int64)
+        lib().oracle_get_jc_counts(self.h, cnt.ctypes.data_as(C.POINTER(C.c_long)))
+        return cnt
 
     def scalars(self):
         out = np.empty(8)

@@ -57,6 +57,12 @@ CONTACT_DECKS = ["blocks_contact_linear_usf", "blocks_contact_bspline_musl", "bl
 # MUSL, smoothed pressure), two sheared bodies (B-spline USF; GIMP USL with
 # gravity and smoothing; SGMP MUSL on a clamped floor); reference-order FP64 only
 TL_DECKS = ["beam_tl_linear_musl", "block_tl_bspline_usf", "block_tl_gimp_usl", "block_tl_sgmp_musl"]
+# Johnson-Cook with heating (john: Taylor impact, linear, MUSL; under a block
+# dropped on a pla1 pad, SGMP, USF) and with damage and failure (jcf with
+# the Mie-Gruneisen EOS, hit by a pla2 cube: GIMP, MUSL); the particles that
+# do not yield in a step heat (and damage) with the plastic strain increment
+# the last yielding particle before them left; reference-order FP64 only
+JC_DECKS = ["taylor_john_linear_musl", "impact_jcf_eos2_gimp_musl", "block_john_sgmp_usf"]
 # auxiliary-grid (SGMP) decks: clamped plate (MUSL), water column with
 # symmetric walls (MUSL), J2 block on a clamped floor (USF)
 SGMP_DECKS = ["plate_neoh_sgmp_musl", "dam_null_sgmp_musl", "block_j2_sgmp_usf", "block_smoo_sgmp_usf"]
@@ -117,6 +123,7 @@ PGROUPS = {
     "ie": ["ie"],
     "fp": ["fp%d%d" % (i, j) for i in (1, 2, 3) for j in (1, 2, 3)],
     "spre": ["spre"],
+    "temp": ["temp"],
 }
 GGROUPS = {"mg": ["mg"], "p": ["px", "py", "pz"], "f": ["fx", "fy", "fz"]}
 

@@ -108,10 +108,8 @@ def test_unsupported_or_bad_input_is_rejected(tmp_path, bad, msg):
 
 
 @pytest.mark.parametrize("mat,extra,msg", [
-    # john / jcf read the plastic strain increment another particle left behind
-    (" 1 john 8.93e-3 117e3 0.35 90 292 0.3 0.025 1.0 20 1800 385 1e-3", "", "order"),
-    (" 1 jcf 8.93e-3 117e3 0.35 90 292 0.3 0.025 1.0 20 1800 385 1e-3 2.0", "seos 1 2 3940 1.49 1.96 0", "order"),
     # models that call seleos need an EOS; JWL only with HE burn
+    (" 1 jcf 8.93e-3 117e3 0.35 90 292 0.3 0.025 1.0 20 1800 385 1e-3 2.0", "", "equation of state"),
     (" 1 sjc 8.93e-3 117e3 0.35 157 425 1.0 0.0 1e-3", "", "equation of state"),
     (" 1 hiex 1.63e-3 6930", "", "equation of state"),
     (" 1 sjc 8.93e-3 117e3 0.35 157 425 1.0 0.0 1e-3", "seos 1 3 3.7e5 3.2e3 4.15 0.95 0.3 6993", "JWL"),
@@ -123,6 +121,22 @@ def test_material_and_eos_combinations_rejected(tmp_path, mat, extra, msg):
     with pytest.raises(mpm.MpmError) as e:
         mpm.read_deck(write(tmp_path, text))
     assert msg in str(e.value)
+
+
+def test_johnson_cook_parameters(tmp_path):
+    """john and jcf (DataIn.f90:427-478): density, E, nu, A, B, n, C, m,
+    room and melting temperature, specific heat, reference strain rate and
+    (jcf) the failure strain, in that order, each read as a default REAL"""
+    mpm = load_package()
+    vals = [8.93e-3, 117e3, 0.35, 90.0, 292.0, 0.3, 0.025, 1.0, 20.0, 1800.0, 385.0, 1e-3]
+    text = BASE.replace(" 1 elas 1000 1e6 0.25", " 1 john " + " ".join(repr(v) for v in vals))
+    m = mpm.read_deck(write(tmp_path, text)).mats[0]
+    got = [m.density, m.young, m.poisson, m.yield0, m.b_jc, m.n_jc, m.c_jc, m.m_jc, m.roomt, m.melt, m.spec_heat,
+           m.epso]
+    assert m.type == 4 and got == [float(np.float32(v)) for v in vals]
+    text = BASE.replace(" 1 elas 1000 1e6 0.25", " 1 jcf " + " ".join(repr(v) for v in vals) + " 0.25")
+    m = mpm.read_deck(write(tmp_path, text.replace("endi", "seos 1 2 3940 1.49 1.96 0\nendi"))).mats[0]
+    assert m.type == 7 and m.epf == 0.25 and m.melt == 1800.0 and m.eos_type == 2 and m.epso == float(np.float32(1e-3))
 
 
 def test_eos_constants_and_lighting_time(tmp_path):

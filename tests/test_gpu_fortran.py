@@ -60,7 +60,8 @@ def test_fortran_host_matches_oracle(oracle_mod, name):
 
 @pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "dam_null_sgmp_musl",
                                   "block_j2_sgmp_usf", "slope_smoo_bspline_musl", "block_smoo_sgmp_usf",
-                                  "blocks_contact_bspline_musl", "beam_tl_linear_musl", "block_tl_sgmp_musl"])
+                                  "blocks_contact_bspline_musl", "beam_tl_linear_musl", "block_tl_sgmp_musl",
+                                  "impact_jcf_eos2_gimp_musl", "block_john_sgmp_usf"])
 def test_reference_driver_runs_the_device_step(oracle_mod, name):
     """The reference's MPM3D.f90 loop, unchanged, over update_step_hip."""
     if not os.path.exists(DROPIN):
@@ -97,7 +98,7 @@ REFPROG = os.path.join(ROOT, "oracle", "_ref", "mpm3d_ref")
 
 
 @pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "block_dp_bspline_pull_musl", "dam_null_sgmp_musl",
-                                  "cube_smoo_linear_usl", "blocks_contact_gimp_usl"])
+                                  "cube_smoo_linear_usl", "blocks_contact_gimp_usl", "impact_jcf_eos2_gimp_musl"])
 def test_dropin_writes_the_reference_output_files(name):
     """The reference program itself (oracle/_ref/mpm3d_ref: MPM3D.f90 with its
     own update_step.f90, on the CPU) and the drop-in (the same main program
