@@ -81,7 +81,10 @@ int mpm_deck_read_outputs(const char* path, mpm_outputs* out, char* errbuf, int 
 /* ---- context -------------------------------------------------------------
  * Uploads the problem (particles as SoA, grid description, materials,
  * flags) to the current HIP device.  The device keeps DT, time and step
- * count; the host copy of the problem may be freed afterwards. */
+ * count; the host copy of the problem may be freed afterwards.  Contact
+ * ('cont lagr'), total-Lagrangian ('TLmp') and Johnson-Cook ('john', 'jcf')
+ * problems run in MPM_PREC_FP64_EXACT only; the other modes return
+ * MPM_ERR_UNSUPPORTED for them. */
 int mpm_ctx_create(const mpm_problem* prob, int precision, mpm_ctx** out);
 int mpm_ctx_destroy(mpm_ctx* ctx);
 
@@ -92,7 +95,9 @@ int mpm_begin_step(mpm_ctx* ctx);
 int mpm_grid_momentum_initial(mpm_ctx* ctx);
 /* ApplyBoundaryConditions (update_step.f90:1795-1830) */
 int mpm_apply_boundary_conditions(mpm_ctx* ctx);
-/* ParticleStressUpdate (update_step.f90:456-647, Constitution.f90:79-368) */
+/* ParticleStressUpdate (update_step.f90:456-647, Constitution.f90:79-368);
+ * with john/jcf materials it also resolves the plastic strain increment each
+ * particle reads from the one before it (Constitution.f90:237, 290, 314) */
 int mpm_particle_stress_update(mpm_ctx* ctx);
 /* GridMomentumUpdate (update_step.f90:831-1013) */
 int mpm_grid_momentum_update(mpm_ctx* ctx);

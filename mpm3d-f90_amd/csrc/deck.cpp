@@ -751,7 +751,6 @@ extern "C" int mpm_deck_read(const char* path, mpm_problem** out, char* errbuf, 
       for (int m = 0; m < d.nb_mat; m++)
         if (d.mats[m].type != MPM_MAT_NEOH)
           d.fail("total-Lagrangian MPM supports the neo-Hookean material only (material " + std::to_string(m + 1) + ")");
-      if (d.contact) d.fail("total-Lagrangian MPM together with contact is not supported");
     }
     p->contact = d.contact;
     p->fricfa = d.fricfa;

@@ -480,7 +480,6 @@ static void check_problem(const mpm_problem* p) {
     for (int m = 0; m < p->nb_mat; m++)
       if (p->mats[m].type != MPM_MAT_NEOH)
         fail(MPM_ERR_UNSUPPORTED, "total-Lagrangian MPM with material type %d (neo-Hookean only)", p->mats[m].type);
-    if (p->contact) fail(MPM_ERR_UNSUPPORTED, "total-Lagrangian MPM together with contact");
   }
   long ncell = (long)p->ncx * p->ncy * p->ncz;
   long nnode = (long)(p->ncx + 1) * (p->ncy + 1) * (p->ncz + 1);

@@ -2048,6 +2048,7 @@ void oracle_tl_particle_stress_update(oracle* o) {
  * GridMomentumUpdate */
 void oracle_tl_grid_momentum_update(oracle* o) {
   for (int k = 0; k < o->nntot; k++) G(o, GF_FX, k) = G(o, GF_FY, k) = G(o, GF_FZ, k) = 0.0;
+  if (o->prob.contact) memset(o->ndir, 0, (size_t)3 * o->nntot * sizeof(double));
   if (o->sgmp) {
     if (!o->cpk) o->cpk = (double*)calloc(9 * (size_t)o->ncell, sizeof(double));
     memset(o->cpk, 0, 9 * (size_t)o->ncell * sizeof(double));
@@ -2102,6 +2103,12 @@ void oracle_tl_grid_momentum_update(oracle* o) {
       G(o, GF_FX, k) = G(o, GF_FX, k) + fi0 + fx[0] * N;
       G(o, GF_FY, k) = G(o, GF_FY, k) + fi1 + fx[1] * N;
       G(o, GF_FZ, k) = G(o, GF_FZ, k) + fi2 + fx[2] * N;
+      if (pr->contact) {   /* update_step.f90:1155-1160 */
+        const double mp = P(o, PF_MASS, p);
+        o->ndir[3 * k] = o->ndir[3 * k] + dX * mp;
+        o->ndir[3 * k + 1] = o->ndir[3 * k + 1] + dY * mp;
+        o->ndir[3 * k + 2] = o->ndir[3 * k + 2] + dZ * mp;
+      }
     }
   }
   if (o->sgmp) {

@@ -57,6 +57,9 @@ CONTACT_DECKS = ["blocks_contact_linear_usf", "blocks_contact_bspline_musl", "bl
 # MUSL, smoothed pressure), two sheared bodies (B-spline USF; GIMP USL with
 # gravity and smoothing; SGMP MUSL on a clamped floor); reference-order FP64 only
 TL_DECKS = ["beam_tl_linear_musl", "block_tl_bspline_usf", "block_tl_gimp_usl", "block_tl_sgmp_musl"]
+# total-Lagrangian MPM with contact: two blocks on grids of their own that
+# share a node layer in the initial configuration (B-spline, MUSL, friction)
+TL_CONTACT_DECKS = ["blocks_tl_contact_bspline_musl"]
 # Johnson-Cook with heating (john: Taylor impact, linear, MUSL; under a block
 # dropped on a pla1 pad, SGMP, USF) and with damage and failure (jcf with
 # the Mie-Gruneisen EOS, hit by a pla2 cube: GIMP, MUSL); the particles that

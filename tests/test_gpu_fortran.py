@@ -61,7 +61,8 @@ def test_fortran_host_matches_oracle(oracle_mod, name):
 @pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "dam_null_sgmp_musl",
                                   "block_j2_sgmp_usf", "slope_smoo_bspline_musl", "block_smoo_sgmp_usf",
                                   "blocks_contact_bspline_musl", "beam_tl_linear_musl", "block_tl_sgmp_musl",
-                                  "impact_jcf_eos2_gimp_musl", "block_john_sgmp_usf"])
+                                  "impact_jcf_eos2_gimp_musl", "block_john_sgmp_usf",
+                                  "blocks_tl_contact_bspline_musl"])
 def test_reference_driver_runs_the_device_step(oracle_mod, name):
     """The reference's MPM3D.f90 loop, unchanged, over update_step_hip."""
     if not os.path.exists(DROPIN):

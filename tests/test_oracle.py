@@ -15,9 +15,10 @@ import tempfile
 import numpy as np
 import pytest
 
-from conftest import ALL_DECKS, CONTACT_DECKS, GOLDEN_DIR, JC_DECKS, TL_DECKS, deck_path, load_package
+from conftest import (ALL_DECKS, CONTACT_DECKS, GOLDEN_DIR, JC_DECKS, TL_CONTACT_DECKS, TL_DECKS, deck_path,
+                      load_package)
 
-DECKS = ALL_DECKS + CONTACT_DECKS + TL_DECKS + JC_DECKS
+DECKS = ALL_DECKS + CONTACT_DECKS + TL_DECKS + TL_CONTACT_DECKS + JC_DECKS
 
 PF_SKIP_NAN = ()
 
