@@ -230,9 +230,13 @@ struct mpm_engine {
   int* i_alt = nullptr;
   int* d_seg = nullptr;           // segment begin/end slots, 2 x max_inst
   std::vector<int> h_seg;         // host copy of d_seg (kept alive for the async upload)
-  // graph
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
+  // graph: a captured step sequence bakes in the particle buffers'
+  // addresses, and the re-sort swaps the buffers with their sort twins, so
+  // up to two graphs are kept, one per buffer set (gkey: the T buffer)
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  const void* gkey[2] = {nullptr, nullptr};
+  int gnext = 0;                  // slot the next capture replaces when both hold a graph
   int graph_steps = 0;
   bool graph_dirty = true;
   bool graph_plain_once = false;  // lazy replay: the changed lists already ran once without a graph
@@ -301,8 +305,10 @@ static void free_engine(mpm_engine* e) {
   if (!e) return;
   if (e->s) (void)hipStreamSynchronize(e->s);   // queued work may still use any buffer below
   for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
-  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-  if (e->graph) (void)hipGraphDestroy(e->graph);
+  for (int k = 0; k < 2; k++) {
+    if (e->gexec[k]) (void)hipGraphExecDestroy(e->gexec[k]);
+    if (e->graph[k]) (void)hipGraphDestroy(e->graph[k]);
+  }
   void* bufs[] = {e->t, e->d, e->ii, e->mg, e->pg, e->fg, e->p2, e->vg, e->ag, e->fix, e->d_inst, e->d_ts, e->d_mats,
                   e->d_bodies, e->part, e->d_ginst, e->d_nginst, e->d_ilist_all, e->k_in, e->k_out, e->v_in,
                   e->v_out, e->rs_scr, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg, e->boxes,
@@ -1473,6 +1479,43 @@ static void step_plain(mpm_engine* e, int nsteps, int fin_flags) {
 // only a second call with the same lists captures (continuous batching
 // changes the instance set every few steps, and a capture costs several
 // plain steps)
+static void drop_graphs(mpm_engine* e) {
+  for (int k = 0; k < 2; k++) {
+    if (e->gexec[k]) HIPCHK(hipGraphExecDestroy(e->gexec[k]));
+    if (e->graph[k]) HIPCHK(hipGraphDestroy(e->graph[k]));
+    e->gexec[k] = nullptr;
+    e->graph[k] = nullptr;
+    e->gkey[k] = nullptr;
+  }
+  e->gnext = 0;
+}
+
+// the graph of `per` steps over the current particle buffers, captured on
+// first use
+static hipGraphExec_t graph_for_buffers(mpm_engine* e, int per, int fin_flags) {
+  for (int k = 0; k < 2; k++)
+    if (e->gexec[k] && e->gkey[k] == e->t) return e->gexec[k];
+  const int k = !e->gexec[0] ? 0 : !e->gexec[1] ? 1 : e->gnext;
+  e->gnext = 1 - k;
+  if (e->gexec[k]) HIPCHK(hipGraphExecDestroy(e->gexec[k]));
+  if (e->graph[k]) HIPCHK(hipGraphDestroy(e->graph[k]));
+  e->gexec[k] = nullptr;
+  e->graph[k] = nullptr;
+  const bool was_prof = e->prof;
+  e->prof = false;
+  ensure_exact(e);   // no allocation inside the capture
+  HIPCHK(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
+  for (int q = 0; q < per; q++) {
+    if (e->prec == MPM_PREC_FP32) enqueue_step<float>(e, fin_flags);
+    else enqueue_step<double>(e, fin_flags);
+  }
+  HIPCHK(hipStreamEndCapture(e->s, &e->graph[k]));
+  e->prof = was_prof;
+  HIPCHK(hipGraphInstantiate(&e->gexec[k], e->graph[k], nullptr, nullptr, 0));
+  e->gkey[k] = e->t;
+  return e->gexec[k];
+}
+
 static void step_graph(mpm_engine* e, int nsteps, int per, int fin_flags, bool lazy) {
   rebuild_lists(e);
   if (per < 1) per = 1;
@@ -1482,21 +1525,7 @@ static void step_graph(mpm_engine* e, int nsteps, int per, int fin_flags, bool l
       e->graph_plain_once = true;
       return;
     }
-    if (e->gexec) HIPCHK(hipGraphExecDestroy(e->gexec));
-    if (e->graph) HIPCHK(hipGraphDestroy(e->graph));
-    e->gexec = nullptr;
-    e->graph = nullptr;
-    const bool was_prof = e->prof;
-    e->prof = false;
-    ensure_exact(e);   // no allocation inside the capture
-    HIPCHK(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < per; k++) {
-      if (e->prec == MPM_PREC_FP32) enqueue_step<float>(e, fin_flags);
-      else enqueue_step<double>(e, fin_flags);
-    }
-    HIPCHK(hipStreamEndCapture(e->s, &e->graph));
-    e->prof = was_prof;
-    HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+    drop_graphs(e);
     e->graph_steps = per;
     e->graph_dirty = false;
     e->graph_plain_once = false;
@@ -1505,7 +1534,7 @@ static void step_graph(mpm_engine* e, int nsteps, int per, int fin_flags, bool l
   while (done + per <= nsteps) {
     const long se = e->sort_every;
     if (se > 0 && e->since_sort + per > se) sort_particles(e);   // at most one period between re-sorts
-    HIPCHK(hipGraphLaunch(e->gexec, e->s));
+    HIPCHK(hipGraphLaunch(graph_for_buffers(e, per, fin_flags), e->s));
     done += per;
     e->steps_done += per;
     e->since_sort += per;
@@ -1670,12 +1699,13 @@ static void sort_particles(mpm_engine* e) {
   }
   // the cached node boxes describe the old particle order
   mpmk::launch_clear_ints(e->boxes, (e->np / 64) * BOX_INTS, e->s);
-  // copy back so the particle arrays keep their addresses (captured graphs stay valid)
-  HIPCHK(hipMemcpyAsync(e->t, e->t_alt, (size_t)T_COUNT * np * e->tsz, hipMemcpyDeviceToDevice, e->s));
-  HIPCHK(hipMemcpyAsync(e->d, e->d_alt, (size_t)D_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
-  HIPCHK(hipMemcpyAsync(e->ii, e->i_alt, (size_t)I_COUNT * np * sizeof(int), hipMemcpyDeviceToDevice, e->s));
-  if (e->pk)
-    HIPCHK(hipMemcpyAsync(e->pk, e->pk_alt, (size_t)TL_PK_COUNT * np * sizeof(double), hipMemcpyDeviceToDevice, e->s));
+  // the sorted twins become the particle arrays (every live granule was
+  // permuted; free granules are never read); captured step graphs are kept
+  // per buffer set
+  std::swap(e->t, e->t_alt);
+  std::swap(e->d, e->d_alt);
+  std::swap(e->ii, e->i_alt);
+  if (e->pk) std::swap(e->pk, e->pk_alt);
 }
 
 // --------------------------------------------------------- autotuner
