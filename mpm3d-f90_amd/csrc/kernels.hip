@@ -2695,6 +2695,15 @@ __device__ __forceinline__ void sgn_shape(const double xi[3], T shp[8]) {
 // Particle scatters to the auxiliary nodes: mass and momentum
 // (GridMomentumInitial, update_step.f90:318-345), forces (GridMomentumUpdate,
 // :906-930), momentum re-map (GridMomentumMUSL, :1689-1707).
+// LDS tile capacity (auxiliary nodes) of a scatter of nf FP64 fields (a
+// 128-node tile sent most blocks of the Cook-plate batch to the per-particle
+// global atomics and doubled the scatter time)
+__host__ __device__ constexpr int sgp_cap(int nf) { return nf > 4 ? TILE_NODES / 2 : TILE_NODES; }
+// staged record of a particle: natural coordinates (3), mass and velocity
+// (4), external force (3) and stress times volume (6)
+__host__ __device__ constexpr int sgp_nvr(int mode) {
+  return 3 + ((mode & (P2G_MASS | P2G_MOM)) ? 4 : 0) + ((mode & P2G_FORCE) ? 9 : 0);
+}
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glist, const int* ginst, int bpg) {
   int gran;
@@ -2706,7 +2715,7 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
   // LDS accumulation tile over the block's box of auxiliary nodes (FP64):
   // mass, 3 momentum, 3 external and 6 internal force components
   constexpr int NF = (MASS ? 1 : 0) + (MOM ? 3 : 0) + (FRC ? 9 : 0);
-  constexpr int CAP = NF > 4 ? TILE_NODES / 2 : TILE_NODES;
+  constexpr int CAP = sgp_cap(NF);
   __shared__ double tile[CAP * NF];
   PV<T> p{P, i};
   const bool live = i < I.p_end;
@@ -2730,7 +2739,8 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
   T* aux = P.aux + I.node_off;
   double* cmg = P.cmg + I.node_off;
   T m = (T)0, v[3] = {(T)0, (T)0, (T)0};
-  T fx[3] = {(T)0, (T)0, (T)0}, sg[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, vol = (T)0;
+  // sg: sigma * vol, the factor of every node's Cfint term N sigma V
+  T fx[3] = {(T)0, (T)0, (T)0}, sg[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0};
   if (act) {
     if (MASS || MOM) {
       m = p.t(T_MASS);
@@ -2738,10 +2748,9 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
     }
     if (FRC) {
       external_force(P, p, I, P.ts[ii].time, fx);
-      const T smq = p.t(T_SM) - p.t(T_Q);
-      sg[0] = smq + p.t(T_SD0); sg[1] = smq + p.t(T_SD1); sg[2] = smq + p.t(T_SD2);
-      sg[3] = p.t(T_SD5); sg[4] = p.t(T_SD3); sg[5] = p.t(T_SD4);   // Cfint(4..6): sxy, syz, sxz
-      vol = (T)p.d(D_VOL);
+      const T smq = p.t(T_SM) - p.t(T_Q), vol = (T)p.d(D_VOL);
+      sg[0] = (smq + p.t(T_SD0)) * vol; sg[1] = (smq + p.t(T_SD1)) * vol; sg[2] = (smq + p.t(T_SD2)) * vol;
+      sg[3] = p.t(T_SD5) * vol; sg[4] = p.t(T_SD3) * vol; sg[5] = p.t(T_SD4) * vol;   // Cfint(4..6): sxy, syz, sxz
     }
   }
   if (fast) {
@@ -2750,7 +2759,7 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
     // node's values over its part of the run in registers and adds the sums
     // to the tile, instead of one LDS atomic per particle, node and value
     constexpr int OM = 3, OF = OM + ((MASS || MOM) ? 4 : 0);
-    constexpr int NVR = OF + (FRC ? 10 : 0);   // fext (3), sigma (6), vol
+    constexpr int NVR = sgp_nvr(MODE);   // ..., fext (3), sigma vol (6)
     extern __shared__ __align__(16) char run_mem[];
     const RunStage<T, NVR> R(run_mem, blockDim.x);
     constexpr int NVV = RunStage<T, NVR>::NG * RunStage<T, NVR>::GV;
@@ -2772,7 +2781,6 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
         for (int d = 0; d < 3; d++) u[OF + d] = fx[d];
 #pragma unroll
         for (int k = 0; k < 6; k++) u[OF + 3 + k] = sg[k];
-        u[OF + 9] = vol;
       }
       R.put(jl, u);
       key = (ax - B.x0) + B.bx * ((ay - B.y0) + B.by * (az - B.z0));
@@ -2810,7 +2818,7 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
 #pragma unroll
           for (int d = 0; d < 3; d++) acc[q++] += (double)(u[OF + d] * N);
 #pragma unroll
-          for (int k = 0; k < 6; k++) acc[q++] += (double)(N * u[OF + 3 + k] * u[OF + 9]);
+          for (int k = 0; k < 6; k++) acc[q++] += (double)(N * u[OF + 3 + k]);
         }
       }
       // node n of the SNX/SNY/SNZ order: x + 1 for n & 3 in {1, 2}, y + 1 for {2, 3}, z + 1 for n >= 4
@@ -2838,7 +2846,7 @@ __global__ void __launch_bounds__(256) k_sgmp_p2g(PoolPtrs<T> P, const int* glis
 #pragma unroll
         for (int d = 0; d < 3; d++) val[q++] = fx[d] * N;
 #pragma unroll
-        for (int k = 0; k < 6; k++) val[q++] = N * sg[k] * vol;
+        for (int k = 0; k < 6; k++) val[q++] = N * sg[k];
       }
       const int k = sgmp_node(I, L, n);
       if (!k) continue;
@@ -3675,8 +3683,8 @@ void launch_sgmp_p2g(int mode, const PoolPtrs<T>& P, const LaunchP& L, hipStream
 #define SG_CASE(M)                                                                                       \
   case M: {                                                                                              \
     constexpr int NF = ((M) & P2G_MASS ? 1 : 0) + ((M) & P2G_MOM ? 3 : 0) + ((M) & P2G_FORCE ? 9 : 0);   \
-    constexpr int NVR = 3 + ((M) & (P2G_MASS | P2G_MOM) ? 4 : 0) + ((M) & P2G_FORCE ? 10 : 0);           \
-    const LaunchP L2 = fit_lds(L, (size_t)(NF > 4 ? TILE_NODES / 2 : TILE_NODES) * NF * 8,               \
+    constexpr int NVR = sgp_nvr(M);                                                                      \
+    const LaunchP L2 = fit_lds(L, (size_t)sgp_cap(NF) * NF * 8,                                          \
                                [](int bs) { return run_stage_bytes<T, NVR>(bs); });                      \
     const size_t sh = run_stage_bytes<T, NVR>(L2.bs);                                                    \
     hipLaunchKernelGGL((k_sgmp_p2g<T, M>), dim3(L2.nblk), dim3(L2.bs), sh, s, P, L2.glist, L2.ginst, L2.bpg); \
