@@ -1937,6 +1937,16 @@ static mpm_tuning autotune(mpm_engine* e, int trial_steps) {
 
 // --------------------------------------------------------- downloads
 
+// XX is ParticlePositionUpdate's scratch (update_step.f90:1378-1526), which
+// skips particles outside the grid (this step's cell index < 0): one that
+// has been outside since the start never had it set (the reference reads 0
+// there), one that left the grid keeps the position it moved to, its Xp.
+// Particles outside the grid do not move, so "outside and still at Xo" is
+// "never set".  x, xo: the particle's X and Xo fields, stride apart.
+static bool xx_never_set(int icell, const double* x, const double* xo, long stride) {
+  return icell < 0 && x[0] == xo[0] && x[stride] == xo[stride] && x[2 * stride] == xo[2 * stride];
+}
+
 template <typename T>
 static void download_particles_T(mpm_engine* e, int id, double* out) {
   HostInst& h = e->inst[id];
@@ -1986,9 +1996,11 @@ static void download_particles_T(mpm_engine* e, int id, double* out) {
     for (int k = 0; k < 3; k++) O(PF_XO + k) = db[(size_t)(D_XO + k) * pad + j];
     O(PF_VOL0) = db[(size_t)D_VOL0 * pad + j];
     for (int k = 0; k < TL_PK_COUNT; k++) O(PF_PK + k) = pkb.empty() ? 0.0 : pkb[(size_t)k * pad + j];
+    const int icell = ib[(size_t)I_CELL * pad + j];
+    const bool never = xx_never_set(icell, &db[(size_t)D_X * pad + j], &db[(size_t)D_XO * pad + j], pad);
     for (int k = 0; k < 3; k++) {
       const double x = db[(size_t)(D_X + k) * pad + j];
-      O(PF_XX + k) = h.xx_pending ? x + Tv(T_BX + k) * ts.dt : x;
+      O(PF_XX + k) = never ? 0.0 : (h.xx_pending && icell >= 0) ? x + Tv(T_BX + k) * ts.dt : x;
     }
   }
 }
@@ -2054,7 +2066,10 @@ static void download_subset_T(mpm_engine* e, int id, int k, const int32_t* ids, 
     for (int c = 0; c < 3; c++) O(PF_XO + c) = Dv(D_XO + c);
     O(PF_VOL0) = Dv(D_VOL0);
     for (int c = 0; c < TL_PK_COUNT; c++) O(PF_PK + c) = 0.0;   // not gathered (mpm_download_particles has it)
-    for (int c = 0; c < 3; c++) O(PF_XX + c) = h.xx_pending ? Dv(D_X + c) + Tv(T_BX + c) * ts.dt : Dv(D_X + c);
+    const int icell = (int)r[T_COUNT + D_COUNT + I_CELL];
+    const bool never = xx_never_set(icell, &r[T_COUNT + D_X], &r[T_COUNT + D_XO], 1);
+    for (int c = 0; c < 3; c++)
+      O(PF_XX + c) = never ? 0.0 : (h.xx_pending && icell >= 0) ? Dv(D_X + c) + Tv(T_BX + c) * ts.dt : Dv(D_X + c);
   }
 }
 

@@ -25,10 +25,12 @@ program mpm3d_host
   integer(8) :: c0, c1, crate
   real(c_double) :: ek, ei, mom(3), dt, sc(8), t_loop
   real(c_double), allocatable :: ps(:)
-  logical :: usf, musl
+  logical :: usf, musl, jc
+  type(mpm_material_t), pointer :: mats(:)
+  integer :: m
 
   if (command_argument_count() < 1) then
-     print *, 'usage: mpm3d_host <deck.mpm> [nsteps] [fp32|fp64] [dumpfile]'
+     print *, 'usage: mpm3d_host <deck.mpm> [nsteps] [fp32|fp64|fp64exact] [dumpfile]'
      stop 2
   end if
   call get_command_argument(1, deck)
@@ -41,6 +43,7 @@ program mpm3d_host
   if (command_argument_count() >= 3) then
      call get_command_argument(3, arg)
      if (trim(arg) == 'fp32') prec = MPM_PREC_FP32
+     if (trim(arg) == 'fp64exact') prec = MPM_PREC_FP64_EXACT
   end if
   dumpfile = ' '
   if (command_argument_count() >= 4) call get_command_argument(4, dumpfile)
@@ -52,6 +55,17 @@ program mpm3d_host
   end if
   call c_f_pointer(pprob, prob)
   n = prob%nb_particle
+  ! contact, total-Lagrangian and Johnson-Cook (john, jcf) problems run in
+  ! the reference-order FP64 mode only
+  call c_f_pointer(prob%mats, mats, [prob%nb_mat])
+  jc = .false.
+  do m = 1, prob%nb_mat
+     jc = jc .or. mats(m)%mtype == 4 .or. mats(m)%mtype == 7
+  end do
+  if ((prob%contact /= 0 .or. prob%tlmpm /= 0 .or. jc) .and. prec /= MPM_PREC_FP64_EXACT) then
+     write(*, '(a)') ' mpm3d_host: this deck runs in the reference-order FP64 mode (fp64exact)'
+     prec = MPM_PREC_FP64_EXACT
+  end if
   usf = prob%scheme == MPM_USF
   musl = prob%scheme == MPM_MUSL
   call mpm_check(mpm_ctx_create(pprob, prec, ctx), 'mpm_ctx_create')

@@ -47,7 +47,9 @@ DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
          "cube_drda_linear_usf",
          # SmoothStressbyGrid (smoo) on every shape family; the cube also
          # carries two grid components without contact
-         "cube_smoo_linear_usl", "bar_smoo_gimp_usf", "slope_smoo_bspline_musl"]
+         "cube_smoo_linear_usl", "bar_smoo_gimp_usf", "slope_smoo_bspline_musl",
+         # particles outside the grid: generated beyond the span, and leaving it
+         "blocks_outside_linear_usl"]
 # contact (cont lagr): two blocks pressed together on grids of their own
 # (linear, USF, frictionless, averaged normal), sliding with friction
 # (B-spline, MUSL, normal of body 1, a J2 block), and a third body on a third

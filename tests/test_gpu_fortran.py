@@ -34,7 +34,8 @@ def read_dump(path, n_expected):
 
 
 @pytest.mark.parametrize("name", ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "plate_neoh_musl_qulo",
-                                  "plate_neoh_sgmp_musl"])
+                                  "plate_neoh_sgmp_musl", "impact_jcf_eos2_gimp_musl",
+                                  "blocks_tl_contact_bspline_musl"])
 def test_fortran_host_matches_oracle(oracle_mod, name):
     if not os.path.exists(HOST):
         pytest.skip("Fortran host not built")
