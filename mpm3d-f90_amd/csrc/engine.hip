@@ -480,7 +480,8 @@ static void check_problem(const mpm_problem* p) {
       if (p->bodies[b].com < 1 || p->bodies[b].com > p->nb_component)
         fail(MPM_ERR_ARG, "body %d: component %d outside 1..%d", b + 1, p->bodies[b].com, p->nb_component);
     if (p->shape == MPM_SHAPE_SGMP) fail(MPM_ERR_UNSUPPORTED, "contact with SGMP");
-    if (p->smooth_stress) fail(MPM_ERR_UNSUPPORTED, "stress smoothing (smoo) together with contact");
+    if (p->smooth_stress && p->nb_component > 8)
+      fail(MPM_ERR_UNSUPPORTED, "stress smoothing (smoo) with contact over more than 8 grid components");
   }
   if (p->tlmpm) {   // TLConstitution has the neo-Hookean model only (Constitution.f90:487-497)
     for (int m = 0; m < p->nb_mat; m++)

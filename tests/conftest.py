@@ -54,7 +54,9 @@ DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
 # (linear, USF, frictionless, averaged normal), sliding with friction
 # (B-spline, MUSL, normal of body 1, a J2 block), and a third body on a third
 # component (GIMP, USL, normal of body 2, gravity); reference-order FP64 only
-CONTACT_DECKS = ["blocks_contact_linear_usf", "blocks_contact_bspline_musl", "blocks_contact_gimp_usl"]
+CONTACT_DECKS = ["blocks_contact_linear_usf", "blocks_contact_bspline_musl", "blocks_contact_gimp_usl",
+                 # with stress smoothing, which sums every body on component 1's grid (GIMP, MUSL, friction)
+                 "blocks_contact_smoo_gimp_musl"]
 # total-Lagrangian MPM (TLmp, neo-Hookean): a beam on a clamped base (linear,
 # MUSL, smoothed pressure), two sheared bodies (B-spline USF; GIMP USL with
 # gravity and smoothing; SGMP MUSL on a clamped floor); reference-order FP64 only

@@ -41,7 +41,8 @@ DECKS = ["bar_j2_gimp_musl", "slope_dp_bspline_usf", "cube_elastic_linear_usl",
          "block_smoo_sgmp_usf", "blocks_contact_linear_usf", "blocks_contact_bspline_musl",
          "blocks_contact_gimp_usl", "beam_tl_linear_musl", "block_tl_bspline_usf", "block_tl_gimp_usl",
          "block_tl_sgmp_musl", "taylor_john_linear_musl", "impact_jcf_eos2_gimp_musl", "block_john_sgmp_usf",
-         "blocks_tl_contact_bspline_musl", "blocks_outside_linear_usl"]
+         "blocks_tl_contact_bspline_musl", "blocks_outside_linear_usl",
+         "blocks_contact_smoo_gimp_musl"]
 S0_FIELDS = ["x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "vol", "sig_y", "ie", "cp", "temp", "lt",
              "xo", "yo", "zo", "vol0"]
 

@@ -8,7 +8,9 @@ device is bit-identical to:
   of every component and the contact force tot_cont_for;
 * the oracle after 50 and 200 steps, and after every phase (the contact pass
   included) of the first two steps through the per-phase entry points;
-and the other precisions refuse contact problems.
+and the other precisions refuse contact problems.  With stress smoothing the
+smoothed pressure of every body comes from component 1's grid, as the
+reference's SmoothStressbyGrid computes it.
 """
 import os
 
@@ -88,6 +90,10 @@ def test_contact_each_phase_entry_point_bitwise(mpm, oracle_mod, name):
             ctx.calcEnergy()
             o.phase("calc_energy")
             assert ctx.UpdateDT() == (o.phase("update_dt") or o.scalars()[2])
+            if prob.smooth_stress:   # every body's pressure smoothed on component 1's grid
+                ctx.SmoothStressbyGrid()
+                o.phase("smooth_stress")
+                assert_bitwise(mpm, ctx.particles(), o.particles(), what="%s step %d smoothed" % (name, step))
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])

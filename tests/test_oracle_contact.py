@@ -19,7 +19,8 @@ from conftest import CONTACT_DECKS, deck_path, load_package
 
 BRANCHES = {"blocks_contact_linear_usf": (True, False, False),
             "blocks_contact_bspline_musl": (False, True, True),
-            "blocks_contact_gimp_usl": (False, True, True)}
+            "blocks_contact_gimp_usl": (False, True, True),
+            "blocks_contact_smoo_gimp_musl": (False, True, True)}
 
 
 @pytest.mark.parametrize("name", CONTACT_DECKS)
