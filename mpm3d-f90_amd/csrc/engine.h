@@ -146,7 +146,8 @@ enum IField { I_BODY = 0, I_ORIG, I_CELL, I_COUNT };
 struct XPtrs {
   unsigned long long* key;   // m sort keys: pool node << obits | original index << 5 | j (invalid: all ones)
   int* rnode;                // m: instance-local node (SGMP: auxiliary node) of entry q, -1 if none
-  double* rw;                // 4 * m: N, dN/dx, dN/dy, dN/dz of entry q (field-major, stride m)
+  double* rw;                // 4 * m: N, dN/dx, dN/dy, dN/dz of entry q (entry-major, 32 B per entry)
+  double* prec;              // 12 * np: per-slot particle records of the node scatters (k_x_prec)
   int* seg0;                 // nn: first sorted entry of every pool node
   int* seg1;                 // nn: one past its last
   double* ndir;              // 3 * nn: contact normals (mass gradients) of every pool node (contact only)

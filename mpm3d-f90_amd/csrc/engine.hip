@@ -248,6 +248,7 @@ struct mpm_engine {
   int* x_val[2] = {nullptr, nullptr};
   int* x_rnode = nullptr;
   double* x_rw = nullptr;
+  double* x_prec = nullptr;   // per-slot particle records of the node scatters
   int* x_seg0 = nullptr;
   int* x_seg1 = nullptr;
   double* x_ndir = nullptr;       // contact normals (allocated with the first contact instance)
@@ -312,7 +313,7 @@ static void free_engine(mpm_engine* e) {
   void* bufs[] = {e->t, e->d, e->ii, e->mg, e->pg, e->fg, e->p2, e->vg, e->ag, e->fix, e->d_inst, e->d_ts, e->d_mats,
                   e->d_bodies, e->part, e->d_ginst, e->d_nginst, e->d_ilist_all, e->k_in, e->k_out, e->v_in,
                   e->v_out, e->rs_scr, e->t_alt, e->d_alt, e->i_alt, e->d_seg, e->aux, e->cmg, e->boxes,
-                  e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1], e->x_rnode, e->x_rw, e->x_seg0, e->x_seg1,
+                  e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1], e->x_rnode, e->x_rw, e->x_prec, e->x_seg0, e->x_seg1,
                   e->x_scr, e->d_sub_ids, e->d_sub_rows, e->sm.gpre, e->sm.gv, e->sm.cgpre, e->sm.cgv, e->sm.map,
                   e->sm.xs, e->x_ndir, e->x_cf, e->pk, e->pk_alt, e->jcf, e->jc};
   for (void* b : bufs)
@@ -1004,6 +1005,7 @@ static void ensure_exact(mpm_engine* e) {
   }
   e->x_rnode = dalloc<int>(m);
   e->x_rw = dalloc<double>(4 * m);
+  e->x_prec = dalloc<double>(12 * e->np);
   e->x_seg0 = dalloc<int>(e->nn);
   e->x_seg1 = dalloc<int>(e->nn);
   e->x_scr = dalloc<unsigned>(mpmk::radix_scratch_words(m));
@@ -1014,6 +1016,7 @@ static XPtrs xptrs(const mpm_engine* e) {
   X.key = e->x_key[0];
   X.rnode = e->x_rnode;
   X.rw = e->x_rw;
+  X.prec = e->x_prec;
   X.seg0 = e->x_seg0;
   X.seg1 = e->x_seg1;
   X.ndir = e->x_ndir;
@@ -1042,6 +1045,7 @@ static void exact_gmi(mpm_engine* e) {
   const int* sval = e->x_val[e->x_sorted];
   for (Group& G : e->groups) {
     const LaunchP N = nlist(e, G);
+    mpmk::launch_x_prec(XS_MASS | XS_MOM, P, X, plist(e, G), s);
     if (G.shape == MPM_SHAPE_SGMP) {
       mpmk::launch_x_scatter(XS_SGMP | XS_MASS | XS_MOM, P, X, sval, N, s);
       mpmk::launch_sgmp_nodes<double>(SGN_MASSMOM, P, N, s);
@@ -1079,6 +1083,7 @@ static void exact_group_phase(mpm_engine* e, Group& G, int ph) {
       for (int id : G.insts) e->inst[id].xx_pending = false;
       break;
     case PH_GMU:
+      mpmk::launch_x_prec(XS_FORCE, P, X, L, s);
       if (sg) {
         mpmk::launch_x_scatter(XS_SGMP | XS_FORCE, P, X, sval, N, s);
         mpmk::launch_sgmp_nodes<double>(SGN_FORCE, P, N, s);
@@ -1103,6 +1108,7 @@ static void exact_group_phase(mpm_engine* e, Group& G, int ph) {
       for (int id : G.insts) e->inst[id].xx_pending = !usf;
       break;
     case PH_MUSL:
+      mpmk::launch_x_prec(XS_MOM, P, X, L, s);
       if (sg) {
         mpmk::launch_x_scatter(XS_SGMP | XS_MOM, P, X, sval, N, s);
         mpmk::launch_sgmp_nodes<double>(SGN_MOM, P, N, s);
