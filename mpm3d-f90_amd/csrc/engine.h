@@ -144,7 +144,8 @@ enum IField { I_BODY = 0, I_ORIG, I_CELL, I_COUNT };
 // function's node list), their sort keys and, after the sort, the range of
 // sorted entries of every pool node.
 struct XPtrs {
-  unsigned long long* key;   // m sort keys: pool node << obits | original index << 5 | j (invalid: all ones)
+  unsigned* key;             // m sort keys: the pool node, at (p_begin + original index) * 27 + j (unused: all ones)
+  int* kval;                 // m sort values: the entry q of the key
   int* rnode;                // m: instance-local node (SGMP: auxiliary node) of entry q, -1 if none
   double* rw;                // 4 * m: N, dN/dx, dN/dy, dN/dz of entry q (entry-major, 32 B per entry)
   double* prec;              // 12 * np: per-slot particle records of the node scatters (k_x_prec)

@@ -244,7 +244,7 @@ struct mpm_engine {
   // sort double buffers and the radix scratch, allocated before the first
   // exact step
   bool exact = false;
-  unsigned long long* x_key[2] = {nullptr, nullptr};
+  unsigned* x_key[2] = {nullptr, nullptr};
   int* x_val[2] = {nullptr, nullptr};
   int* x_rnode = nullptr;
   double* x_rw = nullptr;
@@ -994,13 +994,13 @@ static void ensure_exact(mpm_engine* e) {
   const long m = 27 * e->np;
   if (m >= (1L << 31))
     fail(MPM_ERR_UNSUPPORTED, "reference-order mode: %ld particle slots exceed 2^31 stencil entries", e->np);
-  const int obits = 5 + bits_for(e->np), nbits = obits + bits_for(e->nn);
-  if (nbits > 64) fail(MPM_ERR_UNSUPPORTED, "reference-order mode: sort keys of %d bits", nbits);
+  // keys are pool nodes, below the all-ones key of unused places
+  const int obits = 0, nbits = bits_for(e->nn + 1);
   e->x_m = m;
   e->x_obits = obits;
   e->x_nbits = nbits;
   for (int k = 0; k < 2; k++) {
-    e->x_key[k] = dalloc<unsigned long long>(m);
+    e->x_key[k] = dalloc<unsigned>(m);
     e->x_val[k] = dalloc<int>(m);
   }
   e->x_rnode = dalloc<int>(m);
@@ -1014,6 +1014,7 @@ static void ensure_exact(mpm_engine* e) {
 static XPtrs xptrs(const mpm_engine* e) {
   XPtrs X;
   X.key = e->x_key[0];
+  X.kval = e->x_val[0];
   X.rnode = e->x_rnode;
   X.rw = e->x_rw;
   X.prec = e->x_prec;
@@ -1027,9 +1028,10 @@ static XPtrs xptrs(const mpm_engine* e) {
 }
 
 // GridMomentumInitial of every group: ParticleInitial and the stencil
-// entries, one stable sort of all entries by (node, original particle
-// index, list position), the node ranges, then every node's sums in that
-// order (update_step.f90:288-406)
+// entries placed in the original particle order, one stable sort of them by
+// node (each node's entries then run in ascending original particle index,
+// then list position), the node ranges, then every node's sums in that order
+// (update_step.f90:288-406)
 static void exact_gmi(mpm_engine* e) {
   ensure_exact(e);
   const PoolPtrs<double> P = e->ptrs<double>();
@@ -1037,7 +1039,7 @@ static void exact_gmi(mpm_engine* e) {
   hipStream_t s = e->s;
   mpmk::launch_x_fill_keys(e->x_key[0], e->x_val[0], e->x_m, s);
   for (Group& G : e->groups) mpmk::launch_x_pairs(G.shape, P, X, plist(e, G), s);
-  e->x_sorted = mpmk::radix_sort_pairs<unsigned long long>(e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1],
+  e->x_sorted = mpmk::radix_sort_pairs<unsigned>(e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1],
                                                            e->x_m, 0, e->x_nbits, e->x_scr, s);
   mpmk::launch_clear_ints(e->x_seg0, e->nn, s);
   mpmk::launch_clear_ints(e->x_seg1, e->nn, s);
@@ -1147,7 +1149,7 @@ static void exact_smooth(mpm_engine* e) {
   mpmk::launch_x_fill_keys(e->x_key[0], e->x_val[0], e->x_m, s);
   for (Group& G : e->groups)
     if (G.smooth && G.shape == MPM_SHAPE_GIMP) mpmk::launch_x_pairs_smooth(G.shape, P, X, plist(e, G), s);
-  e->x_sorted = mpmk::radix_sort_pairs<unsigned long long>(e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1],
+  e->x_sorted = mpmk::radix_sort_pairs<unsigned>(e->x_key[0], e->x_key[1], e->x_val[0], e->x_val[1],
                                                            e->x_m, 0, e->x_nbits, e->x_scr, s);
   mpmk::launch_clear_ints(e->x_seg0, e->nn, s);
   mpmk::launch_clear_ints(e->x_seg1, e->nn, s);

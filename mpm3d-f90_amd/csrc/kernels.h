@@ -135,9 +135,9 @@ int radix_sort_pairs(K* k0, K* k1, int* v0, int* v1, long n, int begin_bit, int 
                      hipStream_t s);
 
 // reference-order FP64 step (exact.inc)
-void launch_x_fill_keys(unsigned long long* key, int* val, long m, hipStream_t s);
+void launch_x_fill_keys(unsigned* key, int* val, long m, hipStream_t s);
 void launch_x_pairs(int shape, const PoolPtrs<double>& P, const XPtrs& X, const LaunchP& L, hipStream_t s);
-void launch_x_segments(const unsigned long long* key, long m, int obits, int* seg0, int* seg1, hipStream_t s);
+void launch_x_segments(const unsigned* key, long m, int obits, int* seg0, int* seg1, hipStream_t s);
 void launch_x_prec(int what, const PoolPtrs<double>& P, const XPtrs& X, const LaunchP& L, hipStream_t s);
 void launch_x_scatter(int what, const PoolPtrs<double>& P, const XPtrs& X, const int* sval, const LaunchP& N,
                       hipStream_t s);
